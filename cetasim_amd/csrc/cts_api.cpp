@@ -1,0 +1,953 @@
+// cts_api.cpp -- C ABI of the MI355X tracking turn (include/cetasim_hip.h).
+//
+// The handle owns the device copy of every bunch, the segment table that
+// assigns contiguous particle chunks to workgroups, the per-bunch wake
+// state and the turn-by-turn moment records.  Each entry point maps to one
+// MPBeam seam of the reference (src/MPBeam.cpp:1010-1173).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "cts_internal.hpp"
+
+using namespace cts;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(CTS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+SparseMat compress(const double m[36])
+{
+    // structural pattern of B1H1 / InvH1InvB1 / InvH2InvB2
+    static const int idx[16] = {0 * 6 + 0, 0 * 6 + 5, 1 * 6 + 0, 1 * 6 + 1, 1 * 6 + 5, 2 * 6 + 2,
+                                2 * 6 + 5, 3 * 6 + 2, 3 * 6 + 3, 3 * 6 + 5, 4 * 6 + 0, 4 * 6 + 1,
+                                4 * 6 + 2, 4 * 6 + 3, 4 * 6 + 4, 5 * 6 + 5};
+    SparseMat s;
+    for (int i = 0; i < 16; i++) s.m[i] = m[idx[i]];
+    return s;
+}
+
+bool pattern_ok(const double m[36])
+{
+    static const int allowed[6][6] = {{1, 0, 0, 0, 0, 1}, {1, 1, 0, 0, 0, 1}, {0, 0, 1, 0, 0, 1},
+                                      {0, 0, 1, 1, 0, 1}, {1, 1, 1, 1, 1, 0}, {0, 0, 0, 0, 0, 1}};
+    for (int r = 0; r < 6; r++)
+        for (int c = 0; c < 6; c++)
+            if (!allowed[r][c] && m[r * 6 + c] != 0.0) return false;
+    return true;
+}
+
+struct Timer {
+    double total_ms = 0;
+    int64_t n = 0;
+};
+
+}  // namespace
+
+struct cts_handle {
+    cts_ring ring{};
+    std::vector<cts_section> secs;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+
+    // bunches
+    int nb = 0;
+    int64_t ntot = 0;
+    std::vector<int64_t> counts, offsets;
+    std::vector<double> macro_charge, n_e;
+    std::vector<int32_t> gap;
+    double *d_x = nullptr, *d_px = nullptr, *d_y = nullptr, *d_py = nullptr, *d_z = nullptr, *d_dp = nullptr;
+    uint8_t *d_flag = nullptr;
+
+    // segments
+    int nwg = 0, nseg = 0, wg_target = 1024;
+    int64_t *d_seg_begin = nullptr, *d_seg_end = nullptr, *d_bunch_off = nullptr, *d_bunch_count = nullptr;
+    int32_t *d_seg_bunch = nullptr, *d_wg_seg = nullptr, *d_bunch_seg = nullptr;
+    double *d_seg_part = nullptr;
+
+    // moments
+    double *d_shift = nullptr;
+    cts_moments *d_cur = nullptr;
+    cts_moments *d_hist = nullptr;
+    int hist_cap = 0;
+    int last_turn = -1;  // last turn whose non-dp moments are in d_cur
+    bool dp_pending = false;  // d_cur's dp part belongs to an older state
+    double *d_sigz = nullptr, *d_zrange = nullptr;
+    uint32_t *d_status = nullptr;
+
+    // wake
+    int nbins = 0, wake_mode = 1, nres = 0;
+    double grid_lo = 0, grid_hi = 0;
+    std::vector<double> rs, q, omega;
+    uint32_t *d_bhist = nullptr;
+    int32_t *d_counts = nullptr;
+    double *d_kick = nullptr, *d_poten = nullptr, *d_edges_prev = nullptr, *d_edges_now = nullptr;
+    double *d_table = nullptr, *d_scale = nullptr;
+    bool kick_pending = false;
+
+    // synrad
+    int synrad_on = 0, noise_mode = 0;
+    uint64_t seed = 0;
+    double *d_noise = nullptr;
+
+    // ions (electron-side kick parameters per bunch)
+    int nspec = 0;
+    double *d_ion_e = nullptr;
+
+    // timing
+    bool timing = false;
+    std::map<std::string, Timer> timers;
+    struct Pending { std::string name; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+};
+
+namespace {
+
+void free_bunch_state(cts_handle *h)
+{
+    void *ptrs[] = {h->d_x, h->d_px, h->d_y, h->d_py, h->d_z, h->d_dp, h->d_flag,
+                    h->d_seg_begin, h->d_seg_end, h->d_bunch_off, h->d_bunch_count,
+                    h->d_seg_bunch, h->d_wg_seg, h->d_bunch_seg, h->d_seg_part, h->d_shift,
+                    h->d_cur, h->d_hist, h->d_sigz, h->d_zrange, h->d_noise, h->d_ion_e,
+                    h->d_bhist, h->d_counts, h->d_kick, h->d_poten, h->d_edges_prev,
+                    h->d_edges_now, h->d_table, h->d_scale};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    h->d_x = h->d_px = h->d_y = h->d_py = h->d_z = h->d_dp = nullptr;
+    h->d_flag = nullptr;
+    h->d_seg_begin = h->d_seg_end = h->d_bunch_off = h->d_bunch_count = nullptr;
+    h->d_seg_bunch = h->d_wg_seg = h->d_bunch_seg = nullptr;
+    h->d_seg_part = h->d_shift = h->d_sigz = h->d_zrange = h->d_noise = h->d_ion_e = nullptr;
+    h->d_cur = h->d_hist = nullptr;
+    h->d_bhist = nullptr;
+    h->d_counts = nullptr;
+    h->d_kick = h->d_poten = h->d_edges_prev = h->d_edges_now = h->d_table = h->d_scale = nullptr;
+}
+
+template <typename T>
+int dalloc(T **p, size_t n)
+{
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) return fail(CTS_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    e = hipMemset(*p, 0, n * sizeof(T));
+    if (e != hipSuccess) return fail(CTS_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    return CTS_OK;
+}
+
+hipEvent_t get_event(cts_handle *h)
+{
+    if (!h->event_pool.empty()) {
+        hipEvent_t e = h->event_pool.back();
+        h->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void resolve_timers(cts_handle *h)
+{
+    for (auto &p : h->pending) {
+        float ms = 0;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            Timer &t = h->timers[p.name];
+            t.total_ms += ms;
+            t.n += 1;
+        }
+        h->event_pool.push_back(p.a);
+        h->event_pool.push_back(p.b);
+    }
+    h->pending.clear();
+}
+
+struct TimedScope {
+    cts_handle *h;
+    const char *name;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimedScope(cts_handle *h_, const char *n) : h(h_), name(n)
+    {
+        if (h->timing) {
+            if (h->pending.size() > 4096) resolve_timers(h);
+            a = get_event(h);
+            b = get_event(h);
+            if (a) (void)hipEventRecord(a, h->stream);
+        }
+    }
+    ~TimedScope()
+    {
+        if (h->timing && a && b) {
+            (void)hipEventRecord(b, h->stream);
+            h->pending.push_back({name, a, b});
+        }
+    }
+};
+
+PassParams base_pass(cts_handle *h, uint32_t ops, int k)
+{
+    PassParams p;
+    std::memset(&p, 0, sizeof p);
+    p.x = h->d_x; p.px = h->d_px; p.y = h->d_y; p.py = h->d_py; p.z = h->d_z; p.dp = h->d_dp;
+    p.flag = h->d_flag;
+    p.seg_begin = h->d_seg_begin; p.seg_end = h->d_seg_end;
+    p.seg_bunch = h->d_seg_bunch; p.wg_seg = h->d_wg_seg;
+    p.bunch_off = h->d_bunch_off;
+    p.seg_part = h->d_seg_part;
+    p.bunch_hist = h->d_bhist;
+    p.shift = h->d_shift;
+    p.kick_prev = h->d_kick; p.edges_prev = h->d_edges_prev; p.edges_now = h->d_edges_now;
+    p.ion_e = h->d_ion_e;
+    p.noise = h->d_noise;
+    p.ntot = h->ntot;
+    p.ops = ops;
+    p.nbins = h->nbins;
+    p.nspec = h->nspec;
+    p.n_rf = h->ring.n_rf;
+    p.noise_mode = h->noise_mode;
+    p.seed = h->seed;
+
+    const cts_ring &R = h->ring;
+    const cts_section &S = h->secs[k];
+    p.b1h1 = compress(S.b1h1);
+    p.h2b2 = compress(S.h2b2);
+    p.eta_x = S.eta_x; p.etap_x = S.etap_x; p.eta_y = S.eta_y; p.etap_y = S.etap_y;
+    p.alpha_x = S.alpha_x; p.beta_x = S.beta_x; p.alpha_y = S.alpha_y; p.beta_y = S.beta_y;
+    p.phase_x = S.phase_adv_x; p.phase_y = S.phase_adv_y;
+    // src/Bunch.cpp:658-659
+    p.weigh_x = S.phase_adv_x / (2 * kPi * R.nux);
+    p.weigh_y = S.phase_adv_y / (2 * kPi * R.nuy);
+    p.chrom_x = R.chrom[0]; p.chrom_y = R.chrom[1];
+    for (int i = 0; i < 5; i++) { p.adtx[i] = R.adtx[i]; p.adty[i] = R.adty[i]; }
+    p.skew_k = R.skew_k;
+    // RF phase per unit z: the reference's -z/c/beta*2*pi*h*fRF
+    const double f_rf = R.f0 * R.harmonic;
+    for (int r = 0; r < R.n_rf && r < CTS_MAX_RF; r++) {
+        p.rf_vr[r] = R.rf_vr[r];
+        p.rf_vi[r] = R.rf_vi[r];
+        p.rf_k[r] = (((-1.0 / kClight) / R.rbeta) * 2. * kPi) * (double)R.rf_harm[r] * f_rf;
+    }
+    p.rf_scale = 1.0 / R.energy / (R.rbeta * R.rbeta);
+    p.drift_c = R.circ;
+    for (int i = 0; i < 3; i++) p.alphac[i] = R.alphac[i];
+    p.u0_kick = R.u0 / R.energy / std::pow(R.rbeta, 2);
+    // synrad constants (src/Bunch.cpp:971-980), section 0 matrices
+    p.sr_b1h1 = compress(h->secs[0].b1h1);
+    p.sr_h1b1 = compress(h->secs[0].h1b1);
+    double lam[3];
+    for (int i = 0; i < 3; i++) lam[i] = std::exp(-1.0 / R.synrad_tau[i]);
+    for (int i = 0; i < 3; i++) p.sr_lambda[i] = lam[i];
+    p.sr_lz2 = lam[2] * lam[2];
+    p.sr_coeff[0] = std::sqrt(1 - std::pow(lam[0], 2)) * std::sqrt(R.emit_nat[0]);
+    p.sr_coeff[1] = std::sqrt(1 - std::pow(lam[1], 2)) * std::sqrt(R.emit_nat[1]);
+    p.sr_coeff[2] = std::sqrt(1 - std::pow(lam[2], 4)) * std::sqrt(R.emit_nat[2]);
+    // loss limits (src/Bunch.cpp:180-186)
+    p.zlim = R.t0 * kClight / R.harmonic / 2;
+    p.ap_x = R.ap_x;
+    p.ap_y = R.ap_y;
+    return p;
+}
+
+FinParams base_fin(cts_handle *h, uint32_t ops, int slot, int prev_slot)
+{
+    FinParams f;
+    std::memset(&f, 0, sizeof f);
+    f.seg_part = h->d_seg_part;
+    f.bunch_seg = h->d_bunch_seg;
+    f.bunch_count = h->d_bunch_count;
+    f.shift = h->d_shift;
+    f.cur = h->d_cur;
+    f.hist = h->d_hist;
+    f.slot = slot;
+    f.prev_slot = prev_slot;
+    f.nb = h->nb;
+    f.nbins = h->nbins;
+    f.ops = ops;
+    f.sigz_stale = h->d_sigz;
+    f.zrange = h->d_zrange;
+    f.ref_edges = h->wake_mode == 0 ? 1 : 0;
+    f.edges_now = h->d_edges_now;
+    f.status = h->d_status;
+    return f;
+}
+
+int run_pass(cts_handle *h, const PassParams &p, const char *name = "pass")
+{
+    TimedScope ts(h, name);
+    HIPCHK(launch_pass(p, h->nwg, h->stream));
+    return CTS_OK;
+}
+
+int run_finalize(cts_handle *h, const FinParams &f)
+{
+    TimedScope ts(h, "finalize");
+    HIPCHK(launch_finalize(f, h->stream));
+    return CTS_OK;
+}
+
+int slot_of(cts_handle *h, int turn) { return (h->hist_cap > 0 && turn >= 0) ? turn % h->hist_cap : -1; }
+
+// Wake stage after the histogram is complete: reference-edge mode needs
+// the per-bunch table for this turn's bin width first.
+int run_wake(cts_handle *h)
+{
+    if (h->wake_mode == 0) {
+        WakeTableParams t;
+        std::memset(&t, 0, sizeof t);
+        t.edges = h->d_edges_now;
+        t.table = h->d_table;
+        t.nbins = h->nbins;
+        t.nres = h->nres;
+        for (int r = 0; r < h->nres; r++) { t.rs[r] = h->rs[r]; t.q[r] = h->q[r]; t.omega[r] = h->omega[r]; }
+        TimedScope ts(h, "wake_table");
+        HIPCHK(launch_wake_table(t, h->nb, h->stream));
+    }
+    WakeParams w;
+    std::memset(&w, 0, sizeof w);
+    w.bunch_hist = h->d_bhist;
+    w.counts_out = h->d_counts;
+    w.table = h->d_table;
+    w.per_bunch_table = h->wake_mode == 0 ? 1 : 0;
+    w.scale = h->d_scale;
+    w.poten = h->d_poten;
+    w.kick = h->d_kick;
+    w.edges_now = h->d_edges_now;
+    w.edges_prev = h->d_edges_prev;
+    w.nbins = h->nbins;
+    TimedScope ts(h, "wake");
+    HIPCHK(launch_wake(w, h->nb, h->stream));
+    return CTS_OK;
+}
+
+int run_bin(cts_handle *h)
+{
+    BinParams b;
+    b.z = h->d_z;
+    b.seg_begin = h->d_seg_begin; b.seg_end = h->d_seg_end;
+    b.seg_bunch = h->d_seg_bunch; b.wg_seg = h->d_wg_seg;
+    b.edges = h->d_edges_now;
+    b.bunch_hist = h->d_bhist;
+    b.nbins = h->nbins;
+    TimedScope ts(h, "bin");
+    HIPCHK(launch_bin(b, h->nwg, h->stream));
+    return CTS_OK;
+}
+
+// Apply a pending wake kick; the dp moments of the state it completes go
+// into the current record (and its history slot).
+int flush_kick(cts_handle *h)
+{
+    if (!h->kick_pending) return CTS_OK;
+    PassParams p = base_pass(h, OP_KICK | OP_MOMB | OP_WRITE, 0);
+    int rc = run_pass(h, p, "kick");
+    if (rc) return rc;
+    FinParams f = base_fin(h, OP_MOMB, -1, slot_of(h, h->last_turn));
+    rc = run_finalize(h, f);
+    if (rc) return rc;
+    h->kick_pending = false;
+    h->dp_pending = false;
+    return CTS_OK;
+}
+
+int check_ready(cts_handle *h)
+{
+    if (!h) return fail(CTS_ERR_ARG, "null handle");
+    if (h->nb == 0) return fail(CTS_ERR_STATE, "no bunches: call cts_set_bunches first");
+    return CTS_OK;
+}
+
+int read_status(cts_handle *h, uint32_t *st)
+{
+    HIPCHK(hipMemcpyAsync(st, h->d_status, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemsetAsync(h->d_status, 0, sizeof(uint32_t), h->stream));
+    return CTS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *cts_last_error(void) { return g_err.c_str(); }
+
+int cts_device_info(char *buf, int32_t len)
+{
+    int dev = 0;
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    std::snprintf(buf, (size_t)len, "%s %s CUs=%d", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    return CTS_OK;
+}
+
+int cts_create(const cts_ring *ring, const cts_section *secs, int32_t nsec, cts_handle **out)
+{
+    if (!ring || !secs || nsec < 1 || !out) return fail(CTS_ERR_ARG, "cts_create: bad arguments");
+    if (ring->n_rf < 0 || ring->n_rf > CTS_MAX_RF) return fail(CTS_ERR_ARG, "cts_create: n_rf out of range");
+    for (int k = 0; k < nsec; k++) {
+        if (!pattern_ok(secs[k].b1h1) || !pattern_ok(secs[k].h1b1) || !pattern_ok(secs[k].h2b2))
+            return fail(CTS_ERR_ARG, "cts_create: section matrix outside the B*H structure");
+    }
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (ndev < 1) return fail(CTS_ERR_HIP, "no HIP device");
+    cts_handle *h = new cts_handle();
+    h->ring = *ring;
+    h->secs.assign(secs, secs + nsec);
+    hipError_t e = hipGetDevice(&h->dev);
+    if (e == hipSuccess) e = hipStreamCreate(&h->stream);
+    if (e != hipSuccess) {
+        delete h;
+        return fail(CTS_ERR_HIP, std::string("stream: ") + hipGetErrorString(e));
+    }
+    // Algorithm 916 constants and exp(-a^2 n^2) table
+    double fad[3], tab[64];
+    fad[0] = kPi / std::sqrt(-std::log(DBL_EPSILON * 0.5));
+    fad[1] = (2 / kPi) * fad[0];
+    fad[2] = fad[0] * fad[0];
+    for (int n = 1; n <= 63; n++) tab[n - 1] = std::exp(-fad[2] * (double)(n * n));
+    tab[63] = 0.0;
+    e = upload_faddeeva_tables(tab, fad);
+    if (e == hipSuccess) e = hipMalloc((void **)&h->d_status, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(h->d_status, 0, sizeof(uint32_t));
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return fail(CTS_ERR_HIP, std::string("init: ") + hipGetErrorString(e));
+    }
+    *out = h;
+    return CTS_OK;
+}
+
+int cts_destroy(cts_handle *h)
+{
+    if (!h) return CTS_OK;
+    (void)hipStreamSynchronize(h->stream);
+    resolve_timers(h);
+    for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
+    free_bunch_state(h);
+    if (h->d_status) (void)hipFree(h->d_status);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return CTS_OK;
+}
+
+int cts_set_bunches(cts_handle *h, int32_t nb, const int64_t *counts, const double *macro_charge,
+                    const double *n_e, const int32_t *gap)
+{
+    if (!h || nb < 1 || !counts || !macro_charge || !n_e) return fail(CTS_ERR_ARG, "cts_set_bunches: bad arguments");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    free_bunch_state(h);
+    h->nb = nb;
+    h->counts.assign(counts, counts + nb);
+    h->macro_charge.assign(macro_charge, macro_charge + nb);
+    h->n_e.assign(n_e, n_e + nb);
+    h->gap.assign(nb, 1);
+    if (gap) h->gap.assign(gap, gap + nb);
+    h->offsets.assign(nb + 1, 0);
+    for (int b = 0; b < nb; b++) {
+        if (counts[b] < 0) return fail(CTS_ERR_ARG, "cts_set_bunches: negative count");
+        h->offsets[b + 1] = h->offsets[b] + counts[b];
+    }
+    h->ntot = h->offsets[nb];
+    const int64_t n = h->ntot;
+
+    // workgroups: contiguous chunks (multiples of 64) of the particle range
+    int64_t want = (n + 2047) / 2048;
+    if (want < 1) want = 1;
+    if (want > h->wg_target) want = h->wg_target;
+    int64_t chunk = (n + want - 1) / want;
+    chunk = (chunk + 63) / 64 * 64;
+    if (chunk < 64) chunk = 64;
+    const int nwg = (int)((n + chunk - 1) / chunk) > 0 ? (int)((n + chunk - 1) / chunk) : 1;
+    std::vector<int64_t> sb, se;
+    std::vector<int32_t> sbunch, wgseg(nwg + 1, 0), bseg(nb + 1, 0);
+    int b = 0;
+    for (int w = 0; w < nwg; w++) {
+        wgseg[w] = (int32_t)sb.size();
+        const int64_t lo = (int64_t)w * chunk, hi = std::min<int64_t>(n, lo + chunk);
+        int64_t p = lo;
+        while (p < hi) {
+            while (b < nb && h->offsets[b + 1] <= p) b++;
+            const int64_t e2 = std::min<int64_t>(hi, h->offsets[b + 1]);
+            sb.push_back(p);
+            se.push_back(e2);
+            sbunch.push_back(b);
+            p = e2;
+        }
+    }
+    wgseg[nwg] = (int32_t)sb.size();
+    h->nwg = nwg;
+    h->nseg = (int)sb.size();
+    {
+        int s = 0;
+        for (int bb = 0; bb < nb; bb++) {
+            while (s < h->nseg && sbunch[s] < bb) s++;
+            bseg[bb] = s;
+        }
+        bseg[nb] = h->nseg;
+    }
+    int rc;
+    if ((rc = dalloc(&h->d_x, n)) || (rc = dalloc(&h->d_px, n)) || (rc = dalloc(&h->d_y, n)) ||
+        (rc = dalloc(&h->d_py, n)) || (rc = dalloc(&h->d_z, n)) || (rc = dalloc(&h->d_dp, n)) ||
+        (rc = dalloc(&h->d_flag, n)) || (rc = dalloc(&h->d_seg_begin, h->nseg)) ||
+        (rc = dalloc(&h->d_seg_end, h->nseg)) || (rc = dalloc(&h->d_seg_bunch, h->nseg)) ||
+        (rc = dalloc(&h->d_wg_seg, nwg + 1)) || (rc = dalloc(&h->d_bunch_seg, nb + 1)) ||
+        (rc = dalloc(&h->d_bunch_off, nb + 1)) || (rc = dalloc(&h->d_bunch_count, nb)) ||
+        (rc = dalloc(&h->d_seg_part, (size_t)h->nseg * kNPart)) || (rc = dalloc(&h->d_shift, (size_t)nb * 6)) ||
+        (rc = dalloc(&h->d_cur, nb)) || (rc = dalloc(&h->d_sigz, nb)) || (rc = dalloc(&h->d_zrange, (size_t)nb * 2)) ||
+        (rc = dalloc(&h->d_ion_e, (size_t)nb * CTS_MAX_SPECIES * 5)))
+        return rc;
+    HIPCHK(hipMemcpy(h->d_seg_begin, sb.data(), sb.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_seg_end, se.data(), se.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_seg_bunch, sbunch.data(), sbunch.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_wg_seg, wgseg.data(), wgseg.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_bunch_seg, bseg.data(), bseg.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_bunch_off, h->offsets.data(), (nb + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_bunch_count, h->counts.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice));
+    h->kick_pending = false;
+    h->dp_pending = false;
+    h->last_turn = -1;
+    if (h->hist_cap > 0) {
+        if ((rc = dalloc(&h->d_hist, (size_t)h->hist_cap * nb))) return rc;
+    }
+    if (h->nbins > 0) {
+        const int nbins = h->nbins;
+        h->nbins = 0;  // re-run allocation of wake state for the new layout
+        std::vector<double> fr(h->omega.size());
+        for (size_t i = 0; i < fr.size(); i++) fr[i] = h->omega[i] / (2 * kPi);
+        std::vector<double> rs = h->rs, q = h->q;
+        return cts_set_wake(h, nbins, h->wake_mode, h->grid_lo, h->grid_hi, (int32_t)rs.size(), rs.data(),
+                            q.data(), fr.data());
+    }
+    return CTS_OK;
+}
+
+int cts_upload(cts_handle *h, const double *x, const double *px, const double *y, const double *py,
+               const double *z, const double *dp, const int32_t *lost)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!x || !px || !y || !py || !z || !dp) return fail(CTS_ERR_ARG, "cts_upload: null array");
+    const size_t bytes = (size_t)h->ntot * sizeof(double);
+    HIPCHK(hipMemcpy(h->d_x, x, bytes, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_px, px, bytes, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_y, y, bytes, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_py, py, bytes, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_z, z, bytes, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_dp, dp, bytes, hipMemcpyHostToDevice));
+    std::vector<uint8_t> fl((size_t)h->ntot, 0);
+    if (lost) {
+        for (int64_t i = 0; i < h->ntot; i++) {
+            if (lost[i] < 0 || lost[i] > 2) return fail(CTS_ERR_ARG, "cts_upload: loss code outside 0..2");
+            fl[i] = (uint8_t)lost[i];
+        }
+    }
+    HIPCHK(hipMemcpy(h->d_flag, fl.data(), fl.size(), hipMemcpyHostToDevice));
+    h->kick_pending = false;
+    h->dp_pending = false;
+    h->last_turn = -1;
+    // moments of the uploaded state (the turn-0 MPBeamRMSCal)
+    return cts_rms(h, nullptr);
+}
+
+int cts_download(cts_handle *h, double *x, double *px, double *y, double *py, double *z, double *dp,
+                 int32_t *lost)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if ((rc = flush_kick(h))) return rc;
+    const size_t bytes = (size_t)h->ntot * sizeof(double);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (x) HIPCHK(hipMemcpy(x, h->d_x, bytes, hipMemcpyDeviceToHost));
+    if (px) HIPCHK(hipMemcpy(px, h->d_px, bytes, hipMemcpyDeviceToHost));
+    if (y) HIPCHK(hipMemcpy(y, h->d_y, bytes, hipMemcpyDeviceToHost));
+    if (py) HIPCHK(hipMemcpy(py, h->d_py, bytes, hipMemcpyDeviceToHost));
+    if (z) HIPCHK(hipMemcpy(z, h->d_z, bytes, hipMemcpyDeviceToHost));
+    if (dp) HIPCHK(hipMemcpy(dp, h->d_dp, bytes, hipMemcpyDeviceToHost));
+    if (lost) {
+        std::vector<uint8_t> fl((size_t)h->ntot);
+        HIPCHK(hipMemcpy(fl.data(), h->d_flag, fl.size(), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < h->ntot; i++) lost[i] = fl[i] & 3u;
+    }
+    return CTS_OK;
+}
+
+int cts_set_wake(cts_handle *h, int32_t nbins, int32_t mode, double grid_lo, double grid_hi, int32_t nres,
+                 const double *rs, const double *q, const double *fr_hz)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (nbins < 0 || nbins > 1024 || (mode != 0 && mode != 1) || nres < 0 || nres > CTS_MAX_RES)
+        return fail(CTS_ERR_ARG, "cts_set_wake: nbins in [0,1024], mode 0/1, nres <= 8");
+    if (mode == 1 && nbins > 0 && !(grid_hi > grid_lo)) return fail(CTS_ERR_ARG, "cts_set_wake: empty grid");
+    for (int r = 0; r < nres; r++)
+        if (!(q[r] > 0.5)) return fail(CTS_ERR_ARG, "cts_set_wake: BBR needs Q > 1/2");
+    if ((rc = flush_kick(h))) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    void *old[] = {h->d_bhist, h->d_counts, h->d_kick, h->d_poten, h->d_edges_prev, h->d_edges_now,
+                   h->d_table, h->d_scale};
+    for (void *p : old)
+        if (p) (void)hipFree(p);
+    h->d_bhist = nullptr; h->d_counts = nullptr;
+    h->d_kick = h->d_poten = h->d_edges_prev = h->d_edges_now = h->d_table = h->d_scale = nullptr;
+    h->nbins = (nres > 0) ? nbins : 0;
+    h->wake_mode = mode;
+    h->grid_lo = grid_lo;
+    h->grid_hi = grid_hi;
+    h->nres = nres;
+    h->rs.assign(rs, rs + nres);
+    h->q.assign(q, q + nres);
+    h->omega.resize(nres);
+    for (int r = 0; r < nres; r++) h->omega[r] = fr_hz[r] * 2 * kPi;  // src/WakeFunction.cpp:275
+    h->kick_pending = false;
+    if (h->nbins == 0) return CTS_OK;
+    const int nb = h->nb, n = h->nbins;
+    if ((rc = dalloc(&h->d_bhist, (size_t)nb * n)) || (rc = dalloc(&h->d_counts, (size_t)nb * n)) ||
+        (rc = dalloc(&h->d_kick, (size_t)nb * n)) || (rc = dalloc(&h->d_poten, (size_t)nb * n)) ||
+        (rc = dalloc(&h->d_edges_prev, (size_t)nb * 2)) || (rc = dalloc(&h->d_edges_now, (size_t)nb * 2)) ||
+        (rc = dalloc(&h->d_table, (size_t)(mode == 0 ? nb : 1) * (2 * n - 1))) || (rc = dalloc(&h->d_scale, nb)))
+        return rc;
+    // scale of srWakePoten: (-1)*e*q/E (src/MPBunch.cpp:1996-1998)
+    std::vector<double> sc(nb);
+    for (int b = 0; b < nb; b++) sc[b] = (-1) * kEcharge * h->macro_charge[b] / h->ring.energy;
+    HIPCHK(hipMemcpy(h->d_scale, sc.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+    if (mode == 1) {
+        std::vector<double> ed(2 * nb);
+        const double dz = (grid_hi - grid_lo) / n;
+        for (int b = 0; b < nb; b++) { ed[2 * b] = grid_lo; ed[2 * b + 1] = dz; }
+        HIPCHK(hipMemcpy(h->d_edges_now, ed.data(), ed.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->d_edges_prev, ed.data(), ed.size() * sizeof(double), hipMemcpyHostToDevice));
+        WakeTableParams t;
+        std::memset(&t, 0, sizeof t);
+        t.edges = h->d_edges_now;  // bunch 0's width is the grid's
+        t.table = h->d_table;
+        t.nbins = n;
+        t.nres = nres;
+        for (int r = 0; r < nres; r++) { t.rs[r] = h->rs[r]; t.q[r] = h->q[r]; t.omega[r] = h->omega[r]; }
+        HIPCHK(launch_wake_table(t, 1, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    return CTS_OK;
+}
+
+int cts_set_synrad(cts_handle *h, int32_t on, int32_t noise_mode, uint64_t seed)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (noise_mode < 0 || noise_mode > 2) return fail(CTS_ERR_ARG, "cts_set_synrad: noise_mode 0..2");
+    for (int i = 0; i < 3; i++)
+        if (on && !(h->ring.synrad_tau[i] > 0)) return fail(CTS_ERR_ARG, "cts_set_synrad: damping times must be > 0");
+    h->synrad_on = on ? 1 : 0;
+    h->noise_mode = noise_mode;
+    h->seed = seed;
+    if (noise_mode == 1 && !h->d_noise) {
+        if ((rc = dalloc(&h->d_noise, (size_t)h->ntot * 5))) return rc;
+    }
+    return CTS_OK;
+}
+
+int cts_push_noise(cts_handle *h, const double *noise)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!h->d_noise) return fail(CTS_ERR_STATE, "cts_push_noise: noise_mode 1 not set");
+    HIPCHK(hipMemcpyAsync(h->d_noise, noise, (size_t)h->ntot * 5 * sizeof(double), hipMemcpyHostToDevice,
+                          h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CTS_OK;
+}
+
+// MPBeam::Run body for one section (src/MPBeam.cpp:1059-1102), fused.
+int cts_turn(cts_handle *h, int32_t turn)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (h->secs.size() != 1) return fail(CTS_ERR_STATE, "cts_turn fuses one section; use the seam calls");
+    uint32_t ops = OP_MAP | OP_RF | OP_DRIFT | OP_U0 | OP_LOSS | OP_MOMA | OP_WRITE;
+    if (h->ring.skew_k != 0) ops |= OP_SKEW;
+    if (h->synrad_on) ops |= OP_SYNRAD;
+    if (h->nspec > 0) {
+        // electron kick by the ion cloud of this passage, before the map
+        // (src/MPBeam.cpp:1064-1068)
+        IonKickParams ik;
+        ik.x = h->d_x; ik.y = h->d_y; ik.px = h->d_px; ik.py = h->d_py; ik.flag = h->d_flag;
+        ik.seg_begin = h->d_seg_begin; ik.seg_end = h->d_seg_end;
+        ik.seg_bunch = h->d_seg_bunch; ik.wg_seg = h->d_wg_seg;
+        ik.ion_e = h->d_ion_e;
+        ik.nspec = h->nspec;
+        TimedScope ts(h, "ion_kick");
+        HIPCHK(launch_ion_kick(ik, h->nwg, h->stream));
+    }
+    const bool wake = h->nbins > 0;
+    if (h->kick_pending) ops |= OP_KICK | OP_MOMB;
+    if (wake) ops |= (h->wake_mode == 1) ? OP_BIN : OP_ZSTAT;
+    else ops |= OP_MOMD;  // no deferred kick: dp moments close this turn
+    PassParams p = base_pass(h, ops, 0);
+    p.turn = (uint32_t)turn;
+    if ((rc = run_pass(h, p))) return rc;
+    // finalize: dp part of the previous record (MOMB), this turn's record
+    FinParams f = base_fin(h, ops & (OP_MOMB | OP_MOMA | OP_MOMD | OP_LOSS | OP_ZSTAT), slot_of(h, turn),
+                           slot_of(h, h->last_turn));
+    if ((rc = run_finalize(h, f))) return rc;
+    if (wake) {
+        if (h->wake_mode == 0 && (rc = run_bin(h))) return rc;
+        if ((rc = run_wake(h))) return rc;
+        h->kick_pending = true;
+    }
+    h->last_turn = turn;
+    h->dp_pending = wake;
+    uint32_t st = 0;
+    if ((rc = read_status(h, &st))) return rc;
+    if (st & 1u) return fail(CTS_LOW_TRANSMISSION, "more than 50% particles are lost in a bunch");
+    return CTS_OK;
+}
+
+int cts_flush(cts_handle *h)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    return flush_kick(h);
+}
+
+static int seam(cts_handle *h, uint32_t ops, int k, uint32_t turn)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (k < 0 || k >= (int)h->secs.size()) return fail(CTS_ERR_ARG, "section index out of range");
+    if ((rc = flush_kick(h))) return rc;
+    PassParams p = base_pass(h, ops | OP_WRITE, k);
+    p.turn = turn;
+    return run_pass(h, p);
+}
+
+int cts_section_map(cts_handle *h, int32_t k) { return seam(h, OP_MAP, k, 0); }
+int cts_skew_quad(cts_handle *h) { return seam(h, OP_SKEW, 0, 0); }
+int cts_rf_kick(cts_handle *h) { return seam(h, OP_RF, 0, 0); }
+int cts_drift(cts_handle *h) { return seam(h, OP_DRIFT, 0, 0); }
+int cts_energy_loss(cts_handle *h) { return seam(h, OP_U0, 0, 0); }
+int cts_synrad(cts_handle *h, int32_t turn) { return seam(h, OP_SYNRAD, 0, (uint32_t)turn); }
+
+int cts_sr_wake(cts_handle *h, const double *sigma_z_stale)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (h->nbins == 0) return CTS_OK;
+    if ((rc = flush_kick(h))) return rc;
+    if (h->wake_mode == 0) {
+        // edges from the survivors' z range (GetZMinMax) and the rms bunch
+        // length of the last MPBeamRMSCal (src/MPBunch.cpp:1900-1908)
+        PassParams p = base_pass(h, OP_ZSTAT, 0);
+        if ((rc = run_pass(h, p, "zstat"))) return rc;
+        FinParams f = base_fin(h, OP_ZSTAT, -1, -1);
+        if ((rc = run_finalize(h, f))) return rc;
+        if (sigma_z_stale) {
+            std::vector<double> zr(2 * h->nb), ed(2 * h->nb);
+            HIPCHK(hipMemcpyAsync(zr.data(), h->d_zrange, zr.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                  h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            for (int b = 0; b < h->nb; b++) {
+                const double zMin = zr[2 * b] - 1.e-6, zMax = zr[2 * b + 1] + 1.e-6;
+                const double lo = zMin - 2 * sigma_z_stale[b], hi = zMax + 2 * sigma_z_stale[b];
+                ed[2 * b] = lo;
+                ed[2 * b + 1] = (hi - lo) / h->nbins;
+            }
+            HIPCHK(hipMemcpy(h->d_edges_now, ed.data(), ed.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+        if ((rc = run_bin(h))) return rc;
+    } else {
+        PassParams p = base_pass(h, OP_BIN, 0);
+        if ((rc = run_pass(h, p, "bin"))) return rc;
+    }
+    if ((rc = run_wake(h))) return rc;
+    // immediate kick (the seam applies it within the call)
+    h->kick_pending = true;
+    PassParams p = base_pass(h, OP_KICK | OP_WRITE, 0);
+    if ((rc = run_pass(h, p, "kick"))) return rc;
+    h->kick_pending = false;
+    return CTS_OK;
+}
+
+int cts_mark_lost(cts_handle *h, double *transmission)
+{
+    int rc = seam(h, OP_LOSS, 0, 0);
+    if (rc) return rc;
+    FinParams f = base_fin(h, OP_LOSS, -1, -1);
+    if ((rc = run_finalize(h, f))) return rc;
+    uint32_t st = 0;
+    if ((rc = read_status(h, &st))) return rc;
+    if (transmission) {
+        std::vector<cts_moments> m(h->nb);
+        HIPCHK(hipMemcpy(m.data(), h->d_cur, m.size() * sizeof(cts_moments), hipMemcpyDeviceToHost));
+        for (int b = 0; b < h->nb; b++) transmission[b] = m[b].transmission;
+    }
+    if (st & 1u) return fail(CTS_LOW_TRANSMISSION, "more than 50% particles are lost in a bunch");
+    return CTS_OK;
+}
+
+int cts_rms(cts_handle *h, cts_moments *out)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if ((rc = flush_kick(h))) return rc;
+    // two passes: the first fixes the shift (means), the second takes the
+    // centred sums about it
+    for (int it = 0; it < 2; it++) {
+        PassParams p = base_pass(h, OP_MOMA | OP_MOMB | OP_ZSTAT, 0);
+        if ((rc = run_pass(h, p, "moments"))) return rc;
+        FinParams f = base_fin(h, OP_MOMB | OP_MOMA | OP_ZSTAT, -1, -1);
+        if ((rc = run_finalize(h, f))) return rc;
+    }
+    if (out) {
+        HIPCHK(hipMemcpyAsync(out, h->d_cur, h->nb * sizeof(cts_moments), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    return CTS_OK;
+}
+
+int cts_get_moments(cts_handle *h, cts_moments *out)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if ((rc = flush_kick(h))) return rc;
+    HIPCHK(hipMemcpyAsync(out, h->d_cur, h->nb * sizeof(cts_moments), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CTS_OK;
+}
+
+int cts_set_history(cts_handle *h, int32_t capacity)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (capacity < 0) return fail(CTS_ERR_ARG, "cts_set_history: capacity < 0");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (h->d_hist) (void)hipFree(h->d_hist);
+    h->d_hist = nullptr;
+    h->hist_cap = capacity;
+    if (capacity > 0) {
+        if ((rc = dalloc(&h->d_hist, (size_t)capacity * h->nb))) return rc;
+    }
+    return CTS_OK;
+}
+
+int cts_get_history(cts_handle *h, int32_t first, int32_t nturns, cts_moments *out)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (h->hist_cap == 0) return fail(CTS_ERR_STATE, "cts_get_history: no history buffer");
+    if (first < 0 || nturns < 0 || first + nturns - 1 > h->last_turn || h->last_turn - first >= h->hist_cap)
+        return fail(CTS_ERR_ARG, "cts_get_history: turns not held");
+    if ((rc = flush_kick(h))) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int t = first; t < first + nturns; t++) {
+        const int s = t % h->hist_cap;
+        HIPCHK(hipMemcpy(out + (size_t)(t - first) * h->nb, h->d_hist + (size_t)s * h->nb,
+                         h->nb * sizeof(cts_moments), hipMemcpyDeviceToHost));
+    }
+    return CTS_OK;
+}
+
+int cts_get_wake_state(cts_handle *h, int32_t *counts, double *edges, double *kick)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (h->nbins == 0) return fail(CTS_ERR_STATE, "wake is off");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const size_t n = (size_t)h->nb * h->nbins;
+    if (counts) HIPCHK(hipMemcpy(counts, h->d_counts, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (edges) HIPCHK(hipMemcpy(edges, h->d_edges_prev, 2 * h->nb * sizeof(double), hipMemcpyDeviceToHost));
+    if (kick) HIPCHK(hipMemcpy(kick, h->d_kick, n * sizeof(double), hipMemcpyDeviceToHost));
+    return CTS_OK;
+}
+
+int cts_derive_moments(cts_handle *h, const cts_moments *m, int32_t k, cts_derived *o)
+{
+    if (!h || !m || !o || k < 0 || k >= (int)h->secs.size()) return fail(CTS_ERR_ARG, "cts_derive_moments");
+    const double *c = m->c2, *mu = m->mean;
+    // src/MPBunch.cpp:431-485
+    o->emit[0] = std::sqrt(c[0] * c[2] - c[1] * c[1]);
+    o->emit[1] = std::sqrt(c[3] * c[5] - c[4] * c[4]);
+    o->emit[2] = std::sqrt(c[6] * c[8] - c[7] * c[7]);
+    const double ux = c[0] + mu[0] * mu[0], upx = c[2] + mu[1] * mu[1], uxpx = c[1] + mu[0] * mu[1];
+    const double uy = c[3] + mu[2] * mu[2], upy = c[5] + mu[3] * mu[3], uypy = c[4] + mu[2] * mu[3];
+    const double uz = c[6] + mu[4] * mu[4], ud = c[8] + mu[5] * mu[5], uzd = c[7] + mu[4] * mu[5];
+    o->emit_eff[0] = std::sqrt(ux * upx - uxpx * uxpx);
+    o->emit_eff[1] = std::sqrt(uy * upy - uypy * uypy);
+    o->emit_eff[2] = std::sqrt(uz * ud - uzd * uzd);
+    o->rms_x = std::sqrt(o->emit[0] * h->secs[k].beta_x);
+    o->rms_y = std::sqrt(o->emit[1] * h->secs[k].beta_y);
+    o->rms_eff_x = std::sqrt(o->emit_eff[0] * h->secs[k].beta_x);
+    o->rms_eff_y = std::sqrt(o->emit_eff[1] * h->secs[k].beta_y);
+    o->sigma_z = std::sqrt(c[6]);
+    o->sigma_dp = std::sqrt(c[8]);
+    return CTS_OK;
+}
+
+int cts_sync(cts_handle *h)
+{
+    if (!h) return fail(CTS_ERR_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CTS_OK;
+}
+
+int cts_set_timing(cts_handle *h, int32_t on)
+{
+    if (!h) return fail(CTS_ERR_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    resolve_timers(h);
+    h->timers.clear();
+    h->timing = on != 0;
+    return CTS_OK;
+}
+
+int cts_kernel_time(cts_handle *h, const char *name, double *avg_ms, int64_t *launches)
+{
+    if (!h || !name) return fail(CTS_ERR_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    resolve_timers(h);
+    auto it = h->timers.find(name);
+    if (it == h->timers.end() || it->second.n == 0) {
+        if (avg_ms) *avg_ms = 0;
+        if (launches) *launches = 0;
+        return CTS_OK;
+    }
+    if (avg_ms) *avg_ms = it->second.total_ms / (double)it->second.n;
+    if (launches) *launches = it->second.n;
+    return CTS_OK;
+}
+
+int cts_debug_qe_normals(cts_handle *h, uint64_t seed, int32_t turn, uint64_t base, int64_t n, double *noise_out)
+{
+    if (!h || n <= 0 || !noise_out) return fail(CTS_ERR_ARG, "cts_debug_qe_normals");
+    double *d = nullptr;
+    int rc = dalloc(&d, (size_t)n * 5);
+    if (rc) return rc;
+    hipError_t e = launch_qe_debug(seed, (uint32_t)turn, base, n, d, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(noise_out, d, (size_t)n * 5 * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(CTS_ERR_HIP, std::string("qe debug: ") + hipGetErrorString(e));
+    return CTS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,139 @@
+// cts_device.hpp -- device helpers shared by the tracking kernels.
+//
+// All HIP sources of this library are compiled with -ffp-contract=off:
+// multiply and add stay separate operations in the order written, so the
+// FP64 arithmetic follows the reference's operation order
+// (src/Bunch.cpp:625-755, 940-1044) and the float Box-Muller transform
+// reproduces the host replica bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cts {
+
+constexpr double kPi = 3.141592653589793;  // 4*atan(1), include/Global.h:33
+constexpr double kClight = 299792458.0;
+constexpr double kEcharge = 1.602017733e-19;
+constexpr double kRe = 2.81794092E-15;
+
+// ---------------------------------------------------------------- Philox
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
+struct u32x4 { uint32_t v[4]; };
+
+__device__ __forceinline__ u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
+                                               uint32_t c3, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    u32x4 o;
+    o.v[0] = c0; o.v[1] = c1; o.v[2] = c2; o.v[3] = c3;
+    return o;
+}
+
+// ln(u), u in (0, 1]: u = m*2^e, m in [sqrt(1/2), sqrt(2)), 2*atanh series.
+__device__ __forceinline__ float cts_logf(float u)
+{
+    const uint32_t b = __float_as_uint(u);
+    int e = (int)((b >> 23) & 0xffu) - 127;
+    float m = __uint_as_float((b & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421356f) { m = m * 0.5f; e = e + 1; }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    const float p = 1.0f + s2 * (0.333333333f + s2 * (0.2f + s2 * (0.142857143f + s2 * 0.111111111f)));
+    return (float)e * 0.693147181f + 2.0f * s * p;
+}
+
+// sin and cos of 2*pi*u2, u2 = k*2^-24 in [0, 1).
+__device__ __forceinline__ void cts_sincos2pif(float u2, float &s_out, float &c_out)
+{
+    const float t = u2 * 4.0f;
+    const int q = (int)t;
+    float f = t - (float)q;
+    const bool swap = f > 0.5f;
+    if (swap) f = 1.0f - f;
+    const float th = f * 1.57079633f;
+    const float t2 = th * th;
+    const float s = th * (1.0f - t2 * (0.166666667f - t2 * (0.00833333333f - t2 * (0.000198412698f - t2 * 2.75573192e-6f))));
+    const float c = 1.0f - t2 * (0.5f - t2 * (0.0416666667f - t2 * (0.00138888889f - t2 * (2.48015873e-5f - t2 * 2.75573192e-7f))));
+    const float sn = swap ? c : s, cs = swap ? s : c;
+    switch (q & 3) {
+    case 0: s_out = sn; c_out = cs; break;
+    case 1: s_out = cs; c_out = -sn; break;
+    case 2: s_out = -sn; c_out = -cs; break;
+    default: s_out = -cs; c_out = sn; break;
+    }
+}
+
+__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float &n0, float &n1)
+{
+    const float u1 = (float)((a >> 8) + 1u) * 0x1p-24f;
+    const float u2 = (float)(b >> 8) * 0x1p-24f;
+    const float r = __builtin_sqrtf(-2.0f * cts_logf(u1));
+    float s, c;
+    cts_sincos2pif(u2, s, c);
+    n0 = r * c;
+    n1 = r * s;
+}
+
+// The five quantum-excitation normals (r0 r1 r2 r3 r5) of one particle.
+__device__ __forceinline__ void qe_normals(uint64_t seed, uint32_t turn, uint64_t index,
+                                           double out[5])
+{
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    const uint32_t i0 = (uint32_t)index, i1 = (uint32_t)(index >> 32);
+    const u32x4 a = philox4x32_10(i0, i1, turn, 0u, k0, k1);
+    const u32x4 b = philox4x32_10(i0, i1, turn, 1u, k0, k1);
+    float n0, n1, n2, n3, n4, n5;
+    box_muller(a.v[0], a.v[1], n0, n1);
+    box_muller(a.v[2], a.v[3], n2, n3);
+    box_muller(b.v[0], b.v[1], n4, n5);
+    out[0] = n0; out[1] = n1; out[2] = n2; out[3] = n3; out[4] = n4;
+}
+
+// ------------------------------------------------------- section matrices
+// B1H1, InvH1InvB1 and InvH2InvB2 (src/LatticeInterActionPoint.cpp:326-431)
+// share one structural pattern: row 0 {0,5}, row 1 {0,1,5}, row 2 {2,5},
+// row 3 {2,3,5}, row 4 {0,1,2,3,4}, row 5 {5}.  The 16 entries are stored in
+// that order; the product adds them in ascending column order as gslcblas
+// dgemm does (entries that are exactly zero only change the sign of a zero).
+struct SparseMat { double m[16]; };
+
+__device__ __forceinline__ void spmv(const SparseMat &A, const double x[6], double y[6])
+{
+    y[0] = A.m[0] * x[0] + A.m[1] * x[5];
+    y[1] = A.m[2] * x[0] + A.m[3] * x[1] + A.m[4] * x[5];
+    y[2] = A.m[5] * x[2] + A.m[6] * x[5];
+    y[3] = A.m[7] * x[2] + A.m[8] * x[3] + A.m[9] * x[5];
+    y[4] = A.m[10] * x[0] + A.m[11] * x[1] + A.m[12] * x[2] + A.m[13] * x[3] + A.m[14] * x[4];
+    y[5] = A.m[15] * x[5];
+}
+
+// ---------------------------------------------------------- reductions
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+}  // namespace cts

@@ -1,0 +1,272 @@
+// cts_faddeeva.hpp -- FP64 Faddeeva function w(z) and the Bassetti-Erskine
+// field on the device.
+//
+// Region structure of the reference's vendored Faddeeva package (S. G.
+// Johnson, MIT; src/Faddeeva.cpp:682-974): axis special cases, continued
+// fraction (Gautschi 1970, Poppe & Wijers 1990) for large |z|, Zaghloul &
+// Ali (2011, ACM TOMS Algorithm 916) sums for |x| < 10, large-|x| sums
+// otherwise.  erfcx and Im w(x) on the axes are this project's own
+// evaluations (exp(x^2)erfc(x) with an exact x^2 split, Laplace continued
+// fraction for x >= 25, Dawson's Taylor series for |x| < 0.5).  FP64
+// throughout: the asymptotic and lower-half-plane branches cancel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace cts {
+
+// exp(-a^2 n^2), n = 1..64, a = pi/sqrt(-log(eps/2)); filled at library init.
+extern __constant__ double c_ea2n2[64];
+// {a, c = 2a/pi, a^2}
+extern __constant__ double c_fad[3];
+
+constexpr double kIsPi = 0.56418958354775628694807945156;  // 1/sqrt(pi)
+
+__device__ __forceinline__ double fad_sinc(double x, double sinx)
+{
+    return fabs(x) < 1e-4 ? 1 - (0.1666666666666666666667) * x * x : sinx / x;
+}
+__device__ __forceinline__ double fad_sinh_taylor(double x)
+{
+    return x * (1 + (x * x) * (0.1666666666666666666667 + 0.00833333333333333333333 * (x * x)));
+}
+
+__device__ inline double erfcx_pos(double x)
+{
+    if (x < 25.0) {
+        const double h = x * x;
+        const double l = fma(x, x, -h);
+        return exp(h) * (1.0 + l) * erfc(x);
+    }
+    if (isinf(x)) return 0.0;
+    double t = x;
+    for (int k = 24; k >= 1; k--) t = x + (0.5 * k) / t;
+    return kIsPi / t;
+}
+
+__device__ inline double cts_erfcx(double x)
+{
+    if (isnan(x)) return x;
+    if (x < 0) {
+        if (x < -26.7) return HUGE_VAL;
+        return 2 * exp(x * x) - erfcx_pos(-x);
+    }
+    return erfcx_pos(x);
+}
+
+__device__ inline void w_general(double zr, double zi, double &wr_out, double &wi_out)
+{
+    const double a = c_fad[0], c = c_fad[1], a2 = c_fad[2];
+    const double relerr = 2.220446049250313e-16;
+    const double x = fabs(zr);
+    const double y = zi, ya = fabs(y);
+    double retr = 0, reti = 0;
+    double sum1 = 0, sum2 = 0, sum3 = 0, sum4 = 0, sum5 = 0;
+
+    if (ya > 7 || (x > 6 && (ya > 0.1 || (x > 8 && ya > 1e-10) || x > 28))) {
+        const double xs = y < 0 ? -zr : zr;
+        if (x + ya > 4000) {
+            if (x + ya > 1e7) {
+                if (x > ya) {
+                    const double yax = ya / xs;
+                    const double denom = kIsPi / (xs + yax * ya);
+                    retr = denom * yax; reti = denom;
+                } else if (isinf(ya)) {
+                    if (isnan(x) || y < 0) { wr_out = NAN; wi_out = NAN; }
+                    else { wr_out = 0; wi_out = 0; }
+                    return;
+                } else {
+                    const double xya = xs / ya;
+                    const double denom = kIsPi / (xya * xs + ya);
+                    retr = denom; reti = denom * xya;
+                }
+            } else {
+                const double dr = xs * xs - ya * ya - 0.5, di = 2 * xs * ya;
+                const double denom = kIsPi / (dr * dr + di * di);
+                retr = denom * (xs * di - ya * dr);
+                reti = denom * (xs * dr + ya * di);
+            }
+        } else {
+            // depth fit of the reference's package (src/Faddeeva.cpp:758-759)
+            double nu = floor(3.9 + 11.398 / (0.08254 * x + 0.1421 * ya + 0.2023));
+            double wr = xs, wi = ya;
+            for (nu = 0.5 * (nu - 1); nu > 0.4; nu -= 0.5) {
+                const double denom = nu / (wr * wr + wi * wi);
+                wr = xs - wr * denom;
+                wi = ya + wi * denom;
+            }
+            const double denom = kIsPi / (wr * wr + wi * wi);
+            retr = denom * wi;
+            reti = denom * wr;
+        }
+        if (y < 0) {
+            // w(z) = 2 exp(-z^2) - w(-z); exp(-inf + i*anything) = 0
+            const double re = (ya - xs) * (xs + ya);
+            double er_c = 0.0, er_s = 0.0;
+            if (!(isinf(re) && re < 0)) {
+                const double er = exp(re);
+                double s, cph;
+                sincos(2 * xs * y, &s, &cph);
+                er_c = er * cph;
+                er_s = er * s;
+            }
+            wr_out = 2.0 * er_c - retr;
+            wi_out = 2.0 * er_s - reti;
+        } else {
+            wr_out = retr; wi_out = reti;
+        }
+        return;
+    } else if (x < 10) {
+        double prod2ax = 1, prodm2ax = 1, expx2;
+        if (isnan(y)) { wr_out = y; wi_out = y; return; }
+        if (x < 5e-4) {
+            const double x2 = x * x;
+            expx2 = 1 - x2 * (1 - 0.5 * x2);
+            const double ax2 = (2 * a) * x;
+            const double exp2ax = 1 + ax2 * (1 + ax2 * (0.5 + 0.166666666666666666667 * ax2));
+            const double expm2ax = 1 - ax2 * (1 - ax2 * (0.5 - 0.166666666666666666667 * ax2));
+            for (int n = 1; n <= 64; ++n) {
+                const double coef = c_ea2n2[n - 1] * expx2 / (a2 * (n * n) + y * y);
+                prod2ax *= exp2ax;
+                prodm2ax *= expm2ax;
+                sum1 += coef;
+                sum2 += coef * prodm2ax;
+                sum3 += coef * prod2ax;
+                sum5 += coef * (2 * a) * n * fad_sinh_taylor((2 * a) * n * x);
+                if (coef * prod2ax < relerr * sum3) break;
+            }
+        } else {
+            expx2 = exp(-x * x);
+            const double exp2ax = exp((2 * a) * x), expm2ax = 1 / exp2ax;
+            for (int n = 1; n <= 64; ++n) {
+                const double coef = c_ea2n2[n - 1] * expx2 / (a2 * (n * n) + y * y);
+                prod2ax *= exp2ax;
+                prodm2ax *= expm2ax;
+                sum1 += coef;
+                sum2 += coef * prodm2ax;
+                sum4 += (coef * prodm2ax) * (a * n);
+                sum3 += coef * prod2ax;
+                sum5 += (coef * prod2ax) * (a * n);
+                if ((coef * prod2ax) * (a * n) < relerr * sum5) break;
+            }
+        }
+        const double expx2erfcxy = y > -6 ? expx2 * cts_erfcx(y) : 2 * exp(y * y - x * x);
+        if (y > 5) {
+            const double sinxy = sin(x * y);
+            retr = (expx2erfcxy - c * y * sum1) * cos(2 * x * y) +
+                   (c * x * expx2) * sinxy * fad_sinc(x * y, sinxy);
+            reti = 0;
+        } else {
+            const double xs = zr;
+            const double sinxy = sin(xs * y);
+            double sin2xy, cos2xy;
+            sincos(2 * xs * y, &sin2xy, &cos2xy);
+            const double coef1 = expx2erfcxy - c * y * sum1;
+            const double coef2 = c * xs * expx2;
+            retr = coef1 * cos2xy + coef2 * sinxy * fad_sinc(xs * y, sinxy);
+            reti = coef2 * fad_sinc(2 * xs * y, sin2xy) - coef1 * sin2xy;
+        }
+    } else {
+        if (isnan(x)) { wr_out = x; wi_out = x; return; }
+        if (isnan(y)) { wr_out = y; wi_out = y; return; }
+        retr = exp(-x * x);
+        reti = 0;
+        const double n0 = floor(x / a + 0.5);
+        const double dx = a * n0 - x;
+        sum3 = exp(-dx * dx) / (a2 * (n0 * n0) + y * y);
+        sum5 = a * n0 * sum3;
+        const double exp1 = exp(4 * a * dx);
+        double exp1dn = 1;
+        int dn;
+        bool done = false;
+        for (dn = 1; n0 - dn > 0; ++dn) {
+            const double np = n0 + dn, nm = n0 - dn;
+            double tp = exp(-((a * dn + dx) * (a * dn + dx)));
+            double tm = tp * (exp1dn *= exp1);
+            tp /= (a2 * (np * np) + y * y);
+            tm /= (a2 * (nm * nm) + y * y);
+            sum3 += tp + tm;
+            sum5 += a * (np * tp + nm * tm);
+            if (a * (np * tp + nm * tm) < relerr * sum5) { done = true; break; }
+        }
+        for (int guard = 0; !done && guard < 200; guard++) {
+            const double np = n0 + dn++;
+            const double tp = exp(-((a * dn + dx) * (a * dn + dx))) / (a2 * (np * np) + y * y);
+            sum3 += tp;
+            sum5 += a * np * tp;
+            if (a * np * tp < relerr * sum5) done = true;
+        }
+    }
+    wr_out = retr + (0.5 * c) * y * (sum2 + sum3);
+    wi_out = reti + (0.5 * c) * copysign(sum5 - sum4, zr);
+}
+
+__device__ inline double cts_w_im(double x)
+{
+    if (fabs(x) < 0.5) {
+        const double m2x2 = -2.0 * x * x;
+        double term = x, sum = x;
+        for (int n = 1; n < 40; n++) {
+            term = term * m2x2 / (2 * n + 1);
+            sum += term;
+            if (fabs(term) < 1e-17 * fabs(sum)) break;
+        }
+        return 2 * kIsPi * sum;
+    }
+    double wr, wi;
+    w_general(x, 0.0, wr, wi);
+    return wi;
+}
+
+__device__ inline void faddeeva_w(double re, double im, double &wr, double &wi)
+{
+    if (re == 0.0) { wr = cts_erfcx(im); wi = re; return; }
+    if (im == 0) { wr = exp(-(re * re)); wi = cts_w_im(re); return; }
+    w_general(re, im, wr, wi);
+}
+
+// Bunch::BassettiErskine1 (src/Bunch.cpp:119-170)
+__device__ inline void bassetti_erskine(double posx, double posy, double sx, double sy,
+                                        double &fx, double &fy)
+{
+    const double ax = fabs(posx), ay = fabs(posy);
+    const double sigma = sqrt(2 * (sx * sx) - 2 * (sy * sy));
+    const double r = sy / sx;
+    const double coeffBE = -1 * sqrt(kPi) / sigma;
+    double w1r, w1i, w2r, w2i;
+    faddeeva_w(ax / sigma, ay / sigma, w1r, w1i);
+    faddeeva_w(r * ax / sigma, ay / sigma / r, w2r, w2i);
+    const double tx = ax / sx, ty = ay / sy;
+    const double z3 = -(tx * tx) / 2 - (ty * ty) / 2;
+    const double w3 = -exp(z3);
+    const double w0r = coeffBE * (w1r + w3 * w2r);
+    const double w0i = coeffBE * (w1i + w3 * w2i);
+    fx = posx <= 0 ? -w0i : w0i;
+    fy = posy <= 0 ? -w0r : w0r;
+}
+
+// Bunch::GaussianField (src/Bunch.cpp:202-211)
+__device__ inline void gaussian_field(double posx, double posy, double sx, double &fx, double &fy)
+{
+    const double r2 = posx * posx + posy * posy;
+    const double sigma2 = sx * sx;
+    const double e = 1 - exp(-r2 / 2 / sigma2);
+    fx = -posx / r2 * e;
+    fy = -posy / r2 * e;
+}
+
+// Branches of MPBunch::SSIonBunchInteraction (src/MPBunch.cpp:1128-1147)
+__device__ inline void ion_field(double x, double y, double sx, double sy, double &fx, double &fy)
+{
+    if (fabs(x / sx) + fabs(y / sy) < 1.0E-5) {
+        fx = 0; fy = 0;
+    } else if ((sx - sy) / sy > 1.e-4) {
+        bassetti_erskine(x, y, sx, sy, fx, fy);
+    } else if ((sx - sy) / sy < -1.e-4) {
+        bassetti_erskine(y, x, sy, sx, fy, fx);
+    } else {
+        gaussian_field(x, y, sx, fx, fy);
+    }
+}
+
+}  // namespace cts

@@ -1,0 +1,135 @@
+// cts_internal.hpp -- kernel parameter blocks and launch wrappers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/cetasim_hip.h"
+#include "cts_device.hpp"
+#include "cts_faddeeva.hpp"
+
+namespace cts {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kNewLost = 0x80u;
+
+enum : uint32_t {
+    OP_KICK = 1u << 0,
+    OP_MOMB = 1u << 1,
+    OP_ION = 1u << 2,
+    OP_MAP = 1u << 3,
+    OP_ZSTAT = 1u << 4,
+    OP_SKEW = 1u << 5,
+    OP_RF = 1u << 6,
+    OP_DRIFT = 1u << 7,
+    OP_U0 = 1u << 8,
+    OP_SYNRAD = 1u << 9,
+    OP_BIN = 1u << 10,
+    OP_LOSS = 1u << 11,
+    OP_MOMA = 1u << 12,
+    OP_WRITE = 1u << 13,
+    OP_MOMD = 1u << 14,  // z/dp moments at the end of the pass (no wake)
+};
+
+// per-segment partial sums
+enum {
+    A_N = 0, A_S = 1, A_Q = 6, A_LOSS = 13, A_ZMIN = 14, A_ZMAX = 15,
+    B_N = 16, B_SZ = 17, B_SD = 18, B_QDD = 19, B_QZD = 20,
+    C_N = 21, C_SZ = 22, C_QZZ = 23, C_ZMIN = 24, C_ZMAX = 25,
+    kNPart = 26
+};
+
+struct PassParams {
+    double *x, *px, *y, *py, *z, *dp;
+    uint8_t *flag;
+    const int64_t *seg_begin, *seg_end;
+    const int32_t *seg_bunch, *wg_seg;
+    const int64_t *bunch_off;
+    double *seg_part;
+    uint32_t *bunch_hist;
+    const double *shift;
+    const double *kick_prev, *edges_prev, *edges_now;
+    const double *ion_e;
+    const double *noise;
+    int64_t ntot;
+    uint32_t ops;
+    int32_t nbins, nspec, n_rf, noise_mode;
+    uint32_t turn;
+    uint64_t seed;
+    // section map
+    SparseMat b1h1, h2b2;
+    double eta_x, etap_x, eta_y, etap_y, alpha_x, beta_x, alpha_y, beta_y;
+    double phase_x, phase_y, weigh_x, weigh_y, chrom_x, chrom_y;
+    double adtx[5], adty[5];
+    double skew_k;
+    double rf_vr[CTS_MAX_RF], rf_vi[CTS_MAX_RF], rf_k[CTS_MAX_RF], rf_scale;
+    double drift_c, alphac[3], u0_kick;
+    SparseMat sr_b1h1, sr_h1b1;
+    double sr_lambda[3], sr_lz2, sr_coeff[3];
+    double zlim, ap_x, ap_y;
+};
+
+struct IonKickParams {
+    const double *x, *y;
+    double *px, *py;
+    const uint8_t *flag;
+    const int64_t *seg_begin, *seg_end;
+    const int32_t *seg_bunch, *wg_seg;
+    const double *ion_e;  // per bunch CTS_MAX_SPECIES x {coeffE, xbar, ybar, sx, sy}
+    int32_t nspec;
+};
+
+struct BinParams {
+    const double *z;
+    const int64_t *seg_begin, *seg_end;
+    const int32_t *seg_bunch, *wg_seg;
+    const double *edges;
+    uint32_t *bunch_hist;
+    int32_t nbins;
+};
+
+struct FinParams {
+    const double *seg_part;
+    const int32_t *bunch_seg;
+    const int64_t *bunch_count;
+    double *shift;
+    cts_moments *cur;
+    cts_moments *hist;  // history ring or null
+    int32_t slot, prev_slot, nb, nbins, ref_edges;
+    uint32_t ops;
+    double *sigz_stale;
+    double *zrange;     // survivors' z range (GetZMinMax, without the pad)
+    double *edges_now;
+    uint32_t *status;
+};
+
+struct WakeTableParams {
+    const double *edges;  // per table: {lo, dz}
+    double *table;
+    int32_t nbins, nres;
+    double rs[CTS_MAX_RES], q[CTS_MAX_RES], omega[CTS_MAX_RES];
+};
+
+struct WakeParams {
+    uint32_t *bunch_hist;
+    int32_t *counts_out;
+    const double *table;
+    int32_t per_bunch_table;
+    const double *scale;
+    double *poten, *kick;
+    const double *edges_now;
+    double *edges_prev;
+    int32_t nbins;
+};
+
+hipError_t launch_pass(const PassParams &p, int nwg, hipStream_t st);
+hipError_t launch_bin(const BinParams &p, int nwg, hipStream_t st);
+hipError_t launch_ion_kick(const IonKickParams &p, int nwg, hipStream_t st);
+hipError_t launch_finalize(const FinParams &p, hipStream_t st);
+hipError_t launch_wake_table(const WakeTableParams &p, int ntables, hipStream_t st);
+hipError_t launch_wake(const WakeParams &p, int nb, hipStream_t st);
+hipError_t launch_qe_debug(uint64_t seed, uint32_t turn, uint64_t base, int64_t n, double *out,
+                           hipStream_t st);
+hipError_t upload_faddeeva_tables(const double ea2n2[64], const double fad[3]);
+
+}  // namespace cts

@@ -1,0 +1,144 @@
+"""Thin Python handle over the C ABI (plumbing for tests and the benchmark)."""
+import ctypes
+
+import numpy as np
+
+from . import capi
+
+COORDS = ("x", "px", "y", "py", "z", "dp")
+
+
+class Tracker:
+    """One device handle: ring + sections + bunches."""
+
+    def __init__(self, ring, sections):
+        self.lib = capi.load()
+        self.h = ctypes.c_void_p()
+        secs = (capi.Section * len(sections))(*sections)
+        capi.check(self.lib.cts_create(ring, secs, len(sections), ctypes.byref(self.h)))
+        self.ring = ring
+        self.nb = 0
+        self.ntot = 0
+
+    def close(self):
+        if self.h:
+            self.lib.cts_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_bunches(self, counts, macro_charge, n_e, gap=None):
+        counts = np.ascontiguousarray(counts, dtype=np.int64)
+        q = np.ascontiguousarray(macro_charge, dtype=np.float64)
+        ne = np.ascontiguousarray(n_e, dtype=np.float64)
+        g = None if gap is None else np.ascontiguousarray(gap, dtype=np.int32)
+        capi.check(self.lib.cts_set_bunches(self.h, len(counts), capi.i64ptr(counts), capi.dptr(q),
+                                            capi.dptr(ne), None if g is None else capi.i32ptr(g)))
+        self.nb = len(counts)
+        self.counts = counts
+        self.ntot = int(counts.sum())
+
+    def upload(self, parts, lost=None):
+        arrs = [np.ascontiguousarray(parts[c], dtype=np.float64) for c in COORDS]
+        lp = None
+        if lost is not None:
+            lost = np.ascontiguousarray(lost, dtype=np.int32)
+            lp = capi.i32ptr(lost)
+        capi.check(self.lib.cts_upload(self.h, *[capi.dptr(a) for a in arrs], lp))
+
+    def download(self):
+        out = {c: np.empty(self.ntot) for c in COORDS}
+        lost = np.empty(self.ntot, dtype=np.int32)
+        capi.check(self.lib.cts_download(self.h, *[capi.dptr(out[c]) for c in COORDS], capi.i32ptr(lost)))
+        out["lost"] = lost
+        return out
+
+    def set_wake(self, nbins, mode, grid_lo, grid_hi, rs, q, fr):
+        rs = np.ascontiguousarray(rs, dtype=np.float64)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        fr = np.ascontiguousarray(fr, dtype=np.float64)
+        capi.check(self.lib.cts_set_wake(self.h, nbins, mode, grid_lo, grid_hi, len(rs), capi.dptr(rs),
+                                         capi.dptr(q), capi.dptr(fr)))
+        self.nbins = nbins
+
+    def set_synrad(self, on, noise_mode=0, seed=0):
+        capi.check(self.lib.cts_set_synrad(self.h, int(on), noise_mode, seed))
+
+    def push_noise(self, noise):
+        noise = np.ascontiguousarray(noise, dtype=np.float64)
+        assert noise.size == 5 * self.ntot
+        capi.check(self.lib.cts_push_noise(self.h, capi.dptr(noise)))
+
+    def turn(self, n):
+        return capi.check(self.lib.cts_turn(self.h, n))
+
+    def flush(self):
+        capi.check(self.lib.cts_flush(self.h))
+
+    def sync(self):
+        capi.check(self.lib.cts_sync(self.h))
+
+    def seam(self, name, *args):
+        return capi.check(getattr(self.lib, "cts_" + name)(self.h, *args))
+
+    def mark_lost(self):
+        t = np.empty(self.nb)
+        rc = self.lib.cts_mark_lost(self.h, capi.dptr(t))
+        if rc not in (capi.CTS_OK, capi.CTS_LOW_TRANSMISSION):
+            capi.check(rc)
+        return t, rc
+
+    def sr_wake(self, sigma_z_stale=None):
+        if sigma_z_stale is None:
+            capi.check(self.lib.cts_sr_wake(self.h, None))
+        else:
+            s = np.ascontiguousarray(sigma_z_stale, dtype=np.float64)
+            capi.check(self.lib.cts_sr_wake(self.h, capi.dptr(s)))
+
+    def rms(self):
+        m = (capi.Moments * self.nb)()
+        capi.check(self.lib.cts_rms(self.h, m))
+        return [moments_dict(x) for x in m]
+
+    def moments(self):
+        m = (capi.Moments * self.nb)()
+        capi.check(self.lib.cts_get_moments(self.h, m))
+        return [moments_dict(x) for x in m]
+
+    def set_history(self, cap):
+        capi.check(self.lib.cts_set_history(self.h, cap))
+
+    def history(self, first, nturns):
+        m = (capi.Moments * (self.nb * nturns))()
+        capi.check(self.lib.cts_get_history(self.h, first, nturns, m))
+        return [moments_dict(x) for x in m]
+
+    def wake_state(self):
+        counts = np.empty(self.nb * self.nbins, dtype=np.int32)
+        edges = np.empty(self.nb * 2)
+        kick = np.empty(self.nb * self.nbins)
+        capi.check(self.lib.cts_get_wake_state(self.h, capi.i32ptr(counts), capi.dptr(edges), capi.dptr(kick)))
+        return counts.reshape(self.nb, self.nbins), edges.reshape(self.nb, 2), kick.reshape(self.nb, self.nbins)
+
+    def set_timing(self, on):
+        capi.check(self.lib.cts_set_timing(self.h, int(on)))
+
+    def kernel_time(self, name):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        capi.check(self.lib.cts_kernel_time(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def qe_normals(self, seed, turn, base, n):
+        out = np.empty(5 * n)
+        capi.check(self.lib.cts_debug_qe_normals(self.h, seed, turn, base, n, capi.dptr(out)))
+        return out.reshape(5, n)
+
+
+def moments_dict(m):
+    return dict(n_alive=m.n_alive, mean=np.array(m.mean[:]), c2=np.array(m.c2[:]), zmin=m.zmin,
+                zmax=m.zmax, loss_count=m.loss_count, transmission=m.transmission)

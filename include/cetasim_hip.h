@@ -1,0 +1,206 @@
+/*
+ * cetasim_hip.h -- C ABI of the MI355X multi-particle tracking turn.
+ *
+ * The reference has no plugin or FFI layer for this path: its seams are the
+ * MPBeam member functions called from MPBeam::Run (src/MPBeam.cpp:1010-1173).
+ * Each entry point below names the seam it replaces.  A host integration
+ * keeps MPBeam/MPBunch/LatticeInterActionPoint/WakeFunction and calls these
+ * functions instead of looping over beamVec.
+ *
+ * Conventions
+ *   - every function returns CTS_OK (0) or a negative/positive status code;
+ *     cts_last_error() returns the message of the last failure;
+ *   - all pointers are host pointers; the library owns device memory;
+ *   - particle arrays are structure-of-arrays, all bunches concatenated in
+ *     bunch order (bunch b occupies [offset_b, offset_b + count_b));
+ *   - loss codes are the reference's eSurive values (0 alive, 1 transverse,
+ *     2 longitudinal), src/Bunch.cpp:172-200;
+ *   - one HIP stream per handle; a handle is not thread-safe.
+ */
+#ifndef CETASIM_HIP_H
+#define CETASIM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CTS_OK 0
+#define CTS_ERR_ARG (-1)
+#define CTS_ERR_HIP (-2)
+#define CTS_ERR_STATE (-3)
+/* transmission < 0.5: the reference prints a message and exits
+ * (src/Bunch.cpp:193-198) */
+#define CTS_LOW_TRANSMISSION 1
+/* ion cloud above ionCalIonMaxNumber: the reference exits
+ * (src/LatticeInterActionPoint.cpp:1041-1049) */
+#define CTS_ION_OVERFLOW 2
+
+#define CTS_MAX_RF 4
+#define CTS_MAX_RES 8
+#define CTS_MAX_SPECIES 4
+
+typedef struct cts_handle cts_handle;
+
+/* Ring constants: ReadInputSettings derived quantities
+ * (src/ReadInputSettings.cpp:986-1077) and the cavity phasors of
+ * CavityResonator::Initial (src/CavityResonator.cpp:121-132). */
+typedef struct {
+    double circ;            /* circRing [m] */
+    double energy;          /* electronBeamEnergy [eV] */
+    double rbeta, rgamma;   /* rBeta, rGamma */
+    double t0, f0;          /* revolution period [s], frequency [Hz] */
+    int32_t harmonic;       /* ringHarm */
+    int32_t n_rf;           /* resNum (rfMode 0 cavities) */
+    double u0;              /* energy loss per turn [eV] */
+    double alphac[3];       /* momentum compaction a1 a2 a3 */
+    double nux, nuy;        /* workQx, workQy */
+    double chrom[2];        /* chromaticity */
+    double adtx[5], adty[5];/* amplitude-dependent tune shift coefficients */
+    double skew_k;          /* thin skew quadrupole K (0: off) */
+    double synrad_tau[3];   /* damping times [turns] */
+    double emit_nat[3];     /* natural emittances (emitNat) */
+    double rf_vr[CTS_MAX_RF], rf_vi[CTS_MAX_RF]; /* resCavVolReq [V] */
+    int32_t rf_harm[CTS_MAX_RF];                 /* resHarm */
+    double ap_x, ap_y;      /* pipeAperatureX/Y[0] used by the loss test */
+} cts_ring;
+
+/* One section / interaction point k (LatticeInterActionPoint,
+ * src/LatticeInterActionPoint.cpp:181-431). */
+typedef struct {
+    double b1h1[36];        /* symplecticMapB1H1[k], row-major */
+    double h1b1[36];        /* symplecticMapInvH1InvB1[k] */
+    double h2b2[36];        /* symplecticMapInvH2InvB2[k] */
+    double eta_x, etap_x, eta_y, etap_y;
+    double alpha_x, beta_x, alpha_y, beta_y;
+    double phase_adv_x, phase_adv_y; /* phaseAdvX12[k], phaseAdvY12[k] */
+} cts_section;
+
+/* Per-bunch moments, the quantities of MPBunch::GetMPBunchRMS
+ * (src/MPBunch.cpp:342-498) in raw form: centred second moments c2 =
+ * <xx> <xpx> <pxpx> <yy> <ypy> <pypy> <zz> <zdp> <dpdp>.  Emittances and
+ * sizes follow from them as in the reference (cts_derive_moments). */
+typedef struct {
+    double n_alive;
+    double mean[6];         /* x px y py z dp */
+    double c2[9];
+    double zmin, zmax;      /* survivors' z range, without the +-1e-6 pad */
+    double loss_count;      /* MarkLostParticle's count (double counting kept) */
+    double transmission;
+} cts_moments;
+
+/* Derived quantities of GetMPBunchRMS. */
+typedef struct {
+    double emit[3], emit_eff[3];
+    double rms_x, rms_y, rms_eff_x, rms_eff_y;
+    double sigma_z, sigma_dp;
+} cts_derived;
+
+/* ---------------------------------------------------------------- setup */
+int cts_create(const cts_ring *ring, const cts_section *secs, int32_t nsec,
+               cts_handle **out);
+int cts_destroy(cts_handle *h);
+const char *cts_last_error(void);
+/* Device and build description (for logs). */
+int cts_device_info(char *buf, int32_t len);
+
+/* MPBeam::Initial (src/MPBeam.cpp:64-161): bunch layout. counts[b] =
+ * macroEleNumPerBunch of bunch b (uneven counts allowed), macro_charge[b] =
+ * macroEleCharge (electrons per macroparticle), n_e[b] = electronNumPerBunch,
+ * gap[b] = bunchGap. */
+int cts_set_bunches(cts_handle *h, int32_t nb, const int64_t *counts,
+                    const double *macro_charge, const double *n_e, const int32_t *gap);
+
+/* Copy particle state in / out (the host vectors of Bunch,
+ * include/Bunch.h:43-55).  lost may be NULL on upload (all alive).
+ * Download applies any wake kick still pending from cts_turn first. */
+int cts_upload(cts_handle *h, const double *x, const double *px, const double *y,
+               const double *py, const double *z, const double *dp, const int32_t *lost);
+int cts_download(cts_handle *h, double *x, double *px, double *y, double *py,
+                 double *z, double *dp, int32_t *lost);
+
+/* WakeFunction::InitialSRWake / BBRWakeParaReadIn (src/WakeFunction.cpp:
+ * 232-290): broadband resonators (Rs [Ohm], Q, fr [Hz]) and the binning of
+ * MPBunch::BunchTransferDueToSRWake.  mode 0: the reference's edges
+ * [zmin-2*sigz, zmax+2*sigz]; mode 1: fixed grid [grid_lo, grid_hi].
+ * nbins <= 1024.  nres = 0 switches the wake off. */
+int cts_set_wake(cts_handle *h, int32_t nbins, int32_t mode, double grid_lo,
+                 double grid_hi, int32_t nres, const double *rs, const double *q,
+                 const double *fr_hz);
+
+/* Radiation damping switch and quantum-excitation noise source
+ * (BunchSynRadDamping, src/Bunch.cpp:940-1044).  noise_mode 0: damping
+ * only; 1: host-supplied normals (cts_push_noise) for parity runs;
+ * 2: on-device Philox4x32-10 stream keyed by seed. */
+int cts_set_synrad(cts_handle *h, int32_t on, int32_t noise_mode, uint64_t seed);
+/* noise[5][ntot]: r0 r1 r2 r3 r5 of every particle for the next synrad. */
+int cts_push_noise(cts_handle *h, const double *noise);
+
+/* ------------------------------------------------------------ the turn
+ * One turn of MPBeam::Run (src/MPBeam.cpp:1059-1102) for every bunch:
+ * [beam-ion phase at IP 0] -> section map -> skew quad -> RF -> drift ->
+ * U0 -> synrad -> SR wake -> loss marking -> moments.  One fused streaming
+ * pass; the wake kick is applied at the start of the next pass (or by
+ * cts_flush).  Returns CTS_LOW_TRANSMISSION when a bunch drops below 0.5
+ * (state stays consistent). Only one section is fused (nsec == 1). */
+int cts_turn(cts_handle *h, int32_t turn);
+/* Apply a pending wake kick and complete the last turn's moments. */
+int cts_flush(cts_handle *h);
+
+/* ------------------------------------------- seams, one kernel each
+ * MPBeam::BeamTransferPerInteractionPointDueToLatticeT (src/MPBeam.cpp:2613) */
+int cts_section_map(cts_handle *h, int32_t k);
+/* MPBeam::BeamTransferDueToSkewQuad (src/MPBeam.cpp:1210) */
+int cts_skew_quad(cts_handle *h);
+/* MPBeam::BeamMomtumUpdateDueToRFTest, rfMode 0 (src/MPBeam.cpp:2624) */
+int cts_rf_kick(cts_handle *h);
+/* MPBeam::BeamLongiPosTransferOneTurn (src/MPBeam.cpp:1204) */
+int cts_drift(cts_handle *h);
+/* MPBeam::BeamEnergyLossOneTurn (src/MPBeam.cpp:2708) */
+int cts_energy_loss(cts_handle *h);
+/* MPBeam::BeamSynRadDamping (src/MPBeam.cpp:1219); turn keys the device
+ * stream in noise_mode 2. */
+int cts_synrad(cts_handle *h, int32_t turn);
+/* MPBeam::SRWakeBeamIntaction (src/MPBeam.cpp:2117).  sigma_z_stale[b] is
+ * each bunch's rmsBunchLength as of the last MPBeamRMSCal (mode 0 only;
+ * NULL uses the library's last post-map value). */
+int cts_sr_wake(cts_handle *h, const double *sigma_z_stale);
+/* MPBeam::MarkParticleLostInBunch (src/MPBeam.cpp:1358); transmission[nb]
+ * may be NULL. */
+int cts_mark_lost(cts_handle *h, double *transmission);
+/* MPBeam::MPBeamRMSCal (src/MPBeam.cpp:2126): moments of the current state
+ * (applies a pending wake kick first). out[nb]. */
+int cts_rms(cts_handle *h, cts_moments *out);
+
+/* ------------------------------------------------------------ results */
+/* Moments at the end of the last completed turn (out[nb]). */
+int cts_get_moments(cts_handle *h, cts_moments *out);
+/* Turn-by-turn moments: out[(t - first) * nb + b] for turns
+ * first .. first+nturns-1 (kept for the last `capacity` turns). */
+int cts_set_history(cts_handle *h, int32_t capacity);
+int cts_get_history(cts_handle *h, int32_t first, int32_t nturns, cts_moments *out);
+/* Last binning: counts[nb*nbins], edges[nb*2] = {poszMin, dzBin},
+ * kick[nb*nbins] = srWakePoten[2] of each bunch. */
+int cts_get_wake_state(cts_handle *h, int32_t *counts, double *edges, double *kick);
+/* GetMPBunchRMS derived quantities from raw moments at section k. */
+int cts_derive_moments(cts_handle *h, const cts_moments *m, int32_t k, cts_derived *out);
+
+/* Wait for all work on the handle's stream. */
+int cts_sync(cts_handle *h);
+
+/* Kernel timing with HIP events on the handle's stream.  name is one of
+ * "pass", "finalize", "wake_table", "bin", "ion"; avg_ms is the mean
+ * duration of the launches recorded since timing was switched on. */
+int cts_set_timing(cts_handle *h, int32_t on);
+int cts_kernel_time(cts_handle *h, const char *name, double *avg_ms, int64_t *launches);
+
+/* Test hook: the device quantum-excitation stream for particles
+ * [base, base+n) at `turn` (noise_out[5][n]). */
+int cts_debug_qe_normals(cts_handle *h, uint64_t seed, int32_t turn, uint64_t base,
+                         int64_t n, double *noise_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,112 @@
+/*
+ * rng.c -- oracle replica of the project's counter-based noise stream.
+ * Test infrastructure only (see cts_oracle.h).
+ *
+ * The reference draws quantum-excitation noise and new ion positions from
+ * std::mt19937 seeded by std::random_device (src/Bunch.cpp:989-991,
+ * src/LatticeInterActionPoint.cpp:990-995), a stream no run can reproduce.
+ * The project replaces it with Philox4x32-10 (Salmon et al., "Parallel
+ * random numbers: as easy as 1, 2, 3", SC'11) and a float Box-Muller
+ * transform whose log, sin and cos are written here with +,-,*,/ and sqrt
+ * only, so the HIP path and this replica produce identical bits.
+ */
+#include "cts_oracle.h"
+
+#include <math.h>
+#include <string.h>
+
+void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; r++) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static inline float bits_to_float(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+static inline uint32_t float_to_bits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+
+/* ln(u) for u in (0, 1]: u = m * 2^e with m in [sqrt(1/2), sqrt(2)),
+ * ln m = 2 atanh(s), s = (m-1)/(m+1), odd series to s^9. */
+static float cts_logf(float u)
+{
+    uint32_t b = float_to_bits(u);
+    int e = (int)((b >> 23) & 0xffu) - 127;
+    float m = bits_to_float((b & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421356f) { m = m * 0.5f; e = e + 1; }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    const float p = 1.0f + s2 * (0.333333333f + s2 * (0.2f + s2 * (0.142857143f + s2 * 0.111111111f)));
+    return (float)e * 0.693147181f + 2.0f * s * p;
+}
+
+/* sin and cos of 2*pi*u2 for u2 = k*2^-24 in [0, 1): quadrant and octant
+ * reduction are exact, Taylor series on [0, pi/4]. */
+static void cts_sincos2pif(float u2, float *s_out, float *c_out)
+{
+    const float t = u2 * 4.0f;
+    const int q = (int)t;
+    float f = t - (float)q;
+    int swap = 0;
+    if (f > 0.5f) { f = 1.0f - f; swap = 1; }
+    const float th = f * 1.57079633f;
+    const float t2 = th * th;
+    const float s = th * (1.0f - t2 * (0.166666667f - t2 * (0.00833333333f - t2 * (0.000198412698f - t2 * 2.75573192e-6f))));
+    const float c = 1.0f - t2 * (0.5f - t2 * (0.0416666667f - t2 * (0.00138888889f - t2 * (2.48015873e-5f - t2 * 2.75573192e-7f))));
+    float sn = swap ? c : s, cs = swap ? s : c;
+    float rs, rc;
+    switch (q & 3) {
+    case 0: rs = sn; rc = cs; break;
+    case 1: rs = cs; rc = -sn; break;
+    case 2: rs = -sn; rc = -cs; break;
+    default: rs = -cs; rc = sn; break;
+    }
+    *s_out = rs;
+    *c_out = rc;
+}
+
+void orc_box_muller(uint32_t a, uint32_t b, float *n0, float *n1)
+{
+    const float u1 = (float)((a >> 8) + 1u) * 0x1p-24f; /* (0, 1] */
+    const float u2 = (float)(b >> 8) * 0x1p-24f;        /* [0, 1) */
+    const float r = sqrtf(-2.0f * cts_logf(u1));
+    float s, c;
+    cts_sincos2pif(u2, &s, &c);
+    *n0 = r * c;
+    *n1 = r * s;
+}
+
+void orc_qe_normals(uint64_t seed, uint32_t turn, uint64_t index, double out[5])
+{
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t ctr[4] = {(uint32_t)index, (uint32_t)(index >> 32), turn, 0u};
+    uint32_t o[4], o2[4];
+    orc_philox4x32_10(ctr, key, o);
+    ctr[3] = 1u;
+    orc_philox4x32_10(ctr, key, o2);
+    float n0, n1, n2, n3, n4, n5;
+    orc_box_muller(o[0], o[1], &n0, &n1);
+    orc_box_muller(o[2], o[3], &n2, &n3);
+    orc_box_muller(o2[0], o2[1], &n4, &n5);
+    out[0] = n0; out[1] = n1; out[2] = n2; out[3] = n3; out[4] = n4;
+}
+
+void orc_qe_fill(uint64_t seed, uint32_t turn, uint64_t base, int64_t n, double *noise)
+{
+    double r[5];
+    for (int64_t i = 0; i < n; i++) {
+        orc_qe_normals(seed, turn, base + (uint64_t)i, r);
+        for (int j = 0; j < 5; j++) noise[j * n + i] = r[j];
+    }
+}

@@ -1,0 +1,224 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so) for the tests.
+
+The oracle is test infrastructure: only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg load it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
+REF = os.path.join(ROOT, "oracle", "_ref", "libfaddeeva_ref.so")
+
+D = ctypes.c_double
+PD = ctypes.POINTER(D)
+PI32 = ctypes.POINTER(ctypes.c_int32)
+I64 = ctypes.c_int64
+U32 = ctypes.c_uint32
+U64 = ctypes.c_uint64
+
+
+class OrcBunch(ctypes.Structure):
+    _fields_ = [("n", I64), ("x", PD), ("px", PD), ("y", PD), ("py", PD), ("z", PD), ("dp", PD),
+                ("lost", PI32), ("fx_ion", PD), ("fy_ion", PD)]
+
+
+class OrcMap(ctypes.Structure):
+    _fields_ = [("b1h1", PD), ("h2b2", PD), ("eta_x", D), ("etap_x", D), ("eta_y", D), ("etap_y", D),
+                ("alpha_x", D), ("beta_x", D), ("alpha_y", D), ("beta_y", D),
+                ("phase_adv_x", D), ("phase_adv_y", D), ("nux", D), ("nuy", D),
+                ("chrom_x", D), ("chrom_y", D), ("adtx", D * 5), ("adty", D * 5)]
+
+
+class OrcTwiss(ctypes.Structure):
+    _fields_ = [("beta_x", D), ("alpha_x", D), ("eta_x", D), ("etap_x", D),
+                ("beta_y", D), ("alpha_y", D), ("eta_y", D), ("etap_y", D), ("beta_z", D)]
+
+
+class OrcMoments(ctypes.Structure):
+    _fields_ = [("n_alive", I64), ("mean", D * 6), ("emit", D * 3), ("emit_eff", D * 3),
+                ("rms_x", D), ("rms_y", D), ("rms_eff_x", D), ("rms_eff_y", D),
+                ("sigma_z", D), ("sigma_dp", D), ("zmin", D), ("zmax", D),
+                ("eigen_x", D), ("eigen_y", D), ("emit4d", D), ("xy_alpha", D), ("c2", D * 9)]
+
+
+class OrcIonCloud(ctypes.Structure):
+    _fields_ = [("nspec", ctypes.c_int), ("cap", I64), ("n", I64 * 4),
+                ("x", PD * 4), ("y", PD * 4), ("vx", PD * 4), ("vy", PD * 4), ("fx", PD * 4), ("fy", PD * 4),
+                ("macro_charge", D * 4), ("mean_x", D * 4), ("mean_y", D * 4), ("rms_x", D * 4),
+                ("rms_y", D * 4), ("mass_number", D * 4), ("cross_section", D * 4), ("pressure", D * 4),
+                ("temperature", D), ("length", D), ("n_new", ctypes.c_int), ("loss_boundary", D),
+                ("ap_x", D), ("ap_y", D), ("circ", D), ("harm", ctypes.c_int)]
+
+
+class OrcBunchView(ctypes.Structure):
+    _fields_ = [("xmean", D), ("ymean", D), ("rms_x", D), ("rms_y", D), ("n_e", D),
+                ("gap", ctypes.c_int), ("gamma", D)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE):
+            subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+        L = ctypes.CDLL(ORACLE)
+        L.orc_faddeeva_w.argtypes = [D, D, PD, PD]
+        L.orc_erfcx.restype = D; L.orc_erfcx.argtypes = [D]
+        L.orc_w_im.restype = D; L.orc_w_im.argtypes = [D]
+        L.orc_bassetti_erskine.argtypes = [D, D, D, D, PD, PD]
+        L.orc_gaussian_field.argtypes = [D, D, D, D, PD, PD]
+        L.orc_ion_field.argtypes = [D, D, D, D, PD, PD]
+        L.orc_section_matrices.argtypes = [ctypes.POINTER(OrcTwiss), ctypes.POINTER(OrcTwiss), PD, PD, PD]
+        L.orc_dgemm.argtypes = [ctypes.c_int] * 3 + [PD, PD, PD]
+        L.orc_section_map.argtypes = [ctypes.POINTER(OrcBunch), ctypes.POINTER(OrcMap)]
+        L.orc_skew_quad.argtypes = [ctypes.POINTER(OrcBunch), D]
+        L.orc_rf_kick.argtypes = [ctypes.POINTER(OrcBunch), D, D, ctypes.c_int, D, D, D]
+        L.orc_drift.argtypes = [ctypes.POINTER(OrcBunch), D, PD]
+        L.orc_energy_loss.argtypes = [ctypes.POINTER(OrcBunch), D, D, D]
+        L.orc_synrad.argtypes = [ctypes.POINTER(OrcBunch), PD, PD, PD, PD, PD]
+        L.orc_mark_lost.restype = D
+        L.orc_mark_lost.argtypes = [ctypes.POINTER(OrcBunch), D, ctypes.c_int, D, D, ctypes.POINTER(I64)]
+        L.orc_zminmax.argtypes = [ctypes.POINTER(OrcBunch), PD, PD]
+        L.orc_rms.argtypes = [ctypes.POINTER(OrcBunch), D, D, D, D, D, D, ctypes.POINTER(OrcMoments)]
+        L.orc_det.restype = D; L.orc_det.argtypes = [ctypes.c_int, PD]
+        L.orc_bbr_wake.argtypes = [D, ctypes.c_int] + [PD] * 9 + [PD]
+        L.orc_wake_table.argtypes = [ctypes.c_int, D, ctypes.c_int, PD, PD, PD, PD]
+        L.orc_sr_wake.restype = I64
+        L.orc_sr_wake.argtypes = [ctypes.POINTER(OrcBunch), ctypes.c_int, ctypes.c_int, D, D, D, D, D,
+                                  ctypes.c_int, PD, PD, PD, PD, PI32, PD, PD]
+        L.orc_philox4x32_10.argtypes = [ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32)]
+        L.orc_qe_normals.argtypes = [U64, U32, U64, PD]
+        L.orc_box_muller.argtypes = [U32, U32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+        L.orc_qe_fill.argtypes = [U64, U32, U64, I64, PD]
+        L.orc_ion_generate.argtypes = [U64, U32, U32, ctypes.c_int, ctypes.c_int, D, D, D, D, PD, PD]
+        L.orc_ion_rms.argtypes = [ctypes.POINTER(OrcIonCloud), ctypes.c_int]
+        L.orc_ion_passage.restype = ctypes.c_int
+        L.orc_ion_passage.argtypes = [ctypes.POINTER(OrcIonCloud), ctypes.POINTER(OrcBunch),
+                                      ctypes.POINTER(OrcBunchView), D, D, U64, U32, U32, PD]
+        L.orc_fit_wlinear.argtypes = [PD, PD, PD, ctypes.c_int, PD, PD]
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """Reference Faddeeva.cpp build (oracle/_ref), or None when absent."""
+    if not os.path.exists(REF):
+        return None
+    L = ctypes.CDLL(REF)
+    L.ref_faddeeva_w.argtypes = [D, D, PD, PD]
+    L.ref_erfcx.restype = D; L.ref_erfcx.argtypes = [D]
+    L.ref_w_im.restype = D; L.ref_w_im.argtypes = [D]
+    return L
+
+
+def p(a):
+    return a.ctypes.data_as(PD)
+
+
+COORDS = ("x", "px", "y", "py", "z", "dp")
+
+
+class Bunch:
+    """numpy-owned copy of one bunch with an OrcBunch view."""
+
+    def __init__(self, parts, lost=None, with_force=True):
+        self.a = {c: np.array(parts[c], dtype=np.float64, copy=True) for c in COORDS}
+        n = len(self.a["x"])
+        self.lost = np.zeros(n, np.int32) if lost is None else np.array(lost, dtype=np.int32, copy=True)
+        self.fx = np.zeros(n) if with_force else None
+        self.fy = np.zeros(n) if with_force else None
+        self.s = OrcBunch(n, *[p(self.a[c]) for c in COORDS], self.lost.ctypes.data_as(PI32),
+                          p(self.fx) if with_force else None, p(self.fy) if with_force else None)
+
+    def __getitem__(self, k):
+        return self.lost if k == "lost" else self.a[k]
+
+
+def section_matrices(t1, t2):
+    b1h1, h1b1, h2b2 = np.zeros(36), np.zeros(36), np.zeros(36)
+    lib().orc_section_matrices(t1, t2, p(b1h1), p(h1b1), p(h2b2))
+    return b1h1, h1b1, h2b2
+
+
+def make_map(sec, ring):
+    """OrcMap for a capi.Section + capi.Ring (keeps the matrix arrays alive)."""
+    b1 = np.array(sec.b1h1[:]); h2 = np.array(sec.h2b2[:])
+    m = OrcMap(p(b1), p(h2), sec.eta_x, sec.etap_x, sec.eta_y, sec.etap_y, sec.alpha_x, sec.beta_x,
+               sec.alpha_y, sec.beta_y, sec.phase_adv_x, sec.phase_adv_y, ring.nux, ring.nuy,
+               ring.chrom[0], ring.chrom[1])
+    m.adtx[:] = ring.adtx[:]
+    m.adty[:] = ring.adty[:]
+    m._keep = (b1, h2)
+    return m
+
+
+def rms(b, sec):
+    m = OrcMoments()
+    lib().orc_rms(ctypes.byref(b.s), sec.beta_x, sec.beta_y, sec.eta_x, sec.etap_x, sec.eta_y, sec.etap_y, m)
+    return m
+
+
+def qe_fill(seed, turn, base, n):
+    out = np.empty(5 * n)
+    lib().orc_qe_fill(seed, turn, base, n, p(out))
+    return out.reshape(5, n)
+
+
+class OracleTurn:
+    """The reference turn order (src/MPBeam.cpp:1059-1102) on one bunch,
+    with the bin edges of the reference (mode 0) or a fixed grid (mode 1)."""
+
+    def __init__(self, ring, sec, nbins=0, mode=1, grid=(0, 0), bbr=None, macro_charge=1.0,
+                 synrad=False, noise_seed=None, base=0):
+        self.L = lib()
+        self.ring, self.sec = ring, sec
+        self.map = make_map(sec, ring)
+        self.nbins, self.mode, self.grid = nbins, mode, grid
+        self.q = macro_charge
+        self.synrad, self.noise_seed, self.base = synrad, noise_seed, base
+        self.poten = np.zeros(3 * max(nbins, 1))
+        self.counts = np.zeros(max(nbins, 1), np.int32)
+        self.edges = np.zeros(2)
+        if nbins:
+            self.rs = np.array(bbr["rs"], float); self.qq = np.array(bbr["q"], float)
+            self.om = np.array(bbr["fr"], float) * 2 * np.pi
+            self.table = None
+            if mode == 1:
+                dz = (grid[1] - grid[0]) / nbins
+                self.table = np.zeros(2 * nbins - 1)
+                self.L.orc_wake_table(nbins, dz / 299792458.0, len(self.rs), p(self.rs), p(self.qq), p(self.om),
+                                      p(self.table))
+        self.b1h1 = np.array(sec.b1h1[:]); self.h1b1 = np.array(sec.h1b1[:])
+        self.tau = np.array(ring.synrad_tau[:]); self.en = np.array(ring.emit_nat[:])
+
+    def turn(self, b, turn, noise=None):
+        L, R = self.L, self.ring
+        L.orc_section_map(ctypes.byref(b.s), ctypes.byref(self.map))
+        sig_stale = rms(b, self.sec).sigma_z
+        if R.skew_k != 0:
+            L.orc_skew_quad(ctypes.byref(b.s), R.skew_k)
+        for r in range(R.n_rf):
+            L.orc_rf_kick(ctypes.byref(b.s), R.rf_vr[r], R.rf_vi[r], R.rf_harm[r], R.f0 * R.harmonic, R.rbeta,
+                          R.energy)
+        L.orc_drift(ctypes.byref(b.s), R.circ, p(np.array(R.alphac[:])))
+        L.orc_energy_loss(ctypes.byref(b.s), R.u0, R.energy, R.rbeta)
+        if self.synrad:
+            n = len(b.lost)
+            if noise is None and self.noise_seed is not None:
+                noise = qe_fill(self.noise_seed, turn, self.base, n)
+            L.orc_synrad(ctypes.byref(b.s), p(self.b1h1), p(self.h1b1), p(self.tau), p(self.en),
+                         None if noise is None else p(np.ascontiguousarray(noise).ravel()))
+        if self.nbins:
+            tab = self.table
+            L.orc_sr_wake(ctypes.byref(b.s), self.nbins, self.mode, sig_stale, self.grid[0], self.grid[1],
+                          self.q, R.energy, len(self.rs), p(self.rs), p(self.qq), p(self.om), p(self.poten),
+                          self.counts.ctypes.data_as(PI32), p(self.edges), None if tab is None else p(tab))
+        cnt = I64()
+        t = L.orc_mark_lost(ctypes.byref(b.s), R.t0, R.harmonic, R.ap_x, R.ap_y, ctypes.byref(cnt))
+        return t, cnt.value
