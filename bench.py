@@ -33,7 +33,21 @@ def parse():
     ap.add_argument("--nbins", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (reads MIB, writes MIB) three times, "
+                         "to scale FETCH_SIZE/WRITE_SIZE counters")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "launches": 3}))
 
 
 def cpu_baseline(ring, secs, p, budget_s):
@@ -65,6 +79,9 @@ def cpu_baseline(ring, secs, p, budget_s):
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
                                    "(multi-GPU sharding is a design note only)"}))

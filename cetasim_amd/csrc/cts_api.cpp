@@ -93,6 +93,13 @@ struct cts_handle {
     bool dp_pending = false;  // d_cur's dp part belongs to an older state
     double *d_sigz = nullptr, *d_zrange = nullptr;
     uint32_t *d_status = nullptr;
+    uint32_t *h_status = nullptr;      // pinned copy, written by the stream
+    hipEvent_t status_ev = nullptr;    // recorded after each copy
+    bool status_inflight = false;
+    int low_turn = -1;                 // first turn seen with transmission < 0.5
+
+    // per-section constant blocks (kNConst doubles each)
+    double *d_cst = nullptr;
 
     // wake
     int nbins = 0, wake_mode = 1, nres = 0;
@@ -204,6 +211,59 @@ struct TimedScope {
     }
 };
 
+// Constant block of section k (layout CI_* in cts_internal.hpp).
+void build_consts(const cts_ring &R, const std::vector<cts_section> &secs, int k, double *c)
+{
+    std::memset(c, 0, kNConst * sizeof(double));
+    const cts_section &S = secs[k];
+    const SparseMat b1h1 = compress(S.b1h1), h2b2 = compress(S.h2b2);
+    const SparseMat srb = compress(secs[0].b1h1), srh = compress(secs[0].h1b1);
+    for (int i = 0; i < 16; i++) {
+        c[CI_B1H1 + i] = b1h1.m[i];
+        c[CI_H2B2 + i] = h2b2.m[i];
+        c[CI_SRB + i] = srb.m[i];
+        c[CI_SRH + i] = srh.m[i];
+    }
+    c[CI_ETAX] = S.eta_x; c[CI_ETAPX] = S.etap_x; c[CI_ETAY] = S.eta_y; c[CI_ETAPY] = S.etap_y;
+    c[CI_ALPHAX] = S.alpha_x; c[CI_BETAX] = S.beta_x; c[CI_ALPHAY] = S.alpha_y; c[CI_BETAY] = S.beta_y;
+    c[CI_IBETAX] = 1.0 / S.beta_x;
+    c[CI_IBETAY] = 1.0 / S.beta_y;
+    c[CI_PHX] = S.phase_adv_x;
+    c[CI_PHY] = S.phase_adv_y;
+    // src/Bunch.cpp:658-659
+    c[CI_WX] = S.phase_adv_x / (2 * kPi * R.nux);
+    c[CI_WY] = S.phase_adv_y / (2 * kPi * R.nuy);
+    c[CI_CHX] = R.chrom[0];
+    c[CI_CHY] = R.chrom[1];
+    for (int i = 0; i < 5; i++) { c[CI_ADTX + i] = R.adtx[i]; c[CI_ADTY + i] = R.adty[i]; }
+    c[CI_SKEW] = R.skew_k;
+    // RF phase per unit z: the reference's -z/c/beta*2*pi*h*fRF
+    const double f_rf = R.f0 * R.harmonic;
+    for (int r = 0; r < R.n_rf && r < CTS_MAX_RF; r++) {
+        c[CI_RFVR + r] = R.rf_vr[r];
+        c[CI_RFVI + r] = R.rf_vi[r];
+        c[CI_RFK + r] = (((-1.0 / kClight) / R.rbeta) * 2. * kPi) * (double)R.rf_harm[r] * f_rf;
+    }
+    c[CI_RFSCALE] = 1.0 / R.energy / (R.rbeta * R.rbeta);
+    c[CI_DRIFTC] = R.circ;
+    for (int i = 0; i < 3; i++) c[CI_ALPHAC + i] = R.alphac[i];
+    c[CI_U0] = R.u0 / R.energy / std::pow(R.rbeta, 2);
+    // synrad constants (src/Bunch.cpp:971-980), section 0 matrices
+    double lam[3];
+    for (int i = 0; i < 3; i++) lam[i] = R.synrad_tau[i] > 0 ? std::exp(-1.0 / R.synrad_tau[i]) : 0.0;
+    for (int i = 0; i < 3; i++) c[CI_LAM + i] = lam[i];
+    c[CI_LZ2] = lam[2] * lam[2];
+    c[CI_COEF + 0] = std::sqrt(1 - std::pow(lam[0], 2)) * std::sqrt(R.emit_nat[0]);
+    c[CI_COEF + 1] = std::sqrt(1 - std::pow(lam[1], 2)) * std::sqrt(R.emit_nat[1]);
+    c[CI_COEF + 2] = std::sqrt(1 - std::pow(lam[2], 4)) * std::sqrt(R.emit_nat[2]);
+    // loss limits (src/Bunch.cpp:180-186)
+    c[CI_ZLIM] = R.t0 * kClight / R.harmonic / 2;
+    c[CI_APX] = R.ap_x;
+    c[CI_APY] = R.ap_y;
+    c[CI_IAPX] = 1.0 / R.ap_x;
+    c[CI_IAPY] = 1.0 / R.ap_y;
+}
+
 PassParams base_pass(cts_handle *h, uint32_t ops, int k)
 {
     PassParams p;
@@ -217,54 +277,15 @@ PassParams base_pass(cts_handle *h, uint32_t ops, int k)
     p.bunch_hist = h->d_bhist;
     p.shift = h->d_shift;
     p.kick_prev = h->d_kick; p.edges_prev = h->d_edges_prev; p.edges_now = h->d_edges_now;
-    p.ion_e = h->d_ion_e;
     p.noise = h->d_noise;
+    p.cst = (uint64_t)(uintptr_t)(h->d_cst + (size_t)k * kNConst);
     p.ntot = h->ntot;
     p.ops = ops;
+    p.nseg = h->nseg;
     p.nbins = h->nbins;
-    p.nspec = h->nspec;
     p.n_rf = h->ring.n_rf;
     p.noise_mode = h->noise_mode;
     p.seed = h->seed;
-
-    const cts_ring &R = h->ring;
-    const cts_section &S = h->secs[k];
-    p.b1h1 = compress(S.b1h1);
-    p.h2b2 = compress(S.h2b2);
-    p.eta_x = S.eta_x; p.etap_x = S.etap_x; p.eta_y = S.eta_y; p.etap_y = S.etap_y;
-    p.alpha_x = S.alpha_x; p.beta_x = S.beta_x; p.alpha_y = S.alpha_y; p.beta_y = S.beta_y;
-    p.phase_x = S.phase_adv_x; p.phase_y = S.phase_adv_y;
-    // src/Bunch.cpp:658-659
-    p.weigh_x = S.phase_adv_x / (2 * kPi * R.nux);
-    p.weigh_y = S.phase_adv_y / (2 * kPi * R.nuy);
-    p.chrom_x = R.chrom[0]; p.chrom_y = R.chrom[1];
-    for (int i = 0; i < 5; i++) { p.adtx[i] = R.adtx[i]; p.adty[i] = R.adty[i]; }
-    p.skew_k = R.skew_k;
-    // RF phase per unit z: the reference's -z/c/beta*2*pi*h*fRF
-    const double f_rf = R.f0 * R.harmonic;
-    for (int r = 0; r < R.n_rf && r < CTS_MAX_RF; r++) {
-        p.rf_vr[r] = R.rf_vr[r];
-        p.rf_vi[r] = R.rf_vi[r];
-        p.rf_k[r] = (((-1.0 / kClight) / R.rbeta) * 2. * kPi) * (double)R.rf_harm[r] * f_rf;
-    }
-    p.rf_scale = 1.0 / R.energy / (R.rbeta * R.rbeta);
-    p.drift_c = R.circ;
-    for (int i = 0; i < 3; i++) p.alphac[i] = R.alphac[i];
-    p.u0_kick = R.u0 / R.energy / std::pow(R.rbeta, 2);
-    // synrad constants (src/Bunch.cpp:971-980), section 0 matrices
-    p.sr_b1h1 = compress(h->secs[0].b1h1);
-    p.sr_h1b1 = compress(h->secs[0].h1b1);
-    double lam[3];
-    for (int i = 0; i < 3; i++) lam[i] = std::exp(-1.0 / R.synrad_tau[i]);
-    for (int i = 0; i < 3; i++) p.sr_lambda[i] = lam[i];
-    p.sr_lz2 = lam[2] * lam[2];
-    p.sr_coeff[0] = std::sqrt(1 - std::pow(lam[0], 2)) * std::sqrt(R.emit_nat[0]);
-    p.sr_coeff[1] = std::sqrt(1 - std::pow(lam[1], 2)) * std::sqrt(R.emit_nat[1]);
-    p.sr_coeff[2] = std::sqrt(1 - std::pow(lam[2], 4)) * std::sqrt(R.emit_nat[2]);
-    // loss limits (src/Bunch.cpp:180-186)
-    p.zlim = R.t0 * kClight / R.harmonic / 2;
-    p.ap_x = R.ap_x;
-    p.ap_y = R.ap_y;
     return p;
 }
 
@@ -288,6 +309,7 @@ FinParams base_fin(cts_handle *h, uint32_t ops, int slot, int prev_slot)
     f.ref_edges = h->wake_mode == 0 ? 1 : 0;
     f.edges_now = h->d_edges_now;
     f.status = h->d_status;
+    f.nseg = h->nseg;
     return f;
 }
 
@@ -376,11 +398,41 @@ int check_ready(cts_handle *h)
     return CTS_OK;
 }
 
-int read_status(cts_handle *h, uint32_t *st)
+// Transmission status: the finalize kernel sets a sticky device flag; a
+// pinned copy is refreshed asynchronously once per turn and polled without
+// blocking, so the turn loop never waits on the host.
+int enqueue_status_copy(cts_handle *h)
 {
-    HIPCHK(hipMemcpyAsync(st, h->d_status, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+    if (h->status_inflight) return CTS_OK;
+    HIPCHK(hipMemcpyAsync(h->h_status, h->d_status, sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipEventRecord(h->status_ev, h->stream));
+    h->status_inflight = true;
+    return CTS_OK;
+}
+
+int check_status(cts_handle *h, bool block)
+{
+    if (h->status_inflight) {
+        if (block) {
+            HIPCHK(hipEventSynchronize(h->status_ev));
+        } else {
+            const hipError_t q = hipEventQuery(h->status_ev);
+            if (q == hipErrorNotReady) return CTS_OK;
+            if (q != hipSuccess) return fail(CTS_ERR_HIP, std::string("status event: ") + hipGetErrorString(q));
+        }
+        h->status_inflight = false;
+    }
+    if (*h->h_status & 1u)
+        return fail(CTS_LOW_TRANSMISSION, "more than 50% particles are lost in a bunch (src/Bunch.cpp:193-198)");
+    return CTS_OK;
+}
+
+int reset_status(cts_handle *h)
+{
     HIPCHK(hipStreamSynchronize(h->stream));
-    HIPCHK(hipMemsetAsync(h->d_status, 0, sizeof(uint32_t), h->stream));
+    HIPCHK(hipMemset(h->d_status, 0, sizeof(uint32_t)));
+    *h->h_status = 0;
+    h->status_inflight = false;
     return CTS_OK;
 }
 
@@ -430,6 +482,15 @@ int cts_create(const cts_ring *ring, const cts_section *secs, int32_t nsec, cts_
     e = upload_faddeeva_tables(tab, fad);
     if (e == hipSuccess) e = hipMalloc((void **)&h->d_status, sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(h->d_status, 0, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&h->h_status, sizeof(uint32_t), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->status_ev, hipEventDisableTiming);
+    if (e == hipSuccess) {
+        *h->h_status = 0;
+        std::vector<double> cst((size_t)nsec * kNConst);
+        for (int k = 0; k < nsec; k++) build_consts(h->ring, h->secs, k, cst.data() + (size_t)k * kNConst);
+        e = hipMalloc((void **)&h->d_cst, cst.size() * sizeof(double));
+        if (e == hipSuccess) e = hipMemcpy(h->d_cst, cst.data(), cst.size() * sizeof(double), hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
         (void)hipStreamDestroy(h->stream);
         delete h;
@@ -447,6 +508,9 @@ int cts_destroy(cts_handle *h)
     for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
     free_bunch_state(h);
     if (h->d_status) (void)hipFree(h->d_status);
+    if (h->d_cst) (void)hipFree(h->d_cst);
+    if (h->h_status) (void)hipHostFree(h->h_status);
+    if (h->status_ev) (void)hipEventDestroy(h->status_ev);
     (void)hipStreamDestroy(h->stream);
     delete h;
     return CTS_OK;
@@ -567,6 +631,7 @@ int cts_upload(cts_handle *h, const double *x, const double *px, const double *y
     h->kick_pending = false;
     h->dp_pending = false;
     h->last_turn = -1;
+    if ((rc = reset_status(h))) return rc;
     // moments of the uploaded state (the turn-0 MPBeamRMSCal)
     return cts_rms(h, nullptr);
 }
@@ -684,6 +749,8 @@ int cts_turn(cts_handle *h, int32_t turn)
     int rc = check_ready(h);
     if (rc) return rc;
     if (h->secs.size() != 1) return fail(CTS_ERR_STATE, "cts_turn fuses one section; use the seam calls");
+    // a bunch below 50% transmission ends the run, as the reference exits
+    if ((rc = check_status(h, false))) return rc;
     uint32_t ops = OP_MAP | OP_RF | OP_DRIFT | OP_U0 | OP_LOSS | OP_MOMA | OP_WRITE;
     if (h->ring.skew_k != 0) ops |= OP_SKEW;
     if (h->synrad_on) ops |= OP_SYNRAD;
@@ -717,10 +784,7 @@ int cts_turn(cts_handle *h, int32_t turn)
     }
     h->last_turn = turn;
     h->dp_pending = wake;
-    uint32_t st = 0;
-    if ((rc = read_status(h, &st))) return rc;
-    if (st & 1u) return fail(CTS_LOW_TRANSMISSION, "more than 50% particles are lost in a bunch");
-    return CTS_OK;
+    return enqueue_status_copy(h);
 }
 
 int cts_flush(cts_handle *h)
@@ -794,15 +858,14 @@ int cts_mark_lost(cts_handle *h, double *transmission)
     if (rc) return rc;
     FinParams f = base_fin(h, OP_LOSS, -1, -1);
     if ((rc = run_finalize(h, f))) return rc;
-    uint32_t st = 0;
-    if ((rc = read_status(h, &st))) return rc;
+    if ((rc = enqueue_status_copy(h))) return rc;
     if (transmission) {
         std::vector<cts_moments> m(h->nb);
+        HIPCHK(hipStreamSynchronize(h->stream));
         HIPCHK(hipMemcpy(m.data(), h->d_cur, m.size() * sizeof(cts_moments), hipMemcpyDeviceToHost));
         for (int b = 0; b < h->nb; b++) transmission[b] = m[b].transmission;
     }
-    if (st & 1u) return fail(CTS_LOW_TRANSMISSION, "more than 50% particles are lost in a bunch");
-    return CTS_OK;
+    return check_status(h, true);
 }
 
 int cts_rms(cts_handle *h, cts_moments *out)
@@ -907,7 +970,7 @@ int cts_sync(cts_handle *h)
 {
     if (!h) return fail(CTS_ERR_ARG, "null handle");
     HIPCHK(hipStreamSynchronize(h->stream));
-    return CTS_OK;
+    return check_status(h, true);
 }
 
 int cts_set_timing(cts_handle *h, int32_t on)

@@ -72,10 +72,12 @@ __device__ __forceinline__ void cts_sincos2pif(float u2, float &s_out, float &c_
     }
 }
 
-__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float &n0, float &n1)
+// Box-Muller pair from two 24-bit integers: u1 = (a+1)*2^-24 in (0, 1],
+// u2 = b*2^-24 in [0, 1).
+__device__ __forceinline__ void box_muller24(uint32_t a, uint32_t b, float &n0, float &n1)
 {
-    const float u1 = (float)((a >> 8) + 1u) * 0x1p-24f;
-    const float u2 = (float)(b >> 8) * 0x1p-24f;
+    const float u1 = (float)(a + 1u) * 0x1p-24f;
+    const float u2 = (float)b * 0x1p-24f;
     const float r = __builtin_sqrtf(-2.0f * cts_logf(u1));
     float s, c;
     cts_sincos2pif(u2, s, c);
@@ -83,18 +85,30 @@ __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float &n0, fl
     n1 = r * s;
 }
 
-// The five quantum-excitation normals (r0 r1 r2 r3 r5) of one particle.
+__device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float &n0, float &n1)
+{
+    box_muller24(a >> 8, b >> 8, n0, n1);
+}
+
+// The five quantum-excitation normals (r0 r1 r2 r3 r5) of one particle:
+// one Philox4x32-10 block {index, turn, 0} cut into six 21-bit uniforms
+// (bits 0-20, 21-41, 42-62 of the two 64-bit halves), three Box-Muller
+// pairs, the sixth normal unused.  21-bit radii reach 5.4 sigma.
 __device__ __forceinline__ void qe_normals(uint64_t seed, uint32_t turn, uint64_t index,
                                            double out[5])
 {
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     const uint32_t i0 = (uint32_t)index, i1 = (uint32_t)(index >> 32);
     const u32x4 a = philox4x32_10(i0, i1, turn, 0u, k0, k1);
-    const u32x4 b = philox4x32_10(i0, i1, turn, 1u, k0, k1);
+    const uint64_t lo = ((uint64_t)a.v[1] << 32) | a.v[0], hi = ((uint64_t)a.v[3] << 32) | a.v[2];
+    const uint32_t m21 = 0x1fffffu;
+    const uint32_t u[6] = {(uint32_t)lo & m21, (uint32_t)(lo >> 21) & m21, (uint32_t)(lo >> 42) & m21,
+                           (uint32_t)hi & m21, (uint32_t)(hi >> 21) & m21, (uint32_t)(hi >> 42) & m21};
     float n0, n1, n2, n3, n4, n5;
-    box_muller(a.v[0], a.v[1], n0, n1);
-    box_muller(a.v[2], a.v[3], n2, n3);
-    box_muller(b.v[0], b.v[1], n4, n5);
+    // 21-bit values scaled to the 24-bit grid of box_muller24
+    box_muller24(u[0] << 3, u[1] << 3, n0, n1);
+    box_muller24(u[2] << 3, u[3] << 3, n2, n3);
+    box_muller24(u[4] << 3, u[5] << 3, n4, n5);
     out[0] = n0; out[1] = n1; out[2] = n2; out[3] = n3; out[4] = n4;
 }
 
@@ -114,6 +128,65 @@ __device__ __forceinline__ void spmv(const SparseMat &A, const double x[6], doub
     y[3] = A.m[7] * x[2] + A.m[8] * x[3] + A.m[9] * x[5];
     y[4] = A.m[10] * x[0] + A.m[11] * x[1] + A.m[12] * x[2] + A.m[13] * x[3] + A.m[14] * x[4];
     y[5] = A.m[15] * x[5];
+}
+
+// Constant-address-space view of a device buffer.  reload() hides the
+// address from the optimizer, so the loads it feeds stay inside the loop as
+// scalar loads instead of being hoisted and then spilled.
+typedef const __attribute__((address_space(4))) double *cdptr;
+
+__device__ __forceinline__ cdptr reload(uint64_t a)
+{
+    asm volatile("" : "+s"(a));
+    return (cdptr)a;
+}
+
+__device__ __forceinline__ void spmv_k(cdptr A, const double x[6], double y[6])
+{
+#pragma clang fp contract(fast)
+    y[0] = A[0] * x[0] + A[1] * x[5];
+    y[1] = A[2] * x[0] + A[3] * x[1] + A[4] * x[5];
+    y[2] = A[5] * x[2] + A[6] * x[5];
+    y[3] = A[7] * x[2] + A[8] * x[3] + A[9] * x[5];
+    y[4] = A[10] * x[0] + A[11] * x[1] + A[12] * x[2] + A[13] * x[3] + A[14] * x[4];
+    y[5] = A[15] * x[5];
+}
+
+// sin and cos in FP64: Cody-Waite reduction by pi/2 split into two 30-bit
+// parts and a remainder (n*P1, n*P2 exact for |n| < 2^23), Taylor series on
+// [-pi/4, pi/4] to degree 17/18 (truncation below 1e-19); arguments beyond
+// 1e5 take the library's sincos.
+__device__ __forceinline__ void sincos_fast(double x, double &s, double &c)
+{
+    if (!(fabs(x) < 1e5)) {
+        sincos(x, &s, &c);
+        return;
+    }
+    const double n = rint(x * 0.6366197723675814);
+    double r = x - n * 0x1.921fb548p+0;
+    r = r - n * -0x1.de973dc8p-31;
+    r = r - n * -0x1.9d9cceba3f91fp-62;
+    const double r2 = r * r;
+    const double ps = 2.8114572543455206e-15;
+    double sp = -7.647163731819816e-13 + r2 * ps;
+    sp = 1.6059043836821613e-10 + r2 * sp;
+    sp = -2.505210838544172e-08 + r2 * sp;
+    sp = 2.7557319223985893e-06 + r2 * sp;
+    sp = -0.0001984126984126984 + r2 * sp;
+    sp = 0.008333333333333333 + r2 * sp;
+    sp = -0.16666666666666666 + r2 * sp;
+    const double sn = r + r * (r2 * sp);
+    double cp = 1.5619206968586225e-16 * r2 - 4.779477332387385e-14;
+    cp = 1.1470745597729725e-11 + r2 * cp;
+    cp = -2.08767569878681e-09 + r2 * cp;
+    cp = 2.755731922398589e-07 + r2 * cp;
+    cp = -2.48015873015873e-05 + r2 * cp;
+    cp = 0.001388888888888889 + r2 * cp;
+    cp = -0.041666666666666664 + r2 * cp;
+    const double cs = 1.0 - 0.5 * r2 - r2 * (r2 * cp);
+    const int q = (int)n & 3;
+    s = (q == 0) ? sn : (q == 1) ? cs : (q == 2) ? -sn : -cs;
+    c = (q == 0) ? cs : (q == 1) ? -sn : (q == 2) ? -cs : sn;
 }
 
 // ---------------------------------------------------------- reductions

@@ -16,7 +16,6 @@ constexpr uint32_t kNewLost = 0x80u;
 enum : uint32_t {
     OP_KICK = 1u << 0,
     OP_MOMB = 1u << 1,
-    OP_ION = 1u << 2,
     OP_MAP = 1u << 3,
     OP_ZSTAT = 1u << 4,
     OP_SKEW = 1u << 5,
@@ -31,12 +30,27 @@ enum : uint32_t {
     OP_MOMD = 1u << 14,  // z/dp moments at the end of the pass (no wake)
 };
 
-// per-segment partial sums
+// per-segment partial sums, stored value-major: seg_part[j * nseg + s]
 enum {
     A_N = 0, A_S = 1, A_Q = 6, A_LOSS = 13, A_ZMIN = 14, A_ZMAX = 15,
     B_N = 16, B_SZ = 17, B_SD = 18, B_QDD = 19, B_QZD = 20,
     C_N = 21, C_SZ = 22, C_QZZ = 23, C_ZMIN = 24, C_ZMAX = 25,
     kNPart = 26
+};
+
+// Layout of the per-section constant block in device memory (doubles).
+// The pass reads it through scalar loads inside the particle loop, so the
+// ~120 lattice constants never have to stay resident in SGPRs.
+enum : int {
+    CI_B1H1 = 0, CI_H2B2 = 16, CI_SRB = 32, CI_SRH = 48,
+    CI_ETAX = 64, CI_ETAPX = 65, CI_ETAY = 66, CI_ETAPY = 67,
+    CI_ALPHAX = 68, CI_BETAX = 69, CI_ALPHAY = 70, CI_BETAY = 71,
+    CI_IBETAX = 72, CI_IBETAY = 73, CI_PHX = 74, CI_PHY = 75, CI_WX = 76, CI_WY = 77,
+    CI_CHX = 78, CI_CHY = 79, CI_ADTX = 80, CI_ADTY = 85, CI_SKEW = 90,
+    CI_RFVR = 91, CI_RFVI = 95, CI_RFK = 99, CI_RFSCALE = 103,
+    CI_DRIFTC = 104, CI_ALPHAC = 105, CI_U0 = 108, CI_LAM = 109, CI_LZ2 = 112, CI_COEF = 113,
+    CI_ZLIM = 116, CI_APX = 117, CI_APY = 118, CI_IAPX = 119, CI_IAPY = 120,
+    kNConst = 128
 };
 
 struct PassParams {
@@ -49,24 +63,13 @@ struct PassParams {
     uint32_t *bunch_hist;
     const double *shift;
     const double *kick_prev, *edges_prev, *edges_now;
-    const double *ion_e;
     const double *noise;
+    uint64_t cst;  // device address of the section's constant block
     int64_t ntot;
     uint32_t ops;
-    int32_t nbins, nspec, n_rf, noise_mode;
+    int32_t nseg, nbins, n_rf, noise_mode;
     uint32_t turn;
     uint64_t seed;
-    // section map
-    SparseMat b1h1, h2b2;
-    double eta_x, etap_x, eta_y, etap_y, alpha_x, beta_x, alpha_y, beta_y;
-    double phase_x, phase_y, weigh_x, weigh_y, chrom_x, chrom_y;
-    double adtx[5], adty[5];
-    double skew_k;
-    double rf_vr[CTS_MAX_RF], rf_vi[CTS_MAX_RF], rf_k[CTS_MAX_RF], rf_scale;
-    double drift_c, alphac[3], u0_kick;
-    SparseMat sr_b1h1, sr_h1b1;
-    double sr_lambda[3], sr_lz2, sr_coeff[3];
-    double zlim, ap_x, ap_y;
 };
 
 struct IonKickParams {
@@ -95,7 +98,7 @@ struct FinParams {
     double *shift;
     cts_moments *cur;
     cts_moments *hist;  // history ring or null
-    int32_t slot, prev_slot, nb, nbins, ref_edges;
+    int32_t slot, prev_slot, nb, nbins, ref_edges, nseg;
     uint32_t ops;
     double *sigz_stale;
     double *zrange;     // survivors' z range (GetZMinMax, without the pad)

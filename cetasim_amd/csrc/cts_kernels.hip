@@ -23,13 +23,90 @@ hipError_t upload_faddeeva_tables(const double ea2n2[64], const double fad[3])
     return hipMemcpyToSymbol(HIP_SYMBOL(c_fad), fad, 3 * sizeof(double));
 }
 
+// Bin of z for edges (lo, dz) as the reference computes it,
+// int((z - lo)/dz) (src/MPBunch.cpp:1917), or -1 outside [0, nbins).  The
+// quotient is formed with the reciprocal and redone with the division when
+// it lies within 1e-9 of an integer, where the two could truncate apart.
+__device__ __forceinline__ int bin_of(double z, double lo, double dz, double inv_dz, int nbins)
+{
+    const double d = z - lo;
+    double q = d * inv_dz;
+    const double t = floor(q);
+    if (q - t < 1e-9 || t + 1.0 - q < 1e-9) q = d / dz;
+    return (q > -1.0 && q < (double)nbins) ? (int)q : -1;
+}
+
+// (x/ax)^2 + (y/ay)^2 > 1 (src/Bunch.cpp:186-187); the division form is
+// used when the reciprocal form is within 1e-9 of the boundary.
+__device__ __forceinline__ bool outside_aperture(double x, double y, cdptr K)
+{
+    const double fx = x * K[CI_IAPX], fy = y * K[CI_IAPY];
+    double t = fx * fx + fy * fy;
+    if (fabs(t - 1.0) < 1e-9) {
+        const double ex = x / K[CI_APX], ey = y / K[CI_APY];
+        t = ex * ex + ey * ey;
+    }
+    return t > 1;
+}
+
+// Section map, src/Bunch.cpp:625-755.
+__device__ __forceinline__ void stage_map(double X[6], cdptr K)
+{
+#pragma clang fp contract(fast)
+    const double d = X[5];
+    const double xx = X[0] - K[CI_ETAX] * d, pxx = X[1] - K[CI_ETAPX] * d;
+    const double yy = X[2] - K[CI_ETAY] * d, pyy = X[3] - K[CI_ETAPY] * d;
+    const double tx = K[CI_ALPHAX] * xx + K[CI_BETAX] * pxx;
+    const double ty = K[CI_ALPHAY] * yy + K[CI_BETAY] * pyy;
+    const double ampX = (xx * xx + tx * tx) * K[CI_IBETAX];
+    const double ampY = (yy * yy + ty * ty) * K[CI_IBETAY];
+    const double dnux = (K[CI_CHX] * d + K[CI_ADTX + 0] * ampX + K[CI_ADTX + 1] * ampY +
+                         K[CI_ADTX + 2] * (ampX * ampX) * 0.5 + K[CI_ADTX + 3] * (ampY * ampY) * 0.5 +
+                         K[CI_ADTX + 4] * ampX * ampY) * K[CI_WX];
+    const double dnuy = (K[CI_CHY] * d + K[CI_ADTY + 0] * ampX + K[CI_ADTY + 1] * ampY +
+                         K[CI_ADTY + 2] * (ampX * ampX) * 0.5 + K[CI_ADTY + 3] * (ampY * ampY) * 0.5 +
+                         K[CI_ADTY + 4] * ampX * ampY) * K[CI_WY];
+    double sx, cx, sy, cy;
+    sincos_fast(K[CI_PHX] + 2 * kPi * dnux, sx, cx);
+    sincos_fast(K[CI_PHY] + 2 * kPi * dnuy, sy, cy);
+    double Y[6], R[6];
+    spmv_k(K + CI_B1H1, X, Y);
+    R[0] = cx * Y[0] + sx * Y[1];
+    R[1] = -sx * Y[0] + cx * Y[1];
+    R[2] = cy * Y[2] + sy * Y[3];
+    R[3] = -sy * Y[2] + cy * Y[3];
+    R[4] = Y[4];
+    R[5] = Y[5];
+    spmv_k(K + CI_H2B2, R, X);
+}
+
+// Radiation damping and quantum excitation, src/Bunch.cpp:940-1044.
+__device__ __forceinline__ void stage_synrad(double X[6], cdptr K, const double r[5], bool noise)
+{
+#pragma clang fp contract(fast)
+    double N[6];
+    spmv_k(K + CI_SRB, X, N);
+    N[0] *= K[CI_LAM + 0];
+    N[1] *= K[CI_LAM + 0];
+    N[2] *= K[CI_LAM + 1];
+    N[3] *= K[CI_LAM + 1];
+    N[5] *= K[CI_LZ2];
+    if (noise) {
+        N[0] += K[CI_COEF + 0] * r[0];
+        N[1] += K[CI_COEF + 0] * r[1];
+        N[2] += K[CI_COEF + 1] * r[2];
+        N[3] += K[CI_COEF + 1] * r[3];
+        N[5] += K[CI_COEF + 2] * r[4];
+    }
+    spmv_k(K + CI_SRH, N, X);
+}
+
 // --------------------------------------------------------------- the pass
 // One streaming pass over the particles; `ops` selects the stages, in the
 // order of MPBeam::Run (src/MPBeam.cpp:1059-1102):
 //   KICK   dp += wake kick of the previous binning (deferred SR wake kick,
 //          src/MPBunch.cpp:2015-2025), survivors at kick time only
 //   MOMB   moments of z/dp after the kick (completes the previous turn)
-//   ION    electron kick by the ion cloud (src/Bunch.cpp:213-220)
 //   MAP    section map (src/Bunch.cpp:625-755)
 //   ZSTAT  post-map z moments and survivors' z range (SR wake edges)
 //   SKEW   thin skew quad (src/Bunch.cpp:757-795)
@@ -38,8 +115,9 @@ hipError_t upload_faddeeva_tables(const double ea2n2[64], const double fad[3])
 //   BIN    longitudinal histogram (src/MPBunch.cpp:1915-1919)
 //   LOSS   loss marking (src/Bunch.cpp:172-200)
 //   MOMA   moments after marking (src/MPBunch.cpp:342-498)
+//   MOMD   z/dp moments after marking (no wake to defer)
 // M is the compile-time set of stages a variant may run; stages outside it
-// are compiled out, which keeps the register budget of the fused turn small.
+// are compiled out.
 __host__ __device__ constexpr bool acc_used(uint32_t M, int j)
 {
     return (j <= A_ZMAX && j != A_LOSS) ? (M & OP_MOMA) != 0
@@ -48,16 +126,20 @@ __host__ __device__ constexpr bool acc_used(uint32_t M, int j)
                                         : (M & OP_ZSTAT) != 0;
 }
 
-template <uint32_t M>
+template <uint32_t M, bool EXACT, int NOISE, int NRF>
 __global__ void __launch_bounds__(kBlock)
 k_pass(PassParams P)
 {
     extern __shared__ double smem_d[];
-    double *kick_lds = smem_d;                                       // nbins
+    double *kick_lds = smem_d;                                            // nbins
     uint32_t *hist_lds = reinterpret_cast<uint32_t *>(smem_d + P.nbins);  // nbins
     __shared__ double red[kBlock / 64][kNPart];
 
-    const uint32_t ops = P.ops & M;
+    // EXACT variants run exactly the stages in M (no runtime stage tests);
+    // NOISE >= 0 fixes the noise source at compile time
+    const uint32_t ops = EXACT ? M : (P.ops & M);
+    const int noise_mode = NOISE >= 0 ? NOISE : P.noise_mode;
+    const int n_rf = NRF >= 0 ? NRF : P.n_rf;
     const int tid = threadIdx.x;
     const int s_first = P.wg_seg[blockIdx.x], s_last = P.wg_seg[blockIdx.x + 1];
 
@@ -65,9 +147,8 @@ k_pass(PassParams P)
         const int b = P.seg_bunch[s];
         const int64_t i0 = P.seg_begin[s], i1 = P.seg_end[s];
         const int64_t bunch_first = P.bunch_off[b];
-        double sh[6];
-#pragma unroll
-        for (int j = 0; j < 6; j++) sh[j] = P.shift[b * 6 + j];
+        // the bunch's moment shift, re-read as scalars where it is used
+        const uint64_t sh_addr = (uint64_t)(uintptr_t)(P.shift + 6 * b);
 
         const bool use_hist = (ops & OP_BIN) && P.nbins > 0;
         const bool use_kick = (ops & OP_KICK) && P.nbins > 0;
@@ -78,6 +159,7 @@ k_pass(PassParams P)
         double klo = 0, kdz = 1, blo = 0, bdz = 1;
         if (use_kick) { klo = P.edges_prev[2 * b]; kdz = P.edges_prev[2 * b + 1]; }
         if (use_hist) { blo = P.edges_now[2 * b]; bdz = P.edges_now[2 * b + 1]; }
+        const double kinv = 1.0 / kdz, binv = 1.0 / bdz;
         __syncthreads();
 
         double acc[kNPart];
@@ -96,13 +178,15 @@ k_pass(PassParams P)
                 // survivors when the kick was computed: code 0, or lost at
                 // the marking that followed it
                 if ((f & 3u) == 0u || (f & kNewLost)) {
-                    const double q = (X[4] - klo) / kdz;
-                    if (q > -1.0 && q < (double)P.nbins) X[5] += kick_lds[(int)q];
+                    const int kb = bin_of(X[4], klo, kdz, kinv, P.nbins);
+                    if (kb >= 0) X[5] += kick_lds[kb];
                 }
             }
             f &= 3u;
             const bool alive = f == 0u;
             if ((ops & OP_MOMB) && alive) {
+#pragma clang fp contract(fast)
+                cdptr sh = reload(sh_addr);
                 const double dz = X[4] - sh[4], dd = X[5] - sh[5];
                 acc[B_N] += 1.0;
                 acc[B_SZ] += dz;
@@ -110,86 +194,46 @@ k_pass(PassParams P)
                 acc[B_QDD] += dd * dd;
                 acc[B_QZD] += dz * dd;
             }
-            if ((ops & OP_MAP) && alive) {
-                const double d = X[5];
-                const double xx = X[0] - P.eta_x * d, pxx = X[1] - P.etap_x * d;
-                const double yy = X[2] - P.eta_y * d, pyy = X[3] - P.etap_y * d;
-                const double tx = P.alpha_x * xx + P.beta_x * pxx;
-                const double ty = P.alpha_y * yy + P.beta_y * pyy;
-                const double ampX = (xx * xx + tx * tx) / P.beta_x;
-                const double ampY = (yy * yy + ty * ty) / P.beta_y;
-                const double dnux = (P.chrom_x * d + P.adtx[0] * ampX + P.adtx[1] * ampY +
-                                     P.adtx[2] * (ampX * ampX) / 2 + P.adtx[3] * (ampY * ampY) / 2 +
-                                     P.adtx[4] * ampX * ampY) * P.weigh_x;
-                const double dnuy = (P.chrom_y * d + P.adty[0] * ampX + P.adty[1] * ampY +
-                                     P.adty[2] * (ampX * ampX) / 2 + P.adty[3] * (ampY * ampY) / 2 +
-                                     P.adty[4] * ampX * ampY) * P.weigh_y;
-                const double phiX = P.phase_x + 2 * kPi * dnux;
-                const double phiY = P.phase_y + 2 * kPi * dnuy;
-                double sx, cx, sy, cy;
-                sincos(phiX, &sx, &cx);
-                sincos(phiY, &sy, &cy);
-                double Y[6];
-                spmv(P.b1h1, X, Y);
-                double R[6];
-                R[0] = cx * Y[0] + sx * Y[1];
-                R[1] = -sx * Y[0] + cx * Y[1];
-                R[2] = cy * Y[2] + sy * Y[3];
-                R[3] = -sy * Y[2] + cy * Y[3];
-                R[4] = Y[4];
-                R[5] = Y[5];
-                spmv(P.h2b2, R, X);
-            }
+            if ((ops & OP_MAP) && alive) stage_map(X, reload(P.cst));
             if ((ops & OP_ZSTAT) && alive) {
-                const double dz = X[4] - sh[4];
+                const double dz = X[4] - reload(sh_addr)[4];
                 acc[C_N] += 1.0;
                 acc[C_SZ] += dz;
                 acc[C_QZZ] += dz * dz;
             }
             if ((ops & OP_SKEW) && alive) {
                 // identity with (1,2) = (3,0) = K, summed in column order
-                const double npx = X[1] + P.skew_k * X[2];
-                const double npy = P.skew_k * X[0] + X[3];
+                cdptr K = reload(P.cst);
+                const double npx = X[1] + K[CI_SKEW] * X[2];
+                const double npy = K[CI_SKEW] * X[0] + X[3];
                 X[1] = npx;
                 X[3] = npy;
             }
             if (ops & OP_RF) {
-                for (int r = 0; r < P.n_rf; r++) {
-                    const double th = X[4] * P.rf_k[r];
+                cdptr K = reload(P.cst);
+                for (int r = 0; r < n_rf; r++) {
                     double s, c;
-                    sincos(th, &s, &c);
-                    const double re = P.rf_vr[r] * c - P.rf_vi[r] * s;
-                    X[5] += re * P.rf_scale;
+                    sincos_fast(X[4] * K[CI_RFK + r], s, c);
+                    const double re = K[CI_RFVR + r] * c - K[CI_RFVI + r] * s;
+                    X[5] += re * K[CI_RFSCALE];
                 }
             }
             if (ops & OP_DRIFT) {
+                cdptr K = reload(P.cst);
                 const double d = X[5];
-                X[4] -= P.drift_c * (P.alphac[0] * d + P.alphac[1] * (d * d) + P.alphac[2] * (d * d * d));
+                X[4] -= K[CI_DRIFTC] * (K[CI_ALPHAC] * d + K[CI_ALPHAC + 1] * (d * d) +
+                                         K[CI_ALPHAC + 2] * (d * d * d));
             }
-            if (ops & OP_U0) X[5] -= P.u0_kick;
+            if (ops & OP_U0) X[5] -= reload(P.cst)[CI_U0];
             if ((ops & OP_SYNRAD) && alive) {
-                double N[6];
-                spmv(P.sr_b1h1, X, N);
-                N[0] *= P.sr_lambda[0];
-                N[1] *= P.sr_lambda[0];
-                N[2] *= P.sr_lambda[1];
-                N[3] *= P.sr_lambda[1];
-                N[5] *= P.sr_lz2;
-                if (P.noise_mode != 0) {
-                    double r[5];
-                    if (P.noise_mode == 1) {
+                double r[5] = {0, 0, 0, 0, 0};
+                if (noise_mode == 1) {
 #pragma unroll
-                        for (int j = 0; j < 5; j++) r[j] = P.noise[j * P.ntot + i];
-                    } else {
-                        qe_normals(P.seed, P.turn, (uint64_t)i, r);
-                    }
-                    N[0] += P.sr_coeff[0] * r[0];
-                    N[1] += P.sr_coeff[0] * r[1];
-                    N[2] += P.sr_coeff[1] * r[2];
-                    N[3] += P.sr_coeff[1] * r[3];
-                    N[5] += P.sr_coeff[2] * r[4];
+                    for (int j = 0; j < 5; j++) r[j] = P.noise[j * P.ntot + i];
+                } else if (noise_mode == 2) {
+                    qe_normals(P.seed, P.turn, (uint64_t)i, r);
                 }
-                spmv(P.sr_h1b1, N, X);
+                stage_synrad(X, reload(P.cst), r, noise_mode != 0);
             }
             if ((ops & OP_ZSTAT) && (alive || i == bunch_first)) {
                 // GetZMinMax starts from particle 0 of the bunch even when it
@@ -198,19 +242,21 @@ k_pass(PassParams P)
                 acc[C_ZMAX] = fmax(acc[C_ZMAX], X[4]);
             }
             if (use_hist) {
-                const double q = (X[4] - blo) / bdz;
-                if (q > -1.0 && q < (double)P.nbins) atomicAdd(&hist_lds[(int)q], 1u);
+                const int hb = bin_of(X[4], blo, bdz, binv, P.nbins);
+                if (hb >= 0) atomicAdd(&hist_lds[hb], 1u);
             }
             uint32_t fnew = f;
             if (ops & OP_LOSS) {
+                cdptr K = reload(P.cst);
                 double cnt = 0.0;
-                if (fabs(X[4]) > P.zlim) { fnew = 2u; cnt += 1.0; }
-                const double ax = X[0] / P.ap_x, ay = X[2] / P.ap_y;
-                if (ax * ax + ay * ay > 1) { fnew = 1u; cnt += 1.0; }
+                if (fabs(X[4]) > K[CI_ZLIM]) { fnew = 2u; cnt += 1.0; }
+                if (outside_aperture(X[0], X[2], K)) { fnew = 1u; cnt += 1.0; }
                 acc[A_LOSS] += cnt;
                 if (f == 0u && fnew != 0u) fnew |= kNewLost;
             }
             if ((ops & OP_MOMA) && (fnew & 3u) == 0u) {
+#pragma clang fp contract(fast)
+                cdptr sh = reload(sh_addr);
                 const double d0 = X[0] - sh[0], d1 = X[1] - sh[1], d2 = X[2] - sh[2];
                 const double d3 = X[3] - sh[3], d4 = X[4] - sh[4];
                 acc[A_N] += 1.0;
@@ -223,6 +269,8 @@ k_pass(PassParams P)
                 acc[A_ZMAX] = fmax(acc[A_ZMAX], X[4]);
             }
             if ((ops & OP_MOMD) && (fnew & 3u) == 0u) {
+#pragma clang fp contract(fast)
+                cdptr sh = reload(sh_addr);
                 const double dz = X[4] - sh[4], dd = X[5] - sh[5];
                 acc[B_N] += 1.0;
                 acc[B_SZ] += dz;
@@ -261,7 +309,7 @@ k_pass(PassParams P)
                     else v = v + red[w][tid];
                 }
             }
-            P.seg_part[(int64_t)s * kNPart + tid] = v;
+            P.seg_part[(int64_t)tid * P.nseg + s] = v;
         }
         if (use_hist) {
             for (int j = tid; j < P.nbins; j += kBlock) {
@@ -287,11 +335,11 @@ k_bin(BinParams P)
         const int b = P.seg_bunch[s];
         const int64_t i0 = P.seg_begin[s], i1 = P.seg_end[s];
         for (int j = tid; j < P.nbins; j += kBlock) hist_lds[j] = 0u;
-        const double lo = P.edges[2 * b], dz = P.edges[2 * b + 1];
+        const double lo = P.edges[2 * b], dz = P.edges[2 * b + 1], inv = 1.0 / dz;
         __syncthreads();
         for (int64_t i = i0 + tid; i < i1; i += kBlock) {
-            const double q = (P.z[i] - lo) / dz;
-            if (q > -1.0 && q < (double)P.nbins) atomicAdd(&hist_lds[(int)q], 1u);
+            const int hb = bin_of(P.z[i], lo, dz, inv, P.nbins);
+            if (hb >= 0) atomicAdd(&hist_lds[hb], 1u);
         }
         __syncthreads();
         for (int j = tid; j < P.nbins; j += kBlock) {
@@ -303,47 +351,26 @@ k_bin(BinParams P)
 }
 
 // ----------------------------------------------------------- finalize
-// Per bunch: reduce segment partials in segment order, write the moment
+// Per bunch: reduce segment partials in a fixed order (each wave takes a
+// subset of the values, lanes stride over segments), write the moment
 // record, update the shift and (reference-edge mode) the bin edges.
 __global__ void __launch_bounds__(kBlock)
 k_finalize(FinParams P)
 {
-    __shared__ double red[kBlock / 64][kNPart];
     __shared__ double tot[kNPart];
     const int b = blockIdx.x, tid = threadIdx.x;
-    const int s0 = P.bunch_seg[b], s1 = P.bunch_seg[b + 1];
-    double acc[kNPart];
-#pragma unroll
-    for (int j = 0; j < kNPart; j++) acc[j] = 0.0;
-    acc[A_ZMIN] = INFINITY; acc[A_ZMAX] = -INFINITY;
-    acc[C_ZMIN] = INFINITY; acc[C_ZMAX] = -INFINITY;
-    for (int s = s0 + tid; s < s1; s += kBlock) {
-        const double *p = P.seg_part + (int64_t)s * kNPart;
-#pragma unroll
-        for (int j = 0; j < kNPart; j++) {
-            if (j == A_ZMIN || j == C_ZMIN) acc[j] = fmin(acc[j], p[j]);
-            else if (j == A_ZMAX || j == C_ZMAX) acc[j] = fmax(acc[j], p[j]);
-            else acc[j] += p[j];
-        }
-    }
     const int lane = tid & 63, wave = tid >> 6;
-#pragma unroll
-    for (int j = 0; j < kNPart; j++) {
-        double v;
-        if (j == A_ZMIN || j == C_ZMIN) v = wave_min(acc[j]);
-        else if (j == A_ZMAX || j == C_ZMAX) v = wave_max(acc[j]);
-        else v = wave_sum(acc[j]);
-        if (lane == 0) red[wave][j] = v;
-    }
-    __syncthreads();
-    if (tid < kNPart) {
-        double v = red[0][tid];
-        for (int w = 1; w < kBlock / 64; w++) {
-            if (tid == A_ZMIN || tid == C_ZMIN) v = fmin(v, red[w][tid]);
-            else if (tid == A_ZMAX || tid == C_ZMAX) v = fmax(v, red[w][tid]);
-            else v = v + red[w][tid];
+    const int s0 = P.bunch_seg[b], s1 = P.bunch_seg[b + 1];
+    for (int j = wave; j < kNPart; j += kBlock / 64) {
+        const bool is_min = j == A_ZMIN || j == C_ZMIN, is_max = j == A_ZMAX || j == C_ZMAX;
+        double a = is_min ? INFINITY : is_max ? -INFINITY : 0.0;
+        const double *p = P.seg_part + (int64_t)j * P.nseg;
+        for (int s = s0 + lane; s < s1; s += 64) {
+            const double v = p[s];
+            a = is_min ? fmin(a, v) : is_max ? fmax(a, v) : a + v;
         }
-        tot[tid] = v;
+        a = is_min ? wave_min(a) : is_max ? wave_max(a) : wave_sum(a);
+        if (lane == 0) tot[j] = a;
     }
     __syncthreads();
     if (tid != 0) return;
@@ -454,28 +481,31 @@ k_wake_table(WakeTableParams P)
 
 // --------------------------------------------------------------- wake
 // Per bunch: kick_i = (poten_i + sum_j W[i-j] * n_j) * (-e*q/E)
-// (src/MPBunch.cpp:1983-1998); the histogram is cleared for the next pass.
-__global__ void __launch_bounds__(kBlock)
+// (src/MPBunch.cpp:1983-1998), summed over j in order; table and counts
+// are staged in LDS.  The histogram is cleared for the next pass.
+__global__ void __launch_bounds__(1024)
 k_wake(WakeParams P)
 {
     extern __shared__ double smem_d[];
+    const int nb = P.nbins, nlag = 2 * nb - 1;
     double *n_lds = smem_d;
+    double *w_lds = smem_d + nb;
     const int b = blockIdx.x, tid = threadIdx.x;
-    const int nb = P.nbins;
     uint32_t *hist = P.bunch_hist + (int64_t)b * nb;
-    for (int j = tid; j < nb; j += kBlock) {
+    for (int j = tid; j < nb; j += blockDim.x) {
         const uint32_t c = hist[j];
         n_lds[j] = (double)c;
         P.counts_out[(int64_t)b * nb + j] = (int32_t)c;
         hist[j] = 0u;
     }
+    const double *W = P.table + (P.per_bunch_table ? (int64_t)b * nlag : 0);
+    for (int j = tid; j < nlag; j += blockDim.x) w_lds[j] = W[j];
     __syncthreads();
-    const double *W = P.table + (P.per_bunch_table ? (int64_t)b * (2 * nb - 1) : 0);
     const double scale = P.scale[b];
     double *poten = P.poten + (int64_t)b * nb;
-    for (int i = tid; i < nb; i += kBlock) {
+    for (int i = tid; i < nb; i += blockDim.x) {
         double a = poten[i];
-        const double *Wi = W + i + (nb - 1);
+        const double *Wi = w_lds + i + (nb - 1);
         for (int j = 0; j < nb; j++) a += Wi[-j] * n_lds[j];
         a *= scale;
         poten[i] = a;
@@ -485,17 +515,6 @@ k_wake(WakeParams P)
         P.edges_prev[2 * b] = P.edges_now[2 * b];
         P.edges_prev[2 * b + 1] = P.edges_now[2 * b + 1];
     }
-}
-
-// -------------------------------------------------------- test hooks
-__global__ void k_qe_debug(uint64_t seed, uint32_t turn, uint64_t base, int64_t n, double *out)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double r[5];
-    qe_normals(seed, turn, base + (uint64_t)i, r);
-#pragma unroll
-    for (int j = 0; j < 5; j++) out[j * n + i] = r[j];
 }
 
 // ------------------------------------------------------- ion kick
@@ -526,6 +545,17 @@ k_ion_kick(IonKickParams P)
     }
 }
 
+// -------------------------------------------------------- test hooks
+__global__ void k_qe_debug(uint64_t seed, uint32_t turn, uint64_t base, int64_t n, double *out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double r[5];
+    qe_normals(seed, turn, base + (uint64_t)i, r);
+#pragma unroll
+    for (int j = 0; j < 5; j++) out[j * n + i] = r[j];
+}
+
 // ------------------------------------------------------------ launches
 constexpr uint32_t V_KICK = OP_KICK | OP_MOMB | OP_WRITE;
 constexpr uint32_t V_MOM = OP_MOMA | OP_MOMB | OP_MOMD | OP_ZSTAT | OP_LOSS;
@@ -533,17 +563,31 @@ constexpr uint32_t V_FIX = OP_KICK | OP_MOMB | OP_MAP | OP_SKEW | OP_RF | OP_DRI
                            OP_BIN | OP_LOSS | OP_MOMA | OP_MOMD | OP_WRITE;
 constexpr uint32_t V_REF = OP_KICK | OP_MOMB | OP_MAP | OP_ZSTAT | OP_SKEW | OP_RF | OP_DRIFT | OP_U0 |
                            OP_SYNRAD | OP_LOSS | OP_MOMA | OP_MOMD | OP_WRITE;
-constexpr uint32_t V_ALL = 0xffffffffu & ~OP_ION;
+constexpr uint32_t V_ALL = 0xffffffffu;
+
+// stage sets of the fused turn in its steady state
+constexpr uint32_t T_QE = OP_KICK | OP_MOMB | OP_MAP | OP_RF | OP_DRIFT | OP_U0 | OP_SYNRAD | OP_BIN |
+                          OP_LOSS | OP_MOMA | OP_WRITE;
+constexpr uint32_t T_NOSR = OP_KICK | OP_MOMB | OP_MAP | OP_RF | OP_DRIFT | OP_U0 | OP_BIN | OP_LOSS |
+                            OP_MOMA | OP_WRITE;
 
 hipError_t launch_pass(const PassParams &p, int nwg, hipStream_t st)
 {
     const size_t shm = (size_t)p.nbins * (sizeof(double) + sizeof(uint32_t));
     const uint32_t o = p.ops;
-    if ((o & ~V_KICK) == 0) hipLaunchKernelGGL(k_pass<V_KICK>, dim3(nwg), dim3(kBlock), shm, st, p);
-    else if ((o & ~V_MOM) == 0) hipLaunchKernelGGL(k_pass<V_MOM>, dim3(nwg), dim3(kBlock), shm, st, p);
-    else if ((o & ~V_FIX) == 0) hipLaunchKernelGGL(k_pass<V_FIX>, dim3(nwg), dim3(kBlock), shm, st, p);
-    else if ((o & ~V_REF) == 0) hipLaunchKernelGGL(k_pass<V_REF>, dim3(nwg), dim3(kBlock), shm, st, p);
-    else hipLaunchKernelGGL(k_pass<V_ALL>, dim3(nwg), dim3(kBlock), shm, st, p);
+    const dim3 g(nwg), b(kBlock);
+    const bool one_rf = p.n_rf == 1;
+    if (one_rf && o == T_QE && p.noise_mode == 2)
+        hipLaunchKernelGGL((k_pass<T_QE, true, 2, 1>), g, b, shm, st, p);
+    else if (one_rf && o == T_QE && p.noise_mode == 1)
+        hipLaunchKernelGGL((k_pass<T_QE, true, 1, 1>), g, b, shm, st, p);
+    else if (one_rf && o == T_NOSR)
+        hipLaunchKernelGGL((k_pass<T_NOSR, true, 0, 1>), g, b, shm, st, p);
+    else if ((o & ~V_KICK) == 0) hipLaunchKernelGGL((k_pass<V_KICK, false, -1, -1>), g, b, shm, st, p);
+    else if ((o & ~V_MOM) == 0) hipLaunchKernelGGL((k_pass<V_MOM, false, -1, -1>), g, b, shm, st, p);
+    else if ((o & ~V_FIX) == 0) hipLaunchKernelGGL((k_pass<V_FIX, false, -1, -1>), g, b, shm, st, p);
+    else if ((o & ~V_REF) == 0) hipLaunchKernelGGL((k_pass<V_REF, false, -1, -1>), g, b, shm, st, p);
+    else hipLaunchKernelGGL((k_pass<V_ALL, false, -1, -1>), g, b, shm, st, p);
     return hipGetLastError();
 }
 hipError_t launch_ion_kick(const IonKickParams &p, int nwg, hipStream_t st)
@@ -570,7 +614,11 @@ hipError_t launch_wake_table(const WakeTableParams &p, int ntables, hipStream_t 
 }
 hipError_t launch_wake(const WakeParams &p, int nb, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_wake, dim3(nb), dim3(kBlock), (size_t)p.nbins * sizeof(double), st, p);
+    int threads = ((p.nbins + 63) / 64) * 64;
+    if (threads > 1024) threads = 1024;
+    if (threads < 64) threads = 64;
+    const size_t shm = (size_t)(3 * p.nbins - 1) * sizeof(double);
+    hipLaunchKernelGGL(k_wake, dim3(nb), dim3(threads), shm, st, p);
     return hipGetLastError();
 }
 hipError_t launch_qe_debug(uint64_t seed, uint32_t turn, uint64_t base, int64_t n, double *out,

@@ -76,10 +76,12 @@ static void cts_sincos2pif(float u2, float *s_out, float *c_out)
     *c_out = rc;
 }
 
-void orc_box_muller(uint32_t a, uint32_t b, float *n0, float *n1)
+/* Box-Muller pair from two 24-bit integers: u1 = (a+1)*2^-24 in (0, 1],
+ * u2 = b*2^-24 in [0, 1). */
+static void box_muller24(uint32_t a, uint32_t b, float *n0, float *n1)
 {
-    const float u1 = (float)((a >> 8) + 1u) * 0x1p-24f; /* (0, 1] */
-    const float u2 = (float)(b >> 8) * 0x1p-24f;        /* [0, 1) */
+    const float u1 = (float)(a + 1u) * 0x1p-24f;
+    const float u2 = (float)b * 0x1p-24f;
     const float r = sqrtf(-2.0f * cts_logf(u1));
     float s, c;
     cts_sincos2pif(u2, &s, &c);
@@ -87,18 +89,28 @@ void orc_box_muller(uint32_t a, uint32_t b, float *n0, float *n1)
     *n1 = r * s;
 }
 
+void orc_box_muller(uint32_t a, uint32_t b, float *n0, float *n1)
+{
+    box_muller24(a >> 8, b >> 8, n0, n1);
+}
+
+/* One Philox block {index, turn, 0} cut into six 21-bit uniforms (bits
+ * 0-20, 21-41, 42-62 of each 64-bit half), three Box-Muller pairs; the
+ * sixth normal is unused. */
 void orc_qe_normals(uint64_t seed, uint32_t turn, uint64_t index, double out[5])
 {
     const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
-    uint32_t ctr[4] = {(uint32_t)index, (uint32_t)(index >> 32), turn, 0u};
-    uint32_t o[4], o2[4];
+    const uint32_t ctr[4] = {(uint32_t)index, (uint32_t)(index >> 32), turn, 0u};
+    uint32_t o[4];
     orc_philox4x32_10(ctr, key, o);
-    ctr[3] = 1u;
-    orc_philox4x32_10(ctr, key, o2);
+    const uint64_t lo = ((uint64_t)o[1] << 32) | o[0], hi = ((uint64_t)o[3] << 32) | o[2];
+    const uint32_t m21 = 0x1fffffu;
+    const uint32_t u[6] = {(uint32_t)lo & m21, (uint32_t)(lo >> 21) & m21, (uint32_t)(lo >> 42) & m21,
+                           (uint32_t)hi & m21, (uint32_t)(hi >> 21) & m21, (uint32_t)(hi >> 42) & m21};
     float n0, n1, n2, n3, n4, n5;
-    orc_box_muller(o[0], o[1], &n0, &n1);
-    orc_box_muller(o[2], o[3], &n2, &n3);
-    orc_box_muller(o2[0], o2[1], &n4, &n5);
+    box_muller24(u[0] << 3, u[1] << 3, &n0, &n1);
+    box_muller24(u[2] << 3, u[3] << 3, &n2, &n3);
+    box_muller24(u[4] << 3, u[5] << 3, &n4, &n5);
     out[0] = n0; out[1] = n1; out[2] = n2; out[3] = n3; out[4] = n4;
 }
 
