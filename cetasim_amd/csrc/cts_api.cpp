@@ -6,6 +6,7 @@
 // MPBeam seam of the reference (src/MPBeam.cpp:1010-1173).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -83,6 +84,9 @@ struct cts_handle {
     int64_t *d_seg_begin = nullptr, *d_seg_end = nullptr, *d_bunch_off = nullptr, *d_bunch_count = nullptr;
     int32_t *d_seg_bunch = nullptr, *d_wg_seg = nullptr, *d_bunch_seg = nullptr;
     double *d_seg_part = nullptr;
+    double *d_chunk_part = nullptr;
+    uint32_t *d_chunk_cnt = nullptr;
+    int max_chunks = 1;
 
     // moments
     double *d_shift = nullptr;
@@ -105,7 +109,8 @@ struct cts_handle {
     int nbins = 0, wake_mode = 1, nres = 0;
     double grid_lo = 0, grid_hi = 0;
     std::vector<double> rs, q, omega;
-    uint32_t *d_bhist = nullptr;
+    uint32_t *d_bhist = nullptr;   // two buffers of nb*nbins: turn parity
+    int hist_parity = 0;
     int32_t *d_counts = nullptr;
     double *d_kick = nullptr, *d_poten = nullptr, *d_edges_prev = nullptr, *d_edges_now = nullptr;
     double *d_table = nullptr, *d_scale = nullptr;
@@ -137,7 +142,7 @@ void free_bunch_state(cts_handle *h)
                     h->d_seg_bunch, h->d_wg_seg, h->d_bunch_seg, h->d_seg_part, h->d_shift,
                     h->d_cur, h->d_hist, h->d_sigz, h->d_zrange, h->d_noise, h->d_ion_e,
                     h->d_bhist, h->d_counts, h->d_kick, h->d_poten, h->d_edges_prev,
-                    h->d_edges_now, h->d_table, h->d_scale};
+                    h->d_edges_now, h->d_table, h->d_scale, h->d_chunk_part, h->d_chunk_cnt};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     h->d_x = h->d_px = h->d_y = h->d_py = h->d_z = h->d_dp = nullptr;
@@ -149,6 +154,8 @@ void free_bunch_state(cts_handle *h)
     h->d_bhist = nullptr;
     h->d_counts = nullptr;
     h->d_kick = h->d_poten = h->d_edges_prev = h->d_edges_now = h->d_table = h->d_scale = nullptr;
+    h->d_chunk_part = nullptr;
+    h->d_chunk_cnt = nullptr;
 }
 
 template <typename T>
@@ -230,6 +237,10 @@ void build_consts(const cts_ring &R, const std::vector<cts_section> &secs, int k
     c[CI_IBETAY] = 1.0 / S.beta_y;
     c[CI_PHX] = S.phase_adv_x;
     c[CI_PHY] = S.phase_adv_y;
+    c[CI_CMUX] = std::cos(S.phase_adv_x);
+    c[CI_SMUX] = std::sin(S.phase_adv_x);
+    c[CI_CMUY] = std::cos(S.phase_adv_y);
+    c[CI_SMUY] = std::sin(S.phase_adv_y);
     // src/Bunch.cpp:658-659
     c[CI_WX] = S.phase_adv_x / (2 * kPi * R.nux);
     c[CI_WY] = S.phase_adv_y / (2 * kPi * R.nuy);
@@ -274,7 +285,7 @@ PassParams base_pass(cts_handle *h, uint32_t ops, int k)
     p.seg_bunch = h->d_seg_bunch; p.wg_seg = h->d_wg_seg;
     p.bunch_off = h->d_bunch_off;
     p.seg_part = h->d_seg_part;
-    p.bunch_hist = h->d_bhist;
+    p.bunch_hist = h->d_bhist ? h->d_bhist + (size_t)h->hist_parity * h->nb * h->nbins : nullptr;
     p.shift = h->d_shift;
     p.kick_prev = h->d_kick; p.edges_prev = h->d_edges_prev; p.edges_now = h->d_edges_now;
     p.noise = h->d_noise;
@@ -310,6 +321,9 @@ FinParams base_fin(cts_handle *h, uint32_t ops, int slot, int prev_slot)
     f.edges_now = h->d_edges_now;
     f.status = h->d_status;
     f.nseg = h->nseg;
+    f.chunk_part = h->d_chunk_part;
+    f.chunk_cnt = h->d_chunk_cnt;
+    f.max_chunks = h->max_chunks;
     return f;
 }
 
@@ -346,7 +360,8 @@ int run_wake(cts_handle *h)
     }
     WakeParams w;
     std::memset(&w, 0, sizeof w);
-    w.bunch_hist = h->d_bhist;
+    w.bunch_hist = h->d_bhist + (size_t)h->hist_parity * h->nb * h->nbins;
+    w.hist_clear = h->d_bhist + (size_t)(1 - h->hist_parity) * h->nb * h->nbins;
     w.counts_out = h->d_counts;
     w.table = h->d_table;
     w.per_bunch_table = h->wake_mode == 0 ? 1 : 0;
@@ -358,6 +373,7 @@ int run_wake(cts_handle *h)
     w.nbins = h->nbins;
     TimedScope ts(h, "wake");
     HIPCHK(launch_wake(w, h->nb, h->stream));
+    h->hist_parity = 1 - h->hist_parity;
     return CTS_OK;
 }
 
@@ -368,7 +384,7 @@ int run_bin(cts_handle *h)
     b.seg_begin = h->d_seg_begin; b.seg_end = h->d_seg_end;
     b.seg_bunch = h->d_seg_bunch; b.wg_seg = h->d_wg_seg;
     b.edges = h->d_edges_now;
-    b.bunch_hist = h->d_bhist;
+    b.bunch_hist = h->d_bhist + (size_t)h->hist_parity * h->nb * h->nbins;
     b.nbins = h->nbins;
     TimedScope ts(h, "bin");
     HIPCHK(launch_bin(b, h->nwg, h->stream));
@@ -571,6 +587,8 @@ int cts_set_bunches(cts_handle *h, int32_t nb, const int64_t *counts, const doub
         }
         bseg[nb] = h->nseg;
     }
+    h->max_chunks = 1;
+    for (int bb = 0; bb < nb; bb++) h->max_chunks = std::max(h->max_chunks, (bseg[bb + 1] - bseg[bb] + 63) / 64);
     int rc;
     if ((rc = dalloc(&h->d_x, n)) || (rc = dalloc(&h->d_px, n)) || (rc = dalloc(&h->d_y, n)) ||
         (rc = dalloc(&h->d_py, n)) || (rc = dalloc(&h->d_z, n)) || (rc = dalloc(&h->d_dp, n)) ||
@@ -580,7 +598,9 @@ int cts_set_bunches(cts_handle *h, int32_t nb, const int64_t *counts, const doub
         (rc = dalloc(&h->d_bunch_off, nb + 1)) || (rc = dalloc(&h->d_bunch_count, nb)) ||
         (rc = dalloc(&h->d_seg_part, (size_t)h->nseg * kNPart)) || (rc = dalloc(&h->d_shift, (size_t)nb * 6)) ||
         (rc = dalloc(&h->d_cur, nb)) || (rc = dalloc(&h->d_sigz, nb)) || (rc = dalloc(&h->d_zrange, (size_t)nb * 2)) ||
-        (rc = dalloc(&h->d_ion_e, (size_t)nb * CTS_MAX_SPECIES * 5)))
+        (rc = dalloc(&h->d_ion_e, (size_t)nb * CTS_MAX_SPECIES * 5)) ||
+        (rc = dalloc(&h->d_chunk_part, (size_t)nb * h->max_chunks * kNPart)) ||
+        (rc = dalloc(&h->d_chunk_cnt, nb)))
         return rc;
     HIPCHK(hipMemcpy(h->d_seg_begin, sb.data(), sb.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->d_seg_end, se.data(), se.size() * sizeof(int64_t), hipMemcpyHostToDevice));
@@ -688,7 +708,8 @@ int cts_set_wake(cts_handle *h, int32_t nbins, int32_t mode, double grid_lo, dou
     h->kick_pending = false;
     if (h->nbins == 0) return CTS_OK;
     const int nb = h->nb, n = h->nbins;
-    if ((rc = dalloc(&h->d_bhist, (size_t)nb * n)) || (rc = dalloc(&h->d_counts, (size_t)nb * n)) ||
+    h->hist_parity = 0;
+    if ((rc = dalloc(&h->d_bhist, (size_t)2 * nb * n)) || (rc = dalloc(&h->d_counts, (size_t)nb * n)) ||
         (rc = dalloc(&h->d_kick, (size_t)nb * n)) || (rc = dalloc(&h->d_poten, (size_t)nb * n)) ||
         (rc = dalloc(&h->d_edges_prev, (size_t)nb * 2)) || (rc = dalloc(&h->d_edges_now, (size_t)nb * 2)) ||
         (rc = dalloc(&h->d_table, (size_t)(mode == 0 ? nb : 1) * (2 * n - 1))) || (rc = dalloc(&h->d_scale, nb)))

@@ -50,6 +50,7 @@ enum : int {
     CI_RFVR = 91, CI_RFVI = 95, CI_RFK = 99, CI_RFSCALE = 103,
     CI_DRIFTC = 104, CI_ALPHAC = 105, CI_U0 = 108, CI_LAM = 109, CI_LZ2 = 112, CI_COEF = 113,
     CI_ZLIM = 116, CI_APX = 117, CI_APY = 118, CI_IAPX = 119, CI_IAPY = 120,
+    CI_CMUX = 121, CI_SMUX = 122, CI_CMUY = 123, CI_SMUY = 124,
     kNConst = 128
 };
 
@@ -104,6 +105,9 @@ struct FinParams {
     double *zrange;     // survivors' z range (GetZMinMax, without the pad)
     double *edges_now;
     uint32_t *status;
+    double *chunk_part;   // [nb][max_chunks][kNPart]
+    uint32_t *chunk_cnt;  // [nb], left at zero after each launch
+    int32_t max_chunks;
 };
 
 struct WakeTableParams {
@@ -114,7 +118,8 @@ struct WakeTableParams {
 };
 
 struct WakeParams {
-    uint32_t *bunch_hist;
+    const uint32_t *bunch_hist;  // this turn's histogram
+    uint32_t *hist_clear;        // the other buffer, cleared for the next turn
     int32_t *counts_out;
     const double *table;
     int32_t per_bunch_table;

@@ -66,9 +66,26 @@ __device__ __forceinline__ void stage_map(double X[6], cdptr K)
     const double dnuy = (K[CI_CHY] * d + K[CI_ADTY + 0] * ampX + K[CI_ADTY + 1] * ampY +
                          K[CI_ADTY + 2] * (ampX * ampX) * 0.5 + K[CI_ADTY + 3] * (ampY * ampY) * 0.5 +
                          K[CI_ADTY + 4] * ampX * ampY) * K[CI_WY];
+    // phi = mu + 2*pi*dnu: for the usual small tune shift, angle addition
+    // onto the section's cos/sin(mu); otherwise the full reduction
+    const double ddx = 2 * kPi * dnux, ddy = 2 * kPi * dnuy;
     double sx, cx, sy, cy;
-    sincos_fast(K[CI_PHX] + 2 * kPi * dnux, sx, cx);
-    sincos_fast(K[CI_PHY] + 2 * kPi * dnuy, sy, cy);
+    if (fabs(ddx) <= 0.1) {
+        double sd, cd;
+        sincos_small(ddx, sd, cd);
+        sx = K[CI_SMUX] * cd + K[CI_CMUX] * sd;
+        cx = K[CI_CMUX] * cd - K[CI_SMUX] * sd;
+    } else {
+        sincos_fast(K[CI_PHX] + ddx, sx, cx);
+    }
+    if (fabs(ddy) <= 0.1) {
+        double sd, cd;
+        sincos_small(ddy, sd, cd);
+        sy = K[CI_SMUY] * cd + K[CI_CMUY] * sd;
+        cy = K[CI_CMUY] * cd - K[CI_SMUY] * sd;
+    } else {
+        sincos_fast(K[CI_PHY] + ddy, sy, cy);
+    }
     double Y[6], R[6];
     spmv_k(K + CI_B1H1, X, Y);
     R[0] = cx * Y[0] + sx * Y[1];
@@ -168,11 +185,33 @@ k_pass(PassParams P)
         acc[A_ZMIN] = INFINITY; acc[A_ZMAX] = -INFINITY;
         acc[C_ZMIN] = INFINITY; acc[C_ZMAX] = -INFINITY;
 
-        for (int64_t i = i0 + tid; i < i1; i += kBlock) {
+        // per-segment base pointers (uniform) and a 32-bit lane offset, so the
+        // loads and stores use scalar-base + vector-offset addressing
+        double *__restrict__ sx_ = P.x + i0, *__restrict__ spx_ = P.px + i0, *__restrict__ sy_ = P.y + i0;
+        double *__restrict__ spy_ = P.py + i0, *__restrict__ sz_ = P.z + i0, *__restrict__ sdp_ = P.dp + i0;
+        uint8_t *__restrict__ sfl_ = P.flag + i0;
+        const uint32_t nseg_i = (uint32_t)(i1 - i0);
+        // software pipeline: the next particle's coordinates are loaded while
+        // the current one is computed, so HBM latency overlaps the VALU work
+        double Xn[6] = {0, 0, 0, 0, 0, 0};
+        uint32_t fn = 0;
+        if ((uint32_t)tid < nseg_i) {
+            Xn[0] = sx_[tid]; Xn[1] = spx_[tid]; Xn[2] = sy_[tid];
+            Xn[3] = spy_[tid]; Xn[4] = sz_[tid]; Xn[5] = sdp_[tid];
+            fn = sfl_[tid];
+        }
+        for (uint32_t k = tid; k < nseg_i; k += kBlock) {
+            const int64_t i = i0 + k;
             double X[6];
-            X[0] = P.x[i]; X[1] = P.px[i]; X[2] = P.y[i];
-            X[3] = P.py[i]; X[4] = P.z[i]; X[5] = P.dp[i];
-            uint32_t f = P.flag[i];
+#pragma unroll
+            for (int j = 0; j < 6; j++) X[j] = Xn[j];
+            uint32_t f = fn;
+            const uint32_t kn = k + kBlock;
+            if (kn < nseg_i) {
+                Xn[0] = sx_[kn]; Xn[1] = spx_[kn]; Xn[2] = sy_[kn];
+                Xn[3] = spy_[kn]; Xn[4] = sz_[kn]; Xn[5] = sdp_[kn];
+                fn = sfl_[kn];
+            }
 
             if (use_kick) {
                 // survivors when the kick was computed: code 0, or lost at
@@ -213,7 +252,9 @@ k_pass(PassParams P)
                 cdptr K = reload(P.cst);
                 for (int r = 0; r < n_rf; r++) {
                     double s, c;
-                    sincos_fast(X[4] * K[CI_RFK + r], s, c);
+                    const double th = X[4] * K[CI_RFK + r];
+                    if (fabs(th) <= 0.1) sincos_small(th, s, c);
+                    else sincos_fast(th, s, c);
                     const double re = K[CI_RFVR + r] * c - K[CI_RFVI + r] * s;
                     X[5] += re * K[CI_RFSCALE];
                 }
@@ -235,7 +276,7 @@ k_pass(PassParams P)
                 }
                 stage_synrad(X, reload(P.cst), r, noise_mode != 0);
             }
-            if ((ops & OP_ZSTAT) && (alive || i == bunch_first)) {
+            if ((ops & OP_ZSTAT) && (alive || i0 + (int64_t)k == bunch_first)) {
                 // GetZMinMax starts from particle 0 of the bunch even when it
                 // is lost (src/MPBunch.cpp:1862-1863)
                 acc[C_ZMIN] = fmin(acc[C_ZMIN], X[4]);
@@ -279,9 +320,9 @@ k_pass(PassParams P)
                 acc[B_QZD] += dz * dd;
             }
             if (ops & OP_WRITE) {
-                P.x[i] = X[0]; P.px[i] = X[1]; P.y[i] = X[2];
-                P.py[i] = X[3]; P.z[i] = X[4]; P.dp[i] = X[5];
-                P.flag[i] = (uint8_t)fnew;
+                sx_[k] = X[0]; spx_[k] = X[1]; sy_[k] = X[2];
+                spy_[k] = X[3]; sz_[k] = X[4]; sdp_[k] = X[5];
+                sfl_[k] = (uint8_t)fnew;
             }
         }
 
@@ -351,29 +392,62 @@ k_bin(BinParams P)
 }
 
 // ----------------------------------------------------------- finalize
-// Per bunch: reduce segment partials in a fixed order (each wave takes a
-// subset of the values, lanes stride over segments), write the moment
-// record, update the shift and (reference-edge mode) the bin edges.
-__global__ void __launch_bounds__(kBlock)
+// Per bunch: reduce the segment partials and write the moment record.
+// grid = (chunks, nb), one wave per chunk of 64 segments: every lane folds
+// one segment, the wave reduces, and the last chunk of a bunch to finish
+// (device-scope counter) folds the chunk results in chunk order, so the
+// result does not depend on scheduling.
+__device__ __forceinline__ double part_identity(int j)
+{
+    return (j == A_ZMIN || j == C_ZMIN) ? INFINITY : (j == A_ZMAX || j == C_ZMAX) ? -INFINITY : 0.0;
+}
+__device__ __forceinline__ double part_fold(int j, double a, double v)
+{
+    return (j == A_ZMIN || j == C_ZMIN) ? fmin(a, v) : (j == A_ZMAX || j == C_ZMAX) ? fmax(a, v) : a + v;
+}
+__device__ __forceinline__ double part_wave(int j, double a)
+{
+    return (j == A_ZMIN || j == C_ZMIN) ? wave_min(a) : (j == A_ZMAX || j == C_ZMAX) ? wave_max(a) : wave_sum(a);
+}
+
+__global__ void __launch_bounds__(64)
 k_finalize(FinParams P)
 {
     __shared__ double tot[kNPart];
-    const int b = blockIdx.x, tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
+    __shared__ uint32_t last;
+    const int b = blockIdx.y, c = blockIdx.x, lane = threadIdx.x;
     const int s0 = P.bunch_seg[b], s1 = P.bunch_seg[b + 1];
-    for (int j = wave; j < kNPart; j += kBlock / 64) {
-        const bool is_min = j == A_ZMIN || j == C_ZMIN, is_max = j == A_ZMAX || j == C_ZMAX;
-        double a = is_min ? INFINITY : is_max ? -INFINITY : 0.0;
-        const double *p = P.seg_part + (int64_t)j * P.nseg;
-        for (int s = s0 + lane; s < s1; s += 64) {
-            const double v = p[s];
-            a = is_min ? fmin(a, v) : is_max ? fmax(a, v) : a + v;
+    const int nchunk = (s1 - s0 + 63) / 64;
+    if (c >= nchunk && !(c == 0 && nchunk == 0)) return;
+    {
+        const int s = s0 + c * 64 + lane;
+        double v[kNPart];
+#pragma unroll
+        for (int j = 0; j < kNPart; j++) v[j] = (s < s1) ? P.seg_part[(int64_t)j * P.nseg + s] : part_identity(j);
+        double *dst = P.chunk_part + ((int64_t)b * P.max_chunks + c) * kNPart;
+#pragma unroll
+        for (int j = 0; j < kNPart; j++) {
+            const double r = part_wave(j, v[j]);
+            if (lane == 0) dst[j] = r;
         }
-        a = is_min ? wave_min(a) : is_max ? wave_max(a) : wave_sum(a);
-        if (lane == 0) tot[j] = a;
+    }
+    if (nchunk > 1) {
+        __threadfence();
+        if (lane == 0) last = atomicAdd(&P.chunk_cnt[b], 1u) == (uint32_t)(nchunk - 1);
+        __syncthreads();
+        if (!last) return;
+        __threadfence();
+        if (lane == 0) P.chunk_cnt[b] = 0u;
+    }
+    for (int j = lane; j < kNPart; j += 64) {
+        double a = part_identity(j);
+        const double *src = P.chunk_part + (int64_t)b * P.max_chunks * kNPart;
+        const int nc = nchunk > 0 ? nchunk : 1;
+        for (int q = 0; q < nc; q++) a = part_fold(j, a, src[(int64_t)q * kNPart + j]);
+        tot[j] = a;
     }
     __syncthreads();
-    if (tid != 0) return;
+    if (lane != 0) return;
 
     double *sh = P.shift + 6 * b;
     const double ncount = (double)P.bunch_count[b];
@@ -481,37 +555,38 @@ k_wake_table(WakeTableParams P)
 
 // --------------------------------------------------------------- wake
 // Per bunch: kick_i = (poten_i + sum_j W[i-j] * n_j) * (-e*q/E)
-// (src/MPBunch.cpp:1983-1998), summed over j in order; table and counts
-// are staged in LDS.  The histogram is cleared for the next pass.
-__global__ void __launch_bounds__(1024)
+// (src/MPBunch.cpp:1983-1998), summed over j in order.  grid = (bin groups
+// of 64, nb): each wave stages the table and the counts in LDS and owns 64
+// bins.  The histogram is double-buffered: this turn's buffer is read, the
+// other one (next turn's) is cleared.
+__global__ void __launch_bounds__(64)
 k_wake(WakeParams P)
 {
     extern __shared__ double smem_d[];
     const int nb = P.nbins, nlag = 2 * nb - 1;
     double *n_lds = smem_d;
     double *w_lds = smem_d + nb;
-    const int b = blockIdx.x, tid = threadIdx.x;
-    uint32_t *hist = P.bunch_hist + (int64_t)b * nb;
-    for (int j = tid; j < nb; j += blockDim.x) {
-        const uint32_t c = hist[j];
-        n_lds[j] = (double)c;
-        P.counts_out[(int64_t)b * nb + j] = (int32_t)c;
-        hist[j] = 0u;
-    }
+    const int b = blockIdx.y, lane = threadIdx.x;
+    const int i = blockIdx.x * 64 + lane;
+    const uint32_t *hist = P.bunch_hist + (int64_t)b * nb;
+    for (int j = lane; j < nb; j += 64) n_lds[j] = (double)hist[j];
     const double *W = P.table + (P.per_bunch_table ? (int64_t)b * nlag : 0);
-    for (int j = tid; j < nlag; j += blockDim.x) w_lds[j] = W[j];
+    for (int j = lane; j < nlag; j += 64) w_lds[j] = W[j];
+    if (i < nb) {
+        P.counts_out[(int64_t)b * nb + i] = (int32_t)hist[i];
+        P.hist_clear[(int64_t)b * nb + i] = 0u;
+    }
     __syncthreads();
-    const double scale = P.scale[b];
-    double *poten = P.poten + (int64_t)b * nb;
-    for (int i = tid; i < nb; i += blockDim.x) {
-        double a = poten[i];
+    if (i < nb) {
+        double a = P.poten[(int64_t)b * nb + i];
         const double *Wi = w_lds + i + (nb - 1);
+#pragma unroll 16
         for (int j = 0; j < nb; j++) a += Wi[-j] * n_lds[j];
-        a *= scale;
-        poten[i] = a;
+        a *= P.scale[b];
+        P.poten[(int64_t)b * nb + i] = a;
         P.kick[(int64_t)b * nb + i] = a;
     }
-    if (tid == 0) {
+    if (blockIdx.x == 0 && lane == 0) {
         P.edges_prev[2 * b] = P.edges_now[2 * b];
         P.edges_prev[2 * b + 1] = P.edges_now[2 * b + 1];
     }
@@ -602,7 +677,7 @@ hipError_t launch_bin(const BinParams &p, int nwg, hipStream_t st)
 }
 hipError_t launch_finalize(const FinParams &p, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_finalize, dim3(p.nb), dim3(kBlock), 0, st, p);
+    hipLaunchKernelGGL(k_finalize, dim3(p.max_chunks, p.nb), dim3(64), 0, st, p);
     return hipGetLastError();
 }
 hipError_t launch_wake_table(const WakeTableParams &p, int ntables, hipStream_t st)
@@ -614,11 +689,8 @@ hipError_t launch_wake_table(const WakeTableParams &p, int ntables, hipStream_t 
 }
 hipError_t launch_wake(const WakeParams &p, int nb, hipStream_t st)
 {
-    int threads = ((p.nbins + 63) / 64) * 64;
-    if (threads > 1024) threads = 1024;
-    if (threads < 64) threads = 64;
     const size_t shm = (size_t)(3 * p.nbins - 1) * sizeof(double);
-    hipLaunchKernelGGL(k_wake, dim3(nb), dim3(threads), shm, st, p);
+    hipLaunchKernelGGL(k_wake, dim3((p.nbins + 63) / 64, nb), dim3(64), shm, st, p);
     return hipGetLastError();
 }
 hipError_t launch_qe_debug(uint64_t seed, uint32_t turn, uint64_t base, int64_t n, double *out,

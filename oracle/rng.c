@@ -62,8 +62,8 @@ static void cts_sincos2pif(float u2, float *s_out, float *c_out)
     if (f > 0.5f) { f = 1.0f - f; swap = 1; }
     const float th = f * 1.57079633f;
     const float t2 = th * th;
-    const float s = th * (1.0f - t2 * (0.166666667f - t2 * (0.00833333333f - t2 * (0.000198412698f - t2 * 2.75573192e-6f))));
-    const float c = 1.0f - t2 * (0.5f - t2 * (0.0416666667f - t2 * (0.00138888889f - t2 * (2.48015873e-5f - t2 * 2.75573192e-7f))));
+    const float s = th * (1.0f - t2 * (0.166666667f - t2 * (0.00833333333f - t2 * 0.000198412698f)));
+    const float c = 1.0f - t2 * (0.5f - t2 * (0.0416666667f - t2 * (0.00138888889f - t2 * 2.48015873e-5f)));
     float sn = swap ? c : s, cs = swap ? s : c;
     float rs, rc;
     switch (q & 3) {
