@@ -115,8 +115,7 @@ def main():
     tr.sync()
     dt = time.perf_counter() - t0
     pass_ms, n_pass = tr.kernel_time("pass")
-    fin_ms, _ = tr.kernel_time("finalize")
-    wake_ms, _ = tr.kernel_time("wake")
+    post_ms, _ = tr.kernel_time("post")
     tr.set_timing(False)
     m = tr.moments()[0]
     tr.close()
@@ -143,7 +142,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_pass (fused turn)", "kernel_ms": pass_ms, "launches": n_pass,
-                     "other_kernels_ms": {"finalize": fin_ms, "wake": wake_ms}},
+                     "other_kernels_ms": {"post (moments + wake)": post_ms}},
         "transmission": m["transmission"],
     }
     if not a.no_cpu_baseline:

@@ -345,19 +345,8 @@ int slot_of(cts_handle *h, int turn) { return (h->hist_cap > 0 && turn >= 0) ? t
 
 // Wake stage after the histogram is complete: reference-edge mode needs
 // the per-bunch table for this turn's bin width first.
-int run_wake(cts_handle *h)
+WakeParams wake_params(cts_handle *h)
 {
-    if (h->wake_mode == 0) {
-        WakeTableParams t;
-        std::memset(&t, 0, sizeof t);
-        t.edges = h->d_edges_now;
-        t.table = h->d_table;
-        t.nbins = h->nbins;
-        t.nres = h->nres;
-        for (int r = 0; r < h->nres; r++) { t.rs[r] = h->rs[r]; t.q[r] = h->q[r]; t.omega[r] = h->omega[r]; }
-        TimedScope ts(h, "wake_table");
-        HIPCHK(launch_wake_table(t, h->nb, h->stream));
-    }
     WakeParams w;
     std::memset(&w, 0, sizeof w);
     w.bunch_hist = h->d_bhist + (size_t)h->hist_parity * h->nb * h->nbins;
@@ -371,6 +360,23 @@ int run_wake(cts_handle *h)
     w.edges_now = h->d_edges_now;
     w.edges_prev = h->d_edges_prev;
     w.nbins = h->nbins;
+    return w;
+}
+
+int run_wake(cts_handle *h)
+{
+    if (h->wake_mode == 0) {
+        WakeTableParams t;
+        std::memset(&t, 0, sizeof t);
+        t.edges = h->d_edges_now;
+        t.table = h->d_table;
+        t.nbins = h->nbins;
+        t.nres = h->nres;
+        for (int r = 0; r < h->nres; r++) { t.rs[r] = h->rs[r]; t.q[r] = h->q[r]; t.omega[r] = h->omega[r]; }
+        TimedScope ts(h, "wake_table");
+        HIPCHK(launch_wake_table(t, h->nb, h->stream));
+    }
+    WakeParams w = wake_params(h);
     TimedScope ts(h, "wake");
     HIPCHK(launch_wake(w, h->nb, h->stream));
     h->hist_parity = 1 - h->hist_parity;
@@ -797,11 +803,19 @@ int cts_turn(cts_handle *h, int32_t turn)
     // finalize: dp part of the previous record (MOMB), this turn's record
     FinParams f = base_fin(h, ops & (OP_MOMB | OP_MOMA | OP_MOMD | OP_LOSS | OP_ZSTAT), slot_of(h, turn),
                            slot_of(h, h->last_turn));
-    if ((rc = run_finalize(h, f))) return rc;
-    if (wake) {
-        if (h->wake_mode == 0 && (rc = run_bin(h))) return rc;
-        if ((rc = run_wake(h))) return rc;
+    if (wake && h->wake_mode == 1) {
+        // fixed grid: moments and wake in one launch
+        TimedScope ts(h, "post");
+        HIPCHK(launch_post(f, wake_params(h), h->stream));
+        h->hist_parity = 1 - h->hist_parity;
         h->kick_pending = true;
+    } else {
+        if ((rc = run_finalize(h, f))) return rc;
+        if (wake) {
+            if ((rc = run_bin(h))) return rc;
+            if ((rc = run_wake(h))) return rc;
+            h->kick_pending = true;
+        }
     }
     h->last_turn = turn;
     h->dp_pending = wake;

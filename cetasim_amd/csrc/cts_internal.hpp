@@ -134,6 +134,7 @@ hipError_t launch_pass(const PassParams &p, int nwg, hipStream_t st);
 hipError_t launch_bin(const BinParams &p, int nwg, hipStream_t st);
 hipError_t launch_ion_kick(const IonKickParams &p, int nwg, hipStream_t st);
 hipError_t launch_finalize(const FinParams &p, hipStream_t st);
+hipError_t launch_post(const FinParams &f, const WakeParams &w, hipStream_t st);
 hipError_t launch_wake_table(const WakeTableParams &p, int ntables, hipStream_t st);
 hipError_t launch_wake(const WakeParams &p, int nb, hipStream_t st);
 hipError_t launch_qe_debug(uint64_t seed, uint32_t turn, uint64_t base, int64_t n, double *out,

@@ -9,6 +9,8 @@
 // and reduced per bunch in a fixed order, so moments are reproducible run to
 // run.  Longitudinal histograms are LDS-private per workgroup and merged
 // into the per-bunch histogram with integer atomics (exact, order-free).
+#include <algorithm>
+
 #include "cts_internal.hpp"
 
 namespace cts {
@@ -410,12 +412,11 @@ __device__ __forceinline__ double part_wave(int j, double a)
     return (j == A_ZMIN || j == C_ZMIN) ? wave_min(a) : (j == A_ZMAX || j == C_ZMAX) ? wave_max(a) : wave_sum(a);
 }
 
-__global__ void __launch_bounds__(64)
-k_finalize(FinParams P)
+__device__ __forceinline__ void finalize_body(const FinParams &P, int c, int b)
 {
     __shared__ double tot[kNPart];
     __shared__ uint32_t last;
-    const int b = blockIdx.y, c = blockIdx.x, lane = threadIdx.x;
+    const int lane = threadIdx.x;
     const int s0 = P.bunch_seg[b], s1 = P.bunch_seg[b + 1];
     const int nchunk = (s1 - s0 + 63) / 64;
     if (c >= nchunk && !(c == 0 && nchunk == 0)) return;
@@ -559,15 +560,14 @@ k_wake_table(WakeTableParams P)
 // of 64, nb): each wave stages the table and the counts in LDS and owns 64
 // bins.  The histogram is double-buffered: this turn's buffer is read, the
 // other one (next turn's) is cleared.
-__global__ void __launch_bounds__(64)
-k_wake(WakeParams P)
+__device__ __forceinline__ void wake_body(const WakeParams &P, int group, int b)
 {
     extern __shared__ double smem_d[];
     const int nb = P.nbins, nlag = 2 * nb - 1;
     double *n_lds = smem_d;
     double *w_lds = smem_d + nb;
-    const int b = blockIdx.y, lane = threadIdx.x;
-    const int i = blockIdx.x * 64 + lane;
+    const int lane = threadIdx.x;
+    const int i = group * 64 + lane;
     const uint32_t *hist = P.bunch_hist + (int64_t)b * nb;
     for (int j = lane; j < nb; j += 64) n_lds[j] = (double)hist[j];
     const double *W = P.table + (P.per_bunch_table ? (int64_t)b * nlag : 0);
@@ -586,10 +586,23 @@ k_wake(WakeParams P)
         P.poten[(int64_t)b * nb + i] = a;
         P.kick[(int64_t)b * nb + i] = a;
     }
-    if (blockIdx.x == 0 && lane == 0) {
+    if (group == 0 && lane == 0) {
         P.edges_prev[2 * b] = P.edges_now[2 * b];
         P.edges_prev[2 * b + 1] = P.edges_now[2 * b + 1];
     }
+}
+
+__global__ void __launch_bounds__(64) k_finalize(FinParams P) { finalize_body(P, blockIdx.x, blockIdx.y); }
+__global__ void __launch_bounds__(64) k_wake(WakeParams P)
+{
+    if ((int)blockIdx.x * 64 < P.nbins) wake_body(P, blockIdx.x, blockIdx.y);
+}
+// finalize (z = 0) and wake (z = 1) of one turn in a single launch: the
+// fixed-grid wake needs only the histogram, not the moments
+__global__ void __launch_bounds__(64) k_post(FinParams F, WakeParams W)
+{
+    if (blockIdx.z == 0) finalize_body(F, blockIdx.x, blockIdx.y);
+    else if ((int)blockIdx.x * 64 < W.nbins) wake_body(W, blockIdx.x, blockIdx.y);
 }
 
 // ------------------------------------------------------- ion kick
@@ -678,6 +691,13 @@ hipError_t launch_bin(const BinParams &p, int nwg, hipStream_t st)
 hipError_t launch_finalize(const FinParams &p, hipStream_t st)
 {
     hipLaunchKernelGGL(k_finalize, dim3(p.max_chunks, p.nb), dim3(64), 0, st, p);
+    return hipGetLastError();
+}
+hipError_t launch_post(const FinParams &f, const WakeParams &w, hipStream_t st)
+{
+    const int gx = std::max(f.max_chunks, (w.nbins + 63) / 64);
+    const size_t shm = (size_t)(3 * w.nbins - 1) * sizeof(double);
+    hipLaunchKernelGGL(k_post, dim3(gx, f.nb, 2), dim3(64), shm, st, f, w);
     return hipGetLastError();
 }
 hipError_t launch_wake_table(const WakeTableParams &p, int ntables, hipStream_t st)
