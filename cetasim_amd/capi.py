@@ -66,6 +66,12 @@ class RingInput(ctypes.Structure):
                 ("harmonic", I32), ("rf_volt", D), ("ap_x", D), ("ap_y", D)]
 
 
+class IonConfig(ctypes.Structure):
+    _fields_ = [("nspec", I32), ("n_new", I32), ("max_ions", I64), ("mass_number", D * MAX_SPECIES),
+                ("cross_section", D * MAX_SPECIES), ("pressure", D * MAX_SPECIES), ("temperature", D),
+                ("length", D), ("loss_boundary", D), ("ap_x", D), ("ap_y", D), ("seed", U64)]
+
+
 class RingDerived(ctypes.Structure):
     _fields_ = [("eta", D), ("workqz", D), ("natural_bunch_length", D), ("beta_z", D)]
 
@@ -82,6 +88,10 @@ SIGNATURES = {
     "cts_set_wake": (I32, [ctypes.c_void_p, I32, I32, D, D, I32, PD, PD, PD]),
     "cts_set_synrad": (I32, [ctypes.c_void_p, I32, I32, U64]),
     "cts_push_noise": (I32, [ctypes.c_void_p, PD]),
+    "cts_set_ions": (I32, [ctypes.c_void_p, ctypes.POINTER(IonConfig)]),
+    "cts_ion_phase": (I32, [ctypes.c_void_p, I32]),
+    "cts_get_ion_cloud": (I32, [ctypes.c_void_p, I32, I64, PI64, PD, PD, PD, PD]),
+    "cts_get_ion_passages": (I32, [ctypes.c_void_p, PD]),
     "cts_turn": (I32, [ctypes.c_void_p, I32]),
     "cts_flush": (I32, [ctypes.c_void_p]),
     "cts_section_map": (I32, [ctypes.c_void_p, I32]),

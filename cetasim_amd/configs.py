@@ -102,3 +102,50 @@ def bunch_scalars(counts, ring, p=BASE):
     n_e = np.full(len(counts), p["bunch_charge"] / E_CHARGE)
     q = n_e / np.asarray(counts, dtype=np.float64)
     return q, n_e
+
+
+# Batched train with residual-gas ions: 80 bunches 4 ns apart (2 buckets at
+# h = 1000, T0 = 2 us), macroparticle counts uniform in [1e4, 2e5] (seed 3),
+# 1 nC per bunch, one interaction point with CO (mass 28, cross section
+# 2.0 Mb) at 1e-9 Pa and 300 K over the whole circumference, ion loss at
+# 10 rms bunch sizes, at most 5e5 macro-ions.
+TRAIN = dict(nb=80, spacing=2, count_range=(10_000, 200_000), seed=3, turns=2000,
+             ions=dict(mass_number=[28.0], cross_section=[2.0e-22], pressure=[1.0e-9], temperature=300.0,
+                       n_new=2000, max_ions=500_000, loss_boundary=10.0, seed=3))
+
+
+def train_counts(nb=TRAIN["nb"], lo=TRAIN["count_range"][0], hi=TRAIN["count_range"][1], seed=TRAIN["seed"]):
+    """Macroparticle count of every bunch, uniform integers in [lo, hi]."""
+    return np.random.default_rng(seed).integers(lo, hi + 1, nb).astype(np.int64)
+
+
+def train_gaps(nb, spacing, harmonic):
+    """bunchGap of every bunch: buckets to the next bunch, the last one to the
+    first bunch of the next turn (src/MPBeam.cpp:130-136)."""
+    g = np.full(nb, spacing, dtype=np.int32)
+    g[-1] = harmonic - spacing * (nb - 1)
+    return g
+
+
+def train_bunches(counts, seed, p=BASE):
+    """Concatenated 6-D Gaussians, one independent stream per bunch."""
+    parts = [gaussian_bunch(int(n), seed * 1000 + b, p) for b, n in enumerate(counts)]
+    return {c: np.concatenate([q[c] for q in parts]) for c in ("x", "px", "y", "py", "z", "dp")}
+
+
+def ion_config(ring, ions=TRAIN["ions"], p=BASE):
+    """capi.IonConfig of one interaction point spanning the circumference."""
+    cfg = capi.IonConfig()
+    cfg.nspec = len(ions["mass_number"])
+    cfg.n_new = ions["n_new"]
+    cfg.max_ions = ions["max_ions"]
+    for i in range(cfg.nspec):
+        cfg.mass_number[i] = ions["mass_number"][i]
+        cfg.cross_section[i] = ions["cross_section"][i]
+        cfg.pressure[i] = ions["pressure"][i]
+    cfg.temperature = ions["temperature"]
+    cfg.length = ions.get("length", ring.circ)
+    cfg.loss_boundary = ions["loss_boundary"]
+    cfg.ap_x, cfg.ap_y = p["ap_x"], p["ap_y"]
+    cfg.seed = ions["seed"]
+    return cfg

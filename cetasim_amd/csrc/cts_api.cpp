@@ -124,6 +124,22 @@ struct cts_handle {
     // ions (electron-side kick parameters per bunch)
     int nspec = 0;
     double *d_ion_e = nullptr;
+    // ion cloud at interaction point 0: two buffers of ion_cap slots per
+    // species (x, y, vx, vy, alive); the turn-end compaction swaps them
+    cts_ion_config ion{};
+    int ion_buf = 0;
+    int64_t ion_cap = 0;
+    int64_t ion_len_ub = 0;  // host upper bound of the slots in use at turn start
+    double *d_ion[2][CTS_MAX_SPECIES][4] = {};
+    uint8_t *d_ion_alive[2][CTS_MAX_SPECIES] = {};
+    int64_t *d_ion_len = nullptr;     // [2][CTS_MAX_SPECIES]
+    int64_t *d_ion_nalive = nullptr;  // [CTS_MAX_SPECIES]
+    double *d_ion_mean = nullptr;     // [2 * CTS_MAX_SPECIES]
+    double *d_ion_bview = nullptr, *d_ion_smax = nullptr, *d_ion_q = nullptr;
+    double *d_ion_cloud = nullptr, *d_ion_part = nullptr;
+    uint32_t *d_ion_cnt = nullptr, *d_ion_chunk = nullptr;
+    // d_cur holds the moments of the present coordinates
+    bool cur_valid = false;
 
     // timing
     bool timing = false;
@@ -446,6 +462,9 @@ int check_status(cts_handle *h, bool block)
     }
     if (*h->h_status & 1u)
         return fail(CTS_LOW_TRANSMISSION, "more than 50% particles are lost in a bunch (src/Bunch.cpp:193-198)");
+    if (*h->h_status & 2u)
+        return fail(CTS_ION_OVERFLOW, "the ion cloud would exceed its maximum macro-ion number "
+                                      "(src/LatticeInterActionPoint.cpp:1041-1049)");
     return CTS_OK;
 }
 
@@ -455,6 +474,137 @@ int reset_status(cts_handle *h)
     HIPCHK(hipMemset(h->d_status, 0, sizeof(uint32_t)));
     *h->h_status = 0;
     h->status_inflight = false;
+    return CTS_OK;
+}
+
+// Moments of the present coordinates into d_cur (MPBeamRMSCal): the first
+// pass fixes the shift (means), the second takes the centred sums about it.
+int enqueue_moments(cts_handle *h)
+{
+    int rc = flush_kick(h);
+    if (rc) return rc;
+    for (int it = 0; it < 2; it++) {
+        PassParams p = base_pass(h, OP_MOMA | OP_MOMB | OP_ZSTAT, 0);
+        if ((rc = run_pass(h, p, "moments"))) return rc;
+        FinParams f = base_fin(h, OP_MOMB | OP_MOMA | OP_ZSTAT, -1, -1);
+        if ((rc = run_finalize(h, f))) return rc;
+    }
+    h->cur_valid = true;
+    return CTS_OK;
+}
+
+void free_ion_state(cts_handle *h)
+{
+    for (int b = 0; b < 2; b++)
+        for (int p = 0; p < CTS_MAX_SPECIES; p++) {
+            for (int c = 0; c < 4; c++) {
+                if (h->d_ion[b][p][c]) (void)hipFree(h->d_ion[b][p][c]);
+                h->d_ion[b][p][c] = nullptr;
+            }
+            if (h->d_ion_alive[b][p]) (void)hipFree(h->d_ion_alive[b][p]);
+            h->d_ion_alive[b][p] = nullptr;
+        }
+    void *ptrs[] = {h->d_ion_len, h->d_ion_nalive, h->d_ion_mean, h->d_ion_bview, h->d_ion_smax,
+                    h->d_ion_q, h->d_ion_cloud, h->d_ion_part, h->d_ion_cnt, h->d_ion_chunk};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    h->d_ion_len = h->d_ion_nalive = nullptr;
+    h->d_ion_mean = h->d_ion_bview = h->d_ion_smax = h->d_ion_q = h->d_ion_cloud = h->d_ion_part = nullptr;
+    h->d_ion_cnt = h->d_ion_chunk = nullptr;
+    h->nspec = 0;
+    h->ion_cap = 0;
+    h->ion_len_ub = 0;
+    h->ion_buf = 0;
+}
+
+constexpr int kIonMaxWG = 2048;
+
+// MPBeam::SSBeamIonEffectOneInteractionPoint at interaction point 0
+// (src/MPBeam.cpp:2585-2611): bunch views from the moments of the present
+// state, one launch per bunch passage in train order, the turn-end
+// compaction of the cloud, then the electron kicks of every bunch.
+int ion_phase(cts_handle *h, int32_t turn)
+{
+    int rc;
+    if (!h->cur_valid && (rc = enqueue_moments(h))) return rc;
+    const cts_ion_config &I = h->ion;
+    const cts_section &S = h->secs[0];
+    {
+        TimedScope ts(h, "ion_prep");
+        HIPCHK(launch_ion_prep(h->d_cur, h->nb, S.beta_x, S.beta_y, h->d_ion_bview, h->d_ion_smax, h->stream));
+    }
+    const int buf = h->ion_buf;
+    IonPassParams P;
+    std::memset(&P, 0, sizeof P);
+    for (int p = 0; p < h->nspec; p++) {
+        P.x[p] = h->d_ion[buf][p][0];
+        P.y[p] = h->d_ion[buf][p][1];
+        P.vx[p] = h->d_ion[buf][p][2];
+        P.vy[p] = h->d_ion[buf][p][3];
+        P.alive[p] = h->d_ion_alive[buf][p];
+    }
+    P.len = h->d_ion_len + buf * CTS_MAX_SPECIES;
+    P.n_alive = h->d_ion_nalive;
+    P.mean_state = h->d_ion_mean;
+    P.bview = h->d_ion_bview;
+    P.size_max = h->d_ion_smax;
+    P.ion_q = h->d_ion_q;
+    P.ion_e = h->d_ion_e;
+    P.cloud = h->d_ion_cloud;
+    P.part = h->d_ion_part;
+    P.cnt = h->d_ion_cnt;
+    P.status = h->d_status;
+    P.gamma = h->ring.rgamma;
+    P.loss_boundary = I.loss_boundary;
+    P.ap_x = I.ap_x;
+    P.ap_y = I.ap_y;
+    P.max_ions = I.max_ions;
+    P.seed = I.seed;
+    P.nspec = h->nspec;
+    P.n_new = I.n_new;
+    P.turn = (uint32_t)turn;
+    int64_t ub = h->ion_len_ub;
+    for (int b = 0; b < h->nb; b++) {
+        // src/MPBunch.cpp:1117: 2 N_e r_e (m_e / m_u) / A c
+        for (int p = 0; p < h->nspec; p++)
+            P.coeffI[p] = 2.0 * h->n_e[b] * kRe * kElectronMassEV / kIonMassEV / I.mass_number[p] * kClight;
+        // src/LatticeInterActionPoint.cpp:1169-1170: C / h * gap / c
+        P.dt = h->ring.circ / h->ring.harmonic * h->gap[b] / kClight;
+        P.bunch = b;
+        P.passage = (uint32_t)b;
+        int64_t nwg = (ub + I.n_new + 2 * kIonBlock - 1) / (2 * kIonBlock);
+        nwg = std::max<int64_t>(1, std::min<int64_t>(nwg, kIonMaxWG));
+        TimedScope ts(h, "ion_passage");
+        HIPCHK(launch_ion_passage(P, (int)nwg, h->stream));
+        ub += I.n_new;
+    }
+    // stable compaction of every species into the other buffer
+    for (int p = 0; p < h->nspec; p++) {
+        IonCompactParams c;
+        c.x = h->d_ion[buf][p][0]; c.y = h->d_ion[buf][p][1];
+        c.vx = h->d_ion[buf][p][2]; c.vy = h->d_ion[buf][p][3];
+        c.alive = h->d_ion_alive[buf][p];
+        c.ox = h->d_ion[1 - buf][p][0]; c.oy = h->d_ion[1 - buf][p][1];
+        c.ovx = h->d_ion[1 - buf][p][2]; c.ovy = h->d_ion[1 - buf][p][3];
+        c.oalive = h->d_ion_alive[1 - buf][p];
+        c.len = h->d_ion_len + buf * CTS_MAX_SPECIES + p;
+        c.new_len = h->d_ion_len + (1 - buf) * CTS_MAX_SPECIES + p;
+        c.chunk_cnt = h->d_ion_chunk;
+        TimedScope ts(h, "ion_compact");
+        HIPCHK(launch_ion_compact(c, std::min<int64_t>(ub, h->ion_cap), h->stream));
+    }
+    h->ion_buf = 1 - buf;
+    h->ion_len_ub = std::min<int64_t>(ub, I.max_ions);
+    // electron side (src/MPBunch.cpp:1165-1206, src/Bunch.cpp:213-220)
+    IonKickParams ik;
+    ik.x = h->d_x; ik.y = h->d_y; ik.px = h->d_px; ik.py = h->d_py; ik.flag = h->d_flag;
+    ik.seg_begin = h->d_seg_begin; ik.seg_end = h->d_seg_end;
+    ik.seg_bunch = h->d_seg_bunch; ik.wg_seg = h->d_wg_seg;
+    ik.ion_e = h->d_ion_e;
+    ik.nspec = h->nspec;
+    TimedScope ts(h, "ion_kick");
+    HIPCHK(launch_ion_kick(ik, h->nwg, h->stream));
+    h->cur_valid = false;
     return CTS_OK;
 }
 
@@ -502,6 +652,7 @@ int cts_create(const cts_ring *ring, const cts_section *secs, int32_t nsec, cts_
     for (int n = 1; n <= 63; n++) tab[n - 1] = std::exp(-fad[2] * (double)(n * n));
     tab[63] = 0.0;
     e = upload_faddeeva_tables(tab, fad);
+    if (e == hipSuccess) e = upload_faddeeva_tables_ions(tab, fad);
     if (e == hipSuccess) e = hipMalloc((void **)&h->d_status, sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(h->d_status, 0, sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc((void **)&h->h_status, sizeof(uint32_t), hipHostMallocDefault);
@@ -529,6 +680,7 @@ int cts_destroy(cts_handle *h)
     resolve_timers(h);
     for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
     free_bunch_state(h);
+    free_ion_state(h);
     if (h->d_status) (void)hipFree(h->d_status);
     if (h->d_cst) (void)hipFree(h->d_cst);
     if (h->h_status) (void)hipHostFree(h->h_status);
@@ -544,6 +696,9 @@ int cts_set_bunches(cts_handle *h, int32_t nb, const int64_t *counts, const doub
     if (!h || nb < 1 || !counts || !macro_charge || !n_e) return fail(CTS_ERR_ARG, "cts_set_bunches: bad arguments");
     HIPCHK(hipStreamSynchronize(h->stream));
     free_bunch_state(h);
+    const cts_ion_config ions = h->ion;
+    const bool ions_on = h->nspec > 0;
+    free_ion_state(h);
     h->nb = nb;
     h->counts.assign(counts, counts + nb);
     h->macro_charge.assign(macro_charge, macro_charge + nb);
@@ -618,6 +773,7 @@ int cts_set_bunches(cts_handle *h, int32_t nb, const int64_t *counts, const doub
     h->kick_pending = false;
     h->dp_pending = false;
     h->last_turn = -1;
+    h->cur_valid = false;
     if (h->hist_cap > 0) {
         if ((rc = dalloc(&h->d_hist, (size_t)h->hist_cap * nb))) return rc;
     }
@@ -627,9 +783,11 @@ int cts_set_bunches(cts_handle *h, int32_t nb, const int64_t *counts, const doub
         std::vector<double> fr(h->omega.size());
         for (size_t i = 0; i < fr.size(); i++) fr[i] = h->omega[i] / (2 * kPi);
         std::vector<double> rs = h->rs, q = h->q;
-        return cts_set_wake(h, nbins, h->wake_mode, h->grid_lo, h->grid_hi, (int32_t)rs.size(), rs.data(),
-                            q.data(), fr.data());
+        rc = cts_set_wake(h, nbins, h->wake_mode, h->grid_lo, h->grid_hi, (int32_t)rs.size(), rs.data(),
+                          q.data(), fr.data());
+        if (rc) return rc;
     }
+    if (ions_on) return cts_set_ions(h, &ions);
     return CTS_OK;
 }
 
@@ -657,6 +815,7 @@ int cts_upload(cts_handle *h, const double *x, const double *px, const double *y
     h->kick_pending = false;
     h->dp_pending = false;
     h->last_turn = -1;
+    h->cur_valid = false;
     if ((rc = reset_status(h))) return rc;
     // moments of the uploaded state (the turn-0 MPBeamRMSCal)
     return cts_rms(h, nullptr);
@@ -781,18 +940,9 @@ int cts_turn(cts_handle *h, int32_t turn)
     uint32_t ops = OP_MAP | OP_RF | OP_DRIFT | OP_U0 | OP_LOSS | OP_MOMA | OP_WRITE;
     if (h->ring.skew_k != 0) ops |= OP_SKEW;
     if (h->synrad_on) ops |= OP_SYNRAD;
-    if (h->nspec > 0) {
-        // electron kick by the ion cloud of this passage, before the map
-        // (src/MPBeam.cpp:1064-1068)
-        IonKickParams ik;
-        ik.x = h->d_x; ik.y = h->d_y; ik.px = h->d_px; ik.py = h->d_py; ik.flag = h->d_flag;
-        ik.seg_begin = h->d_seg_begin; ik.seg_end = h->d_seg_end;
-        ik.seg_bunch = h->d_seg_bunch; ik.wg_seg = h->d_wg_seg;
-        ik.ion_e = h->d_ion_e;
-        ik.nspec = h->nspec;
-        TimedScope ts(h, "ion_kick");
-        HIPCHK(launch_ion_kick(ik, h->nwg, h->stream));
-    }
+    // beam-ion phase at interaction point 0, before the map
+    // (src/MPBeam.cpp:1061-1068)
+    if (h->nspec > 0 && (rc = ion_phase(h, turn))) return rc;
     const bool wake = h->nbins > 0;
     if (h->kick_pending) ops |= OP_KICK | OP_MOMB;
     if (wake) ops |= (h->wake_mode == 1) ? OP_BIN : OP_ZSTAT;
@@ -819,6 +969,7 @@ int cts_turn(cts_handle *h, int32_t turn)
     }
     h->last_turn = turn;
     h->dp_pending = wake;
+    h->cur_valid = true;
     return enqueue_status_copy(h);
 }
 
@@ -837,6 +988,7 @@ static int seam(cts_handle *h, uint32_t ops, int k, uint32_t turn)
     if ((rc = flush_kick(h))) return rc;
     PassParams p = base_pass(h, ops | OP_WRITE, k);
     p.turn = turn;
+    h->cur_valid = false;
     return run_pass(h, p);
 }
 
@@ -903,19 +1055,101 @@ int cts_mark_lost(cts_handle *h, double *transmission)
     return check_status(h, true);
 }
 
+int cts_set_ions(cts_handle *h, const cts_ion_config *cfg)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!cfg || cfg->nspec < 0 || cfg->nspec > CTS_MAX_SPECIES) return fail(CTS_ERR_ARG, "cts_set_ions: nspec 0..4");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    free_ion_state(h);
+    if (cfg->nspec == 0) return CTS_OK;
+    if (cfg->n_new < 1 || cfg->max_ions < 1 || !(cfg->temperature > 0) || !(cfg->length >= 0) ||
+        !(cfg->ap_x > 0) || !(cfg->ap_y > 0) || !(cfg->loss_boundary > 0))
+        return fail(CTS_ERR_ARG, "cts_set_ions: n_new, max_ions, temperature, apertures and loss boundary must be > 0");
+    for (int p = 0; p < cfg->nspec; p++)
+        if (!(cfg->mass_number[p] > 0)) return fail(CTS_ERR_ARG, "cts_set_ions: mass number must be > 0");
+    if (h->secs.empty()) return fail(CTS_ERR_STATE, "cts_set_ions: no sections");
+    // one passage appends n_new per species; a turn appends nb * n_new
+    // before the turn-end compaction
+    const int64_t cap = cfg->max_ions + (int64_t)h->nb * cfg->n_new;
+    if (cap > (int64_t)1 << 31) return fail(CTS_ERR_ARG, "cts_set_ions: cloud capacity above 2^31 slots");
+    h->ion = *cfg;
+    h->ion_cap = cap;
+    for (int b = 0; b < 2; b++)
+        for (int p = 0; p < cfg->nspec; p++) {
+            for (int c = 0; c < 4; c++)
+                if ((rc = dalloc(&h->d_ion[b][p][c], (size_t)cap))) return rc;
+            if ((rc = dalloc(&h->d_ion_alive[b][p], (size_t)cap))) return rc;
+        }
+    const int64_t nchunk = (cap + kIonChunk - 1) / kIonChunk;
+    if ((rc = dalloc(&h->d_ion_len, 2 * CTS_MAX_SPECIES)) || (rc = dalloc(&h->d_ion_nalive, CTS_MAX_SPECIES)) ||
+        (rc = dalloc(&h->d_ion_mean, 2 * CTS_MAX_SPECIES)) || (rc = dalloc(&h->d_ion_bview, (size_t)h->nb * 4)) ||
+        (rc = dalloc(&h->d_ion_smax, 2)) || (rc = dalloc(&h->d_ion_q, (size_t)h->nb * cfg->nspec)) ||
+        (rc = dalloc(&h->d_ion_cloud, (size_t)h->nb * CTS_MAX_SPECIES * 6)) ||
+        (rc = dalloc(&h->d_ion_part, (size_t)kIonMaxWG * CTS_MAX_SPECIES * 6)) || (rc = dalloc(&h->d_ion_cnt, 1)) ||
+        (rc = dalloc(&h->d_ion_chunk, (size_t)nchunk)))
+        return rc;
+    // macro-ion charge per bunch passage: line density sigma P / (k_B T) N_e
+    // over the interaction length, shared by n_new macro-ions
+    // (src/LatticeInterActionPoint.cpp:974-981, 998)
+    std::vector<double> q((size_t)h->nb * cfg->nspec);
+    for (int b = 0; b < h->nb; b++)
+        for (int p = 0; p < cfg->nspec; p++) {
+            const double line = cfg->cross_section[p] * cfg->pressure[p] / cfg->temperature / kBoltzmann * h->n_e[b];
+            q[(size_t)b * cfg->nspec + p] = line * cfg->length / cfg->n_new;
+        }
+    HIPCHK(hipMemcpy(h->d_ion_q, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(h->d_ion_e, 0, (size_t)h->nb * CTS_MAX_SPECIES * 5 * sizeof(double)));
+    h->nspec = cfg->nspec;
+    return CTS_OK;
+}
+
+int cts_ion_phase(cts_handle *h, int32_t turn)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (h->nspec == 0) return fail(CTS_ERR_STATE, "cts_ion_phase: ions are off (cts_set_ions)");
+    if ((rc = flush_kick(h))) return rc;
+    if ((rc = ion_phase(h, turn))) return rc;
+    return enqueue_status_copy(h);
+}
+
+int cts_get_ion_cloud(cts_handle *h, int32_t species, int64_t cap, int64_t *n, double *x, double *y, double *vx,
+                      double *vy)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (species < 0 || species >= h->nspec || !n) return fail(CTS_ERR_ARG, "cts_get_ion_cloud: bad species");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    int64_t len = 0;
+    HIPCHK(hipMemcpy(&len, h->d_ion_len + h->ion_buf * CTS_MAX_SPECIES + species, sizeof len,
+                     hipMemcpyDeviceToHost));
+    *n = len;
+    const int64_t m = std::min(len, cap);
+    double *dst[4] = {x, y, vx, vy};
+    for (int c = 0; c < 4; c++)
+        if (dst[c] && m > 0)
+            HIPCHK(hipMemcpy(dst[c], h->d_ion[h->ion_buf][species][c], (size_t)m * sizeof(double),
+                             hipMemcpyDeviceToHost));
+    return CTS_OK;
+}
+
+int cts_get_ion_passages(cts_handle *h, double *out)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (h->nspec == 0 || !out) return fail(CTS_ERR_STATE, "cts_get_ion_passages: ions are off");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(out, h->d_ion_cloud, (size_t)h->nb * CTS_MAX_SPECIES * 6 * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    return CTS_OK;
+}
+
 int cts_rms(cts_handle *h, cts_moments *out)
 {
     int rc = check_ready(h);
     if (rc) return rc;
-    if ((rc = flush_kick(h))) return rc;
-    // two passes: the first fixes the shift (means), the second takes the
-    // centred sums about it
-    for (int it = 0; it < 2; it++) {
-        PassParams p = base_pass(h, OP_MOMA | OP_MOMB | OP_ZSTAT, 0);
-        if ((rc = run_pass(h, p, "moments"))) return rc;
-        FinParams f = base_fin(h, OP_MOMB | OP_MOMA | OP_ZSTAT, -1, -1);
-        if ((rc = run_finalize(h, f))) return rc;
-    }
+    if ((rc = enqueue_moments(h))) return rc;
     if (out) {
         HIPCHK(hipMemcpyAsync(out, h->d_cur, h->nb * sizeof(cts_moments), hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));

@@ -16,6 +16,9 @@ constexpr double kPi = 3.141592653589793;  // 4*atan(1), include/Global.h:33
 constexpr double kClight = 299792458.0;
 constexpr double kEcharge = 1.602017733e-19;
 constexpr double kRe = 2.81794092E-15;
+constexpr double kElectronMassEV = 0.51099906e6;  // include/Global.h:30
+constexpr double kIonMassEV = 931.49432e+6;       // atomic mass unit, include/Global.h:32
+constexpr double kBoltzmann = 1.3806505E-23;      // include/Global.h:36
 
 // ---------------------------------------------------------------- Philox
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).

@@ -15,10 +15,12 @@
 
 namespace cts {
 
-// exp(-a^2 n^2), n = 1..64, a = pi/sqrt(-log(eps/2)); filled at library init.
-extern __constant__ double c_ea2n2[64];
-// {a, c = 2a/pi, a^2}
-extern __constant__ double c_fad[3];
+// exp(-a^2 n^2), n = 1..64, a = pi/sqrt(-log(eps/2)), and {a, c = 2a/pi,
+// a^2}.  The library is built without relocatable device code, so every
+// translation unit that evaluates w(z) holds its own copy of the tables and
+// exports an upload function that fills it; cts_create calls each of them.
+static __constant__ double c_ea2n2[64];
+static __constant__ double c_fad[3];
 
 constexpr double kIsPi = 0.56418958354775628694807945156;  // 1/sqrt(pi)
 

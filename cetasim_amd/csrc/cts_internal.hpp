@@ -83,6 +83,39 @@ struct IonKickParams {
     int32_t nspec;
 };
 
+// One bunch passage through the ion cloud (all species).
+struct IonPassParams {
+    double *x[CTS_MAX_SPECIES], *y[CTS_MAX_SPECIES], *vx[CTS_MAX_SPECIES], *vy[CTS_MAX_SPECIES];
+    uint8_t *alive[CTS_MAX_SPECIES];
+    int64_t *len;            // [nspec] slots in use (alive or not) of the current buffer
+    int64_t *n_alive;        // [nspec] ions alive after the last passage
+    double *mean_state;      // [2 * nspec] cloud mean kept when the cloud has < 2 ions
+    const double *bview;     // [nb][4] bunch xbar, ybar, rmsRx, rmsRy
+    const double *size_max;  // [2] largest rmsRx, rmsRy over the bunches
+    const double *ion_q;     // [nb][nspec] macro-ion charge (ions per macro-ion)
+    double *ion_e;           // [nb][CTS_MAX_SPECIES][5] electron-side kick parameters
+    double *cloud;           // [nb][CTS_MAX_SPECIES][6] count, mean x/y, rms x/y, ion number
+    double *part;            // [grid][CTS_MAX_SPECIES][6] workgroup partial sums
+    uint32_t *cnt;           // last-workgroup counter, left at zero
+    uint32_t *status;
+    double coeffI[CTS_MAX_SPECIES];
+    double dt, gamma, loss_boundary, ap_x, ap_y;
+    int64_t max_ions;
+    uint64_t seed;
+    int32_t nspec, n_new, bunch;
+    uint32_t turn, passage;
+};
+
+struct IonCompactParams {
+    const double *x, *y, *vx, *vy;
+    const uint8_t *alive;
+    double *ox, *oy, *ovx, *ovy;
+    uint8_t *oalive;
+    const int64_t *len;   // slots in use of the source buffer
+    int64_t *new_len;     // slots in use of the destination buffer
+    uint32_t *chunk_cnt;
+};
+
 struct BinParams {
     const double *z;
     const int64_t *seg_begin, *seg_end;
@@ -137,8 +170,15 @@ hipError_t launch_finalize(const FinParams &p, hipStream_t st);
 hipError_t launch_post(const FinParams &f, const WakeParams &w, hipStream_t st);
 hipError_t launch_wake_table(const WakeTableParams &p, int ntables, hipStream_t st);
 hipError_t launch_wake(const WakeParams &p, int nb, hipStream_t st);
+hipError_t launch_ion_prep(const cts_moments *cur, int nb, double beta_x, double beta_y, double *bview,
+                           double *size_max, hipStream_t st);
+hipError_t launch_ion_passage(const IonPassParams &p, int nwg, hipStream_t st);
+hipError_t launch_ion_compact(const IonCompactParams &p, int64_t len_bound, hipStream_t st);
+constexpr int kIonBlock = 256;
+constexpr int kIonChunk = 4096;  // compaction chunk (16 ions per thread)
 hipError_t launch_qe_debug(uint64_t seed, uint32_t turn, uint64_t base, int64_t n, double *out,
                            hipStream_t st);
 hipError_t upload_faddeeva_tables(const double ea2n2[64], const double fad[3]);
+hipError_t upload_faddeeva_tables_ions(const double ea2n2[64], const double fad[3]);
 
 }  // namespace cts

@@ -15,9 +15,6 @@
 
 namespace cts {
 
-__constant__ double c_ea2n2[64];
-__constant__ double c_fad[3];
-
 hipError_t upload_faddeeva_tables(const double ea2n2[64], const double fad[3])
 {
     hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_ea2n2), ea2n2, 64 * sizeof(double));

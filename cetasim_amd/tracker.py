@@ -73,6 +73,30 @@ class Tracker:
         assert noise.size == 5 * self.ntot
         capi.check(self.lib.cts_push_noise(self.h, capi.dptr(noise)))
 
+    def set_ions(self, cfg):
+        """cfg: capi.IonConfig, or None to switch ions off."""
+        if cfg is None:
+            cfg = capi.IonConfig()
+        capi.check(self.lib.cts_set_ions(self.h, ctypes.byref(cfg)))
+        self.nspec = cfg.nspec
+
+    def ion_phase(self, turn):
+        return capi.check(self.lib.cts_ion_phase(self.h, turn))
+
+    def ion_cloud(self, species=0):
+        n = ctypes.c_int64()
+        capi.check(self.lib.cts_get_ion_cloud(self.h, species, 0, ctypes.byref(n), None, None, None, None))
+        out = {c: np.empty(n.value) for c in ("x", "y", "vx", "vy")}
+        capi.check(self.lib.cts_get_ion_cloud(self.h, species, n.value, ctypes.byref(n),
+                                              *[capi.dptr(out[c]) for c in ("x", "y", "vx", "vy")]))
+        return out
+
+    def ion_passages(self):
+        """[nb][MAX_SPECIES][6]: count, mean x, mean y, rms x, rms y, ion number."""
+        out = np.empty(self.nb * capi.MAX_SPECIES * 6)
+        capi.check(self.lib.cts_get_ion_passages(self.h, capi.dptr(out)))
+        return out.reshape(self.nb, capi.MAX_SPECIES, 6)
+
     def turn(self, n):
         return capi.check(self.lib.cts_turn(self.h, n))
 

@@ -137,6 +137,29 @@ int cts_set_synrad(cts_handle *h, int32_t on, int32_t noise_mode, uint64_t seed)
 /* noise[5][ntot]: r0 r1 r2 r3 r5 of every particle for the next synrad. */
 int cts_push_noise(cts_handle *h, const double *noise);
 
+/* Residual-gas ions at interaction point 0: the ion part of
+ * LatticeInterActionPoint::Initial / InitialLatticeIonInfo
+ * (src/LatticeInterActionPoint.cpp:39-75, 293-324) and the ionCal* settings
+ * of input.dat (src/ReadInputSettings.cpp:480-550).  pressure[p] is
+ * gasPercent[p] times the IP's vacuum pressure.
+ * New macro-ions come from the counter-based Philox stream keyed by seed
+ * (the reference draws them from a random_device-seeded mt19937). */
+typedef struct {
+    int32_t nspec;                 /* ionCalGasSpec, <= CTS_MAX_SPECIES; 0 = ions off */
+    int32_t n_new;                 /* ionCalMacroIonNumberGeneratedPerIP */
+    int64_t max_ions;              /* ionCalIonMaxNumber, per species */
+    double mass_number[CTS_MAX_SPECIES];   /* ionCalIonMassNumber */
+    double cross_section[CTS_MAX_SPECIES]; /* corssSectionEI [m^2] */
+    double pressure[CTS_MAX_SPECIES];      /* partial pressure at the IP [Pa] */
+    double temperature;            /* [K] */
+    double length;                 /* interactionLength [m] */
+    double loss_boundary;          /* ionCalIonlossboundary, in largest bunch rms sizes */
+    double ap_x, ap_y;             /* pipe aperture at the IP [m] */
+    uint64_t seed;
+} cts_ion_config;
+/* Set up (or switch off, nspec = 0) the ion cloud; the cloud starts empty. */
+int cts_set_ions(cts_handle *h, const cts_ion_config *cfg);
+
 /* ------------------------------------------------------------ the turn
  * One turn of MPBeam::Run (src/MPBeam.cpp:1059-1102) for every bunch:
  * [beam-ion phase at IP 0] -> section map -> skew quad -> RF -> drift ->
@@ -169,6 +192,21 @@ int cts_sr_wake(cts_handle *h, const double *sigma_z_stale);
 /* MPBeam::MarkParticleLostInBunch (src/MPBeam.cpp:1358); transmission[nb]
  * may be NULL. */
 int cts_mark_lost(cts_handle *h, double *transmission);
+/* MPBeam::SSBeamIonEffectOneInteractionPoint at IP 0 (src/MPBeam.cpp:
+ * 2585-2611) preceded by its MPBeamRMSCal/MPGetBeamInfo (src/MPBeam.cpp:
+ * 1061-1063): every bunch passes the cloud in train order (generation,
+ * cloud moments, Bassetti-Erskine kicks both ways, ion drift and loss);
+ * the electrons' kicks are applied.  turn keys the ion generation stream.
+ * cts_turn calls this itself when ions are on. */
+int cts_ion_phase(cts_handle *h, int32_t turn);
+/* The cloud of one species after the last ion phase (survivors in order):
+ * *n gets the count; arrays of `cap` entries may be NULL. */
+int cts_get_ion_cloud(cts_handle *h, int32_t species, int64_t cap, int64_t *n, double *x,
+                      double *y, double *vx, double *vy);
+/* Per passage of the last ion phase: out[(b * CTS_MAX_SPECIES + p) * 6 ..]
+ * = macro-ion count, mean x, mean y, rms x, rms y, ion number seen by bunch
+ * b (IonRMSCal(k, p) and GetTotIonCharge inputs). out has nb*CTS_MAX_SPECIES*6. */
+int cts_get_ion_passages(cts_handle *h, double *out);
 /* MPBeam::MPBeamRMSCal (src/MPBeam.cpp:2126): moments of the current state
  * (applies a pending wake kick first). out[nb]. */
 int cts_rms(cts_handle *h, cts_moments *out);

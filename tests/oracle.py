@@ -222,3 +222,80 @@ class OracleTurn:
         cnt = I64()
         t = L.orc_mark_lost(ctypes.byref(b.s), R.t0, R.harmonic, R.ap_x, R.ap_y, ctypes.byref(cnt))
         return t, cnt.value
+
+
+KBOLTZ = 1.3806505E-23
+
+
+class IonCloud:
+    """numpy-owned orc_ion_cloud (one interaction point)."""
+
+    def __init__(self, cfg, circ, harmonic, nb):
+        self.cap = int(cfg.max_ions)
+        self.nspec = cfg.nspec
+        size = self.cap + int(cfg.n_new) + 1
+        self.arr = [{k: np.zeros(size) for k in ("x", "y", "vx", "vy", "fx", "fy")} for _ in range(cfg.nspec)]
+        c = OrcIonCloud()
+        c.nspec = cfg.nspec
+        c.cap = self.cap
+        for p_ in range(cfg.nspec):
+            for k in ("x", "y", "vx", "vy", "fx", "fy"):
+                getattr(c, k)[p_] = p(self.arr[p_][k])
+            c.mass_number[p_] = cfg.mass_number[p_]
+            c.cross_section[p_] = cfg.cross_section[p_]
+            c.pressure[p_] = cfg.pressure[p_]
+        c.temperature = cfg.temperature
+        c.length = cfg.length
+        c.n_new = cfg.n_new
+        c.loss_boundary = cfg.loss_boundary
+        c.ap_x, c.ap_y = cfg.ap_x, cfg.ap_y
+        c.circ = circ
+        c.harm = harmonic
+        self.c = c
+
+    def species(self, p_=0):
+        n = self.c.n[p_]
+        return {k: self.arr[p_][k][:n].copy() for k in ("x", "y", "vx", "vy")}
+
+
+class OracleTrain:
+    """The reference turn of a bunch train with the beam-ion phase at
+    interaction point 0 (src/MPBeam.cpp:1059-1102, 2585-2611): moments of
+    every bunch, passages in train order, then each bunch's own turn."""
+
+    def __init__(self, ring, sec, counts, n_e, gap, ion_cfg, seed, **turn_kw):
+        self.L = lib()
+        self.ring, self.sec = ring, sec
+        self.counts = np.asarray(counts, dtype=np.int64)
+        self.offsets = np.concatenate([[0], np.cumsum(self.counts)])
+        self.n_e = np.asarray(n_e, dtype=np.float64)
+        self.gap = np.asarray(gap, dtype=np.int32)
+        self.cloud = IonCloud(ion_cfg, ring.circ, ring.harmonic, len(counts)) if ion_cfg is not None else None
+        self.seed = seed
+        self.turns = [OracleTurn(ring, sec, base=int(self.offsets[b]), **{**turn_kw, **(
+            {"macro_charge": turn_kw["macro_charge"][b]} if "macro_charge" in turn_kw else {})})
+            for b in range(len(counts))]
+        self.last_cloud = None
+
+    def ion_phase(self, bunches, turn):
+        ms = [rms(b, self.sec) for b in bunches]
+        smax_x = max(m.rms_x for m in ms)
+        smax_y = max(m.rms_y for m in ms)
+        out = np.zeros((len(bunches), self.cloud.nspec, 5))
+        for j, b in enumerate(bunches):
+            m = ms[j]
+            v = OrcBunchView(m.mean[0], m.mean[2], m.rms_x, m.rms_y, self.n_e[j], int(self.gap[j]),
+                             self.ring.rgamma)
+            o = np.zeros(5 * self.cloud.nspec)
+            rc = self.L.orc_ion_passage(ctypes.byref(self.cloud.c), ctypes.byref(b.s), ctypes.byref(v),
+                                        smax_x, smax_y, self.seed, turn, j, p(o))
+            if rc != 0:
+                raise RuntimeError("ion cloud above its maximum")
+            out[j] = o.reshape(self.cloud.nspec, 5)
+        self.last_cloud = out
+        return out
+
+    def turn(self, bunches, turn):
+        if self.cloud is not None:
+            self.ion_phase(bunches, turn)
+        return [t.turn(b, turn) for t, b in zip(self.turns, bunches)]

@@ -38,7 +38,7 @@ def test_struct_layout_matches_headers(tmp_path):
     """sizeof/offsetof of every struct as the C compiler lays it out."""
     structs = {"cts_ring": capi.Ring, "cts_section": capi.Section, "cts_moments": capi.Moments,
                "cts_derived": capi.Derived, "cts_twiss": capi.Twiss, "cts_ring_input": capi.RingInput,
-               "cts_ring_derived": capi.RingDerived}
+               "cts_ring_derived": capi.RingDerived, "cts_ion_config": capi.IonConfig}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "cetasim_host.h"', "int main(void){"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')

@@ -1,0 +1,152 @@
+"""Bunch trains on the HIP path vs the CPU oracle (MI355X): uneven
+macroparticle counts in one launch with per-bunch bins, and the beam-ion
+phase (ion generation, cloud moments, Bassetti-Erskine kicks both ways,
+ion drift and loss, bunch by bunch in train order).
+
+Tolerances: bin counts and loss codes bit-exact; coordinates 1e-11 relative
+to each coordinate's largest magnitude; ion cloud 1e-9 relative (its moments
+are summed in a different order than the oracle's sequential loops, and the
+drift step C/h*gap/c is folded into one factor).
+"""
+import numpy as np
+import pytest
+
+from cetasim_amd import capi, configs
+from cetasim_amd.tracker import Tracker
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+COORDS = O.COORDS
+BBR = configs.BASE["bbr"]
+
+
+def setup_train(counts, seed, ions=None, spacing=2, gap=None, p=configs.BASE):
+    ring, secs, _ = configs.lattice(p)
+    counts = np.asarray(counts, dtype=np.int64)
+    nb = len(counts)
+    gap = configs.train_gaps(nb, spacing, ring.harmonic) if gap is None else np.asarray(gap, dtype=np.int32)
+    q, ne = configs.bunch_scalars(counts, ring, p)
+    parts = configs.train_bunches(counts, seed, p)
+    tr = Tracker(ring, secs)
+    tr.set_bunches(counts, q, ne, gap)
+    tr.upload(parts)
+    cfg = None
+    if ions is not None:
+        cfg = configs.ion_config(ring, ions, p)
+        tr.set_ions(cfg)
+    off = np.concatenate([[0], np.cumsum(counts)])
+    bunches = [O.Bunch({c: parts[c][off[b]:off[b + 1]] for c in COORDS}) for b in range(nb)]
+    return ring, secs, counts, gap, q, ne, cfg, tr, bunches, off
+
+
+def compare_coords(got, bunches, off, tol):
+    for b, ob in enumerate(bunches):
+        for c in COORDS:
+            a = got[c][off[b]:off[b + 1]]
+            e = ob[c]
+            scale = max(np.max(np.abs(e)), 1e-300)
+            err = np.max(np.abs(a - e)) / scale
+            assert err <= tol, (b, c, err)
+        assert np.array_equal(got["lost"][off[b]:off[b + 1]], ob.lost), b
+
+
+ION_TEST = dict(mass_number=[28.0], cross_section=[2.0e-22], pressure=[1.0e-5], temperature=300.0,
+                n_new=300, max_ions=50_000, loss_boundary=10.0, seed=11)
+
+
+@pytest.mark.parametrize("mode", [1, 0])
+def test_train_uneven_counts(mode):
+    """Five bunches of uneven size, per-bunch SR-wake bins (fixed grid or
+    the reference's edges), device quantum excitation: 4 turns."""
+    counts = [3000, 7001, 1500, 4096, 2222]
+    ring, secs, counts, gap, q, ne, _, tr, bunches, off = setup_train(counts, 5)
+    sz = configs.BASE["sig"][2]
+    nbins = 120
+    tr.set_wake(nbins, mode, -5 * sz, 5 * sz, BBR["rs"], BBR["q"], BBR["fr"])
+    tr.set_synrad(True, 2, 9)
+    ot = O.OracleTrain(ring, secs[0], counts, ne, gap, None, 0, nbins=nbins, mode=mode, grid=(-5 * sz, 5 * sz),
+                       bbr=BBR, macro_charge=list(q), synrad=True, noise_seed=9)
+    for t in range(4):
+        tr.turn(t)
+        ot.turn(bunches, t)
+        cnt, _, _ = tr.wake_state()
+        for b in range(len(counts)):
+            assert np.array_equal(cnt[b], ot.turns[b].counts[:nbins]), (t, b)
+    got = tr.download()
+    compare_coords(got, bunches, off, 1e-11)
+    m = tr.moments()
+    for b, ob in enumerate(bunches):
+        om = O.rms(ob, secs[0])
+        assert m[b]["n_alive"] == om.n_alive
+        for i in (0, 2, 4):
+            assert abs(m[b]["mean"][i] - om.mean[i]) <= 1e-9 * abs(m[b]["c2"][3 * (i // 2)]) ** 0.5
+    tr.close()
+
+
+def test_ion_phase_matches_oracle():
+    """One ion phase (4 bunch passages) from an empty cloud: the cloud, the
+    per-passage cloud moments and the electrons' kicks."""
+    counts = [2000, 3500, 1200, 2800]
+    # short gaps throughout, so that part of the cloud survives the phase
+    ring, secs, counts, gap, q, ne, cfg, tr, bunches, off = setup_train(counts, 7, ions=ION_TEST, gap=[2, 2, 2, 3])
+    ot = O.OracleTrain(ring, secs[0], counts, ne, gap, cfg, ION_TEST["seed"])
+    before = {c: np.concatenate([b[c] for b in bunches]) for c in COORDS}
+    tr.ion_phase(0)
+    exp = ot.ion_phase(bunches, 0)
+    got = tr.ion_passages()
+    for b in range(len(counts)):
+        # ion number, mean x/y, rms x/y seen by passage b
+        np.testing.assert_allclose(got[b, 0, 5], exp[b, 0, 0], rtol=1e-12)
+        np.testing.assert_allclose(got[b, 0, 1:5], exp[b, 0, 1:5], rtol=1e-9, atol=1e-18)
+        # macro-ion count: survivors of the earlier passages plus n_new
+        assert got[b, 0, 0] <= (b + 1) * ION_TEST["n_new"]
+    assert got[0, 0, 0] == ION_TEST["n_new"]
+    cloud = tr.ion_cloud(0)
+    ec = ot.cloud.species(0)
+    assert len(cloud["x"]) == len(ec["x"]) > 0
+    for k in ("x", "y", "vx", "vy"):
+        np.testing.assert_allclose(cloud[k], ec[k], rtol=1e-9, atol=1e-9 * np.max(np.abs(ec[k])))
+    after = tr.download()
+    compare_coords(after, bunches, off, 1e-12)
+    # the kick is far above the tolerance (the comparison is not vacuous)
+    dpx = np.max(np.abs(after["px"] - before["px"]))
+    assert dpx > 1e-6 * np.max(np.abs(before["px"])), dpx
+    tr.close()
+
+
+def test_ion_train_turns():
+    """Three full turns with the ion phase, wake and excitation on."""
+    counts = [2500, 1800, 3300]
+    ring, secs, counts, gap, q, ne, cfg, tr, bunches, off = setup_train(counts, 8, ions=ION_TEST)
+    sz = configs.BASE["sig"][2]
+    nbins = 100
+    tr.set_wake(nbins, 1, -5 * sz, 5 * sz, BBR["rs"], BBR["q"], BBR["fr"])
+    tr.set_synrad(True, 2, 4)
+    ot = O.OracleTrain(ring, secs[0], counts, ne, gap, cfg, ION_TEST["seed"], nbins=nbins, mode=1,
+                       grid=(-5 * sz, 5 * sz), bbr=BBR, macro_charge=list(q), synrad=True, noise_seed=4)
+    for t in range(3):
+        tr.turn(t)
+        ot.turn(bunches, t)
+        got = tr.ion_passages()
+        np.testing.assert_allclose(got[:, 0, 5], ot.last_cloud[:, 0, 0], rtol=1e-12)
+        np.testing.assert_allclose(got[:, 0, 3:5], ot.last_cloud[:, 0, 3:5], rtol=1e-8)
+    got = tr.download()
+    compare_coords(got, bunches, off, 1e-11)
+    cloud = tr.ion_cloud(0)
+    assert len(cloud["x"]) == ot.cloud.c.n[0]
+    tr.close()
+
+
+def test_ion_overflow_status():
+    """A cloud that would pass max_ions ends the turn with CTS_ION_OVERFLOW,
+    as the reference exits (src/LatticeInterActionPoint.cpp:1041-1049)."""
+    counts = [1000, 1000, 1000]
+    ions = dict(ION_TEST, max_ions=500, n_new=300)
+    ring, secs, counts, gap, q, ne, cfg, tr, bunches, off = setup_train(counts, 9, ions=ions)
+    tr.ion_phase(0)
+    with pytest.raises(capi.CtsError) as e:
+        tr.sync()
+    assert e.value.code == capi.CTS_ION_OVERFLOW
+    tr.close()
