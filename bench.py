@@ -24,6 +24,20 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_MPT = 96.0  # 6 FP64 read + 6 FP64 written per macroparticle-turn
 
 
+def measured_traffic_per_mpt():
+    """HBM bytes per macroparticle-turn of k_pass from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, "profiles", "k_pass_c2_hbm_traffic.txt")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if "traffic per launch" in line and "B per macroparticle-turn" in line:
+                    return float(line.split("(")[1].split()[0]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -33,6 +47,9 @@ def parse():
     ap.add_argument("--nbins", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--workload", choices=("c2", "c3"), default="c2",
+                    help="c2: single bunch 1e7 macroparticles with wake and excitation; "
+                         "c3: 80-bunch train with the beam-ion phase")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reads MIB, writes MIB) three times, "
                          "to scale FETCH_SIZE/WRITE_SIZE counters")
@@ -77,6 +94,116 @@ def cpu_baseline(ring, secs, p, budget_s):
                       f"(oracle/ -O2 C restatement, {platform.processor() or platform.machine()})"}
 
 
+def cpu_baseline_train(ring, secs, p, budget_s):
+    """The oracle (one core) on the train with ions, timed in three parts
+    and extrapolated to the workload: one full ion phase (80 passages, 6000
+    new macro-ions each, 300 macroparticles per bunch), the electron-side
+    kick per macroparticle (one 200 000-macroparticle bunch through a
+    10-ion cloud), and the rest of the turn per macroparticle (turns of a
+    20 000-macroparticle bunch)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import oracle as O
+    from cetasim_amd import configs
+
+    T = configs.TRAIN
+    nb = T["nb"]
+    gap = configs.train_gaps(nb, T["spacing"], ring.harmonic)
+    cfg = configs.ion_config(ring, T["ions"], p)
+
+    def phase_time(counts, cfg_):
+        counts = np.asarray(counts)
+        q, ne = configs.bunch_scalars(counts, ring, p)
+        parts = configs.train_bunches(counts, 3, p)
+        off = np.concatenate([[0], np.cumsum(counts)])
+        bunches = [O.Bunch({c: parts[c][off[b]:off[b + 1]] for c in O.COORDS}) for b in range(len(counts))]
+        ot = O.OracleTrain(ring, secs[0], counts, ne, gap[:len(counts)], cfg_, T["ions"]["seed"])
+        t0 = time.perf_counter()
+        ot.ion_phase(bunches, 0)
+        return time.perf_counter() - t0
+
+    t_ion = phase_time(np.full(nb, 300), cfg)
+    small = configs.ion_config(ring, dict(T["ions"], n_new=10), p)
+    ne_big = 200_000
+    t_e = phase_time([ne_big], small) / ne_big
+    n = 20_000
+    ot = O.OracleTurn(ring, secs[0], synrad=True, noise_seed=2)
+    ob = O.Bunch(configs.gaussian_bunch(n, 2, p))
+    turns = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s or turns < 2:
+        ot.turn(ob, turns)
+        turns += 1
+    t_rest = (time.perf_counter() - t0) / (n * turns)
+    ntot = int(configs.train_counts().sum())
+    t_turn = t_ion + ntot * (t_e + t_rest)
+    return {"value": ntot / t_turn, "unit": "macroparticle-turns/s", "cores": 1, "kind": "port",
+            "sample": f"one full ion phase (80 passages, cloud to {int(cfg.n_new) * nb} macro-ions, "
+                      f"{t_ion:.2f} s), electron-side kick {t_e * 1e9:.0f} ns per macroparticle, rest of the "
+                      f"turn {t_rest * 1e9:.0f} ns per macroparticle ({turns} turns of {n}); extrapolated to "
+                      f"{ntot} macroparticles (oracle/ -O2 C restatement, "
+                      f"{platform.processor() or platform.machine()})"}
+
+
+def run_train(a):
+    """80-bunch train with the beam-ion phase at one interaction point."""
+    from cetasim_amd import configs
+    from cetasim_amd.tracker import Tracker
+
+    p = configs.BASE
+    T = configs.TRAIN
+    ring, secs, _ = configs.lattice(p)
+    counts = configs.train_counts()
+    nb = len(counts)
+    gap = configs.train_gaps(nb, T["spacing"], ring.harmonic)
+    q, ne = configs.bunch_scalars(counts, ring, p)
+    parts = configs.train_bunches(counts, T["seed"], p)
+    tr = Tracker(ring, secs)
+    tr.set_bunches(counts, q, ne, gap)
+    tr.upload(parts)
+    del parts
+    tr.set_ions(configs.ion_config(ring, T["ions"], p))
+    tr.set_synrad(True, 2, 2)
+    for t in range(a.warmup):
+        tr.turn(t)
+    tr.sync()
+    tr.set_timing(True)
+    t0 = time.perf_counter()
+    for t in range(a.warmup, a.warmup + a.steps):
+        tr.turn(t)
+    tr.sync()
+    dt = time.perf_counter() - t0
+    ktimes = {k: tr.kernel_time(k) for k in ("pass", "finalize", "ion_prep", "ion_passage", "ion_compact",
+                                              "ion_kick")}
+    tr.set_timing(False)
+    cloud = tr.ion_passages()
+    tr.close()
+    n = int(counts.sum())
+    pass_ms = ktimes["pass"][0]
+    achieved = BYTES_PER_MPT * n / (pass_ms * 1e-3) / 1e9
+    per_turn = {k: v[0] * v[1] / a.steps for k, v in ktimes.items()}
+    out = {
+        "metric": "macroparticle-turns/s on 1 MI355X (plain FP64 path, batched train with ions)",
+        "value": n * a.steps / dt,
+        "unit": "macroparticle-turns/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt / a.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{nb} bunches 4 ns apart, counts uniform in [1e4, 2e5] (seed 3, total {n}), "
+                               "1 nC, CO ions at 1e-9 Pa / 300 K over the circumference, "
+                               f"{T['ions']['n_new']} macro-ions per passage, <= {T['ions']['max_ions']}, "
+                               "Bassetti-Erskine both ways in FP64, excitation on (device Philox)",
+                   "macroparticles": n, "bunches": nb, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_pass (fused turn)",
+                     "kernel_ms": pass_ms, "launches": ktimes["pass"][1],
+                     "other_kernels_ms_per_turn": per_turn},
+        "ion_macroparticles_last_passage": float(cloud[-1, 0, 0]),
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_train(ring, secs, p, a.cpu_seconds)
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -86,6 +213,8 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
                                    "(multi-GPU sharding is a design note only)"}))
         sys.exit(2)
+    if a.workload == "c3":
+        return run_train(a)
     import numpy as np
 
     from cetasim_amd import configs
@@ -122,6 +251,7 @@ def main():
 
     value = n * a.steps / dt
     achieved = BYTES_PER_MPT * n / (pass_ms * 1e-3) / 1e9
+    tpm, tsrc = measured_traffic_per_mpt()
     out = {
         "metric": "macroparticle-turns/s on 1 MI355X (plain FP64 path, single-bunch scale-up)",
         "value": value,
@@ -140,7 +270,8 @@ def main():
                                "(device Philox, seed 2), one-section ring Qx/Qy/Qs=0.18/0.27/0.005",
                    "macroparticles": n, "bins": a.nbins, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None if tpm is None else tpm * n,
+                     "traffic_source": tsrc,
                      "kernel": "k_pass (fused turn)", "kernel_ms": pass_ms, "launches": n_pass,
                      "other_kernels_ms": {"post (moments + wake)": post_ms}},
         "transmission": m["transmission"],

@@ -111,7 +111,7 @@ def bunch_scalars(counts, ring, p=BASE):
 # 10 rms bunch sizes, at most 5e5 macro-ions.
 TRAIN = dict(nb=80, spacing=2, count_range=(10_000, 200_000), seed=3, turns=2000,
              ions=dict(mass_number=[28.0], cross_section=[2.0e-22], pressure=[1.0e-9], temperature=300.0,
-                       n_new=2000, max_ions=500_000, loss_boundary=10.0, seed=3))
+                       n_new=6000, max_ions=500_000, loss_boundary=10.0, seed=3))
 
 
 def train_counts(nb=TRAIN["nb"], lo=TRAIN["count_range"][0], hi=TRAIN["count_range"][1], seed=TRAIN["seed"]):

@@ -517,7 +517,9 @@ void free_ion_state(cts_handle *h)
     h->ion_buf = 0;
 }
 
-constexpr int kIonMaxWG = 2048;
+// passage grid: one ion per thread up to 512 workgroups (two waves per SIMD
+// on all 256 CUs at the passage kernel's register use), grid-stride beyond
+constexpr int kIonMaxWG = 512;
 
 // MPBeam::SSBeamIonEffectOneInteractionPoint at interaction point 0
 // (src/MPBeam.cpp:2585-2611): bunch views from the moments of the present
@@ -572,7 +574,7 @@ int ion_phase(cts_handle *h, int32_t turn)
         P.dt = h->ring.circ / h->ring.harmonic * h->gap[b] / kClight;
         P.bunch = b;
         P.passage = (uint32_t)b;
-        int64_t nwg = (ub + I.n_new + 2 * kIonBlock - 1) / (2 * kIonBlock);
+        int64_t nwg = (ub + I.n_new + kIonBlock - 1) / kIonBlock;
         nwg = std::max<int64_t>(1, std::min<int64_t>(nwg, kIonMaxWG));
         TimedScope ts(h, "ion_passage");
         HIPCHK(launch_ion_passage(P, (int)nwg, h->stream));

@@ -49,7 +49,8 @@ __device__ __forceinline__ void ion_generate(uint64_t seed, uint32_t turn, uint3
 __global__ void __launch_bounds__(kIonBlock)
 k_ion_passage(IonPassParams P)
 {
-    __shared__ double red[kIonBlock / 64][CTS_MAX_SPECIES][kIonPart];
+    constexpr int kNV = CTS_MAX_SPECIES * kIonPart;
+    __shared__ double red[kIonBlock / 64][kNV];
     __shared__ uint32_t last;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int b = P.bunch;
@@ -98,15 +99,15 @@ k_ion_passage(IonPassParams P)
 #pragma unroll
         for (int j = 0; j < kIonPart; j++) {
             const double r = wave_sum(acc[j]);
-            if (lane == 0) red[wave][p][j] = r;
+            if (lane == 0) red[wave][p * kIonPart + j] = r;
         }
     }
     __syncthreads();
-    if (tid < P.nspec * kIonPart) {
-        const int p = tid / kIonPart, j = tid % kIonPart;
-        double r = red[0][p][j];
-        for (int w = 1; w < kIonBlock / 64; w++) r += red[w][p][j];
-        P.part[((int64_t)blockIdx.x * CTS_MAX_SPECIES + p) * kIonPart + j] = r;
+    if (tid < kNV) {
+        double r = 0.0;
+        if (tid < P.nspec * kIonPart)
+            for (int w = 0; w < kIonBlock / 64; w++) r += red[w][tid];
+        P.part[(int64_t)blockIdx.x * kNV + tid] = r;
     }
     __threadfence();
     __syncthreads();
@@ -114,28 +115,46 @@ k_ion_passage(IonPassParams P)
     __syncthreads();
     if (!last) return;
     __threadfence();
-    if (tid < P.nspec * kIonPart) {
-        const int p = tid / kIonPart, j = tid % kIonPart;
-        double r = 0.0;
-        for (uint32_t w = 0; w < gridDim.x; w++) r += P.part[((int64_t)w * CTS_MAX_SPECIES + p) * kIonPart + j];
-        red[0][p][j] = r;
+    // the last workgroup folds the partials: thread t takes workgroups
+    // t, t + 256, ... in order, then a fixed wave/LDS tree (deterministic)
+    {
+        double r[kNV];
+#pragma unroll
+        for (int j = 0; j < kNV; j++) r[j] = 0.0;
+        for (uint32_t w = tid; w < gridDim.x; w += kIonBlock) {
+            const double *src = P.part + (int64_t)w * kNV;
+#pragma unroll
+            for (int j = 0; j < kNV; j++) r[j] += src[j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kNV; j++) {
+            const double s = wave_sum(r[j]);
+            if (lane == 0) red[wave][j] = s;
+        }
+        __syncthreads();
+        double s = 0.0;
+        if (tid < kNV)
+            for (int w = 0; w < kIonBlock / 64; w++) s += red[w][tid];
+        __syncthreads();
+        if (tid < kNV) red[0][tid] = s;
+        __syncthreads();
     }
-    __syncthreads();
     if (tid < P.nspec) {
         const int p = tid;
-        const double n = red[0][p][0];
+        const double n = red[0][p * kIonPart + 0];
         double mx = P.mean_state[2 * p], my = P.mean_state[2 * p + 1], rx = 0, ry = 0;
         if (n >= 2) {
-            const double ax = red[0][p][1] / n, ay = red[0][p][2] / n;
+            const double ax = red[0][p * kIonPart + 1] / n, ay = red[0][p * kIonPart + 2] / n;
             mx = xm + ax;
             my = ym + ay;
-            rx = sqrt(fmax(red[0][p][3] / n - ax * ax, 0.0));
-            ry = sqrt(fmax(red[0][p][4] / n - ay * ay, 0.0));
+            rx = sqrt(fmax(red[0][p * kIonPart + 3] / n - ax * ax, 0.0));
+            ry = sqrt(fmax(red[0][p * kIonPart + 4] / n - ay * ay, 0.0));
         } else if (n == 1) {
             // one ion: uncentred rms, the mean keeps its last value
             // (src/LatticeInterActionPoint.cpp:1125-1129)
-            rx = fabs(xm + red[0][p][1]);
-            ry = fabs(ym + red[0][p][2]);
+            rx = fabs(xm + red[0][p * kIonPart + 1]);
+            ry = fabs(ym + red[0][p * kIonPart + 2]);
         }
         P.mean_state[2 * p] = mx;
         P.mean_state[2 * p + 1] = my;
@@ -152,7 +171,7 @@ k_ion_passage(IonPassParams P)
         c[0] = n; c[1] = mx; c[2] = my; c[3] = rx; c[4] = ry; c[5] = nI0;
         const bool overflow = P.n_alive[p] + P.n_new > P.max_ions;
         if (overflow) atomicOr(P.status, 2u);
-        P.n_alive[p] = (int64_t)red[0][p][5];
+        P.n_alive[p] = (int64_t)red[0][p * kIonPart + 5];
         if (!overflow) P.len[p] += P.n_new;
     }
     if (tid == 0) *P.cnt = 0u;

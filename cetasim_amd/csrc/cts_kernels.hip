@@ -437,12 +437,24 @@ __device__ __forceinline__ void finalize_body(const FinParams &P, int c, int b)
         __threadfence();
         if (lane == 0) P.chunk_cnt[b] = 0u;
     }
-    for (int j = lane; j < kNPart; j += 64) {
-        double a = part_identity(j);
+    {
+        // lane q folds chunks q, q + 64, ...; then the fixed wave tree
         const double *src = P.chunk_part + (int64_t)b * P.max_chunks * kNPart;
         const int nc = nchunk > 0 ? nchunk : 1;
-        for (int q = 0; q < nc; q++) a = part_fold(j, a, src[(int64_t)q * kNPart + j]);
-        tot[j] = a;
+        double v[kNPart];
+#pragma unroll
+        for (int j = 0; j < kNPart; j++) v[j] = part_identity(j);
+        for (int q0 = 0; q0 < nc; q0 += 64) {
+            const int q = q0 + lane;
+#pragma unroll
+            for (int j = 0; j < kNPart; j++)
+                v[j] = part_fold(j, v[j], q < nc ? src[(int64_t)q * kNPart + j] : part_identity(j));
+        }
+#pragma unroll
+        for (int j = 0; j < kNPart; j++) {
+            const double r = part_wave(j, v[j]);
+            if (lane == 0) tot[j] = r;
+        }
     }
     __syncthreads();
     if (lane != 0) return;
