@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--workload", choices=("c2", "c3"), default="c2",
                     help="c2: single bunch 1e7 macroparticles with wake and excitation; "
                          "c3: 80-bunch train with the beam-ion phase")
+    ap.add_argument("--plain-only", action="store_true", help="c3: skip the surrogate run")
+    ap.add_argument("--sur-steps", type=int, default=20000, help="c3: Adam steps of the surrogate training")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reads MIB, writes MIB) three times, "
                          "to scale FETCH_SIZE/WRITE_SIZE counters")
@@ -145,8 +147,27 @@ def cpu_baseline_train(ring, secs, p, budget_s):
                       f"{platform.processor() or platform.machine()})"}
 
 
-def run_train(a):
-    """80-bunch train with the beam-ion phase at one interaction point."""
+def train_surrogate(steps):
+    """The field surrogate trained on the GPU (untimed set-up): 4e6 samples of
+    the FP64 field (seed 4), Adam with a cosine schedule in three stages."""
+    from cetasim_amd.surrogate import Surrogate
+
+    s = Surrogate()
+    t0 = time.perf_counter()
+    s.generate(4_000_000, 1_000_000, 4)
+    s.init_params(4)
+    k = max(steps // 10, 1)
+    s.train(2 * k, nwg=512, chunks=2, lr_max=3e-3, lr_min=1e-3, seed=4)
+    s.train(4 * k, nwg=512, chunks=2, lr_max=1e-3, lr_min=1e-4, seed=5)
+    loss = s.train(4 * k, nwg=512, chunks=2, lr_max=1e-4, lr_min=1e-6, seed=6)
+    mx, rms = s.validate()
+    return s, {"samples": 4_000_000, "validation_points": 1_000_000, "adam_steps": 10 * k,
+               "batch": 512 * 2 * 64, "train_s": time.perf_counter() - t0, "final_loss": loss,
+               "max_rel_field_error": mx, "rms_rel_field_error": rms}
+
+
+def train_turns(a, sur=None, mode=0):
+    """Time a.steps turns of the 80-bunch train; returns (s, kernel times, cloud)."""
     from cetasim_amd import configs
     from cetasim_amd.tracker import Tracker
 
@@ -164,6 +185,8 @@ def run_train(a):
     del parts
     tr.set_ions(configs.ion_config(ring, T["ions"], p))
     tr.set_synrad(True, 2, 2)
+    if sur is not None:
+        tr.set_field_model(sur, mode)
     for t in range(a.warmup):
         tr.turn(t)
     tr.sync()
@@ -177,27 +200,54 @@ def run_train(a):
                                               "ion_kick")}
     tr.set_timing(False)
     cloud = tr.ion_passages()
+    m = tr.moments()
     tr.close()
-    n = int(counts.sum())
+    return dt, ktimes, cloud, m, int(counts.sum()), nb
+
+
+def run_train(a):
+    """80-bunch train with the beam-ion phase at one interaction point: the
+    plain FP64 Faddeeva path, then (unless --plain-only) the surrogate path."""
+    from cetasim_amd import configs
+
+    p = configs.BASE
+    T = configs.TRAIN
+    ring, secs, _ = configs.lattice(p)
+    dt, ktimes, cloud, m_plain, n, nb = train_turns(a)
+    plain = n * a.steps / dt
+    sur_info = None
+    value = plain
+    per_turn = {k: v[0] * v[1] / a.steps for k, v in ktimes.items()}
+    if not a.plain_only:
+        sur, sur_info = train_surrogate(a.sur_steps)
+        dts, kts, cloud_s, m_sur, _, _ = train_turns(a, sur, 3)
+        sur.close()
+        value = n * a.steps / dts
+        sur_info.update({"plain_value": plain, "speedup_vs_plain": value / plain,
+                         "ms_per_step_plain": dt / a.steps * 1e3,
+                         "kernels_ms_per_turn": {k: v[0] * v[1] / a.steps for k, v in kts.items()},
+                         "centroid_x_last_bunch": [m_plain[-1]["mean"][0], m_sur[-1]["mean"][0]]})
+        dt = dts
     pass_ms = ktimes["pass"][0]
     achieved = BYTES_PER_MPT * n / (pass_ms * 1e-3) / 1e9
-    per_turn = {k: v[0] * v[1] / a.steps for k, v in ktimes.items()}
     out = {
-        "metric": "macroparticle-turns/s on 1 MI355X (plain FP64 path, batched train with ions)",
-        "value": n * a.steps / dt,
+        "metric": "macroparticle-turns/s on 1 MI355X: surrogate vs plain kick (batched train with ions)",
+        "value": value,
         "unit": "macroparticle-turns/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt / a.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{nb} bunches 4 ns apart, counts uniform in [1e4, 2e5] (seed 3, total {n}), "
                                "1 nC, CO ions at 1e-9 Pa / 300 K over the circumference, "
                                f"{T['ions']['n_new']} macro-ions per passage, <= {T['ions']['max_ions']}, "
-                               "Bassetti-Erskine both ways in FP64, excitation on (device Philox)",
+                               "Bassetti-Erskine both ways (FP64 Faddeeva; with the surrogate inside its "
+                               "domain when value is the surrogate rate), excitation on (device Philox)",
                    "macroparticles": n, "bunches": nb, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_pass (fused turn)",
                      "kernel_ms": pass_ms, "launches": ktimes["pass"][1],
                      "other_kernels_ms_per_turn": per_turn},
         "ion_macroparticles_last_passage": float(cloud[-1, 0, 0]),
+        "surrogate": sur_info,
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_train(ring, secs, p, a.cpu_seconds)

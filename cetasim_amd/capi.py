@@ -19,6 +19,7 @@ U64 = ctypes.c_uint64
 PD = ctypes.POINTER(ctypes.c_double)
 PI32 = ctypes.POINTER(ctypes.c_int32)
 PI64 = ctypes.POINTER(ctypes.c_int64)
+PF = ctypes.POINTER(ctypes.c_float)
 
 CTS_OK = 0
 CTS_LOW_TRANSMISSION = 1
@@ -112,6 +113,19 @@ SIGNATURES = {
     "cts_set_timing": (I32, [ctypes.c_void_p, I32]),
     "cts_kernel_time": (I32, [ctypes.c_void_p, ctypes.c_char_p, PD, PI64]),
     "cts_debug_qe_normals": (I32, [ctypes.c_void_p, U64, I32, U64, I64, PD]),
+    "cts_sur_create": (I32, [ctypes.POINTER(ctypes.c_void_p)]),
+    "cts_sur_destroy": (I32, [ctypes.c_void_p]),
+    "cts_sur_last_error": (ctypes.c_char_p, []),
+    "cts_sur_init_params": (I32, [ctypes.c_void_p, U64]),
+    "cts_sur_set_params": (I32, [ctypes.c_void_p, PF, PF]),
+    "cts_sur_get_params": (I32, [ctypes.c_void_p, PF, PF]),
+    "cts_sur_generate": (I32, [ctypes.c_void_p, I64, I64, U64]),
+    "cts_sur_get_samples": (I32, [ctypes.c_void_p, I64, I64, PF, PD]),
+    "cts_sur_train": (I32, [ctypes.c_void_p, I32, I32, I32, ctypes.c_float, ctypes.c_float, U64, PF]),
+    "cts_sur_gradient": (I32, [ctypes.c_void_p, PF, I64, PF, PF]),
+    "cts_sur_eval": (I32, [ctypes.c_void_p, PF, I64, PF]),
+    "cts_sur_validate": (I32, [ctypes.c_void_p, PD, PD, PD]),
+    "cts_set_field_model": (I32, [ctypes.c_void_p, ctypes.c_void_p, I32]),
     "cts_build_section": (I32, [ctypes.POINTER(Twiss), ctypes.POINTER(Twiss), D, D, ctypes.POINTER(Section)]),
     "cts_derive_ring": (I32, [ctypes.POINTER(RingInput), ctypes.POINTER(Ring), ctypes.POINTER(RingDerived)]),
 }
@@ -146,6 +160,17 @@ def check(rc):
     if rc != CTS_OK:
         raise CtsError(rc, load().cts_last_error().decode())
     return rc
+
+
+def check_sur(rc):
+    if rc != CTS_OK:
+        raise CtsError(rc, load().cts_sur_last_error().decode())
+    return rc
+
+
+def fptr(a):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(PF)
 
 
 def dptr(a):

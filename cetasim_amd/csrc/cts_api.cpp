@@ -140,6 +140,11 @@ struct cts_handle {
     uint32_t *d_ion_cnt = nullptr, *d_ion_chunk = nullptr;
     // d_cur holds the moments of the present coordinates
     bool cur_valid = false;
+    // field model of the beam-ion kicks: surrogate parameters (null: exact)
+    const cts_surrogate *sur = nullptr;
+    int32_t sur_mode = 0;  // bit 0: ions kicked by bunches, bit 1: electrons kicked by ions
+    uint32_t *d_fb_count = nullptr;  // exact-field fallback queue of the electron-side kick
+    uint64_t *d_fb_list = nullptr;
 
     // timing
     bool timing = false;
@@ -158,7 +163,8 @@ void free_bunch_state(cts_handle *h)
                     h->d_seg_bunch, h->d_wg_seg, h->d_bunch_seg, h->d_seg_part, h->d_shift,
                     h->d_cur, h->d_hist, h->d_sigz, h->d_zrange, h->d_noise, h->d_ion_e,
                     h->d_bhist, h->d_counts, h->d_kick, h->d_poten, h->d_edges_prev,
-                    h->d_edges_now, h->d_table, h->d_scale, h->d_chunk_part, h->d_chunk_cnt};
+                    h->d_edges_now, h->d_table, h->d_scale, h->d_chunk_part, h->d_chunk_cnt,
+                    h->d_fb_count, h->d_fb_list};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     h->d_x = h->d_px = h->d_y = h->d_py = h->d_z = h->d_dp = nullptr;
@@ -172,6 +178,8 @@ void free_bunch_state(cts_handle *h)
     h->d_kick = h->d_poten = h->d_edges_prev = h->d_edges_now = h->d_table = h->d_scale = nullptr;
     h->d_chunk_part = nullptr;
     h->d_chunk_cnt = nullptr;
+    h->d_fb_count = nullptr;
+    h->d_fb_list = nullptr;
 }
 
 template <typename T>
@@ -533,7 +541,8 @@ int ion_phase(cts_handle *h, int32_t turn)
     const cts_section &S = h->secs[0];
     {
         TimedScope ts(h, "ion_prep");
-        HIPCHK(launch_ion_prep(h->d_cur, h->nb, S.beta_x, S.beta_y, h->d_ion_bview, h->d_ion_smax, h->stream));
+        HIPCHK(launch_ion_prep(h->d_cur, h->nb, S.beta_x, S.beta_y, h->d_ion_bview, h->d_ion_smax, h->d_fb_count,
+                               h->stream));
     }
     const int buf = h->ion_buf;
     IonPassParams P;
@@ -557,6 +566,8 @@ int ion_phase(cts_handle *h, int32_t turn)
     P.cnt = h->d_ion_cnt;
     P.status = h->d_status;
     P.gamma = h->ring.rgamma;
+    P.sur_params = nullptr;
+    if (h->sur && (h->sur_mode & 1)) sur_export(h->sur, &P.sur_params, &P.sur_norm);
     P.loss_boundary = I.loss_boundary;
     P.ap_x = I.ap_x;
     P.ap_y = I.ap_y;
@@ -604,6 +615,12 @@ int ion_phase(cts_handle *h, int32_t turn)
     ik.seg_bunch = h->d_seg_bunch; ik.wg_seg = h->d_wg_seg;
     ik.ion_e = h->d_ion_e;
     ik.nspec = h->nspec;
+    ik.sur_params = nullptr;
+    if (h->sur && (h->sur_mode & 2)) sur_export(h->sur, &ik.sur_params, &ik.sur_norm);
+    ik.fb_count = h->d_fb_count;
+    ik.fb_list = h->d_fb_list;
+    ik.bunch_off = h->d_bunch_off;
+    ik.nb = h->nb;
     TimedScope ts(h, "ion_kick");
     HIPCHK(launch_ion_kick(ik, h->nwg, h->stream));
     h->cur_valid = false;
@@ -788,6 +805,9 @@ int cts_set_bunches(cts_handle *h, int32_t nb, const int64_t *counts, const doub
         rc = cts_set_wake(h, nbins, h->wake_mode, h->grid_lo, h->grid_hi, (int32_t)rs.size(), rs.data(),
                           q.data(), fr.data());
         if (rc) return rc;
+    }
+    if (h->sur_mode & 2) {
+        if ((rc = dalloc(&h->d_fb_count, 1)) || (rc = dalloc(&h->d_fb_list, (size_t)h->ntot))) return rc;
     }
     if (ions_on) return cts_set_ions(h, &ions);
     return CTS_OK;
@@ -1103,6 +1123,20 @@ int cts_set_ions(cts_handle *h, const cts_ion_config *cfg)
     HIPCHK(hipMemcpy(h->d_ion_q, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(h->d_ion_e, 0, (size_t)h->nb * CTS_MAX_SPECIES * 5 * sizeof(double)));
     h->nspec = cfg->nspec;
+    return CTS_OK;
+}
+
+int cts_set_field_model(cts_handle *h, const cts_surrogate *s, int32_t mode)
+{
+    if (!h) return fail(CTS_ERR_ARG, "null handle");
+    if (mode < 0 || mode > 3) return fail(CTS_ERR_ARG, "cts_set_field_model: mode 0..3");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->sur = mode ? s : nullptr;
+    h->sur_mode = s ? mode : 0;
+    if ((h->sur_mode & 2) && !h->d_fb_list) {
+        int rc;
+        if ((rc = dalloc(&h->d_fb_count, 1)) || (rc = dalloc(&h->d_fb_list, (size_t)h->ntot))) return rc;
+    }
     return CTS_OK;
 }
 

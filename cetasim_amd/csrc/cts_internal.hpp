@@ -7,6 +7,7 @@
 #include "../../include/cetasim_hip.h"
 #include "cts_device.hpp"
 #include "cts_faddeeva.hpp"
+#include "cts_mlp.hpp"
 
 namespace cts {
 
@@ -80,6 +81,14 @@ struct IonKickParams {
     const int64_t *seg_begin, *seg_end;
     const int32_t *seg_bunch, *wg_seg;
     const double *ion_e;  // per bunch CTS_MAX_SPECIES x {coeffE, xbar, ybar, sx, sy}
+    const float *sur_params;  // field surrogate (null: exact field)
+    SurNorm sur_norm;
+    // exact-field fallback list of the surrogate variant: entries
+    // (particle << 4) | species mask; fb_count is zeroed before each phase
+    uint32_t *fb_count;
+    uint64_t *fb_list;
+    const int64_t *bunch_off;
+    int32_t nb;
     int32_t nspec;
 };
 
@@ -99,6 +108,8 @@ struct IonPassParams {
     uint32_t *cnt;           // last-workgroup counter, left at zero
     uint32_t *status;
     double coeffI[CTS_MAX_SPECIES];
+    const float *sur_params;  // field surrogate (null: exact field)
+    SurNorm sur_norm;
     double dt, gamma, loss_boundary, ap_x, ap_y;
     int64_t max_ions;
     uint64_t seed;
@@ -171,7 +182,7 @@ hipError_t launch_post(const FinParams &f, const WakeParams &w, hipStream_t st);
 hipError_t launch_wake_table(const WakeTableParams &p, int ntables, hipStream_t st);
 hipError_t launch_wake(const WakeParams &p, int nb, hipStream_t st);
 hipError_t launch_ion_prep(const cts_moments *cur, int nb, double beta_x, double beta_y, double *bview,
-                           double *size_max, hipStream_t st);
+                           double *size_max, uint32_t *fb_count, hipStream_t st);
 hipError_t launch_ion_passage(const IonPassParams &p, int nwg, hipStream_t st);
 hipError_t launch_ion_compact(const IonCompactParams &p, int64_t len_bound, hipStream_t st);
 constexpr int kIonBlock = 256;
@@ -180,5 +191,6 @@ hipError_t launch_qe_debug(uint64_t seed, uint32_t turn, uint64_t base, int64_t 
                            hipStream_t st);
 hipError_t upload_faddeeva_tables(const double ea2n2[64], const double fad[3]);
 hipError_t upload_faddeeva_tables_ions(const double ea2n2[64], const double fad[3]);
+void sur_export(const cts_surrogate *s, const float **params, SurNorm *norm);
 
 }  // namespace cts

@@ -46,12 +46,20 @@ __device__ __forceinline__ void ion_generate(uint64_t seed, uint32_t turn, uint3
     gy = ty;
 }
 
+// SUR: the field of the bunch at each ion from the surrogate inside its
+// domain (exact field outside it).
+template <bool SUR>
 __global__ void __launch_bounds__(kIonBlock)
 k_ion_passage(IonPassParams P)
 {
     constexpr int kNV = CTS_MAX_SPECIES * kIonPart;
     __shared__ double red[kIonBlock / 64][kNV];
     __shared__ uint32_t last;
+    __shared__ __attribute__((aligned(16))) MlpLds L;
+    if (SUR) {
+        load_mlp_lds(L, P.sur_params, threadIdx.x, kIonBlock);
+        __syncthreads();
+    }
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int b = P.bunch;
     const double xm = P.bview[4 * b + 0], ym = P.bview[4 * b + 1];
@@ -69,24 +77,35 @@ k_ion_passage(IonPassParams P)
         double acc[kIonPart] = {0, 0, 0, 0, 0, 0};
         double *X = P.x[p], *Y = P.y[p], *VX = P.vx[p], *VY = P.vy[p];
         uint8_t *A = P.alive[p];
-        for (int64_t i = (int64_t)blockIdx.x * kIonBlock + tid; i < end; i += (int64_t)gridDim.x * kIonBlock) {
-            double x, y, vx, vy;
-            if (i >= len) {
-                ion_generate(P.seed, P.turn, P.passage, p, (uint32_t)(i - len), xm, ym, sx, sy, x, y);
-                vx = 0.0;
-                vy = 0.0;
-            } else {
-                if (!A[i]) continue;
-                x = X[i]; y = Y[i]; vx = VX[i]; vy = VY[i];
+        // whole blocks per iteration, so that every lane of a wave reaches
+        // the surrogate together
+        for (int64_t base = (int64_t)blockIdx.x * kIonBlock; base < end; base += (int64_t)gridDim.x * kIonBlock) {
+            const int64_t i = base + tid;
+            bool act = i < end;
+            double x = 0.0, y = 0.0, vx = 0.0, vy = 0.0;
+            if (act) {
+                if (i >= len) {
+                    ion_generate(P.seed, P.turn, P.passage, p, (uint32_t)(i - len), xm, ym, sx, sy, x, y);
+                } else if (A[i]) {
+                    x = X[i]; y = Y[i]; vx = VX[i]; vy = VY[i];
+                } else {
+                    act = false;
+                }
             }
             const double dx = x - xm, dy = y - ym;
+            double fx = 0.0, fy = 0.0;
+            if (SUR) {
+                const bool ok = sur_field(L, P.sur_norm, act, dx, dy, sx, sy, fx, fy);
+                if (act && !ok) ion_field(dx, dy, sx, sy, fx, fy);
+            } else if (act) {
+                ion_field(dx, dy, sx, sy, fx, fy);
+            }
+            if (!act) continue;
             acc[0] += 1.0;
             acc[1] += dx;
             acc[2] += dy;
             acc[3] += dx * dx;
             acc[4] += dy * dy;
-            double fx, fy;
-            ion_field(dx, dy, sx, sy, fx, fy);
             vx += cI * fx;
             vy += cI * fy;
             x += vx * P.dt;
@@ -182,8 +201,10 @@ k_ion_passage(IonPassParams P)
 // src/MPBunch.cpp:481-482), and their maxima (MPGetBeamInfo,
 // src/MPBeam.cpp:2203-2258).
 __global__ void __launch_bounds__(256)
-k_ion_prep(const cts_moments *cur, int nb, double beta_x, double beta_y, double *bview, double *size_max)
+k_ion_prep(const cts_moments *cur, int nb, double beta_x, double beta_y, double *bview, double *size_max,
+           uint32_t *fb_count)
 {
+    if (threadIdx.x == 0 && fb_count) *fb_count = 0u;
     __shared__ double mx[256], my[256];
     double ax = -INFINITY, ay = -INFINITY;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
@@ -279,14 +300,17 @@ hipError_t upload_faddeeva_tables_ions(const double ea2n2[64], const double fad[
 
 hipError_t launch_ion_passage(const IonPassParams &p, int nwg, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_ion_passage, dim3(nwg), dim3(kIonBlock), 0, st, p);
+    if (p.sur_params)
+        hipLaunchKernelGGL(k_ion_passage<true>, dim3(nwg), dim3(kIonBlock), 0, st, p);
+    else
+        hipLaunchKernelGGL(k_ion_passage<false>, dim3(nwg), dim3(kIonBlock), 0, st, p);
     return hipGetLastError();
 }
 
 hipError_t launch_ion_prep(const cts_moments *cur, int nb, double bx, double by, double *bview, double *size_max,
-                           hipStream_t st)
+                           uint32_t *fb_count, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_ion_prep, dim3(1), dim3(256), 0, st, cur, nb, bx, by, bview, size_max);
+    hipLaunchKernelGGL(k_ion_prep, dim3(1), dim3(256), 0, st, cur, nb, bx, by, bview, size_max, fb_count);
     return hipGetLastError();
 }
 

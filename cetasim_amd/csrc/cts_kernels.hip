@@ -642,6 +642,91 @@ k_ion_kick(IonKickParams P)
     }
 }
 
+// The same kick with the field surrogate inside its domain and the exact
+// field elsewhere.  The loop runs in whole blocks so that every lane of a
+// wave reaches the network together.
+__global__ void __launch_bounds__(kBlock)
+k_ion_kick_sur(IonKickParams P)
+{
+    __shared__ __attribute__((aligned(16))) MlpLds L;
+    const int tid = threadIdx.x;
+    load_mlp_lds(L, P.sur_params, tid, kBlock);
+    __syncthreads();
+    const int s_first = P.wg_seg[blockIdx.x], s_last = P.wg_seg[blockIdx.x + 1];
+    for (int s = s_first; s < s_last; s++) {
+        const int b = P.seg_bunch[s];
+        const double *c = P.ion_e + (int64_t)b * CTS_MAX_SPECIES * 5;
+        for (int64_t base = P.seg_begin[s]; base < P.seg_end[s]; base += kBlock) {
+            const int64_t i = base + tid;
+            const bool active = i < P.seg_end[s] && (P.flag[i] & 3u) == 0;
+            const double x = active ? P.x[i] : 0.0, y = active ? P.y[i] : 0.0;
+            double fxs = 0.0, fys = 0.0;
+            uint32_t mask = 0;
+            for (int p = 0; p < P.nspec; p++) {
+                double fx = 0.0, fy = 0.0;
+                const double dx = x - c[5 * p + 1], dy = y - c[5 * p + 2];
+                const bool ok = sur_field(L, P.sur_norm, active, dx, dy, c[5 * p + 3], c[5 * p + 4], fx, fy);
+                if (ok) {
+                    fxs = fxs + c[5 * p] * fx;
+                    fys = fys + c[5 * p] * fy;
+                } else {
+                    mask |= 1u << p;
+                }
+            }
+            if (active) {
+                P.px[i] += fxs;
+                P.py[i] += fys;
+            }
+            // points outside the surrogate's domain: queued for the exact
+            // kernel, one counter update per wave
+            const bool q = active && mask != 0u;
+            const uint64_t bal = __ballot(q);
+            if (bal) {
+                const int lane = tid & 63;
+                const uint32_t nq = (uint32_t)__popcll(bal);
+                uint32_t first = 0;
+                const int leader = __ffsll((long long)bal) - 1;
+                if (lane == leader) first = atomicAdd(P.fb_count, nq);
+                first = __shfl(first, leader, 64);
+                if (q) {
+                    const uint32_t pos = first + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+                    P.fb_list[pos] = ((uint64_t)i << 4) | mask;
+                }
+            }
+        }
+    }
+}
+
+// Exact kicks of the queued points (the species in each entry's mask).
+__global__ void __launch_bounds__(kBlock)
+k_ion_kick_fb(IonKickParams P)
+{
+    const uint32_t n = *P.fb_count;
+    for (uint32_t e = blockIdx.x * kBlock + threadIdx.x; e < n; e += gridDim.x * kBlock) {
+        const uint64_t ent = P.fb_list[e];
+        const int64_t i = (int64_t)(ent >> 4);
+        const uint32_t mask = (uint32_t)(ent & 15u);
+        int lo = 0, hi = P.nb;  // bunch of particle i: bunch_off[b] <= i < bunch_off[b + 1]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (P.bunch_off[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        const double *c = P.ion_e + (int64_t)lo * CTS_MAX_SPECIES * 5;
+        const double x = P.x[i], y = P.y[i];
+        double fxs = 0.0, fys = 0.0;
+        for (int p = 0; p < P.nspec; p++) {
+            if (!(mask & (1u << p))) continue;
+            double fx, fy;
+            ion_field(x - c[5 * p + 1], y - c[5 * p + 2], c[5 * p + 3], c[5 * p + 4], fx, fy);
+            fxs = fxs + c[5 * p] * fx;
+            fys = fys + c[5 * p] * fy;
+        }
+        P.px[i] += fxs;
+        P.py[i] += fys;
+    }
+}
+
 // -------------------------------------------------------- test hooks
 __global__ void k_qe_debug(uint64_t seed, uint32_t turn, uint64_t base, int64_t n, double *out)
 {
@@ -689,6 +774,11 @@ hipError_t launch_pass(const PassParams &p, int nwg, hipStream_t st)
 }
 hipError_t launch_ion_kick(const IonKickParams &p, int nwg, hipStream_t st)
 {
+    if (p.sur_params) {
+        hipLaunchKernelGGL(k_ion_kick_sur, dim3(nwg), dim3(kBlock), 0, st, p);
+        hipLaunchKernelGGL(k_ion_kick_fb, dim3(512), dim3(kBlock), 0, st, p);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_ion_kick, dim3(nwg), dim3(kBlock), 0, st, p);
     return hipGetLastError();
 }

@@ -80,6 +80,11 @@ class Tracker:
         capi.check(self.lib.cts_set_ions(self.h, ctypes.byref(cfg)))
         self.nspec = cfg.nspec
 
+    def set_field_model(self, sur, mode):
+        """sur: surrogate.Surrogate or None; mode bit 0 ion side, bit 1 electron side."""
+        capi.check(self.lib.cts_set_field_model(self.h, sur.h if sur is not None else None, mode))
+        self._sur = sur
+
     def ion_phase(self, turn):
         return capi.check(self.lib.cts_ion_phase(self.h, turn))
 

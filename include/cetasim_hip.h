@@ -238,6 +238,47 @@ int cts_kernel_time(cts_handle *h, const char *name, double *avg_ms, int64_t *la
 int cts_debug_qe_normals(cts_handle *h, uint64_t seed, int32_t turn, uint64_t base,
                          int64_t n, double *noise_out);
 
+/* ---------------------------------------------------- field surrogate
+ * No counterpart in the reference, which evaluates every Bassetti-Erskine
+ * kick through the Faddeeva function (src/Bunch.cpp:119-211).  A 3-64-64-2
+ * MLP (GELU, tanh form) approximates the normalized field G = F sigma_x of
+ * a Gaussian beam on |x/sigma_x|, |y/sigma_y| <= 6, sigma_y/sigma_x in
+ * [0.02, 1] (axes ordered so that sigma_x >= sigma_y); it is trained on the
+ * GPU from samples of the FP64 field by the library's own bf16-MFMA forward
+ * and backward kernels.  Parameters: 4546 FP32 values (W1[64][3], b1[64],
+ * W2[64][64], b2[64], W3[2][64], b3[2]) and norm[4] = target mean / scale. */
+typedef struct cts_surrogate cts_surrogate;
+int cts_sur_create(cts_surrogate **out);
+int cts_sur_destroy(cts_surrogate *s);
+const char *cts_sur_last_error(void);
+/* deterministic uniform(-1/sqrt(fan_in), +) weights, zero biases */
+int cts_sur_init_params(cts_surrogate *s, uint64_t seed);
+int cts_sur_set_params(cts_surrogate *s, const float *params, const float *norm);
+int cts_sur_get_params(cts_surrogate *s, float *params, float *norm);
+/* n_train training records (a0 a1 a2 t0 t1, FP32) and n_val validation
+ * points (|u| |v| r Gx Gy, FP64) from the device FP64 field, Philox
+ * stream keyed by seed; sets norm from the training targets. */
+int cts_sur_generate(cts_surrogate *s, int64_t n_train, int64_t n_val, uint64_t seed);
+int cts_sur_get_samples(cts_surrogate *s, int64_t first, int64_t n, float *train5, double *val5);
+/* Adam steps on random minibatches of nwg*chunks*64 samples, learning rate
+ * cosine from lr_max to lr_min; *loss_out = last step's mean loss. */
+int cts_sur_train(cts_surrogate *s, int32_t steps, int32_t nwg, int32_t chunks, float lr_max, float lr_min,
+                  uint64_t seed, float *loss_out);
+/* Gradient (4546) and mean loss of the normalized-target MSE over the given
+ * records (samples5[n][5] in the cts_sur_generate format). */
+int cts_sur_gradient(cts_surrogate *s, const float *samples5, int64_t n, float *grad, float *loss);
+/* Raw network outputs out2[n][2] for features3[n][3]. */
+int cts_sur_eval(cts_surrogate *s, const float *features3, int64_t n, float *out2);
+/* Field error on the validation set: per point |G_sur - G| / |G|;
+ * err_out[n_val] may be NULL. */
+int cts_sur_validate(cts_surrogate *s, double *max_err, double *rms_err, double *err_out);
+/* Field model of the beam-ion kicks of a tracking handle: the surrogate
+ * inside its domain, the exact FP64 field outside it.  mode bit 0: ions
+ * kicked by the bunches (MPBunch::SSIonBunchInteraction, src/MPBunch.cpp:
+ * 1113-1160), bit 1: electrons kicked by the ion cloud (1165-1206); mode 0
+ * or s == NULL: exact field everywhere.  s must outlive its use. */
+int cts_set_field_model(cts_handle *h, const cts_surrogate *s, int32_t mode);
+
 #ifdef __cplusplus
 }
 #endif
