@@ -126,6 +126,10 @@ SIGNATURES = {
     "cts_sur_eval": (I32, [ctypes.c_void_p, PF, I64, PF]),
     "cts_sur_validate": (I32, [ctypes.c_void_p, PD, PD, PD]),
     "cts_set_field_model": (I32, [ctypes.c_void_p, ctypes.c_void_p, I32]),
+    "cts_cbm_growth_from_moments": (I32, [ctypes.POINTER(Moments), I32, I32, PI32, I32, PD, PD, I32, I32, I32, D,
+                                          PD, PD]),
+    "cts_fit_wlinear": (I32, [PD, PD, PD, I32, PD, PD]),
+    "cts_growth_rates": (I32, [ctypes.c_void_p, I32, I32, I32, I32, I32, D, D, PD]),
     "cts_build_section": (I32, [ctypes.POINTER(Twiss), ctypes.POINTER(Twiss), D, D, ctypes.POINTER(Section)]),
     "cts_derive_ring": (I32, [ctypes.POINTER(RingInput), ctypes.POINTER(Ring), ctypes.POINTER(RingDerived)]),
 }

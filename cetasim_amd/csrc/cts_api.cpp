@@ -136,8 +136,10 @@ struct cts_handle {
     int64_t *d_ion_nalive = nullptr;  // [CTS_MAX_SPECIES]
     double *d_ion_mean = nullptr;     // [2 * CTS_MAX_SPECIES]
     double *d_ion_bview = nullptr, *d_ion_smax = nullptr, *d_ion_q = nullptr;
-    double *d_ion_cloud = nullptr, *d_ion_part = nullptr;
-    uint32_t *d_ion_cnt = nullptr, *d_ion_chunk = nullptr;
+    double *d_ion_cloud = nullptr;
+    double *d_ion_coeff = nullptr, *d_ion_dt = nullptr;  // [nb][CTS_MAX_SPECIES], [nb]
+    unsigned long long *d_ion_acc = nullptr;             // [nspec][nb][kIonAcc][2]
+    uint32_t *d_ion_chunk = nullptr;
     // d_cur holds the moments of the present coordinates
     bool cur_valid = false;
     // field model of the beam-ion kicks: surrogate parameters (null: exact)
@@ -513,21 +515,20 @@ void free_ion_state(cts_handle *h)
             h->d_ion_alive[b][p] = nullptr;
         }
     void *ptrs[] = {h->d_ion_len, h->d_ion_nalive, h->d_ion_mean, h->d_ion_bview, h->d_ion_smax,
-                    h->d_ion_q, h->d_ion_cloud, h->d_ion_part, h->d_ion_cnt, h->d_ion_chunk};
+                    h->d_ion_q, h->d_ion_cloud, h->d_ion_coeff, h->d_ion_dt, h->d_ion_acc, h->d_ion_chunk};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     h->d_ion_len = h->d_ion_nalive = nullptr;
-    h->d_ion_mean = h->d_ion_bview = h->d_ion_smax = h->d_ion_q = h->d_ion_cloud = h->d_ion_part = nullptr;
-    h->d_ion_cnt = h->d_ion_chunk = nullptr;
+    h->d_ion_mean = h->d_ion_bview = h->d_ion_smax = h->d_ion_q = h->d_ion_cloud = nullptr;
+    h->d_ion_coeff = h->d_ion_dt = nullptr;
+    h->d_ion_acc = nullptr;
+    h->d_ion_chunk = nullptr;
     h->nspec = 0;
     h->ion_cap = 0;
     h->ion_len_ub = 0;
     h->ion_buf = 0;
 }
 
-// passage grid: one ion per thread up to 512 workgroups (two waves per SIMD
-// on all 256 CUs at the passage kernel's register use), grid-stride beyond
-constexpr int kIonMaxWG = 512;
 
 // MPBeam::SSBeamIonEffectOneInteractionPoint at interaction point 0
 // (src/MPBeam.cpp:2585-2611): bunch views from the moments of the present
@@ -539,13 +540,14 @@ int ion_phase(cts_handle *h, int32_t turn)
     if (!h->cur_valid && (rc = enqueue_moments(h))) return rc;
     const cts_ion_config &I = h->ion;
     const cts_section &S = h->secs[0];
+    const int64_t nacc = (int64_t)h->nspec * h->nb * kIonAcc * 2;
     {
         TimedScope ts(h, "ion_prep");
         HIPCHK(launch_ion_prep(h->d_cur, h->nb, S.beta_x, S.beta_y, h->d_ion_bview, h->d_ion_smax, h->d_fb_count,
-                               h->stream));
+                               h->d_ion_acc, nacc, h->stream));
     }
     const int buf = h->ion_buf;
-    IonPassParams P;
+    IonTurnParams P;
     std::memset(&P, 0, sizeof P);
     for (int p = 0; p < h->nspec; p++) {
         P.x[p] = h->d_ion[buf][p][0];
@@ -555,42 +557,50 @@ int ion_phase(cts_handle *h, int32_t turn)
         P.alive[p] = h->d_ion_alive[buf][p];
     }
     P.len = h->d_ion_len + buf * CTS_MAX_SPECIES;
-    P.n_alive = h->d_ion_nalive;
-    P.mean_state = h->d_ion_mean;
     P.bview = h->d_ion_bview;
     P.size_max = h->d_ion_smax;
-    P.ion_q = h->d_ion_q;
-    P.ion_e = h->d_ion_e;
-    P.cloud = h->d_ion_cloud;
-    P.part = h->d_ion_part;
-    P.cnt = h->d_ion_cnt;
-    P.status = h->d_status;
-    P.gamma = h->ring.rgamma;
+    P.coeffI = h->d_ion_coeff;
+    P.dt = h->d_ion_dt;
+    P.acc = h->d_ion_acc;
     P.sur_params = nullptr;
     if (h->sur && (h->sur_mode & 1)) sur_export(h->sur, &P.sur_params, &P.sur_norm);
     P.loss_boundary = I.loss_boundary;
     P.ap_x = I.ap_x;
     P.ap_y = I.ap_y;
-    P.max_ions = I.max_ions;
+    P.cap = h->ion_cap;
     P.seed = I.seed;
-    P.nspec = h->nspec;
+    P.nb = h->nb;
     P.n_new = I.n_new;
+    // waves: the ions present at the start of the turn (the host's bound of
+    // their count), then the new ions of every passage
+    P.w_old = (int32_t)((h->ion_len_ub + 63) / 64);
+    P.w_new = (I.n_new + 63) / 64;
     P.turn = (uint32_t)turn;
-    int64_t ub = h->ion_len_ub;
-    for (int b = 0; b < h->nb; b++) {
-        // src/MPBunch.cpp:1117: 2 N_e r_e (m_e / m_u) / A c
-        for (int p = 0; p < h->nspec; p++)
-            P.coeffI[p] = 2.0 * h->n_e[b] * kRe * kElectronMassEV / kIonMassEV / I.mass_number[p] * kClight;
-        // src/LatticeInterActionPoint.cpp:1169-1170: C / h * gap / c
-        P.dt = h->ring.circ / h->ring.harmonic * h->gap[b] / kClight;
-        P.bunch = b;
-        P.passage = (uint32_t)b;
-        int64_t nwg = (ub + I.n_new + kIonBlock - 1) / kIonBlock;
-        nwg = std::max<int64_t>(1, std::min<int64_t>(nwg, kIonMaxWG));
-        TimedScope ts(h, "ion_passage");
-        HIPCHK(launch_ion_passage(P, (int)nwg, h->stream));
-        ub += I.n_new;
+    {
+        TimedScope ts(h, "ion_turn");
+        HIPCHK(launch_ion_turn(P, h->nspec, h->stream));
     }
+    IonFinishParams F;
+    F.acc = h->d_ion_acc;
+    F.bview = h->d_ion_bview;
+    F.ion_q = h->d_ion_q;
+    F.len = h->d_ion_len + buf * CTS_MAX_SPECIES;
+    F.n_alive = h->d_ion_nalive;
+    F.mean_state = h->d_ion_mean;
+    F.ion_e = h->d_ion_e;
+    F.cloud = h->d_ion_cloud;
+    F.status = h->d_status;
+    F.gamma = h->ring.rgamma;
+    F.max_ions = I.max_ions;
+    F.cap = h->ion_cap;
+    F.nb = h->nb;
+    F.nspec = h->nspec;
+    F.n_new = I.n_new;
+    {
+        TimedScope ts(h, "ion_finish");
+        HIPCHK(launch_ion_finish(F, h->stream));
+    }
+    const int64_t ub = std::min<int64_t>(h->ion_len_ub + (int64_t)h->nb * I.n_new, h->ion_cap);
     // stable compaction of every species into the other buffer
     for (int p = 0; p < h->nspec; p++) {
         IonCompactParams c;
@@ -604,10 +614,10 @@ int ion_phase(cts_handle *h, int32_t turn)
         c.new_len = h->d_ion_len + (1 - buf) * CTS_MAX_SPECIES + p;
         c.chunk_cnt = h->d_ion_chunk;
         TimedScope ts(h, "ion_compact");
-        HIPCHK(launch_ion_compact(c, std::min<int64_t>(ub, h->ion_cap), h->stream));
+        HIPCHK(launch_ion_compact(c, ub, h->stream));
     }
     h->ion_buf = 1 - buf;
-    h->ion_len_ub = std::min<int64_t>(ub, I.max_ions);
+    h->ion_len_ub = ub;
     // electron side (src/MPBunch.cpp:1165-1206, src/Bunch.cpp:213-220)
     IonKickParams ik;
     ik.x = h->d_x; ik.y = h->d_y; ik.px = h->d_px; ik.py = h->d_py; ik.flag = h->d_flag;
@@ -1108,9 +1118,22 @@ int cts_set_ions(cts_handle *h, const cts_ion_config *cfg)
         (rc = dalloc(&h->d_ion_mean, 2 * CTS_MAX_SPECIES)) || (rc = dalloc(&h->d_ion_bview, (size_t)h->nb * 4)) ||
         (rc = dalloc(&h->d_ion_smax, 2)) || (rc = dalloc(&h->d_ion_q, (size_t)h->nb * cfg->nspec)) ||
         (rc = dalloc(&h->d_ion_cloud, (size_t)h->nb * CTS_MAX_SPECIES * 6)) ||
-        (rc = dalloc(&h->d_ion_part, (size_t)kIonMaxWG * CTS_MAX_SPECIES * 6)) || (rc = dalloc(&h->d_ion_cnt, 1)) ||
+        (rc = dalloc(&h->d_ion_coeff, (size_t)h->nb * CTS_MAX_SPECIES)) || (rc = dalloc(&h->d_ion_dt, (size_t)h->nb)) ||
+        (rc = dalloc(&h->d_ion_acc, (size_t)cfg->nspec * h->nb * kIonAcc * 2)) ||
         (rc = dalloc(&h->d_ion_chunk, (size_t)nchunk)))
         return rc;
+    // per bunch: ion kick coefficient 2 N_e r_e (m_e / m_u) / A c
+    // (src/MPBunch.cpp:1117) and drift time C / h * gap / c
+    // (src/LatticeInterActionPoint.cpp:1169-1170)
+    std::vector<double> coeff((size_t)h->nb * CTS_MAX_SPECIES, 0.0), dtv((size_t)h->nb);
+    for (int b = 0; b < h->nb; b++) {
+        for (int p = 0; p < cfg->nspec; p++)
+            coeff[(size_t)b * CTS_MAX_SPECIES + p] =
+                2.0 * h->n_e[b] * kRe * kElectronMassEV / kIonMassEV / cfg->mass_number[p] * kClight;
+        dtv[b] = h->ring.circ / h->ring.harmonic * h->gap[b] / kClight;
+    }
+    HIPCHK(hipMemcpy(h->d_ion_coeff, coeff.data(), coeff.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_ion_dt, dtv.data(), dtv.size() * sizeof(double), hipMemcpyHostToDevice));
     // macro-ion charge per bunch passage: line density sigma P / (k_B T) N_e
     // over the interaction length, shared by n_new macro-ions
     // (src/LatticeInterActionPoint.cpp:974-981, 998)
@@ -1232,6 +1255,34 @@ int cts_get_history(cts_handle *h, int32_t first, int32_t nturns, cts_moments *o
         HIPCHK(hipMemcpy(out + (size_t)(t - first) * h->nb, h->d_hist + (size_t)s * h->nb,
                          h->nb * sizeof(cts_moments), hipMemcpyDeviceToHost));
     }
+    return CTS_OK;
+}
+
+int cts_growth_rates(cts_handle *h, int32_t first, int32_t interval, int32_t nsamples, int32_t fit_start,
+                     int32_t fit_end, double nuz, double beta_z, double *rates)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!rates || interval < 1 || nsamples < 2 || first < 0) return fail(CTS_ERR_ARG, "cts_growth_rates");
+    const int last = first + (nsamples - 1) * interval;
+    if (h->hist_cap == 0 || last > h->last_turn || h->last_turn - first >= h->hist_cap)
+        return fail(CTS_ERR_ARG, "cts_growth_rates: turns not in the history");
+    if ((rc = flush_kick(h))) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<cts_moments> m((size_t)nsamples * h->nb);
+    for (int n = 0; n < nsamples; n++) {
+        const int t = first + n * interval;
+        HIPCHK(hipMemcpy(m.data() + (size_t)n * h->nb, h->d_hist + (size_t)(t % h->hist_cap) * h->nb,
+                         h->nb * sizeof(cts_moments), hipMemcpyDeviceToHost));
+    }
+    std::vector<int32_t> harm(h->nb, 0);
+    for (int b = 1; b < h->nb; b++) harm[b] = harm[b - 1] + h->gap[b - 1];
+    const cts_section &S = h->secs[0];
+    const double tunes[3] = {h->ring.nux, h->ring.nuy, nuz};
+    const double twiss[5] = {S.alpha_x, S.beta_x, S.alpha_y, S.beta_y, beta_z};
+    rc = cts_cbm_growth_from_moments(m.data(), nsamples, h->nb, harm.data(), h->ring.harmonic, tunes, twiss, interval,
+                                     fit_start, fit_end, h->ring.t0, rates, nullptr);
+    if (rc) return fail(rc, "cts_growth_rates: fit window outside the samples");
     return CTS_OK;
 }
 

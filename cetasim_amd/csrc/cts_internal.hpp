@@ -92,30 +92,45 @@ struct IonKickParams {
     int32_t nspec;
 };
 
-// One bunch passage through the ion cloud (all species).
-struct IonPassParams {
+// One turn of the ion cloud (all bunch passages, all species).  An ion's
+// motion depends only on the bunches' moments, which are fixed for the
+// turn, so every ion walks through all passages on its own thread; the
+// per-passage cloud moments are accumulated in 128-bit fixed point
+// (order-independent, hence deterministic) and finished passage by passage.
+struct IonTurnParams {
     double *x[CTS_MAX_SPECIES], *y[CTS_MAX_SPECIES], *vx[CTS_MAX_SPECIES], *vy[CTS_MAX_SPECIES];
     uint8_t *alive[CTS_MAX_SPECIES];
-    int64_t *len;            // [nspec] slots in use (alive or not) of the current buffer
-    int64_t *n_alive;        // [nspec] ions alive after the last passage
-    double *mean_state;      // [2 * nspec] cloud mean kept when the cloud has < 2 ions
+    const int64_t *len;      // [nspec] ions of the current buffer (compacted)
     const double *bview;     // [nb][4] bunch xbar, ybar, rmsRx, rmsRy
     const double *size_max;  // [2] largest rmsRx, rmsRy over the bunches
-    const double *ion_q;     // [nb][nspec] macro-ion charge (ions per macro-ion)
-    double *ion_e;           // [nb][CTS_MAX_SPECIES][5] electron-side kick parameters
-    double *cloud;           // [nb][CTS_MAX_SPECIES][6] count, mean x/y, rms x/y, ion number
-    double *part;            // [grid][CTS_MAX_SPECIES][6] workgroup partial sums
-    uint32_t *cnt;           // last-workgroup counter, left at zero
-    uint32_t *status;
-    double coeffI[CTS_MAX_SPECIES];
-    const float *sur_params;  // field surrogate (null: exact field)
+    const double *coeffI;    // [nb][CTS_MAX_SPECIES] 2 N_e r_e (m_e/m_u) / A c
+    const double *dt;        // [nb] C / h * gap / c
+    unsigned long long *acc; // [nspec][nb][kIonAcc][2] fixed-point sums (lo, hi)
+    const float *sur_params; // field surrogate (null: exact field)
     SurNorm sur_norm;
-    double dt, gamma, loss_boundary, ap_x, ap_y;
-    int64_t max_ions;
+    double loss_boundary, ap_x, ap_y;
+    int64_t cap;             // slots per species buffer
     uint64_t seed;
-    int32_t nspec, n_new, bunch;
-    uint32_t turn, passage;
+    int32_t nb, n_new, w_old, w_new;  // waves of old ions, waves per passage of new ions
+    uint32_t turn;
 };
+
+// Cloud moments and electron-side parameters of every passage.
+struct IonFinishParams {
+    const unsigned long long *acc;
+    const double *bview;
+    const double *ion_q;     // [nb][nspec]
+    int64_t *len;            // [nspec] in: ions at the start of the turn; out: slots written
+    int64_t *n_alive;        // [nspec] out: after the last passage
+    double *mean_state;      // [2 * nspec]
+    double *ion_e;           // [nb][CTS_MAX_SPECIES][5]
+    double *cloud;           // [nb][CTS_MAX_SPECIES][6]
+    uint32_t *status;
+    double gamma;
+    int64_t max_ions, cap;
+    int32_t nb, nspec, n_new;
+};
+constexpr int kIonAcc = 6;  // n, sum dx, sum dy, sum dx^2, sum dy^2, n kept
 
 struct IonCompactParams {
     const double *x, *y, *vx, *vy;
@@ -182,8 +197,10 @@ hipError_t launch_post(const FinParams &f, const WakeParams &w, hipStream_t st);
 hipError_t launch_wake_table(const WakeTableParams &p, int ntables, hipStream_t st);
 hipError_t launch_wake(const WakeParams &p, int nb, hipStream_t st);
 hipError_t launch_ion_prep(const cts_moments *cur, int nb, double beta_x, double beta_y, double *bview,
-                           double *size_max, uint32_t *fb_count, hipStream_t st);
-hipError_t launch_ion_passage(const IonPassParams &p, int nwg, hipStream_t st);
+                           double *size_max, uint32_t *fb_count, unsigned long long *acc, int64_t nacc,
+                           hipStream_t st);
+hipError_t launch_ion_turn(const IonTurnParams &p, int nspec, hipStream_t st);
+hipError_t launch_ion_finish(const IonFinishParams &p, hipStream_t st);
 hipError_t launch_ion_compact(const IonCompactParams &p, int64_t len_bound, hipStream_t st);
 constexpr int kIonBlock = 256;
 constexpr int kIonChunk = 4096;  // compaction chunk (16 ions per thread)

@@ -2,25 +2,27 @@
 //
 // The ion cloud of each gas species lives in HBM as FP64 arrays x, y, vx,
 // vy plus an alive byte, in two buffers (the turn-end compaction writes the
-// other one).  Bunch passages are sequential (bunch j+1 sees the cloud
-// bunch j left, src/MPBeam.cpp:2590-2609); each passage is one launch over
-// the cloud that, per ion,
-//   - generates the passage's new ions from the bunch's Gaussian
-//     (IonGenerator, src/LatticeInterActionPoint.cpp:985-1019),
-//   - accumulates the cloud moments of the state before the kick
-//     (IonRMSCal, src/LatticeInterActionPoint.cpp:1113-1155),
-//   - applies the Bassetti-Erskine kick of the bunch, the drift to the next
-//     bunch and the loss test (src/MPBunch.cpp:1113-1160,
-//     src/LatticeInterActionPoint.cpp:1158-1210).
-// The last workgroup to finish reduces the moments (fixed order) into the
-// electron-side kick parameters of that bunch (src/MPBunch.cpp:1165-1206).
-// Lost ions stay in their slots, flagged, until the compaction at the end
-// of the turn, which keeps the survivors in order.
+// other one).  In the reference the bunches pass the cloud one after the
+// other (src/MPBeam.cpp:2590-2609): per passage j it generates the bunch's
+// new ions (IonGenerator, src/LatticeInterActionPoint.cpp:985-1019), takes
+// the cloud moments (IonRMSCal, 1113-1155), kicks the ions with the bunch's
+// field and the electrons with the cloud's (src/MPBunch.cpp:1094-1210), and
+// drifts the ions to the next bunch and removes the lost ones
+// (src/LatticeInterActionPoint.cpp:1158-1210).
+//
+// An ion's motion depends only on the bunches' centroids and sizes, which
+// are fixed for the whole phase (taken before it, src/MPBeam.cpp:1061-1063),
+// not on the other ions.  So one launch moves every ion through all the
+// passages of the turn, one ion per lane (ions born at passage j start
+// there), with the same operations in the same order as passage by
+// passage; the cloud moments of each passage are summed on the way.  The
+// sums use 128-bit fixed point (2^-100 units): integer addition is
+// associative, so the result does not depend on the order in which waves
+// finish.  A second small kernel turns them into each passage's moments and
+// electron-side kick parameters, in passage order.
 #include "cts_internal.hpp"
 
 namespace cts {
-
-constexpr int kIonPart = 6;  // n, sum dx, sum dy, sum dx^2, sum dy^2, n alive after the loss test
 
 // One new ion of species p for passage `passage`: Box-Muller pairs of the
 // Philox blocks {t, passage | p << 24, turn, 2^31 | attempt/2}, rejected
@@ -46,154 +48,185 @@ __device__ __forceinline__ void ion_generate(uint64_t seed, uint32_t turn, uint3
     gy = ty;
 }
 
-// SUR: the field of the bunch at each ion from the surrogate inside its
-// domain (exact field outside it).
-template <bool SUR>
-__global__ void __launch_bounds__(kIonBlock)
-k_ion_passage(IonPassParams P)
+// v * 2^100 as a signed 128-bit integer, added to acc[0] (low word) and
+// acc[1] (high word) with a carry.  |v| < 2^27.
+__device__ __forceinline__ void fx_add(unsigned long long *acc, double v)
 {
-    constexpr int kNV = CTS_MAX_SPECIES * kIonPart;
-    __shared__ double red[kIonBlock / 64][kNV];
-    __shared__ uint32_t last;
+    const double a = fabs(v);
+    const double hs = floor(ldexp(a, 36));
+    const double rem = a - ldexp(hs, -36);  // exact (Sterbenz)
+    unsigned long long lo = (unsigned long long)ldexp(rem, 100);
+    unsigned long long hi = (unsigned long long)hs;
+    if (v < 0) {
+        lo = ~lo + 1ull;
+        hi = ~hi + (lo == 0ull ? 1ull : 0ull);
+    }
+    const unsigned long long old = atomicAdd(&acc[0], lo);
+    const unsigned long long carry = (old + lo < old) ? 1ull : 0ull;
+    if (hi + carry) atomicAdd(&acc[1], hi + carry);
+}
+
+__device__ __forceinline__ double fx_value(const unsigned long long *acc)
+{
+    unsigned long long lo = acc[0], hi = acc[1];
+    const bool neg = (long long)hi < 0;
+    if (neg) {
+        lo = ~lo + 1ull;
+        hi = ~hi + (lo == 0ull ? 1ull : 0ull);
+    }
+    const double m = ldexp((double)hi, -36) + ldexp((double)lo, -100);
+    return neg ? -m : m;
+}
+
+// grid = (w_old + nb * w_new waves, nspec), one wave per block.  Wave w <
+// w_old holds the ions of slots [64 w, 64 w + 64) from passage 0; the other
+// waves hold new ions of passage j0 (slot len + j0 n_new + t).  The lanes
+// of a wave visit the passages in step; a lane drops out when its ion is
+// lost.  SUR: the bunch's field at the ion from the surrogate inside its
+// domain, the exact field elsewhere.
+template <bool SUR>
+__global__ void __launch_bounds__(64)
+k_ion_turn(IonTurnParams P)
+{
     __shared__ __attribute__((aligned(16))) MlpLds L;
+    const int lane = threadIdx.x;
+    const int p = blockIdx.y;
+    const int64_t len = P.len[p];
+    const int w = blockIdx.x;
+    if (w < P.w_old && (int64_t)w * 64 >= len) return;  // whole wave past the old ions
     if (SUR) {
-        load_mlp_lds(L, P.sur_params, threadIdx.x, kIonBlock);
+        load_mlp_lds(L, P.sur_params, threadIdx.x, 64);
         __syncthreads();
     }
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int b = P.bunch;
-    const double xm = P.bview[4 * b + 0], ym = P.bview[4 * b + 1];
-    const double sx = P.bview[4 * b + 2], sy = P.bview[4 * b + 3];
+    int j0;
+    int64_t slot;
+    bool born = false;
+    if (w < P.w_old) {
+        j0 = 0;
+        slot = (int64_t)w * 64 + lane;
+    } else {
+        const int k = w - P.w_old;
+        j0 = k / P.w_new;
+        const int t = (k % P.w_new) * 64 + lane;
+        slot = len + (int64_t)j0 * P.n_new + t;
+        born = t < P.n_new;
+        if (slot >= P.cap) born = false;  // past the capacity: only after an overflow
+    }
+    double *X = P.x[p], *Y = P.y[p], *VX = P.vx[p], *VY = P.vy[p];
+    uint8_t *A = P.alive[p];
+    bool alive;
+    double x = 0.0, y = 0.0, vx = 0.0, vy = 0.0;
+    if (w < P.w_old) {
+        alive = slot < len && A[slot];
+        if (alive) { x = X[slot]; y = Y[slot]; vx = VX[slot]; vy = VY[slot]; }
+    } else {
+        alive = born;
+        if (alive) {
+            const double *bv = P.bview + 4 * j0;
+            ion_generate(P.seed, P.turn, (uint32_t)j0, p, (uint32_t)(slot - len - (int64_t)j0 * P.n_new), bv[0],
+                         bv[1], bv[2], bv[3], x, y);
+        }
+    }
     // loss boundary: min(ionLossBoundary * largest bunch size, aperture)
     const double lbx = P.loss_boundary * P.size_max[0], lby = P.loss_boundary * P.size_max[1];
     const double bx = P.ap_x > lbx ? lbx : P.ap_x;
     const double by = P.ap_y > lby ? lby : P.ap_y;
-
-    for (int p = 0; p < P.nspec; p++) {
-        const int64_t len = P.len[p];
-        const bool overflow = P.n_alive[p] + P.n_new > P.max_ions;
-        const int64_t end = overflow ? len : len + P.n_new;
-        const double cI = P.coeffI[p];
-        double acc[kIonPart] = {0, 0, 0, 0, 0, 0};
-        double *X = P.x[p], *Y = P.y[p], *VX = P.vx[p], *VY = P.vy[p];
-        uint8_t *A = P.alive[p];
-        // whole blocks per iteration, so that every lane of a wave reaches
-        // the surrogate together
-        for (int64_t base = (int64_t)blockIdx.x * kIonBlock; base < end; base += (int64_t)gridDim.x * kIonBlock) {
-            const int64_t i = base + tid;
-            bool act = i < end;
-            double x = 0.0, y = 0.0, vx = 0.0, vy = 0.0;
-            if (act) {
-                if (i >= len) {
-                    ion_generate(P.seed, P.turn, P.passage, p, (uint32_t)(i - len), xm, ym, sx, sy, x, y);
-                } else if (A[i]) {
-                    x = X[i]; y = Y[i]; vx = VX[i]; vy = VY[i];
-                } else {
-                    act = false;
-                }
-            }
-            const double dx = x - xm, dy = y - ym;
-            double fx = 0.0, fy = 0.0;
-            if (SUR) {
-                const bool ok = sur_field(L, P.sur_norm, act, dx, dy, sx, sy, fx, fy);
-                if (act && !ok) ion_field(dx, dy, sx, sy, fx, fy);
-            } else if (act) {
-                ion_field(dx, dy, sx, sy, fx, fy);
-            }
-            if (!act) continue;
-            acc[0] += 1.0;
-            acc[1] += dx;
-            acc[2] += dy;
-            acc[3] += dx * dx;
-            acc[4] += dy * dy;
+    unsigned long long *acc = P.acc + (int64_t)p * P.nb * kIonAcc * 2;
+    for (int j = j0; j < P.nb; j++) {
+        if (!__any(alive)) break;
+        const double *bv = P.bview + 4 * j;
+        const double xm = bv[0], ym = bv[1], sx = bv[2], sy = bv[3];
+        const double dx = x - xm, dy = y - ym;
+        double fx = 0.0, fy = 0.0;
+        if (SUR) {
+            const bool ok = sur_field(L, P.sur_norm, alive, dx, dy, sx, sy, fx, fy);
+            if (alive && !ok) ion_field(dx, dy, sx, sy, fx, fy);
+        } else if (alive) {
+            ion_field(dx, dy, sx, sy, fx, fy);
+        }
+        bool keep = false;
+        if (alive) {
+            const double cI = P.coeffI[j * CTS_MAX_SPECIES + p];
             vx += cI * fx;
             vy += cI * fy;
-            x += vx * P.dt;
-            y += vy * P.dt;
-            const bool keep = !(fabs(x) > bx || fabs(y) > by);
-            acc[5] += keep ? 1.0 : 0.0;
-            X[i] = x; Y[i] = y; VX[i] = vx; VY[i] = vy;
-            A[i] = keep ? 1 : 0;
+            x += vx * P.dt[j];
+            y += vy * P.dt[j];
+            keep = !(fabs(x) > bx || fabs(y) > by);
         }
+        // the cloud as the bunch finds it (before the kick), and the ions kept
+        double c[kIonAcc];
+        c[0] = alive ? 1.0 : 0.0;
+        c[1] = alive ? dx : 0.0;
+        c[2] = alive ? dy : 0.0;
+        c[3] = alive ? dx * dx : 0.0;
+        c[4] = alive ? dy * dy : 0.0;
+        c[5] = keep ? 1.0 : 0.0;
 #pragma unroll
-        for (int j = 0; j < kIonPart; j++) {
-            const double r = wave_sum(acc[j]);
-            if (lane == 0) red[wave][p * kIonPart + j] = r;
+        for (int v = 0; v < kIonAcc; v++) c[v] = wave_sum(c[v]);
+        if (lane == 0) {
+#pragma unroll
+            for (int v = 0; v < kIonAcc; v++)
+                if (c[v] != 0.0) fx_add(acc + ((int64_t)j * kIonAcc + v) * 2, c[v]);
         }
+        alive = keep;
     }
-    __syncthreads();
-    if (tid < kNV) {
-        double r = 0.0;
-        if (tid < P.nspec * kIonPart)
-            for (int w = 0; w < kIonBlock / 64; w++) r += red[w][tid];
-        P.part[(int64_t)blockIdx.x * kNV + tid] = r;
+    const bool mine = (w < P.w_old) ? slot < len : born;
+    if (mine) {
+        X[slot] = x; Y[slot] = y; VX[slot] = vx; VY[slot] = vy;
+        A[slot] = alive ? 1 : 0;
     }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) last = atomicAdd(P.cnt, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    // the last workgroup folds the partials: thread t takes workgroups
-    // t, t + 256, ... in order, then a fixed wave/LDS tree (deterministic)
-    {
-        double r[kNV];
-#pragma unroll
-        for (int j = 0; j < kNV; j++) r[j] = 0.0;
-        for (uint32_t w = tid; w < gridDim.x; w += kIonBlock) {
-            const double *src = P.part + (int64_t)w * kNV;
-#pragma unroll
-            for (int j = 0; j < kNV; j++) r[j] += src[j];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < kNV; j++) {
-            const double s = wave_sum(r[j]);
-            if (lane == 0) red[wave][j] = s;
-        }
-        __syncthreads();
-        double s = 0.0;
-        if (tid < kNV)
-            for (int w = 0; w < kIonBlock / 64; w++) s += red[w][tid];
-        __syncthreads();
-        if (tid < kNV) red[0][tid] = s;
-        __syncthreads();
-    }
-    if (tid < P.nspec) {
-        const int p = tid;
-        const double n = red[0][p * kIonPart + 0];
-        double mx = P.mean_state[2 * p], my = P.mean_state[2 * p + 1], rx = 0, ry = 0;
+}
+
+// Passage by passage, per species (one thread each): cloud moments with
+// the reference's one-ion and empty-cloud cases
+// (src/LatticeInterActionPoint.cpp:1113-1155), the ion number
+// (src/MPBunch.cpp:1172), the electron-side kick parameters, and the
+// capacity test of IonsUpdate (src/LatticeInterActionPoint.cpp:1041-1049).
+__global__ void k_ion_finish(IonFinishParams P)
+{
+    const int p = threadIdx.x;
+    if (p >= P.nspec) return;
+    const unsigned long long *acc = P.acc + (int64_t)p * P.nb * kIonAcc * 2;
+    double mx = P.mean_state[2 * p], my = P.mean_state[2 * p + 1];
+    double before = (double)P.len[p];
+    bool overflow = false;
+    for (int j = 0; j < P.nb; j++) {
+        const unsigned long long *a = acc + (int64_t)j * kIonAcc * 2;
+        const double n = fx_value(a);
+        const double s1 = fx_value(a + 2), s2 = fx_value(a + 4), q1 = fx_value(a + 6), q2 = fx_value(a + 8);
+        const double xm = P.bview[4 * j], ym = P.bview[4 * j + 1];
+        double rx = 0.0, ry = 0.0;
         if (n >= 2) {
-            const double ax = red[0][p * kIonPart + 1] / n, ay = red[0][p * kIonPart + 2] / n;
+            const double ax = s1 / n, ay = s2 / n;
             mx = xm + ax;
             my = ym + ay;
-            rx = sqrt(fmax(red[0][p * kIonPart + 3] / n - ax * ax, 0.0));
-            ry = sqrt(fmax(red[0][p * kIonPart + 4] / n - ay * ay, 0.0));
+            rx = sqrt(fmax(q1 / n - ax * ax, 0.0));
+            ry = sqrt(fmax(q2 / n - ay * ay, 0.0));
         } else if (n == 1) {
             // one ion: uncentred rms, the mean keeps its last value
-            // (src/LatticeInterActionPoint.cpp:1125-1129)
-            rx = fabs(xm + red[0][p * kIonPart + 1]);
-            ry = fabs(ym + red[0][p * kIonPart + 2]);
+            rx = fabs(xm + s1);
+            ry = fabs(ym + s2);
         }
-        P.mean_state[2 * p] = mx;
-        P.mean_state[2 * p + 1] = my;
-        // ion number of the cloud: this bunch's macro-ion charge times the
-        // macro-ion count (src/MPBunch.cpp:1172)
-        const double nI0 = P.ion_q[b * P.nspec + p] * n;
-        double *e = P.ion_e + ((int64_t)b * CTS_MAX_SPECIES + p) * 5;
+        const double nI0 = P.ion_q[j * P.nspec + p] * n;
+        double *e = P.ion_e + ((int64_t)j * CTS_MAX_SPECIES + p) * 5;
         e[0] = 2.0 * nI0 * kRe / P.gamma;
         e[1] = mx;
         e[2] = my;
         e[3] = rx;
         e[4] = ry;
-        double *c = P.cloud + ((int64_t)b * CTS_MAX_SPECIES + p) * 6;
+        double *c = P.cloud + ((int64_t)j * CTS_MAX_SPECIES + p) * 6;
         c[0] = n; c[1] = mx; c[2] = my; c[3] = rx; c[4] = ry; c[5] = nI0;
-        const bool overflow = P.n_alive[p] + P.n_new > P.max_ions;
-        if (overflow) atomicOr(P.status, 2u);
-        P.n_alive[p] = (int64_t)red[0][p * kIonPart + 5];
-        if (!overflow) P.len[p] += P.n_new;
+        if (before + P.n_new > (double)P.max_ions) overflow = true;
+        before = fx_value(a + 10);
     }
-    if (tid == 0) *P.cnt = 0u;
+    P.mean_state[2 * p] = mx;
+    P.mean_state[2 * p + 1] = my;
+    P.n_alive[p] = (int64_t)before;
+    // slots written by the turn (old ions, then every passage's new ones),
+    // the range the compaction reads
+    const int64_t used = P.len[p] + (int64_t)P.nb * P.n_new;
+    P.len[p] = used < P.cap ? used : P.cap;
+    if (overflow) atomicOr(P.status, 2u);
 }
 
 // Bunch views for the ion phase: centroid and rms sizes of every bunch
@@ -202,9 +235,10 @@ k_ion_passage(IonPassParams P)
 // src/MPBeam.cpp:2203-2258).
 __global__ void __launch_bounds__(256)
 k_ion_prep(const cts_moments *cur, int nb, double beta_x, double beta_y, double *bview, double *size_max,
-           uint32_t *fb_count)
+           uint32_t *fb_count, unsigned long long *acc, int64_t nacc)
 {
     if (threadIdx.x == 0 && fb_count) *fb_count = 0u;
+    for (int64_t i = threadIdx.x; i < nacc; i += blockDim.x) acc[i] = 0ull;
     __shared__ double mx[256], my[256];
     double ax = -INFINITY, ay = -INFINITY;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
@@ -298,19 +332,27 @@ hipError_t upload_faddeeva_tables_ions(const double ea2n2[64], const double fad[
     return hipMemcpyToSymbol(HIP_SYMBOL(c_fad), fad, 3 * sizeof(double));
 }
 
-hipError_t launch_ion_passage(const IonPassParams &p, int nwg, hipStream_t st)
+hipError_t launch_ion_turn(const IonTurnParams &p, int nspec, hipStream_t st)
 {
+    const dim3 grid((unsigned)(p.w_old + p.nb * p.w_new), (unsigned)nspec);
     if (p.sur_params)
-        hipLaunchKernelGGL(k_ion_passage<true>, dim3(nwg), dim3(kIonBlock), 0, st, p);
+        hipLaunchKernelGGL(k_ion_turn<true>, grid, dim3(64), 0, st, p);
     else
-        hipLaunchKernelGGL(k_ion_passage<false>, dim3(nwg), dim3(kIonBlock), 0, st, p);
+        hipLaunchKernelGGL(k_ion_turn<false>, grid, dim3(64), 0, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_ion_finish(const IonFinishParams &p, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ion_finish, dim3(1), dim3(64), 0, st, p);
     return hipGetLastError();
 }
 
 hipError_t launch_ion_prep(const cts_moments *cur, int nb, double bx, double by, double *bview, double *size_max,
-                           uint32_t *fb_count, hipStream_t st)
+                           uint32_t *fb_count, unsigned long long *acc, int64_t nacc, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_ion_prep, dim3(1), dim3(256), 0, st, cur, nb, bx, by, bview, size_max, fb_count);
+    hipLaunchKernelGGL(k_ion_prep, dim3(1), dim3(256), 0, st, cur, nb, bx, by, bview, size_max, fb_count, acc,
+                       nacc);
     return hipGetLastError();
 }
 

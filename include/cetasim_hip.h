@@ -238,6 +238,30 @@ int cts_kernel_time(cts_handle *h, const char *name, double *avg_ms, int64_t *la
 int cts_debug_qe_normals(cts_handle *h, uint64_t seed, int32_t turn, uint64_t base,
                          int64_t n, double *noise_out);
 
+/* ------------------------------------------------------- growth rates
+ * MPBeam::GetCBMGR, "ideal" coupled-bunch modes (src/MPBeam.cpp:1496-1891):
+ * per sample n and bunch b the normalized phasor
+ *   (x/sqrt(bx) - i (sqrt(bx) px + ax/sqrt(bx) x)) e^{-2 pi i nu h_b / h}
+ * (and y, and (z/sqrt(bz) + i sqrt(bz) dp) for z), its forward DFT over the
+ * bunches, and for every mode the least-squares slope of log|amplitude|
+ * against turn over samples [fit_start/interval, fit_end/interval) with unit
+ * weights (FittingGSL::FitALinear, src/FittingGSL.cpp:231-243), divided by
+ * T0.  m[nsamples][nb] are moment records taken every `interval` turns;
+ * harm_num[b] = bunchHarmNum; tunes = {nux, nuy, nuz}; twiss = {alpha_x,
+ * beta_x, alpha_y, beta_y, beta_z}.  rates[3][nb] (1/s); amps[nsamples][3][nb]
+ * (mode amplitudes) may be NULL. */
+int cts_cbm_growth_from_moments(const cts_moments *m, int32_t nsamples, int32_t nb, const int32_t *harm_num,
+                                int32_t harmonics, const double *tunes, const double *twiss, int32_t interval,
+                                int32_t fit_start, int32_t fit_end, double t0, double *rates, double *amps);
+/* gsl_fit_wlinear's line fit y = c0 + c1 x (the fit above). */
+int cts_fit_wlinear(const double *x, const double *w, const double *y, int32_t n, double *c0, double *c1);
+/* The same from the handle's moment history (cts_set_history): samples at
+ * turns first, first + interval, ...; bunch harmonic numbers from the gaps
+ * (bunch 0 at 0); nuz and beta_z as the caller's ReadInputSettings has
+ * them (workQz, rmsBunchLength / rmsEnergySpread). */
+int cts_growth_rates(cts_handle *h, int32_t first, int32_t interval, int32_t nsamples, int32_t fit_start,
+                     int32_t fit_end, double nuz, double beta_z, double *rates);
+
 /* ---------------------------------------------------- field surrogate
  * No counterpart in the reference, which evaluates every Bassetti-Erskine
  * kick through the Faddeeva function (src/Bunch.cpp:119-211).  A 3-64-64-2

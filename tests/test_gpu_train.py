@@ -150,3 +150,46 @@ def test_ion_overflow_status():
         tr.sync()
     assert e.value.code == capi.CTS_ION_OVERFLOW
     tr.close()
+
+
+def test_growth_rates_match_oracle():
+    """Coupled-bunch growth rates (MPBeam::GetCBMGR's ideal modes) fitted to
+    the device's turn-by-turn centroids against the same fit on the oracle's
+    centroids: within 1%."""
+    from cetasim_amd import capi as C
+    counts = [1800, 2600, 1500, 2200, 3000, 1700, 2400, 2000]
+    nb = len(counts)
+    p = configs.BASE
+    ring, secs, der = configs.lattice(p)
+    ions = dict(ION_TEST, pressure=[2.0e-6], n_new=200)
+    ring, secs, counts, gap, q, ne, cfg, tr, bunches, off = setup_train(counts, 21, ions=ions)
+    turns = 120
+    tr.set_history(turns)
+    ot = O.OracleTrain(ring, secs[0], counts, ne, gap, cfg, ions["seed"])
+    omean = np.zeros((turns, nb, 6))
+    for t in range(turns):
+        tr.turn(t)
+        ot.turn(bunches, t)
+        for b in range(nb):
+            omean[t, b] = O.rms(bunches[b], secs[0]).mean[:]
+    rates = np.empty(3 * nb)
+    capi.check(tr.lib.cts_growth_rates(tr.h, 0, 1, turns, 20, 120, der.workqz, der.beta_z, capi.dptr(rates)))
+    rates = rates.reshape(3, nb)
+    # the same fit on the oracle's centroids
+    m = (C.Moments * (turns * nb))()
+    for t in range(turns):
+        for b in range(nb):
+            m[t * nb + b].mean[:] = list(omean[t, b])
+    harm = np.concatenate([[0], np.cumsum(gap[:-1])]).astype(np.int32)
+    eref = np.empty(3 * nb)
+    assert tr.lib.cts_cbm_growth_from_moments(
+        m, turns, nb, capi.i32ptr(harm), ring.harmonic, capi.dptr(np.array([ring.nux, ring.nuy, der.workqz])),
+        capi.dptr(np.array([secs[0].alpha_x, secs[0].beta_x, secs[0].alpha_y, secs[0].beta_y, der.beta_z])),
+        1, 20, 120, ring.t0, capi.dptr(eref), None) == 0
+    eref = eref.reshape(3, nb)
+    for pl in range(2):
+        scale = np.max(np.abs(eref[pl]))
+        assert scale > 0
+        tol = 0.01 * np.maximum(np.abs(eref[pl]), 0.1 * scale)
+        assert np.all(np.abs(rates[pl] - eref[pl]) <= tol), (pl, rates[pl], eref[pl])
+    tr.close()
