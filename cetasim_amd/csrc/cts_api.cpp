@@ -138,7 +138,7 @@ struct cts_handle {
     double *d_ion_bview = nullptr, *d_ion_smax = nullptr, *d_ion_q = nullptr;
     double *d_ion_cloud = nullptr;
     double *d_ion_coeff = nullptr, *d_ion_dt = nullptr;  // [nb][CTS_MAX_SPECIES], [nb]
-    unsigned long long *d_ion_acc = nullptr;             // [nspec][nb][kIonAcc][2]
+    unsigned long long *d_ion_acc = nullptr;             // [nspec][nb][kIonAcc][3]
     uint32_t *d_ion_chunk = nullptr;
     // d_cur holds the moments of the present coordinates
     bool cur_valid = false;
@@ -540,7 +540,7 @@ int ion_phase(cts_handle *h, int32_t turn)
     if (!h->cur_valid && (rc = enqueue_moments(h))) return rc;
     const cts_ion_config &I = h->ion;
     const cts_section &S = h->secs[0];
-    const int64_t nacc = (int64_t)h->nspec * h->nb * kIonAcc * 2;
+    const int64_t nacc = (int64_t)h->nspec * h->nb * kIonAcc * 3;
     {
         TimedScope ts(h, "ion_prep");
         HIPCHK(launch_ion_prep(h->d_cur, h->nb, S.beta_x, S.beta_y, h->d_ion_bview, h->d_ion_smax, h->d_fb_count,
@@ -1119,7 +1119,7 @@ int cts_set_ions(cts_handle *h, const cts_ion_config *cfg)
         (rc = dalloc(&h->d_ion_smax, 2)) || (rc = dalloc(&h->d_ion_q, (size_t)h->nb * cfg->nspec)) ||
         (rc = dalloc(&h->d_ion_cloud, (size_t)h->nb * CTS_MAX_SPECIES * 6)) ||
         (rc = dalloc(&h->d_ion_coeff, (size_t)h->nb * CTS_MAX_SPECIES)) || (rc = dalloc(&h->d_ion_dt, (size_t)h->nb)) ||
-        (rc = dalloc(&h->d_ion_acc, (size_t)cfg->nspec * h->nb * kIonAcc * 2)) ||
+        (rc = dalloc(&h->d_ion_acc, (size_t)cfg->nspec * h->nb * kIonAcc * 3)) ||
         (rc = dalloc(&h->d_ion_chunk, (size_t)nchunk)))
         return rc;
     // per bunch: ion kick coefficient 2 N_e r_e (m_e / m_u) / A c

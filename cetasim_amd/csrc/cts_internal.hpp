@@ -105,7 +105,7 @@ struct IonTurnParams {
     const double *size_max;  // [2] largest rmsRx, rmsRy over the bunches
     const double *coeffI;    // [nb][CTS_MAX_SPECIES] 2 N_e r_e (m_e/m_u) / A c
     const double *dt;        // [nb] C / h * gap / c
-    unsigned long long *acc; // [nspec][nb][kIonAcc][2] fixed-point sums (lo, hi)
+    unsigned long long *acc; // [nspec][nb][kIonAcc][3] fixed-point sums (three limbs)
     const float *sur_params; // field surrogate (null: exact field)
     SurNorm sur_norm;
     double loss_boundary, ap_x, ap_y;

@@ -48,56 +48,49 @@ __device__ __forceinline__ void ion_generate(uint64_t seed, uint32_t turn, uint3
     gy = ty;
 }
 
-// v * 2^100 as a signed 128-bit integer, added to acc[0] (low word) and
-// acc[1] (high word) with a carry.  |v| < 2^27.
-__device__ __forceinline__ void fx_add(unsigned long long *acc, double v)
+// v * 2^100 as three signed limbs of < 2^42 each,
+// v = (l2 2^84 + l1 2^42 + l0) 2^-100, for |v| < 2^26.  Limb sums of up to
+// 2^21 terms stay inside 64 bits, so the limbs are added with plain
+// (non-returning) integer atomics and no carries.
+__device__ __forceinline__ long long fx_limb(double v, int k)
 {
     const double a = fabs(v);
-    const double hs = floor(ldexp(a, 36));
-    const double rem = a - ldexp(hs, -36);  // exact (Sterbenz)
-    unsigned long long lo = (unsigned long long)ldexp(rem, 100);
-    unsigned long long hi = (unsigned long long)hs;
-    if (v < 0) {
-        lo = ~lo + 1ull;
-        hi = ~hi + (lo == 0ull ? 1ull : 0ull);
-    }
-    const unsigned long long old = atomicAdd(&acc[0], lo);
-    const unsigned long long carry = (old + lo < old) ? 1ull : 0ull;
-    if (hi + carry) atomicAdd(&acc[1], hi + carry);
+    const double t2 = floor(ldexp(a, 16));
+    const double r2 = a - ldexp(t2, -16);  // exact (Sterbenz)
+    const double t1 = floor(ldexp(r2, 58));
+    const double r1 = r2 - ldexp(t1, -58);  // exact
+    const double t = k == 2 ? t2 : (k == 1 ? t1 : rint(ldexp(r1, 100)));
+    const long long m = (long long)t;
+    return v < 0 ? -m : m;
 }
 
 __device__ __forceinline__ double fx_value(const unsigned long long *acc)
 {
-    unsigned long long lo = acc[0], hi = acc[1];
-    const bool neg = (long long)hi < 0;
-    if (neg) {
-        lo = ~lo + 1ull;
-        hi = ~hi + (lo == 0ull ? 1ull : 0ull);
-    }
-    const double m = ldexp((double)hi, -36) + ldexp((double)lo, -100);
-    return neg ? -m : m;
+    return ldexp((double)(long long)acc[2], -16) + ldexp((double)(long long)acc[1], -58) +
+           ldexp((double)(long long)acc[0], -100);
 }
 
-// grid = (w_old + nb * w_new waves, nspec), one wave per block.  Wave w <
+// grid = (waves / 4, nspec), four independent waves per block.  Wave w <
 // w_old holds the ions of slots [64 w, 64 w + 64) from passage 0; the other
 // waves hold new ions of passage j0 (slot len + j0 n_new + t).  The lanes
 // of a wave visit the passages in step; a lane drops out when its ion is
 // lost.  SUR: the bunch's field at the ion from the surrogate inside its
 // domain, the exact field elsewhere.
 template <bool SUR>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(256)
 k_ion_turn(IonTurnParams P)
 {
     __shared__ __attribute__((aligned(16))) MlpLds L;
-    const int lane = threadIdx.x;
-    const int p = blockIdx.y;
-    const int64_t len = P.len[p];
-    const int w = blockIdx.x;
-    if (w < P.w_old && (int64_t)w * 64 >= len) return;  // whole wave past the old ions
     if (SUR) {
-        load_mlp_lds(L, P.sur_params, threadIdx.x, 64);
+        load_mlp_lds(L, P.sur_params, threadIdx.x, 256);
         __syncthreads();
     }
+    const int lane = threadIdx.x & 63;
+    const int p = blockIdx.y;
+    const int64_t len = P.len[p];
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= P.w_old + P.nb * P.w_new) return;
+    if (w < P.w_old && (int64_t)w * 64 >= len) return;  // whole wave past the old ions
     int j0;
     int64_t slot;
     bool born = false;
@@ -131,7 +124,7 @@ k_ion_turn(IonTurnParams P)
     const double lbx = P.loss_boundary * P.size_max[0], lby = P.loss_boundary * P.size_max[1];
     const double bx = P.ap_x > lbx ? lbx : P.ap_x;
     const double by = P.ap_y > lby ? lby : P.ap_y;
-    unsigned long long *acc = P.acc + (int64_t)p * P.nb * kIonAcc * 2;
+    unsigned long long *acc = P.acc + (int64_t)p * P.nb * kIonAcc * 3;
     for (int j = j0; j < P.nb; j++) {
         if (!__any(alive)) break;
         const double *bv = P.bview + 4 * j;
@@ -163,10 +156,14 @@ k_ion_turn(IonTurnParams P)
         c[5] = keep ? 1.0 : 0.0;
 #pragma unroll
         for (int v = 0; v < kIonAcc; v++) c[v] = wave_sum(c[v]);
-        if (lane == 0) {
+        // lane 3 v + k adds limb k of value v
+        if (lane < 3 * kIonAcc) {
+            const int v = lane / 3, k = lane - 3 * v;
+            double cv = c[0];
 #pragma unroll
-            for (int v = 0; v < kIonAcc; v++)
-                if (c[v] != 0.0) fx_add(acc + ((int64_t)j * kIonAcc + v) * 2, c[v]);
+            for (int u = 1; u < kIonAcc; u++) cv = v == u ? c[u] : cv;
+            const long long l = fx_limb(cv, k);
+            if (l != 0) atomicAdd(acc + ((int64_t)j * kIonAcc + v) * 3 + k, (unsigned long long)l);
         }
         alive = keep;
     }
@@ -177,56 +174,63 @@ k_ion_turn(IonTurnParams P)
     }
 }
 
-// Passage by passage, per species (one thread each): cloud moments with
+// The passage sums converted in parallel, then passage by passage in order
+// (one thread per species): cloud moments with
 // the reference's one-ion and empty-cloud cases
 // (src/LatticeInterActionPoint.cpp:1113-1155), the ion number
 // (src/MPBunch.cpp:1172), the electron-side kick parameters, and the
 // capacity test of IonsUpdate (src/LatticeInterActionPoint.cpp:1041-1049).
-__global__ void k_ion_finish(IonFinishParams P)
+__global__ void __launch_bounds__(256)
+k_ion_finish(IonFinishParams P)
 {
-    const int p = threadIdx.x;
-    if (p >= P.nspec) return;
-    const unsigned long long *acc = P.acc + (int64_t)p * P.nb * kIonAcc * 2;
-    double mx = P.mean_state[2 * p], my = P.mean_state[2 * p + 1];
-    double before = (double)P.len[p];
-    bool overflow = false;
-    for (int j = 0; j < P.nb; j++) {
-        const unsigned long long *a = acc + (int64_t)j * kIonAcc * 2;
-        const double n = fx_value(a);
-        const double s1 = fx_value(a + 2), s2 = fx_value(a + 4), q1 = fx_value(a + 6), q2 = fx_value(a + 8);
-        const double xm = P.bview[4 * j], ym = P.bview[4 * j + 1];
-        double rx = 0.0, ry = 0.0;
-        if (n >= 2) {
-            const double ax = s1 / n, ay = s2 / n;
-            mx = xm + ax;
-            my = ym + ay;
-            rx = sqrt(fmax(q1 / n - ax * ax, 0.0));
-            ry = sqrt(fmax(q2 / n - ay * ay, 0.0));
-        } else if (n == 1) {
-            // one ion: uncentred rms, the mean keeps its last value
-            rx = fabs(xm + s1);
-            ry = fabs(ym + s2);
+    extern __shared__ double sums[];  // [nb][kIonAcc]
+    for (int p = 0; p < P.nspec; p++) {
+        const unsigned long long *acc = P.acc + (int64_t)p * P.nb * kIonAcc * 3;
+        for (int i = threadIdx.x; i < P.nb * kIonAcc; i += blockDim.x) sums[i] = fx_value(acc + (int64_t)i * 3);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double mx = P.mean_state[2 * p], my = P.mean_state[2 * p + 1];
+            double before = (double)P.len[p];
+            bool overflow = false;
+            for (int j = 0; j < P.nb; j++) {
+                const double *a = sums + j * kIonAcc;
+                const double n = a[0], s1 = a[1], s2 = a[2], q1 = a[3], q2 = a[4];
+                const double xm = P.bview[4 * j], ym = P.bview[4 * j + 1];
+                double rx = 0.0, ry = 0.0;
+                if (n >= 2) {
+                    const double ax = s1 / n, ay = s2 / n;
+                    mx = xm + ax;
+                    my = ym + ay;
+                    rx = sqrt(fmax(q1 / n - ax * ax, 0.0));
+                    ry = sqrt(fmax(q2 / n - ay * ay, 0.0));
+                } else if (n == 1) {
+                    // one ion: uncentred rms, the mean keeps its last value
+                    rx = fabs(xm + s1);
+                    ry = fabs(ym + s2);
+                }
+                const double nI0 = P.ion_q[j * P.nspec + p] * n;
+                double *e = P.ion_e + ((int64_t)j * CTS_MAX_SPECIES + p) * 5;
+                e[0] = 2.0 * nI0 * kRe / P.gamma;
+                e[1] = mx;
+                e[2] = my;
+                e[3] = rx;
+                e[4] = ry;
+                double *c = P.cloud + ((int64_t)j * CTS_MAX_SPECIES + p) * 6;
+                c[0] = n; c[1] = mx; c[2] = my; c[3] = rx; c[4] = ry; c[5] = nI0;
+                if (before + P.n_new > (double)P.max_ions) overflow = true;
+                before = a[5];
+            }
+            P.mean_state[2 * p] = mx;
+            P.mean_state[2 * p + 1] = my;
+            P.n_alive[p] = (int64_t)before;
+            // slots written by the turn (old ions, then every passage's new
+            // ones), the range the compaction reads
+            const int64_t used = P.len[p] + (int64_t)P.nb * P.n_new;
+            P.len[p] = used < P.cap ? used : P.cap;
+            if (overflow) atomicOr(P.status, 2u);
         }
-        const double nI0 = P.ion_q[j * P.nspec + p] * n;
-        double *e = P.ion_e + ((int64_t)j * CTS_MAX_SPECIES + p) * 5;
-        e[0] = 2.0 * nI0 * kRe / P.gamma;
-        e[1] = mx;
-        e[2] = my;
-        e[3] = rx;
-        e[4] = ry;
-        double *c = P.cloud + ((int64_t)j * CTS_MAX_SPECIES + p) * 6;
-        c[0] = n; c[1] = mx; c[2] = my; c[3] = rx; c[4] = ry; c[5] = nI0;
-        if (before + P.n_new > (double)P.max_ions) overflow = true;
-        before = fx_value(a + 10);
+        __syncthreads();
     }
-    P.mean_state[2 * p] = mx;
-    P.mean_state[2 * p + 1] = my;
-    P.n_alive[p] = (int64_t)before;
-    // slots written by the turn (old ions, then every passage's new ones),
-    // the range the compaction reads
-    const int64_t used = P.len[p] + (int64_t)P.nb * P.n_new;
-    P.len[p] = used < P.cap ? used : P.cap;
-    if (overflow) atomicOr(P.status, 2u);
 }
 
 // Bunch views for the ion phase: centroid and rms sizes of every bunch
@@ -334,17 +338,18 @@ hipError_t upload_faddeeva_tables_ions(const double ea2n2[64], const double fad[
 
 hipError_t launch_ion_turn(const IonTurnParams &p, int nspec, hipStream_t st)
 {
-    const dim3 grid((unsigned)(p.w_old + p.nb * p.w_new), (unsigned)nspec);
+    const int64_t waves = (int64_t)p.w_old + (int64_t)p.nb * p.w_new;
+    const dim3 grid((unsigned)((waves + 3) / 4), (unsigned)nspec);
     if (p.sur_params)
-        hipLaunchKernelGGL(k_ion_turn<true>, grid, dim3(64), 0, st, p);
+        hipLaunchKernelGGL(k_ion_turn<true>, grid, dim3(256), 0, st, p);
     else
-        hipLaunchKernelGGL(k_ion_turn<false>, grid, dim3(64), 0, st, p);
+        hipLaunchKernelGGL(k_ion_turn<false>, grid, dim3(256), 0, st, p);
     return hipGetLastError();
 }
 
 hipError_t launch_ion_finish(const IonFinishParams &p, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_ion_finish, dim3(1), dim3(64), 0, st, p);
+    hipLaunchKernelGGL(k_ion_finish, dim3(1), dim3(256), (size_t)p.nb * kIonAcc * sizeof(double), st, p);
     return hipGetLastError();
 }
 
