@@ -47,9 +47,10 @@ def parse():
     ap.add_argument("--nbins", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--workload", choices=("c2", "c3"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c3", "c5"), default="c2",
                     help="c2: single bunch 1e7 macroparticles with wake and excitation; "
-                         "c3: 80-bunch train with the beam-ion phase")
+                         "c3: 80-bunch train with the beam-ion phase; "
+                         "c5: 480-bunch train (6e7 macroparticles) with wake and ions")
     ap.add_argument("--plain-only", action="store_true", help="c3: skip the surrogate run")
     ap.add_argument("--sur-steps", type=int, default=20000, help="c3: Adam steps of the surrogate training")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -166,25 +167,39 @@ def train_surrogate(steps):
                "max_rel_field_error": mx, "rms_rel_field_error": rms}
 
 
+_PARTS = {}
+
+
+def workload_train(a):
+    from cetasim_amd import configs
+    return configs.LARGE if a.workload == "c5" else configs.TRAIN
+
+
 def train_turns(a, sur=None, mode=0):
-    """Time a.steps turns of the 80-bunch train; returns (s, kernel times, cloud)."""
+    """Time a.steps turns of the train; returns (s, kernel times, cloud, moments, n, nb)."""
     from cetasim_amd import configs
     from cetasim_amd.tracker import Tracker
 
     p = configs.BASE
-    T = configs.TRAIN
+    T = workload_train(a)
     ring, secs, _ = configs.lattice(p)
-    counts = configs.train_counts()
+    counts = configs.train_counts(T["nb"], T["count_range"][0], T["count_range"][1], T["seed"])
     nb = len(counts)
     gap = configs.train_gaps(nb, T["spacing"], ring.harmonic)
     q, ne = configs.bunch_scalars(counts, ring, p)
-    parts = configs.train_bunches(counts, T["seed"], p)
+    key = (a.workload, T["seed"])
+    if key not in _PARTS:  # the same synthetic bunches for every run of one invocation
+        _PARTS.clear()
+        _PARTS[key] = configs.train_bunches(counts, T["seed"], p)
     tr = Tracker(ring, secs)
     tr.set_bunches(counts, q, ne, gap)
-    tr.upload(parts)
-    del parts
+    tr.upload(_PARTS[key])
     tr.set_ions(configs.ion_config(ring, T["ions"], p))
     tr.set_synrad(True, 2, 2)
+    if T.get("nbins"):
+        sz = p["sig"][2]
+        bbr = p["bbr"]
+        tr.set_wake(T["nbins"], 1, -5 * sz, 5 * sz, bbr["rs"], bbr["q"], bbr["fr"])
     if sur is not None:
         tr.set_field_model(sur, mode)
     for t in range(a.warmup):
@@ -211,7 +226,7 @@ def run_train(a):
     from cetasim_amd import configs
 
     p = configs.BASE
-    T = configs.TRAIN
+    T = workload_train(a)
     ring, secs, _ = configs.lattice(p)
     dt, ktimes, cloud, m_plain, n, nb = train_turns(a)
     plain = n * a.steps / dt
@@ -241,11 +256,13 @@ def run_train(a):
         "unit": "macroparticle-turns/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt / a.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{nb} bunches 4 ns apart, counts uniform in [1e4, 2e5] (seed 3, total {n}), "
-                               "1 nC, CO ions at 1e-9 Pa / 300 K over the circumference, "
+        "config": {"workload": f"{nb} bunches 4 ns apart, counts uniform in "
+                               f"[{T['count_range'][0]:.0e}, {T['count_range'][1]:.0e}] (seed {T['seed']}, "
+                               f"total {n}), 1 nC, CO ions at 1e-9 Pa / 300 K over the circumference, "
                                f"{T['ions']['n_new']} macro-ions per passage, <= {T['ions']['max_ions']}, "
                                "Bassetti-Erskine both ways (FP64 Faddeeva; with the surrogate inside its "
-                               "domain when value is the surrogate rate), excitation on (device Philox)",
+                               "domain when value is the surrogate rate), excitation on (device Philox)"
+                               + (f", BBR wake on {T['nbins']} fixed bins per bunch" if T.get("nbins") else ""),
                    "macroparticles": n, "bunches": nb, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_pass (fused turn)",
@@ -254,7 +271,7 @@ def run_train(a):
         "ion_macroparticles_last_passage": float(cloud[-1, 0, 0]),
         "surrogate": sur_info,
     }
-    if not a.no_cpu_baseline:
+    if not a.no_cpu_baseline and a.workload == "c3":
         out["cpu_baseline"] = cpu_baseline_train(ring, secs, p, a.cpu_seconds)
     print(json.dumps(out))
 
@@ -268,7 +285,7 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
                                    "(multi-GPU sharding is a design note only)"}))
         sys.exit(2)
-    if a.workload == "c3":
+    if a.workload in ("c3", "c5"):
         return run_train(a)
     import numpy as np
 

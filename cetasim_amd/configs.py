@@ -149,3 +149,11 @@ def ion_config(ring, ions=TRAIN["ions"], p=BASE):
     cfg.ap_x, cfg.ap_y = p["ap_x"], p["ap_y"]
     cfg.seed = ions["seed"]
     return cfg
+
+
+# Large batched train: 480 bunches 4 ns apart with a 40-bucket gap
+# (480 * 2 + 40 = 1000 = h), macroparticle counts uniform in [5e4, 2e5]
+# (seed 5), BBR wake per bunch on 100 fixed bins over +-5 sigma_z, the ion
+# cloud of TRAIN with 10 000 new macro-ions per passage (at most 5e6).
+LARGE = dict(nb=480, spacing=2, count_range=(50_000, 200_000), seed=5, turns=1000, nbins=100,
+             ions=dict(TRAIN["ions"], n_new=10_000, max_ions=5_000_000, seed=5))
