@@ -27,7 +27,7 @@ BYTES_PER_MPT = 96.0  # 6 FP64 read + 6 FP64 written per macroparticle-turn
 def measured_traffic_per_mpt():
     """HBM bytes per macroparticle-turn of k_pass from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (scaled by the copy-kernel calibration)."""
-    path = os.path.join(ROOT, "profiles", "k_pass_c2_hbm_traffic.txt")
+    path = os.path.join(ROOT, "profiles", "k_pass_single_bunch_1e7_hbm_traffic.txt")
     try:
         with open(path) as fh:
             for line in fh:
