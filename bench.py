@@ -202,6 +202,7 @@ def train_turns(a, sur=None, mode=0):
         tr.set_wake(T["nbins"], 1, -5 * sz, 5 * sz, bbr["rs"], bbr["q"], bbr["fr"])
     if sur is not None:
         tr.set_field_model(sur, mode)
+    tr.autotune()
     for t in range(a.warmup):
         tr.turn(t)
     tr.sync()
@@ -305,6 +306,7 @@ def main():
     bbr = p["bbr"]
     tr.set_wake(a.nbins, 1, -5 * sz, 5 * sz, bbr["rs"], bbr["q"], bbr["fr"])
     tr.set_synrad(True, 2, 2)
+    nwg = tr.autotune()
 
     for t in range(a.warmup):
         tr.turn(t)
@@ -340,7 +342,8 @@ def main():
         "config": {"workload": f"single bunch, {n:.0e} macroparticles FP64 SoA, {a.nbins} fixed bins "
                                "+-5 sigma_z, BBR wake Rs=10kOhm fr=10GHz Q=1, quantum excitation "
                                "(device Philox, seed 2), one-section ring Qx/Qy/Qs=0.18/0.27/0.005",
-                   "macroparticles": n, "bins": a.nbins, "parallelism": "single GPU"},
+                   "macroparticles": n, "bins": a.nbins, "parallelism": "single GPU",
+                   "workgroups": nwg},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None if tpm is None else tpm * n,
                      "traffic_source": tsrc,

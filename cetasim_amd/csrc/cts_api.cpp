@@ -487,6 +487,73 @@ int reset_status(cts_handle *h)
     return CTS_OK;
 }
 
+// Segment table: the particle range cut into wg_target contiguous chunks
+// (multiples of 64), each cut again at bunch boundaries; workgroup w owns
+// the segments wg_seg[w] .. wg_seg[w+1].  (Re)allocates everything sized
+// by the number of segments.
+int layout_segments(cts_handle *h, int wg_target)
+{
+    const int nb = h->nb;
+    const int64_t n = h->ntot;
+    int64_t want = (n + 2047) / 2048;
+    if (want < 1) want = 1;
+    if (want > wg_target) want = wg_target;
+    int64_t chunk = (n + want - 1) / want;
+    chunk = (chunk + 63) / 64 * 64;
+    if (chunk < 64) chunk = 64;
+    const int nwg = (int)((n + chunk - 1) / chunk) > 0 ? (int)((n + chunk - 1) / chunk) : 1;
+    std::vector<int64_t> sb, se;
+    std::vector<int32_t> sbunch, wgseg(nwg + 1, 0), bseg(nb + 1, 0);
+    int b = 0;
+    for (int w = 0; w < nwg; w++) {
+        wgseg[w] = (int32_t)sb.size();
+        const int64_t lo = (int64_t)w * chunk, hi = std::min<int64_t>(n, lo + chunk);
+        int64_t p = lo;
+        while (p < hi) {
+            while (b < nb && h->offsets[b + 1] <= p) b++;
+            const int64_t e2 = std::min<int64_t>(hi, h->offsets[b + 1]);
+            sb.push_back(p);
+            se.push_back(e2);
+            sbunch.push_back(b);
+            p = e2;
+        }
+    }
+    wgseg[nwg] = (int32_t)sb.size();
+    const int nseg = (int)sb.size();
+    {
+        int s = 0;
+        for (int bb = 0; bb < nb; bb++) {
+            while (s < nseg && sbunch[s] < bb) s++;
+            bseg[bb] = s;
+        }
+        bseg[nb] = nseg;
+    }
+    int max_chunks = 1;
+    for (int bb = 0; bb < nb; bb++) max_chunks = std::max(max_chunks, (bseg[bb + 1] - bseg[bb] + 63) / 64);
+    void *old[] = {h->d_seg_begin, h->d_seg_end, h->d_seg_bunch, h->d_wg_seg, h->d_bunch_seg, h->d_seg_part,
+                   h->d_chunk_part};
+    for (void *p : old)
+        if (p) (void)hipFree(p);
+    h->d_seg_begin = h->d_seg_end = nullptr;
+    h->d_seg_bunch = h->d_wg_seg = h->d_bunch_seg = nullptr;
+    h->d_seg_part = h->d_chunk_part = nullptr;
+    int rc;
+    if ((rc = dalloc(&h->d_seg_begin, nseg)) || (rc = dalloc(&h->d_seg_end, nseg)) ||
+        (rc = dalloc(&h->d_seg_bunch, nseg)) || (rc = dalloc(&h->d_wg_seg, nwg + 1)) ||
+        (rc = dalloc(&h->d_bunch_seg, nb + 1)) || (rc = dalloc(&h->d_seg_part, (size_t)nseg * kNPart)) ||
+        (rc = dalloc(&h->d_chunk_part, (size_t)nb * max_chunks * kNPart)))
+        return rc;
+    HIPCHK(hipMemcpy(h->d_seg_begin, sb.data(), sb.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_seg_end, se.data(), se.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_seg_bunch, sbunch.data(), sbunch.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_wg_seg, wgseg.data(), wgseg.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_bunch_seg, bseg.data(), bseg.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    h->nwg = nwg;
+    h->nseg = nseg;
+    h->max_chunks = max_chunks;
+    return CTS_OK;
+}
+
 // Moments of the present coordinates into d_cur (MPBeamRMSCal): the first
 // pass fixes the shift (means), the second takes the centred sums about it.
 int enqueue_moments(cts_handle *h)
@@ -742,61 +809,15 @@ int cts_set_bunches(cts_handle *h, int32_t nb, const int64_t *counts, const doub
     h->ntot = h->offsets[nb];
     const int64_t n = h->ntot;
 
-    // workgroups: contiguous chunks (multiples of 64) of the particle range
-    int64_t want = (n + 2047) / 2048;
-    if (want < 1) want = 1;
-    if (want > h->wg_target) want = h->wg_target;
-    int64_t chunk = (n + want - 1) / want;
-    chunk = (chunk + 63) / 64 * 64;
-    if (chunk < 64) chunk = 64;
-    const int nwg = (int)((n + chunk - 1) / chunk) > 0 ? (int)((n + chunk - 1) / chunk) : 1;
-    std::vector<int64_t> sb, se;
-    std::vector<int32_t> sbunch, wgseg(nwg + 1, 0), bseg(nb + 1, 0);
-    int b = 0;
-    for (int w = 0; w < nwg; w++) {
-        wgseg[w] = (int32_t)sb.size();
-        const int64_t lo = (int64_t)w * chunk, hi = std::min<int64_t>(n, lo + chunk);
-        int64_t p = lo;
-        while (p < hi) {
-            while (b < nb && h->offsets[b + 1] <= p) b++;
-            const int64_t e2 = std::min<int64_t>(hi, h->offsets[b + 1]);
-            sb.push_back(p);
-            se.push_back(e2);
-            sbunch.push_back(b);
-            p = e2;
-        }
-    }
-    wgseg[nwg] = (int32_t)sb.size();
-    h->nwg = nwg;
-    h->nseg = (int)sb.size();
-    {
-        int s = 0;
-        for (int bb = 0; bb < nb; bb++) {
-            while (s < h->nseg && sbunch[s] < bb) s++;
-            bseg[bb] = s;
-        }
-        bseg[nb] = h->nseg;
-    }
-    h->max_chunks = 1;
-    for (int bb = 0; bb < nb; bb++) h->max_chunks = std::max(h->max_chunks, (bseg[bb + 1] - bseg[bb] + 63) / 64);
     int rc;
     if ((rc = dalloc(&h->d_x, n)) || (rc = dalloc(&h->d_px, n)) || (rc = dalloc(&h->d_y, n)) ||
         (rc = dalloc(&h->d_py, n)) || (rc = dalloc(&h->d_z, n)) || (rc = dalloc(&h->d_dp, n)) ||
-        (rc = dalloc(&h->d_flag, n)) || (rc = dalloc(&h->d_seg_begin, h->nseg)) ||
-        (rc = dalloc(&h->d_seg_end, h->nseg)) || (rc = dalloc(&h->d_seg_bunch, h->nseg)) ||
-        (rc = dalloc(&h->d_wg_seg, nwg + 1)) || (rc = dalloc(&h->d_bunch_seg, nb + 1)) ||
-        (rc = dalloc(&h->d_bunch_off, nb + 1)) || (rc = dalloc(&h->d_bunch_count, nb)) ||
-        (rc = dalloc(&h->d_seg_part, (size_t)h->nseg * kNPart)) || (rc = dalloc(&h->d_shift, (size_t)nb * 6)) ||
+        (rc = dalloc(&h->d_flag, n)) || (rc = dalloc(&h->d_bunch_off, nb + 1)) ||
+        (rc = dalloc(&h->d_bunch_count, nb)) || (rc = dalloc(&h->d_shift, (size_t)nb * 6)) ||
         (rc = dalloc(&h->d_cur, nb)) || (rc = dalloc(&h->d_sigz, nb)) || (rc = dalloc(&h->d_zrange, (size_t)nb * 2)) ||
-        (rc = dalloc(&h->d_ion_e, (size_t)nb * CTS_MAX_SPECIES * 5)) ||
-        (rc = dalloc(&h->d_chunk_part, (size_t)nb * h->max_chunks * kNPart)) ||
-        (rc = dalloc(&h->d_chunk_cnt, nb)))
+        (rc = dalloc(&h->d_ion_e, (size_t)nb * CTS_MAX_SPECIES * 5)) || (rc = dalloc(&h->d_chunk_cnt, nb)))
         return rc;
-    HIPCHK(hipMemcpy(h->d_seg_begin, sb.data(), sb.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->d_seg_end, se.data(), se.size() * sizeof(int64_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->d_seg_bunch, sbunch.data(), sbunch.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->d_wg_seg, wgseg.data(), wgseg.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->d_bunch_seg, bseg.data(), bseg.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if ((rc = layout_segments(h, h->wg_target))) return rc;
     HIPCHK(hipMemcpy(h->d_bunch_off, h->offsets.data(), (nb + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->d_bunch_count, h->counts.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice));
     h->kick_pending = false;
@@ -1003,6 +1024,91 @@ int cts_turn(cts_handle *h, int32_t turn)
     h->dp_pending = wake;
     h->cur_valid = true;
     return enqueue_status_copy(h);
+}
+
+// Workgroup count of the streaming pass, picked by timing: the candidates
+// are multiples of the number of workgroups the device holds at once (so
+// the last round is not a partial one) and the current setting; each is
+// timed on a copy of the particles with the turn's own kernel variant.
+int cts_autotune(cts_handle *h, int32_t *nwg_out)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if ((rc = flush_kick(h))) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, h->dev));
+    const int cus = prop.multiProcessorCount;
+    // the turn's pass, as cts_turn builds it (with a pending kick)
+    uint32_t ops = OP_MAP | OP_RF | OP_DRIFT | OP_U0 | OP_LOSS | OP_MOMA | OP_WRITE;
+    if (h->ring.skew_k != 0) ops |= OP_SKEW;
+    if (h->synrad_on) ops |= OP_SYNRAD;
+    const bool wake = h->nbins > 0;
+    if (wake) ops |= OP_KICK | OP_MOMB | ((h->wake_mode == 1) ? OP_BIN : OP_ZSTAT);
+    else ops |= OP_MOMD;
+    // scratch copies of the particles and of the histogram
+    const int64_t n = h->ntot;
+    double *cx[6] = {};
+    uint8_t *cf = nullptr;
+    uint32_t *chist = nullptr;
+    auto release = [&]() {
+        for (double *p : cx)
+            if (p) (void)hipFree(p);
+        if (cf) (void)hipFree(cf);
+        if (chist) (void)hipFree(chist);
+    };
+    double *src[6] = {h->d_x, h->d_px, h->d_y, h->d_py, h->d_z, h->d_dp};
+    for (int c = 0; c < 6 && !rc; c++) {
+        rc = dalloc(&cx[c], (size_t)n);
+        if (!rc && hipMemcpy(cx[c], src[c], (size_t)n * sizeof(double), hipMemcpyDeviceToDevice) != hipSuccess)
+            rc = fail(CTS_ERR_HIP, "autotune copy");
+    }
+    if (!rc) rc = dalloc(&cf, (size_t)n);
+    if (!rc && hipMemcpy(cf, h->d_flag, (size_t)n, hipMemcpyDeviceToDevice) != hipSuccess)
+        rc = fail(CTS_ERR_HIP, "autotune copy");
+    if (!rc && wake) rc = dalloc(&chist, (size_t)h->nb * h->nbins);
+    if (rc) {
+        release();
+        return rc;
+    }
+    std::vector<int> cand;
+    for (int m = 1; m <= 8; m++) cand.push_back(cus * m);
+    cand.push_back(h->wg_target);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int best = h->wg_target;
+    float best_ms = 1e30f;
+    for (int c : cand) {
+        if ((rc = layout_segments(h, c))) break;
+        PassParams p = base_pass(h, ops, 0);
+        p.x = cx[0]; p.px = cx[1]; p.y = cx[2]; p.py = cx[3]; p.z = cx[4]; p.dp = cx[5];
+        p.flag = cf;
+        p.bunch_hist = chist;
+        hipError_t e = launch_pass(p, h->nwg, h->stream);
+        if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
+        for (int r = 0; r < 5 && e == hipSuccess; r++) e = launch_pass(p, h->nwg, h->stream);
+        if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+            rc = fail(CTS_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+            break;
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    release();
+    if (rc) return rc;
+    h->wg_target = best;
+    if ((rc = layout_segments(h, best))) return rc;
+    if (nwg_out) *nwg_out = h->nwg;
+    return CTS_OK;
 }
 
 int cts_flush(cts_handle *h)

@@ -105,6 +105,11 @@ class Tracker:
     def turn(self, n):
         return capi.check(self.lib.cts_turn(self.h, n))
 
+    def autotune(self):
+        n = ctypes.c_int32()
+        capi.check(self.lib.cts_autotune(self.h, ctypes.byref(n)))
+        return n.value
+
     def flush(self):
         capi.check(self.lib.cts_flush(self.h))
 

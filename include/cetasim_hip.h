@@ -170,6 +170,11 @@ int cts_set_ions(cts_handle *h, const cts_ion_config *cfg);
 int cts_turn(cts_handle *h, int32_t turn);
 /* Apply a pending wake kick and complete the last turn's moments. */
 int cts_flush(cts_handle *h);
+/* Autotuner (no counterpart in the reference): times the turn's streaming
+ * pass on a copy of the particles for workgroup counts that are multiples
+ * of the device's compute units, and keeps the fastest layout; *nwg_out
+ * gets the chosen count. */
+int cts_autotune(cts_handle *h, int32_t *nwg_out);
 
 /* ------------------------------------------- seams, one kernel each
  * MPBeam::BeamTransferPerInteractionPointDueToLatticeT (src/MPBeam.cpp:2613) */

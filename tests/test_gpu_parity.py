@@ -297,3 +297,28 @@ def test_full_size_properties():
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1], outs[1][1])
     assert np.array_equal(outs[0][2], outs[1][2])
+
+
+def test_autotuned_layout_keeps_parity():
+    """The autotuner only changes the workgroup layout: turns after it still
+    match the oracle (and its timing runs leave the particles untouched)."""
+    (ring, secs, _), p = lattice()
+    parts = configs.gaussian_bunch(30_000, 31, p)
+    tr = tracker(ring, secs, parts, p=p)
+    sz = p["sig"][2]
+    q, _ = configs.bunch_scalars([30_000], ring, p)
+    tr.set_wake(100, 1, -5 * sz, 5 * sz, BBR["rs"], BBR["q"], BBR["fr"])
+    tr.set_synrad(True, 2, 3)
+    nwg = tr.autotune()
+    assert nwg >= 1
+    before = tr.download()
+    for c in COORDS:
+        assert np.array_equal(before[c], parts[c])
+    ot = O.OracleTurn(ring, secs[0], nbins=100, mode=1, grid=(-5 * sz, 5 * sz), bbr=BBR, macro_charge=q[0],
+                      synrad=True, noise_seed=3)
+    ob = O.Bunch(parts)
+    for t in range(3):
+        tr.turn(t)
+        ot.turn(ob, t)
+    assert_coords(tr.download(), ob, 1e-11)
+    tr.close()
