@@ -1,0 +1,790 @@
+// cetasim_model.cpp -- host classes of the tracking path over the MI355X
+// library's C ABI (see cetasim_model.hpp).
+#include "cetasim_model.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+
+namespace cetasim {
+
+namespace {
+
+constexpr double kPi = 3.141592653589793;
+constexpr double kClight = 299792458.0;
+constexpr double kEcharge = 1.602017733e-19;
+
+// key (blanks removed, lower case) and the tokens after '='
+// (StringVecSplit, src/Global.cpp:210-260)
+bool split_line(const std::string &line, std::string &key, std::vector<std::string> &vals)
+{
+    const size_t eq = line.find('=');
+    if (line.empty() || eq == std::string::npos) return false;
+    key.clear();
+    for (size_t i = 0; i < eq; i++)
+        if (!std::isblank((unsigned char)line[i])) key.push_back((char)std::tolower((unsigned char)line[i]));
+    vals.clear();
+    std::istringstream rest(line.substr(eq + 1));
+    std::string tok;
+    while (rest >> tok) vals.push_back(tok);
+    return true;
+}
+
+double num(const std::vector<std::string> &v, size_t i)
+{
+    if (i >= v.size()) throw std::runtime_error("missing value");
+    return std::stod(v[i]);  // parses the leading number, as the reference's stod
+}
+
+int inum(const std::vector<std::string> &v, size_t i)
+{
+    if (i >= v.size()) throw std::runtime_error("missing value");
+    return std::stoi(v[i]);
+}
+
+uint64_t splitmix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- input.dat
+int ReadInputSettings::ParamRead(int argc, char *argv[])
+{
+    return ParamReadFile(argc >= 2 ? argv[1] : "input.dat");
+}
+
+int ReadInputSettings::ParamReadFile(const std::string &path)
+{
+    std::ifstream fin(path);
+    if (!fin.is_open()) {
+        std::cerr << "Error opening file " << path << std::endl;
+        return 1;
+    }
+    std::string line, k;
+    std::vector<std::string> v;
+    int lineno = 0;
+    while (std::getline(fin, line)) {
+        lineno++;
+        if (!split_line(line, k, v)) continue;
+        try {
+            RingParBasic &B = basic_;
+            if (k == "ringcircring") B.circRing = num(v, 0);
+            else if (k == "ringworkq") { B.workQx = num(v, 0); B.workQy = num(v, 1); B.workQz = num(v, 2); }
+            else if (k == "ringchrom") { B.chrom[0] = num(v, 0); B.chrom[1] = num(v, 1); }
+            else if (k == "ringsynchraddamptime") for (int i = 0; i < 3; i++) B.synchRadDampTime[i] = num(v, i);
+            else if (k == "ringskewquadk") B.skewQuadK = num(v, 0);
+            else if (k == "ringpipeaperaturex") B.pipeAperature[1] = num(v, 0);
+            else if (k == "ringpipeaperaturey") B.pipeAperature[2] = num(v, 0);
+            else if (k == "ringpipeaperaturer") B.pipeAperature[0] = num(v, 0);
+            else if (k == "ringelectronbeamenergy") B.electronBeamEnergy = num(v, 0);
+            else if (k == "ringalphac") for (int i = 0; i < 3; i++) B.alphac[i] = num(v, i);
+            else if (k == "ringadtx") for (int i = 0; i < 5; i++) B.aDTX[i] = num(v, i);  // Q8: five values
+            else if (k == "ringadty") for (int i = 0; i < 5; i++) B.aDTY[i] = num(v, i);
+            else if (k == "ringu0") B.u0 = num(v, 0);
+            else if (k == "ringsdelta0") B.sdelta0 = num(v, 0);
+            else if (k == "ringcurrent") B.ringCurrent = num(v, 0);
+            else if (k == "ringnaturalemit") B.naturalEmit = num(v, 0);
+            else if (k == "ringcoupling") B.couplingfactorXY = num(v, 0);
+            else if (k == "ringtwiss") B.twissInput = v.at(0);
+            else if (k == "ringsectnum") B.ringSectNum = inum(v, 0);
+            // rf (sized by rfresnum, which must come first)
+            else if (k == "rfringharm") rf_.ringHarm = inum(v, 0);
+            else if (k == "rfresnum") {
+                rf_.resNum = inum(v, 0);
+                rf_.resHarm.assign(rf_.resNum, 1);
+                rf_.rfMode.assign(rf_.resNum, 0);
+                rf_.resVol.assign(rf_.resNum, 0.0);
+                rf_.resPhase.assign(rf_.resNum, 0.0);
+            } else if (k == "rfresharm") for (int i = 0; i < rf_.resNum; i++) rf_.resHarm[i] = inum(v, i);
+            else if (k == "rfmode") for (int i = 0; i < rf_.resNum; i++) rf_.rfMode[i] = inum(v, i);
+            else if (k == "rfresvol") for (int i = 0; i < rf_.resNum; i++) rf_.resVol[i] = num(v, i);
+            else if (k == "rfresphase") for (int i = 0; i < rf_.resNum; i++) rf_.resPhase[i] = num(v, i);
+            // filling pattern (sized by fptrainnumber)
+            else if (k == "fptotbunchnumber") fill_.totBunchNumber = inum(v, 0);
+            else if (k == "fptrainnumber") {
+                fill_.trainNumber = inum(v, 0);
+                fill_.bunchNumberPerTrain.assign(fill_.trainNumber, 0);
+                fill_.trainGaps.assign(fill_.trainNumber, 0);
+                fill_.bunchGaps.assign(fill_.trainNumber, 0);
+            } else if (k == "fpbunchnumberpertrain") for (int i = 0; i < fill_.trainNumber; i++) fill_.bunchNumberPerTrain[i] = inum(v, i);
+            else if (k == "fptraingaps") for (int i = 0; i < fill_.trainNumber; i++) fill_.trainGaps[i] = inum(v, i);
+            else if (k == "fpbunchgaps") for (int i = 0; i < fill_.trainNumber; i++) fill_.bunchGaps[i] = inum(v, i);
+            else if (k == "fpsetbunchchargenum") {
+                fill_.bunchChargeNum = inum(v, 0);
+                fill_.bunchChargeIndex.assign(fill_.bunchChargeNum, 0);
+                fill_.bunchCharge.assign(fill_.bunchChargeNum, 1);
+            } else if (k == "fpsetbunchchargeindex") for (int i = 0; i < fill_.bunchChargeNum; i++) fill_.bunchChargeIndex[i] = inum(v, i);
+            else if (k == "fpsetbunchcharge") for (int i = 0; i < fill_.bunchChargeNum; i++) fill_.bunchCharge[i] = inum(v, i);
+            // bunch
+            else if (k == "bunchcurrent") bunch_.current = num(v, 0);
+            else if (k == "bunchemittance") bunch_.emittanceX = num(v, 0);
+            else if (k == "bunchkappa") bunch_.kappa = num(v, 0);
+            else if (k == "bunchrmsbunchlength") bunch_.rmsBunchLength = num(v, 0);
+            else if (k == "bunchrmsenergyspread") bunch_.rmsEnergySpread = num(v, 0);
+            else if (k == "bunchdistributiontype") bunch_.distributionType = (int)num(v, 0);
+            else if (k == "bunchmacroelenumperbunch") bunch_.macroEleNumPerBunch = inum(v, 0);
+            else if (k == "bunchinitialstaticoffset") for (int i = 0; i < 6; i++) bunch_.initialStaticOffSet[i] = num(v, i);
+            else if (k == "bunchinitialdynamicoffset") for (int i = 0; i < 6; i++) bunch_.initialDynamicOffSet[i] = num(v, i);
+            // ions (sized by ioncalgasspec)
+            else if (k == "ioncalgasspec") {
+                ion_.gasSpec = inum(v, 0);
+                ion_.ionMassNumber.assign(ion_.gasSpec, 0.0);
+                ion_.gasPercent.assign(ion_.gasSpec, 0.0);
+                ion_.corssSectionEI.assign(ion_.gasSpec, 0.0);
+            } else if (k == "ioncalionmassnumber") for (int i = 0; i < ion_.gasSpec; i++) ion_.ionMassNumber[i] = num(v, i);
+            else if (k == "ioncalcorsssectionei") for (int i = 0; i < ion_.gasSpec; i++) ion_.corssSectionEI[i] = num(v, i) * 1.e-22;
+            else if (k == "ioncaliongaspercent") for (int i = 0; i < ion_.gasSpec; i++) ion_.gasPercent[i] = num(v, i);
+            else if (k == "ioncalionmaxnumber") ion_.ionMaxNumber = num(v, 0);
+            else if (k == "ioncalionlossboundary") ion_.ionLossBoundary = num(v, 0);
+            else if (k == "ioncalmacroionnumbergeneratedperip") ion_.macroIonNumberGeneratedPerIP = (int)num(v, 0);
+            // short-range wake
+            else if (k == "srwbbrinput") srw_.SRWBBRInput = v.at(0);
+            else if (k == "srwbunchbinnum") srw_.SRWBunchBinNum = (int)num(v, 0);
+            // run
+            else if (k == "runcalsetting") run_.calSetting = inum(v, 0);
+            else if (k == "runnturns") run_.nTurns = inum(v, 0);
+            else if (k == "runbunchinfoprintinterval") run_.bunchInfoPrintInterval = inum(v, 0);
+            else if (k == "runsynraddampingflag") run_.synRadDampingFlag = inum(v, 0);
+            else if (k == "runbeamionflag") run_.beamIonFlag = inum(v, 0);
+            else if (k == "runshortrangewake") run_.sRWakeFlag = inum(v, 0);
+            else if (k == "rungrowthratefittingstart") run_.growthRateFittingStart = inum(v, 0);
+            else if (k == "rungrowthratefittingend") run_.growthRateFittingEnd = inum(v, 0);
+            else if (k == "runtbtbunchaverdata") run_.TBTBunchAverData = v.at(0);
+            else if (k == "runcbmgr") run_.runCBMGR = v.at(0);
+            else if (k == "runseed") run_.seed = (uint64_t)std::stoull(v.at(0));
+        } catch (const std::exception &) {
+            std::cerr << path << ":" << lineno << ": bad value for " << k << std::endl;
+            return 1;
+        }
+    }
+    // the growth-rate window is checked even when unused
+    // (src/ReadInputSettings.cpp:946-956)
+    if (run_.nTurns > 0 && !(run_.growthRateFittingStart < run_.growthRateFittingEnd &&
+                             run_.growthRateFittingEnd <= run_.nTurns)) {
+        std::cerr << "growth rate fitting window has to satisfy start < end <= nTurns" << std::endl;
+        return 1;
+    }
+    Derive();
+    return 0;
+}
+
+// src/ReadInputSettings.cpp:986-1077 through cts_derive_ring
+void ReadInputSettings::Derive()
+{
+    ion_.numberofIonBeamInterPoint = basic_.ringSectNum;
+    ion_.twissInput = basic_.twissInput;
+    bunch_.emittanceY = bunch_.emittanceX / (1 + bunch_.kappa) * bunch_.kappa;
+    bunch_.emittanceX = bunch_.emittanceX / (1 + bunch_.kappa);
+    cts_ring_input in{};
+    in.circ = basic_.circRing;
+    in.energy = basic_.electronBeamEnergy;
+    in.u0 = basic_.u0;
+    in.sdelta0 = basic_.sdelta0;
+    for (int i = 0; i < 3; i++) in.alphac[i] = basic_.alphac[i];
+    in.nux = basic_.workQx;
+    in.nuy = basic_.workQy;
+    in.chrom[0] = basic_.chrom[0];
+    in.chrom[1] = basic_.chrom[1];
+    for (int i = 0; i < 5; i++) { in.adtx[i] = basic_.aDTX[i]; in.adty[i] = basic_.aDTY[i]; }
+    in.skew_k = basic_.skewQuadK;
+    for (int i = 0; i < 3; i++) in.synrad_tau[i] = basic_.synchRadDampTime[i];
+    in.natural_emit = basic_.naturalEmit;
+    in.coupling = basic_.couplingfactorXY;
+    in.harmonic = rf_.ringHarm;
+    in.rf_volt = rf_.resVol.empty() ? 0.0 : rf_.resVol[0];
+    in.ap_x = basic_.pipeAperature[1];
+    in.ap_y = basic_.pipeAperature[2];
+    cts_ring ring{};
+    cts_ring_derived der{};
+    if (cts_derive_ring(&in, &ring, &der) == CTS_OK) {
+        basic_.rGamma = ring.rgamma;
+        basic_.rBeta = ring.rbeta;
+        basic_.t0 = ring.t0;
+        basic_.f0 = ring.f0;
+        basic_.harmonics = ring.harmonic;
+        basic_.eta = der.eta;
+        basic_.workQz = der.workqz;  // the input value is replaced (Q9)
+        basic_.naturalBunchLength = der.natural_bunch_length;
+        for (int i = 0; i < 3; i++) basic_.emitNat[i] = ring.emit_nat[i];
+    }
+}
+
+// ----------------------------------------------------------------- train
+int Train::Initial(ReadInputSettings &in)
+{
+    const int harmonics = in.ringParBasic->harmonics;
+    trainNumber = in.ringFillPatt->trainNumber;
+    totBunchNum = in.ringFillPatt->totBunchNumber;
+    bunchNumberPerTrain = in.ringFillPatt->bunchNumberPerTrain;
+    trainGaps = in.ringFillPatt->trainGaps;
+    bunchGaps = in.ringFillPatt->bunchGaps;
+    int nCheck = 0, hCheck = 0;
+    for (int i = 0; i < trainNumber; i++) {
+        nCheck += bunchNumberPerTrain[i];
+        hCheck += bunchNumberPerTrain[i] * (bunchGaps[i] + 1) + trainGaps[i];
+    }
+    if (nCheck != totBunchNum || hCheck != harmonics) {
+        std::cerr << "mistakes in settings for bunch filling pattern, sum of gaps + totBunchNumber has to be harmonics"
+                  << std::endl;
+        return 1;
+    }
+    trainStart.assign(trainNumber, 0);
+    for (int i = 1; i < trainNumber; i++)
+        for (int j = 0; j < i; j++) trainStart[i] += bunchNumberPerTrain[j] * (bunchGaps[i] + 1) + trainGaps[j];
+    return 0;
+}
+
+// --------------------------------------------------------------- lattice
+int LatticeInterActionPoint::Initial(const ReadInputSettings &in)
+{
+    const auto *B = in.ringParBasic;
+    numberOfInteraction = in.ringIonEffPara->numberofIonBeamInterPoint;
+    gasSpec = in.ringIonEffPara->gasSpec;
+    ionMassNumber = in.ringIonEffPara->ionMassNumber;
+    corssSectionEI = in.ringIonEffPara->corssSectionEI;
+    gasPercent = in.ringIonEffPara->gasPercent;
+    ionLossBoundary = in.ringIonEffPara->ionLossBoundary;
+    ionMaxNumberOneInterPoint = in.ringIonEffPara->ionMaxNumber;
+    if (gasSpec > 0) {
+        double tot = 0;
+        for (double p : gasPercent) tot += p;
+        if (tot != 1.e0) {  // src/LatticeInterActionPoint.cpp:56-65
+            std::cerr << "non 100 percent gas" << std::endl;
+            return 1;
+        }
+    }
+    std::ifstream fin(B->twissInput);
+    if (!fin.is_open()) {
+        std::cerr << "Error opening file " << B->twissInput << std::endl;
+        return 1;
+    }
+    std::string str;
+    int index = 0;
+    twissBetaZ = B->naturalBunchLength / B->sdelta0;
+    while (std::getline(fin, str)) {
+        if (index++ <= 4) continue;  // five header lines
+        std::istringstream ls(str);
+        std::vector<double> c;
+        std::string tok;
+        while (ls >> tok) c.push_back(std::stod(tok));
+        if (c.empty()) continue;
+        if (c.size() < 16) {
+            std::cerr << B->twissInput << ": 16 columns expected" << std::endl;
+            return 1;
+        }
+        elemPos.push_back(c[0]);
+        twissBetaX.push_back(c[1]); twissAlphaX.push_back(c[2]); xPhaseAdv.push_back(c[3]);
+        twissDispX.push_back(c[4]); twissDispPX.push_back(c[5]);
+        pipeAperatureX.push_back(std::min(c[6], B->pipeAperature[1]));
+        twissBetaY.push_back(c[7]); twissAlphaY.push_back(c[8]); yPhaseAdv.push_back(c[9]);
+        twissDispY.push_back(c[10]); twissDispPY.push_back(c[11]);
+        pipeAperatureY.push_back(std::min(c[12], B->pipeAperature[2]));
+        interactionLength.push_back(c[13]);
+        vacuumPressure.push_back(c[14] * 1.0E-9 * 133.3224);  // nTorr -> Pa
+        temperature.push_back(c[15]);
+    }
+    const int n = (int)elemPos.size();
+    if (n != numberOfInteraction) {
+        std::cerr << "data of rintSecNum in input.dat and InterPointParameter.dat files does not match" << std::endl;
+        return 1;
+    }
+    phaseAdvX12.assign(n, 0.0);
+    phaseAdvY12.assign(n, 0.0);
+    for (int i = 0; i < n; i++) {
+        if (i != n - 1) {
+            interactionLength[i] = elemPos[i + 1] - elemPos[i];
+            phaseAdvX12[i] = xPhaseAdv[i + 1] - xPhaseAdv[i];
+            phaseAdvY12[i] = yPhaseAdv[i + 1] - yPhaseAdv[i];
+        } else {
+            interactionLength[i] = B->circRing - elemPos[i] + elemPos[0];
+            phaseAdvX12[i] = 2 * kPi * B->workQx - xPhaseAdv[i] + xPhaseAdv[0];
+            phaseAdvY12[i] = 2 * kPi * B->workQy - yPhaseAdv[i] + yPhaseAdv[0];
+        }
+    }
+    sections.assign(n, cts_section{});
+    for (int k = 0; k < n; k++) {
+        const int k2 = (k + 1) % n;
+        cts_twiss t1{twissBetaX[k], twissAlphaX[k], twissDispX[k], twissDispPX[k],
+                     twissBetaY[k], twissAlphaY[k], twissDispY[k], twissDispPY[k], twissBetaZ};
+        cts_twiss t2{twissBetaX[k2], twissAlphaX[k2], twissDispX[k2], twissDispPX[k2],
+                     twissBetaY[k2], twissAlphaY[k2], twissDispY[k2], twissDispPY[k2], twissBetaZ};
+        if (cts_build_section(&t1, &t2, phaseAdvX12[k], phaseAdvY12[k], &sections[k]) != CTS_OK) return 1;
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------ wake
+int WakeFunction::BBRWakeParaReadIn(const std::string &path)
+{
+    std::ifstream fin(path);
+    if (!fin.is_open()) {
+        std::cerr << "Error opening file " << path << std::endl;
+        return 1;
+    }
+    std::string str;
+    int index = 0;
+    while (std::getline(fin, str)) {
+        if (index++ < 12) continue;  // twelve header lines
+        std::istringstream ls(str);
+        std::vector<double> c;
+        std::string tok;
+        while (ls >> tok) c.push_back(std::stod(tok));
+        if (c.empty()) continue;
+        if (c.size() < 9) {
+            std::cerr << path << ": 9 columns expected (Q11)" << std::endl;
+            return 1;
+        }
+        lRs.push_back(c[0]);
+        lQ.push_back(c[1]);
+        lOmega.push_back(c[2] * 2 * kPi);
+        txRs.push_back(c[3]);
+        txQ.push_back(c[4]);
+        txOmega.push_back(c[5] * 2 * kPi);
+        tyRs.push_back(c[6]);
+        tyQ.push_back(c[7]);
+        tyOmega.push_back(c[8] * 2 * kPi);
+    }
+    return 0;
+}
+
+int WakeFunction::InitialSRWake(const ReadInputSettings &in)
+{
+    if (in.ringSRWake->SRWBBRInput.empty()) return 0;
+    return BBRWakeParaReadIn(in.ringSRWake->SRWBBRInput);
+}
+
+// ---------------------------------------------------------- distribution
+namespace {
+
+// Uniform [0, 1) and standard normal draws from a splitmix64 counter stream.
+struct SeededStream {
+    uint64_t key, ctr = 0;
+    explicit SeededStream(uint64_t k) : key(k) {}
+    double uniform() { return (double)(splitmix64(key + ctr++) >> 11) * 0x1p-53; }
+    double normal()
+    {
+        const double u1 = 1.0 - uniform(), u2 = uniform();
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(2 * kPi * u2);
+    }
+};
+
+// Root of 1 - f - (1 + a) e^{-a} on [0, 20] by bisection: the action of a
+// Gaussian beam with cumulative fraction f (MPBunch::GSSlover,
+// src/MPBunch.cpp:300-338: same bracket, tolerances and early exit).
+double gaussian_action(double f)
+{
+    double lo = 0.0, hi = 20.0;
+    while (hi - lo > 1.0e-5) {
+        const double mid = 0.5 * (lo + hi);
+        const double g = 1 - f - (1 + mid) * std::exp(-mid);
+        if (std::abs(g) < 1.0e-6) return mid;
+        if (g < 0) lo = mid;
+        else if (g > 0) hi = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+}  // namespace
+
+int DistriGenerator(MPBunch &b, const LatticeInterActionPoint &lat, const ReadInputSettings &in, int bunchIndex)
+{
+    const auto *P = in.ringBunchPara;
+    const int n = b.macroEleNumPerBunch;
+    const double rBeta = in.ringParBasic->rBeta;
+    b.ePositionX.assign(n, 0.0); b.eMomentumX.assign(n, 0.0); b.ePositionY.assign(n, 0.0);
+    b.eMomentumY.assign(n, 0.0); b.ePositionZ.assign(n, 0.0); b.eMomentumZ.assign(n, 0.0);
+    b.eSurive.assign(n, 0);
+    SeededStream rng(splitmix64(in.ringRun->seed ^ ((uint64_t)(bunchIndex + 1) << 40)));
+    double off[6];
+    for (int i = 0; i < 6; i++) off[i] = rng.normal() * P->initialDynamicOffSet[i] + P->initialStaticOffSet[i];
+    const double sz = P->rmsBunchLength, sd = P->rmsEnergySpread;
+    // longitudinal: Gaussian truncated at 3 sigma in (z, dE/E)
+    for (int i = 0; i < n;) {
+        const double z = rng.normal() * sz, e = rng.normal() * sd;
+        if ((z / sz) * (z / sz) + (e / sd) * (e / sd) < 9.0) {
+            b.ePositionZ[i] = z;
+            b.eMomentumZ[i] = e / (rBeta * rBeta);
+            i++;
+        }
+    }
+    // transverse: action-angle sampling at interaction point 0
+    const double ax = lat.twissAlphaX[0], ay = lat.twissAlphaY[0], bx = lat.twissBetaX[0], by = lat.twissBetaY[0];
+    const double sgx = std::sqrt(bx), sgy = std::sqrt(by), dsx = -ax / std::sqrt(bx), dsy = -ay / std::sqrt(by);
+    const double ex = P->emittanceX, rX = std::sqrt(P->emittanceX * bx), rY = std::sqrt(P->emittanceY * by);
+    const int type = P->distributionType;
+    if (type < 1 || type > 3) {
+        std::cerr << "no distribution type " << type << std::endl;
+        return 1;
+    }
+    for (int i = 0; i < n;) {
+        double fi;
+        const double u = rng.uniform();
+        if (type == 1) fi = 4 * ex;
+        else if (type == 2) fi = 6 * ex * std::sqrt(u);
+        else fi = gaussian_action(u) * 2 * ex;
+        const double axax = fi * rng.uniform();
+        const double ayay = (fi - axax) * P->kappa;
+        const double a_x = std::sqrt(axax), a_y = std::sqrt(ayay);
+        const double phx = 2 * kPi * rng.uniform(), phy = 2 * kPi * rng.uniform();
+        const double x = a_x * sgx * std::cos(phx), y = a_y * sgy * std::cos(phy);
+        if (type == 3 && (x / rX) * (x / rX) + (y / rY) * (y / rY) > 9) continue;
+        b.ePositionX[i] = x;
+        b.ePositionY[i] = y;
+        b.eMomentumX[i] = a_x * (dsx * std::cos(phx) - std::sin(phx) / sgx);
+        b.eMomentumY[i] = a_y * (dsy * std::cos(phy) - std::sin(phy) / sgy);
+        i++;
+    }
+    // offsets and dispersion
+    for (int i = 0; i < n; i++) {
+        b.ePositionZ[i] += off[2];
+        b.eMomentumZ[i] += off[5];
+        b.ePositionX[i] += off[0] + lat.twissDispX[0] * b.eMomentumZ[i];
+        b.ePositionY[i] += off[1] + lat.twissDispY[0] * b.eMomentumZ[i];
+        b.eMomentumX[i] += off[3] + lat.twissDispPX[0] * b.eMomentumZ[i];
+        b.eMomentumY[i] += off[4] + lat.twissDispPY[0] * b.eMomentumZ[i];
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------ beam
+MPBeam::~MPBeam()
+{
+    if (gpu) cts_destroy(gpu);
+}
+
+int MPBeam::check(int rc, const char *what)
+{
+    if (rc == CTS_OK) return 0;
+    if (rc == CTS_LOW_TRANSMISSION)
+        std::cerr << "more than 50% particles are lost in a bunch (" << what << ")" << std::endl;
+    else if (rc == CTS_ION_OVERFLOW)
+        std::cerr << "the accumulated ions are larger than the limit; reduce the ions generated per interaction "
+                     "or enlarge ionCalIonMaxNumber (" << what << ")" << std::endl;
+    else
+        std::cerr << what << ": " << cts_last_error() << std::endl;
+    return rc;
+}
+
+int MPBeam::InitialBunches(Train &train, LatticeInterActionPoint &lat, ReadInputSettings &in)
+{
+    const auto *B = in.ringParBasic;
+    const int nb = in.ringFillPatt->totBunchNumber;
+    beamVec.assign(nb, MPBunch{});
+    int counter = 0;
+    for (int i = 0; i < train.trainNumber; i++)
+        for (int j = 0; j < train.bunchNumberPerTrain[i]; j++)
+            beamVec[counter++].bunchHarmNum = train.trainStart[i] + j * (train.bunchGaps[i] + 1);
+    for (int i = 0; i < in.ringFillPatt->bunchChargeNum; i++) {
+        const int idx = in.ringFillPatt->bunchChargeIndex[i];
+        if (idx < 0 || idx >= nb) {
+            std::cerr << "fpSetBunchChargeIndex " << idx << " is not a bunch index" << std::endl;
+            return 1;
+        }
+        beamVec[idx].normCurrent = in.ringFillPatt->bunchCharge[i];
+    }
+    int totNorm = 0;
+    for (auto &b : beamVec) totNorm += b.normCurrent;
+    for (int i = 0; i < nb; i++) {
+        MPBunch &b = beamVec[i];
+        b.current = b.normCurrent * B->ringCurrent / totNorm;
+        b.macroEleNumPerBunch = in.ringBunchPara->macroEleNumPerBunch;
+        b.electronNumPerBunch = b.current / B->f0 / kEcharge;  // src/Bunch.cpp:69-70
+        b.macroEleCharge = b.electronNumPerBunch / b.macroEleNumPerBunch;
+        b.rmsBunchLength = in.ringBunchPara->rmsBunchLength;
+        b.rmsEnergySpread = in.ringBunchPara->rmsEnergySpread;
+        if (DistriGenerator(b, lat, in, i)) return 1;
+    }
+    for (int i = 0; i < nb - 1; i++) beamVec[i].bunchGap = beamVec[i + 1].bunchHarmNum - beamVec[i].bunchHarmNum;
+    beamVec[nb - 1].bunchGap = B->harmonics - beamVec[nb - 1].bunchHarmNum;
+    return 0;
+}
+
+int MPBeam::DeviceRing(const LatticeInterActionPoint &lat, const ReadInputSettings &in, cts_ring &ring)
+{
+    const auto *B = in.ringParBasic;
+    cts_ring_input ri{};
+    ri.circ = B->circRing; ri.energy = B->electronBeamEnergy; ri.u0 = B->u0; ri.sdelta0 = B->sdelta0;
+    for (int i = 0; i < 3; i++) ri.alphac[i] = B->alphac[i];
+    ri.nux = B->workQx; ri.nuy = B->workQy; ri.chrom[0] = B->chrom[0]; ri.chrom[1] = B->chrom[1];
+    for (int i = 0; i < 5; i++) { ri.adtx[i] = B->aDTX[i]; ri.adty[i] = B->aDTY[i]; }
+    ri.skew_k = B->skewQuadK;
+    for (int i = 0; i < 3; i++) ri.synrad_tau[i] = B->synchRadDampTime[i];
+    ri.natural_emit = B->naturalEmit; ri.coupling = B->couplingfactorXY;
+    ri.harmonic = B->harmonics;
+    ri.rf_volt = in.ringParRf->resVol.empty() ? 0.0 : in.ringParRf->resVol[0];
+    ri.ap_x = lat.pipeAperatureX[0]; ri.ap_y = lat.pipeAperatureY[0];
+    cts_ring_derived der{};
+    if (check(cts_derive_ring(&ri, &ring, &der), "cts_derive_ring")) return 1;
+    // cavities (CavityResonator::Initial, src/CavityResonator.cpp:60-139):
+    // one cavity takes the phase that restores U0, several keep their
+    // input phases; only ideal (rfMode 0) cavities are on this path
+    const auto *RF = in.ringParRf;
+    if (RF->resNum > CTS_MAX_RF) {
+        std::cerr << "at most " << CTS_MAX_RF << " cavities" << std::endl;
+        return 1;
+    }
+    for (int i = 0; i < RF->resNum; i++)
+        if (RF->rfMode[i] != 0) {
+            std::cerr << "cavity " << i << ": rfMode " << RF->rfMode[i]
+                      << " (beam loading) is not supported; use rfMode 0" << std::endl;
+            return 1;
+        }
+    if (RF->resNum > 1) {
+        ring.n_rf = RF->resNum;
+        for (int i = 0; i < RF->resNum; i++) {
+            ring.rf_vr[i] = RF->resVol[i] * std::cos(RF->resPhase[i]);
+            ring.rf_vi[i] = RF->resVol[i] * std::sin(RF->resPhase[i]);
+            ring.rf_harm[i] = RF->resHarm[i];
+        }
+    }
+    return 0;
+}
+
+int MPBeam::Initial(Train &train, LatticeInterActionPoint &lat, ReadInputSettings &in)
+{
+    const int nb = in.ringFillPatt->totBunchNumber;
+    if (InitialBunches(train, lat, in)) return 1;
+
+    cts_ring ring{};
+    if (DeviceRing(lat, in, ring)) return 1;
+    if (check(cts_create(&ring, lat.sections.data(), (int32_t)lat.sections.size(), &gpu), "cts_create")) return 1;
+    std::vector<int64_t> counts(nb);
+    std::vector<double> q(nb), ne(nb);
+    std::vector<int32_t> gap(nb);
+    for (int i = 0; i < nb; i++) {
+        counts[i] = beamVec[i].macroEleNumPerBunch;
+        q[i] = beamVec[i].macroEleCharge;
+        ne[i] = beamVec[i].electronNumPerBunch;
+        gap[i] = beamVec[i].bunchGap;
+    }
+    if (check(cts_set_bunches(gpu, nb, counts.data(), q.data(), ne.data(), gap.data()), "cts_set_bunches")) return 1;
+    std::vector<double> X[6];
+    for (auto &b : beamVec) {
+        const std::vector<double> *src[6] = {&b.ePositionX, &b.eMomentumX, &b.ePositionY,
+                                             &b.eMomentumY, &b.ePositionZ, &b.eMomentumZ};
+        for (int c = 0; c < 6; c++) X[c].insert(X[c].end(), src[c]->begin(), src[c]->end());
+    }
+    return check(cts_upload(gpu, X[0].data(), X[1].data(), X[2].data(), X[3].data(), X[4].data(), X[5].data(),
+                            nullptr), "cts_upload");
+}
+
+int MPBeam::BeamTransferPerInteractionPointDueToLatticeT(int k) { return check(cts_section_map(gpu, k), "lattice"); }
+int MPBeam::BeamTransferDueToSkewQuad() { return check(cts_skew_quad(gpu), "skew quad"); }
+int MPBeam::BeamMomtumUpdateDueToRFTest() { return check(cts_rf_kick(gpu), "rf"); }
+int MPBeam::BeamLongiPosTransferOneTurn() { return check(cts_drift(gpu), "drift"); }
+int MPBeam::BeamEnergyLossOneTurn() { return check(cts_energy_loss(gpu), "energy loss"); }
+int MPBeam::BeamSynRadDamping(int turn) { return check(cts_synrad(gpu, turn), "synrad"); }
+int MPBeam::SSBeamIonEffectOneInteractionPoint(int turn) { return check(cts_ion_phase(gpu, turn), "ions"); }
+
+int MPBeam::SRWakeBeamIntaction()
+{
+    std::vector<double> sz(beamVec.size());
+    for (size_t i = 0; i < beamVec.size(); i++) sz[i] = beamVec[i].rmsBunchLength;
+    return check(cts_sr_wake(gpu, sz.data()), "sr wake");
+}
+
+int MPBeam::MarkParticleLostInBunch()
+{
+    std::vector<double> t(beamVec.size());
+    const int rc = check(cts_mark_lost(gpu, t.data()), "mark lost");
+    for (size_t i = 0; i < beamVec.size(); i++) beamVec[i].transmission = t[i];
+    return rc;
+}
+
+int MPBeam::MPBeamRMSCal(int k)
+{
+    std::vector<cts_moments> m(beamVec.size());
+    if (check(cts_rms(gpu, m.data()), "rms")) return 1;
+    for (size_t i = 0; i < beamVec.size(); i++) {
+        MPBunch &b = beamVec[i];
+        cts_derived d{};
+        cts_derive_moments(gpu, &m[i], k, &d);
+        b.xAver = m[i].mean[0]; b.pxAver = m[i].mean[1]; b.yAver = m[i].mean[2];
+        b.pyAver = m[i].mean[3]; b.zAver = m[i].mean[4]; b.pzAver = m[i].mean[5];
+        for (int j = 0; j < 3; j++) { b.emittance[j] = d.emit[j]; b.effectiveEmittance[j] = d.emit_eff[j]; }
+        b.rmsRx = d.rms_x; b.rmsRy = d.rms_y; b.rmsEffectiveRx = d.rms_eff_x; b.rmsEffectiveRy = d.rms_eff_y;
+        b.rmsBunchLength = d.sigma_z; b.rmsEnergySpread = d.sigma_dp;
+    }
+    return 0;
+}
+
+int MPBeam::DownloadParticles()
+{
+    size_t n = 0;
+    for (auto &b : beamVec) n += b.macroEleNumPerBunch;
+    std::vector<double> X[6];
+    for (auto &c : X) c.resize(n);
+    std::vector<int32_t> lost(n);
+    if (check(cts_download(gpu, X[0].data(), X[1].data(), X[2].data(), X[3].data(), X[4].data(), X[5].data(),
+                           lost.data()), "download"))
+        return 1;
+    size_t off = 0;
+    for (auto &b : beamVec) {
+        std::vector<double> *dst[6] = {&b.ePositionX, &b.eMomentumX, &b.ePositionY,
+                                       &b.eMomentumY, &b.ePositionZ, &b.eMomentumZ};
+        for (int c = 0; c < 6; c++) dst[c]->assign(X[c].begin() + off, X[c].begin() + off + b.macroEleNumPerBunch);
+        b.eSurive.assign(lost.begin() + off, lost.begin() + off + b.macroEleNumPerBunch);
+        off += b.macroEleNumPerBunch;
+    }
+    return 0;
+}
+
+int MPBeam::GetCBMGR(ReadInputSettings &in, std::vector<double> &rates)
+{
+    const auto *R = in.ringRun;
+    const int interval = R->bunchInfoPrintInterval;
+    const int nb = (int)beamVec.size();
+    const int nsamples = (int)(samples_.size() / (size_t)std::max(nb, 1));
+    rates.assign(3 * (size_t)nb, 0.0);
+    std::vector<int32_t> harm(nb);
+    for (int b = 0; b < nb; b++) harm[b] = beamVec[b].bunchHarmNum;
+    // GetCBMGR reads workQz and rmsBunchLength / rmsEnergySpread of the
+    // input (src/MPBeam.cpp:1498-1508)
+    const double tunes[3] = {in.ringParBasic->workQx, in.ringParBasic->workQy, in.ringParBasic->workQz};
+    const double twiss[5] = {alphaX0_, betaX0_, alphaY0_, betaY0_,
+                             in.ringBunchPara->rmsBunchLength / in.ringBunchPara->rmsEnergySpread};
+    return check(cts_cbm_growth_from_moments(samples_.data(), nsamples, nb, harm.data(), in.ringParBasic->harmonics,
+                                             tunes, twiss, interval, R->growthRateFittingStart,
+                                             R->growthRateFittingEnd, in.ringParBasic->t0, rates.data(), nullptr),
+                 "growth rates");
+}
+
+void MPBeam::StoreSample(const std::vector<cts_moments> &m, int turn, std::ofstream &tbt)
+{
+    samples_.insert(samples_.end(), m.begin(), m.end());
+    for (size_t b = 0; b < beamVec.size(); b++) {
+        const cts_moments &q = m[b];
+        cts_derived d{};
+        cts_derive_moments(gpu, &q, 0, &d);
+        char line[640];
+        std::snprintf(line, sizeof line,
+                      "%d %zu %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", turn, b,
+                      q.mean[0], q.mean[1], q.mean[2], q.mean[3], q.mean[4], q.mean[5], d.sigma_z, d.sigma_dp, d.rms_x,
+                      d.rms_y, d.emit[0], d.emit[1], q.transmission);
+        tbt << line;
+    }
+}
+
+int MPBeam::RunSettings(const LatticeInterActionPoint &lat, const ReadInputSettings &in, RunCalls &c)
+{
+    const auto *R = in.ringRun;
+    c = RunCalls{};
+    if (R->sRWakeFlag) {
+        WakeFunction sr;
+        if (sr.InitialSRWake(in)) return 1;
+        c.wake = true;
+        c.nbins = in.ringSRWake->SRWBunchBinNum;
+        c.rs = sr.lRs;
+        c.q = sr.lQ;
+        c.fr.resize(sr.lOmega.size());
+        for (size_t i = 0; i < c.fr.size(); i++) c.fr[i] = sr.lOmega[i] / (2 * kPi);
+    }
+    c.synrad = R->synRadDampingFlag != 0;
+    c.seed = R->seed;
+    if (R->beamIonFlag) {
+        if (lat.sections.size() != 1) {
+            std::cerr << "beam-ion interaction: one interaction point only" << std::endl;
+            return 1;
+        }
+        if (lat.gasSpec > CTS_MAX_SPECIES) {
+            std::cerr << "at most " << CTS_MAX_SPECIES << " gas species" << std::endl;
+            return 1;
+        }
+        cts_ion_config &ic = c.ion;
+        c.ions = true;
+        ic.nspec = lat.gasSpec;
+        ic.n_new = in.ringIonEffPara->macroIonNumberGeneratedPerIP;
+        ic.max_ions = (int64_t)lat.ionMaxNumberOneInterPoint;
+        for (int p = 0; p < ic.nspec; p++) {
+            ic.mass_number[p] = lat.ionMassNumber[p];
+            ic.cross_section[p] = lat.corssSectionEI[p];
+            ic.pressure[p] = lat.gasPercent[p] * lat.vacuumPressure[0];
+        }
+        ic.temperature = lat.temperature[0];
+        ic.length = lat.interactionLength[0];
+        ic.loss_boundary = lat.ionLossBoundary;
+        ic.ap_x = lat.pipeAperatureX[0];
+        ic.ap_y = lat.pipeAperatureY[0];
+        ic.seed = R->seed;
+    }
+    return 0;
+}
+
+// MPBeam::Run, multi-particle tracking (src/MPBeam.cpp:1010-1173).  With one
+// section the whole turn is one cts_turn call and the moments of the print
+// turns come from the device history after the loop (no host round trip
+// per turn); with several sections the seams run in the reference's order.
+// Samples are the moments at the end of turns n with n % interval == 0, as
+// MPBeamDataPrintPerTurn and GetCBMGR take them (src/MPBeam.cpp:1161-1172).
+int MPBeam::Run(Train &, LatticeInterActionPoint &lat, ReadInputSettings &in)
+{
+    const auto *R = in.ringRun;
+    const int nsec = (int)lat.sections.size();
+    const int nb = (int)beamVec.size();
+    alphaX0_ = lat.twissAlphaX[0];
+    betaX0_ = lat.twissBetaX[0];
+    alphaY0_ = lat.twissAlphaY[0];
+    betaY0_ = lat.twissBetaY[0];
+    RunCalls c;
+    if (RunSettings(lat, in, c)) return 1;
+    if (c.wake && check(cts_set_wake(gpu, c.nbins, 0, 0, 0, (int32_t)c.rs.size(), c.rs.data(), c.q.data(),
+                                     c.fr.data()), "wake"))
+        return 1;
+    if (check(cts_set_synrad(gpu, c.synrad, 2, c.seed), "synrad")) return 1;
+    if (c.ions && check(cts_set_ions(gpu, &c.ion), "ions")) return 1;
+    const int interval = std::max(R->bunchInfoPrintInterval, 1);
+    samples_.clear();
+    std::ofstream tbt(R->TBTBunchAverData + ".dat");
+    tbt << "# turn bunch xAver pxAver yAver pyAver zAver pzAver rmsBunchLength rmsEnergySpread rmsRx rmsRy "
+           "emittanceX emittanceY transmission\n";
+    std::vector<cts_moments> m(nb);
+    if (nsec == 1) {
+        if (check(cts_set_history(gpu, R->nTurns), "history")) return 1;
+        for (int n = 0; n < R->nTurns; n++)
+            if (check(cts_turn(gpu, n), "turn")) return 1;
+        for (int n = 0; n < R->nTurns; n += interval) {
+            if (check(cts_get_history(gpu, n, 1, m.data()), "history")) return 1;
+            StoreSample(m, n, tbt);
+        }
+    } else {
+        for (int n = 0; n < R->nTurns; n++) {
+            for (int k = 0; k < nsec; k++)
+                if (BeamTransferPerInteractionPointDueToLatticeT(k)) return 1;
+            if (in.ringParBasic->skewQuadK != 0 && BeamTransferDueToSkewQuad()) return 1;
+            if (BeamMomtumUpdateDueToRFTest() || BeamLongiPosTransferOneTurn() || BeamEnergyLossOneTurn()) return 1;
+            if (R->synRadDampingFlag && BeamSynRadDamping(n)) return 1;
+            if (R->sRWakeFlag && check(cts_sr_wake(gpu, nullptr), "sr wake")) return 1;
+            if (MarkParticleLostInBunch()) return 1;
+            if (n % interval == 0) {
+                if (check(cts_rms(gpu, m.data()), "rms")) return 1;
+                StoreSample(m, n, tbt);
+            }
+        }
+    }
+    if (check(cts_sync(gpu), "sync")) return 1;
+    if (MPBeamRMSCal(0)) return 1;
+    if (!R->runCBMGR.empty() && R->growthRateFittingEnd > R->growthRateFittingStart &&
+        (int)(samples_.size() / nb) >= R->growthRateFittingEnd / interval) {
+        std::vector<double> rates;
+        if (GetCBMGR(in, rates)) return 1;
+        std::ofstream gr(R->runCBMGR + ".dat");
+        gr << "# mode growth_rate_x growth_rate_y growth_rate_z [1/s]\n";
+        char line[128];
+        for (int k = 0; k < nb; k++) {
+            std::snprintf(line, sizeof line, "%d %.17g %.17g %.17g\n", k, rates[k], rates[nb + k], rates[2 * nb + k]);
+            gr << line;
+        }
+    }
+    return 0;
+}
+
+}  // namespace cetasim

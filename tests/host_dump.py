@@ -1,0 +1,100 @@
+"""Helpers for the tracking program cetasim_amd/cetasim_run: running it on a
+copy of the input fixture and reading what its --dump option writes."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+
+from cetasim_amd import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "cetasim_amd", "cetasim_run")
+FIXTURE = os.path.join(ROOT, "tests", "golden", "host")
+
+
+def exe():
+    if not os.path.isfile(EXE):
+        raise RuntimeError(f"{EXE} is not built: run __graft_entry__.build() (make -C cetasim_amd/host)")
+    return EXE
+
+
+def copy_fixture(dst, edits=None, name="input.dat"):
+    """Copy the fixture files into dst; edits maps an input key (as written)
+    to its new value line, applied to input.dat, written as `name`."""
+    for f in os.listdir(FIXTURE):
+        shutil.copy(os.path.join(FIXTURE, f), dst)
+    if edits:
+        lines = open(os.path.join(FIXTURE, "input.dat")).read().splitlines()
+        out = []
+        for line in lines:
+            key = line.split("=")[0].strip() if "=" in line else None
+            out.append(f"{key} = {edits[key]}" if key in edits else line)
+        for key, val in edits.items():
+            if not any(line.split("=")[0].strip() == key for line in lines if "=" in line):
+                out.append(f"{key} = {val}")
+        open(os.path.join(dst, name), "w").write("\n".join(out) + "\n")
+    return os.path.join(dst, name)
+
+
+def run(args, cwd, timeout=120):
+    return subprocess.run([exe(), *args], cwd=cwd, capture_output=True, text=True, timeout=timeout)
+
+
+def check(cwd, name="input.dat"):
+    r = run(["--check", name], cwd)
+    assert r.returncode == 0, r.stderr
+    return json.loads(r.stdout)
+
+
+def load_dump(d):
+    s = json.load(open(os.path.join(d, "settings.json")))
+    nb, ntot, nsec = s["nb"], s["ntot"], s["nsec"]
+    ring = capi.Ring.from_buffer_copy(open(os.path.join(d, "ring.bin"), "rb").read())
+    raw = open(os.path.join(d, "sections.bin"), "rb").read()
+    sz = ctypes_sizeof(capi.Section)
+    assert len(raw) == nsec * sz
+    secs = [capi.Section.from_buffer_copy(raw[i * sz:(i + 1) * sz]) for i in range(nsec)]
+    b = open(os.path.join(d, "bunches.bin"), "rb").read()
+    o = 0
+    counts = np.frombuffer(b, np.int64, nb, o); o += 8 * nb
+    q = np.frombuffer(b, np.float64, nb, o); o += 8 * nb
+    ne = np.frombuffer(b, np.float64, nb, o); o += 8 * nb
+    gap = np.frombuffer(b, np.int32, nb, o); o += 4 * nb
+    harm = np.frombuffer(b, np.int32, nb, o); o += 4 * nb
+    assert o == len(b)
+    X = np.fromfile(os.path.join(d, "particles.bin"), np.float64)
+    assert X.size == 6 * ntot
+    X = X.reshape(6, ntot)
+    parts = {c: X[i].copy() for i, c in enumerate(("x", "px", "y", "py", "z", "dp"))}
+    return dict(settings=s, ring=ring, sections=secs, counts=counts.copy(), q=q.copy(), ne=ne.copy(),
+                gap=gap.copy(), harm=harm.copy(), parts=parts)
+
+
+def ion_config(s):
+    cfg = capi.IonConfig()
+    cfg.nspec = s["ion_nspec"]
+    cfg.n_new = s["ion_n_new"]
+    cfg.max_ions = s["ion_max"]
+    for p in range(capi.MAX_SPECIES):
+        cfg.mass_number[p] = s["ion_mass"][p]
+        cfg.cross_section[p] = s["ion_cross"][p]
+        cfg.pressure[p] = s["ion_pressure"][p]
+    cfg.temperature = s["ion_temperature"]
+    cfg.length = s["ion_length"]
+    cfg.loss_boundary = s["ion_loss"]
+    cfg.ap_x, cfg.ap_y = s["ion_ap"]
+    cfg.seed = s["ion_seed"]
+    return cfg
+
+
+def ctypes_sizeof(t):
+    import ctypes
+    return ctypes.sizeof(t)
+
+
+def read_table(path):
+    """Rows of a whitespace table with '#' comment lines."""
+    rows = [list(map(float, line.split())) for line in open(path) if line.strip() and not line.startswith("#")]
+    return np.array(rows)

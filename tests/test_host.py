@@ -1,0 +1,217 @@
+"""The host layer (cetasim_amd/host): the input.dat reader, Train,
+LatticeInterActionPoint, WakeFunction and the initial distribution of
+MPBunch, through the tracking program's host-only modes (--check, --dump).
+
+Expected values come from the fixture's numbers by the reference's rules
+(src/ReadInputSettings.cpp, src/Train.cpp, src/LatticeInterActionPoint.cpp,
+src/WakeFunction.cpp, src/MPBeam.cpp:64-161, src/MPBunch.cpp:98-338) and,
+for the distribution, from the moments of a truncated Gaussian.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+import pytest
+
+from cetasim_amd import capi
+
+import host_dump as H
+
+E_CHARGE = 1.602017733e-19  # include/Global.h:28
+# per-axis variance factor of a 2-D standard normal cut at radius 3:
+# E[R^2 | R^2 <= 9] / 2 with R^2 ~ Exp(mean 2)
+CUT3 = (2.0 - 9.0 * math.exp(-4.5) / (1.0 - math.exp(-4.5))) / 2.0
+
+
+@pytest.fixture(scope="module")
+def fixture_dir(tmp_path_factory):
+    d = tmp_path_factory.mktemp("host")
+    H.copy_fixture(str(d))
+    return str(d)
+
+
+@pytest.fixture(scope="module")
+def chk(fixture_dir):
+    return H.check(fixture_dir)
+
+
+def ring_input():
+    ri = capi.RingInput()
+    ri.circ, ri.energy, ri.u0, ri.sdelta0 = 2304.0, 6.0e9, 4.17e6, 8.97e-4
+    ri.alphac[0] = 3.33e-5
+    ri.nux, ri.nuy = 135.18, 86.27
+    ri.chrom[0], ri.chrom[1] = 1.5, 1.0
+    ri.synrad_tau[0], ri.synrad_tau[1], ri.synrad_tau[2] = 2310, 2880, 1642
+    ri.natural_emit, ri.coupling = 19.4e-12, 0.1
+    ri.harmonic, ri.rf_volt = 3840, 8.0e6
+    ri.ap_x, ri.ap_y = 0.15, 0.05
+    return ri
+
+
+def test_ring_quantities(chk):
+    lib = capi.load()
+    ring, der = capi.Ring(), capi.RingDerived()
+    capi.check(lib.cts_derive_ring(ctypes.byref(ring_input()), ctypes.byref(ring), ctypes.byref(der)))
+    assert chk["f0"] == ring.f0 and chk["t0"] == ring.t0 and chk["rGamma"] == ring.rgamma
+    assert chk["workQz"] == der.workqz and chk["naturalBunchLength"] == der.natural_bunch_length
+    assert chk["harmonics"] == 3840
+    # the input workQz (0.0048) is replaced by the RF value
+    assert chk["workQz"] != 0.0048
+    # bunchEmittance is split by kappa
+    assert chk["emittanceX"] == pytest.approx(22e-12 / 1.1, rel=1e-15)
+    assert chk["emittanceY"] == pytest.approx(22e-12 / 1.1 * 0.1, rel=1e-15)
+    assert chk["nTurns"] == 40 and chk["seed"] == 7
+    assert (chk["beamIonFlag"], chk["sRWakeFlag"], chk["synRadDampingFlag"]) == (1, 1, 1)
+
+
+def test_filling_pattern(chk):
+    # Train::Initial sums bunchNumberPerTrain[j] * (bunchGaps[i] + 1): train 1
+    # starts at 2 * (1 + 1) + 1000 = 1004, not 2 * (3 + 1) + 1000
+    assert chk["trainStart"] == [0, 1004]
+    assert chk["bunchHarmNum"] == [0, 4, 1004, 1006, 1008, 1010]
+    assert chk["bunchGap"] == [4, 1000, 2, 2, 2, 3840 - 1010]
+    assert chk["normCurrent"] == [2, 1, 1, 3, 1, 1]
+    cur = np.array([2, 1, 1, 3, 1, 1]) * 0.02 / 9
+    np.testing.assert_allclose(chk["current"], cur, rtol=1e-15)
+    ne = cur / chk["f0"] / E_CHARGE
+    np.testing.assert_allclose(chk["electronNumPerBunch"], ne, rtol=1e-15)
+    np.testing.assert_allclose(chk["macroEleCharge"], ne / 8000, rtol=1e-15)
+
+
+def test_lattice_and_wake(chk):
+    np.testing.assert_allclose(chk["phaseAdvX12"], [2 * math.pi * 135.18], rtol=1e-15)
+    np.testing.assert_allclose(chk["phaseAdvY12"], [2 * math.pi * 86.27], rtol=1e-15)
+    assert chk["interactionLength"] == [2304.0]
+    assert chk["vacuumPressure"] == pytest.approx([1.0e-9 * 133.3224], rel=1e-15)
+    assert chk["pipeAperatureX"] == [0.15]     # min(twiss 0.2, input 0.15)
+    assert chk["pipeAperatureY"] == [0.05]     # min(twiss 0.05, input 0.10)
+    assert chk["corssSectionEI"] == pytest.approx([2.0e-22], rel=1e-15)
+    np.testing.assert_allclose(chk["lRs"], [1e4, 3e3])
+    np.testing.assert_allclose(chk["lQ"], [1.0, 2.0])
+    np.testing.assert_allclose(chk["lOmega"], [2 * math.pi * 1e10, 2 * math.pi * 5e9], rtol=1e-15)
+    # the section map is built from the twiss columns in file order
+    lib = capi.load()
+    bz = chk["twissBetaZ"]
+    t = capi.Twiss(2.71, 0.05, 0.001, 0.0002, 4.25, -0.03, 0.0, 0.0, bz)
+    sec = capi.Section()
+    capi.check(lib.cts_build_section(ctypes.byref(t), ctypes.byref(t), 2 * math.pi * 135.18,
+                                     2 * math.pi * 86.27, ctypes.byref(sec)))
+    assert list(sec.b1h1) == chk["b1h1"]
+    assert bz == chk["naturalBunchLength"] / 8.97e-4
+
+
+def test_initial_distribution(chk):
+    m = np.array(chk["moments"]).reshape(6, 6, 2)  # bunch, coordinate, (mean, rms)
+    n = 8000
+    ex, ey = 22e-12 / 1.1, 22e-12 / 1.1 * 0.1
+    bx, ax, dx, dpx = 2.71, 0.05, 0.001, 0.0002
+    by, ay = 4.25, -0.03
+    beta = math.sqrt(1 - (0.51099906e6 / 6e9) ** 2)
+    sd = math.sqrt(CUT3) * 8.97e-4 / beta ** 2
+    rms = np.array([
+        math.sqrt(CUT3 * ex * bx + dx ** 2 * sd ** 2),
+        math.sqrt((ax ** 2 * CUT3 * ex + ex) / bx + dpx ** 2 * sd ** 2),
+        math.sqrt(CUT3 * ey * by),
+        math.sqrt((ay ** 2 * CUT3 * ey + ey) / by),
+        math.sqrt(CUT3) * 2.5e-3,
+        sd,
+    ])
+    for b in range(6):
+        np.testing.assert_allclose(m[b, :, 1], rms, rtol=0.035)
+        assert np.all(np.abs(m[b, :, 0]) <= 4.5 * rms / math.sqrt(n))
+    # bunches are independent draws
+    assert len({m[b, 0, 0] for b in range(6)}) == 6
+
+
+def test_two_sections(tmp_path):
+    H.copy_fixture(str(tmp_path), {"RingTwiss": "twiss2.dat", "ringSectNum": "2"})
+    c = H.check(str(tmp_path))
+    np.testing.assert_allclose(c["phaseAdvX12"], [400.0, 2 * math.pi * 135.18 - 400.0], rtol=1e-15)
+    np.testing.assert_allclose(c["phaseAdvY12"], [250.0, 2 * math.pi * 86.27 - 250.0], rtol=1e-15)
+    assert c["interactionLength"] == [1000.0, 1304.0]
+    assert c["vacuumPressure"] == pytest.approx([1.0e-9 * 133.3224, 2.0e-9 * 133.3224], rel=1e-15)
+    assert c["pipeAperatureX"] == [0.15, 0.01]
+    assert c["pipeAperatureY"] == [0.05, 0.1]
+    assert len(c["b1h1"]) == 72
+
+
+@pytest.mark.parametrize("edit,needle", [
+    ({"ringADTX": "0 0    // two coefficients"}, "ringadtx"),
+    ({"fpTrainGaps": "1000 2800"}, "filling pattern"),
+    ({"ionCalIonGasPercent": "0.5"}, "gas"),
+    ({"ringSectNum": "2"}, "does not match"),
+    ({"BunchDistributionType": "4"}, "distribution type"),
+    ({"fpSetBunchChargeIndex": "0 9"}, "not a bunch index"),
+])
+def test_input_errors(tmp_path, edit, needle):
+    H.copy_fixture(str(tmp_path), edit)
+    r = H.run(["--check", "input.dat"], str(tmp_path))
+    assert r.returncode != 0
+    assert needle in r.stderr.lower(), r.stderr
+
+
+def test_last_occurrence_wins_and_case(tmp_path):
+    # keys are case-insensitive and blanks inside a key are ignored; a later
+    # line overrides an earlier one (the reference example files rely on it)
+    H.copy_fixture(str(tmp_path), {"ring Current": "0.04"})
+    c = H.check(str(tmp_path))
+    np.testing.assert_allclose(sum(c["current"]), 0.04, rtol=1e-14)
+
+
+def test_dump(tmp_path):
+    H.copy_fixture(str(tmp_path))
+    d = tmp_path / "dump"
+    d.mkdir()
+    r = H.run(["--dump", str(d), "input.dat"], str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    D = H.load_dump(str(d))
+    s = D["settings"]
+    assert (s["nb"], s["nsec"], s["ntot"]) == (6, 1, 48000)
+    assert D["counts"].tolist() == [8000] * 6
+    assert D["harm"].tolist() == [0, 4, 1004, 1006, 1008, 1010]
+    assert D["gap"].tolist() == [4, 1000, 2, 2, 2, 2830]
+    ring = D["ring"]
+    assert ring.harmonic == 3840 and ring.n_rf == 1 and ring.nux == 135.18 and ring.chrom[0] == 1.5
+    assert (ring.ap_x, ring.ap_y) == (0.15, 0.05)
+    assert s["wake"] == 1 and s["nbins"] == 100
+    np.testing.assert_allclose(s["fr"], [1e10, 5e9], rtol=1e-15)
+    assert s["rs"] == [1e4, 3e3] and s["q"] == [1.0, 2.0]
+    assert s["synrad"] == 1 and s["seed"] == 7 and s["ions"] == 1
+    assert (s["ion_nspec"], s["ion_n_new"], s["ion_max"]) == (1, 300, 200000)
+    assert s["ion_mass"][0] == 28.0 and s["ion_cross"][0] == pytest.approx(2e-22, rel=1e-15)
+    assert s["ion_pressure"][0] == pytest.approx(1.333224e-7, rel=1e-15)
+    assert (s["ion_temperature"], s["ion_length"], s["ion_loss"]) == (300.0, 2304.0, 10.0)
+    assert s["ion_ap"] == [0.15, 0.05] and s["ion_seed"] == 7
+    assert s["betaZ"] == 2.5e-3 / 8.97e-4
+    # the dumped particles are the ones --check summarised
+    c = H.check(str(tmp_path))
+    m = np.array(c["moments"]).reshape(6, 6, 2)
+    X = np.stack([D["parts"][k] for k in ("x", "px", "y", "py", "z", "dp")])
+    for b in range(6):
+        seg = X[:, b * 8000:(b + 1) * 8000]
+        np.testing.assert_allclose(seg.mean(axis=1), m[b, :, 0], rtol=1e-12, atol=1e-20)
+
+
+def test_rf_modes(tmp_path):
+    # rfMode 1 (beam loading) is refused; several ideal cavities keep their
+    # input phases (CavityResonator::Initial, src/CavityResonator.cpp:60-139)
+    H.copy_fixture(str(tmp_path), {"rfMode": "1"})
+    d = tmp_path / "d1"
+    d.mkdir()
+    r = H.run(["--dump", str(d), "input.dat"], str(tmp_path))
+    assert r.returncode != 0 and "rfmode" in r.stderr.lower()
+    H.copy_fixture(str(tmp_path), {"rfResNum": "2", "rfMode": "0 0", "rfResHarm": "1 3",
+                                   "rfResVol": "8.0E+6 2.0E+6", "rfResPhase": "1.0 -1.6"})
+    d = tmp_path / "d2"
+    d.mkdir()
+    r = H.run(["--dump", str(d), "input.dat"], str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    ring = H.load_dump(str(d))["ring"]
+    assert ring.n_rf == 2 and list(ring.rf_harm)[:2] == [1, 3]
+    np.testing.assert_allclose(list(ring.rf_vr)[:2], [8e6 * math.cos(1.0), 2e6 * math.cos(-1.6)], rtol=1e-15)
+    np.testing.assert_allclose(list(ring.rf_vi)[:2], [8e6 * math.sin(1.0), 2e6 * math.sin(-1.6)], rtol=1e-15)
+
+
+def test_exe_exists():
+    assert os.access(H.exe(), os.X_OK)
