@@ -51,12 +51,37 @@ def parse():
                     help="c2: single bunch 1e7 macroparticles with wake and excitation; "
                          "c3: 80-bunch train with the beam-ion phase; "
                          "c5: 480-bunch train (6e7 macroparticles) with wake and ions")
+    ap.add_argument("--eager", action="store_true",
+                    help="time turns launched one by one instead of HIP-graph replays")
     ap.add_argument("--plain-only", action="store_true", help="c3: skip the surrogate run")
     ap.add_argument("--sur-steps", type=int, default=20000, help="c3: Adam steps of the surrogate training")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reads MIB, writes MIB) three times, "
                          "to scale FETCH_SIZE/WRITE_SIZE counters")
     return ap.parse_args()
+
+
+def timed_turns(tr, a, names):
+    """Warm-up turns (the turn graphs are captured there), then the timed
+    turns, then as many turns again launched one by one with HIP events
+    around every kernel on the handle's stream: the kernel durations.
+    Returns (timed seconds, event-stretch seconds, {name: (avg ms, launches)})."""
+    graph = not a.eager
+    tr.run_turns(0, a.warmup, graph)
+    tr.sync()
+    t0 = time.perf_counter()
+    tr.run_turns(a.warmup, a.steps, graph)
+    tr.sync()
+    dt = time.perf_counter() - t0
+    tr.set_timing(True)
+    t0 = time.perf_counter()
+    for t in range(a.warmup + a.steps, a.warmup + 2 * a.steps):
+        tr.turn(t)
+    tr.sync()
+    dt_ev = time.perf_counter() - t0
+    kt = {k: tr.kernel_time(k) for k in names}
+    tr.set_timing(False)
+    return dt, dt_ev, kt
 
 
 def copy_calibration(mib):
@@ -176,7 +201,7 @@ def workload_train(a):
 
 
 def train_turns(a, sur=None, mode=0):
-    """Time a.steps turns of the train; returns (s, kernel times, cloud, moments, n, nb)."""
+    """Time a.steps turns of the train; returns (s, kernel times, cloud, moments, n, nb, event-stretch s)."""
     from cetasim_amd import configs
     from cetasim_amd.tracker import Tracker
 
@@ -203,22 +228,13 @@ def train_turns(a, sur=None, mode=0):
     if sur is not None:
         tr.set_field_model(sur, mode)
     tr.autotune()
-    for t in range(a.warmup):
-        tr.turn(t)
-    tr.sync()
-    tr.set_timing(True)
-    t0 = time.perf_counter()
-    for t in range(a.warmup, a.warmup + a.steps):
-        tr.turn(t)
-    tr.sync()
-    dt = time.perf_counter() - t0
-    ktimes = {k: tr.kernel_time(k) for k in ("pass", "finalize", "ion_prep", "ion_turn", "ion_finish",
-                                              "ion_compact", "ion_kick")}
-    tr.set_timing(False)
+    dt, dt_ev, ktimes = timed_turns(tr, a, ("pass", "post", "finalize", "ion_prep", "ion_turn", "ion_finish",
+                                            "ion_compact", "ion_kick"))
+    ktimes = {k: v for k, v in ktimes.items() if v[1] > 0}
     cloud = tr.ion_passages()
     m = tr.moments()
     tr.close()
-    return dt, ktimes, cloud, m, int(counts.sum()), nb
+    return dt, ktimes, cloud, m, int(counts.sum()), nb, dt_ev
 
 
 def run_train(a):
@@ -229,7 +245,7 @@ def run_train(a):
     p = configs.BASE
     T = workload_train(a)
     ring, secs, _ = configs.lattice(p)
-    dt, ktimes, cloud, m_plain, n, nb = train_turns(a)
+    dt, ktimes, cloud, m_plain, n, nb, dt_ev = train_turns(a)
     plain = n * a.steps / dt
     sur_info = None
     value = plain
@@ -237,8 +253,8 @@ def run_train(a):
     if not a.plain_only:
         sur, sur_info = train_surrogate(a.sur_steps)
         # electron-side kicks from the surrogate (mode 2), and both sides (mode 3)
-        dts, kts, cloud_s, m_sur, _, _ = train_turns(a, sur, 2)
-        dt3, kt3, _, _, _, _ = train_turns(a, sur, 3)
+        dts, kts, cloud_s, m_sur, _, _, _ = train_turns(a, sur, 2)
+        dt3, kt3, _, _, _, _, _ = train_turns(a, sur, 3)
         sur.close()
         value = n * a.steps / dts
         sur_info.update({"mode": "electron-side kicks (ions kicked by the exact field)",
@@ -264,7 +280,9 @@ def run_train(a):
                                "Bassetti-Erskine both ways (FP64 Faddeeva; with the surrogate inside its "
                                "domain when value is the surrogate rate), excitation on (device Philox)"
                                + (f", BBR wake on {T['nbins']} fixed bins per bunch" if T.get("nbins") else ""),
-                   "macroparticles": n, "bunches": nb, "parallelism": "single GPU"},
+                   "macroparticles": n, "bunches": nb, "parallelism": "single GPU",
+                   "launch": "eager" if a.eager else "HIP graph replays of two turns"},
+        "ms_per_step_kernel_timing_run_plain": dt_ev / a.steps * 1e3,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_pass (fused turn)",
                      "kernel_ms": pass_ms, "launches": ktimes["pass"][1],
@@ -308,18 +326,9 @@ def main():
     tr.set_synrad(True, 2, 2)
     nwg = tr.autotune()
 
-    for t in range(a.warmup):
-        tr.turn(t)
-    tr.sync()
-    tr.set_timing(True)
-    t0 = time.perf_counter()
-    for t in range(a.warmup, a.warmup + a.steps):
-        tr.turn(t)
-    tr.sync()
-    dt = time.perf_counter() - t0
-    pass_ms, n_pass = tr.kernel_time("pass")
-    post_ms, _ = tr.kernel_time("post")
-    tr.set_timing(False)
+    dt, dt_ev, kt = timed_turns(tr, a, ("pass", "post"))
+    pass_ms, n_pass = kt["pass"]
+    post_ms, _ = kt["post"]
     m = tr.moments()[0]
     tr.close()
 
@@ -343,7 +352,9 @@ def main():
                                "+-5 sigma_z, BBR wake Rs=10kOhm fr=10GHz Q=1, quantum excitation "
                                "(device Philox, seed 2), one-section ring Qx/Qy/Qs=0.18/0.27/0.005",
                    "macroparticles": n, "bins": a.nbins, "parallelism": "single GPU",
-                   "workgroups": nwg},
+                   "workgroups": nwg,
+                   "launch": "eager" if a.eager else "HIP graph replays of two turns"},
+        "ms_per_step_kernel_timing_run": dt_ev / a.steps * 1e3,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None if tpm is None else tpm * n,
                      "traffic_source": tsrc,

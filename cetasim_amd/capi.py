@@ -95,6 +95,7 @@ SIGNATURES = {
     "cts_get_ion_passages": (I32, [ctypes.c_void_p, PD]),
     "cts_turn": (I32, [ctypes.c_void_p, I32]),
     "cts_flush": (I32, [ctypes.c_void_p]),
+    "cts_run_turns": (I32, [ctypes.c_void_p, I32, I32, I32]),
     "cts_autotune": (I32, [ctypes.c_void_p, PI32]),
     "cts_section_map": (I32, [ctypes.c_void_p, I32]),
     "cts_skew_quad": (I32, [ctypes.c_void_p]),

@@ -104,6 +104,8 @@ struct cts_handle {
 
     // per-section constant blocks (kNConst doubles each)
     double *d_cst = nullptr;
+    // tables of the excitation normals (NormalTab layout)
+    float2 *d_ntab = nullptr;
 
     // wake
     int nbins = 0, wake_mode = 1, nres = 0;
@@ -147,6 +149,21 @@ struct cts_handle {
     int32_t sur_mode = 0;  // bit 0: ions kicked by bunches, bit 1: electrons kicked by ions
     uint32_t *d_fb_count = nullptr;  // exact-field fallback queue of the electron-side kick
     uint64_t *d_fb_list = nullptr;
+
+    // graph replays of two consecutive turns (cts_run_turns): the turn is
+    // read from d_turn, which the graph advances by 2; one instance per
+    // (wake buffer parity, ion buffer) at capture
+    uint32_t *d_turn = nullptr;
+    bool capturing = false;
+    int cap_off = 0;  // turn offset (0, 1) inside the graph being captured
+    struct GraphSlot {
+        hipGraphExec_t exec = nullptr;
+        int parity = -1, ion_buf = -1;
+        const float *sur_params = nullptr;
+        SurNorm sur_norm{};
+    };
+    GraphSlot graphs[4];
+    int64_t turn_dev_next = -1;  // value d_turn holds for the next replay (-1: unknown)
 
     // timing
     bool timing = false;
@@ -244,6 +261,25 @@ struct TimedScope {
     }
 };
 
+}  // namespace
+
+namespace cts {
+// sc[j] = {sin, cos}(2 pi j / 256) and lg[j] = {ln(1 + j/256), 256/(256 + j)},
+// each the float rounding of the double value
+void build_normal_tables(float2 *out)
+{
+    for (int j = 0; j < kNormTab; j++) {
+        const double a = 6.283185307179586 * (double)j / 256.0;
+        out[j].x = (float)std::sin(a);
+        out[j].y = (float)std::cos(a);
+        out[kNormTab + j].x = (float)std::log1p((double)j / 256.0);
+        out[kNormTab + j].y = (float)(256.0 / (256.0 + (double)j));
+    }
+}
+}  // namespace cts
+
+namespace {
+
 // Constant block of section k (layout CI_* in cts_internal.hpp).
 void build_consts(const cts_ring &R, const std::vector<cts_section> &secs, int k, double *c)
 {
@@ -323,6 +359,11 @@ PassParams base_pass(cts_handle *h, uint32_t ops, int k)
     p.n_rf = h->ring.n_rf;
     p.noise_mode = h->noise_mode;
     p.seed = h->seed;
+    p.ntab = h->d_ntab;
+    if (h->capturing) {
+        p.turn_dev = h->d_turn;
+        p.turn = (uint32_t)h->cap_off;
+    }
     return p;
 }
 
@@ -350,6 +391,11 @@ FinParams base_fin(cts_handle *h, uint32_t ops, int slot, int prev_slot)
     f.chunk_part = h->d_chunk_part;
     f.chunk_cnt = h->d_chunk_cnt;
     f.max_chunks = h->max_chunks;
+    if (h->capturing) {
+        f.turn_dev = h->d_turn;
+        f.turn_off = h->cap_off;
+        f.hist_cap = h->hist_cap;
+    }
     return f;
 }
 
@@ -601,6 +647,17 @@ void free_ion_state(cts_handle *h)
 // (src/MPBeam.cpp:2585-2611): bunch views from the moments of the present
 // state, one launch per bunch passage in train order, the turn-end
 // compaction of the cloud, then the electron kicks of every bunch.
+// Drop the captured turn graphs (their kernel arguments name buffers and
+// settings that are about to change).
+void graph_reset(cts_handle *h)
+{
+    for (auto &g : h->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g = cts_handle::GraphSlot{};
+    }
+    h->turn_dev_next = -1;
+}
+
 int ion_phase(cts_handle *h, int32_t turn)
 {
     int rc;
@@ -639,10 +696,16 @@ int ion_phase(cts_handle *h, int32_t turn)
     P.nb = h->nb;
     P.n_new = I.n_new;
     // waves: the ions present at the start of the turn (the host's bound of
-    // their count), then the new ions of every passage
-    P.w_old = (int32_t)((h->ion_len_ub + 63) / 64);
+    // their count; the capacity in a graph, which replays any turn), then
+    // the new ions of every passage
+    const int64_t old_ub = h->capturing ? h->ion_cap : h->ion_len_ub;
+    P.w_old = (int32_t)((old_ub + 63) / 64);
     P.w_new = (I.n_new + 63) / 64;
     P.turn = (uint32_t)turn;
+    if (h->capturing) {
+        P.turn_dev = h->d_turn;
+        P.turn = (uint32_t)h->cap_off;
+    }
     {
         TimedScope ts(h, "ion_turn");
         HIPCHK(launch_ion_turn(P, h->nspec, h->stream));
@@ -667,7 +730,8 @@ int ion_phase(cts_handle *h, int32_t turn)
         TimedScope ts(h, "ion_finish");
         HIPCHK(launch_ion_finish(F, h->stream));
     }
-    const int64_t ub = std::min<int64_t>(h->ion_len_ub + (int64_t)h->nb * I.n_new, h->ion_cap);
+    const int64_t ub = h->capturing ? h->ion_cap
+                                    : std::min<int64_t>(h->ion_len_ub + (int64_t)h->nb * I.n_new, h->ion_cap);
     // stable compaction of every species into the other buffer
     for (int p = 0; p < h->nspec; p++) {
         IonCompactParams c;
@@ -751,6 +815,14 @@ int cts_create(const cts_ring *ring, const cts_section *secs, int32_t nsec, cts_
     if (e == hipSuccess) e = upload_faddeeva_tables_ions(tab, fad);
     if (e == hipSuccess) e = hipMalloc((void **)&h->d_status, sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(h->d_status, 0, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&h->d_turn, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&h->d_ntab, 2 * kNormTab * sizeof(float2));
+    if (e == hipSuccess) {
+        std::vector<float2> tab(2 * kNormTab);
+        build_normal_tables(tab.data());
+        e = hipMemcpy(h->d_ntab, tab.data(), tab.size() * sizeof(float2), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMemset(h->d_turn, 0, sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc((void **)&h->h_status, sizeof(uint32_t), hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->status_ev, hipEventDisableTiming);
     if (e == hipSuccess) {
@@ -774,7 +846,10 @@ int cts_destroy(cts_handle *h)
     if (!h) return CTS_OK;
     (void)hipStreamSynchronize(h->stream);
     resolve_timers(h);
+    graph_reset(h);
     for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
+    if (h->d_turn) (void)hipFree(h->d_turn);
+    if (h->d_ntab) (void)hipFree(h->d_ntab);
     free_bunch_state(h);
     free_ion_state(h);
     if (h->d_status) (void)hipFree(h->d_status);
@@ -791,6 +866,7 @@ int cts_set_bunches(cts_handle *h, int32_t nb, const int64_t *counts, const doub
 {
     if (!h || nb < 1 || !counts || !macro_charge || !n_e) return fail(CTS_ERR_ARG, "cts_set_bunches: bad arguments");
     HIPCHK(hipStreamSynchronize(h->stream));
+    graph_reset(h);
     free_bunch_state(h);
     const cts_ion_config ions = h->ion;
     const bool ions_on = h->nspec > 0;
@@ -903,6 +979,8 @@ int cts_set_wake(cts_handle *h, int32_t nbins, int32_t mode, double grid_lo, dou
     if (rc) return rc;
     if (nbins < 0 || nbins > 1024 || (mode != 0 && mode != 1) || nres < 0 || nres > CTS_MAX_RES)
         return fail(CTS_ERR_ARG, "cts_set_wake: nbins in [0,1024], mode 0/1, nres <= 8");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    graph_reset(h);
     if (mode == 1 && nbins > 0 && !(grid_hi > grid_lo)) return fail(CTS_ERR_ARG, "cts_set_wake: empty grid");
     for (int r = 0; r < nres; r++)
         if (!(q[r] > 0.5)) return fail(CTS_ERR_ARG, "cts_set_wake: BBR needs Q > 1/2");
@@ -960,6 +1038,8 @@ int cts_set_synrad(cts_handle *h, int32_t on, int32_t noise_mode, uint64_t seed)
     int rc = check_ready(h);
     if (rc) return rc;
     if (noise_mode < 0 || noise_mode > 2) return fail(CTS_ERR_ARG, "cts_set_synrad: noise_mode 0..2");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    graph_reset(h);
     for (int i = 0; i < 3; i++)
         if (on && !(h->ring.synrad_tau[i] > 0)) return fail(CTS_ERR_ARG, "cts_set_synrad: damping times must be > 0");
     h->synrad_on = on ? 1 : 0;
@@ -983,13 +1063,11 @@ int cts_push_noise(cts_handle *h, const double *noise)
 }
 
 // MPBeam::Run body for one section (src/MPBeam.cpp:1059-1102), fused.
-int cts_turn(cts_handle *h, int32_t turn)
+// The kernels of one turn (everything cts_turn enqueues after its status
+// check); also what a graph capture records, twice.
+static int turn_body(cts_handle *h, int32_t turn)
 {
-    int rc = check_ready(h);
-    if (rc) return rc;
-    if (h->secs.size() != 1) return fail(CTS_ERR_STATE, "cts_turn fuses one section; use the seam calls");
-    // a bunch below 50% transmission ends the run, as the reference exits
-    if ((rc = check_status(h, false))) return rc;
+    int rc;
     uint32_t ops = OP_MAP | OP_RF | OP_DRIFT | OP_U0 | OP_LOSS | OP_MOMA | OP_WRITE;
     if (h->ring.skew_k != 0) ops |= OP_SKEW;
     if (h->synrad_on) ops |= OP_SYNRAD;
@@ -1001,7 +1079,7 @@ int cts_turn(cts_handle *h, int32_t turn)
     if (wake) ops |= (h->wake_mode == 1) ? OP_BIN : OP_ZSTAT;
     else ops |= OP_MOMD;  // no deferred kick: dp moments close this turn
     PassParams p = base_pass(h, ops, 0);
-    p.turn = (uint32_t)turn;
+    if (!h->capturing) p.turn = (uint32_t)turn;
     if ((rc = run_pass(h, p))) return rc;
     // finalize: dp part of the previous record (MOMB), this turn's record
     FinParams f = base_fin(h, ops & (OP_MOMB | OP_MOMA | OP_MOMD | OP_LOSS | OP_ZSTAT), slot_of(h, turn),
@@ -1023,7 +1101,146 @@ int cts_turn(cts_handle *h, int32_t turn)
     h->last_turn = turn;
     h->dp_pending = wake;
     h->cur_valid = true;
+    return CTS_OK;
+}
+
+int cts_turn(cts_handle *h, int32_t turn)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (h->secs.size() != 1) return fail(CTS_ERR_STATE, "cts_turn fuses one section; use the seam calls");
+    // a bunch below 50% transmission ends the run, as the reference exits
+    if ((rc = check_status(h, false))) return rc;
+    if ((rc = turn_body(h, turn))) return rc;
     return enqueue_status_copy(h);
+}
+
+// Graph of turns (t, t+1) for the handle's present configuration, captured
+// from turn_body with the turn read from d_turn.  The host state that the
+// capture advances (buffer parities, pending flags) is put back: the graph
+// only takes effect when it is launched.
+static int capture_pair(cts_handle *h, hipGraphExec_t *out)
+{
+    const int parity = h->hist_parity, ion_buf = h->ion_buf, last_turn = h->last_turn;
+    const bool kick_pending = h->kick_pending, dp_pending = h->dp_pending, cur_valid = h->cur_valid;
+    const bool timing = h->timing;
+    const int64_t ion_len_ub = h->ion_len_ub;
+    h->timing = false;
+    h->capturing = true;
+    hipGraph_t g = nullptr;
+    int rc = CTS_OK;
+    hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        for (int off = 0; off < 2 && rc == CTS_OK; off++) {
+            h->cap_off = off;
+            rc = turn_body(h, last_turn + 1 + off);
+        }
+        if (rc == CTS_OK) {
+            e = launch_advance_turn(h->d_turn, 2u, h->stream);
+            if (e != hipSuccess) rc = fail(CTS_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        }
+        const hipError_t e2 = hipStreamEndCapture(h->stream, &g);
+        if (rc == CTS_OK && e2 != hipSuccess)
+            rc = fail(CTS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e2));
+    } else {
+        rc = fail(CTS_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    }
+    h->capturing = false;
+    h->cap_off = 0;
+    h->timing = timing;
+    h->hist_parity = parity;
+    h->ion_buf = ion_buf;
+    h->last_turn = last_turn;
+    h->kick_pending = kick_pending;
+    h->dp_pending = dp_pending;
+    h->cur_valid = cur_valid;
+    h->ion_len_ub = ion_len_ub;
+    if (rc == CTS_OK) {
+        e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = fail(CTS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+}
+
+// The surrogate parameters a graph captured (pointer and normalisation are
+// kernel arguments).
+static void sur_state(cts_handle *h, const float **params, SurNorm *norm)
+{
+    *params = nullptr;
+    *norm = SurNorm{0.f, 0.f, 0.f, 0.f};
+    if (h->sur && h->sur_mode) sur_export(h->sur, params, norm);
+}
+
+int cts_run_turns(cts_handle *h, int32_t first, int32_t nturns, int32_t use_graph)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (h->secs.size() != 1) return fail(CTS_ERR_STATE, "cts_run_turns fuses one section; use the seam calls");
+    if (nturns < 0 || first < 0) return fail(CTS_ERR_ARG, "cts_run_turns: first, nturns >= 0");
+    int t = first;
+    const int end = first + nturns;
+    while (t < end) {
+        // a replay starts from the steady state of the turn loop: the last
+        // turn was t - 1 and, with a wake, its kick is pending; host noise
+        // (pushed per turn) is not replayable
+        const bool steady = use_graph && end - t >= 2 && h->cur_valid && h->last_turn == t - 1 &&
+                            h->kick_pending == (h->nbins > 0) && h->noise_mode != 1;
+        if (!steady) {
+            if ((rc = cts_turn(h, t))) return rc;
+            t++;
+            continue;
+        }
+        if ((rc = check_status(h, false))) return rc;
+        const float *sp;
+        SurNorm sn;
+        sur_state(h, &sp, &sn);
+        cts_handle::GraphSlot *slot = nullptr;
+        for (auto &g : h->graphs)
+            if (g.exec && g.parity == h->hist_parity && g.ion_buf == h->ion_buf) slot = &g;
+        if (slot && (slot->sur_params != sp || std::memcmp(&slot->sur_norm, &sn, sizeof sn) != 0)) {
+            graph_reset(h);
+            slot = nullptr;
+        }
+        if (!slot) {
+            // a graph captured one turn later in the buffer cycle serves
+            // after one eager turn (cheaper than a capture)
+            const int p1 = h->nbins > 0 ? 1 - h->hist_parity : h->hist_parity;
+            const int b1 = h->nspec > 0 ? 1 - h->ion_buf : h->ion_buf;
+            bool later = false;
+            for (auto &g : h->graphs)
+                if (g.exec && g.parity == p1 && g.ion_buf == b1 && g.sur_params == sp &&
+                    std::memcmp(&g.sur_norm, &sn, sizeof sn) == 0)
+                    later = true;
+            if (later && end - t >= 3) {
+                if ((rc = cts_turn(h, t))) return rc;
+                t++;
+                continue;
+            }
+            for (auto &g : h->graphs)
+                if (!g.exec) { slot = &g; break; }
+            if (!slot) {
+                graph_reset(h);
+                slot = &h->graphs[0];
+            }
+            if ((rc = capture_pair(h, &slot->exec))) return rc;
+            slot->parity = h->hist_parity;
+            slot->ion_buf = h->ion_buf;
+            slot->sur_params = sp;
+            slot->sur_norm = sn;
+        }
+        if (h->turn_dev_next != t) HIPCHK(launch_set_turn(h->d_turn, (uint32_t)t, h->stream));
+        HIPCHK(hipGraphLaunch(slot->exec, h->stream));
+        h->turn_dev_next = t + 2;
+        // the host state after two turns: parities back where they were
+        h->last_turn = t + 1;
+        h->dp_pending = h->nbins > 0;
+        h->cur_valid = true;
+        if (h->nspec > 0) h->ion_len_ub = h->ion_cap;
+        if ((rc = enqueue_status_copy(h))) return rc;
+        t += 2;
+    }
+    return CTS_OK;
 }
 
 // Workgroup count of the streaming pass, picked by timing: the candidates
@@ -1036,6 +1253,7 @@ int cts_autotune(cts_handle *h, int32_t *nwg_out)
     if (rc) return rc;
     if ((rc = flush_kick(h))) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
+    graph_reset(h);
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, h->dev));
     const int cus = prop.multiProcessorCount;
@@ -1199,6 +1417,7 @@ int cts_set_ions(cts_handle *h, const cts_ion_config *cfg)
     if (rc) return rc;
     if (!cfg || cfg->nspec < 0 || cfg->nspec > CTS_MAX_SPECIES) return fail(CTS_ERR_ARG, "cts_set_ions: nspec 0..4");
     HIPCHK(hipStreamSynchronize(h->stream));
+    graph_reset(h);
     free_ion_state(h);
     if (cfg->nspec == 0) return CTS_OK;
     if (cfg->n_new < 1 || cfg->max_ions < 1 || !(cfg->temperature > 0) || !(cfg->length >= 0) ||
@@ -1260,6 +1479,7 @@ int cts_set_field_model(cts_handle *h, const cts_surrogate *s, int32_t mode)
     if (!h) return fail(CTS_ERR_ARG, "null handle");
     if (mode < 0 || mode > 3) return fail(CTS_ERR_ARG, "cts_set_field_model: mode 0..3");
     HIPCHK(hipStreamSynchronize(h->stream));
+    graph_reset(h);
     h->sur = mode ? s : nullptr;
     h->sur_mode = s ? mode : 0;
     if ((h->sur_mode & 2) && !h->d_fb_list) {
@@ -1338,6 +1558,7 @@ int cts_set_history(cts_handle *h, int32_t capacity)
     if (rc) return rc;
     if (capacity < 0) return fail(CTS_ERR_ARG, "cts_set_history: capacity < 0");
     HIPCHK(hipStreamSynchronize(h->stream));
+    graph_reset(h);
     if (h->d_hist) (void)hipFree(h->d_hist);
     h->d_hist = nullptr;
     h->hist_cap = capacity;
@@ -1467,7 +1688,7 @@ int cts_debug_qe_normals(cts_handle *h, uint64_t seed, int32_t turn, uint64_t ba
     double *d = nullptr;
     int rc = dalloc(&d, (size_t)n * 5);
     if (rc) return rc;
-    hipError_t e = launch_qe_debug(seed, (uint32_t)turn, base, n, d, h->stream);
+    hipError_t e = launch_qe_debug(h->d_ntab, seed, (uint32_t)turn, base, n, d, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess) e = hipMemcpy(noise_out, d, (size_t)n * 5 * sizeof(double), hipMemcpyDeviceToHost);
     (void)hipFree(d);

@@ -95,11 +95,68 @@ __device__ __forceinline__ void box_muller(uint32_t a, uint32_t b, float &n0, fl
     box_muller24(a >> 8, b >> 8, n0, n1);
 }
 
+// --------------------------------------------- table-driven normal pairs
+// The quantum-excitation normals use a Box-Muller transform whose ln and
+// sincos come from two 256-entry tables (4 KB, staged in LDS by the
+// kernels that draw them) and short polynomials written with explicit
+// fmaf, so the oracle's C replica produces the same bits:
+//   sc[j] = {sin(2 pi j/256), cos(2 pi j/256)},
+//   lg[j] = {ln(1 + j/256), 256/(256 + j)}   (float roundings of the
+//           double values; the host builds them with build_normal_tables).
+constexpr int kNormTab = 256;
+struct NormalTab {
+    float2 sc[kNormTab];
+    float2 lg[kNormTab];
+};
+
+// -2 ln(u1) for u1 = (a + 1) 2^-21, a < 2^21: k = a + 1 = 2^e m with m in
+// [1, 2); ln m = lg[j].x + ln(1 + t), t = (m - m_j) / m_j, |t| < 2^-8, to
+// third order.  Next to u1 = 1 (k > 2^21 - 2^13), where that sum would
+// cancel, -ln(1 - y) with y = 1 - u1 exact, to fourth order.
+__device__ __forceinline__ float neg2ln_u21(const float2 *lg, uint32_t a)
+{
+    const uint32_t k = a + 1u;
+    const uint32_t bits = __float_as_uint((float)k);  // exact: k <= 2^21
+    const int e = (int)(bits >> 23) - 127;
+    const float2 L = lg[(bits >> 15) & 0xffu];
+    const float t = (float)(bits & 0x7fffu) * 0x1p-23f * L.y;
+    const float q = __builtin_fmaf(__builtin_fmaf(t, 0x1.555556p-2f, -0.5f), t, 1.0f);
+    float lnu = __builtin_fmaf((float)(e - 21), 0x1.62e43p-1f, L.x + t * q);
+    if (k > 0x1fe000u) {
+        const float y = (float)(0x200000u - k) * 0x1p-21f;
+        const float r = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(y, 0.25f, 0x1.555556p-2f), y, 0.5f), y, 1.0f);
+        lnu = -(y * r);
+    }
+    return -2.0f * lnu;
+}
+
+// sin and cos of 2 pi b 2^-21, b < 2^21: table angle 2 pi (b >> 13)/256
+// plus delta < 2 pi/256 (sin to delta^3, cos to delta^4).
+__device__ __forceinline__ void sincos2pi_u21(const float2 *sc, uint32_t b, float &s, float &c)
+{
+    const float d = (float)(b & 0x1fffu) * 0x1.921fb6p-19f;  // 2 pi 2^-21
+    const float d2 = d * d;
+    const float sd = d * __builtin_fmaf(d2, -0x1.555556p-3f, 1.0f);
+    const float cd = __builtin_fmaf(d2, __builtin_fmaf(d2, 0x1.555556p-5f, -0.5f), 1.0f);
+    const float2 T = sc[b >> 13];
+    s = __builtin_fmaf(T.x, cd, T.y * sd);
+    c = __builtin_fmaf(T.y, cd, -(T.x * sd));
+}
+
+__device__ __forceinline__ void box_muller_u21(const NormalTab &T, uint32_t a, uint32_t b, float &n0, float &n1)
+{
+    const float r = __builtin_sqrtf(neg2ln_u21(T.lg, a));
+    float s, c;
+    sincos2pi_u21(T.sc, b, s, c);
+    n0 = r * c;
+    n1 = r * s;
+}
+
 // The five quantum-excitation normals (r0 r1 r2 r3 r5) of one particle:
 // one Philox4x32-10 block {index, turn, 0} cut into six 21-bit uniforms
 // (bits 0-20, 21-41, 42-62 of the two 64-bit halves), three Box-Muller
 // pairs, the sixth normal unused.  21-bit radii reach 5.4 sigma.
-__device__ __forceinline__ void qe_normals(uint64_t seed, uint32_t turn, uint64_t index,
+__device__ __forceinline__ void qe_normals(const NormalTab &T, uint64_t seed, uint32_t turn, uint64_t index,
                                            double out[5])
 {
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
@@ -107,14 +164,21 @@ __device__ __forceinline__ void qe_normals(uint64_t seed, uint32_t turn, uint64_
     const u32x4 a = philox4x32_10(i0, i1, turn, 0u, k0, k1);
     const uint64_t lo = ((uint64_t)a.v[1] << 32) | a.v[0], hi = ((uint64_t)a.v[3] << 32) | a.v[2];
     const uint32_t m21 = 0x1fffffu;
-    const uint32_t u[6] = {(uint32_t)lo & m21, (uint32_t)(lo >> 21) & m21, (uint32_t)(lo >> 42) & m21,
-                           (uint32_t)hi & m21, (uint32_t)(hi >> 21) & m21, (uint32_t)(hi >> 42) & m21};
     float n0, n1, n2, n3, n4, n5;
-    // 21-bit values scaled to the 24-bit grid of box_muller24
-    box_muller24(u[0] << 3, u[1] << 3, n0, n1);
-    box_muller24(u[2] << 3, u[3] << 3, n2, n3);
-    box_muller24(u[4] << 3, u[5] << 3, n4, n5);
+    box_muller_u21(T, (uint32_t)lo & m21, (uint32_t)(lo >> 21) & m21, n0, n1);
+    box_muller_u21(T, (uint32_t)(lo >> 42) & m21, (uint32_t)hi & m21, n2, n3);
+    box_muller_u21(T, (uint32_t)(hi >> 21) & m21, (uint32_t)(hi >> 42) & m21, n4, n5);
     out[0] = n0; out[1] = n1; out[2] = n2; out[3] = n3; out[4] = n4;
+}
+
+// Stage the normal tables from global memory into LDS (all threads of the
+// block take part; the caller synchronizes).
+__device__ __forceinline__ void load_normal_tab(NormalTab &T, const float2 *g, int tid, int nthreads)
+{
+    for (int j = tid; j < 2 * kNormTab; j += nthreads) {
+        if (j < kNormTab) T.sc[j] = g[j];
+        else T.lg[j - kNormTab] = g[j];
+    }
 }
 
 // ------------------------------------------------------- section matrices

@@ -70,8 +70,12 @@ struct PassParams {
     int64_t ntot;
     uint32_t ops;
     int32_t nseg, nbins, n_rf, noise_mode;
+    // turn number keying the noise stream: turn, or *turn_dev + turn when
+    // turn_dev is set (graph replays read the turn from device memory)
     uint32_t turn;
+    const uint32_t *turn_dev;
     uint64_t seed;
+    const float2 *ntab;  // normal tables (NormalTab layout: sc then lg)
 };
 
 struct IonKickParams {
@@ -112,7 +116,8 @@ struct IonTurnParams {
     int64_t cap;             // slots per species buffer
     uint64_t seed;
     int32_t nb, n_new, w_old, w_new;  // waves of old ions, waves per passage of new ions
-    uint32_t turn;
+    uint32_t turn;                    // or *turn_dev + turn when turn_dev is set
+    const uint32_t *turn_dev;
 };
 
 // Cloud moments and electron-side parameters of every passage.
@@ -167,6 +172,10 @@ struct FinParams {
     double *chunk_part;   // [nb][max_chunks][kNPart]
     uint32_t *chunk_cnt;  // [nb], left at zero after each launch
     int32_t max_chunks;
+    // history slots from the device turn counter (graph replays):
+    // slot = t % hist_cap, prev_slot = (t - 1) % hist_cap, t = *turn_dev + turn_off
+    const uint32_t *turn_dev;
+    int32_t turn_off, hist_cap;
 };
 
 struct WakeTableParams {
@@ -190,6 +199,11 @@ struct WakeParams {
 };
 
 hipError_t launch_pass(const PassParams &p, int nwg, hipStream_t st);
+// Host: the NormalTab contents (2 * kNormTab float2, sc then lg).
+void build_normal_tables(float2 *out);
+// device turn counter of graph replays: *t = v, and *t += by
+hipError_t launch_set_turn(uint32_t *t, uint32_t v, hipStream_t st);
+hipError_t launch_advance_turn(uint32_t *t, uint32_t by, hipStream_t st);
 hipError_t launch_bin(const BinParams &p, int nwg, hipStream_t st);
 hipError_t launch_ion_kick(const IonKickParams &p, int nwg, hipStream_t st);
 hipError_t launch_finalize(const FinParams &p, hipStream_t st);
@@ -204,7 +218,7 @@ hipError_t launch_ion_finish(const IonFinishParams &p, hipStream_t st);
 hipError_t launch_ion_compact(const IonCompactParams &p, int64_t len_bound, hipStream_t st);
 constexpr int kIonBlock = 256;
 constexpr int kIonChunk = 4096;  // compaction chunk (16 ions per thread)
-hipError_t launch_qe_debug(uint64_t seed, uint32_t turn, uint64_t base, int64_t n, double *out,
+hipError_t launch_qe_debug(const float2 *tab, uint64_t seed, uint32_t turn, uint64_t base, int64_t n, double *out,
                            hipStream_t st);
 hipError_t upload_faddeeva_tables(const double ea2n2[64], const double fad[3]);
 hipError_t upload_faddeeva_tables_ions(const double ea2n2[64], const double fad[3]);

@@ -81,13 +81,21 @@ __global__ void __launch_bounds__(256)
 k_ion_turn(IonTurnParams P)
 {
     __shared__ __attribute__((aligned(16))) MlpLds L;
+    const int p = blockIdx.y;
+    const int64_t len = P.len[p];
+    {
+        // blocks whose four waves all have nothing to do leave before the
+        // (block-wide) weight load: the grid covers a bound of the old ions
+        const int wb = blockIdx.x * 4;
+        if (wb >= P.w_old + P.nb * P.w_new) return;
+        if (wb + 3 < P.w_old && (int64_t)wb * 64 >= len) return;
+    }
     if (SUR) {
         load_mlp_lds(L, P.sur_params, threadIdx.x, 256);
         __syncthreads();
     }
     const int lane = threadIdx.x & 63;
-    const int p = blockIdx.y;
-    const int64_t len = P.len[p];
+    const uint32_t turn = P.turn_dev ? *P.turn_dev + P.turn : P.turn;
     const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (w >= P.w_old + P.nb * P.w_new) return;
     if (w < P.w_old && (int64_t)w * 64 >= len) return;  // whole wave past the old ions
@@ -116,7 +124,7 @@ k_ion_turn(IonTurnParams P)
         alive = born;
         if (alive) {
             const double *bv = P.bview + 4 * j0;
-            ion_generate(P.seed, P.turn, (uint32_t)j0, p, (uint32_t)(slot - len - (int64_t)j0 * P.n_new), bv[0],
+            ion_generate(P.seed, turn, (uint32_t)j0, p, (uint32_t)(slot - len - (int64_t)j0 * P.n_new), bv[0],
                          bv[1], bv[2], bv[3], x, y);
         }
     }

@@ -158,6 +158,12 @@ k_pass(PassParams P)
     const int n_rf = NRF >= 0 ? NRF : P.n_rf;
     const int tid = threadIdx.x;
     const int s_first = P.wg_seg[blockIdx.x], s_last = P.wg_seg[blockIdx.x + 1];
+    const uint32_t turn = P.turn_dev ? *P.turn_dev + P.turn : P.turn;
+    __shared__ NormalTab ntab;
+    if ((ops & OP_SYNRAD) && noise_mode == 2) {
+        load_normal_tab(ntab, P.ntab, tid, kBlock);
+        __syncthreads();
+    }
 
     for (int s = s_first; s < s_last; s++) {
         const int b = P.seg_bunch[s];
@@ -271,7 +277,7 @@ k_pass(PassParams P)
 #pragma unroll
                     for (int j = 0; j < 5; j++) r[j] = P.noise[j * P.ntot + i];
                 } else if (noise_mode == 2) {
-                    qe_normals(P.seed, P.turn, (uint64_t)i, r);
+                    qe_normals(ntab, P.seed, turn, (uint64_t)i, r);
                 }
                 stage_synrad(X, reload(P.cst), r, noise_mode != 0);
             }
@@ -462,6 +468,12 @@ __device__ __forceinline__ void finalize_body(const FinParams &P, int c, int b)
     double *sh = P.shift + 6 * b;
     const double ncount = (double)P.bunch_count[b];
     cts_moments *cur = P.cur + b;
+    int32_t slot = P.slot, prev_slot = P.prev_slot;
+    if (P.turn_dev && P.hist_cap > 0) {
+        const uint32_t t = *P.turn_dev + (uint32_t)P.turn_off;
+        slot = (int32_t)(t % (uint32_t)P.hist_cap);
+        prev_slot = t >= 1u ? (int32_t)((t - 1u) % (uint32_t)P.hist_cap) : -1;
+    }
     double dmean = 0, dzd = 0, ddd = 0;
     if (P.ops & (OP_MOMB | OP_MOMD)) {
         const double n = tot[B_N];
@@ -475,7 +487,7 @@ __device__ __forceinline__ void finalize_body(const FinParams &P, int c, int b)
         cur->mean[5] = dmean;
         cur->c2[7] = dzd;
         cur->c2[8] = ddd;
-        if (P.hist && P.prev_slot >= 0) P.hist[(int64_t)P.prev_slot * P.nb + b] = *cur;
+        if (P.hist && prev_slot >= 0) P.hist[(int64_t)prev_slot * P.nb + b] = *cur;
         sh[5] = dmean;
     }
     if (P.ops & OP_MOMA) {
@@ -506,8 +518,8 @@ __device__ __forceinline__ void finalize_body(const FinParams &P, int c, int b)
         cur->transmission = (ncount - tot[A_LOSS]) / ncount;
         if (cur->transmission < 0.5) atomicOr(P.status, 1u);
     }
-    if ((P.ops & (OP_MOMA | OP_MOMD | OP_LOSS)) && P.hist && P.slot >= 0)
-        P.hist[(int64_t)P.slot * P.nb + b] = *cur;
+    if ((P.ops & (OP_MOMA | OP_MOMD | OP_LOSS)) && P.hist && slot >= 0)
+        P.hist[(int64_t)slot * P.nb + b] = *cur;
     if (P.ops & OP_ZSTAT) {
         const double n = tot[C_N];
         const double mz = tot[C_SZ] / n;
@@ -728,12 +740,16 @@ k_ion_kick_fb(IonKickParams P)
 }
 
 // -------------------------------------------------------- test hooks
-__global__ void k_qe_debug(uint64_t seed, uint32_t turn, uint64_t base, int64_t n, double *out)
+__global__ void __launch_bounds__(kBlock)
+k_qe_debug(const float2 *tab, uint64_t seed, uint32_t turn, uint64_t base, int64_t n, double *out)
 {
+    __shared__ NormalTab T;
+    load_normal_tab(T, tab, threadIdx.x, kBlock);
+    __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double r[5];
-    qe_normals(seed, turn, base + (uint64_t)i, r);
+    qe_normals(T, seed, turn, base + (uint64_t)i, r);
 #pragma unroll
     for (int j = 0; j < 5; j++) out[j * n + i] = r[j];
 }
@@ -782,6 +798,24 @@ hipError_t launch_ion_kick(const IonKickParams &p, int nwg, hipStream_t st)
     hipLaunchKernelGGL(k_ion_kick, dim3(nwg), dim3(kBlock), 0, st, p);
     return hipGetLastError();
 }
+__global__ void k_set_turn(uint32_t *t, uint32_t v)
+{
+    if (threadIdx.x == 0) *t = v;
+}
+__global__ void k_advance_turn(uint32_t *t, uint32_t by)
+{
+    if (threadIdx.x == 0) *t += by;
+}
+hipError_t launch_set_turn(uint32_t *t, uint32_t v, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_set_turn, dim3(1), dim3(64), 0, st, t, v);
+    return hipGetLastError();
+}
+hipError_t launch_advance_turn(uint32_t *t, uint32_t by, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_advance_turn, dim3(1), dim3(64), 0, st, t, by);
+    return hipGetLastError();
+}
 hipError_t launch_bin(const BinParams &p, int nwg, hipStream_t st)
 {
     hipLaunchKernelGGL(k_bin, dim3(nwg), dim3(kBlock), (size_t)p.nbins * sizeof(uint32_t), st, p);
@@ -812,11 +846,11 @@ hipError_t launch_wake(const WakeParams &p, int nb, hipStream_t st)
     hipLaunchKernelGGL(k_wake, dim3((p.nbins + 63) / 64, nb), dim3(64), shm, st, p);
     return hipGetLastError();
 }
-hipError_t launch_qe_debug(uint64_t seed, uint32_t turn, uint64_t base, int64_t n, double *out,
+hipError_t launch_qe_debug(const float2 *tab, uint64_t seed, uint32_t turn, uint64_t base, int64_t n, double *out,
                            hipStream_t st)
 {
     const int blocks = (int)((n + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_qe_debug, dim3(blocks), dim3(kBlock), 0, st, seed, turn, base, n, out);
+    hipLaunchKernelGGL(k_qe_debug, dim3(blocks), dim3(kBlock), 0, st, tab, seed, turn, base, n, out);
     return hipGetLastError();
 }
 

@@ -105,6 +105,9 @@ class Tracker:
     def turn(self, n):
         return capi.check(self.lib.cts_turn(self.h, n))
 
+    def run_turns(self, first, nturns, graph=True):
+        return capi.check(self.lib.cts_run_turns(self.h, first, nturns, int(graph)))
+
     def autotune(self):
         n = ctypes.c_int32()
         capi.check(self.lib.cts_autotune(self.h, ctypes.byref(n)))

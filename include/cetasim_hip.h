@@ -170,6 +170,15 @@ int cts_set_ions(cts_handle *h, const cts_ion_config *cfg);
 int cts_turn(cts_handle *h, int32_t turn);
 /* Apply a pending wake kick and complete the last turn's moments. */
 int cts_flush(cts_handle *h);
+/* Turns first .. first+nturns-1 of MPBeam::Run's loop (src/MPBeam.cpp:
+ * 1010-1173), each as cts_turn.  use_graph = 1 replays them two at a time
+ * from a HIP graph captured from the same kernels (the turn number is read
+ * from device memory), once the loop is in its steady state (the previous
+ * turn was first-1 and, with a wake, its kick is pending); other turns and
+ * host-supplied noise run eagerly.  Results are bit-identical to cts_turn.
+ * The graphs are rebuilt after any cts_set_* call or cts_autotune; the
+ * transmission/ion status is polled after every replay. */
+int cts_run_turns(cts_handle *h, int32_t first, int32_t nturns, int32_t use_graph);
 /* Autotuner (no counterpart in the reference): times the turn's streaming
  * pass on a copy of the particles for workgroup counts that are multiples
  * of the device's compute units, and keeps the fastest layout; *nwg_out

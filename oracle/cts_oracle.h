@@ -197,6 +197,10 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 void orc_qe_normals(uint64_t seed, uint32_t turn, uint64_t index, double out[5]);
 /* One standard-normal Box-Muller pair from two 32-bit words. */
 void orc_box_muller(uint32_t a, uint32_t b, float *n0, float *n1);
+/* the quantum-excitation pair transform on 21-bit uniforms, and its
+ * tables (out[1024]: sc[256][2] then lg[256][2]) */
+void orc_box_muller_u21(uint32_t a, uint32_t b, float *n0, float *n1);
+void orc_normal_tables(float *out);
 /* Fill SoA noise[5][n] for one bunch, global particle offset `base`. */
 void orc_qe_fill(uint64_t seed, uint32_t turn, uint64_t base, int64_t n, double *noise);
 

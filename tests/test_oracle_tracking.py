@@ -303,6 +303,37 @@ def test_qe_normals_are_standard_normal():
         assert f1.value == pytest.approx(r * math.sin(2 * math.pi * u2), abs=3e-6)
 
 
+def test_qe_transform_accuracy():
+    """The table-driven Box-Muller pair of the excitation normals against
+    libm in double: every table interval, both ends of the radius range
+    (including the fourth-order branch next to u1 = 1) and random points;
+    the tables against their defining values."""
+    L.orc_box_muller_u21.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float),
+                                     ctypes.POINTER(ctypes.c_float)]
+    L.orc_normal_tables.argtypes = [ctypes.POINTER(ctypes.c_float)]
+    tab = (ctypes.c_float * 1024)()
+    L.orc_normal_tables(tab)
+    t = np.array(tab[:], dtype=np.float64)
+    j = np.arange(256)
+    np.testing.assert_allclose(t[0:512:2], np.sin(2 * np.pi * j / 256), atol=6e-8)
+    np.testing.assert_allclose(t[1:512:2], np.cos(2 * np.pi * j / 256), atol=6e-8)
+    np.testing.assert_allclose(t[512::2], np.log1p(j / 256), rtol=6e-8, atol=1e-30)
+    np.testing.assert_allclose(t[513::2], 256 / (256 + j), rtol=6e-8)
+    rng = np.random.default_rng(1)
+    a = np.concatenate([np.arange(0, 2 ** 21, 2053), np.arange(2 ** 21 - 9000, 2 ** 21), np.arange(300),
+                        rng.integers(0, 2 ** 21, 3000)])
+    b = np.concatenate([rng.integers(0, 2 ** 21, len(a) - 512), np.arange(256) << 13, (np.arange(256) << 13) + 8191])
+    f0, f1 = ctypes.c_float(), ctypes.c_float()
+    worst = 0.0
+    for ai, bi in zip(a.tolist(), b.tolist()):
+        L.orc_box_muller_u21(ai, bi, f0, f1)
+        r = math.sqrt(-2 * math.log((ai + 1) * 2.0 ** -21))
+        th = 2 * math.pi * bi * 2.0 ** -21
+        worst = max(worst, abs(f0.value - r * math.cos(th)), abs(f1.value - r * math.sin(th)))
+    # float results: a few units of the last place of values up to 5.4
+    assert worst < 1.5e-6, worst
+
+
 def test_fit_wlinear_exact_line():
     x = np.arange(10, dtype=float)
     y = 3.5 - 0.25 * x
