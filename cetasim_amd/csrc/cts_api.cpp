@@ -335,6 +335,23 @@ void build_consts(const cts_ring &R, const std::vector<cts_section> &secs, int k
     c[CI_APY] = R.ap_y;
     c[CI_IAPX] = 1.0 / R.ap_x;
     c[CI_IAPY] = 1.0 / R.ap_y;
+    // angle functions of the map and the RF kick (sincos_small_k and
+    // sincos_fast_k in cts_kernels.hip)
+    c[CI_TWOPI] = 2 * kPi;
+    c[CI_SMALL] = 0.1;
+    const double ps[4] = {2.7557319223985893e-06, -0.0001984126984126984, 0.008333333333333333,
+                          -0.16666666666666666};
+    const double pc[4] = {-2.755731922398589e-07, 2.48015873015873e-05, -0.001388888888888889,
+                          0.041666666666666664};
+    const double fr[4] = {0.6366197723675814, 0x1.921fb548p+0, -0x1.de973dc8p-31, -0x1.9d9cceba3f91fp-62};
+    const double fs[7] = {-7.647163731819816e-13, 1.6059043836821613e-10, -2.505210838544172e-08,
+                          2.7557319223985893e-06, -0.0001984126984126984, 0.008333333333333333,
+                          -0.16666666666666666};
+    const double fc[7] = {-4.779477332387385e-14, 1.1470745597729725e-11, -2.08767569878681e-09,
+                          2.755731922398589e-07, -2.48015873015873e-05, 0.001388888888888889,
+                          -0.041666666666666664};
+    for (int i = 0; i < 4; i++) { c[CI_PS + i] = ps[i]; c[CI_PC + i] = pc[i]; c[CI_FR + i] = fr[i]; }
+    for (int i = 0; i < 7; i++) { c[CI_FS + i] = fs[i]; c[CI_FC + i] = fc[i]; }
 }
 
 PassParams base_pass(cts_handle *h, uint32_t ops, int k)

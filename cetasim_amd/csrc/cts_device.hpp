@@ -221,57 +221,6 @@ __device__ __forceinline__ void spmv_k(cdptr A, const double x[6], double y[6])
     y[5] = A[15] * x[5];
 }
 
-// sin and cos of a small angle |d| <= 0.1 by Taylor series to degree 9/10
-// (truncation below 3e-19).
-__device__ __forceinline__ void sincos_small(double d, double &s, double &c)
-{
-#pragma clang fp contract(fast)
-    const double d2 = d * d;
-    double sp = 2.7557319223985893e-06;
-    sp = -0.0001984126984126984 + d2 * sp;
-    sp = 0.008333333333333333 + d2 * sp;
-    sp = -0.16666666666666666 + d2 * sp;
-    s = d + d * (d2 * sp);
-    double cp = -2.755731922398589e-07;
-    cp = 2.48015873015873e-05 + d2 * cp;
-    cp = -0.001388888888888889 + d2 * cp;
-    cp = 0.041666666666666664 + d2 * cp;
-    c = 1.0 - 0.5 * d2 + (d2 * d2) * cp;
-}
-
-// sin and cos in FP64: Cody-Waite reduction by pi/2 split into two 30-bit
-// parts and a remainder (n*P1, n*P2 exact for |n| < 2^23, i.e. |x| < 1.3e7),
-// Taylor series on [-pi/4, pi/4] to degree 15/16 (truncation below 1e-16
-// relative).  Beyond 1.3e7 rad the reduction loses bits; no particle that
-// stays in the machine has a phase advance of that size.
-__device__ __forceinline__ void sincos_fast(double x, double &s, double &c)
-{
-#pragma clang fp contract(fast)
-    const double n = rint(x * 0.6366197723675814);
-    double r = x - n * 0x1.921fb548p+0;
-    r = r - n * -0x1.de973dc8p-31;
-    r = r - n * -0x1.9d9cceba3f91fp-62;
-    const double r2 = r * r;
-    double sp = -7.647163731819816e-13;
-    sp = 1.6059043836821613e-10 + r2 * sp;
-    sp = -2.505210838544172e-08 + r2 * sp;
-    sp = 2.7557319223985893e-06 + r2 * sp;
-    sp = -0.0001984126984126984 + r2 * sp;
-    sp = 0.008333333333333333 + r2 * sp;
-    sp = -0.16666666666666666 + r2 * sp;
-    const double sn = r + r * (r2 * sp);
-    double cp = -4.779477332387385e-14;
-    cp = 1.1470745597729725e-11 + r2 * cp;
-    cp = -2.08767569878681e-09 + r2 * cp;
-    cp = 2.755731922398589e-07 + r2 * cp;
-    cp = -2.48015873015873e-05 + r2 * cp;
-    cp = 0.001388888888888889 + r2 * cp;
-    cp = -0.041666666666666664 + r2 * cp;
-    const double cs = 1.0 - 0.5 * r2 - r2 * (r2 * cp);
-    const int q = (int)n & 3;
-    s = (q == 0) ? sn : (q == 1) ? cs : (q == 2) ? -sn : -cs;
-    c = (q == 0) ? cs : (q == 1) ? -sn : (q == 2) ? -cs : sn;
-}
 
 // ---------------------------------------------------------- reductions
 __device__ __forceinline__ double wave_sum(double v)

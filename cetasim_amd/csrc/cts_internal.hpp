@@ -52,7 +52,14 @@ enum : int {
     CI_DRIFTC = 104, CI_ALPHAC = 105, CI_U0 = 108, CI_LAM = 109, CI_LZ2 = 112, CI_COEF = 113,
     CI_ZLIM = 116, CI_APX = 117, CI_APY = 118, CI_IAPX = 119, CI_IAPY = 120,
     CI_CMUX = 121, CI_SMUX = 122, CI_CMUY = 123, CI_SMUY = 124,
-    kNConst = 128
+    // polynomial coefficients and thresholds of the angle functions: kept
+    // in the block so they reach the ALU as scalar operands instead of
+    // occupying vector registers for the whole kernel
+    CI_TWOPI = 125, CI_SMALL = 126,
+    CI_PS = 128, CI_PC = 132,               // small-angle sin, cos (4 each)
+    CI_FR = 136,                            // 2/pi and the three parts of pi/2
+    CI_FS = 140, CI_FC = 147,               // reduced-angle sin, cos (7 each)
+    kNConst = 160
 };
 
 struct PassParams {

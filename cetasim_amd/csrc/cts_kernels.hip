@@ -22,6 +22,18 @@ hipError_t upload_faddeeva_tables(const double ea2n2[64], const double fad[3])
     return hipMemcpyToSymbol(HIP_SYMBOL(c_fad), fad, 3 * sizeof(double));
 }
 
+// Streaming (nontemporal) global accesses.
+template <class T>
+__device__ __forceinline__ T ntload(const T *p)
+{
+    return __builtin_nontemporal_load(p);
+}
+template <class T>
+__device__ __forceinline__ void ntstore(T *p, T v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+
 // Bin of z for edges (lo, dz) as the reference computes it,
 // int((z - lo)/dz) (src/MPBunch.cpp:1917), or -1 outside [0, nbins).  The
 // quotient is formed with the reciprocal and redone with the division when
@@ -48,6 +60,47 @@ __device__ __forceinline__ bool outside_aperture(double x, double y, cdptr K)
     return t > 1;
 }
 
+// sin and cos for |d| <= 0.1: Taylor polynomials in d^2 (coefficients
+// K[CI_PS..], K[CI_PC..]: 1/9!, -1/7!, 1/5!, -1/3! and -1/10!, 1/8!,
+// -1/6!, 1/4!).
+__device__ __forceinline__ void sincos_small_k(double d, double &s, double &c, cdptr K)
+{
+#pragma clang fp contract(fast)
+    const double d2 = d * d;
+    double sp = K[CI_PS];
+    sp = K[CI_PS + 1] + d2 * sp;
+    sp = K[CI_PS + 2] + d2 * sp;
+    sp = K[CI_PS + 3] + d2 * sp;
+    s = d + d * (d2 * sp);
+    double cp = K[CI_PC];
+    cp = K[CI_PC + 1] + d2 * cp;
+    cp = K[CI_PC + 2] + d2 * cp;
+    cp = K[CI_PC + 3] + d2 * cp;
+    c = 1.0 - 0.5 * d2 + (d2 * d2) * cp;
+}
+
+// sin and cos of any x: reduction by pi/2 in three parts (Cody-Waite,
+// K[CI_FR..]: 2/pi, pi/2 split), polynomials of degree 15/14 on the
+// reduced angle (K[CI_FS..], K[CI_FC..]), quadrant selection.
+__device__ __forceinline__ void sincos_fast_k(double x, double &s, double &c, cdptr K)
+{
+#pragma clang fp contract(fast)
+    const double n = rint(x * K[CI_FR]);
+    double r = x - n * K[CI_FR + 1];
+    r = r - n * K[CI_FR + 2];
+    r = r - n * K[CI_FR + 3];
+    const double r2 = r * r;
+    double sp = K[CI_FS];
+    for (int i = 1; i < 7; i++) sp = K[CI_FS + i] + r2 * sp;
+    const double sn = r + r * (r2 * sp);
+    double cp = K[CI_FC];
+    for (int i = 1; i < 7; i++) cp = K[CI_FC + i] + r2 * cp;
+    const double cs = 1.0 - 0.5 * r2 - r2 * (r2 * cp);
+    const int q = (int)n & 3;
+    s = (q == 0) ? sn : (q == 1) ? cs : (q == 2) ? -sn : -cs;
+    c = (q == 0) ? cs : (q == 1) ? -sn : (q == 2) ? -cs : sn;
+}
+
 // Section map, src/Bunch.cpp:625-755.
 __device__ __forceinline__ void stage_map(double X[6], cdptr K)
 {
@@ -67,23 +120,23 @@ __device__ __forceinline__ void stage_map(double X[6], cdptr K)
                          K[CI_ADTY + 4] * ampX * ampY) * K[CI_WY];
     // phi = mu + 2*pi*dnu: for the usual small tune shift, angle addition
     // onto the section's cos/sin(mu); otherwise the full reduction
-    const double ddx = 2 * kPi * dnux, ddy = 2 * kPi * dnuy;
+    const double ddx = K[CI_TWOPI] * dnux, ddy = K[CI_TWOPI] * dnuy;
     double sx, cx, sy, cy;
-    if (fabs(ddx) <= 0.1) {
+    if (fabs(ddx) <= K[CI_SMALL]) {
         double sd, cd;
-        sincos_small(ddx, sd, cd);
+        sincos_small_k(ddx, sd, cd, K);
         sx = K[CI_SMUX] * cd + K[CI_CMUX] * sd;
         cx = K[CI_CMUX] * cd - K[CI_SMUX] * sd;
     } else {
-        sincos_fast(K[CI_PHX] + ddx, sx, cx);
+        sincos_fast_k(K[CI_PHX] + ddx, sx, cx, K);
     }
-    if (fabs(ddy) <= 0.1) {
+    if (fabs(ddy) <= K[CI_SMALL]) {
         double sd, cd;
-        sincos_small(ddy, sd, cd);
+        sincos_small_k(ddy, sd, cd, K);
         sy = K[CI_SMUY] * cd + K[CI_CMUY] * sd;
         cy = K[CI_CMUY] * cd - K[CI_SMUY] * sd;
     } else {
-        sincos_fast(K[CI_PHY] + ddy, sy, cy);
+        sincos_fast_k(K[CI_PHY] + ddy, sy, cy, K);
     }
     double Y[6], R[6];
     spmv_k(K + CI_B1H1, X, Y);
@@ -200,10 +253,12 @@ k_pass(PassParams P)
         // the current one is computed, so HBM latency overlaps the VALU work
         double Xn[6] = {0, 0, 0, 0, 0, 0};
         uint32_t fn = 0;
+        // the coordinates are streamed once per pass: nontemporal loads and
+        // stores keep them from displacing the tables in the caches
         if ((uint32_t)tid < nseg_i) {
-            Xn[0] = sx_[tid]; Xn[1] = spx_[tid]; Xn[2] = sy_[tid];
-            Xn[3] = spy_[tid]; Xn[4] = sz_[tid]; Xn[5] = sdp_[tid];
-            fn = sfl_[tid];
+            Xn[0] = ntload(sx_ + tid); Xn[1] = ntload(spx_ + tid); Xn[2] = ntload(sy_ + tid);
+            Xn[3] = ntload(spy_ + tid); Xn[4] = ntload(sz_ + tid); Xn[5] = ntload(sdp_ + tid);
+            fn = ntload(sfl_ + tid);
         }
         for (uint32_t k = tid; k < nseg_i; k += kBlock) {
             const int64_t i = i0 + k;
@@ -213,9 +268,9 @@ k_pass(PassParams P)
             uint32_t f = fn;
             const uint32_t kn = k + kBlock;
             if (kn < nseg_i) {
-                Xn[0] = sx_[kn]; Xn[1] = spx_[kn]; Xn[2] = sy_[kn];
-                Xn[3] = spy_[kn]; Xn[4] = sz_[kn]; Xn[5] = sdp_[kn];
-                fn = sfl_[kn];
+                Xn[0] = ntload(sx_ + kn); Xn[1] = ntload(spx_ + kn); Xn[2] = ntload(sy_ + kn);
+                Xn[3] = ntload(spy_ + kn); Xn[4] = ntload(sz_ + kn); Xn[5] = ntload(sdp_ + kn);
+                fn = ntload(sfl_ + kn);
             }
 
             if (use_kick) {
@@ -258,8 +313,8 @@ k_pass(PassParams P)
                 for (int r = 0; r < n_rf; r++) {
                     double s, c;
                     const double th = X[4] * K[CI_RFK + r];
-                    if (fabs(th) <= 0.1) sincos_small(th, s, c);
-                    else sincos_fast(th, s, c);
+                    if (fabs(th) <= K[CI_SMALL]) sincos_small_k(th, s, c, K);
+                    else sincos_fast_k(th, s, c, K);
                     const double re = K[CI_RFVR + r] * c - K[CI_RFVI + r] * s;
                     X[5] += re * K[CI_RFSCALE];
                 }
@@ -325,9 +380,9 @@ k_pass(PassParams P)
                 acc[B_QZD] += dz * dd;
             }
             if (ops & OP_WRITE) {
-                sx_[k] = X[0]; spx_[k] = X[1]; sy_[k] = X[2];
-                spy_[k] = X[3]; sz_[k] = X[4]; sdp_[k] = X[5];
-                sfl_[k] = (uint8_t)fnew;
+                ntstore(sx_ + k, X[0]); ntstore(spx_ + k, X[1]); ntstore(sy_ + k, X[2]);
+                ntstore(spy_ + k, X[3]); ntstore(sz_ + k, X[4]); ntstore(sdp_ + k, X[5]);
+                ntstore(sfl_ + k, (uint8_t)fnew);
             }
         }
 
