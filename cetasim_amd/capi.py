@@ -73,6 +73,11 @@ class IonConfig(ctypes.Structure):
                 ("length", D), ("loss_boundary", D), ("ap_x", D), ("ap_y", D), ("seed", U64)]
 
 
+class CbmResult(ctypes.Structure):
+    _fields_ = [(n, PD) for n in ("ideal_amp", "ideal_arg", "iq_amp", "iq_arg", "ana_re", "ana_im", "hil_amp",
+                                  "hil_arg", "rate_ideal", "rate_iq", "rate_hil", "rate_bunch")]
+
+
 class RingDerived(ctypes.Structure):
     _fields_ = [("eta", D), ("workqz", D), ("natural_bunch_length", D), ("beta_z", D)]
 
@@ -131,6 +136,8 @@ SIGNATURES = {
     "cts_cbm_growth_from_moments": (I32, [ctypes.POINTER(Moments), I32, I32, PI32, I32, PD, PD, I32, I32, I32, D,
                                           PD, PD]),
     "cts_fit_wlinear": (I32, [PD, PD, PD, I32, PD, PD]),
+    "cts_cbm_analysis": (I32, [ctypes.POINTER(Moments), I32, I32, PI32, I32, PD, PD, I32, I32, I32, D,
+                               ctypes.POINTER(CbmResult)]),
     "cts_growth_rates": (I32, [ctypes.c_void_p, I32, I32, I32, I32, I32, D, D, PD]),
     "cts_build_section": (I32, [ctypes.POINTER(Twiss), ctypes.POINTER(Twiss), D, D, ctypes.POINTER(Section)]),
     "cts_derive_ring": (I32, [ctypes.POINTER(RingInput), ctypes.POINTER(Ring), ctypes.POINTER(RingDerived)]),

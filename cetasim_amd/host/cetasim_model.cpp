@@ -5,8 +5,10 @@
 #include <algorithm>
 #include <cctype>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <sstream>
 
@@ -556,6 +558,11 @@ int MPBeam::Initial(Train &train, LatticeInterActionPoint &lat, ReadInputSetting
 
     cts_ring ring{};
     if (DeviceRing(lat, in, ring)) return 1;
+    cavity_.clear();
+    for (int j = 0; j < ring.n_rf; j++) {
+        cavity_.push_back(ring.rf_vr[j]);
+        cavity_.push_back(ring.rf_vi[j]);
+    }
     if (check(cts_create(&ring, lat.sections.data(), (int32_t)lat.sections.size(), &gpu), "cts_create")) return 1;
     std::vector<int64_t> counts(nb);
     std::vector<double> q(nb), ne(nb);
@@ -641,36 +648,179 @@ int MPBeam::DownloadParticles()
 int MPBeam::GetCBMGR(ReadInputSettings &in, std::vector<double> &rates)
 {
     const auto *R = in.ringRun;
-    const int interval = R->bunchInfoPrintInterval;
+    const int interval = std::max(R->bunchInfoPrintInterval, 1);
     const int nb = (int)beamVec.size();
-    const int nsamples = (int)(samples_.size() / (size_t)std::max(nb, 1));
-    rates.assign(3 * (size_t)nb, 0.0);
+    const int ns = (int)(samples_.size() / (size_t)std::max(nb, 1));
+    const size_t N = (size_t)ns * 3 * nb;
     std::vector<int32_t> harm(nb);
     for (int b = 0; b < nb; b++) harm[b] = beamVec[b].bunchHarmNum;
-    // GetCBMGR reads workQz and rmsBunchLength / rmsEnergySpread of the
+    // GetCBMGR reads workQx/y/z and rmsBunchLength / rmsEnergySpread of the
     // input (src/MPBeam.cpp:1498-1508)
     const double tunes[3] = {in.ringParBasic->workQx, in.ringParBasic->workQy, in.ringParBasic->workQz};
     const double twiss[5] = {alphaX0_, betaX0_, alphaY0_, betaY0_,
                              in.ringBunchPara->rmsBunchLength / in.ringBunchPara->rmsEnergySpread};
-    return check(cts_cbm_growth_from_moments(samples_.data(), nsamples, nb, harm.data(), in.ringParBasic->harmonics,
-                                             tunes, twiss, interval, R->growthRateFittingStart,
-                                             R->growthRateFittingEnd, in.ringParBasic->t0, rates.data(), nullptr),
-                 "growth rates");
+    std::vector<double> a[8], r[4];
+    for (auto &v : a) v.assign(N, 0.0);
+    for (auto &v : r) v.assign(3 * (size_t)nb, 0.0);
+    cts_cbm_result res{a[0].data(), a[1].data(), a[2].data(), a[3].data(), a[4].data(), a[5].data(),
+                       a[6].data(), a[7].data(), r[0].data(), r[1].data(), r[2].data(), r[3].data()};
+    if (check(cts_cbm_analysis(samples_.data(), ns, nb, harm.data(), in.ringParBasic->harmonics, tunes, twiss,
+                               interval, R->growthRateFittingStart, R->growthRateFittingEnd, in.ringParBasic->t0,
+                               &res), "growth rates"))
+        return 1;
+    rates = r[0];
+    // runCBMGR.sdds in the reference's layout (src/MPBeam.cpp:1735-1886),
+    // its repeated columns included (ArgIQY/Z print the x phase;
+    // AbsCBMHilbertAnaY and ArgCBMHilbertAnaY print x)
+    std::ofstream fout(R->runCBMGR + ".sdds");
+    const char *head[] = {
+        "SDDS1",
+        "&parameter name=dataPointsForFit                       type=long     &end",
+        "&parameter name=ModeIndex                              type=long     &end",
+        "&parameter name=CBMIdealGRX,                  units=1/s,    type=float    &end",
+        "&parameter name=CBMIdealGRY,                  units=1/s,    type=float    &end",
+        "&parameter name=CBMIdealGRZ,                  units=1/s,    type=float    &end",
+        "&parameter name=CBMIQGRX,                units=1/s,    type=float    &end",
+        "&parameter name=CBMIQGRY,                units=1/s,    type=float    &end",
+        "&parameter name=CBMIQGRZ,                units=1/s,    type=float    &end",
+        "&parameter name=CBMHTGRX,                units=1/s,    type=float    &end",
+        "&parameter name=CBMHTGRY,                units=1/s,    type=float    &end",
+        "&parameter name=CBMHTGRZ,                units=1/s,    type=float    &end",
+        "&parameter name=BunchHarmIndex                         type=long     &end",
+        "&parameter name=BGRX,                    units=1/s,    type=float    &end",
+        "&parameter name=BGRY,                    units=1/s,    type=float    &end",
+        "&parameter name=BGRZ,                    units=1/s,    type=float    &end",
+        "&column name=Turns,                                    type=long      &end",
+        "&column name=AbsCBMIdealX,                              type=float     &end",
+        "&column name=AbsCBMIdealY,                              type=float     &end",
+        "&column name=AbsCBMIdealZ,                              type=float     &end",
+        "&column name=ArgCBMIdealX,                units=rad,    type=float     &end",
+        "&column name=ArgCBMIdealY,                units=rad,    type=float     &end",
+        "&column name=ArgCBMIdealZ,                units=rad,    type=float     &end",
+        "&column name=AbsIQX,                                   type=float     &end",
+        "&column name=AbsIQY,                                   type=float     &end",
+        "&column name=AbsIQZ,                                   type=float     &end",
+        "&column name=ArgIQX,                     units=rad,    type=float     &end",
+        "&column name=ArgIQY,                     units=rad,    type=float     &end",
+        "&column name=ArgIQZ,                     units=rad,    type=float     &end",
+        "&column name=AbsCBMHilbertAnaX,         units=m,      type=float     &end",
+        "&column name=AbsCBMHilbertAnaY,         units=m,      type=float     &end",
+        "&column name=AbsCBMHilbertAnaZ,         units=m,      type=float     &end",
+        "&column name=ArgCBMHilbertAnaX,         units=rad,    type=float     &end",
+        "&column name=ArgCBMHilbertAnaY,         units=rad,    type=float     &end",
+        "&column name=ArgCBMHilbertAnaZ,         units=rad,    type=float     &end",
+        "&column name=AverX,                      units=m,      type=float     &end",
+        "&column name=AverY,                      units=m,      type=float     &end",
+        "&column name=AverZ,                      units=m,      type=float     &end",
+        "&column name=AnalyticalAverX,            units=m,      type=float     &end",
+        "&column name=AnalyticalAverY,            units=m,      type=float     &end",
+        "&column name=AnalyticalAverZ,            units=m,      type=float     &end",
+        "&column name=AbsHilbertAnaX,                 units=m,      type=float     &end",
+        "&column name=AbsHilbertAnaY,                 units=m,      type=float     &end",
+        "&column name=AbsHilbertAnaZ,                 units=m,      type=float     &end",
+        "&data mode=ascii &end"};
+    for (const char *l : head) fout << l << std::endl;
+    const int dim = R->growthRateFittingEnd / interval - R->growthRateFittingStart / interval;
+    auto at = [&](int n, int pl, int k) { return ((size_t)n * 3 + pl) * nb + k; };
+    auto w = [&](double v) { fout << std::setw(15) << std::left << v; };
+    for (int i = 0; i < nb; i++) {
+        fout << "! page number " << i << std::endl;
+        fout << std::setw(15) << std::left << dim << std::endl;
+        fout << std::setw(15) << std::left << i << std::endl;
+        for (int k = 0; k < 3; k++)
+            for (int pl = 0; pl < 3; pl++) {
+                w(r[k][(size_t)pl * nb + i]);
+                fout << std::endl;
+            }
+        fout << std::setw(15) << std::left << beamVec[i].bunchHarmNum << std::endl;
+        for (int pl = 0; pl < 3; pl++) {
+            w(r[3][(size_t)pl * nb + i]);
+            fout << std::endl;
+        }
+        fout << ns << std::endl;
+        for (int n = 0; n < ns; n++) {
+            fout << std::setw(15) << std::left << n * interval;
+            for (int pl = 0; pl < 3; pl++) w(a[0][at(n, pl, i)]);
+            for (int pl = 0; pl < 3; pl++) w(a[1][at(n, pl, i)]);
+            for (int pl = 0; pl < 3; pl++) w(a[2][at(n, pl, i)]);
+            for (int pl = 0; pl < 3; pl++) w(a[3][at(n, 0, i)]);
+            w(a[6][at(n, 0, i)]); w(a[6][at(n, 0, i)]); w(a[6][at(n, 2, i)]);
+            w(a[7][at(n, 0, i)]); w(a[7][at(n, 0, i)]); w(a[7][at(n, 2, i)]);
+            for (int pl = 0; pl < 3; pl++) w(samples_[(size_t)n * nb + i].mean[2 * pl]);
+            for (int pl = 0; pl < 3; pl++) w(a[4][at(n, pl, i)]);
+            for (int pl = 0; pl < 3; pl++) w(std::hypot(a[4][at(n, pl, i)], a[5][at(n, pl, i)]));
+            fout << std::endl;
+        }
+    }
+    return 0;
 }
 
-void MPBeam::StoreSample(const std::vector<cts_moments> &m, int turn, std::ofstream &tbt)
+// One page of TBTBunchAverData.sdds (MPBeam::MPBeamDataPrintPerTurn,
+// src/MPBeam.cpp:2319-2411): the moments of every bunch at the end of turn
+// `turn`, the ion charge its passage saw, and the cavity phasors.
+void MPBeam::PrintTBTBunchAverData(std::ofstream &fout, int turn, const cts_moments *m, const double *ionCharge,
+                                   const ReadInputSettings &in, bool header)
 {
-    samples_.insert(samples_.end(), m.begin(), m.end());
-    for (size_t b = 0; b < beamVec.size(); b++) {
-        const cts_moments &q = m[b];
+    const auto *RF = in.ringParRf;
+    if (header) {
+        const char *head[] = {
+            "SDDS1",
+            "&column name=Turns,                           type=long,  &end",
+            "&column name=HarmIndex,                       type=long,  &end",
+            "&column name=BunchIndex,                       type=long,  &end",
+            "&column name=TotIonCharge,       units=e,     type=float,  &end",
+            "&column name=AverX,              units=m,     type=float,  &end",
+            "&column name=AverY,              units=m,     type=float,  &end",
+            "&column name=AverZ,              units=m,     type=float,  &end",
+            "&column name=AverXP,             units=rad,   type=float,  &end",
+            "&column name=AverYP,             units=rad,   type=float,  &end",
+            "&column name=AverZP,             units=rad,   type=float,  &end",
+            "&column name=RmsEmitX,           units=m*rad, type=float,  &end",
+            "&column name=RmsEmitY,           units=m*rad, type=float,  &end",
+            "&column name=RmsEmitZ,           units=m*rad, type=float,  &end",
+            "&column name=RmsRX,              units=m*rad, type=float,  &end",
+            "&column name=RmsRY,              units=m*rad, type=float,  &end",
+            "&column name=RmsBunchLength,     units=m,     type=float,  &end",
+            "&column name=RmsBunchEnergySpread, units=rad, type=float,  &end"};
+        for (const char *l : head) fout << l << std::endl;
+        for (int j = 0; j < RF->resNum; j++) {
+            const std::string s = std::to_string(j);
+            fout << "&column name=cavAmp_" << s << ", units=V,   type=float,  &end" << std::endl;
+            fout << "&column name=cavPhase_" << s << ", units=rad,   type=float,  &end" << std::endl;
+            fout << "&column name=cavReal_" << s << ", units=V,   type=float,  &end" << std::endl;
+            fout << "&column name=genAmp_" << s << ", units=V,   type=float,  &end" << std::endl;
+            fout << "&column name=genPhase_" << s << ", units=rad,   type=float,  &end" << std::endl;
+            fout << "&column name=beamIndAmp_" << s << ", units=V,   type=float,  &end" << std::endl;
+            fout << "&column name=beamIndPhase_" << s << ", units=rad,   type=float,  &end" << std::endl;
+        }
+        fout << "&data mode=ascii, &end" << std::endl;
+    }
+    fout << "! page number " << turn + 1 << std::endl;
+    fout << beamVec.size() << std::endl;
+    const double rBeta = in.ringParBasic->rBeta, fRF = in.ringParBasic->f0 * in.ringParRf->ringHarm;
+    for (size_t i = 0; i < beamVec.size(); i++) {
         cts_derived d{};
-        cts_derive_moments(gpu, &q, 0, &d);
-        char line[640];
-        std::snprintf(line, sizeof line,
-                      "%d %zu %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", turn, b,
-                      q.mean[0], q.mean[1], q.mean[2], q.mean[3], q.mean[4], q.mean[5], d.sigma_z, d.sigma_dp, d.rms_x,
-                      d.rms_y, d.emit[0], d.emit[1], q.transmission);
-        tbt << line;
+        cts_derive_moments(gpu, &m[i], 0, &d);
+        auto w = [&](double v) { fout << std::setw(24) << std::left << std::setprecision(16) << v; };
+        fout << std::setw(24) << std::left << std::setprecision(16) << turn;
+        fout << std::setw(24) << std::left << std::setprecision(16) << beamVec[i].bunchHarmNum;
+        fout << std::setw(24) << std::left << std::setprecision(16) << i;
+        w(ionCharge ? ionCharge[i] : 0.0);
+        w(m[i].mean[0]); w(m[i].mean[2]); w(m[i].mean[4]);
+        w(m[i].mean[1]); w(m[i].mean[3]); w(m[i].mean[5]);
+        w(d.emit[0]); w(d.emit[1]); w(d.emit[2]);
+        w(d.rms_x); w(d.rms_y); w(d.sigma_z); w(d.sigma_dp);
+        // ideal cavities: the generator phasor, which is also the cavity
+        // voltage, at the bunch's mean z; no beam-induced part
+        for (int j = 0; j < RF->resNum; j++) {
+            const std::complex<double> req(cavity_[2 * j], cavity_[2 * j + 1]);
+            const double ph = -m[i].mean[4] / kClight / rBeta * 2. * kPi * (double)RF->resHarm[j] * fRF;
+            const std::complex<double> v = req * std::exp(std::complex<double>(0.0, ph));
+            w(std::abs(v)); w(std::arg(v)); w(v.real());
+            w(std::abs(v)); w(std::arg(v));
+            w(0.0); w(0.0);
+        }
+        fout << std::endl;
     }
 }
 
@@ -743,17 +893,40 @@ int MPBeam::Run(Train &, LatticeInterActionPoint &lat, ReadInputSettings &in)
     if (c.ions && check(cts_set_ions(gpu, &c.ion), "ions")) return 1;
     const int interval = std::max(R->bunchInfoPrintInterval, 1);
     samples_.clear();
-    std::ofstream tbt(R->TBTBunchAverData + ".dat");
-    tbt << "# turn bunch xAver pxAver yAver pyAver zAver pzAver rmsBunchLength rmsEnergySpread rmsRx rmsRy "
-           "emittanceX emittanceY transmission\n";
+    // output files start empty, as the reference removes *.sdds at set-up
+    // (RMOutPutFiles, src/Global.cpp:48-55)
+    std::ofstream tbt;
+    if (!R->TBTBunchAverData.empty()) tbt.open(R->TBTBunchAverData + ".sdds");
+    const int nsamp = (R->nTurns + interval - 1) / interval;
+    std::vector<double> ionq((size_t)nsamp * nb, 0.0);
+    std::vector<double> pas((size_t)nb * CTS_MAX_SPECIES * 6);
+    auto record_ions = [&](int sample) -> int {
+        if (!c.ions) return 0;
+        if (check(cts_get_ion_passages(gpu, pas.data()), "ion passages")) return 1;
+        for (int b = 0; b < nb; b++) {
+            double q = 0;
+            for (int p = 0; p < lat.gasSpec; p++) q += pas[((size_t)b * CTS_MAX_SPECIES + p) * 6 + 5];
+            ionq[(size_t)sample * nb + b] = q;
+        }
+        return 0;
+    };
     std::vector<cts_moments> m(nb);
     if (nsec == 1) {
+        // the turn loop in stretches that end on the print turns; HIP-graph
+        // replays inside a stretch
         if (check(cts_set_history(gpu, R->nTurns), "history")) return 1;
-        for (int n = 0; n < R->nTurns; n++)
-            if (check(cts_turn(gpu, n), "turn")) return 1;
-        for (int n = 0; n < R->nTurns; n += interval) {
-            if (check(cts_get_history(gpu, n, 1, m.data()), "history")) return 1;
-            StoreSample(m, n, tbt);
+        for (int s = 0; s < nsamp; s++) {
+            const int t0 = s == 0 ? 0 : (s - 1) * interval + 1, t1 = s * interval;
+            if (check(cts_run_turns(gpu, t0, t1 - t0 + 1, 1), "turns")) return 1;
+            if (record_ions(s)) return 1;
+        }
+        if (R->nTurns > (nsamp - 1) * interval + 1) {
+            const int t0 = (nsamp - 1) * interval + 1;
+            if (check(cts_run_turns(gpu, t0, R->nTurns - t0, 1), "turns")) return 1;
+        }
+        for (int s = 0; s < nsamp; s++) {
+            if (check(cts_get_history(gpu, s * interval, 1, m.data()), "history")) return 1;
+            samples_.insert(samples_.end(), m.begin(), m.end());
         }
     } else {
         for (int n = 0; n < R->nTurns; n++) {
@@ -766,23 +939,22 @@ int MPBeam::Run(Train &, LatticeInterActionPoint &lat, ReadInputSettings &in)
             if (MarkParticleLostInBunch()) return 1;
             if (n % interval == 0) {
                 if (check(cts_rms(gpu, m.data()), "rms")) return 1;
-                StoreSample(m, n, tbt);
+                samples_.insert(samples_.end(), m.begin(), m.end());
             }
         }
     }
     if (check(cts_sync(gpu), "sync")) return 1;
+    if (tbt.is_open())
+        for (int s = 0; s < nsamp; s++)
+            PrintTBTBunchAverData(tbt, s * interval, samples_.data() + (size_t)s * nb, ionq.data() + (size_t)s * nb,
+                                  in, s == 0);
     if (MPBeamRMSCal(0)) return 1;
-    if (!R->runCBMGR.empty() && R->growthRateFittingEnd > R->growthRateFittingStart &&
-        (int)(samples_.size() / nb) >= R->growthRateFittingEnd / interval) {
+    // GetCBMGR fits and writes its file at the turn with turn + interval ==
+    // nTurns (src/MPBeam.cpp:1651), i.e. when nTurns is a multiple of the
+    // print interval
+    if (!R->runCBMGR.empty() && R->nTurns % interval == 0 && nsamp >= 2) {
         std::vector<double> rates;
         if (GetCBMGR(in, rates)) return 1;
-        std::ofstream gr(R->runCBMGR + ".dat");
-        gr << "# mode growth_rate_x growth_rate_y growth_rate_z [1/s]\n";
-        char line[128];
-        for (int k = 0; k < nb; k++) {
-            std::snprintf(line, sizeof line, "%d %.17g %.17g %.17g\n", k, rates[k], rates[nb + k], rates[2 * nb + k]);
-            gr << line;
-        }
     }
     return 0;
 }

@@ -187,15 +187,19 @@ public:
     int MPBeamRMSCal(int k);
     int SSBeamIonEffectOneInteractionPoint(int turn);
     int DownloadParticles();
-    // growth rates of the coupled-bunch modes at the end of the run
-    // (GetCBMGR's ideal modes, src/MPBeam.cpp:1496-1891); rates[3][nb]
+    // growth rates of the coupled-bunch modes from the run's samples
+    // (MPBeam::GetCBMGR, src/MPBeam.cpp:1496-1891): ideal modes, IQ,
+    // Hilbert modes and per-bunch rates, written to runCBMGR.sdds;
+    // rates[3][nb] gets the ideal-mode rates
     int GetCBMGR(ReadInputSettings &inputParameter, std::vector<double> &rates);
+    void PrintTBTBunchAverData(std::ofstream &fout, int turn, const cts_moments *m, const double *ionCharge,
+                               const ReadInputSettings &inputParameter, bool header);
 
 private:
     int check(int rc, const char *what);
-    void StoreSample(const std::vector<cts_moments> &m, int turn, std::ofstream &tbt);
     std::vector<cts_moments> samples_;  // [sample][bunch], turns 0, interval, ...
     double alphaX0_ = 0, betaX0_ = 1, alphaY0_ = 0, betaY0_ = 1;
+    std::vector<double> cavity_;  // resCavVolReq of each cavity (re, im)
 };
 
 // Initial distribution of one bunch (MPBunch::DistriGenerator,

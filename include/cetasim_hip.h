@@ -267,6 +267,30 @@ int cts_debug_qe_normals(cts_handle *h, uint64_t seed, int32_t turn, uint64_t ba
 int cts_cbm_growth_from_moments(const cts_moments *m, int32_t nsamples, int32_t nb, const int32_t *harm_num,
                                 int32_t harmonics, const double *tunes, const double *twiss, int32_t interval,
                                 int32_t fit_start, int32_t fit_end, double t0, double *rates, double *amps);
+/* The whole of GetCBMGR's analysis (src/MPBeam.cpp:1496-1891) of the same
+ * samples; every array may be NULL.  Per sample n, plane p (x, y, z) and
+ * index k ([nsamples][3][nb]):
+ *   ideal_amp/arg  |mode k| of the phasors above and atan2(Re, Im) (the
+ *                  reference's argument order);
+ *   iq_amp/arg     |<centroid_i e^{-2 pi i f_k t_i}>_i| and its arg, f_k =
+ *                  (k + nu) f0, t_i = h_i T0 / h (IQ decomposition);
+ *   ana_re/im      analytic signal of bunch k's centroid history (one-sided
+ *                  spectrum over all samples, GetHilbertAnalytical,
+ *                  src/MPBeam.cpp:1936-2011);
+ *   hil_amp/arg    modes of the phasors built from (Re, -Im) of it.
+ * Rates ([3][nb], 1/s) are the slopes of log amplitude over the fit window
+ * as above: ideal modes, IQ components, Hilbert modes and, per bunch,
+ * |analytic signal| (the reference's BGR). */
+typedef struct {
+    double *ideal_amp, *ideal_arg;
+    double *iq_amp, *iq_arg;
+    double *ana_re, *ana_im;
+    double *hil_amp, *hil_arg;
+    double *rate_ideal, *rate_iq, *rate_hil, *rate_bunch;
+} cts_cbm_result;
+int cts_cbm_analysis(const cts_moments *m, int32_t nsamples, int32_t nb, const int32_t *harm_num, int32_t harmonics,
+                     const double *tunes, const double *twiss, int32_t interval, int32_t fit_start, int32_t fit_end,
+                     double t0, const cts_cbm_result *out);
 /* gsl_fit_wlinear's line fit y = c0 + c1 x (the fit above). */
 int cts_fit_wlinear(const double *x, const double *w, const double *y, int32_t n, double *c0, double *c1);
 /* The same from the handle's moment history (cts_set_history): samples at

@@ -254,6 +254,14 @@ void orc_ion_rms(orc_ion_cloud *c, int p);
 
 /* ------------------------------------------------------------ growth rate
  * gsl_fit_wlinear (FittingGSL::FitALinear, src/FittingGSL.cpp:231-243). */
+/* GetCBMGR's analysis (ideal modes, IQ, analytic signals, Hilbert modes,
+ * rates) of centroids mean[ns][nb][6] with direct DFTs; layout as
+ * cts_cbm_analysis.  Returns -1 when the fit window lies outside. */
+int orc_cbm_analysis(const double *mean, int ns, int nb, const int *harm, int harmonics, const double *tunes,
+                     const double *twiss, int interval, int fit_start, int fit_end, double t0,
+                     double *ideal_amp, double *ideal_arg, double *iq_amp, double *iq_arg, double *ana_re,
+                     double *ana_im, double *hil_amp, double *hil_arg, double *rate_ideal, double *rate_iq,
+                     double *rate_hil, double *rate_bunch);
 void orc_fit_wlinear(const double *x, const double *w, const double *y, int n,
                      double *c0, double *c1);
 

@@ -102,6 +102,8 @@ def lib():
         L.orc_ion_passage.argtypes = [ctypes.POINTER(OrcIonCloud), ctypes.POINTER(OrcBunch),
                                       ctypes.POINTER(OrcBunchView), D, D, U64, U32, U32, PD]
         L.orc_fit_wlinear.argtypes = [PD, PD, PD, ctypes.c_int, PD, PD]
+        L.orc_cbm_analysis.argtypes = ([PD, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, PD,
+                                        PD, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double] + [PD] * 12)
         _lib = L
     return _lib
 

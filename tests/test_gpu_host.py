@@ -37,39 +37,65 @@ def program_run(tmp_path_factory):
     return str(d), H.load_dump(str(d / "dump"))
 
 
+def read_sdds_pages(path):
+    """Data pages of an ASCII SDDS file as lists of token rows (header up to
+    the &data line skipped; '! page number' lines start a page)."""
+    pages, cur, data = [], None, False
+    for line in open(path):
+        if not data:
+            data = line.startswith("&data")
+            continue
+        if line.startswith("! page number"):
+            cur = []
+            pages.append(cur)
+            continue
+        if line.strip():
+            cur.append(line.split())
+    return pages
+
+
 def test_program_matches_capi_run(program_run):
     d, D = program_run
     s = D["settings"]
+    nb, interval, nturns = s["nb"], s["interval"], s["nTurns"]
     tr = Tracker(D["ring"], D["sections"])
     tr.set_bunches(D["counts"], D["q"], D["ne"], D["gap"])
     tr.upload(D["parts"])
     tr.set_wake(s["nbins"], 0, 0.0, 0.0, s["rs"], s["q"], s["fr"])
     tr.set_synrad(True, 2, s["seed"])
     tr.set_ions(H.ion_config(s))
-    tr.set_history(s["nTurns"])
-    for n in range(s["nTurns"]):
-        tr.turn(n)
-    nb, interval = s["nb"], s["interval"]
-    turns = list(range(0, s["nTurns"], interval))
-    tbt = H.read_table(os.path.join(d, s["tbt"] + ".dat"))
-    assert tbt.shape == (len(turns) * nb, 15)
+    tr.set_history(nturns)
+    turns = list(range(0, nturns, interval))
+    ionq = []
+    for t in range(nturns):
+        tr.turn(t)
+        if t % interval == 0:
+            ionq.append(tr.ion_passages()[:, :, 5].sum(axis=1))
     lib = capi.load()
     samples = []
-    for i, t in enumerate(turns):
+    for t in turns:
         m = (capi.Moments * nb)()
         capi.check(lib.cts_get_history(tr.h, t, 1, m))
         samples.append(m)
+    # TBTBunchAverData.sdds: one page per print turn, one row per bunch
+    pages = read_sdds_pages(os.path.join(d, s["tbt"] + ".sdds"))
+    assert len(pages) == len(turns)
+    for i, t in enumerate(turns):
+        assert int(pages[i][0][0]) == nb
+        rows = pages[i][1:]
+        assert len(rows) == nb
         for b in range(nb):
-            row = tbt[i * nb + b]
-            assert row[0] == t and row[1] == b
-            mm = m[b]
-            np.testing.assert_array_equal(row[2:8], np.array(mm.mean[:]))
+            row = rows[b]
+            mm = samples[i][b]
             der = capi.Derived()
             capi.check(lib.cts_derive_moments(tr.h, ctypes.byref(mm), 0, ctypes.byref(der)))
-            np.testing.assert_array_equal(row[8:14], [der.sigma_z, der.sigma_dp, der.rms_x, der.rms_y,
-                                                      der.emit[0], der.emit[1]])
-            assert row[14] == mm.transmission
-    # growth rates of the program against the same fit of the same samples
+            assert (int(row[0]), int(row[1]), int(row[2])) == (t, int(D["harm"][b]), b)
+            exp = [ionq[i][b], mm.mean[0], mm.mean[2], mm.mean[4], mm.mean[1], mm.mean[3], mm.mean[5],
+                   der.emit[0], der.emit[1], der.emit[2], der.rms_x, der.rms_y, der.sigma_z, der.sigma_dp]
+            np.testing.assert_allclose([float(x) for x in row[3:17]], exp, rtol=2e-15, atol=0)
+            assert len(row) == 17 + 7  # one ideal cavity: 7 phasor columns
+    # runCBMGR.sdds: parameters and per-sample columns of every mode, printed
+    # with six significant digits
     allm = (capi.Moments * (len(turns) * nb))()
     for i, m in enumerate(samples):
         for b in range(nb):
@@ -78,14 +104,36 @@ def test_program_matches_capi_run(program_run):
     tunes = np.array([D["ring"].nux, D["ring"].nuy, s["workQz"]])
     sec = D["sections"][0]
     twiss = np.array([sec.alpha_x, sec.beta_x, sec.alpha_y, sec.beta_y, s["betaZ"]])
-    rates = np.empty(3 * nb)
-    capi.check(lib.cts_cbm_growth_from_moments(allm, len(turns), nb, capi.i32ptr(harm), D["ring"].harmonic,
-                                               capi.dptr(tunes), capi.dptr(twiss), interval, s["fit_start"],
-                                               s["fit_end"], D["ring"].t0, capi.dptr(rates), None))
-    gr = H.read_table(os.path.join(d, s["cbmgr"] + ".dat"))
-    assert gr.shape == (nb, 4)
-    np.testing.assert_array_equal(gr[:, 0], np.arange(nb))
-    np.testing.assert_array_equal(gr[:, 1:].T.ravel(), rates)
+    ns = len(turns)
+    keys = ("ideal_amp", "ideal_arg", "iq_amp", "iq_arg", "ana_re", "ana_im", "hil_amp", "hil_arg")
+    rk = ("rate_ideal", "rate_iq", "rate_hil", "rate_bunch")
+    out = {k: np.zeros(ns * 3 * nb) for k in keys}
+    out.update({k: np.zeros(3 * nb) for k in rk})
+    res = capi.CbmResult(*[capi.dptr(out[k]) for k in keys + rk])
+    capi.check(lib.cts_cbm_analysis(allm, ns, nb, capi.i32ptr(harm), D["ring"].harmonic, capi.dptr(tunes),
+                                    capi.dptr(twiss), interval, s["fit_start"], s["fit_end"], D["ring"].t0,
+                                    ctypes.byref(res)))
+    A = {k: out[k].reshape(ns, 3, nb) for k in keys}
+    R = {k: out[k].reshape(3, nb) for k in rk}
+    g = lambda v: float("%g" % v)  # noqa: E731  (iostream default: 6 significant digits)
+    cb = read_sdds_pages(os.path.join(d, s["cbmgr"] + ".sdds"))
+    assert len(cb) == nb
+    for i in range(nb):
+        page = cb[i]
+        dim = s["fit_end"] // interval - s["fit_start"] // interval
+        assert int(page[0][0]) == dim and int(page[1][0]) == i
+        par = [float(r[0]) for r in page[2:11]]
+        exp = [g(R[k][pl, i]) for k in ("rate_ideal", "rate_iq", "rate_hil") for pl in range(3)]
+        assert par == exp
+        assert int(page[11][0]) == int(D["harm"][i])
+        assert [float(r[0]) for r in page[12:15]] == [g(R["rate_bunch"][pl, i]) for pl in range(3)]
+        assert int(page[15][0]) == ns
+        for n in range(ns):
+            row = [float(x) for x in page[16 + n]]
+            assert row[0] == n * interval
+            assert row[1:4] == [g(A["ideal_amp"][n, pl, i]) for pl in range(3)]
+            assert row[7:10] == [g(A["iq_amp"][n, pl, i]) for pl in range(3)]
+            assert row[19:22] == [g(samples[n][i].mean[2 * pl]) for pl in range(3)]
     # the ions were on: the cloud is not empty after the run
     assert tr.ion_passages()[:, 0, 0].sum() > 0
     tr.close()

@@ -101,3 +101,117 @@ def test_matches_numpy_dft():
         for k in range(nb):
             slope = np.polyfit(t, np.log(A[i0:i1, pl, k]), 1)[0]
             assert abs(rates[pl, k] - slope / 2e-6) <= 1e-9 * abs(slope / 2e-6) + 1e-6
+
+
+# ---------------------------------------------------------------- full analysis
+KEYS = ("ideal_amp", "ideal_arg", "iq_amp", "iq_arg", "ana_re", "ana_im", "hil_amp", "hil_arg")
+RATES = ("rate_ideal", "rate_iq", "rate_hil", "rate_bunch")
+
+
+def analysis(mean, harm, h, tunes, twiss, interval, fs, fe, t0):
+    ns, nb, _ = mean.shape
+    m = moments_array(mean)
+    out = {k: np.zeros(ns * 3 * nb) for k in KEYS}
+    out.update({k: np.zeros(3 * nb) for k in RATES})
+    r = capi.CbmResult(*[capi.dptr(out[k]) for k in KEYS + RATES])
+    hh = np.ascontiguousarray(harm, dtype=np.int32)
+    rc = lib().cts_cbm_analysis(m, ns, nb, capi.i32ptr(hh), h, capi.dptr(np.array(tunes, float)),
+                                capi.dptr(np.array(twiss, float)), interval, fs, fe, t0, ctypes.byref(r))
+    assert rc == 0
+    return {k: v.reshape(ns, 3, nb) for k, v in out.items() if k in KEYS} | \
+        {k: v.reshape(3, nb) for k, v in out.items() if k in RATES}
+
+
+def oracle_analysis(mean, harm, h, tunes, twiss, interval, fs, fe, t0):
+    ns, nb, _ = mean.shape
+    out = {k: np.zeros(ns * 3 * nb) for k in KEYS}
+    out.update({k: np.zeros(3 * nb) for k in RATES})
+    hh = (ctypes.c_int * nb)(*[int(x) for x in harm])
+    mm = np.ascontiguousarray(mean, dtype=float)
+    rc = O.lib().orc_cbm_analysis(O.p(mm), ns, nb, hh, h, O.p(np.array(tunes, float)), O.p(np.array(twiss, float)),
+                                  interval, fs, fe, t0, *[O.p(out[k]) for k in KEYS + RATES])
+    assert rc == 0
+    return {k: v.reshape(ns, 3, nb) for k, v in out.items() if k in KEYS} | \
+        {k: v.reshape(3, nb) for k, v in out.items() if k in RATES}
+
+
+import pytest  # noqa: E402
+
+
+@pytest.mark.parametrize("nb,ns", [(12, 40), (7, 41), (30, 64)])
+def test_cbm_analysis_matches_oracle(nb, ns):
+    """Mixed-radix FFTs (prime, composite, power of two) against the
+    oracle's direct DFTs, every output of the analysis."""
+    rng = np.random.default_rng(nb * 100 + ns)
+    h = 600
+    harm = np.sort(rng.choice(h, nb, replace=False))
+    tunes, twiss, interval, t0 = (0.18, 0.27, 0.005), (0.3, 8.0, -0.2, 12.0, 2.5), 5, 2e-6
+    turns = np.arange(ns)[:, None] * interval
+    mean = rng.normal(0, 1e-5, (ns, nb, 6)) * np.exp(2e-3 * turns)[..., None]
+    got = analysis(mean, harm, h, tunes, twiss, interval, 20, 20 + 25 * interval, t0)
+    exp = oracle_analysis(mean, harm, h, tunes, twiss, interval, 20, 20 + 25 * interval, t0)
+    for k in ("ideal_amp", "iq_amp", "hil_amp"):
+        np.testing.assert_allclose(got[k], exp[k], rtol=1e-11, atol=1e-11 * np.max(np.abs(exp[k])))
+    for k in ("ana_re", "ana_im"):
+        np.testing.assert_allclose(got[k], exp[k], rtol=0, atol=1e-11 * np.max(np.abs(mean)))
+    for k in ("ideal_arg", "iq_arg", "hil_arg"):
+        d = np.angle(np.exp(1j * (got[k] - exp[k])))
+        assert np.max(np.abs(d)) < 1e-9, k
+    for k in RATES:
+        np.testing.assert_allclose(got[k], exp[k], rtol=1e-8, atol=1e-6)
+
+
+def test_analytic_signal_closed_form():
+    """A centroid history c + A cos(2 pi k n / N + phi) with k whole cycles
+    has the analytic signal c + A e^{i(2 pi k n / N + phi)}, for N even and
+    odd."""
+    for ns in (64, 45):
+        nb = 3
+        n = np.arange(ns)
+        mean = np.zeros((ns, nb, 6))
+        cases = [(1e-4, 2e-5, 5, 0.3), (-3e-5, 1e-5, 1, -1.1), (0.0, 4e-6, ns // 2 - 1, 2.0)]
+        for b, (c, A, k, phi) in enumerate(cases):
+            mean[:, b, 0] = c + A * np.cos(2 * np.pi * k * n / ns + phi)
+        got = analysis(mean, [0, 10, 20], 100, (0.18, 0.27, 0.005), (0, 1, 0, 1, 1), 1, 0, ns, 1e-6)
+        for b, (c, A, k, phi) in enumerate(cases):
+            exp = c + A * np.exp(1j * (2 * np.pi * k * n / ns + phi))
+            np.testing.assert_allclose(got["ana_re"][:, 0, b], exp.real, atol=1e-18)
+            np.testing.assert_allclose(got["ana_im"][:, 0, b], exp.imag, atol=1e-18)
+
+
+def test_iq_closed_form():
+    """Centroids A cos(2 pi f_j t_i) of bunches equally spaced around the
+    ring: the IQ component at f_j is A/2 (1 + G), G the geometric sum
+    (1 - r^nb) / (nb (1 - r)), r = e^{-4 pi i (j + nu) / nb}."""
+    nb, h, nu, t0 = 16, 320, 0.27, 2e-6
+    harm = np.arange(nb) * (h // nb)
+    f0, trf = 1 / t0, t0 / h
+    A = 3e-5
+    for j in (0, 3, 9):
+        fj = (j + nu) * f0
+        mean = np.zeros((2, nb, 6))
+        mean[:, :, 2] = A * np.cos(2 * np.pi * fj * harm * trf)
+        got = analysis(mean, harm, h, (0.18, nu, 0.005), (0, 1, 0, 1, 1), 1, 0, 2, t0)
+        r = np.exp(-4j * np.pi * (j + nu) / nb)
+        G = (1 - r ** nb) / (nb * (1 - r))
+        assert got["iq_amp"][0, 1, j] == pytest.approx(abs(A / 2 * (1 + G)), rel=1e-12)
+        assert got["iq_arg"][0, 1, j] == pytest.approx(np.angle(A / 2 * (1 + G)), abs=1e-12)
+
+
+def test_hilbert_modes_follow_ideal_modes():
+    """With unit beta and no alpha, centroids rotating in phase space
+    (x = cos theta, px = -sin theta) over whole cycles give Hilbert modes
+    equal to the ideal modes, and growth rates of an exponentially growing
+    rotation agree."""
+    nb, ns, h, interval = 8, 64, 80, 1
+    harm = np.arange(nb) * (h // nb)
+    nu = 5 / ns  # whole cycles over the samples
+    n = np.arange(ns)[:, None]
+    phi = 2 * np.pi * 0.37 * np.arange(nb)[None, :]
+    theta = 2 * np.pi * nu * n + phi
+    mean = np.zeros((ns, nb, 6))
+    mean[:, :, 0] = 1e-5 * np.cos(theta)
+    mean[:, :, 1] = -1e-5 * np.sin(theta)
+    got = analysis(mean, harm, h, (nu, 0.27, 0.005), (0, 1, 0, 1, 1), interval, 0, ns, 1e-6)
+    np.testing.assert_allclose(got["hil_amp"][:, 0], got["ideal_amp"][:, 0], rtol=1e-9,
+                               atol=1e-9 * got["ideal_amp"][:, 0].max())
