@@ -821,15 +821,11 @@ int cts_create(const cts_ring *ring, const cts_section *secs, int32_t nsec, cts_
         delete h;
         return fail(CTS_ERR_HIP, std::string("stream: ") + hipGetErrorString(e));
     }
-    // Algorithm 916 constants and exp(-a^2 n^2) table
-    double fad[3], tab[64];
-    fad[0] = kPi / std::sqrt(-std::log(DBL_EPSILON * 0.5));
-    fad[1] = (2 / kPi) * fad[0];
-    fad[2] = fad[0] * fad[0];
-    for (int n = 1; n <= 63; n++) tab[n - 1] = std::exp(-fad[2] * (double)(n * n));
-    tab[63] = 0.0;
-    e = upload_faddeeva_tables(tab, fad);
-    if (e == hipSuccess) e = upload_faddeeva_tables_ions(tab, fad);
+    // w(z) tables: Algorithm 916 constants, exp(-a^2 n^2), Weideman's coefficients
+    FadTables ft;
+    build_fad_tables(ft);
+    e = upload_faddeeva_tables(ft);
+    if (e == hipSuccess) e = upload_faddeeva_tables_ions(ft);
     if (e == hipSuccess) e = hipMalloc((void **)&h->d_status, sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(h->d_status, 0, sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc((void **)&h->d_turn, sizeof(uint32_t));
@@ -1696,6 +1692,23 @@ int cts_kernel_time(cts_handle *h, const char *name, double *avg_ms, int64_t *la
     }
     if (avg_ms) *avg_ms = it->second.total_ms / (double)it->second.n;
     if (launches) *launches = it->second.n;
+    return CTS_OK;
+}
+
+int cts_debug_faddeeva(cts_handle *h, int64_t n, const double *re, const double *im, double *wr, double *wi)
+{
+    if (!h || n <= 0 || !re || !im || !wr || !wi) return fail(CTS_ERR_ARG, "cts_debug_faddeeva");
+    double *d = nullptr;
+    int rc = dalloc(&d, (size_t)n * 4);
+    if (rc) return rc;
+    hipError_t e = hipMemcpy(d, re, (size_t)n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + n, im, (size_t)n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_w_debug(d, d + n, n, d + 2 * n, d + 3 * n, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(wr, d + 2 * n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(wi, d + 3 * n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(CTS_ERR_HIP, std::string("faddeeva debug: ") + hipGetErrorString(e));
     return CTS_OK;
 }
 

@@ -3,15 +3,23 @@
 //
 // Region structure of the reference's vendored Faddeeva package (S. G.
 // Johnson, MIT; src/Faddeeva.cpp:682-974): axis special cases, continued
-// fraction (Gautschi 1970, Poppe & Wijers 1990) for large |z|, Zaghloul &
-// Ali (2011, ACM TOMS Algorithm 916) sums for |x| < 10, large-|x| sums
-// otherwise.  erfcx and Im w(x) on the axes are this project's own
+// fraction (Gautschi 1970, Poppe & Wijers 1990) for large |z|, large-|x|
+// sums near the real axis.  Where the package sums Zaghloul & Ali's
+// series (ACM TOMS Algorithm 916, |x| < 10), the upper half-plane uses
+// Weideman's rational approximation instead (J. A. C. Weideman, SIAM J.
+// Numer. Anal. 31 (1994) 1497-1518): w(z) = 2 p(Z)/(L - iz)^2 +
+// pi^{-1/2}/(L - iz), Z = (L + iz)/(L - iz), p of degree N - 1 = 39,
+// L = (N/sqrt 2)^{1/2}; |w| relative error below 4e-14 there (1e-15
+// typical), one division and 40 complex multiply-adds instead of a
+// division-heavy series; the lower half-plane keeps the series.  erfcx and Im w(x) on the axes are this project's own
 // evaluations (exp(x^2)erfc(x) with an exact x^2 split, Laplace continued
 // fraction for x >= 25, Dawson's Taylor series for |x| < 0.5).  FP64
 // throughout: the asymptotic and lower-half-plane branches cancel.
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <cmath>
 
 namespace cts {
 
@@ -21,6 +29,44 @@ namespace cts {
 // exports an upload function that fills it; cts_create calls each of them.
 static __constant__ double c_ea2n2[64];
 static __constant__ double c_fad[3];
+constexpr int kWeidN = 40;
+static __constant__ double c_weid[kWeidN + 1];  // p's coefficients, highest degree first, then L
+
+// Host: the tables above.
+struct FadTables {
+    double ea2n2[64];
+    double fad[3];
+    double weid[kWeidN + 1];
+};
+inline void build_fad_tables(FadTables &t)
+{
+    const double pi = 3.141592653589793;
+    // Algorithm 916 constants and exp(-a^2 n^2)
+    t.fad[0] = pi / std::sqrt(-std::log(2.220446049250313e-16 * 0.5));
+    t.fad[1] = (2 / pi) * t.fad[0];
+    t.fad[2] = t.fad[0] * t.fad[0];
+    for (int n = 1; n <= 63; n++) t.ea2n2[n - 1] = std::exp(-t.fad[2] * (double)(n * n));
+    t.ea2n2[63] = 0.0;
+    // Weideman: f(t) = e^{-t^2}(L^2 + t^2) at t = L tan(theta/2), theta = k pi/M,
+    // k = -M+1..M-1 (and 0 at k = -M), M = 2N; the coefficients are the real
+    // parts of the 2M-point DFT of f in FFT order, terms 1..N
+    const int N = kWeidN, M = 2 * N, M2 = 2 * M;
+    const double L = std::sqrt(N / std::sqrt(2.0));
+    double f[2 * 2 * kWeidN];
+    f[0] = 0.0;
+    for (int k = -M + 1; k <= M - 1; k++) {
+        const double tt = L * std::tan(0.5 * k * pi / M);
+        f[k + M] = std::exp(-tt * tt) * (L * L + tt * tt);
+    }
+    double g[2 * 2 * kWeidN];
+    for (int j = 0; j < M2; j++) g[j] = f[(j + M) % M2];  // fftshift
+    for (int m = 1; m <= N; m++) {
+        double acc = 0.0;
+        for (int j = 0; j < M2; j++) acc += g[j] * std::cos(2 * pi * (double)((j * m) % M2) / M2);
+        t.weid[N - m] = acc / M2;
+    }
+    t.weid[N] = L;
+}
 
 constexpr double kIsPi = 0.56418958354775628694807945156;  // 1/sqrt(pi)
 
@@ -54,6 +100,29 @@ __device__ inline double cts_erfcx(double x)
         return 2 * exp(x * x) - erfcx_pos(-x);
     }
     return erfcx_pos(x);
+}
+
+// Weideman's approximation for Im z >= 0 (see the header comment).
+__device__ __forceinline__ void w_weideman(double zr, double zi, double &wr, double &wi)
+{
+#pragma clang fp contract(fast)
+    const double L = c_weid[kWeidN];
+    // 1/(L - iz) = u, Z = (L + iz) u
+    const double dr = L + zi, di = -zr;
+    const double inv = 1.0 / (dr * dr + di * di);
+    const double ur = dr * inv, ui = -di * inv;
+    const double nr = L - zi, ni = zr;
+    const double Zr = nr * ur - ni * ui, Zi = nr * ui + ni * ur;
+    double pr = c_weid[0], pi = 0.0;
+#pragma unroll 8
+    for (int k = 1; k < kWeidN; k++) {
+        const double t = pr * Zr - pi * Zi + c_weid[k];
+        pi = pr * Zi + pi * Zr;
+        pr = t;
+    }
+    const double u2r = ur * ur - ui * ui, u2i = 2.0 * ur * ui;
+    wr = 2.0 * (pr * u2r - pi * u2i) + kIsPi * ur;
+    wi = 2.0 * (pr * u2i + pi * u2r) + kIsPi * ui;
 }
 
 __device__ inline void w_general(double zr, double zi, double &wr_out, double &wi_out)
@@ -121,6 +190,10 @@ __device__ inline void w_general(double zr, double zi, double &wr_out, double &w
     } else if (x < 10) {
         double prod2ax = 1, prodm2ax = 1, expx2;
         if (isnan(y)) { wr_out = y; wi_out = y; return; }
+        if (y >= 0) {
+            w_weideman(zr, y, wr_out, wi_out);
+            return;
+        }
         if (x < 5e-4) {
             const double x2 = x * x;
             expx2 = 1 - x2 * (1 - 0.5 * x2);

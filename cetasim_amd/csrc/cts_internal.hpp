@@ -227,8 +227,9 @@ constexpr int kIonBlock = 256;
 constexpr int kIonChunk = 4096;  // compaction chunk (16 ions per thread)
 hipError_t launch_qe_debug(const float2 *tab, uint64_t seed, uint32_t turn, uint64_t base, int64_t n, double *out,
                            hipStream_t st);
-hipError_t upload_faddeeva_tables(const double ea2n2[64], const double fad[3]);
-hipError_t upload_faddeeva_tables_ions(const double ea2n2[64], const double fad[3]);
+hipError_t upload_faddeeva_tables(const FadTables &t);
+hipError_t launch_w_debug(const double *re, const double *im, int64_t n, double *wr, double *wi, hipStream_t st);
+hipError_t upload_faddeeva_tables_ions(const FadTables &t);
 void sur_export(const cts_surrogate *s, const float **params, SurNorm *norm);
 
 }  // namespace cts

@@ -15,11 +15,12 @@
 
 namespace cts {
 
-hipError_t upload_faddeeva_tables(const double ea2n2[64], const double fad[3])
+hipError_t upload_faddeeva_tables(const FadTables &t)
 {
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_ea2n2), ea2n2, 64 * sizeof(double));
-    if (e != hipSuccess) return e;
-    return hipMemcpyToSymbol(HIP_SYMBOL(c_fad), fad, 3 * sizeof(double));
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_ea2n2), t.ea2n2, 64 * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_fad), t.fad, 3 * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_weid), t.weid, (kWeidN + 1) * sizeof(double));
+    return e;
 }
 
 // Streaming (nontemporal) global accesses.
@@ -807,6 +808,24 @@ k_qe_debug(const float2 *tab, uint64_t seed, uint32_t turn, uint64_t base, int64
     qe_normals(T, seed, turn, base + (uint64_t)i, r);
 #pragma unroll
     for (int j = 0; j < 5; j++) out[j * n + i] = r[j];
+}
+
+// Test hook: w(z) at given points.
+__global__ void __launch_bounds__(kBlock)
+k_w_debug(const double *re, const double *im, int64_t n, double *wr, double *wi)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a, b;
+    faddeeva_w(re[i], im[i], a, b);
+    wr[i] = a;
+    wi[i] = b;
+}
+hipError_t launch_w_debug(const double *re, const double *im, int64_t n, double *wr, double *wi, hipStream_t st)
+{
+    const int blocks = (int)((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_w_debug, dim3(blocks), dim3(kBlock), 0, st, re, im, n, wr, wi);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------ launches

@@ -371,11 +371,12 @@ hipError_t launch_mlp_field_error(const float *W, const double *exact, int64_t n
     return hipGetLastError();
 }
 
-hipError_t upload_faddeeva_tables_mlp(const double ea2n2[64], const double fad[3])
+hipError_t upload_faddeeva_tables_mlp(const FadTables &t)
 {
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_ea2n2), ea2n2, 64 * sizeof(double));
-    if (e != hipSuccess) return e;
-    return hipMemcpyToSymbol(HIP_SYMBOL(c_fad), fad, 3 * sizeof(double));
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_ea2n2), t.ea2n2, 64 * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_fad), t.fad, 3 * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_weid), t.weid, (kWeidN + 1) * sizeof(double));
+    return e;
 }
 
 }  // namespace cts

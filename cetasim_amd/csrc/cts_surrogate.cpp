@@ -26,7 +26,7 @@ hipError_t launch_mlp_adam(float *params, float *m, float *v, const float *part,
 hipError_t launch_mlp_eval(const float *W, const float *feat, int64_t n, int stride, float *out, hipStream_t st);
 hipError_t launch_mlp_field_error(const float *W, const double *exact, int64_t n, const float norm[4], double *err,
                                   hipStream_t st);
-hipError_t upload_faddeeva_tables_mlp(const double ea2n2[64], const double fad[3]);
+hipError_t upload_faddeeva_tables_mlp(const FadTables &t);
 }  // namespace cts
 
 struct cts_surrogate {
@@ -115,15 +115,11 @@ int cts_sur_create(cts_surrogate **out)
         delete s;
         return sfail(CTS_ERR_HIP, std::string("stream: ") + hipGetErrorString(e));
     }
-    // Algorithm 916 constants for the sample generator's FP64 field
-    double fad[3], tab[64];
-    fad[0] = kPi / std::sqrt(-std::log(DBL_EPSILON * 0.5));
-    fad[1] = (2 / kPi) * fad[0];
-    fad[2] = fad[0] * fad[0];
-    for (int n = 1; n <= 63; n++) tab[n - 1] = std::exp(-fad[2] * (double)(n * n));
-    tab[63] = 0.0;
+    // w(z) tables of the sample generator's FP64 field
+    FadTables ft;
+    build_fad_tables(ft);
     int rc;
-    if ((e = upload_faddeeva_tables_mlp(tab, fad)) != hipSuccess ||
+    if ((e = upload_faddeeva_tables_mlp(ft)) != hipSuccess ||
         (rc = salloc(&s->d_params, kMlpParams)) || (rc = salloc(&s->d_m, kMlpParams)) ||
         (rc = salloc(&s->d_v, kMlpParams)) || (rc = salloc(&s->d_grad, kMlpParams)) || (rc = salloc(&s->d_loss, 1))) {
         if (e != hipSuccess) sfail(CTS_ERR_HIP, std::string("tables: ") + hipGetErrorString(e));

@@ -247,6 +247,9 @@ int cts_sync(cts_handle *h);
 int cts_set_timing(cts_handle *h, int32_t on);
 int cts_kernel_time(cts_handle *h, const char *name, double *avg_ms, int64_t *launches);
 
+/* Test hook: the device's w(z) (the Faddeeva function behind every
+ * Bassetti-Erskine kick) at n points. */
+int cts_debug_faddeeva(cts_handle *h, int64_t n, const double *re, const double *im, double *wr, double *wi);
 /* Test hook: the device quantum-excitation stream for particles
  * [base, base+n) at `turn` (noise_out[5][n]). */
 int cts_debug_qe_normals(cts_handle *h, uint64_t seed, int32_t turn, uint64_t base,

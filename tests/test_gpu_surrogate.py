@@ -1,8 +1,11 @@
 """The learned field surrogate on the GPU against its CPU restatement
 (tests/mlp_ref.py) and the oracle's FP64 field.
 
-Tolerances: training/validation samples' field values 1e-13 relative to
-the oracle (same FP64 algorithm, libm last bits); network gradient and
+Tolerances: training/validation samples' field values within 1e-13
+relative plus 2e-14 absolute of the oracle (normalized fields are O(1);
+the device evaluates w(z) near the beam by Weideman's rational
+approximation, |w| relative error below 4e-14, the oracle by the
+reference's Algorithm 916 series); network gradient and
 outputs within the bf16x3 matrix-core precision (~2^-16 relative per
 product): gradients 2e-3 relative to the largest component, outputs 2e-4
 absolute on O(1) outputs.
@@ -35,8 +38,8 @@ def test_samples_match_oracle(sur):
     fx, fy = ctypes.c_double(), ctypes.c_double()
     for au, av, r, gx, gy in val[:2000:7]:
         L.orc_ion_field(au, av * r, 1.0, r, fx, fy)
-        assert abs(gx - fx.value) <= 1e-13 * abs(fx.value) + 1e-300
-        assert abs(gy - fy.value) <= 1e-13 * abs(fy.value) + 1e-300
+        assert abs(gx - fx.value) <= 1e-13 * abs(fx.value) + 2e-14
+        assert abs(gy - fy.value) <= 1e-13 * abs(fy.value) + 2e-14
     assert np.all((val[:, 0] > 0) & (val[:, 0] <= 6) & (val[:, 1] > 0) & (val[:, 1] <= 6))
     assert np.all((val[:, 2] >= 0.02 * (1 - 1e-12)) & (val[:, 2] <= 1))
     # training records encode the same kind of points: features in [-1, 1],
