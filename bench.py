@@ -1,14 +1,20 @@
 """Throughput of the multi-particle tracking turn on one MI355X.
 
-Workload (default): single bunch, 1e7 FP64 macroparticles, the one-section
-ring of cetasim_amd.configs, quantum excitation from the on-device Philox
-stream (seed 2), broadband-resonator short-range wake on 400 fixed bins over
-+-5 sigma_z.  A step is one full turn (section map, RF, drift, U0, radiation
-damping + excitation, binning, wake kick, loss marking, moments).
+Workload (default, --workload c5): a 480-bunch train (5.9e7 FP64
+macroparticles, counts uniform in [5e4, 2e5], seed 5) with a broadband-
+resonator wake on 100 fixed bins per bunch, quantum excitation from the
+device Philox stream, and residual-gas ions at one interaction point (10 000
+new macro-ions per passage, at most 5e6) kicked both ways by the
+Bassetti-Erskine field; the turns run as HIP-graph replays.  It is timed
+with the exact FP64 field (plain) and with the learned field surrogate
+(trained on the GPU first); `value` is the surrogate rate.  --workload c3 is
+the 80-bunch train, c2 the single 1e7-macroparticle bunch without ions.  A
+step is one full turn.
 
-Prints one JSON line: macroparticle-turns/s, the dominant kernel's HBM
-roofline fraction (algorithmic 96 B per macroparticle-turn over the kernel's
-HIP-event duration) and the CPU oracle's rate on a bounded sample.
+Prints one JSON line: macroparticle-turns/s, the streaming pass's HBM
+roofline fraction (algorithmic 96 B per macroparticle-turn over the
+kernel's HIP-event duration), per-kernel times, the surrogate's field error
+and the CPU oracle's rate extrapolated from bounded samples.
 """
 import argparse
 import json
@@ -21,6 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "macroparticle-turns/s on 1 MI355X: surrogate vs plain kick at \u22641e-4 field err"
 BYTES_PER_MPT = 96.0  # 6 FP64 read + 6 FP64 written per macroparticle-turn
 
 
@@ -47,10 +54,10 @@ def parse():
     ap.add_argument("--nbins", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--workload", choices=("c2", "c3", "c5"), default="c2",
-                    help="c2: single bunch 1e7 macroparticles with wake and excitation; "
-                         "c3: 80-bunch train with the beam-ion phase; "
-                         "c5: 480-bunch train (6e7 macroparticles) with wake and ions")
+    ap.add_argument("--workload", choices=("c2", "c3", "c5"), default="c5",
+                    help="c5 (default): 480-bunch train (6e7 macroparticles) with wake and ions, plain vs "
+                         "surrogate kicks; c3: 80-bunch train with the beam-ion phase; "
+                         "c2: single bunch 1e7 macroparticles with wake and excitation (plain path)")
     ap.add_argument("--eager", action="store_true",
                     help="time turns launched one by one instead of HIP-graph replays")
     ap.add_argument("--plain-only", action="store_true", help="c3: skip the surrogate run")
@@ -122,40 +129,43 @@ def cpu_baseline(ring, secs, p, budget_s):
                       f"(oracle/ -O2 C restatement, {platform.processor() or platform.machine()})"}
 
 
-def cpu_baseline_train(ring, secs, p, budget_s):
-    """The oracle (one core) on the train with ions, timed in three parts
-    and extrapolated to the workload: one full ion phase (80 passages, 6000
-    new macro-ions each, 300 macroparticles per bunch), the electron-side
-    kick per macroparticle (one 200 000-macroparticle bunch through a
-    10-ion cloud), and the rest of the turn per macroparticle (turns of a
-    20 000-macroparticle bunch)."""
+def cpu_baseline_train(ring, secs, p, budget_s, T, ion_evals, ntot):
+    """The oracle (one core) on bounded samples of the train, extrapolated to
+    the workload: the ion phase per macro-ion passage (a 40-bunch train of
+    300-macroparticle bunches, 1000 new macro-ions per passage), the
+    electron-side kick per macroparticle (one 200 000-macroparticle bunch
+    through a 10-ion cloud) and the rest of the turn per macroparticle
+    (turns of a 20 000-macroparticle bunch with the workload's wake and
+    excitation); ion_evals = macro-ion passages per turn of the GPU run."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
     import oracle as O
     from cetasim_amd import configs
 
-    T = configs.TRAIN
-    nb = T["nb"]
-    gap = configs.train_gaps(nb, T["spacing"], ring.harmonic)
-    cfg = configs.ion_config(ring, T["ions"], p)
-
-    def phase_time(counts, cfg_):
+    def phase(counts, ions):
         counts = np.asarray(counts)
+        nb = len(counts)
+        gap = configs.train_gaps(nb, T["spacing"], ring.harmonic)
+        cfg = configs.ion_config(ring, ions, p)
         q, ne = configs.bunch_scalars(counts, ring, p)
         parts = configs.train_bunches(counts, 3, p)
         off = np.concatenate([[0], np.cumsum(counts)])
-        bunches = [O.Bunch({c: parts[c][off[b]:off[b + 1]] for c in O.COORDS}) for b in range(len(counts))]
-        ot = O.OracleTrain(ring, secs[0], counts, ne, gap[:len(counts)], cfg_, T["ions"]["seed"])
+        bunches = [O.Bunch({c: parts[c][off[b]:off[b + 1]] for c in O.COORDS}) for b in range(nb)]
+        ot = O.OracleTrain(ring, secs[0], counts, ne, gap, cfg, ions["seed"])
         t0 = time.perf_counter()
-        ot.ion_phase(bunches, 0)
-        return time.perf_counter() - t0
+        out = ot.ion_phase(bunches, 0)
+        return time.perf_counter() - t0, float(out[:, :, 0].sum())
 
-    t_ion = phase_time(np.full(nb, 300), cfg)
-    small = configs.ion_config(ring, dict(T["ions"], n_new=10), p)
+    t_ph, evals = phase(np.full(40, 300), dict(T["ions"], n_new=1000))
+    t_eval = t_ph / evals
     ne_big = 200_000
-    t_e = phase_time([ne_big], small) / ne_big
+    t_k, _ = phase([ne_big], dict(T["ions"], n_new=10))
+    t_e = t_k / ne_big
     n = 20_000
-    ot = O.OracleTurn(ring, secs[0], synrad=True, noise_seed=2)
+    sz = p["sig"][2]
+    kw = dict(nbins=T["nbins"], mode=1, grid=(-5 * sz, 5 * sz), bbr=p["bbr"],
+              macro_charge=configs.bunch_scalars([n], ring, p)[0][0]) if T.get("nbins") else {}
+    ot = O.OracleTurn(ring, secs[0], synrad=True, noise_seed=2, **kw)
     ob = O.Bunch(configs.gaussian_bunch(n, 2, p))
     turns = 0
     t0 = time.perf_counter()
@@ -163,14 +173,13 @@ def cpu_baseline_train(ring, secs, p, budget_s):
         ot.turn(ob, turns)
         turns += 1
     t_rest = (time.perf_counter() - t0) / (n * turns)
-    ntot = int(configs.train_counts().sum())
-    t_turn = t_ion + ntot * (t_e + t_rest)
+    t_turn = ion_evals * t_eval + ntot * (t_e + t_rest)
     return {"value": ntot / t_turn, "unit": "macroparticle-turns/s", "cores": 1, "kind": "port",
-            "sample": f"one full ion phase (80 passages, cloud to {int(cfg.n_new) * nb} macro-ions, "
-                      f"{t_ion:.2f} s), electron-side kick {t_e * 1e9:.0f} ns per macroparticle, rest of the "
-                      f"turn {t_rest * 1e9:.0f} ns per macroparticle ({turns} turns of {n}); extrapolated to "
-                      f"{ntot} macroparticles (oracle/ -O2 C restatement, "
-                      f"{platform.processor() or platform.machine()})"}
+            "sample": f"ion phase {t_eval * 1e9:.0f} ns per macro-ion passage (40-bunch sample, {evals:.0f} "
+                      f"passages, {t_ph:.2f} s) x {ion_evals:.3g} passages per turn of the GPU run; "
+                      f"electron-side kick {t_e * 1e9:.0f} ns and rest of the turn {t_rest * 1e9:.0f} ns per "
+                      f"macroparticle ({turns} turns of {n}) x {ntot} macroparticles "
+                      f"(oracle/ -O2 C restatement, {platform.processor() or platform.machine()})"}
 
 
 def train_surrogate(steps):
@@ -268,7 +277,7 @@ def run_train(a):
     pass_ms = ktimes["pass"][0]
     achieved = BYTES_PER_MPT * n / (pass_ms * 1e-3) / 1e9
     out = {
-        "metric": "macroparticle-turns/s on 1 MI355X: surrogate vs plain kick (batched train with ions)",
+        "metric": METRIC,
         "value": value,
         "unit": "macroparticle-turns/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt / a.steps * 1e3,
@@ -290,8 +299,8 @@ def run_train(a):
         "ion_macroparticles_last_passage": float(cloud[-1, 0, 0]),
         "surrogate": sur_info,
     }
-    if not a.no_cpu_baseline and a.workload == "c3":
-        out["cpu_baseline"] = cpu_baseline_train(ring, secs, p, a.cpu_seconds)
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_train(ring, secs, p, a.cpu_seconds, T, float(cloud[:, :, 0].sum()), n)
     print(json.dumps(out))
 
 

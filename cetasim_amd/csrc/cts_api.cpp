@@ -134,14 +134,16 @@ struct cts_handle {
     int64_t ion_len_ub = 0;  // host upper bound of the slots in use at turn start
     double *d_ion[2][CTS_MAX_SPECIES][4] = {};
     uint8_t *d_ion_alive[2][CTS_MAX_SPECIES] = {};
-    int64_t *d_ion_len = nullptr;     // [2][CTS_MAX_SPECIES]
+    int64_t *d_ion_len = nullptr;     // [3][CTS_MAX_SPECIES]: the two buffers, ions at turn start
     int64_t *d_ion_nalive = nullptr;  // [CTS_MAX_SPECIES]
     double *d_ion_mean = nullptr;     // [2 * CTS_MAX_SPECIES]
     double *d_ion_bview = nullptr, *d_ion_smax = nullptr, *d_ion_q = nullptr;
     double *d_ion_cloud = nullptr;
     double *d_ion_coeff = nullptr, *d_ion_dt = nullptr;  // [nb][CTS_MAX_SPECIES], [nb]
-    unsigned long long *d_ion_acc = nullptr;             // [nspec][nb][kIonAcc][3]
-    uint32_t *d_ion_chunk = nullptr;
+    unsigned long long *d_ion_acc = nullptr;  // kIonReplicas + 1 copies of [nspec][nb][kIonAcc][3]
+    uint32_t *d_ion_wcnt = nullptr;  // [nspec][ion_wstride]: survivors per wave of a segment
+    uint32_t *d_ion_wofs = nullptr;  // [ion_wstride]: their prefix sums
+    int64_t ion_wstride = 0;
     // d_cur holds the moments of the present coordinates
     bool cur_valid = false;
     // field model of the beam-ion kicks: surrogate parameters (null: exact)
@@ -645,14 +647,15 @@ void free_ion_state(cts_handle *h)
             h->d_ion_alive[b][p] = nullptr;
         }
     void *ptrs[] = {h->d_ion_len, h->d_ion_nalive, h->d_ion_mean, h->d_ion_bview, h->d_ion_smax,
-                    h->d_ion_q, h->d_ion_cloud, h->d_ion_coeff, h->d_ion_dt, h->d_ion_acc, h->d_ion_chunk};
+                    h->d_ion_q, h->d_ion_cloud, h->d_ion_coeff, h->d_ion_dt, h->d_ion_acc, h->d_ion_wcnt, h->d_ion_wofs};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     h->d_ion_len = h->d_ion_nalive = nullptr;
     h->d_ion_mean = h->d_ion_bview = h->d_ion_smax = h->d_ion_q = h->d_ion_cloud = nullptr;
     h->d_ion_coeff = h->d_ion_dt = nullptr;
     h->d_ion_acc = nullptr;
-    h->d_ion_chunk = nullptr;
+    h->d_ion_wcnt = h->d_ion_wofs = nullptr;
+    h->ion_wstride = 0;
     h->nspec = 0;
     h->ion_cap = 0;
     h->ion_len_ub = 0;
@@ -660,10 +663,6 @@ void free_ion_state(cts_handle *h)
 }
 
 
-// MPBeam::SSBeamIonEffectOneInteractionPoint at interaction point 0
-// (src/MPBeam.cpp:2585-2611): bunch views from the moments of the present
-// state, one launch per bunch passage in train order, the turn-end
-// compaction of the cloud, then the electron kicks of every bunch.
 // Drop the captured turn graphs (their kernel arguments name buffers and
 // settings that are about to change).
 void graph_reset(cts_handle *h)
@@ -675,6 +674,18 @@ void graph_reset(cts_handle *h)
     h->turn_dev_next = -1;
 }
 
+// The cloud buffer after one ion phase: one compaction per segment.
+int ion_buf_after_turn(const cts_handle *h)
+{
+    const int segs = (h->nb + kIonSegment - 1) / kIonSegment;
+    return (h->nspec > 0 && segs % 2 == 1) ? 1 - h->ion_buf : h->ion_buf;
+}
+
+// MPBeam::SSBeamIonEffectOneInteractionPoint at interaction point 0
+// (src/MPBeam.cpp:2585-2611): bunch views from the moments of the present
+// state, the bunch passages in train order (segments of kIonSegment
+// passages, each followed by a compaction of the cloud), the passage
+// moments, then the electron kicks of every bunch.
 int ion_phase(cts_handle *h, int32_t turn)
 {
     int rc;
@@ -682,27 +693,22 @@ int ion_phase(cts_handle *h, int32_t turn)
     const cts_ion_config &I = h->ion;
     const cts_section &S = h->secs[0];
     const int64_t nacc = (int64_t)h->nspec * h->nb * kIonAcc * 3;
+    int64_t *len0 = h->d_ion_len + 2 * CTS_MAX_SPECIES;
     {
         TimedScope ts(h, "ion_prep");
         HIPCHK(launch_ion_prep(h->d_cur, h->nb, S.beta_x, S.beta_y, h->d_ion_bview, h->d_ion_smax, h->d_fb_count,
-                               h->d_ion_acc, nacc, h->stream));
+                               h->d_ion_len + h->ion_buf * CTS_MAX_SPECIES, len0, h->nspec, h->stream));
     }
-    const int buf = h->ion_buf;
     IonTurnParams P;
     std::memset(&P, 0, sizeof P);
-    for (int p = 0; p < h->nspec; p++) {
-        P.x[p] = h->d_ion[buf][p][0];
-        P.y[p] = h->d_ion[buf][p][1];
-        P.vx[p] = h->d_ion[buf][p][2];
-        P.vy[p] = h->d_ion[buf][p][3];
-        P.alive[p] = h->d_ion_alive[buf][p];
-    }
-    P.len = h->d_ion_len + buf * CTS_MAX_SPECIES;
     P.bview = h->d_ion_bview;
     P.size_max = h->d_ion_smax;
     P.coeffI = h->d_ion_coeff;
     P.dt = h->d_ion_dt;
     P.acc = h->d_ion_acc;
+    P.acc_stride = nacc;
+    P.wcnt = h->d_ion_wcnt;
+    P.wcnt_stride = h->ion_wstride;
     P.sur_params = nullptr;
     if (h->sur && (h->sur_mode & 1)) sur_export(h->sur, &P.sur_params, &P.sur_norm);
     P.loss_boundary = I.loss_boundary;
@@ -712,26 +718,67 @@ int ion_phase(cts_handle *h, int32_t turn)
     P.seed = I.seed;
     P.nb = h->nb;
     P.n_new = I.n_new;
-    // waves: the ions present at the start of the turn (the host's bound of
-    // their count; the capacity in a graph, which replays any turn), then
-    // the new ions of every passage
-    const int64_t old_ub = h->capturing ? h->ion_cap : h->ion_len_ub;
-    P.w_old = (int32_t)((old_ub + 63) / 64);
     P.w_new = (I.n_new + 63) / 64;
     P.turn = (uint32_t)turn;
     if (h->capturing) {
         P.turn_dev = h->d_turn;
         P.turn = (uint32_t)h->cap_off;
     }
+    // segments of kIonSegment passages; after each, the cloud (survivors and
+    // the segment's new ions, in slot order) is compacted into the other
+    // buffer.  ub: the host's bound of the ions at the segment's start (the
+    // capacity in a graph, which replays any turn).
+    int64_t ub = h->capturing ? h->ion_cap : h->ion_len_ub;
+    for (int ja = 0; ja < h->nb; ja += kIonSegment) {
+        const int jb = std::min(h->nb, ja + kIonSegment);
+        const int buf = h->ion_buf;
+        for (int p = 0; p < h->nspec; p++) {
+            P.x[p] = h->d_ion[buf][p][0];
+            P.y[p] = h->d_ion[buf][p][1];
+            P.vx[p] = h->d_ion[buf][p][2];
+            P.vy[p] = h->d_ion[buf][p][3];
+            P.alive[p] = h->d_ion_alive[buf][p];
+        }
+        P.len = h->d_ion_len + buf * CTS_MAX_SPECIES;
+        P.j_begin = ja;
+        P.j_end = jb;
+        const int64_t waves = (ub + 63) / 64 + (int64_t)(jb - ja) * P.w_new;
+        {
+            TimedScope ts(h, "ion_turn");
+            HIPCHK(launch_ion_turn(P, h->nspec, waves, h->stream));
+        }
+        ub = std::min<int64_t>(ub + (int64_t)(jb - ja) * I.n_new, h->ion_cap);
+        for (int p = 0; p < h->nspec; p++) {
+            IonCompactParams c;
+            c.x = h->d_ion[buf][p][0]; c.y = h->d_ion[buf][p][1];
+            c.vx = h->d_ion[buf][p][2]; c.vy = h->d_ion[buf][p][3];
+            c.alive = h->d_ion_alive[buf][p];
+            c.ox = h->d_ion[1 - buf][p][0]; c.oy = h->d_ion[1 - buf][p][1];
+            c.ovx = h->d_ion[1 - buf][p][2]; c.ovy = h->d_ion[1 - buf][p][3];
+            c.oalive = h->d_ion_alive[1 - buf][p];
+            c.len = h->d_ion_len + buf * CTS_MAX_SPECIES + p;
+            c.new_len = h->d_ion_len + (1 - buf) * CTS_MAX_SPECIES + p;
+            c.wcnt = h->d_ion_wcnt + p * h->ion_wstride;
+            c.wofs = h->d_ion_wofs;
+            c.cap = h->ion_cap;
+            c.n_new = I.n_new;
+            c.w_new = P.w_new;
+            c.nseg = jb - ja;
+            TimedScope ts(h, "ion_compact");
+            HIPCHK(launch_ion_compact(c, waves, h->stream));
+        }
+        h->ion_buf = 1 - buf;
+    }
+    h->ion_len_ub = ub;
     {
-        TimedScope ts(h, "ion_turn");
-        HIPCHK(launch_ion_turn(P, h->nspec, h->stream));
+        TimedScope ts(h, "ion_reduce");
+        HIPCHK(launch_ion_reduce(h->d_ion_acc, nacc, h->d_ion_acc + kIonReplicas * nacc, h->stream));
     }
     IonFinishParams F;
-    F.acc = h->d_ion_acc;
+    F.acc = h->d_ion_acc + kIonReplicas * nacc;
     F.bview = h->d_ion_bview;
     F.ion_q = h->d_ion_q;
-    F.len = h->d_ion_len + buf * CTS_MAX_SPECIES;
+    F.len0 = len0;
     F.n_alive = h->d_ion_nalive;
     F.mean_state = h->d_ion_mean;
     F.ion_e = h->d_ion_e;
@@ -739,7 +786,6 @@ int ion_phase(cts_handle *h, int32_t turn)
     F.status = h->d_status;
     F.gamma = h->ring.rgamma;
     F.max_ions = I.max_ions;
-    F.cap = h->ion_cap;
     F.nb = h->nb;
     F.nspec = h->nspec;
     F.n_new = I.n_new;
@@ -747,25 +793,6 @@ int ion_phase(cts_handle *h, int32_t turn)
         TimedScope ts(h, "ion_finish");
         HIPCHK(launch_ion_finish(F, h->stream));
     }
-    const int64_t ub = h->capturing ? h->ion_cap
-                                    : std::min<int64_t>(h->ion_len_ub + (int64_t)h->nb * I.n_new, h->ion_cap);
-    // stable compaction of every species into the other buffer
-    for (int p = 0; p < h->nspec; p++) {
-        IonCompactParams c;
-        c.x = h->d_ion[buf][p][0]; c.y = h->d_ion[buf][p][1];
-        c.vx = h->d_ion[buf][p][2]; c.vy = h->d_ion[buf][p][3];
-        c.alive = h->d_ion_alive[buf][p];
-        c.ox = h->d_ion[1 - buf][p][0]; c.oy = h->d_ion[1 - buf][p][1];
-        c.ovx = h->d_ion[1 - buf][p][2]; c.ovy = h->d_ion[1 - buf][p][3];
-        c.oalive = h->d_ion_alive[1 - buf][p];
-        c.len = h->d_ion_len + buf * CTS_MAX_SPECIES + p;
-        c.new_len = h->d_ion_len + (1 - buf) * CTS_MAX_SPECIES + p;
-        c.chunk_cnt = h->d_ion_chunk;
-        TimedScope ts(h, "ion_compact");
-        HIPCHK(launch_ion_compact(c, ub, h->stream));
-    }
-    h->ion_buf = 1 - buf;
-    h->ion_len_ub = ub;
     // electron side (src/MPBunch.cpp:1165-1206, src/Bunch.cpp:213-220)
     IonKickParams ik;
     ik.x = h->d_x; ik.y = h->d_y; ik.px = h->d_px; ik.py = h->d_py; ik.flag = h->d_flag;
@@ -1219,7 +1246,7 @@ int cts_run_turns(cts_handle *h, int32_t first, int32_t nturns, int32_t use_grap
             // a graph captured one turn later in the buffer cycle serves
             // after one eager turn (cheaper than a capture)
             const int p1 = h->nbins > 0 ? 1 - h->hist_parity : h->hist_parity;
-            const int b1 = h->nspec > 0 ? 1 - h->ion_buf : h->ion_buf;
+            const int b1 = ion_buf_after_turn(h);
             bool later = false;
             for (auto &g : h->graphs)
                 if (g.exec && g.parity == p1 && g.ion_buf == b1 && g.sur_params == sp &&
@@ -1443,22 +1470,25 @@ int cts_set_ions(cts_handle *h, const cts_ion_config *cfg)
     // before the turn-end compaction
     const int64_t cap = cfg->max_ions + (int64_t)h->nb * cfg->n_new;
     if (cap > (int64_t)1 << 31) return fail(CTS_ERR_ARG, "cts_set_ions: cloud capacity above 2^31 slots");
+    // waves of a segment: the compacted cloud, then kIonSegment passages' new ions
+    const int64_t wstride = (cap + 63) / 64 + (int64_t)kIonSegment * ((cfg->n_new + 63) / 64) + 1;
     h->ion = *cfg;
     h->ion_cap = cap;
+    h->ion_wstride = wstride;
     for (int b = 0; b < 2; b++)
         for (int p = 0; p < cfg->nspec; p++) {
             for (int c = 0; c < 4; c++)
                 if ((rc = dalloc(&h->d_ion[b][p][c], (size_t)cap))) return rc;
             if ((rc = dalloc(&h->d_ion_alive[b][p], (size_t)cap))) return rc;
         }
-    const int64_t nchunk = (cap + kIonChunk - 1) / kIonChunk;
-    if ((rc = dalloc(&h->d_ion_len, 2 * CTS_MAX_SPECIES)) || (rc = dalloc(&h->d_ion_nalive, CTS_MAX_SPECIES)) ||
+    if ((rc = dalloc(&h->d_ion_len, 3 * CTS_MAX_SPECIES)) || (rc = dalloc(&h->d_ion_nalive, CTS_MAX_SPECIES)) ||
         (rc = dalloc(&h->d_ion_mean, 2 * CTS_MAX_SPECIES)) || (rc = dalloc(&h->d_ion_bview, (size_t)h->nb * 4)) ||
         (rc = dalloc(&h->d_ion_smax, 2)) || (rc = dalloc(&h->d_ion_q, (size_t)h->nb * cfg->nspec)) ||
         (rc = dalloc(&h->d_ion_cloud, (size_t)h->nb * CTS_MAX_SPECIES * 6)) ||
         (rc = dalloc(&h->d_ion_coeff, (size_t)h->nb * CTS_MAX_SPECIES)) || (rc = dalloc(&h->d_ion_dt, (size_t)h->nb)) ||
-        (rc = dalloc(&h->d_ion_acc, (size_t)cfg->nspec * h->nb * kIonAcc * 3)) ||
-        (rc = dalloc(&h->d_ion_chunk, (size_t)nchunk)))
+        (rc = dalloc(&h->d_ion_acc, (size_t)(kIonReplicas + 1) * cfg->nspec * h->nb * kIonAcc * 3)) ||
+        (rc = dalloc(&h->d_ion_wcnt, (size_t)(cfg->nspec * wstride))) ||
+        (rc = dalloc(&h->d_ion_wofs, (size_t)wstride)))
         return rc;
     // per bunch: ion kick coefficient 2 N_e r_e (m_e / m_u) / A c
     // (src/MPBunch.cpp:1117) and drift time C / h * gap / c

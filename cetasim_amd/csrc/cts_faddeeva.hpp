@@ -125,6 +125,105 @@ __device__ __forceinline__ void w_weideman(double zr, double zi, double &wr, dou
     wi = 2.0 * (pr * u2i + pi * u2r) + kIsPi * ui;
 }
 
+// Continued fraction / asymptotic forms for large |z| (the package's
+// first region, src/Faddeeva.cpp:740-790), for xs = Re z reflected to
+// Im z >= 0 and ya = |Im z|; x = |xs|.
+__device__ __forceinline__ void w_cf(double xs, double ya, double x, double &retr, double &reti, bool &inf_case)
+{
+    inf_case = false;
+    if (x + ya > 4000) {
+        if (x + ya > 1e7) {
+            if (x > ya) {
+                const double yax = ya / xs;
+                const double denom = kIsPi / (xs + yax * ya);
+                retr = denom * yax; reti = denom;
+            } else if (isinf(ya)) {
+                inf_case = true;
+                retr = 0; reti = 0;
+            } else {
+                const double xya = xs / ya;
+                const double denom = kIsPi / (xya * xs + ya);
+                retr = denom; reti = denom * xya;
+            }
+        } else {
+            const double dr = xs * xs - ya * ya - 0.5, di = 2 * xs * ya;
+            const double denom = kIsPi / (dr * dr + di * di);
+            retr = denom * (xs * di - ya * dr);
+            reti = denom * (xs * dr + ya * di);
+        }
+    } else {
+        // depth fit of the reference's package (src/Faddeeva.cpp:758-759)
+        double nu = floor(3.9 + 11.398 / (0.08254 * x + 0.1421 * ya + 0.2023));
+        double wr = xs, wi = ya;
+        for (nu = 0.5 * (nu - 1); nu > 0.4; nu -= 0.5) {
+            const double denom = nu / (wr * wr + wi * wi);
+            wr = xs - wr * denom;
+            wi = ya + wi * denom;
+        }
+        const double denom = kIsPi / (wr * wr + wi * wi);
+        retr = denom * wi;
+        reti = denom * wr;
+    }
+}
+
+// The package's sums for |x| >= 10 next to the real axis
+// (src/Faddeeva.cpp:880-945); returns w.
+__device__ inline void w_large_x(double zr, double y, double x, double &wr_out, double &wi_out)
+{
+    const double a = c_fad[0], c = c_fad[1], a2 = c_fad[2];
+    const double relerr = 2.220446049250313e-16;
+    double sum3, sum5;
+    const double retr = exp(-x * x);
+    const double n0 = floor(x / a + 0.5);
+    const double dx = a * n0 - x;
+    sum3 = exp(-dx * dx) / (a2 * (n0 * n0) + y * y);
+    sum5 = a * n0 * sum3;
+    const double exp1 = exp(4 * a * dx);
+    double exp1dn = 1;
+    int dn;
+    bool done = false;
+    for (dn = 1; n0 - dn > 0; ++dn) {
+        const double np = n0 + dn, nm = n0 - dn;
+        double tp = exp(-((a * dn + dx) * (a * dn + dx)));
+        double tm = tp * (exp1dn *= exp1);
+        tp /= (a2 * (np * np) + y * y);
+        tm /= (a2 * (nm * nm) + y * y);
+        sum3 += tp + tm;
+        sum5 += a * (np * tp + nm * tm);
+        if (a * (np * tp + nm * tm) < relerr * sum5) { done = true; break; }
+    }
+    for (int guard = 0; !done && guard < 200; guard++) {
+        const double np = n0 + dn++;
+        const double tp = exp(-((a * dn + dx) * (a * dn + dx))) / (a2 * (np * np) + y * y);
+        sum3 += tp;
+        sum5 += a * np * tp;
+        if (a * np * tp < relerr * sum5) done = true;
+    }
+    wr_out = retr + (0.5 * c) * y * sum3;
+    wi_out = (0.5 * c) * copysign(sum5, zr);
+}
+
+__device__ __forceinline__ bool w_cf_region(double x, double ya)
+{
+    return ya > 7 || (x > 6 && (ya > 0.1 || (x > 8 && ya > 1e-10) || x > 28));
+}
+
+// w(z) for Re z >= 0, Im z >= 0, both finite and not on an axis (the
+// arguments of the Bassetti-Erskine field): continued fraction far out,
+// Weideman's approximation near the beam, the large-x sums near the axis.
+__device__ inline void w_quadrant1(double x, double y, double &wr, double &wi)
+{
+    if (w_cf_region(x, y)) {
+        bool inf_case;
+        w_cf(x, y, x, wr, wi, inf_case);
+        if (inf_case) { wr = 0; wi = 0; }
+    } else if (x < 10) {
+        w_weideman(x, y, wr, wi);
+    } else {
+        w_large_x(x, y, x, wr, wi);
+    }
+}
+
 __device__ inline void w_general(double zr, double zi, double &wr_out, double &wi_out)
 {
     const double a = c_fad[0], c = c_fad[1], a2 = c_fad[2];
@@ -134,41 +233,14 @@ __device__ inline void w_general(double zr, double zi, double &wr_out, double &w
     double retr = 0, reti = 0;
     double sum1 = 0, sum2 = 0, sum3 = 0, sum4 = 0, sum5 = 0;
 
-    if (ya > 7 || (x > 6 && (ya > 0.1 || (x > 8 && ya > 1e-10) || x > 28))) {
+    if (w_cf_region(x, ya)) {
         const double xs = y < 0 ? -zr : zr;
-        if (x + ya > 4000) {
-            if (x + ya > 1e7) {
-                if (x > ya) {
-                    const double yax = ya / xs;
-                    const double denom = kIsPi / (xs + yax * ya);
-                    retr = denom * yax; reti = denom;
-                } else if (isinf(ya)) {
-                    if (isnan(x) || y < 0) { wr_out = NAN; wi_out = NAN; }
-                    else { wr_out = 0; wi_out = 0; }
-                    return;
-                } else {
-                    const double xya = xs / ya;
-                    const double denom = kIsPi / (xya * xs + ya);
-                    retr = denom; reti = denom * xya;
-                }
-            } else {
-                const double dr = xs * xs - ya * ya - 0.5, di = 2 * xs * ya;
-                const double denom = kIsPi / (dr * dr + di * di);
-                retr = denom * (xs * di - ya * dr);
-                reti = denom * (xs * dr + ya * di);
-            }
-        } else {
-            // depth fit of the reference's package (src/Faddeeva.cpp:758-759)
-            double nu = floor(3.9 + 11.398 / (0.08254 * x + 0.1421 * ya + 0.2023));
-            double wr = xs, wi = ya;
-            for (nu = 0.5 * (nu - 1); nu > 0.4; nu -= 0.5) {
-                const double denom = nu / (wr * wr + wi * wi);
-                wr = xs - wr * denom;
-                wi = ya + wi * denom;
-            }
-            const double denom = kIsPi / (wr * wr + wi * wi);
-            retr = denom * wi;
-            reti = denom * wr;
+        bool inf_case;
+        w_cf(xs, ya, x, retr, reti, inf_case);
+        if (inf_case) {
+            if (isnan(x) || y < 0) { wr_out = NAN; wi_out = NAN; }
+            else { wr_out = 0; wi_out = 0; }
+            return;
         }
         if (y < 0) {
             // w(z) = 2 exp(-z^2) - w(-z); exp(-inf + i*anything) = 0
@@ -244,33 +316,8 @@ __device__ inline void w_general(double zr, double zi, double &wr_out, double &w
     } else {
         if (isnan(x)) { wr_out = x; wi_out = x; return; }
         if (isnan(y)) { wr_out = y; wi_out = y; return; }
-        retr = exp(-x * x);
-        reti = 0;
-        const double n0 = floor(x / a + 0.5);
-        const double dx = a * n0 - x;
-        sum3 = exp(-dx * dx) / (a2 * (n0 * n0) + y * y);
-        sum5 = a * n0 * sum3;
-        const double exp1 = exp(4 * a * dx);
-        double exp1dn = 1;
-        int dn;
-        bool done = false;
-        for (dn = 1; n0 - dn > 0; ++dn) {
-            const double np = n0 + dn, nm = n0 - dn;
-            double tp = exp(-((a * dn + dx) * (a * dn + dx)));
-            double tm = tp * (exp1dn *= exp1);
-            tp /= (a2 * (np * np) + y * y);
-            tm /= (a2 * (nm * nm) + y * y);
-            sum3 += tp + tm;
-            sum5 += a * (np * tp + nm * tm);
-            if (a * (np * tp + nm * tm) < relerr * sum5) { done = true; break; }
-        }
-        for (int guard = 0; !done && guard < 200; guard++) {
-            const double np = n0 + dn++;
-            const double tp = exp(-((a * dn + dx) * (a * dn + dx))) / (a2 * (np * np) + y * y);
-            sum3 += tp;
-            sum5 += a * np * tp;
-            if (a * np * tp < relerr * sum5) done = true;
-        }
+        w_large_x(zr, y, x, wr_out, wi_out);
+        return;
     }
     wr_out = retr + (0.5 * c) * y * (sum2 + sum3);
     wi_out = reti + (0.5 * c) * copysign(sum5 - sum4, zr);
@@ -300,6 +347,14 @@ __device__ inline void faddeeva_w(double re, double im, double &wr, double &wi)
     w_general(re, im, wr, wi);
 }
 
+// w(z) for Re z >= 0, Im z >= 0 (any finite values, axes included).
+__device__ inline void faddeeva_w_q1(double re, double im, double &wr, double &wi)
+{
+    if (re == 0.0) { wr = cts_erfcx(im); wi = re; return; }
+    if (im == 0) { wr = exp(-(re * re)); wi = cts_w_im(re); return; }
+    w_quadrant1(re, im, wr, wi);
+}
+
 // Bunch::BassettiErskine1 (src/Bunch.cpp:119-170)
 __device__ inline void bassetti_erskine(double posx, double posy, double sx, double sy,
                                         double &fx, double &fy)
@@ -309,8 +364,8 @@ __device__ inline void bassetti_erskine(double posx, double posy, double sx, dou
     const double r = sy / sx;
     const double coeffBE = -1 * sqrt(kPi) / sigma;
     double w1r, w1i, w2r, w2i;
-    faddeeva_w(ax / sigma, ay / sigma, w1r, w1i);
-    faddeeva_w(r * ax / sigma, ay / sigma / r, w2r, w2i);
+    faddeeva_w_q1(ax / sigma, ay / sigma, w1r, w1i);
+    faddeeva_w_q1(r * ax / sigma, ay / sigma / r, w2r, w2i);
     const double tx = ax / sx, ty = ay / sy;
     const double z3 = -(tx * tx) / 2 - (ty * ty) / 2;
     const double w3 = -exp(z3);

@@ -103,55 +103,70 @@ struct IonKickParams {
     int32_t nspec;
 };
 
-// One turn of the ion cloud (all bunch passages, all species).  An ion's
-// motion depends only on the bunches' moments, which are fixed for the
-// turn, so every ion walks through all passages on its own thread; the
-// per-passage cloud moments are accumulated in 128-bit fixed point
-// (order-independent, hence deterministic) and finished passage by passage.
+// One segment of the ion turn: passages j_begin .. j_end - 1, all species.
+// An ion's motion depends only on the bunches' moments, which are fixed for
+// the turn, so every ion walks through the segment's passages on its own
+// thread; the per-passage cloud moments are accumulated in 128-bit fixed
+// point (order-independent, hence deterministic) and finished passage by
+// passage after the last segment.  Between segments the cloud is compacted,
+// so the waves of a segment carry few lost ions.
 struct IonTurnParams {
     double *x[CTS_MAX_SPECIES], *y[CTS_MAX_SPECIES], *vx[CTS_MAX_SPECIES], *vy[CTS_MAX_SPECIES];
     uint8_t *alive[CTS_MAX_SPECIES];
-    const int64_t *len;      // [nspec] ions of the current buffer (compacted)
+    const int64_t *len;      // [nspec] ions of the current buffer (compacted, all alive)
     const double *bview;     // [nb][4] bunch xbar, ybar, rmsRx, rmsRy
     const double *size_max;  // [2] largest rmsRx, rmsRy over the bunches
     const double *coeffI;    // [nb][CTS_MAX_SPECIES] 2 N_e r_e (m_e/m_u) / A c
     const double *dt;        // [nb] C / h * gap / c
-    unsigned long long *acc; // [nspec][nb][kIonAcc][3] fixed-point sums (three limbs)
+    // fixed-point sums (three limbs), kIonReplicas copies of
+    // [nspec][nb][kIonAcc][3] acc_stride apart: wave w adds into copy
+    // w % kIonReplicas, which spreads the atomics of one passage over as
+    // many addresses; k_ion_reduce adds the copies up and clears them
+    unsigned long long *acc;
+    int64_t acc_stride;
+    uint32_t *wcnt;          // [nspec][wcnt_stride]: survivors of each wave
+    int64_t wcnt_stride;
     const float *sur_params; // field surrogate (null: exact field)
     SurNorm sur_norm;
     double loss_boundary, ap_x, ap_y;
     int64_t cap;             // slots per species buffer
     uint64_t seed;
-    int32_t nb, n_new, w_old, w_new;  // waves of old ions, waves per passage of new ions
+    int32_t nb, n_new, w_new;         // w_new: waves per passage of new ions
+    int32_t j_begin, j_end;           // the segment's passages
     uint32_t turn;                    // or *turn_dev + turn when turn_dev is set
     const uint32_t *turn_dev;
 };
 
 // Cloud moments and electron-side parameters of every passage.
 struct IonFinishParams {
-    const unsigned long long *acc;
+    const unsigned long long *acc;  // the reduced sums, [nspec][nb][kIonAcc][3]
     const double *bview;
     const double *ion_q;     // [nb][nspec]
-    int64_t *len;            // [nspec] in: ions at the start of the turn; out: slots written
+    const int64_t *len0;     // [nspec] ions at the start of the turn
     int64_t *n_alive;        // [nspec] out: after the last passage
     double *mean_state;      // [2 * nspec]
     double *ion_e;           // [nb][CTS_MAX_SPECIES][5]
     double *cloud;           // [nb][CTS_MAX_SPECIES][6]
     uint32_t *status;
     double gamma;
-    int64_t max_ions, cap;
+    int64_t max_ions;
     int32_t nb, nspec, n_new;
 };
 constexpr int kIonAcc = 6;  // n, sum dx, sum dy, sum dx^2, sum dy^2, n kept
 
+// Stable compaction of one species after a segment, wave by wave in the
+// segment's wave order (old ions, then each passage's new ones: slot order).
 struct IonCompactParams {
     const double *x, *y, *vx, *vy;
     const uint8_t *alive;
     double *ox, *oy, *ovx, *ovy;
     uint8_t *oalive;
-    const int64_t *len;   // slots in use of the source buffer
-    int64_t *new_len;     // slots in use of the destination buffer
-    uint32_t *chunk_cnt;
+    const int64_t *len;      // compacted ions of the source buffer before the segment
+    int64_t *new_len;        // ions of the destination buffer
+    const uint32_t *wcnt;    // survivors of every wave of the segment
+    uint32_t *wofs;          // their exclusive prefix sums
+    int64_t cap;
+    int32_t n_new, w_new, nseg;
 };
 
 struct BinParams {
@@ -218,13 +233,24 @@ hipError_t launch_post(const FinParams &f, const WakeParams &w, hipStream_t st);
 hipError_t launch_wake_table(const WakeTableParams &p, int ntables, hipStream_t st);
 hipError_t launch_wake(const WakeParams &p, int nb, hipStream_t st);
 hipError_t launch_ion_prep(const cts_moments *cur, int nb, double beta_x, double beta_y, double *bview,
-                           double *size_max, uint32_t *fb_count, unsigned long long *acc, int64_t nacc,
+                           double *size_max, uint32_t *fb_count, const int64_t *len, int64_t *len0, int nspec,
                            hipStream_t st);
-hipError_t launch_ion_turn(const IonTurnParams &p, int nspec, hipStream_t st);
+// out[i] = sum over the kIonReplicas copies (stride n) of acc[i]; the
+// copies are left at zero
+hipError_t launch_ion_reduce(unsigned long long *acc, int64_t n, unsigned long long *out, hipStream_t st);
+hipError_t launch_ion_turn(const IonTurnParams &p, int nspec, int64_t wave_bound, hipStream_t st);
 hipError_t launch_ion_finish(const IonFinishParams &p, hipStream_t st);
-hipError_t launch_ion_compact(const IonCompactParams &p, int64_t len_bound, hipStream_t st);
-constexpr int kIonBlock = 256;
-constexpr int kIonChunk = 4096;  // compaction chunk (16 ions per thread)
+// wave_bound: a host bound of the segment's waves
+hipError_t launch_ion_compact(const IonCompactParams &p, int64_t wave_bound, hipStream_t st);
+constexpr int kIonTurnMaxBlocks = 4096;  // grid bound of an ion segment or compaction (blocks stride over the waves)
+#ifndef CTS_ION_REPLICAS
+#define CTS_ION_REPLICAS 8
+#endif
+constexpr int kIonReplicas = CTS_ION_REPLICAS;  // copies of the passage sums
+#ifndef CTS_ION_SEGMENT
+#define CTS_ION_SEGMENT 16
+#endif
+constexpr int kIonSegment = CTS_ION_SEGMENT;  // bunch passages per ion segment
 hipError_t launch_qe_debug(const float2 *tab, uint64_t seed, uint32_t turn, uint64_t base, int64_t n, double *out,
                            hipStream_t st);
 hipError_t upload_faddeeva_tables(const FadTables &t);

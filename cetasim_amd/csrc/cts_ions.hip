@@ -70,115 +70,137 @@ __device__ __forceinline__ double fx_value(const unsigned long long *acc)
            ldexp((double)(long long)acc[0], -100);
 }
 
-// grid = (waves / 4, nspec), four independent waves per block.  Wave w <
-// w_old holds the ions of slots [64 w, 64 w + 64) from passage 0; the other
-// waves hold new ions of passage j0 (slot len + j0 n_new + t).  The lanes
-// of a wave visit the passages in step; a lane drops out when its ion is
-// lost.  SUR: the bunch's field at the ion from the surrogate inside its
-// domain, the exact field elsewhere.
+// The slot of lane `lane` of wave w of a segment: waves below w_old hold
+// the compacted ions [64 w, 64 w + 64) (below len); the others the new ions
+// t of passage j_begin + q0, at slot len + q0 n_new + t (t below n_new;
+// past the capacity only after an overflow).
+__device__ __forceinline__ bool ion_wave_slot(int w, int w_old, int64_t len, int n_new, int w_new, int64_t cap,
+                                              int lane, int64_t &slot, int &q0, int &t)
+{
+    if (w < w_old) {
+        q0 = 0;
+        t = 0;
+        slot = (int64_t)w * 64 + lane;
+        return slot < len;
+    }
+    const int k = w - w_old;
+    q0 = k / w_new;
+    t = (k - q0 * w_new) * 64 + lane;
+    slot = len + (int64_t)q0 * n_new + t;
+    return t < n_new && slot < cap;
+}
+
+// grid = (blocks, nspec); the waves of the segment, four per block at a
+// time, striding over the grid.  Wave w < w_old holds the compacted ions of
+// slots [64 w, 64 w + 64), from the segment's first passage; the other
+// waves hold the new ions of passage j0 (slot len + (j0 - j_begin) n_new + t).
+// The lanes of a wave visit the passages in step; a lane drops out when its
+// ion is lost.  The segment's bunch data sit in LDS and the passage sums
+// are gathered in LDS (64-bit integer atomics), so the passage loop makes
+// no global memory access; each block adds its sums to one copy of the
+// global sums at the end.  SUR: the bunch's field at the ion from the
+// surrogate inside its domain, the exact field elsewhere.
 template <bool SUR>
 __global__ void __launch_bounds__(256)
 k_ion_turn(IonTurnParams P)
 {
     __shared__ __attribute__((aligned(16))) MlpLds L;
+    __shared__ double sbv[kIonSegment][4], scI[kIonSegment], sdt[kIonSegment];
+    __shared__ unsigned long long sacc[kIonSegment * kIonAcc * 3];
     const int p = blockIdx.y;
     const int64_t len = P.len[p];
-    {
-        // blocks whose four waves all have nothing to do leave before the
-        // (block-wide) weight load: the grid covers a bound of the old ions
-        const int wb = blockIdx.x * 4;
-        if (wb >= P.w_old + P.nb * P.w_new) return;
-        if (wb + 3 < P.w_old && (int64_t)wb * 64 >= len) return;
+    // waves: the ions alive at the start of the segment (compacted), then
+    // the new ions of every passage of the segment
+    const int w_old = (int)((len + 63) / 64);
+    const int nseg = P.j_end - P.j_begin;
+    const int total = w_old + nseg * P.w_new;
+    if (blockIdx.x * 4 >= total) return;  // whole block idle (uniform)
+    if (SUR) load_mlp_lds(L, P.sur_params, threadIdx.x, 256);
+    for (int i = threadIdx.x; i < nseg * 4; i += 256) sbv[i >> 2][i & 3] = P.bview[4 * P.j_begin + i];
+    for (int i = threadIdx.x; i < nseg; i += 256) {
+        scI[i] = P.coeffI[(P.j_begin + i) * CTS_MAX_SPECIES + p];
+        sdt[i] = P.dt[P.j_begin + i];
     }
-    if (SUR) {
-        load_mlp_lds(L, P.sur_params, threadIdx.x, 256);
-        __syncthreads();
-    }
+    for (int i = threadIdx.x; i < nseg * kIonAcc * 3; i += 256) sacc[i] = 0ull;
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const uint32_t turn = P.turn_dev ? *P.turn_dev + P.turn : P.turn;
-    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (w >= P.w_old + P.nb * P.w_new) return;
-    if (w < P.w_old && (int64_t)w * 64 >= len) return;  // whole wave past the old ions
-    int j0;
-    int64_t slot;
-    bool born = false;
-    if (w < P.w_old) {
-        j0 = 0;
-        slot = (int64_t)w * 64 + lane;
-    } else {
-        const int k = w - P.w_old;
-        j0 = k / P.w_new;
-        const int t = (k % P.w_new) * 64 + lane;
-        slot = len + (int64_t)j0 * P.n_new + t;
-        born = t < P.n_new;
-        if (slot >= P.cap) born = false;  // past the capacity: only after an overflow
-    }
     double *X = P.x[p], *Y = P.y[p], *VX = P.vx[p], *VY = P.vy[p];
     uint8_t *A = P.alive[p];
-    bool alive;
-    double x = 0.0, y = 0.0, vx = 0.0, vy = 0.0;
-    if (w < P.w_old) {
-        alive = slot < len && A[slot];
-        if (alive) { x = X[slot]; y = Y[slot]; vx = VX[slot]; vy = VY[slot]; }
-    } else {
-        alive = born;
-        if (alive) {
-            const double *bv = P.bview + 4 * j0;
-            ion_generate(P.seed, turn, (uint32_t)j0, p, (uint32_t)(slot - len - (int64_t)j0 * P.n_new), bv[0],
-                         bv[1], bv[2], bv[3], x, y);
-        }
-    }
     // loss boundary: min(ionLossBoundary * largest bunch size, aperture)
     const double lbx = P.loss_boundary * P.size_max[0], lby = P.loss_boundary * P.size_max[1];
     const double bx = P.ap_x > lbx ? lbx : P.ap_x;
     const double by = P.ap_y > lby ? lby : P.ap_y;
-    unsigned long long *acc = P.acc + (int64_t)p * P.nb * kIonAcc * 3;
-    for (int j = j0; j < P.nb; j++) {
-        if (!__any(alive)) break;
-        const double *bv = P.bview + 4 * j;
-        const double xm = bv[0], ym = bv[1], sx = bv[2], sy = bv[3];
-        const double dx = x - xm, dy = y - ym;
-        double fx = 0.0, fy = 0.0;
-        if (SUR) {
-            const bool ok = sur_field(L, P.sur_norm, alive, dx, dy, sx, sy, fx, fy);
-            if (alive && !ok) ion_field(dx, dy, sx, sy, fx, fy);
-        } else if (alive) {
-            ion_field(dx, dy, sx, sy, fx, fy);
+    for (int w = blockIdx.x * 4 + (threadIdx.x >> 6); w < total; w += gridDim.x * 4) {
+        // slot: this lane's ion in the buffer; mine: the lane holds one;
+        // q0: the wave's first passage, counted from j_begin
+        int q0, t;
+        int64_t slot;
+        const bool mine = ion_wave_slot(w, w_old, len, P.n_new, P.w_new, P.cap, lane, slot, q0, t);
+        double x = 0.0, y = 0.0, vx = 0.0, vy = 0.0;
+        if (mine) {
+            if (w < w_old) {
+                x = X[slot]; y = Y[slot]; vx = VX[slot]; vy = VY[slot];
+            } else {
+                ion_generate(P.seed, turn, (uint32_t)(P.j_begin + q0), p, (uint32_t)t, sbv[q0][0], sbv[q0][1],
+                             sbv[q0][2], sbv[q0][3], x, y);
+            }
         }
-        bool keep = false;
-        if (alive) {
-            const double cI = P.coeffI[j * CTS_MAX_SPECIES + p];
-            vx += cI * fx;
-            vy += cI * fy;
-            x += vx * P.dt[j];
-            y += vy * P.dt[j];
-            keep = !(fabs(x) > bx || fabs(y) > by);
-        }
-        // the cloud as the bunch finds it (before the kick), and the ions kept
-        double c[kIonAcc];
-        c[0] = alive ? 1.0 : 0.0;
-        c[1] = alive ? dx : 0.0;
-        c[2] = alive ? dy : 0.0;
-        c[3] = alive ? dx * dx : 0.0;
-        c[4] = alive ? dy * dy : 0.0;
-        c[5] = keep ? 1.0 : 0.0;
+        bool alive = mine;
+        for (int q = q0; q < nseg; q++) {
+            if (!__any(alive)) break;
+            const double xm = sbv[q][0], ym = sbv[q][1], sx = sbv[q][2], sy = sbv[q][3];
+            const double dx = x - xm, dy = y - ym;
+            double fx = 0.0, fy = 0.0;
+            if (SUR) {
+                const bool ok = sur_field(L, P.sur_norm, alive, dx, dy, sx, sy, fx, fy);
+                if (alive && !ok) ion_field(dx, dy, sx, sy, fx, fy);
+            } else if (alive) {
+                ion_field(dx, dy, sx, sy, fx, fy);
+            }
+            bool keep = false;
+            if (alive) {
+                const double cI = scI[q];
+                vx += cI * fx;
+                vy += cI * fy;
+                x += vx * sdt[q];
+                y += vy * sdt[q];
+                keep = !(fabs(x) > bx || fabs(y) > by);
+            }
+            // the cloud as the bunch finds it (before the kick), and the ions kept
+            double c[kIonAcc];
+            c[0] = alive ? 1.0 : 0.0;
+            c[1] = alive ? dx : 0.0;
+            c[2] = alive ? dy : 0.0;
+            c[3] = alive ? dx * dx : 0.0;
+            c[4] = alive ? dy * dy : 0.0;
+            c[5] = keep ? 1.0 : 0.0;
 #pragma unroll
-        for (int v = 0; v < kIonAcc; v++) c[v] = wave_sum(c[v]);
-        // lane 3 v + k adds limb k of value v
-        if (lane < 3 * kIonAcc) {
-            const int v = lane / 3, k = lane - 3 * v;
-            double cv = c[0];
+            for (int v = 0; v < kIonAcc; v++) c[v] = wave_sum(c[v]);
+            // lane 3 v + k adds limb k of value v
+            if (lane < 3 * kIonAcc) {
+                const int v = lane / 3, kk = lane - 3 * v;
+                double cv = c[0];
 #pragma unroll
-            for (int u = 1; u < kIonAcc; u++) cv = v == u ? c[u] : cv;
-            const long long l = fx_limb(cv, k);
-            if (l != 0) atomicAdd(acc + ((int64_t)j * kIonAcc + v) * 3 + k, (unsigned long long)l);
+                for (int u = 1; u < kIonAcc; u++) cv = v == u ? c[u] : cv;
+                const long long l = fx_limb(cv, kk);
+                if (l != 0) atomicAdd(&sacc[(q * kIonAcc + v) * 3 + kk], (unsigned long long)l);
+            }
+            alive = keep;
         }
-        alive = keep;
+        if (mine) {
+            X[slot] = x; Y[slot] = y; VX[slot] = vx; VY[slot] = vy;
+            A[slot] = alive ? 1 : 0;
+        }
+        const uint32_t kept = (uint32_t)__popcll(__ballot(mine && alive));
+        if (lane == 0) P.wcnt[p * P.wcnt_stride + w] = kept;
     }
-    const bool mine = (w < P.w_old) ? slot < len : born;
-    if (mine) {
-        X[slot] = x; Y[slot] = y; VX[slot] = vx; VY[slot] = vy;
-        A[slot] = alive ? 1 : 0;
+    __syncthreads();
+    unsigned long long *acc = P.acc + (int64_t)(blockIdx.x % kIonReplicas) * P.acc_stride +
+                              ((int64_t)p * P.nb + P.j_begin) * kIonAcc * 3;
+    for (int i = threadIdx.x; i < nseg * kIonAcc * 3; i += 256) {
+        const unsigned long long v = sacc[i];
+        if (v != 0ull) atomicAdd(acc + i, v);
     }
 }
 
@@ -198,7 +220,7 @@ k_ion_finish(IonFinishParams P)
         __syncthreads();
         if (threadIdx.x == 0) {
             double mx = P.mean_state[2 * p], my = P.mean_state[2 * p + 1];
-            double before = (double)P.len[p];
+            double before = (double)P.len0[p];
             bool overflow = false;
             for (int j = 0; j < P.nb; j++) {
                 const double *a = sums + j * kIonAcc;
@@ -231,10 +253,6 @@ k_ion_finish(IonFinishParams P)
             P.mean_state[2 * p] = mx;
             P.mean_state[2 * p + 1] = my;
             P.n_alive[p] = (int64_t)before;
-            // slots written by the turn (old ions, then every passage's new
-            // ones), the range the compaction reads
-            const int64_t used = P.len[p] + (int64_t)P.nb * P.n_new;
-            P.len[p] = used < P.cap ? used : P.cap;
             if (overflow) atomicOr(P.status, 2u);
         }
         __syncthreads();
@@ -247,10 +265,10 @@ k_ion_finish(IonFinishParams P)
 // src/MPBeam.cpp:2203-2258).
 __global__ void __launch_bounds__(256)
 k_ion_prep(const cts_moments *cur, int nb, double beta_x, double beta_y, double *bview, double *size_max,
-           uint32_t *fb_count, unsigned long long *acc, int64_t nacc)
+           uint32_t *fb_count, const int64_t *len, int64_t *len0, int nspec)
 {
     if (threadIdx.x == 0 && fb_count) *fb_count = 0u;
-    for (int64_t i = threadIdx.x; i < nacc; i += blockDim.x) acc[i] = 0ull;
+    if ((int)threadIdx.x < nspec) len0[threadIdx.x] = len[threadIdx.x];
     __shared__ double mx[256], my[256];
     double ax = -INFINITY, ay = -INFINITY;
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
@@ -275,65 +293,72 @@ k_ion_prep(const cts_moments *cur, int nb, double beta_x, double beta_y, double 
     }
 }
 
-// Stable compaction of one species' cloud into the other buffer: chunks of
-// kIonChunk slots, 16 per thread; pass 1 counts the survivors of each chunk,
-// pass 2 places them after the survivors of all earlier chunks.
 __global__ void __launch_bounds__(256)
-k_compact_count(const uint8_t *alive, const int64_t *len_p, uint32_t *chunk_cnt)
+k_ion_reduce(unsigned long long *acc, int64_t n, unsigned long long *out)
 {
-    __shared__ uint32_t s[256];
-    const int64_t len = *len_p;
-    const int64_t base = (int64_t)blockIdx.x * kIonChunk + threadIdx.x * 16;
-    uint32_t c = 0;
-    for (int k = 0; k < 16; k++)
-        if (base + k < len) c += alive[base + k];
-    s[threadIdx.x] = c;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
-        __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    unsigned long long s = 0ull;  // modulo 2^64, as the atomics add
+    for (int r = 0; r < kIonReplicas; r++) {
+        s += acc[r * n + i];
+        acc[r * n + i] = 0ull;
     }
-    if (threadIdx.x == 0) chunk_cnt[blockIdx.x] = s[0];
+    out[i] = s;
 }
 
-__global__ void __launch_bounds__(256)
-k_compact_scatter(IonCompactParams P)
+// Stable compaction of one species' cloud into the other buffer after a
+// segment.  Pass 1 (one block): exclusive prefix sums of the survivors of
+// the segment's waves.  Pass 2: every wave moves its survivors to their
+// places, in the waves' order (old ions, then each passage's new ones), so
+// the slot order is kept.
+__global__ void __launch_bounds__(1024)
+k_compact_scan(IonCompactParams P)
 {
-    __shared__ uint32_t s[256];
-    __shared__ uint64_t part[256];
+    __shared__ uint32_t s[1024];
     const int64_t len = *P.len;
-    // survivors of the chunks before this one
-    uint64_t a = 0;
-    for (uint32_t w = threadIdx.x; w < blockIdx.x; w += 256) a += P.chunk_cnt[w];
-    part[threadIdx.x] = a;
-    const int64_t base = (int64_t)blockIdx.x * kIonChunk + threadIdx.x * 16;
+    const int w_old = (int)((len + 63) / 64);
+    const int total = w_old + P.nseg * P.w_new;
+    const int per = (total + 1023) / 1024;
+    const int a = min((int)threadIdx.x * per, total), b = min(a + per, total);
     uint32_t c = 0;
-    for (int k = 0; k < 16; k++)
-        if (base + k < len) c += P.alive[base + k];
+    for (int w = a; w < b; w++) c += P.wcnt[w];
     s[threadIdx.x] = c;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
-        __syncthreads();
-    }
-    const uint64_t off = part[0];
-    // inclusive scan of the per-thread counts (Hillis-Steele)
-    for (int o = 1; o < 256; o <<= 1) {
+    for (int o = 1; o < 1024; o <<= 1) {
         const uint32_t v = (int)threadIdx.x >= o ? s[threadIdx.x - o] : 0u;
         __syncthreads();
         s[threadIdx.x] += v;
         __syncthreads();
     }
-    uint64_t dst = off + s[threadIdx.x] - c;
-    for (int k = 0; k < 16; k++) {
-        const int64_t i = base + k;
-        if (i < len && P.alive[i]) {
-            P.ox[dst] = P.x[i]; P.oy[dst] = P.y[i]; P.ovx[dst] = P.vx[i]; P.ovy[dst] = P.vy[i];
-            P.oalive[dst] = 1;
-            dst++;
+    uint32_t run = s[threadIdx.x] - c;
+    for (int w = a; w < b; w++) {
+        const uint32_t n = P.wcnt[w];
+        P.wofs[w] = run;
+        run += n;
+    }
+    if (threadIdx.x == 1023) *P.new_len = (int64_t)s[1023];
+}
+
+__global__ void __launch_bounds__(256)
+k_compact_scatter(IonCompactParams P)
+{
+    const int64_t len = *P.len;
+    const int w_old = (int)((len + 63) / 64);
+    const int total = w_old + P.nseg * P.w_new;
+    const int lane = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int w = blockIdx.x * 4 + (threadIdx.x >> 6); w < total; w += gridDim.x * 4) {
+        int q0, t;
+        int64_t i;
+        const bool mine = ion_wave_slot(w, w_old, len, P.n_new, P.w_new, P.cap, lane, i, q0, t);
+        const bool on = mine && P.alive[i];
+        const uint64_t m = __ballot(on);
+        if (on) {
+            const uint64_t d = P.wofs[w] + (uint64_t)__popcll(m & below);
+            P.ox[d] = P.x[i]; P.oy[d] = P.y[i]; P.ovx[d] = P.vx[i]; P.ovy[d] = P.vy[i];
+            P.oalive[d] = 1;
         }
     }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 255) *P.new_len = (int64_t)(off + s[255]);
 }
 
 // this translation unit's copy of the Faddeeva tables
@@ -345,10 +370,14 @@ hipError_t upload_faddeeva_tables_ions(const FadTables &t)
     return e;
 }
 
-hipError_t launch_ion_turn(const IonTurnParams &p, int nspec, hipStream_t st)
+// wave_bound: a host bound of the segment's waves; the kernel reads the
+// actual count from the device and strides over it.
+hipError_t launch_ion_turn(const IonTurnParams &p, int nspec, int64_t wave_bound, hipStream_t st)
 {
-    const int64_t waves = (int64_t)p.w_old + (int64_t)p.nb * p.w_new;
-    const dim3 grid((unsigned)((waves + 3) / 4), (unsigned)nspec);
+    int64_t blocks = (wave_bound + 3) / 4;
+    if (blocks > kIonTurnMaxBlocks) blocks = kIonTurnMaxBlocks;
+    if (blocks < 1) blocks = 1;
+    const dim3 grid((unsigned)blocks, (unsigned)nspec);
     if (p.sur_params)
         hipLaunchKernelGGL(k_ion_turn<true>, grid, dim3(256), 0, st, p);
     else
@@ -363,20 +392,26 @@ hipError_t launch_ion_finish(const IonFinishParams &p, hipStream_t st)
 }
 
 hipError_t launch_ion_prep(const cts_moments *cur, int nb, double bx, double by, double *bview, double *size_max,
-                           uint32_t *fb_count, unsigned long long *acc, int64_t nacc, hipStream_t st)
+                           uint32_t *fb_count, const int64_t *len, int64_t *len0, int nspec, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_ion_prep, dim3(1), dim3(256), 0, st, cur, nb, bx, by, bview, size_max, fb_count, acc,
-                       nacc);
+    hipLaunchKernelGGL(k_ion_prep, dim3(1), dim3(256), 0, st, cur, nb, bx, by, bview, size_max, fb_count, len,
+                       len0, nspec);
     return hipGetLastError();
 }
 
-// len_bound: an upper bound of *p.len known on the host; the grid covers it.
-hipError_t launch_ion_compact(const IonCompactParams &p, int64_t len_bound, hipStream_t st)
+hipError_t launch_ion_reduce(unsigned long long *acc, int64_t n, unsigned long long *out, hipStream_t st)
 {
-    int64_t nch = (len_bound + kIonChunk - 1) / kIonChunk;
-    if (nch < 1) nch = 1;
-    hipLaunchKernelGGL(k_compact_count, dim3((unsigned)nch), dim3(256), 0, st, p.alive, p.len, p.chunk_cnt);
-    hipLaunchKernelGGL(k_compact_scatter, dim3((unsigned)nch), dim3(256), 0, st, p);
+    hipLaunchKernelGGL(k_ion_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, acc, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_ion_compact(const IonCompactParams &p, int64_t wave_bound, hipStream_t st)
+{
+    int64_t blocks = (wave_bound + 3) / 4;
+    if (blocks > kIonTurnMaxBlocks) blocks = kIonTurnMaxBlocks;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(1024), 0, st, p);
+    hipLaunchKernelGGL(k_compact_scatter, dim3((unsigned)blocks), dim3(256), 0, st, p);
     return hipGetLastError();
 }
 
