@@ -121,6 +121,7 @@ SIGNATURES = {
     "cts_kernel_time": (I32, [ctypes.c_void_p, ctypes.c_char_p, PD, PI64]),
     "cts_debug_qe_normals": (I32, [ctypes.c_void_p, U64, I32, U64, I64, PD]),
     "cts_debug_faddeeva": (I32, [ctypes.c_void_p, I64, PD, PD, PD, PD]),
+    "cts_debug_kick_field": (I32, [ctypes.c_void_p, I64, PD, PD]),
     "cts_sur_create": (I32, [ctypes.POINTER(ctypes.c_void_p)]),
     "cts_sur_destroy": (I32, [ctypes.c_void_p]),
     "cts_sur_last_error": (ctypes.c_char_p, []),

@@ -1742,6 +1742,21 @@ int cts_debug_faddeeva(cts_handle *h, int64_t n, const double *re, const double 
     return CTS_OK;
 }
 
+int cts_debug_kick_field(cts_handle *h, int64_t n, const double *in, double *out)
+{
+    if (!h || n <= 0 || !in || !out) return fail(CTS_ERR_ARG, "cts_debug_kick_field");
+    double *d = nullptr;
+    int rc = dalloc(&d, (size_t)n * 6);
+    if (rc) return rc;
+    hipError_t e = hipMemcpy(d, in, (size_t)n * 4 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_kick_field_debug(d, n, d + 4 * n, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, d + 4 * n, (size_t)n * 2 * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(CTS_ERR_HIP, std::string("kick field debug: ") + hipGetErrorString(e));
+    return CTS_OK;
+}
+
 int cts_debug_qe_normals(cts_handle *h, uint64_t seed, int32_t turn, uint64_t base, int64_t n, double *noise_out)
 {
     if (!h || n <= 0 || !noise_out) return fail(CTS_ERR_ARG, "cts_debug_qe_normals");

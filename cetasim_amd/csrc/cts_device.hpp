@@ -229,6 +229,29 @@ __device__ __forceinline__ double wave_sum(double v)
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+// Sum over the 64 lanes by DPP moves (no LDS): pairs within quads, quads
+// within rows (half mirror, mirror), then rows through the row_bcast15 and
+// row_bcast31 broadcasts, which leave the total in lane 63; returned to
+// every lane.  The association order differs from wave_sum's.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_sum_dpp(double v)
+{
+    v += dpp_f64<0xb1, 0xf>(v);   // quad_perm [1, 0, 3, 2]
+    v += dpp_f64<0x4e, 0xf>(v);   // quad_perm [2, 3, 0, 1]
+    v += dpp_f64<0x141, 0xf>(v);  // row_half_mirror
+    v += dpp_f64<0x140, 0xf>(v);  // row_mirror
+    v += dpp_f64<0x142, 0xa>(v);  // row_bcast15 into rows 1, 3
+    v += dpp_f64<0x143, 0xc>(v);  // row_bcast31 into rows 2, 3
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
+}
 __device__ __forceinline__ double wave_min(double v)
 {
 #pragma unroll

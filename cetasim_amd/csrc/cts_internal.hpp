@@ -255,6 +255,7 @@ hipError_t launch_qe_debug(const float2 *tab, uint64_t seed, uint32_t turn, uint
                            hipStream_t st);
 hipError_t upload_faddeeva_tables(const FadTables &t);
 hipError_t launch_w_debug(const double *re, const double *im, int64_t n, double *wr, double *wi, hipStream_t st);
+hipError_t launch_kick_field_debug(const double *in, int64_t n, double *out, hipStream_t st);
 hipError_t upload_faddeeva_tables_ions(const FadTables &t);
 void sur_export(const cts_surrogate *s, const float **params, SurNorm *norm);
 

@@ -106,6 +106,7 @@ k_ion_turn(IonTurnParams P)
 {
     __shared__ __attribute__((aligned(16))) MlpLds L;
     __shared__ double sbv[kIonSegment][4], scI[kIonSegment], sdt[kIonSegment];
+    __shared__ BeField sF[kIonSegment];
     __shared__ unsigned long long sacc[kIonSegment * kIonAcc * 3];
     const int p = blockIdx.y;
     const int64_t len = P.len[p];
@@ -120,6 +121,7 @@ k_ion_turn(IonTurnParams P)
     for (int i = threadIdx.x; i < nseg; i += 256) {
         scI[i] = P.coeffI[(P.j_begin + i) * CTS_MAX_SPECIES + p];
         sdt[i] = P.dt[P.j_begin + i];
+        sF[i] = be_plan(P.bview[4 * (P.j_begin + i) + 2], P.bview[4 * (P.j_begin + i) + 3]);
     }
     for (int i = threadIdx.x; i < nseg * kIonAcc * 3; i += 256) sacc[i] = 0ull;
     __syncthreads();
@@ -149,14 +151,14 @@ k_ion_turn(IonTurnParams P)
         bool alive = mine;
         for (int q = q0; q < nseg; q++) {
             if (!__any(alive)) break;
-            const double xm = sbv[q][0], ym = sbv[q][1], sx = sbv[q][2], sy = sbv[q][3];
+            const double xm = sbv[q][0], ym = sbv[q][1];
             const double dx = x - xm, dy = y - ym;
             double fx = 0.0, fy = 0.0;
             if (SUR) {
-                const bool ok = sur_field(L, P.sur_norm, alive, dx, dy, sx, sy, fx, fy);
-                if (alive && !ok) ion_field(dx, dy, sx, sy, fx, fy);
+                const bool ok = sur_field(L, P.sur_norm, alive, dx, dy, sbv[q][2], sbv[q][3], fx, fy);
+                if (alive && !ok) be_field(dx, dy, sF[q], fx, fy);
             } else if (alive) {
-                ion_field(dx, dy, sx, sy, fx, fy);
+                be_field(dx, dy, sF[q], fx, fy);
             }
             bool keep = false;
             if (alive) {
@@ -167,16 +169,15 @@ k_ion_turn(IonTurnParams P)
                 y += vy * sdt[q];
                 keep = !(fabs(x) > bx || fabs(y) > by);
             }
-            // the cloud as the bunch finds it (before the kick), and the ions kept
+            // the cloud as the bunch finds it (before the kick), and the ions
+            // kept: counts by ballot, sums by DPP reductions
             double c[kIonAcc];
-            c[0] = alive ? 1.0 : 0.0;
-            c[1] = alive ? dx : 0.0;
-            c[2] = alive ? dy : 0.0;
-            c[3] = alive ? dx * dx : 0.0;
-            c[4] = alive ? dy * dy : 0.0;
-            c[5] = keep ? 1.0 : 0.0;
-#pragma unroll
-            for (int v = 0; v < kIonAcc; v++) c[v] = wave_sum(c[v]);
+            c[0] = (double)__popcll(__ballot(alive));
+            c[1] = wave_sum_dpp(alive ? dx : 0.0);
+            c[2] = wave_sum_dpp(alive ? dy : 0.0);
+            c[3] = wave_sum_dpp(alive ? dx * dx : 0.0);
+            c[4] = wave_sum_dpp(alive ? dy * dy : 0.0);
+            c[5] = (double)__popcll(__ballot(keep));
             // lane 3 v + k adds limb k of value v
             if (lane < 3 * kIonAcc) {
                 const int v = lane / 3, kk = lane - 3 * v;

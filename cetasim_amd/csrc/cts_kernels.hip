@@ -689,24 +689,29 @@ __global__ void __launch_bounds__(64) k_post(FinParams F, WakeParams W)
 __global__ void __launch_bounds__(kBlock)
 k_ion_kick(IonKickParams P)
 {
+    __shared__ BeField sF[CTS_MAX_SPECIES];
     const int tid = threadIdx.x;
     const int s_first = P.wg_seg[blockIdx.x], s_last = P.wg_seg[blockIdx.x + 1];
     for (int s = s_first; s < s_last; s++) {
         const int b = P.seg_bunch[s];
         const double *c = P.ion_e + (int64_t)b * CTS_MAX_SPECIES * 5;
+        // the clouds' constants, once per segment
+        if (tid < P.nspec) sF[tid] = be_plan(c[5 * tid + 3], c[5 * tid + 4]);
+        __syncthreads();
         for (int64_t i = P.seg_begin[s] + tid; i < P.seg_end[s]; i += kBlock) {
             if (P.flag[i] & 3u) continue;
             const double x = P.x[i], y = P.y[i];
             double fxs = 0.0, fys = 0.0;
             for (int p = 0; p < P.nspec; p++) {
                 double fx, fy;
-                ion_field(x - c[5 * p + 1], y - c[5 * p + 2], c[5 * p + 3], c[5 * p + 4], fx, fy);
+                be_field(x - c[5 * p + 1], y - c[5 * p + 2], sF[p], fx, fy);
                 fxs = fxs + c[5 * p] * fx;
                 fys = fys + c[5 * p] * fy;
             }
             P.px[i] += fxs;
             P.py[i] += fys;
         }
+        __syncthreads();
     }
 }
 
@@ -820,6 +825,24 @@ k_w_debug(const double *re, const double *im, int64_t n, double *wr, double *wi)
     faddeeva_w(re[i], im[i], a, b);
     wr[i] = a;
     wi[i] = b;
+}
+// the kick field (be_plan + be_field) at n points: in = {x, y, sx, sy}[n], out = {fx, fy}[n]
+__global__ void __launch_bounds__(kBlock)
+k_kick_field_debug(const double *in, int64_t n, double *out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const BeField F = be_plan(in[2 * n + i], in[3 * n + i]);
+    double fx, fy;
+    be_field(in[i], in[n + i], F, fx, fy);
+    out[i] = fx;
+    out[n + i] = fy;
+}
+hipError_t launch_kick_field_debug(const double *in, int64_t n, double *out, hipStream_t st)
+{
+    const int blocks = (int)((n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_kick_field_debug, dim3(blocks), dim3(kBlock), 0, st, in, n, out);
+    return hipGetLastError();
 }
 hipError_t launch_w_debug(const double *re, const double *im, int64_t n, double *wr, double *wi, hipStream_t st)
 {

@@ -250,6 +250,10 @@ int cts_kernel_time(cts_handle *h, const char *name, double *avg_ms, int64_t *la
 /* Test hook: the device's w(z) (the Faddeeva function behind every
  * Bassetti-Erskine kick) at n points. */
 int cts_debug_faddeeva(cts_handle *h, int64_t n, const double *re, const double *im, double *wr, double *wi);
+/* Test hook: the field the beam-ion kicks apply (MPBunch::SSIonBunchInteraction's
+ * branches, src/MPBunch.cpp:1128-1147, per unit line charge) at n points:
+ * in = {x[n], y[n], sx[n], sy[n]}, out = {fx[n], fy[n]}. */
+int cts_debug_kick_field(cts_handle *h, int64_t n, const double *in, double *out);
 /* Test hook: the device quantum-excitation stream for particles
  * [base, base+n) at `turn` (noise_out[5][n]). */
 int cts_debug_qe_normals(cts_handle *h, uint64_t seed, int32_t turn, uint64_t base,

@@ -1,6 +1,7 @@
 """The device's w(z) (behind every Bassetti-Erskine kick) against the
 reference's known answers and the oracle's restatement of the reference's
-package.
+package, and the field the beam-ion kicks apply against the oracle's
+Bassetti-Erskine field.
 
 Near the beam (|Re z| < 10, outside the continued-fraction region, upper
 half-plane) the device uses Weideman's rational approximation with 40
@@ -101,3 +102,46 @@ def test_against_oracle_grid(dev):
     print(f"max |w| relative error: rational region {np.max(rel[rat]):.3g}, elsewhere {np.max(rel[~rat]):.3g}")
     assert np.max(rel[rat]) <= 5e-14, z[rat][np.argmax(rel[rat])]
     assert np.max(rel[~rat]) <= 1e-13, z[~rat][np.argmax(rel[~rat])]
+
+
+def test_kick_field_against_oracle(dev):
+    """The kicks' field (per-beam constants, Weideman's approximation for
+    both w(z) on Re z < 100, Im z < 1000) against the oracle's
+    MPBunch::SSIonBunchInteraction branches (src/MPBunch.cpp:1128-1147):
+    flat, inverted, near-round and round beams; points from the centre to far
+    outside (the package's continued-fraction and large-x regions), on the
+    axes and at the zero-field cut.  Tolerance: 1e-12 |f| + 2e-13 / max(sx, sy)
+    (w's |w| relative error of 3e-14 times the Bassetti-Erskine prefactor)."""
+    import ctypes
+    rng = np.random.default_rng(17)
+    beams = [(1e-4, 1e-5), (2e-5, 4e-6), (1e-5, 1e-4), (1e-5, 9e-6), (2e-4, 4e-6), (1e-5, 1.00005e-5),
+             (3e-5, 3e-5)]
+    pts = []
+    for sx, sy in beams:
+        u = np.concatenate([rng.uniform(-12, 12, 3000), np.sign(rng.uniform(-1, 1, 1000)) * 10 ** rng.uniform(-7, 0, 1000),
+                            rng.uniform(-150, 150, 500), np.zeros(100), rng.uniform(-5, 5, 100)])
+        v = np.concatenate([rng.uniform(-12, 12, 3000), np.sign(rng.uniform(-1, 1, 1000)) * 10 ** rng.uniform(-7, 0, 1000),
+                            rng.uniform(-150, 150, 500), rng.uniform(-5, 5, 100), np.zeros(100)])
+        v[:20] = 0.0
+        u[20:25] = 3e-6  # |x/sx| + |y/sy| below the zero-field cut
+        v[20:25] = 1e-6
+        pts.append(np.stack([u * sx, v * sy, np.full(len(u), sx), np.full(len(u), sy)]))
+    P = np.concatenate(pts, axis=1)
+    n = P.shape[1]
+    inp = np.ascontiguousarray(P.reshape(-1))
+    out = np.empty(2 * n)
+    capi.check(dev.lib.cts_debug_kick_field(dev.h, n, capi.dptr(inp), capi.dptr(out)))
+    L = O.lib()
+    a, b = ctypes.c_double(), ctypes.c_double()
+    exp = np.empty((2, n))
+    for i in range(n):
+        L.orc_ion_field(P[0, i], P[1, i], P[2, i], P[3, i], a, b)
+        exp[0, i], exp[1, i] = a.value, b.value
+    got = out.reshape(2, n)
+    scale = 1.0 / np.maximum(P[2], P[3])
+    err = np.abs(got - exp)
+    tol = 1e-12 * np.abs(exp) + 2e-13 * scale
+    worst = np.max(err / (np.abs(exp) + scale))
+    print(f"kick field: max |df| / (|f| + 1/max(sx, sy)) = {worst:.3g}")
+    bad = np.argwhere(err > tol)
+    assert len(bad) == 0, (bad[:5], P[:, bad[0][1]], got[:, bad[0][1]], exp[:, bad[0][1]])
