@@ -686,7 +686,10 @@ __global__ void __launch_bounds__(64) k_post(FinParams F, WakeParams W)
 // Electron side of MPBunch::SSIonBunchInteraction (src/MPBunch.cpp:1165-1206)
 // followed by BunchTransferDueToIon (src/Bunch.cpp:213-220): survivors get
 // px/py += sum over species of 2*N_ion*r_e/gamma * F(x - xbar, y - ybar).
-__global__ void __launch_bounds__(kBlock)
+// Four waves per SIMD (128 VGPRs): the field is compute-bound and the
+// extra waves hide its latency better than the registers they cost
+// (a few spills, in the rarely taken far-field branch).
+__global__ void __launch_bounds__(kBlock, 4)
 k_ion_kick(IonKickParams P)
 {
     __shared__ BeField sF[CTS_MAX_SPECIES];
