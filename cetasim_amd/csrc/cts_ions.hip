@@ -205,58 +205,72 @@ k_ion_turn(IonTurnParams P)
     }
 }
 
-// The passage sums converted in parallel, then passage by passage in order
-// (one thread per species): cloud moments with
-// the reference's one-ion and empty-cloud cases
-// (src/LatticeInterActionPoint.cpp:1113-1155), the ion number
-// (src/MPBunch.cpp:1172), the electron-side kick parameters, and the
-// capacity test of IonsUpdate (src/LatticeInterActionPoint.cpp:1041-1049).
+// Cloud moments and electron-side parameters of every passage, one block
+// per species, a thread per passage: the cloud moments with the
+// reference's one-ion and empty-cloud cases
+// (src/LatticeInterActionPoint.cpp:1113-1155; with fewer than two ions the
+// mean keeps the last passage's value, from the latest passage with two or
+// more, or from the previous turn), the ion number (src/MPBunch.cpp:1172),
+// the electron-side kick parameters, and the capacity test of IonsUpdate
+// (src/LatticeInterActionPoint.cpp:1041-1049) against the ions present
+// before each passage.
+__device__ __forceinline__ double ion_sum(const unsigned long long *acc, int j, int v)
+{
+    return fx_value(acc + ((int64_t)j * kIonAcc + v) * 3);
+}
+
 __global__ void __launch_bounds__(256)
 k_ion_finish(IonFinishParams P)
 {
-    extern __shared__ double sums[];  // [nb][kIonAcc]
-    for (int p = 0; p < P.nspec; p++) {
-        const unsigned long long *acc = P.acc + (int64_t)p * P.nb * kIonAcc * 3;
-        for (int i = threadIdx.x; i < P.nb * kIonAcc; i += blockDim.x) sums[i] = fx_value(acc + (int64_t)i * 3);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double mx = P.mean_state[2 * p], my = P.mean_state[2 * p + 1];
-            double before = (double)P.len0[p];
-            bool overflow = false;
-            for (int j = 0; j < P.nb; j++) {
-                const double *a = sums + j * kIonAcc;
-                const double n = a[0], s1 = a[1], s2 = a[2], q1 = a[3], q2 = a[4];
-                const double xm = P.bview[4 * j], ym = P.bview[4 * j + 1];
-                double rx = 0.0, ry = 0.0;
-                if (n >= 2) {
-                    const double ax = s1 / n, ay = s2 / n;
-                    mx = xm + ax;
-                    my = ym + ay;
-                    rx = sqrt(fmax(q1 / n - ax * ax, 0.0));
-                    ry = sqrt(fmax(q2 / n - ay * ay, 0.0));
-                } else if (n == 1) {
-                    // one ion: uncentred rms, the mean keeps its last value
-                    rx = fabs(xm + s1);
-                    ry = fabs(ym + s2);
-                }
-                const double nI0 = P.ion_q[j * P.nspec + p] * n;
-                double *e = P.ion_e + ((int64_t)j * CTS_MAX_SPECIES + p) * 5;
-                e[0] = 2.0 * nI0 * kRe / P.gamma;
-                e[1] = mx;
-                e[2] = my;
-                e[3] = rx;
-                e[4] = ry;
-                double *c = P.cloud + ((int64_t)j * CTS_MAX_SPECIES + p) * 6;
-                c[0] = n; c[1] = mx; c[2] = my; c[3] = rx; c[4] = ry; c[5] = nI0;
-                if (before + P.n_new > (double)P.max_ions) overflow = true;
-                before = a[5];
+    __shared__ double last[2];
+    const int p = blockIdx.x;
+    const unsigned long long *acc = P.acc + (int64_t)p * P.nb * kIonAcc * 3;
+    const double mx0 = P.mean_state[2 * p], my0 = P.mean_state[2 * p + 1];
+    for (int j = threadIdx.x; j < P.nb; j += blockDim.x) {
+        const double n = ion_sum(acc, j, 0), s1 = ion_sum(acc, j, 1), s2 = ion_sum(acc, j, 2);
+        const double q1 = ion_sum(acc, j, 3), q2 = ion_sum(acc, j, 4);
+        const double xm = P.bview[4 * j], ym = P.bview[4 * j + 1];
+        double mx = mx0, my = my0;
+        for (int k = j; k >= 0; k--) {
+            const double nk = k == j ? n : ion_sum(acc, k, 0);
+            if (nk >= 2) {
+                const double sx = k == j ? s1 : ion_sum(acc, k, 1), sy = k == j ? s2 : ion_sum(acc, k, 2);
+                mx = P.bview[4 * k] + sx / nk;
+                my = P.bview[4 * k + 1] + sy / nk;
+                break;
             }
-            P.mean_state[2 * p] = mx;
-            P.mean_state[2 * p + 1] = my;
-            P.n_alive[p] = (int64_t)before;
-            if (overflow) atomicOr(P.status, 2u);
         }
-        __syncthreads();
+        double rx = 0.0, ry = 0.0;
+        if (n >= 2) {
+            const double ax = s1 / n, ay = s2 / n;
+            rx = sqrt(fmax(q1 / n - ax * ax, 0.0));
+            ry = sqrt(fmax(q2 / n - ay * ay, 0.0));
+        } else if (n == 1) {
+            // one ion: uncentred rms
+            rx = fabs(xm + s1);
+            ry = fabs(ym + s2);
+        }
+        const double nI0 = P.ion_q[j * P.nspec + p] * n;
+        double *e = P.ion_e + ((int64_t)j * CTS_MAX_SPECIES + p) * 5;
+        e[0] = 2.0 * nI0 * kRe / P.gamma;
+        e[1] = mx;
+        e[2] = my;
+        e[3] = rx;
+        e[4] = ry;
+        double *c = P.cloud + ((int64_t)j * CTS_MAX_SPECIES + p) * 6;
+        c[0] = n; c[1] = mx; c[2] = my; c[3] = rx; c[4] = ry; c[5] = nI0;
+        const double before = j == 0 ? (double)P.len0[p] : ion_sum(acc, j - 1, 5);
+        if (before + P.n_new > (double)P.max_ions) atomicOr(P.status, 2u);
+        if (j == P.nb - 1) {
+            last[0] = mx;
+            last[1] = my;
+            P.n_alive[p] = (int64_t)ion_sum(acc, j, 5);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && P.nb > 0) {
+        P.mean_state[2 * p] = last[0];
+        P.mean_state[2 * p + 1] = last[1];
     }
 }
 
@@ -388,7 +402,7 @@ hipError_t launch_ion_turn(const IonTurnParams &p, int nspec, int64_t wave_bound
 
 hipError_t launch_ion_finish(const IonFinishParams &p, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_ion_finish, dim3(1), dim3(256), (size_t)p.nb * kIonAcc * sizeof(double), st, p);
+    hipLaunchKernelGGL(k_ion_finish, dim3(p.nspec), dim3(256), 0, st, p);
     return hipGetLastError();
 }
 
