@@ -22,8 +22,12 @@ PI64 = ctypes.POINTER(ctypes.c_int64)
 PF = ctypes.POINTER(ctypes.c_float)
 
 CTS_OK = 0
+CTS_ERR_ARG = -1
+CTS_ERR_HIP = -2
+CTS_ERR_STATE = -3
 CTS_LOW_TRANSMISSION = 1
 CTS_ION_OVERFLOW = 2
+CTS_LRWAKE_TAU = 3
 MAX_RF = 4
 MAX_RES = 8
 MAX_SPECIES = 4
@@ -73,6 +77,11 @@ class IonConfig(ctypes.Structure):
                 ("length", D), ("loss_boundary", D), ("ap_x", D), ("ap_y", D), ("seed", U64)]
 
 
+class LrwakeConfig(ctypes.Structure):
+    _fields_ = [("nturns", I32), ("n_sectors", I32), ("sectors", PD), ("n_res", I32), ("res", PD),
+                ("beta_x", D), ("beta_y", D)]
+
+
 class CbmResult(ctypes.Structure):
     _fields_ = [(n, PD) for n in ("ideal_amp", "ideal_arg", "iq_amp", "iq_arg", "ana_re", "ana_im", "hil_amp",
                                   "hil_arg", "rate_ideal", "rate_iq", "rate_hil", "rate_bunch")]
@@ -95,6 +104,9 @@ SIGNATURES = {
     "cts_set_synrad": (I32, [ctypes.c_void_p, I32, I32, U64]),
     "cts_push_noise": (I32, [ctypes.c_void_p, PD]),
     "cts_set_ions": (I32, [ctypes.c_void_p, ctypes.POINTER(IonConfig)]),
+    "cts_set_lrwake": (I32, [ctypes.c_void_p, ctypes.POINTER(LrwakeConfig)]),
+    "cts_get_lrwake_table": (I32, [ctypes.c_void_p, PI32, PD, PD, PD, PD]),
+    "cts_get_lrwake_kick": (I32, [ctypes.c_void_p, PD]),
     "cts_ion_phase": (I32, [ctypes.c_void_p, I32]),
     "cts_get_ion_cloud": (I32, [ctypes.c_void_p, I32, I64, PI64, PD, PD, PD, PD]),
     "cts_get_ion_passages": (I32, [ctypes.c_void_p, PD]),

@@ -152,6 +152,18 @@ struct cts_handle {
     uint32_t *d_fb_count = nullptr;  // exact-field fallback queue of the electron-side kick
     uint64_t *d_fb_list = nullptr;
 
+    // long-range wake (cts_set_lrwake); its kick is deferred to the next
+    // pass like the short-range one
+    int lrw_T = 0;                      // nturns, 0: off
+    LrwTable lrw;                       // host copy of the table
+    double *d_lrw_table = nullptr;      // [3][np]
+    double *d_lrw_ring = nullptr;       // [T][nb][3] post-map centroids by turn mod T
+    double *d_lrw_cen = nullptr;        // [nb][3] this turn's
+    double *d_lrw_kick = nullptr;       // [nb][3]
+    double *d_lrw_ne = nullptr;         // [nb]
+    int32_t *d_lrw_harm = nullptr;      // [nb]
+    bool lr_pending = false;
+
     // graph replays of two consecutive turns (cts_run_turns): the turn is
     // read from d_turn, which the graph advances by 2; one instance per
     // (wake buffer parity, ion buffer) at capture
@@ -177,8 +189,21 @@ struct cts_handle {
 
 namespace {
 
+void free_lrw_state(cts_handle *h)
+{
+    void *ptrs[] = {h->d_lrw_table, h->d_lrw_ring, h->d_lrw_cen, h->d_lrw_kick, h->d_lrw_ne, h->d_lrw_harm};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    h->d_lrw_table = h->d_lrw_ring = h->d_lrw_cen = h->d_lrw_kick = h->d_lrw_ne = nullptr;
+    h->d_lrw_harm = nullptr;
+    h->lrw_T = 0;
+    h->lrw = LrwTable{};
+    h->lr_pending = false;
+}
+
 void free_bunch_state(cts_handle *h)
 {
+    free_lrw_state(h);
     void *ptrs[] = {h->d_x, h->d_px, h->d_y, h->d_py, h->d_z, h->d_dp, h->d_flag,
                     h->d_seg_begin, h->d_seg_end, h->d_bunch_off, h->d_bunch_count,
                     h->d_seg_bunch, h->d_wg_seg, h->d_bunch_seg, h->d_seg_part, h->d_shift,
@@ -379,6 +404,7 @@ PassParams base_pass(cts_handle *h, uint32_t ops, int k)
     p.noise_mode = h->noise_mode;
     p.seed = h->seed;
     p.ntab = h->d_ntab;
+    p.lr_kick = h->d_lrw_kick;
     if (h->capturing) {
         p.turn_dev = h->d_turn;
         p.turn = (uint32_t)h->cap_off;
@@ -410,6 +436,7 @@ FinParams base_fin(cts_handle *h, uint32_t ops, int slot, int prev_slot)
     f.chunk_part = h->d_chunk_part;
     f.chunk_cnt = h->d_chunk_cnt;
     f.max_chunks = h->max_chunks;
+    f.lr_cen = h->d_lrw_cen;
     if (h->capturing) {
         f.turn_dev = h->d_turn;
         f.turn_off = h->cap_off;
@@ -492,14 +519,20 @@ int run_bin(cts_handle *h)
 // into the current record (and its history slot).
 int flush_kick(cts_handle *h)
 {
-    if (!h->kick_pending) return CTS_OK;
-    PassParams p = base_pass(h, OP_KICK | OP_MOMB | OP_WRITE, 0);
+    if (!h->kick_pending && !h->lr_pending) return CTS_OK;
+    uint32_t ops = OP_WRITE;
+    if (h->kick_pending) ops |= OP_KICK | OP_MOMB;
+    if (h->lr_pending) ops |= OP_LRK;
+    PassParams p = base_pass(h, ops, 0);
     int rc = run_pass(h, p, "kick");
     if (rc) return rc;
-    FinParams f = base_fin(h, OP_MOMB, -1, slot_of(h, h->last_turn));
-    rc = run_finalize(h, f);
-    if (rc) return rc;
+    if (h->kick_pending) {
+        FinParams f = base_fin(h, OP_MOMB, -1, slot_of(h, h->last_turn));
+        rc = run_finalize(h, f);
+        if (rc) return rc;
+    }
     h->kick_pending = false;
+    h->lr_pending = false;
     h->dp_pending = false;
     return CTS_OK;
 }
@@ -540,6 +573,9 @@ int check_status(cts_handle *h, bool block)
     if (*h->h_status & 2u)
         return fail(CTS_ION_OVERFLOW, "the ion cloud would exceed its maximum macro-ion number "
                                       "(src/LatticeInterActionPoint.cpp:1041-1049)");
+    if (*h->h_status & 4u)
+        return fail(CTS_LRWAKE_TAU, "long-range wake: a source bunch lies ahead of its receiver in time "
+                                    "(tau > 0, src/WakeFunction.cpp:206-210)");
     return CTS_OK;
 }
 
@@ -1102,6 +1138,42 @@ int cts_push_noise(cts_handle *h, const double *noise)
     return CTS_OK;
 }
 
+// MPBeam::LRWakeBeamIntaction (src/MPBeam.cpp:3232-3351) after the turn's
+// moments: the kick of every bunch, applied by the next pass (or a flush).
+static int run_lrwake(cts_handle *h, int32_t turn)
+{
+    LrwParams L;
+    std::memset(&L, 0, sizeof L);
+    L.table = h->d_lrw_table;
+    L.np = h->lrw.np;
+    L.nturns = h->lrw_T;
+    L.nb = h->nb;
+    L.harmonic = h->ring.harmonic;
+    L.dt = h->lrw.dt;
+    L.trf = h->lrw.trf;
+    L.rbeta = h->ring.rbeta;
+    L.scale = kEcharge / h->ring.energy / std::pow(h->ring.rbeta, 2);
+    L.harm = h->d_lrw_harm;
+    L.ne = h->d_lrw_ne;
+    L.cen = h->d_lrw_cen;
+    L.ring = h->d_lrw_ring;
+    L.kick = h->d_lrw_kick;
+    L.cur = h->d_cur;
+    L.hist = h->d_hist;
+    L.hist_cap = h->hist_cap;
+    L.fix_dp = h->nbins > 0 ? 0 : 1;  // with a short-range wake the dp moments close in the next pass
+    L.turn = (uint32_t)turn;
+    if (h->capturing) {
+        L.turn_dev = h->d_turn;
+        L.turn = (uint32_t)h->cap_off;
+    }
+    L.status = h->d_status;
+    TimedScope ts(h, "lrwake");
+    HIPCHK(launch_lrwake(L, h->stream));
+    h->lr_pending = true;
+    return CTS_OK;
+}
+
 // MPBeam::Run body for one section (src/MPBeam.cpp:1059-1102), fused.
 // The kernels of one turn (everything cts_turn enqueues after its status
 // check); also what a graph capture records, twice.
@@ -1118,11 +1190,13 @@ static int turn_body(cts_handle *h, int32_t turn)
     if (h->kick_pending) ops |= OP_KICK | OP_MOMB;
     if (wake) ops |= (h->wake_mode == 1) ? OP_BIN : OP_ZSTAT;
     else ops |= OP_MOMD;  // no deferred kick: dp moments close this turn
+    if (h->lr_pending) ops |= OP_LRK;
+    if (h->lrw_T > 0) ops |= OP_LRC;
     PassParams p = base_pass(h, ops, 0);
     if (!h->capturing) p.turn = (uint32_t)turn;
     if ((rc = run_pass(h, p))) return rc;
     // finalize: dp part of the previous record (MOMB), this turn's record
-    FinParams f = base_fin(h, ops & (OP_MOMB | OP_MOMA | OP_MOMD | OP_LOSS | OP_ZSTAT), slot_of(h, turn),
+    FinParams f = base_fin(h, ops & (OP_MOMB | OP_MOMA | OP_MOMD | OP_LOSS | OP_ZSTAT | OP_LRC), slot_of(h, turn),
                            slot_of(h, h->last_turn));
     if (wake && h->wake_mode == 1) {
         // fixed grid: moments and wake in one launch
@@ -1138,6 +1212,7 @@ static int turn_body(cts_handle *h, int32_t turn)
             h->kick_pending = true;
         }
     }
+    if (h->lrw_T > 0 && (rc = run_lrwake(h, turn))) return rc;
     h->last_turn = turn;
     h->dp_pending = wake;
     h->cur_valid = true;
@@ -1163,6 +1238,7 @@ static int capture_pair(cts_handle *h, hipGraphExec_t *out)
 {
     const int parity = h->hist_parity, ion_buf = h->ion_buf, last_turn = h->last_turn;
     const bool kick_pending = h->kick_pending, dp_pending = h->dp_pending, cur_valid = h->cur_valid;
+    const bool lr_pending = h->lr_pending;
     const bool timing = h->timing;
     const int64_t ion_len_ub = h->ion_len_ub;
     h->timing = false;
@@ -1194,6 +1270,7 @@ static int capture_pair(cts_handle *h, hipGraphExec_t *out)
     h->kick_pending = kick_pending;
     h->dp_pending = dp_pending;
     h->cur_valid = cur_valid;
+    h->lr_pending = lr_pending;
     h->ion_len_ub = ion_len_ub;
     if (rc == CTS_OK) {
         e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
@@ -1225,7 +1302,8 @@ int cts_run_turns(cts_handle *h, int32_t first, int32_t nturns, int32_t use_grap
         // turn was t - 1 and, with a wake, its kick is pending; host noise
         // (pushed per turn) is not replayable
         const bool steady = use_graph && end - t >= 2 && h->cur_valid && h->last_turn == t - 1 &&
-                            h->kick_pending == (h->nbins > 0) && h->noise_mode != 1;
+                            h->kick_pending == (h->nbins > 0) && h->lr_pending == (h->lrw_T > 0) &&
+                            h->noise_mode != 1;
         if (!steady) {
             if ((rc = cts_turn(h, t))) return rc;
             t++;
@@ -1275,6 +1353,7 @@ int cts_run_turns(cts_handle *h, int32_t first, int32_t nturns, int32_t use_grap
         // the host state after two turns: parities back where they were
         h->last_turn = t + 1;
         h->dp_pending = h->nbins > 0;
+        h->lr_pending = h->lrw_T > 0;
         h->cur_valid = true;
         if (h->nspec > 0) h->ion_len_ub = h->ion_cap;
         if ((rc = enqueue_status_copy(h))) return rc;
@@ -1449,6 +1528,80 @@ int cts_mark_lost(cts_handle *h, double *transmission)
         for (int b = 0; b < h->nb; b++) transmission[b] = m[b].transmission;
     }
     return check_status(h, true);
+}
+
+int cts_set_lrwake(cts_handle *h, const cts_lrwake_config *cfg)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!cfg || cfg->nturns < 0) return fail(CTS_ERR_ARG, "cts_set_lrwake: nturns >= 0");
+    if ((rc = flush_kick(h))) return rc;  // a pending kick belongs to the old settings
+    HIPCHK(hipStreamSynchronize(h->stream));
+    graph_reset(h);
+    free_lrw_state(h);
+    if (cfg->nturns == 0) return CTS_OK;
+    if (h->secs.empty()) return fail(CTS_ERR_STATE, "cts_set_lrwake: no sections");
+    if (cfg->n_sectors < 0 || cfg->n_res < 0 || (cfg->n_sectors > 0 && !cfg->sectors) ||
+        (cfg->n_res > 0 && !cfg->res) || cfg->n_sectors + cfg->n_res == 0)
+        return fail(CTS_ERR_ARG, "cts_set_lrwake: pipe sectors and/or resonators required");
+    if (!(cfg->beta_x > 0) || !(cfg->beta_y > 0)) return fail(CTS_ERR_ARG, "cts_set_lrwake: beta functions > 0");
+    for (int i = 0; i < cfg->n_sectors; i++) {
+        const double *s = cfg->sectors + 7 * i;
+        if (!(s[0] > 0) || !(s[1] > 0) || !(s[6] > 0))
+            return fail(CTS_ERR_ARG, "cts_set_lrwake: sector radii and conductivity must be > 0");
+    }
+    for (int r = 0; r < cfg->n_res; r++) {
+        const double *q = cfg->res + 9 * r;
+        if (!(q[1] > 0.5) || !(q[4] > 0.5) || !(q[7] > 0.5))
+            return fail(CTS_ERR_ARG, "cts_set_lrwake: resonator Q must exceed 1/2");
+    }
+    std::string err;
+    LrwTable tab;
+    if ((rc = lrwake_table(*cfg, h->ring.t0, h->ring.harmonic, tab, err))) return fail(rc, err);
+    const int nb = h->nb;
+    if ((rc = dalloc(&h->d_lrw_table, (size_t)3 * tab.np)) ||
+        (rc = dalloc(&h->d_lrw_ring, (size_t)cfg->nturns * nb * 3)) || (rc = dalloc(&h->d_lrw_cen, (size_t)nb * 3)) ||
+        (rc = dalloc(&h->d_lrw_kick, (size_t)nb * 3)) || (rc = dalloc(&h->d_lrw_ne, (size_t)nb)) ||
+        (rc = dalloc(&h->d_lrw_harm, (size_t)nb))) {
+        free_lrw_state(h);
+        return rc;
+    }
+    // bunch harmonic numbers from the gaps (only differences enter)
+    std::vector<int32_t> harm(nb, 0);
+    for (int b = 1; b < nb; b++) harm[b] = harm[b - 1] + h->gap[b - 1];
+    HIPCHK(hipMemcpy(h->d_lrw_table, tab.w.data(), tab.w.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_lrw_ne, h->n_e.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_lrw_harm, harm.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice));
+    h->lrw = std::move(tab);
+    h->lrw_T = cfg->nturns;
+    h->lr_pending = false;
+    return CTS_OK;
+}
+
+int cts_get_lrwake_table(cts_handle *h, int32_t *np, double *time, double *w, double *rw, double *bbr)
+{
+    if (!h || !np) return fail(CTS_ERR_ARG, "null argument");
+    if (h->lrw_T == 0) return fail(CTS_ERR_STATE, "cts_get_lrwake_table: no long-range wake");
+    const int n = h->lrw.np;
+    if (*np < n) {
+        *np = n;
+        return fail(CTS_ERR_ARG, "cts_get_lrwake_table: capacity below np (np returned)");
+    }
+    *np = n;
+    if (time) std::memcpy(time, h->lrw.time.data(), (size_t)n * sizeof(double));
+    if (w) std::memcpy(w, h->lrw.w.data(), (size_t)3 * n * sizeof(double));
+    if (rw) std::memcpy(rw, h->lrw.rw.data(), (size_t)3 * n * sizeof(double));
+    if (bbr) std::memcpy(bbr, h->lrw.bbr.data(), (size_t)3 * n * sizeof(double));
+    return CTS_OK;
+}
+
+int cts_get_lrwake_kick(cts_handle *h, double *kick)
+{
+    if (!h || !kick) return fail(CTS_ERR_ARG, "null argument");
+    if (h->lrw_T == 0) return fail(CTS_ERR_STATE, "cts_get_lrwake_kick: no long-range wake");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(kick, h->d_lrw_kick, (size_t)h->nb * 3 * sizeof(double), hipMemcpyDeviceToHost));
+    return CTS_OK;
 }
 
 int cts_set_ions(cts_handle *h, const cts_ion_config *cfg)

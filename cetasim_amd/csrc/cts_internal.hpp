@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+#include <vector>
+
 #include "../../include/cetasim_hip.h"
 #include "cts_device.hpp"
 #include "cts_faddeeva.hpp"
@@ -29,6 +32,8 @@ enum : uint32_t {
     OP_MOMA = 1u << 12,
     OP_WRITE = 1u << 13,
     OP_MOMD = 1u << 14,  // z/dp moments at the end of the pass (no wake)
+    OP_LRC = 1u << 15,   // post-map centroids of the long-range wake
+    OP_LRK = 1u << 16,   // the previous turn's long-range wake kick (deferred)
 };
 
 // per-segment partial sums, stored value-major: seg_part[j * nseg + s]
@@ -36,7 +41,8 @@ enum {
     A_N = 0, A_S = 1, A_Q = 6, A_LOSS = 13, A_ZMIN = 14, A_ZMAX = 15,
     B_N = 16, B_SZ = 17, B_SD = 18, B_QDD = 19, B_QZD = 20,
     C_N = 21, C_SZ = 22, C_QZZ = 23, C_ZMIN = 24, C_ZMAX = 25,
-    kNPart = 26
+    L_N = 26, L_SX = 27, L_SY = 28, L_SZ = 29,
+    kNPart = 30
 };
 
 // Layout of the per-section constant block in device memory (doubles).
@@ -83,6 +89,7 @@ struct PassParams {
     const uint32_t *turn_dev;
     uint64_t seed;
     const float2 *ntab;  // normal tables (NormalTab layout: sc then lg)
+    const double *lr_kick;  // [nb][3] long-range wake kick (px, py, dp) of OP_LRK
 };
 
 struct IonKickParams {
@@ -198,7 +205,35 @@ struct FinParams {
     // slot = t % hist_cap, prev_slot = (t - 1) % hist_cap, t = *turn_dev + turn_off
     const uint32_t *turn_dev;
     int32_t turn_off, hist_cap;
+    double *lr_cen;       // [nb][3] post-map centroids (OP_LRC)
 };
+
+// MPBeam::LRWakeBeamIntaction for one turn: one block per receiving bunch.
+struct LrwParams {
+    const double *table;   // [3][np] W_x, W_y, W_z at tau = -i dt
+    int32_t np, nturns, nb, harmonic;
+    double dt, trf, rbeta, scale;  // scale = e / E / rbeta^2
+    const int32_t *harm;   // bunch harmonic numbers
+    const double *ne;      // electrons per bunch
+    const double *cen;     // [nb][3] this turn's post-map centroids
+    double *ring;          // [nturns][nb][3] centroids by turn mod nturns
+    double *kick;          // [nb][3] out
+    cts_moments *cur, *hist;
+    int32_t hist_cap, fix_dp;
+    uint32_t turn;         // or *turn_dev + turn when turn_dev is set
+    const uint32_t *turn_dev;
+    uint32_t *status;
+};
+hipError_t launch_lrwake(const LrwParams &p, hipStream_t st);
+
+// Host: the long-range wake table (w, rw, bbr: [3][np]; time[np] = -tau).
+struct LrwTable {
+    int np = 0;
+    double dt = 0, trf = 0;
+    std::vector<double> time, w, rw, bbr;
+};
+int lrwake_table(const cts_lrwake_config &c, double t0, int harmonic, LrwTable &tab, std::string &err);
+void lrwake_yokoya(double rx, double ry, double out[5]);
 
 struct WakeTableParams {
     const double *edges;  // per table: {lo, dz}

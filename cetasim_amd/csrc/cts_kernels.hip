@@ -193,7 +193,8 @@ __host__ __device__ constexpr bool acc_used(uint32_t M, int j)
     return (j <= A_ZMAX && j != A_LOSS) ? (M & OP_MOMA) != 0
          : j == A_LOSS                  ? (M & OP_LOSS) != 0
          : j <= B_QZD                   ? (M & (OP_MOMB | OP_MOMD)) != 0
-                                        : (M & OP_ZSTAT) != 0;
+         : j <= C_ZMAX                  ? (M & OP_ZSTAT) != 0
+                                        : (M & OP_LRC) != 0;
 }
 
 template <uint32_t M, bool EXACT, int NOISE, int NRF>
@@ -232,6 +233,13 @@ k_pass(PassParams P)
             for (int j = tid; j < P.nbins; j += kBlock) kick_lds[j] = P.kick_prev[(int64_t)b * P.nbins + j];
         if (use_hist)
             for (int j = tid; j < P.nbins; j += kBlock) hist_lds[j] = 0u;
+        // the bunch's long-range wake kick of the previous turn (uniform)
+        double lrk0 = 0, lrk1 = 0, lrk2 = 0;
+        if (ops & OP_LRK) {
+            lrk0 = P.lr_kick[3 * b];
+            lrk1 = P.lr_kick[3 * b + 1];
+            lrk2 = P.lr_kick[3 * b + 2];
+        }
         double klo = 0, kdz = 1, blo = 0, bdz = 1;
         if (use_kick) { klo = P.edges_prev[2 * b]; kdz = P.edges_prev[2 * b + 1]; }
         if (use_hist) { blo = P.edges_now[2 * b]; bdz = P.edges_now[2 * b + 1]; }
@@ -274,6 +282,14 @@ k_pass(PassParams P)
                 fn = ntload(sfl_ + kn);
             }
 
+            // long-range wake kick (src/Bunch.cpp:1091-1101), applied before
+            // the short-range one as the reference orders them
+            // (src/MPBeam.cpp:1085-1086); survivors at kick time
+            if ((ops & OP_LRK) && ((f & 3u) == 0u || (f & kNewLost))) {
+                X[1] += lrk0;
+                X[3] += lrk1;
+                X[5] += lrk2;
+            }
             if (use_kick) {
                 // survivors when the kick was computed: code 0, or lost at
                 // the marking that followed it
@@ -295,6 +311,15 @@ k_pass(PassParams P)
                 acc[B_QZD] += dz * dd;
             }
             if ((ops & OP_MAP) && alive) stage_map(X, reload(P.cst));
+            if ((ops & OP_LRC) && alive) {
+                // bunch centroid after the section map (MPBeamRMSCal,
+                // src/MPBeam.cpp:1068), the long-range wake's source
+                cdptr sh = reload(sh_addr);
+                acc[L_N] += 1.0;
+                acc[L_SX] += X[0] - sh[0];
+                acc[L_SY] += X[2] - sh[2];
+                acc[L_SZ] += X[4] - sh[4];
+            }
             if ((ops & OP_ZSTAT) && alive) {
                 const double dz = X[4] - reload(sh_addr)[4];
                 acc[C_N] += 1.0;
@@ -529,6 +554,13 @@ __device__ __forceinline__ void finalize_body(const FinParams &P, int c, int b)
         const uint32_t t = *P.turn_dev + (uint32_t)P.turn_off;
         slot = (int32_t)(t % (uint32_t)P.hist_cap);
         prev_slot = t >= 1u ? (int32_t)((t - 1u) % (uint32_t)P.hist_cap) : -1;
+    }
+    if ((P.ops & OP_LRC) && P.lr_cen) {
+        // post-map centroids (with the shift of the pass, before MOMA moves it)
+        const double n = tot[L_N];
+        P.lr_cen[3 * b] = sh[0] + tot[L_SX] / n;
+        P.lr_cen[3 * b + 1] = sh[2] + tot[L_SY] / n;
+        P.lr_cen[3 * b + 2] = sh[4] + tot[L_SZ] / n;
     }
     double dmean = 0, dzd = 0, ddd = 0;
     if (P.ops & (OP_MOMB | OP_MOMD)) {
@@ -854,13 +886,92 @@ hipError_t launch_w_debug(const double *re, const double *im, int64_t n, double 
     return hipGetLastError();
 }
 
+// ------------------------------------------------------ long-range wake
+// MPBeam::LRWakeBeamIntaction (src/MPBeam.cpp:3232-3351) for one turn,
+// one block per receiving bunch j: over the last nturns turns n (n = 0 the
+// present one, bunches ahead of j only; n = nturns - 1 bunches from j on),
+// tau = (h_i - h_j - n h) T_rf + (z_j - z_i[n]) / (c beta), the wake table
+// interpolated linearly at -tau (GetTotLRWakeLinearFit,
+// src/WakeFunction.cpp:204-229), F -= W N_i {x_i[n], y_i[n], 1}; scaled by
+// e / E / beta^2.  The px, py (and, without a short-range wake, dp) means of
+// the turn's moment record move by the kick, which every survivor gets with
+// the next pass.  The threads' partial sums are folded in a fixed order.
+__global__ void __launch_bounds__(256)
+k_lrwake(LrwParams P)
+{
+    __shared__ double red[3][256];
+    const int j = blockIdx.x, tid = threadIdx.x;
+    const uint32_t t = P.turn_dev ? *P.turn_dev + P.turn : P.turn;
+    const int T = P.nturns, nb = P.nb;
+    const double zj = P.cen[3 * j + 2];
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    bool ahead = false;
+    for (int n = 0; n < T; n++) {
+        int i0 = 0, i1 = nb - 1;
+        if (n == 0) i1 = j - 1;
+        else if (n == T - 1) i0 = j;
+        const int64_t slot = (((int64_t)t - n) % T + T) % T;
+        const double *pn = n == 0 ? P.cen : P.ring + slot * nb * 3;
+        for (int i = i0 + tid; i <= i1; i += 256) {
+            const int ntau = P.harm[i] - P.harm[j] - n * P.harmonic;
+            const double tau = ntau * P.trf + (zj - pn[3 * i + 2]) / kClight / P.rbeta;
+            if (tau > 0) {
+                ahead = true;
+                continue;
+            }
+            const double u = -tau / P.dt;
+            int idx = (int)floor(u);
+            idx = idx < 0 ? 0 : (idx > P.np - 2 ? P.np - 2 : idx);
+            const double v1 = u - idx, v0 = 1 - v1;
+            const double w0 = P.table[idx] * v0 + P.table[idx + 1] * v1;
+            const double w1 = P.table[P.np + idx] * v0 + P.table[P.np + idx + 1] * v1;
+            const double w2 = P.table[2 * P.np + idx] * v0 + P.table[2 * P.np + idx + 1] * v1;
+            f0 -= w0 * P.ne[i] * pn[3 * i];
+            f1 -= w1 * P.ne[i] * pn[3 * i + 1];
+            f2 -= w2 * P.ne[i];
+        }
+    }
+    if (ahead) atomicOr(P.status, 4u);
+    red[0][tid] = f0;
+    red[1][tid] = f1;
+    red[2][tid] = f2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o)
+            for (int k = 0; k < 3; k++) red[k][tid] += red[k][tid + o];
+        __syncthreads();
+    }
+    if (tid < 3) P.ring[(int64_t)(t % (uint32_t)T) * nb * 3 + 3 * j + tid] = P.cen[3 * j + tid];
+    if (tid == 0) {
+        const double F[3] = {red[0][0] * P.scale, red[1][0] * P.scale, red[2][0] * P.scale};
+        P.kick[3 * j] = F[0];
+        P.kick[3 * j + 1] = F[1];
+        P.kick[3 * j + 2] = F[2];
+        cts_moments *m = P.cur + j;
+        m->mean[1] += F[0];
+        m->mean[3] += F[1];
+        if (P.fix_dp) m->mean[5] += F[2];
+        if (P.hist && P.hist_cap > 0) {
+            cts_moments *hm = P.hist + (int64_t)(t % (uint32_t)P.hist_cap) * nb + j;
+            hm->mean[1] = m->mean[1];
+            hm->mean[3] = m->mean[3];
+            if (P.fix_dp) hm->mean[5] = m->mean[5];
+        }
+    }
+}
+hipError_t launch_lrwake(const LrwParams &p, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lrwake, dim3(p.nb), dim3(256), 0, st, p);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------ launches
-constexpr uint32_t V_KICK = OP_KICK | OP_MOMB | OP_WRITE;
+constexpr uint32_t V_KICK = OP_KICK | OP_MOMB | OP_LRK | OP_WRITE;
 constexpr uint32_t V_MOM = OP_MOMA | OP_MOMB | OP_MOMD | OP_ZSTAT | OP_LOSS;
 constexpr uint32_t V_FIX = OP_KICK | OP_MOMB | OP_MAP | OP_SKEW | OP_RF | OP_DRIFT | OP_U0 | OP_SYNRAD |
-                           OP_BIN | OP_LOSS | OP_MOMA | OP_MOMD | OP_WRITE;
+                           OP_BIN | OP_LOSS | OP_MOMA | OP_MOMD | OP_WRITE | OP_LRC | OP_LRK;
 constexpr uint32_t V_REF = OP_KICK | OP_MOMB | OP_MAP | OP_ZSTAT | OP_SKEW | OP_RF | OP_DRIFT | OP_U0 |
-                           OP_SYNRAD | OP_LOSS | OP_MOMA | OP_MOMD | OP_WRITE;
+                           OP_SYNRAD | OP_LOSS | OP_MOMA | OP_MOMD | OP_WRITE | OP_LRC | OP_LRK;
 constexpr uint32_t V_ALL = 0xffffffffu;
 
 // stage sets of the fused turn in its steady state

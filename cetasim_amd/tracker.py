@@ -73,6 +73,38 @@ class Tracker:
         assert noise.size == 5 * self.ntot
         capi.check(self.lib.cts_push_noise(self.h, capi.dptr(noise)))
 
+    def set_lrwake(self, nturns, sectors=None, res=None, beta_x=1.0, beta_y=1.0):
+        """Long-range wake: nturns = lrwNTurnsWakeTrunction + 1 (0: off);
+        sectors [n][7] (radiusX, radiusY, length, betaX, betaY, number,
+        conductivity), res [n][9] (Rs, Q, f, x Rs, Q, f, y Rs, Q, f)."""
+        self._lrw_keep = []
+        cfg = capi.LrwakeConfig()
+        cfg.nturns = nturns
+        for name, arr, width in (("sectors", sectors, 7), ("res", res, 9)):
+            if arr is not None and len(arr):
+                a = np.ascontiguousarray(np.asarray(arr, dtype=np.float64).reshape(-1, width))
+                self._lrw_keep.append(a)
+                setattr(cfg, name, capi.dptr(a))
+                setattr(cfg, "n_sectors" if name == "sectors" else "n_res", a.shape[0])
+        cfg.beta_x, cfg.beta_y = beta_x, beta_y
+        capi.check(self.lib.cts_set_lrwake(self.h, ctypes.byref(cfg)))
+
+    def lrwake_table(self):
+        n = ctypes.c_int32(0)
+        rc = self.lib.cts_get_lrwake_table(self.h, ctypes.byref(n), None, None, None, None)
+        if rc != capi.CTS_ERR_ARG:
+            capi.check(rc)
+        t = np.empty(n.value)
+        w, rw, bbr = (np.empty(3 * n.value) for _ in range(3))
+        capi.check(self.lib.cts_get_lrwake_table(self.h, ctypes.byref(n), capi.dptr(t), capi.dptr(w), capi.dptr(rw),
+                                                 capi.dptr(bbr)))
+        return t, w.reshape(3, -1), rw.reshape(3, -1), bbr.reshape(3, -1)
+
+    def lrwake_kick(self):
+        out = np.empty(3 * self.nb)
+        capi.check(self.lib.cts_get_lrwake_kick(self.h, capi.dptr(out)))
+        return out.reshape(self.nb, 3)
+
     def set_ions(self, cfg):
         """cfg: capi.IonConfig, or None to switch ions off."""
         if cfg is None:

@@ -36,6 +36,9 @@ extern "C" {
 /* ion cloud above ionCalIonMaxNumber: the reference exits
  * (src/LatticeInterActionPoint.cpp:1041-1049) */
 #define CTS_ION_OVERFLOW 2
+/* long-range wake: a source bunch lies ahead of the receiving one in time
+ * (tau > 0): the reference exits (src/WakeFunction.cpp:206-210) */
+#define CTS_LRWAKE_TAU 3
 
 #define CTS_MAX_RF 4
 #define CTS_MAX_RES 8
@@ -128,6 +131,36 @@ int cts_download(cts_handle *h, double *x, double *px, double *y, double *py,
 int cts_set_wake(cts_handle *h, int32_t nbins, int32_t mode, double grid_lo,
                  double grid_hi, int32_t nres, const double *rs, const double *q,
                  const double *fr_hz);
+
+/* Long-range (coupled-bunch) wake: WakeFunction::InitialLRWake
+ * (src/WakeFunction.cpp:49-205) with RWWakeParaReadIn / BBRWakeParaReadIn
+ * (src/WakeFunction.cpp:251-350).  The wake table W(tau) of the resistive
+ * wall (GetRWLRWakeFun, src/WakeFunction.cpp:499-555) plus the broadband
+ * resonators (GetBBRWakeFun, src/WakeFunction.cpp:655-720) is tabulated at
+ * tau = -i * T_rf / 2, i < np = int(nturns * h * T_rf / (T_rf / 2)) + 2, and
+ * every turn MPBeam::LRWakeBeamIntaction (src/MPBeam.cpp:3232-3351) sums it
+ * over the bunches' post-map centroids of the last nturns turns into one
+ * px, py, delta kick per bunch (Bunch::BunchTransferDueToWake,
+ * src/Bunch.cpp:1091-1101), applied with the turn's deferred kicks.
+ * nturns = lrwNTurnsWakeTrunction + 1 (the value the reference stores);
+ * 0 switches the wake off.  Fused turn (one section) only. */
+typedef struct {
+    int32_t nturns;
+    int32_t n_sectors;       /* resistive-wall pipe sectors (0: none) */
+    const double *sectors;   /* [n_sectors][7]: radiusX, radiusY [m], length [m], betaX,
+                                betaY [m], number, conductivity [1/(Ohm m)] (the pipe file's columns) */
+    int32_t n_res;           /* broadband resonators (0: none) */
+    const double *res;       /* [n_res][9]: Rs [Ohm], Q, f [Hz], x Rs [Ohm/m^2], Q, f, y Rs, Q, f
+                                (the BBR file's columns) */
+    double beta_x, beta_y;   /* beta functions at the first twiss point [m] */
+} cts_lrwake_config;
+int cts_set_lrwake(cts_handle *h, const cts_lrwake_config *cfg);
+/* The wake table (time[np] >= 0 and w[3][np]: W_x, W_y [V/C/m], W_z [V/C]
+ * at tau = -time; rw[3][np] and bbr[3][np] its two parts, may be null);
+ * *np in: capacity, out: np. */
+int cts_get_lrwake_table(cts_handle *h, int32_t *np, double *time, double *w, double *rw, double *bbr);
+/* The last turn's long-range kick of every bunch: kick[nb][3] (px, py, delta). */
+int cts_get_lrwake_kick(cts_handle *h, double *kick);
 
 /* Radiation damping switch and quantum-excitation noise source
  * (BunchSynRadDamping, src/Bunch.cpp:940-1044).  noise_mode 0: damping

@@ -257,6 +257,16 @@ void orc_ion_rms(orc_ion_cloud *c, int p);
 /* GetCBMGR's analysis (ideal modes, IQ, analytic signals, Hilbert modes,
  * rates) of centroids mean[ns][nb][6] with direct DFTs; layout as
  * cts_cbm_analysis.  Returns -1 when the fit window lies outside. */
+/* ------------------------------------------------------ long-range wake
+ * (lrwake.c) WakeFunction::InitialLRWake's table, the Yokoya factors of a
+ * pipe sector and MPBeam::LRWakeBeamIntaction's per-turn sum. */
+void orc_lrw_yokoya(double rx, double ry, double out[5]);
+int orc_lrw_table(int nturns, int harmonic, double t0, int nsec, const double *sectors, int nres,
+                  const double *res, double beta_x, double beta_y, int cap, double *time, double *w,
+                  double *rw, double *bbr);
+int orc_lrw_kick(int nturns, int nb, const int *harm, int harmonic, double t0, double rbeta, double energy,
+                 const double *ne, const double *pos, const double *w, int np, double *kick);
+
 int orc_cbm_analysis(const double *mean, int ns, int nb, const int *harm, int harmonics, const double *tunes,
                      const double *twiss, int interval, int fit_start, int fit_end, double t0,
                      double *ideal_amp, double *ideal_arg, double *iq_amp, double *iq_arg, double *ana_re,

@@ -69,6 +69,12 @@ def lib():
             subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
         L = ctypes.CDLL(ORACLE)
         L.orc_faddeeva_w.argtypes = [D, D, PD, PD]
+        I = ctypes.c_int
+        L.orc_lrw_yokoya.argtypes = [D, D, PD]
+        L.orc_lrw_table.restype = I
+        L.orc_lrw_table.argtypes = [I, I, D, I, PD, I, PD, D, D, I, PD, PD, PD, PD]
+        L.orc_lrw_kick.restype = I
+        L.orc_lrw_kick.argtypes = [I, I, ctypes.POINTER(I), I, D, D, D, PD, PD, PD, I, PD]
         L.orc_erfcx.restype = D; L.orc_erfcx.argtypes = [D]
         L.orc_w_im.restype = D; L.orc_w_im.argtypes = [D]
         L.orc_bassetti_erskine.argtypes = [D, D, D, D, PD, PD]
@@ -200,9 +206,18 @@ class OracleTurn:
         self.tau = np.array(ring.synrad_tau[:]); self.en = np.array(ring.emit_nat[:])
 
     def turn(self, b, turn, noise=None):
+        self.first_half(b, turn, noise)
+        return self.second_half(b)
+
+    def first_half(self, b, turn, noise=None):
+        """Map, skew, RF, drift, U0, synrad; returns the post-map centroid
+        (x, y, z) of the survivors (MPBeamRMSCal, src/MPBeam.cpp:1068)."""
         L, R = self.L, self.ring
         L.orc_section_map(ctypes.byref(b.s), ctypes.byref(self.map))
-        sig_stale = rms(b, self.sec).sigma_z
+        m = rms(b, self.sec)
+        sig_stale = m.sigma_z
+        self._sig_stale = sig_stale
+        cen = (m.mean[0], m.mean[2], m.mean[4])
         if R.skew_k != 0:
             L.orc_skew_quad(ctypes.byref(b.s), R.skew_k)
         for r in range(R.n_rf):
@@ -216,6 +231,12 @@ class OracleTurn:
                 noise = qe_fill(self.noise_seed, turn, self.base, n)
             L.orc_synrad(ctypes.byref(b.s), p(self.b1h1), p(self.h1b1), p(self.tau), p(self.en),
                          None if noise is None else p(np.ascontiguousarray(noise).ravel()))
+        return cen
+
+    def second_half(self, b):
+        """Short-range wake and loss marking."""
+        L, R = self.L, self.ring
+        sig_stale = self._sig_stale
         if self.nbins:
             tab = self.table
             L.orc_sr_wake(ctypes.byref(b.s), self.nbins, self.mode, sig_stale, self.grid[0], self.grid[1],
@@ -265,8 +286,23 @@ class OracleTrain:
     interaction point 0 (src/MPBeam.cpp:1059-1102, 2585-2611): moments of
     every bunch, passages in train order, then each bunch's own turn."""
 
-    def __init__(self, ring, sec, counts, n_e, gap, ion_cfg, seed, **turn_kw):
+    def __init__(self, ring, sec, counts, n_e, gap, ion_cfg, seed, lrw=None, **turn_kw):
         self.L = lib()
+        # long-range wake: dict(nturns, sectors [n][7], res [n][9], beta_x, beta_y)
+        self.lrw = None
+        if lrw is not None:
+            cap = 2 * lrw["nturns"] * ring.harmonic + 4
+            t, w = np.zeros(cap), np.zeros(3 * cap)
+            sec_a = np.ascontiguousarray(np.asarray(lrw.get("sectors", np.zeros((0, 7))), float).reshape(-1, 7))
+            res_a = np.ascontiguousarray(np.asarray(lrw.get("res", np.zeros((0, 9))), float).reshape(-1, 9))
+            npt = self.L.orc_lrw_table(lrw["nturns"], ring.harmonic, ring.t0, len(sec_a), p(sec_a), len(res_a),
+                                       p(res_a), lrw["beta_x"], lrw["beta_y"], cap, p(t), p(w), None, None)
+            assert npt > 0
+            w = np.ascontiguousarray(np.concatenate([w[k * npt:(k + 1) * npt] for k in range(3)]))
+            self.lrw = dict(lrw, np=npt, w=w, time=t[:npt].copy(),
+                            pos=np.zeros((lrw["nturns"], len(counts), 3)))
+            self.lrw_harm = np.concatenate([[0], np.cumsum(np.asarray(gap)[:-1])]).astype(np.int32)
+            self.last_lrw_kick = None
         self.ring, self.sec = ring, sec
         self.counts = np.asarray(counts, dtype=np.int64)
         self.offsets = np.concatenate([[0], np.cumsum(self.counts)])
@@ -300,4 +336,28 @@ class OracleTrain:
     def turn(self, bunches, turn):
         if self.cloud is not None:
             self.ion_phase(bunches, turn)
-        return [t.turn(b, turn) for t, b in zip(self.turns, bunches)]
+        if self.lrw is None:
+            return [t.turn(b, turn) for t, b in zip(self.turns, bunches)]
+        # every bunch up to synrad, then MPBeam::LRWakeBeamIntaction
+        # (src/MPBeam.cpp:1085, 3232-3351), then the short-range wake and
+        # the loss marking
+        cen = np.array([t.first_half(b, turn) for t, b in zip(self.turns, bunches)])
+        W = self.lrw
+        W["pos"] = np.concatenate([W["pos"][1:], cen[None]], axis=0)
+        nb = len(bunches)
+        kick = np.zeros(3 * nb)
+        pos = np.ascontiguousarray(W["pos"].reshape(-1))
+        harm = self.lrw_harm.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        rc = self.L.orc_lrw_kick(W["nturns"], nb, harm, self.ring.harmonic, self.ring.t0, self.ring.rbeta,
+                                 self.ring.energy, p(np.ascontiguousarray(self.n_e)), p(pos), p(W["w"]), W["np"],
+                                 p(kick))
+        if rc != 0:
+            raise RuntimeError(f"long-range wake: tau > 0 ({rc})")
+        kick = kick.reshape(nb, 3)
+        self.last_lrw_kick = kick
+        for j, b in enumerate(bunches):
+            # Bunch::BunchTransferDueToWake (src/Bunch.cpp:1091-1101): every particle
+            b["px"][:] += kick[j, 0]
+            b["py"][:] += kick[j, 1]
+            b["dp"][:] += kick[j, 2]
+        return [t.second_half(b) for t, b in zip(self.turns, bunches)]

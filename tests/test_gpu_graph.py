@@ -89,6 +89,29 @@ def test_graph_replays_match_eager(counts, wake_mode, ions):
     g.close()
 
 
+@pytest.mark.parametrize("wake_mode", [1, None])
+def test_graph_replays_with_lrwake(wake_mode):
+    """The long-range wake's kick is deferred like the short-range one and
+    its centroid history is indexed by the device turn counter."""
+    e, g = make([3000, 5001, 2500, 1800], wake_mode, False)
+    sec = [[0.01, 0.01, 2304.0, 2.7, 4.25, 1, 3.5e5]]
+    res = [[2e4, 1.5, 1.1e9, 3e6, 2.0, 0.9e9, 5e6, 2.5, 1.05e9]]
+    for tr in (e, g):
+        tr.set_lrwake(4, sec, res, 5.0, 7.0)
+    print(f"graph case with long-range wake, wake={wake_mode}", flush=True)
+    for t in range(9):
+        e.turn(t)
+    g.run_turns(0, 9)
+    same(e, g, 0, 8, "turns 0-8")
+    np.testing.assert_array_equal(e.lrwake_kick(), g.lrwake_kick())
+    for t in range(9, 14):
+        e.turn(t)
+    g.run_turns(9, 5)
+    same(e, g, 9, 13, "turns 9-13")
+    e.close()
+    g.close()
+
+
 def test_graph_rebuilt_after_settings_change():
     """A replay after cts_set_wake uses the new settings (the graphs are
     dropped and captured again)."""
