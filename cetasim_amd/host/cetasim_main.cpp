@@ -168,6 +168,10 @@ int dump(const std::string &dir, ReadInputSettings &in, LatticeInterActionPoint 
                      ic.temperature, ic.length, ic.loss_boundary);
         std::fprintf(f, "\"ion_ap\": [%.17g, %.17g], \"ion_seed\": %llu,\n", ic.ap_x, ic.ap_y,
                      (unsigned long long)ic.seed);
+        std::fprintf(f, "\"lrw\": %d, \"lrw_nturns\": %d, \"lrw_beta\": [%.17g, %.17g],\n", c.lrw, c.lrw_nturns,
+                     c.lrw_beta[0], c.lrw_beta[1]);
+        arr("lrw_sectors", c.lrw_sectors.data(), c.lrw_sectors.size());
+        arr("lrw_res", c.lrw_res.data(), c.lrw_res.size());
         std::fprintf(f, "\"workQz\": %.17g, \"betaZ\": %.17g, \"tbt\": \"%s\", \"cbmgr\": \"%s\"\n}\n",
                      in.ringParBasic->workQz, in.ringBunchPara->rmsBunchLength / in.ringBunchPara->rmsEnergySpread,
                      R->TBTBunchAverData.c_str(), R->runCBMGR.c_str());

@@ -148,6 +148,11 @@ int ReadInputSettings::ParamReadFile(const std::string &path)
             else if (k == "ioncalionmaxnumber") ion_.ionMaxNumber = num(v, 0);
             else if (k == "ioncalionlossboundary") ion_.ionLossBoundary = num(v, 0);
             else if (k == "ioncalmacroionnumbergeneratedperip") ion_.macroIonNumberGeneratedPerIP = (int)num(v, 0);
+            // long-range wake (src/ReadInputSettings.cpp:688-705)
+            else if (k == "lrwpipegeoinput") lrw_.pipeGeoInput = v.at(0);
+            else if (k == "lrwoutput") lrw_.lrwOutput = v.at(0);
+            else if (k == "lrwbbrinput") lrw_.bbrInput = v.at(0);
+            else if (k == "lrwnturnswaketrunction") lrw_.nTurnswakeTrunction = inum(v, 0) + 1;
             // short-range wake
             else if (k == "srwbbrinput") srw_.SRWBBRInput = v.at(0);
             else if (k == "srwbunchbinnum") srw_.SRWBunchBinNum = (int)num(v, 0);
@@ -158,6 +163,7 @@ int ReadInputSettings::ParamReadFile(const std::string &path)
             else if (k == "runsynraddampingflag") run_.synRadDampingFlag = inum(v, 0);
             else if (k == "runbeamionflag") run_.beamIonFlag = inum(v, 0);
             else if (k == "runshortrangewake") run_.sRWakeFlag = inum(v, 0);
+            else if (k == "runlongrangewake") run_.lRWakeFlag = inum(v, 0);
             else if (k == "rungrowthratefittingstart") run_.growthRateFittingStart = inum(v, 0);
             else if (k == "rungrowthratefittingend") run_.growthRateFittingEnd = inum(v, 0);
             else if (k == "runtbtbunchaverdata") run_.TBTBunchAverData = v.at(0);
@@ -362,6 +368,42 @@ int WakeFunction::InitialSRWake(const ReadInputSettings &in)
 {
     if (in.ringSRWake->SRWBBRInput.empty()) return 0;
     return BBRWakeParaReadIn(in.ringSRWake->SRWBBRInput);
+}
+
+int WakeFunction::RWWakeParaReadIn(const std::string &path)
+{
+    std::ifstream fin(path);
+    if (!fin.is_open()) {
+        std::cerr << "Error opening file: " << path << std::endl;
+        return 1;
+    }
+    std::string str;
+    std::getline(fin, str);  // one header line
+    std::vector<double> *col[7] = {&sectorRadiusX, &sectorRadiusY, &sectorLength, &sectorBetaX,
+                                   &sectorBetaY,   &sectorNum,     &sectormatSigma};
+    while (std::getline(fin, str)) {
+        std::istringstream ls(str);
+        std::vector<double> c;
+        std::string tok;
+        while (ls >> tok) c.push_back(std::stod(tok));
+        if (c.empty()) continue;
+        if (c.size() < 7) {
+            std::cerr << path << ": 7 columns expected" << std::endl;
+            return 1;
+        }
+        for (int k = 0; k < 7; k++) col[k]->push_back(c[k]);
+    }
+    return 0;
+}
+
+int WakeFunction::InitialLRWake(const ReadInputSettings &in, const LatticeInterActionPoint &lat)
+{
+    betaFunIntPoint[0] = lat.twissBetaX[0];
+    betaFunIntPoint[1] = lat.twissBetaY[0];
+    const auto *L = in.ringLRWake;
+    if (!L->pipeGeoInput.empty() && RWWakeParaReadIn(L->pipeGeoInput)) return 1;
+    if (!L->bbrInput.empty() && BBRWakeParaReadIn(L->bbrInput)) return 1;
+    return 0;
 }
 
 // ---------------------------------------------------------- distribution
@@ -838,6 +880,27 @@ int MPBeam::RunSettings(const LatticeInterActionPoint &lat, const ReadInputSetti
         c.fr.resize(sr.lOmega.size());
         for (size_t i = 0; i < c.fr.size(); i++) c.fr[i] = sr.lOmega[i] / (2 * kPi);
     }
+    if (R->lRWakeFlag) {
+        if (lat.sections.size() != 1) {
+            std::cerr << "long-range wake: one interaction point (fused turn) only" << std::endl;
+            return 1;
+        }
+        WakeFunction lw;
+        if (lw.InitialLRWake(in, lat)) return 1;
+        c.lrw = true;
+        c.lrw_nturns = in.ringLRWake->nTurnswakeTrunction;
+        for (size_t i = 0; i < lw.sectorRadiusX.size(); i++)
+            c.lrw_sectors.insert(c.lrw_sectors.end(), {lw.sectorRadiusX[i], lw.sectorRadiusY[i], lw.sectorLength[i],
+                                                       lw.sectorBetaX[i], lw.sectorBetaY[i], lw.sectorNum[i],
+                                                       lw.sectormatSigma[i]});
+        for (size_t i = 0; i < lw.lRs.size(); i++)
+            c.lrw_res.insert(c.lrw_res.end(), {lw.lRs[i], lw.lQ[i], lw.lOmega[i] / (2 * kPi), lw.txRs[i], lw.txQ[i],
+                                               lw.txOmega[i] / (2 * kPi), lw.tyRs[i], lw.tyQ[i],
+                                               lw.tyOmega[i] / (2 * kPi)});
+        c.lrw_beta[0] = lw.betaFunIntPoint[0];
+        c.lrw_beta[1] = lw.betaFunIntPoint[1];
+        c.lrw_output = in.ringLRWake->lrwOutput;
+    }
     c.synrad = R->synRadDampingFlag != 0;
     c.seed = R->seed;
     if (R->beamIonFlag) {
@@ -869,6 +932,34 @@ int MPBeam::RunSettings(const LatticeInterActionPoint &lat, const ReadInputSetti
     return 0;
 }
 
+int MPBeam::PrintLRWakeTable(const std::string &path)
+{
+    int32_t np = 0;
+    (void)cts_get_lrwake_table(gpu, &np, nullptr, nullptr, nullptr, nullptr);  // np
+    std::vector<double> t(np), w(3 * (size_t)np), rw(3 * (size_t)np), bbr(3 * (size_t)np);
+    if (check(cts_get_lrwake_table(gpu, &np, t.data(), w.data(), rw.data(), bbr.data()), "wake table")) return 1;
+    std::ofstream fout(path);
+    // the SDDS layout of the reference's file, column header lines verbatim
+    fout << "SDDS1" << std::endl;
+    fout << "&column name=time,                    units=s,      type=float,  &end" << std::endl;
+    const char *names[9] = {"rw_wx", "rw_wy", "rw_wz", "bbr_wx", "bbr_wy", "bbr_wz", "tot_wx", "tot_wy", "tot_wz"};
+    for (int c = 0; c < 9; c++)
+        fout << "&column name=" << names[c] << ",              units=" << (c % 3 == 2 ? "V/C,    " : "V/C/m,  ")
+             << "type=float,  &end" << std::endl;
+    fout << "&data mode=ascii, &end" << std::endl;
+    fout << "! page number 1" << std::endl;
+    fout << np << std::endl;
+    for (int i = 0; i < np; i++) {
+        // the reference prints the table's times before flipping their sign
+        fout << std::setw(15) << std::left << (t[i] == 0.0 ? 0.0 : -t[i]);
+        const double *src[3] = {rw.data(), bbr.data(), w.data()};
+        for (int g = 0; g < 3; g++)
+            for (int k = 0; k < 3; k++) fout << std::setw(15) << std::left << src[g][(size_t)k * np + i];
+        fout << std::endl;
+    }
+    return fout.good() ? 0 : 1;
+}
+
 // MPBeam::Run, multi-particle tracking (src/MPBeam.cpp:1010-1173).  With one
 // section the whole turn is one cts_turn call and the moments of the print
 // turns come from the device history after the loop (no host round trip
@@ -891,6 +982,18 @@ int MPBeam::Run(Train &, LatticeInterActionPoint &lat, ReadInputSettings &in)
         return 1;
     if (check(cts_set_synrad(gpu, c.synrad, 2, c.seed), "synrad")) return 1;
     if (c.ions && check(cts_set_ions(gpu, &c.ion), "ions")) return 1;
+    if (c.lrw) {
+        cts_lrwake_config lc{};
+        lc.nturns = c.lrw_nturns;
+        lc.n_sectors = (int32_t)(c.lrw_sectors.size() / 7);
+        lc.sectors = c.lrw_sectors.data();
+        lc.n_res = (int32_t)(c.lrw_res.size() / 9);
+        lc.res = c.lrw_res.data();
+        lc.beta_x = c.lrw_beta[0];
+        lc.beta_y = c.lrw_beta[1];
+        if (check(cts_set_lrwake(gpu, &lc), "long-range wake")) return 1;
+        if (!c.lrw_output.empty() && PrintLRWakeTable(c.lrw_output)) return 1;
+    }
     const int interval = std::max(R->bunchInfoPrintInterval, 1);
     samples_.clear();
     // output files start empty, as the reference removes *.sdds at set-up

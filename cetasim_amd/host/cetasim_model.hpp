@@ -62,9 +62,13 @@ public:
         std::string SRWBBRInput;
         int SRWBunchBinNum = 100;
     };
+    struct RingLRWake {
+        std::string pipeGeoInput, bbrInput, lrwOutput;
+        int nTurnswakeTrunction = 0;  // lrwNTurnsWakeTrunction + 1, as the reference stores it
+    };
     struct RingRun {
         int calSetting = 2, nTurns = 0, bunchInfoPrintInterval = 1, synRadDampingFlag = 0, beamIonFlag = 0;
-        int sRWakeFlag = 0, growthRateFittingStart = 0, growthRateFittingEnd = 0;
+        int sRWakeFlag = 0, lRWakeFlag = 0, growthRateFittingStart = 0, growthRateFittingEnd = 0;
         std::string TBTBunchAverData = "TBTBunchAverData", runCBMGR = "CBMGR";
         uint64_t seed = 1;  // seed of the injected distribution, ions and noise (Q13)
     };
@@ -75,6 +79,7 @@ public:
     RingBunchPara *ringBunchPara = &bunch_;
     RingIonEffPara *ringIonEffPara = &ion_;
     RingSRWake *ringSRWake = &srw_;
+    RingLRWake *ringLRWake = &lrw_;
     RingRun *ringRun = &run_;
 
     // Returns 0; prints the reason and returns a non-zero code where the
@@ -89,6 +94,7 @@ private:
     RingBunchPara bunch_;
     RingIonEffPara ion_;
     RingSRWake srw_;
+    RingLRWake lrw_;
     RingRun run_;
     void Derive();
 };
@@ -123,6 +129,15 @@ public:
 class WakeFunction {
 public:
     std::vector<double> lRs, lQ, lOmega, txRs, txQ, txOmega, tyRs, tyQ, tyOmega;
+    // resistive-wall pipe sectors (RWWakeParaReadIn, src/WakeFunction.cpp:
+    // 292-320): one header line, 7 columns
+    std::vector<double> sectorRadiusX, sectorRadiusY, sectorLength, sectorBetaX, sectorBetaY, sectorNum,
+        sectormatSigma;
+    double betaFunIntPoint[2] = {0, 0};
+    int RWWakeParaReadIn(const std::string &filename);
+    // WakeFunction::InitialLRWake (src/WakeFunction.cpp:49-90): the pipe
+    // and resonator files of the long-range wake, beta at the first twiss point
+    int InitialLRWake(const ReadInputSettings &inputParameter, const LatticeInterActionPoint &lattice);
     // WakeFunction::InitialSRWake (src/WakeFunction.cpp:232-249): reads the
     // resonator file named by srwBBRInput when there is one.
     int InitialSRWake(const ReadInputSettings &inputParameter);
@@ -152,6 +167,11 @@ struct RunCalls {
     uint64_t seed = 0;
     bool ions = false;
     cts_ion_config ion{};
+    bool lrw = false;
+    int lrw_nturns = 0;
+    std::vector<double> lrw_sectors, lrw_res;  // [n][7], [n][9] (cts_lrwake_config's layout)
+    double lrw_beta[2] = {0, 0};
+    std::string lrw_output;
 };
 
 // The beam of bunches and the per-turn seams of MPBeam::Run
@@ -194,6 +214,9 @@ public:
     int GetCBMGR(ReadInputSettings &inputParameter, std::vector<double> &rates);
     void PrintTBTBunchAverData(std::ofstream &fout, int turn, const cts_moments *m, const double *ionCharge,
                                const ReadInputSettings &inputParameter, bool header);
+    // the long-range wake table file of InitialLRWake (src/WakeFunction.cpp:
+    // 100-155) from the library's table
+    int PrintLRWakeTable(const std::string &path);
 
 private:
     int check(int rc, const char *what);

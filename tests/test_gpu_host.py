@@ -23,10 +23,17 @@ import host_dump as H
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def program_run(tmp_path_factory):
-    d = tmp_path_factory.mktemp("run")
-    H.copy_fixture(str(d))
+PIPE = ("radiusX radiusY length betaX betaY num matSigma\n"
+        "0.01 0.01 2304.0 2.71263 4.25053 1 3.5e+5\n"
+        "0.03 0.012 300 5 8 2 1.4e6\n")
+LRW_EDITS = {"runLongRangeWake": "1", "LRWPipeGeoInput": "pipe.dat", "LRWBBRInput": "bbr.dat",
+             "LRWNTurnsWakeTrunction": "3", "LRWOutput": "lrw.sdds"}
+
+
+def run_program(d, edits=None):
+    H.copy_fixture(str(d), edits=edits)
+    if edits:
+        (d / "pipe.dat").write_text(PIPE)
     (d / "dump").mkdir()
     r = H.run(["--dump", "dump", "input.dat"], str(d))
     assert r.returncode == 0, r.stderr
@@ -35,6 +42,16 @@ def program_run(tmp_path_factory):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "End of Tracking 40 Turns" in r.stdout
     return str(d), H.load_dump(str(d / "dump"))
+
+
+@pytest.fixture(scope="module")
+def program_run(tmp_path_factory):
+    return run_program(tmp_path_factory.mktemp("run"))
+
+
+@pytest.fixture(scope="module")
+def program_run_lrw(tmp_path_factory):
+    return run_program(tmp_path_factory.mktemp("run_lrw"), LRW_EDITS)
 
 
 def read_sdds_pages(path):
@@ -55,7 +72,19 @@ def read_sdds_pages(path):
 
 
 def test_program_matches_capi_run(program_run):
-    d, D = program_run
+    check_against_capi(*program_run)
+
+
+def test_program_with_lrwake_matches_capi_run(program_run_lrw):
+    """The same with the long-range wake on (resistive wall of a round and
+    an elliptic sector, the fixture's resonators); the program's wake-table
+    file against the library's table."""
+    d, D = program_run_lrw
+    assert D["settings"]["lrw"] == 1 and D["settings"]["lrw_nturns"] == 4
+    check_against_capi(d, D)
+
+
+def check_against_capi(d, D):
     s = D["settings"]
     nb, interval, nturns = s["nb"], s["interval"], s["nTurns"]
     tr = Tracker(D["ring"], D["sections"])
@@ -64,6 +93,18 @@ def test_program_matches_capi_run(program_run):
     tr.set_wake(s["nbins"], 0, 0.0, 0.0, s["rs"], s["q"], s["fr"])
     tr.set_synrad(True, 2, s["seed"])
     tr.set_ions(H.ion_config(s))
+    if s.get("lrw"):
+        tr.set_lrwake(s["lrw_nturns"], np.array(s["lrw_sectors"]).reshape(-1, 7),
+                      np.array(s["lrw_res"]).reshape(-1, 9), *s["lrw_beta"])
+        t, w, rw, bbr = tr.lrwake_table()
+        pages = read_sdds_pages(os.path.join(d, "lrw.sdds"))
+        rows = pages[0]
+        assert int(rows[0][0]) == len(t)
+        g = lambda v: float("%g" % v)  # noqa: E731
+        for i in (0, 1, 2, len(t) // 2, len(t) - 1):
+            row = [float(x) for x in rows[1 + i]]
+            assert row[0] == g(-t[i])
+            assert row[1:] == [g(v) for v in np.concatenate([rw[:, i], bbr[:, i], w[:, i]])]
     tr.set_history(nturns)
     turns = list(range(0, nturns, interval))
     ionq = []

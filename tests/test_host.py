@@ -193,6 +193,34 @@ def test_dump(tmp_path):
         np.testing.assert_allclose(seg.mean(axis=1), m[b, :, 0], rtol=1e-12, atol=1e-20)
 
 
+def test_lrwake_settings(tmp_path):
+    """runLongRangeWake and the LRW* keys (src/ReadInputSettings.cpp:688-705,
+    812-815): the stored truncation is the input plus one; the pipe file
+    has one header line and 7 columns (src/WakeFunction.cpp:292-320), the
+    resonator file the 12 header lines and 9 columns of the short-range
+    one, frequencies in Hz; beta at the first twiss point."""
+    H.copy_fixture(str(tmp_path), {"runLongRangeWake": "1", "LRWPipeGeoInput": "pipe.dat", "LRWBBRInput": "bbr.dat",
+                                   "LRWNTurnsWakeTrunction": "3"})
+    (tmp_path / "pipe.dat").write_text("radiusX radiusY length betaX betaY num matSigma\n"
+                                       "0.01 0.02 2304.0 2.5 4.5 1 3.5e+7\n\n0.03 0.012 300 5 8 2 1.4e6\n")
+    d = tmp_path / "dump"
+    d.mkdir()
+    r = H.run(["--dump", str(d), "input.dat"], str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    s = H.load_dump(str(d))["settings"]
+    assert s["lrw"] == 1 and s["lrw_nturns"] == 4
+    assert s["lrw_sectors"] == [0.01, 0.02, 2304.0, 2.5, 4.5, 1, 3.5e7, 0.03, 0.012, 300, 5, 8, 2, 1.4e6]
+    np.testing.assert_allclose(s["lrw_res"], [1e4, 1, 1e10, 0, 1, 1e10, 0, 1, 1e10,
+                                              3e3, 2, 5e9, 0, 1, 5e9, 0, 1, 5e9], rtol=1e-15)
+    assert s["lrw_beta"] == [2.71, 4.25]  # betax, betay of the first row of twiss.dat
+    # off by default; more than one interaction point is refused
+    H.copy_fixture(str(tmp_path), {"runLongRangeWake": "1", "LRWPipeGeoInput": "pipe.dat", "RingTwiss": "twiss2.dat",
+                                   "ringSectNum": "2"})
+    (tmp_path / "pipe.dat").write_text("h\n0.01 0.01 1 1 1 1 1e7\n")
+    r = H.run(["--dump", str(d), "input.dat"], str(tmp_path))
+    assert r.returncode != 0 and "long-range wake" in r.stderr
+
+
 def test_rf_modes(tmp_path):
     # rfMode 1 (beam loading) is refused; several ideal cavities keep their
     # input phases (CavityResonator::Initial, src/CavityResonator.cpp:60-139)
