@@ -63,6 +63,9 @@ def parse():
     ap.add_argument("--eager", action="store_true",
                     help="time turns launched one by one instead of HIP-graph replays")
     ap.add_argument("--plain-only", action="store_true", help="c3: skip the surrogate run")
+    ap.add_argument("--lrwake", type=int, default=0, metavar="NTURNS",
+                    help="c3/c5: add the long-range wake (lrwNTurnsWakeTrunction = NTURNS - 1; a 2304 m round "
+                         "resistive-wall pipe and the workload's resonator)")
     ap.add_argument("--sur-steps", type=int, default=20000, help="c3: Adam steps of the surrogate training")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reads MIB, writes MIB) three times, "
@@ -236,11 +239,16 @@ def train_turns(a, sur=None, mode=0):
         sz = p["sig"][2]
         bbr = p["bbr"]
         tr.set_wake(T["nbins"], 1, -5 * sz, 5 * sz, bbr["rs"], bbr["q"], bbr["fr"])
+    if a.lrwake:
+        bbr = p["bbr"]
+        res = [[bbr["rs"][r], bbr["q"][r], bbr["fr"][r], 0.0, 1.0, bbr["fr"][r], 0.0, 1.0, bbr["fr"][r]]
+               for r in range(len(bbr["rs"]))]
+        tr.set_lrwake(a.lrwake, [[0.01, 0.01, 2304.0, 2.7, 4.25, 1, 3.5e7]], res, secs[0].beta_x, secs[0].beta_y)
     if sur is not None:
         tr.set_field_model(sur, mode)
     tr.autotune()
     dt, dt_ev, ktimes = timed_turns(tr, a, ("pass", "post", "finalize", "ion_prep", "ion_turn", "ion_compact",
-                                            "ion_reduce", "ion_finish", "ion_kick"))
+                                            "ion_reduce", "ion_finish", "ion_kick", "lrwake"))
     ktimes = {k: v for k, v in ktimes.items() if v[1] > 0}
     cloud = tr.ion_passages()
     m = tr.moments()
@@ -300,7 +308,9 @@ def run_train(a):
                                f"{T['ions']['n_new']} macro-ions per passage, <= {T['ions']['max_ions']}, "
                                "Bassetti-Erskine both ways (FP64 Faddeeva; with the surrogate inside its "
                                "domain when value_path names it), excitation on (device Philox)"
-                               + (f", BBR wake on {T['nbins']} fixed bins per bunch" if T.get("nbins") else ""),
+                               + (f", BBR wake on {T['nbins']} fixed bins per bunch" if T.get("nbins") else "")
+                               + (f", long-range wake over {a.lrwake} turns (2304 m round resistive-wall pipe "
+                                  "and the resonator)" if a.lrwake else ""),
                    "macroparticles": n, "bunches": nb, "parallelism": "single GPU",
                    "launch": "eager" if a.eager else "HIP graph replays of two turns"},
         "ms_per_step_kernel_timing_run_plain": dt_ev / a.steps * 1e3,

@@ -979,6 +979,7 @@ constexpr uint32_t T_QE = OP_KICK | OP_MOMB | OP_MAP | OP_RF | OP_DRIFT | OP_U0 
                           OP_LOSS | OP_MOMA | OP_WRITE;
 constexpr uint32_t T_NOSR = OP_KICK | OP_MOMB | OP_MAP | OP_RF | OP_DRIFT | OP_U0 | OP_BIN | OP_LOSS |
                             OP_MOMA | OP_WRITE;
+constexpr uint32_t T_QE_LR = T_QE | OP_LRC | OP_LRK;  // with the long-range wake
 
 hipError_t launch_pass(const PassParams &p, int nwg, hipStream_t st)
 {
@@ -988,6 +989,8 @@ hipError_t launch_pass(const PassParams &p, int nwg, hipStream_t st)
     const bool one_rf = p.n_rf == 1;
     if (one_rf && o == T_QE && p.noise_mode == 2)
         hipLaunchKernelGGL((k_pass<T_QE, true, 2, 1>), g, b, shm, st, p);
+    else if (one_rf && o == T_QE_LR && p.noise_mode == 2)
+        hipLaunchKernelGGL((k_pass<T_QE_LR, true, 2, 1>), g, b, shm, st, p);
     else if (one_rf && o == T_QE && p.noise_mode == 1)
         hipLaunchKernelGGL((k_pass<T_QE, true, 1, 1>), g, b, shm, st, p);
     else if (one_rf && o == T_NOSR)
