@@ -82,6 +82,13 @@ class LrwakeConfig(ctypes.Structure):
                 ("beta_x", D), ("beta_y", D)]
 
 
+class BeamLoadingConfig(ctypes.Structure):
+    _fields_ = [("nres", I32), ("rf_mode", I32 * MAX_RF), ("res_type", I32 * MAX_RF), ("res_harm", I32 * MAX_RF),
+                ("rs", D * MAX_RF), ("q0", D * MAX_RF), ("coupling", D * MAX_RF), ("detune_hz", D * MAX_RF),
+                ("vol", D * MAX_RF), ("phase", D * MAX_RF), ("cold", I32 * MAX_RF),
+                ("excite_instability", I32 * MAX_RF), ("dir_fb", I32 * MAX_RF), ("nbins", I32), ("nturns", I32)]
+
+
 class CbmResult(ctypes.Structure):
     _fields_ = [(n, PD) for n in ("ideal_amp", "ideal_arg", "iq_amp", "iq_arg", "ana_re", "ana_im", "hil_amp",
                                   "hil_arg", "rate_ideal", "rate_iq", "rate_hil", "rate_bunch")]
@@ -107,6 +114,8 @@ SIGNATURES = {
     "cts_set_lrwake": (I32, [ctypes.c_void_p, ctypes.POINTER(LrwakeConfig)]),
     "cts_get_lrwake_table": (I32, [ctypes.c_void_p, PI32, PD, PD, PD, PD]),
     "cts_get_lrwake_kick": (I32, [ctypes.c_void_p, PD]),
+    "cts_set_beam_loading": (I32, [ctypes.c_void_p, ctypes.POINTER(BeamLoadingConfig)]),
+    "cts_get_cavity_info": (I32, [ctypes.c_void_p, I32, PD]),
     "cts_ion_phase": (I32, [ctypes.c_void_p, I32]),
     "cts_get_ion_cloud": (I32, [ctypes.c_void_p, I32, I64, PI64, PD, PD, PD, PD]),
     "cts_get_ion_passages": (I32, [ctypes.c_void_p, PD]),

@@ -164,6 +164,11 @@ struct cts_handle {
     int32_t *d_lrw_harm = nullptr;      // [nb]
     bool lr_pending = false;
 
+    // beam-loaded cavities (cts_set_beam_loading)
+    bool bl_on = false;
+    BlTable bl;
+    double *d_bl_tab = nullptr;
+
     // graph replays of two consecutive turns (cts_run_turns): the turn is
     // read from d_turn, which the graph advances by 2; one instance per
     // (wake buffer parity, ion buffer) at capture
@@ -201,9 +206,18 @@ void free_lrw_state(cts_handle *h)
     h->lr_pending = false;
 }
 
+void free_bl_state(cts_handle *h)
+{
+    if (h->d_bl_tab) (void)hipFree(h->d_bl_tab);
+    h->d_bl_tab = nullptr;
+    h->bl_on = false;
+    h->bl = BlTable{};
+}
+
 void free_bunch_state(cts_handle *h)
 {
     free_lrw_state(h);
+    free_bl_state(h);
     void *ptrs[] = {h->d_x, h->d_px, h->d_y, h->d_py, h->d_z, h->d_dp, h->d_flag,
                     h->d_seg_begin, h->d_seg_end, h->d_bunch_off, h->d_bunch_count,
                     h->d_seg_bunch, h->d_wg_seg, h->d_bunch_seg, h->d_seg_part, h->d_shift,
@@ -405,6 +419,17 @@ PassParams base_pass(cts_handle *h, uint32_t ops, int k)
     p.seed = h->seed;
     p.ntab = h->d_ntab;
     p.lr_kick = h->d_lrw_kick;
+    if (h->bl_on) {
+        p.bl_tab = h->d_bl_tab;
+        p.zrange = h->d_zrange;
+        p.bl_nbl = h->bl.nbl;
+        p.bl_nbins = h->bl.nbins;
+        p.bl_nturns = h->bl.nturns;
+        p.nb = h->nb;
+        for (int q = 0; q < h->bl.nbl; q++) p.bl_kfac[q] = h->bl.kfac[q];
+        p.bl_energy = h->ring.energy;
+        p.bl_beta2 = std::pow(h->ring.rbeta, 2);
+    }
     if (h->capturing) {
         p.turn_dev = h->d_turn;
         p.turn = (uint32_t)h->cap_off;
@@ -1192,12 +1217,33 @@ static int turn_body(cts_handle *h, int32_t turn)
     else ops |= OP_MOMD;  // no deferred kick: dp moments close this turn
     if (h->lr_pending) ops |= OP_LRK;
     if (h->lrw_T > 0) ops |= OP_LRC;
-    PassParams p = base_pass(h, ops, 0);
-    if (!h->capturing) p.turn = (uint32_t)turn;
-    if ((rc = run_pass(h, p))) return rc;
-    // finalize: dp part of the previous record (MOMB), this turn's record
-    FinParams f = base_fin(h, ops & (OP_MOMB | OP_MOMA | OP_MOMD | OP_LOSS | OP_ZSTAT | OP_LRC), slot_of(h, turn),
-                           slot_of(h, h->last_turn));
+    FinParams f;
+    if (!h->bl_on) {
+        PassParams p = base_pass(h, ops, 0);
+        if (!h->capturing) p.turn = (uint32_t)turn;
+        if ((rc = run_pass(h, p))) return rc;
+        // finalize: dp part of the previous record (MOMB), this turn's record
+        f = base_fin(h, ops & (OP_MOMB | OP_MOMA | OP_MOMD | OP_LOSS | OP_ZSTAT | OP_LRC), slot_of(h, turn),
+                     slot_of(h, h->last_turn));
+    } else {
+        // beam-loaded cavities bin every bunch over its z range after the
+        // section map (src/MPBeam.cpp:2637, 2716-2752), so the turn runs
+        // as two passes: up to the map (and the skew quad), then from the
+        // RF stage on
+        const uint32_t pre = OP_KICK | OP_MOMB | OP_MAP | OP_SKEW | OP_LRK | OP_LRC;
+        const uint32_t opsA = (ops & pre) | OP_ZSTAT | OP_WRITE;
+        uint32_t opsB = (ops & ~pre) | OP_BL;
+        if (wake && h->wake_mode == 1) opsB &= ~OP_ZSTAT;
+        PassParams pa = base_pass(h, opsA, 0);
+        if (!h->capturing) pa.turn = (uint32_t)turn;
+        if ((rc = run_pass(h, pa, "pass_a"))) return rc;
+        FinParams fa = base_fin(h, opsA & (OP_MOMB | OP_ZSTAT | OP_LRC), -1, slot_of(h, h->last_turn));
+        if ((rc = run_finalize(h, fa))) return rc;
+        PassParams pb = base_pass(h, opsB, 0);
+        if (!h->capturing) pb.turn = (uint32_t)turn;
+        if ((rc = run_pass(h, pb))) return rc;
+        f = base_fin(h, opsB & (OP_MOMA | OP_MOMD | OP_LOSS | OP_ZSTAT), slot_of(h, turn), -1);
+    }
     if (wake && h->wake_mode == 1) {
         // fixed grid: moments and wake in one launch
         TimedScope ts(h, "post");
@@ -1224,6 +1270,8 @@ int cts_turn(cts_handle *h, int32_t turn)
     int rc = check_ready(h);
     if (rc) return rc;
     if (h->secs.size() != 1) return fail(CTS_ERR_STATE, "cts_turn fuses one section; use the seam calls");
+    if (h->bl_on && (turn < 0 || turn >= h->bl.nturns))
+        return fail(CTS_ERR_ARG, "cts_turn: turn outside the beam-loading table (cts_set_beam_loading nturns)");
     // a bunch below 50% transmission ends the run, as the reference exits
     if ((rc = check_status(h, false))) return rc;
     if ((rc = turn_body(h, turn))) return rc;
@@ -1295,6 +1343,8 @@ int cts_run_turns(cts_handle *h, int32_t first, int32_t nturns, int32_t use_grap
     if (rc) return rc;
     if (h->secs.size() != 1) return fail(CTS_ERR_STATE, "cts_run_turns fuses one section; use the seam calls");
     if (nturns < 0 || first < 0) return fail(CTS_ERR_ARG, "cts_run_turns: first, nturns >= 0");
+    if (h->bl_on && first + nturns > h->bl.nturns)
+        return fail(CTS_ERR_ARG, "cts_run_turns: turns beyond the beam-loading table (cts_set_beam_loading nturns)");
     int t = first;
     const int end = first + nturns;
     while (t < end) {
@@ -1575,6 +1625,54 @@ int cts_set_lrwake(cts_handle *h, const cts_lrwake_config *cfg)
     h->lrw = std::move(tab);
     h->lrw_T = cfg->nturns;
     h->lr_pending = false;
+    return CTS_OK;
+}
+
+int cts_set_beam_loading(cts_handle *h, const cts_beam_loading_config *cfg)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!cfg || cfg->nres < 0 || cfg->nres > CTS_MAX_RF) return fail(CTS_ERR_ARG, "cts_set_beam_loading: nres 0..4");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    graph_reset(h);
+    free_bl_state(h);
+    if (cfg->nres == 0) return CTS_OK;
+    if (h->secs.size() != 1) return fail(CTS_ERR_STATE, "cts_set_beam_loading: the fused turn (one section) only");
+    if (cfg->nbins < 1 || cfg->nturns < 1) return fail(CTS_ERR_ARG, "cts_set_beam_loading: nbins, nturns >= 1");
+    for (int i = 0; i < cfg->nres; i++) {
+        if (cfg->rf_mode[i] != 0 && cfg->rf_mode[i] != 1) return fail(CTS_ERR_ARG, "cts_set_beam_loading: rf_mode 0 or 1");
+        if (!(cfg->q0[i] > 0) || cfg->res_harm[i] < 1 || !(cfg->coupling[i] >= 0))
+            return fail(CTS_ERR_ARG, "cts_set_beam_loading: Q0 > 0, harmonic >= 1, coupling >= 0");
+        if (cfg->rf_mode[i] == 1) {
+            if (cfg->excite_instability[i] != 0)
+                return fail(CTS_ERR_ARG, "cts_set_beam_loading: the bin-by-bin variant (rfResExciteIntability 1) "
+                                         "is not built");
+            if (cfg->dir_fb[i] != 0)
+                return fail(CTS_ERR_ARG, "cts_set_beam_loading: the direct cavity feedback is not built");
+            if (!(cfg->rs[i] > 0)) return fail(CTS_ERR_ARG, "cts_set_beam_loading: Rs > 0");
+        }
+    }
+    if (cfg->nres == 1 && !(std::fabs(h->ring.u0 / cfg->vol[0]) <= 1))
+        return fail(CTS_ERR_ARG, "cts_set_beam_loading: one cavity needs |U0 / V| <= 1");
+    const double bytes = (double)cfg->nturns * h->nb * CTS_MAX_RF * 3 * 8;
+    if (bytes > 4e9) return fail(CTS_ERR_ARG, "cts_set_beam_loading: table above 4 GB (fewer turns per call)");
+    std::string err;
+    BlTable t;
+    if ((rc = beam_loading_tables(*cfg, h->ring, h->n_e, h->gap, t, err))) return fail(rc, err);
+    if ((rc = dalloc(&h->d_bl_tab, t.dev.size()))) return rc;
+    HIPCHK(hipMemcpy(h->d_bl_tab, t.dev.data(), t.dev.size() * sizeof(double), hipMemcpyHostToDevice));
+    h->bl = std::move(t);
+    h->bl_on = true;
+    return CTS_OK;
+}
+
+int cts_get_cavity_info(cts_handle *h, int32_t turn, double *info)
+{
+    if (!h || !info) return fail(CTS_ERR_ARG, "null argument");
+    if (!h->bl_on) return fail(CTS_ERR_STATE, "cts_get_cavity_info: no beam loading");
+    if (turn < 0 || turn >= h->bl.nturns) return fail(CTS_ERR_ARG, "cts_get_cavity_info: turn outside the table");
+    const size_t n = (size_t)h->nb * h->bl.nres * 8;
+    std::memcpy(info, h->bl.info.data() + (size_t)turn * n, n * sizeof(double));
     return CTS_OK;
 }
 

@@ -34,6 +34,7 @@ enum : uint32_t {
     OP_MOMD = 1u << 14,  // z/dp moments at the end of the pass (no wake)
     OP_LRC = 1u << 15,   // post-map centroids of the long-range wake
     OP_LRK = 1u << 16,   // the previous turn's long-range wake kick (deferred)
+    OP_BL = 1u << 17,    // beam-loaded cavities (rfMode 1), after the ideal ones
 };
 
 // per-segment partial sums, stored value-major: seg_part[j * nseg + s]
@@ -90,6 +91,14 @@ struct PassParams {
     uint64_t seed;
     const float2 *ntab;  // normal tables (NormalTab layout: sc then lg)
     const double *lr_kick;  // [nb][3] long-range wake kick (px, py, dp) of OP_LRK
+    // beam loading (OP_BL): per turn, bunch and cavity {Re V, Im V, Re V_b0};
+    // the bunch's z range after the map (zrange, unpadded); kfac = 2 pi h
+    // f_rf / (c beta) per cavity
+    const double *bl_tab;
+    const double *zrange;
+    int32_t bl_nbl, bl_nbins, bl_nturns, nb;
+    double bl_kfac[CTS_MAX_RF];
+    double bl_energy, bl_beta2;
 };
 
 struct IonKickParams {
@@ -233,6 +242,17 @@ struct LrwTable {
     std::vector<double> time, w, rw, bbr;
 };
 int lrwake_table(const cts_lrwake_config &c, double t0, int harmonic, LrwTable &tab, std::string &err);
+
+// Host: beam-loaded cavities, tabulated per turn (cts_bl.cpp).
+struct BlTable {
+    int nturns = 0, nbl = 0, nbins = 0, nres = 0;
+    std::vector<int> bl_index;   // cavity index of each beam-loaded cavity
+    std::vector<double> kfac;    // 2 pi h f_rf / (c beta) of each
+    std::vector<double> dev;     // [nturns][nb][nbl][3]: Re V, Im V, Re V_b0
+    std::vector<double> info;    // [nturns][nb][nres][8]
+};
+int beam_loading_tables(const cts_beam_loading_config &c, const cts_ring &ring, const std::vector<double> &ne,
+                        const std::vector<int32_t> &gap, BlTable &t, std::string &err);
 void lrwake_yokoya(double rx, double ry, double out[5]);
 
 struct WakeTableParams {

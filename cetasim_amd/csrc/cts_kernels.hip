@@ -233,6 +233,17 @@ k_pass(PassParams P)
             for (int j = tid; j < P.nbins; j += kBlock) kick_lds[j] = P.kick_prev[(int64_t)b * P.nbins + j];
         if (use_hist)
             for (int j = tid; j < P.nbins; j += kBlock) hist_lds[j] = 0u;
+        // beam loading: the cavities' voltages this bunch sees this turn and
+        // its bins over the z range after the map (GetZMinMax's padding)
+        const double *blv = nullptr;
+        double bl_zmax = 0, bl_dz = 1;
+        if (ops & OP_BL) {
+            const int bt = (int)turn < P.bl_nturns ? (int)turn : P.bl_nturns - 1;
+            blv = P.bl_tab + ((int64_t)bt * P.nb + b) * P.bl_nbl * 3;
+            bl_zmax = P.zrange[2 * b + 1] + 1.e-6;
+            const double zmin = P.zrange[2 * b] - 1.e-6;
+            bl_dz = (bl_zmax - zmin) / P.bl_nbins;
+        }
         // the bunch's long-range wake kick of the previous turn (uniform)
         double lrk0 = 0, lrk1 = 0, lrk2 = 0;
         if (ops & OP_LRK) {
@@ -343,6 +354,19 @@ k_pass(PassParams P)
                     else sincos_fast_k(th, s, c, K);
                     const double re = K[CI_RFVR + r] * c - K[CI_RFVI + r] * s;
                     X[5] += re * K[CI_RFSCALE];
+                }
+            }
+            if ((ops & OP_BL) && alive) {
+                // MPBunch::BunchMomentumUpdateDueToRFModeStable
+                // (src/MPBunch.cpp:1338-1375): the bin centre, head first
+                int kb = (int)floor((bl_zmax - X[4]) / bl_dz);
+                kb = kb < 0 ? 0 : (kb >= P.bl_nbins ? P.bl_nbins - 1 : kb);
+                const double zc = bl_zmax - kb * bl_dz - bl_dz / 2.0;
+                for (int q = 0; q < P.bl_nbl; q++) {
+                    double sn, cs;
+                    sincos_fast_k(-zc * P.bl_kfac[q], sn, cs, reload(P.cst));
+                    X[5] += (blv[3 * q] * cs - blv[3 * q + 1] * sn) / P.bl_energy / P.bl_beta2;
+                    X[5] += blv[3 * q + 2] / 2.0 / P.bl_energy / P.bl_beta2;
                 }
             }
             if (ops & OP_DRIFT) {
@@ -972,6 +996,11 @@ constexpr uint32_t V_FIX = OP_KICK | OP_MOMB | OP_MAP | OP_SKEW | OP_RF | OP_DRI
                            OP_BIN | OP_LOSS | OP_MOMA | OP_MOMD | OP_WRITE | OP_LRC | OP_LRK;
 constexpr uint32_t V_REF = OP_KICK | OP_MOMB | OP_MAP | OP_ZSTAT | OP_SKEW | OP_RF | OP_DRIFT | OP_U0 |
                            OP_SYNRAD | OP_LOSS | OP_MOMA | OP_MOMD | OP_WRITE | OP_LRC | OP_LRK;
+// the two passes of a turn with beam-loaded cavities: up to the section
+// map (z range of the RF stage), then from the RF stage on
+constexpr uint32_t V_BLA = OP_KICK | OP_MOMB | OP_MAP | OP_ZSTAT | OP_SKEW | OP_WRITE | OP_LRC | OP_LRK;
+constexpr uint32_t V_BLB = OP_RF | OP_BL | OP_DRIFT | OP_U0 | OP_SYNRAD | OP_BIN | OP_ZSTAT | OP_LOSS | OP_MOMA |
+                           OP_MOMD | OP_WRITE;
 constexpr uint32_t V_ALL = 0xffffffffu;
 
 // stage sets of the fused turn in its steady state
@@ -996,6 +1025,9 @@ hipError_t launch_pass(const PassParams &p, int nwg, hipStream_t st)
     else if (one_rf && o == T_NOSR)
         hipLaunchKernelGGL((k_pass<T_NOSR, true, 0, 1>), g, b, shm, st, p);
     else if ((o & ~V_KICK) == 0) hipLaunchKernelGGL((k_pass<V_KICK, false, -1, -1>), g, b, shm, st, p);
+    else if ((o & OP_BL) && (o & ~V_BLB) == 0) hipLaunchKernelGGL((k_pass<V_BLB, false, -1, -1>), g, b, shm, st, p);
+    else if ((o & OP_MAP) && (o & ~V_BLA) == 0 && !(o & OP_RF))
+        hipLaunchKernelGGL((k_pass<V_BLA, false, -1, -1>), g, b, shm, st, p);
     else if ((o & ~V_MOM) == 0) hipLaunchKernelGGL((k_pass<V_MOM, false, -1, -1>), g, b, shm, st, p);
     else if ((o & ~V_FIX) == 0) hipLaunchKernelGGL((k_pass<V_FIX, false, -1, -1>), g, b, shm, st, p);
     else if ((o & ~V_REF) == 0) hipLaunchKernelGGL((k_pass<V_REF, false, -1, -1>), g, b, shm, st, p);

@@ -105,6 +105,28 @@ class Tracker:
         capi.check(self.lib.cts_get_lrwake_kick(self.h, capi.dptr(out)))
         return out.reshape(self.nb, 3)
 
+    def set_beam_loading(self, bl):
+        """bl: dict(nres, mode, type, harm, cold, rs, q0, coupling, detune, vol,
+        phase, nbins, nturns), per-cavity lists; None switches it off."""
+        cfg = capi.BeamLoadingConfig()
+        if bl is not None:
+            cfg.nres = bl["nres"]
+            for i in range(bl["nres"]):
+                cfg.rf_mode[i] = bl["mode"][i]
+                cfg.res_type[i] = bl["type"][i]
+                cfg.res_harm[i] = bl["harm"][i]
+                cfg.cold[i] = bl["cold"][i]
+                cfg.rs[i], cfg.q0[i], cfg.coupling[i] = bl["rs"][i], bl["q0"][i], bl["coupling"][i]
+                cfg.detune_hz[i], cfg.vol[i], cfg.phase[i] = bl["detune"][i], bl["vol"][i], bl["phase"][i]
+            cfg.nbins, cfg.nturns = bl["nbins"], bl["nturns"]
+        capi.check(self.lib.cts_set_beam_loading(self.h, ctypes.byref(cfg)))
+        self.bl_nres = cfg.nres
+
+    def cavity_info(self, turn):
+        out = np.empty(self.nb * self.bl_nres * 8)
+        capi.check(self.lib.cts_get_cavity_info(self.h, turn, capi.dptr(out)))
+        return out.reshape(self.nb, self.bl_nres, 8)
+
     def set_ions(self, cfg):
         """cfg: capi.IonConfig, or None to switch ions off."""
         if cfg is None:

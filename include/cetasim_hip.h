@@ -162,6 +162,45 @@ int cts_get_lrwake_table(cts_handle *h, int32_t *np, double *time, double *w, do
 /* The last turn's long-range kick of every bunch: kick[nb][3] (px, py, delta). */
 int cts_get_lrwake_kick(cts_handle *h, double *kick);
 
+/* Beam loading of rfMode-1 cavities (transient, bunch by bunch):
+ * CavityResonator::Initial (src/CavityResonator.cpp:36-176), the initial
+ * beam-induced and generator voltages of MPBeam::InitialcavityResonator
+ * (src/MPBeam.cpp:236-465), the cavity's evolution over the train
+ * (Resonator::ResonatorDynamics / GetBeamInducedVol,
+ * src/Resonator.cpp:205-261, driven by MPBeam::BeamMomtumUpdateDueToRFTest,
+ * src/MPBeam.cpp:2624-2706) and the kick of
+ * MPBunch::BunchMomentumUpdateDueToRFModeStable (src/MPBunch.cpp:1323-1417):
+ * survivors binned over the bunch's z range after the section map get
+ * delta += Re(V exp(-i z_bin 2 pi h f_rf / (c beta))) / (E beta^2) +
+ * Re(V_b0) / 2 / (E beta^2).  The cavity voltages do not depend on the
+ * particles in this model, so they are tabulated on the host for turns
+ * 0 .. nturns-1.  The ring's rf_* fields keep the rfMode-0 cavities; this
+ * call lists every cavity in input order (rfMode-0 entries only enter the
+ * cavity set-up).  excite_instability and dir_fb must be 0 (the bin-by-bin
+ * variant and the direct feedback are not built).  nres = 0 switches it off. */
+typedef struct {
+    int32_t nres;
+    int32_t rf_mode[CTS_MAX_RF];      /* 0 ideal, 1 beam loading */
+    int32_t res_type[CTS_MAX_RF];     /* rfResType: 0 passive, 1 active */
+    int32_t res_harm[CTS_MAX_RF];     /* rfResHarm */
+    double rs[CTS_MAX_RF];            /* rfResShuntImpRs [Ohm] */
+    double q0[CTS_MAX_RF];            /* rfResQualityQ0 */
+    double coupling[CTS_MAX_RF];      /* rfResCouplingBeta */
+    double detune_hz[CTS_MAX_RF];     /* rfResDetuneFre [Hz] */
+    double vol[CTS_MAX_RF];           /* rfResVol [V] */
+    double phase[CTS_MAX_RF];         /* rfResPhase [rad] (single cavity: set from U0) */
+    int32_t cold[CTS_MAX_RF];         /* rfResCold */
+    int32_t excite_instability[CTS_MAX_RF];
+    int32_t dir_fb[CTS_MAX_RF];
+    int32_t nbins;                    /* rfBunchBinNum */
+    int32_t nturns;
+} cts_beam_loading_config;
+int cts_set_beam_loading(cts_handle *h, const cts_beam_loading_config *cfg);
+/* What bunch b saw from every cavity at turn t (the reference's
+ * bunchRFModeInfo): info[nb][nres][8] = cavity, beam-induced, generator
+ * and self-loss voltages (re, im); ideal cavities give zeros here. */
+int cts_get_cavity_info(cts_handle *h, int32_t turn, double *info);
+
 /* Radiation damping switch and quantum-excitation noise source
  * (BunchSynRadDamping, src/Bunch.cpp:940-1044).  noise_mode 0: damping
  * only; 1: host-supplied normals (cts_push_noise) for parity runs;

@@ -267,6 +267,13 @@ int orc_lrw_table(int nturns, int harmonic, double t0, int nsec, const double *s
 int orc_lrw_kick(int nturns, int nb, const int *harm, int harmonic, double t0, double rbeta, double energy,
                  const double *ne, const double *pos, const double *w, int np, double *kick);
 
+/* ------------------------------------------------------ beam loading
+ * (beamload.c) rfMode-1 cavities, the reference's stable variant. */
+int orc_bl_tables(int nres, const int *ip, const double *dp, int harmonic, double f0, double t0, double u0,
+                  double rbeta, int nb, const double *ne, const int *gap, int nturns, double *dev, double *info);
+void orc_bl_kick(orc_bunch *b, double vr, double vi, double vb0, double zmin, double zmax, int nbins, int res_harm,
+                 double f_rf, double rbeta, double energy);
+
 int orc_cbm_analysis(const double *mean, int ns, int nb, const int *harm, int harmonics, const double *tunes,
                      const double *twiss, int interval, int fit_start, int fit_end, double t0,
                      double *ideal_amp, double *ideal_arg, double *iq_amp, double *iq_arg, double *ana_re,

@@ -71,6 +71,10 @@ def lib():
         L.orc_faddeeva_w.argtypes = [D, D, PD, PD]
         I = ctypes.c_int
         L.orc_lrw_yokoya.argtypes = [D, D, PD]
+        PIc = ctypes.POINTER(ctypes.c_int)
+        L.orc_bl_tables.restype = I
+        L.orc_bl_tables.argtypes = [I, PIc, PD, I, D, D, D, D, I, PD, PIc, I, PD, PD]
+        L.orc_bl_kick.argtypes = [ctypes.POINTER(OrcBunch), D, D, D, D, D, I, I, D, D, D]
         L.orc_lrw_table.restype = I
         L.orc_lrw_table.argtypes = [I, I, D, I, PD, I, PD, D, D, I, PD, PD, PD, PD]
         L.orc_lrw_kick.restype = I
@@ -183,7 +187,7 @@ class OracleTurn:
     with the bin edges of the reference (mode 0) or a fixed grid (mode 1)."""
 
     def __init__(self, ring, sec, nbins=0, mode=1, grid=(0, 0), bbr=None, macro_charge=1.0,
-                 synrad=False, noise_seed=None, base=0):
+                 synrad=False, noise_seed=None, base=0, bl=None):
         self.L = lib()
         self.ring, self.sec = ring, sec
         self.map = make_map(sec, ring)
@@ -202,6 +206,8 @@ class OracleTurn:
                 self.table = np.zeros(2 * nbins - 1)
                 self.L.orc_wake_table(nbins, dz / 299792458.0, len(self.rs), p(self.rs), p(self.qq), p(self.om),
                                       p(self.table))
+        # beam-loaded cavities: callable(turn) -> [(vr, vi, vb0, harm)], nbins
+        self.bl = bl
         self.b1h1 = np.array(sec.b1h1[:]); self.h1b1 = np.array(sec.h1b1[:])
         self.tau = np.array(ring.synrad_tau[:]); self.en = np.array(ring.emit_nat[:])
 
@@ -220,9 +226,18 @@ class OracleTurn:
         cen = (m.mean[0], m.mean[2], m.mean[4])
         if R.skew_k != 0:
             L.orc_skew_quad(ctypes.byref(b.s), R.skew_k)
+        if self.bl is not None:
+            # GetZMinMax at the RF stage (src/MPBeam.cpp:2637, 2716-2752)
+            zmin, zmax = ctypes.c_double(), ctypes.c_double()
+            L.orc_zminmax(ctypes.byref(b.s), ctypes.byref(zmin), ctypes.byref(zmax))
         for r in range(R.n_rf):
             L.orc_rf_kick(ctypes.byref(b.s), R.rf_vr[r], R.rf_vi[r], R.rf_harm[r], R.f0 * R.harmonic, R.rbeta,
                           R.energy)
+        if self.bl is not None:
+            cavs, nb_bins = self.bl(turn)
+            for vr, vi, vb0, harm in cavs:
+                L.orc_bl_kick(ctypes.byref(b.s), vr, vi, vb0, zmin.value, zmax.value, nb_bins, harm,
+                              R.f0 * R.harmonic, R.rbeta, R.energy)
         L.orc_drift(ctypes.byref(b.s), R.circ, p(np.array(R.alphac[:])))
         L.orc_energy_loss(ctypes.byref(b.s), R.u0, R.energy, R.rbeta)
         if self.synrad:
@@ -286,8 +301,27 @@ class OracleTrain:
     interaction point 0 (src/MPBeam.cpp:1059-1102, 2585-2611): moments of
     every bunch, passages in train order, then each bunch's own turn."""
 
-    def __init__(self, ring, sec, counts, n_e, gap, ion_cfg, seed, lrw=None, **turn_kw):
+    def __init__(self, ring, sec, counts, n_e, gap, ion_cfg, seed, lrw=None, bl=None, **turn_kw):
         self.L = lib()
+        # beam loading: dict(nres, mode, type, harm, cold, rs, q0, coupling, detune, vol, phase, nbins, nturns)
+        self.bl_dev = self.bl_info = None
+        if bl is not None:
+            nb, nres = len(counts), bl["nres"]
+            ip = np.ascontiguousarray(np.concatenate([bl[k] for k in ("mode", "type", "harm", "cold")]), np.int32)
+            dp = np.ascontiguousarray(np.concatenate([np.asarray(bl[k], float) for k in
+                                                      ("rs", "q0", "coupling", "detune", "vol", "phase")]))
+            nbl = int(np.sum(np.asarray(bl["mode"]) == 1))
+            dev = np.zeros(bl["nturns"] * nb * nbl * 3)
+            info = np.zeros(bl["nturns"] * nb * nres * 8)
+            gi = np.ascontiguousarray(gap, np.int32)
+            PIc = ctypes.POINTER(ctypes.c_int)
+            r = self.L.orc_bl_tables(nres, ip.ctypes.data_as(PIc), p(dp), ring.harmonic, ring.f0, ring.t0, ring.u0,
+                                     ring.rbeta, nb, p(np.ascontiguousarray(n_e, float)), gi.ctypes.data_as(PIc),
+                                     bl["nturns"], p(dev), p(info))
+            assert r == nbl
+            self.bl_dev = dev.reshape(bl["nturns"], nb, nbl, 3)
+            self.bl_info = info.reshape(bl["nturns"], nb, nres, 8)
+            bl_harm = [bl["harm"][i] for i in range(nres) if bl["mode"][i] == 1]
         # long-range wake: dict(nturns, sectors [n][7], res [n][9], beta_x, beta_y)
         self.lrw = None
         if lrw is not None:
@@ -310,7 +344,11 @@ class OracleTrain:
         self.gap = np.asarray(gap, dtype=np.int32)
         self.cloud = IonCloud(ion_cfg, ring.circ, ring.harmonic, len(counts)) if ion_cfg is not None else None
         self.seed = seed
-        self.turns = [OracleTurn(ring, sec, base=int(self.offsets[b]), **{**turn_kw, **(
+        def bl_of(b):
+            if self.bl_dev is None:
+                return None
+            return lambda t: ([(*self.bl_dev[t, b, q], bl_harm[q]) for q in range(len(bl_harm))], bl["nbins"])
+        self.turns = [OracleTurn(ring, sec, base=int(self.offsets[b]), bl=bl_of(b), **{**turn_kw, **(
             {"macro_charge": turn_kw["macro_charge"][b]} if "macro_charge" in turn_kw else {})})
             for b in range(len(counts))]
         self.last_cloud = None
