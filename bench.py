@@ -66,6 +66,9 @@ def parse():
     ap.add_argument("--lrwake", type=int, default=0, metavar="NTURNS",
                     help="c3/c5: add the long-range wake (lrwNTurnsWakeTrunction = NTURNS - 1; a 2304 m round "
                          "resistive-wall pipe and the workload's resonator)")
+    ap.add_argument("--beam-loading", action="store_true",
+                    help="c3/c5: the cavity beam-loaded (rfMode 1: active, Rs 5 MOhm, Q0 2e4, coupling 3, "
+                         "detuned -20 kHz, 60 bins per bunch) in place of the ideal one")
     ap.add_argument("--sur-steps", type=int, default=20000, help="c3: Adam steps of the surrogate training")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reads MIB, writes MIB) three times, "
@@ -222,6 +225,8 @@ def train_turns(a, sur=None, mode=0):
     p = configs.BASE
     T = workload_train(a)
     ring, secs, _ = configs.lattice(p)
+    if a.beam_loading:
+        ring.n_rf = 0  # the beam-loaded cavity replaces the ideal one
     counts = configs.train_counts(T["nb"], T["count_range"][0], T["count_range"][1], T["seed"])
     nb = len(counts)
     gap = configs.train_gaps(nb, T["spacing"], ring.harmonic)
@@ -244,10 +249,14 @@ def train_turns(a, sur=None, mode=0):
         res = [[bbr["rs"][r], bbr["q"][r], bbr["fr"][r], 0.0, 1.0, bbr["fr"][r], 0.0, 1.0, bbr["fr"][r]]
                for r in range(len(bbr["rs"]))]
         tr.set_lrwake(a.lrwake, [[0.01, 0.01, 2304.0, 2.7, 4.25, 1, 3.5e7]], res, secs[0].beta_x, secs[0].beta_y)
+    if a.beam_loading:
+        tr.set_beam_loading(dict(nres=1, mode=[1], type=[1], harm=[1], cold=[0], rs=[5e6], q0=[2e4],
+                                 coupling=[3.0], detune=[-2e4], vol=[configs.rf_voltage(p)], phase=[0.0],
+                                 nbins=60, nturns=a.warmup + 2 * a.steps))
     if sur is not None:
         tr.set_field_model(sur, mode)
     tr.autotune()
-    dt, dt_ev, ktimes = timed_turns(tr, a, ("pass", "post", "finalize", "ion_prep", "ion_turn", "ion_compact",
+    dt, dt_ev, ktimes = timed_turns(tr, a, ("pass", "pass_a", "post", "finalize", "ion_prep", "ion_turn", "ion_compact",
                                             "ion_reduce", "ion_finish", "ion_kick", "lrwake"))
     ktimes = {k: v for k, v in ktimes.items() if v[1] > 0}
     cloud = tr.ion_passages()
@@ -310,7 +319,9 @@ def run_train(a):
                                "domain when value_path names it), excitation on (device Philox)"
                                + (f", BBR wake on {T['nbins']} fixed bins per bunch" if T.get("nbins") else "")
                                + (f", long-range wake over {a.lrwake} turns (2304 m round resistive-wall pipe "
-                                  "and the resonator)" if a.lrwake else ""),
+                                  "and the resonator)" if a.lrwake else "")
+                               + (", beam-loaded cavity (rfMode 1, stable model, 60 bins per bunch; the turn "
+                                  "runs as two passes)" if a.beam_loading else ""),
                    "macroparticles": n, "bunches": nb, "parallelism": "single GPU",
                    "launch": "eager" if a.eager else "HIP graph replays of two turns"},
         "ms_per_step_kernel_timing_run_plain": dt_ev / a.steps * 1e3,

@@ -172,6 +172,18 @@ int dump(const std::string &dir, ReadInputSettings &in, LatticeInterActionPoint 
                      c.lrw_beta[0], c.lrw_beta[1]);
         arr("lrw_sectors", c.lrw_sectors.data(), c.lrw_sectors.size());
         arr("lrw_res", c.lrw_res.data(), c.lrw_res.size());
+        const cts_beam_loading_config &b = c.blc;
+        std::fprintf(f, "\"bl\": %d, \"bl_nres\": %d, \"bl_nbins\": %d, \"bl_nturns\": %d,\n", c.bl, b.nres,
+                     b.nbins, b.nturns);
+        std::fprintf(f, "\"bl_int\": [");
+        for (int i = 0; i < b.nres; i++)
+            std::fprintf(f, "%s[%d, %d, %d, %d, %d, %d]", i ? ", " : "", b.rf_mode[i], b.res_type[i], b.res_harm[i],
+                         b.cold[i], b.excite_instability[i], b.dir_fb[i]);
+        std::fprintf(f, "],\n\"bl_real\": [");
+        for (int i = 0; i < b.nres; i++)
+            std::fprintf(f, "%s[%.17g, %.17g, %.17g, %.17g, %.17g, %.17g]", i ? ", " : "", b.rs[i], b.q0[i],
+                         b.coupling[i], b.detune_hz[i], b.vol[i], b.phase[i]);
+        std::fprintf(f, "],\n");
         std::fprintf(f, "\"workQz\": %.17g, \"betaZ\": %.17g, \"tbt\": \"%s\", \"cbmgr\": \"%s\"\n}\n",
                      in.ringParBasic->workQz, in.ringBunchPara->rmsBunchLength / in.ringBunchPara->rmsEnergySpread,
                      R->TBTBunchAverData.c_str(), R->runCBMGR.c_str());

@@ -106,10 +106,29 @@ int ReadInputSettings::ParamReadFile(const std::string &path)
                 rf_.rfMode.assign(rf_.resNum, 0);
                 rf_.resVol.assign(rf_.resNum, 0.0);
                 rf_.resPhase.assign(rf_.resNum, 0.0);
+                rf_.resType.assign(rf_.resNum, 0);
+                rf_.resCold.assign(rf_.resNum, 0);
+                rf_.resExciteIntability.assign(rf_.resNum, 0);
+                rf_.resDirFB.assign(rf_.resNum, 0);
+                rf_.resShuntImpRs.assign(rf_.resNum, 0.0);
+                rf_.resQualityQ0.assign(rf_.resNum, 0.0);
+                rf_.resCouplingBeta.assign(rf_.resNum, 0.0);
+                rf_.resDetuneFre.assign(rf_.resNum, 0.0);
             } else if (k == "rfresharm") for (int i = 0; i < rf_.resNum; i++) rf_.resHarm[i] = inum(v, i);
             else if (k == "rfmode") for (int i = 0; i < rf_.resNum; i++) rf_.rfMode[i] = inum(v, i);
             else if (k == "rfresvol") for (int i = 0; i < rf_.resNum; i++) rf_.resVol[i] = num(v, i);
             else if (k == "rfresphase") for (int i = 0; i < rf_.resNum; i++) rf_.resPhase[i] = num(v, i);
+            else if (k == "rfrestype") for (int i = 0; i < rf_.resNum; i++) rf_.resType[i] = inum(v, i);
+            else if (k == "rfresshuntimprs") for (int i = 0; i < rf_.resNum; i++) rf_.resShuntImpRs[i] = num(v, i);
+            else if (k == "rfresqualityq0") for (int i = 0; i < rf_.resNum; i++) rf_.resQualityQ0[i] = num(v, i);
+            else if (k == "rfrescouplingbeta") for (int i = 0; i < rf_.resNum; i++) rf_.resCouplingBeta[i] = num(v, i);
+            else if (k == "rfresdetunefre") for (int i = 0; i < rf_.resNum; i++) rf_.resDetuneFre[i] = num(v, i);
+            else if (k == "rfrescold") for (int i = 0; i < rf_.resNum; i++) rf_.resCold[i] = inum(v, i);
+            else if (k == "rfresexciteintability")
+                for (int i = 0; i < rf_.resNum; i++) rf_.resExciteIntability[i] = (int)num(v, i);
+            else if (k == "rfresdirfb") for (int i = 0; i < rf_.resNum; i++) rf_.resDirFB[i] = (int)num(v, i);
+            else if (k == "rfbunchbinnum") rf_.rfBunchBinNum = (int)num(v, 0);
+            else if (k == "rftransientcavparwriteto") rf_.transResonParWriteTo = v.at(0);
             // filling pattern (sized by fptrainnumber)
             else if (k == "fptotbunchnumber") fill_.totBunchNumber = inum(v, 0);
             else if (k == "fptrainnumber") {
@@ -570,24 +589,27 @@ int MPBeam::DeviceRing(const LatticeInterActionPoint &lat, const ReadInputSettin
     if (check(cts_derive_ring(&ri, &ring, &der), "cts_derive_ring")) return 1;
     // cavities (CavityResonator::Initial, src/CavityResonator.cpp:60-139):
     // one cavity takes the phase that restores U0, several keep their
-    // input phases; only ideal (rfMode 0) cavities are on this path
+    // input phases.  The ring lists the ideal (rfMode 0) cavities; the
+    // beam-loaded ones (rfMode 1) are set up by cts_set_beam_loading.
     const auto *RF = in.ringParRf;
     if (RF->resNum > CTS_MAX_RF) {
         std::cerr << "at most " << CTS_MAX_RF << " cavities" << std::endl;
         return 1;
     }
     for (int i = 0; i < RF->resNum; i++)
-        if (RF->rfMode[i] != 0) {
-            std::cerr << "cavity " << i << ": rfMode " << RF->rfMode[i]
-                      << " (beam loading) is not supported; use rfMode 0" << std::endl;
+        if (RF->rfMode[i] != 0 && RF->rfMode[i] != 1) {
+            std::cerr << "cavity " << i << ": rfMode " << RF->rfMode[i] << " (0 ideal, 1 beam loading)" << std::endl;
             return 1;
         }
+    if (RF->resNum == 1 && RF->rfMode[0] == 1) ring.n_rf = 0;
     if (RF->resNum > 1) {
-        ring.n_rf = RF->resNum;
+        ring.n_rf = 0;
         for (int i = 0; i < RF->resNum; i++) {
-            ring.rf_vr[i] = RF->resVol[i] * std::cos(RF->resPhase[i]);
-            ring.rf_vi[i] = RF->resVol[i] * std::sin(RF->resPhase[i]);
-            ring.rf_harm[i] = RF->resHarm[i];
+            if (RF->rfMode[i] != 0) continue;
+            ring.rf_vr[ring.n_rf] = RF->resVol[i] * std::cos(RF->resPhase[i]);
+            ring.rf_vi[ring.n_rf] = RF->resVol[i] * std::sin(RF->resPhase[i]);
+            ring.rf_harm[ring.n_rf] = RF->resHarm[i];
+            ring.n_rf++;
         }
     }
     return 0;
@@ -600,11 +622,15 @@ int MPBeam::Initial(Train &train, LatticeInterActionPoint &lat, ReadInputSetting
 
     cts_ring ring{};
     if (DeviceRing(lat, in, ring)) return 1;
-    cavity_.clear();
-    for (int j = 0; j < ring.n_rf; j++) {
-        cavity_.push_back(ring.rf_vr[j]);
-        cavity_.push_back(ring.rf_vi[j]);
-    }
+    // the ideal cavities' phasors in input order (zeros for beam-loaded ones)
+    const auto *RF = in.ringParRf;
+    cavity_.assign(2 * (size_t)RF->resNum, 0.0);
+    for (int j = 0, k = 0; j < RF->resNum; j++)
+        if (RF->rfMode[j] == 0 && k < ring.n_rf) {
+            cavity_[2 * j] = ring.rf_vr[k];
+            cavity_[2 * j + 1] = ring.rf_vi[k];
+            k++;
+        }
     if (check(cts_create(&ring, lat.sections.data(), (int32_t)lat.sections.size(), &gpu), "cts_create")) return 1;
     std::vector<int64_t> counts(nb);
     std::vector<double> q(nb), ne(nb);
@@ -799,8 +825,10 @@ int MPBeam::GetCBMGR(ReadInputSettings &in, std::vector<double> &rates)
 
 // One page of TBTBunchAverData.sdds (MPBeam::MPBeamDataPrintPerTurn,
 // src/MPBeam.cpp:2319-2411): the moments of every bunch at the end of turn
-// `turn`, the ion charge its passage saw, and the cavity phasors.
+// `turn`, the ion charge its passage saw, and the cavity phasors
+// (cavInfo: cts_get_cavity_info of that turn when beam loading is on).
 void MPBeam::PrintTBTBunchAverData(std::ofstream &fout, int turn, const cts_moments *m, const double *ionCharge,
+                                   const double *cavInfo,
                                    const ReadInputSettings &in, bool header)
 {
     const auto *RF = in.ringParRf;
@@ -855,6 +883,16 @@ void MPBeam::PrintTBTBunchAverData(std::ofstream &fout, int turn, const cts_mome
         // ideal cavities: the generator phasor, which is also the cavity
         // voltage, at the bunch's mean z; no beam-induced part
         for (int j = 0; j < RF->resNum; j++) {
+            if (cavInfo && RF->rfMode[j] == 1) {
+                // beam-loaded: cavity, generator and beam-induced voltages
+                // the bunch saw (BunchMomentumUpdateDueToRFModeStable)
+                const double *f = cavInfo + ((size_t)i * RF->resNum + j) * 8;
+                const std::complex<double> cav(f[0], f[1]), ind(f[2], f[3]), gen(f[4], f[5]);
+                w(std::abs(cav)); w(std::arg(cav)); w(cav.real());
+                w(std::abs(gen)); w(std::arg(gen));
+                w(std::abs(ind)); w(std::arg(ind));
+                continue;
+            }
             const std::complex<double> req(cavity_[2 * j], cavity_[2 * j + 1]);
             const double ph = -m[i].mean[4] / kClight / rBeta * 2. * kPi * (double)RF->resHarm[j] * fRF;
             const std::complex<double> v = req * std::exp(std::complex<double>(0.0, ph));
@@ -900,6 +938,34 @@ int MPBeam::RunSettings(const LatticeInterActionPoint &lat, const ReadInputSetti
         c.lrw_beta[0] = lw.betaFunIntPoint[0];
         c.lrw_beta[1] = lw.betaFunIntPoint[1];
         c.lrw_output = in.ringLRWake->lrwOutput;
+    }
+    // rfMode-1 cavities: the reference's stable beam-loading model
+    // (MPBeam::BeamMomtumUpdateDueToRFTest, src/MPBeam.cpp:2624-2706)
+    const auto *RF = in.ringParRf;
+    for (int i = 0; i < RF->resNum; i++) c.bl = c.bl || RF->rfMode[i] == 1;
+    if (c.bl) {
+        if (lat.sections.size() != 1) {
+            std::cerr << "beam loading (rfMode 1): one interaction point (fused turn) only" << std::endl;
+            return 1;
+        }
+        cts_beam_loading_config &b = c.blc;
+        b.nres = RF->resNum;
+        for (int i = 0; i < RF->resNum; i++) {
+            b.rf_mode[i] = RF->rfMode[i];
+            b.res_type[i] = RF->resType[i];
+            b.res_harm[i] = RF->resHarm[i];
+            b.rs[i] = RF->resShuntImpRs[i];
+            b.q0[i] = RF->resQualityQ0[i];
+            b.coupling[i] = RF->resCouplingBeta[i];
+            b.detune_hz[i] = RF->resDetuneFre[i];
+            b.vol[i] = RF->resVol[i];
+            b.phase[i] = RF->resPhase[i];
+            b.cold[i] = RF->resCold[i];
+            b.excite_instability[i] = RF->resExciteIntability[i];
+            b.dir_fb[i] = RF->resDirFB[i];
+        }
+        b.nbins = RF->rfBunchBinNum;
+        b.nturns = R->nTurns;
     }
     c.synrad = R->synRadDampingFlag != 0;
     c.seed = R->seed;
@@ -994,6 +1060,7 @@ int MPBeam::Run(Train &, LatticeInterActionPoint &lat, ReadInputSettings &in)
         if (check(cts_set_lrwake(gpu, &lc), "long-range wake")) return 1;
         if (!c.lrw_output.empty() && PrintLRWakeTable(c.lrw_output)) return 1;
     }
+    if (c.bl && check(cts_set_beam_loading(gpu, &c.blc), "beam loading")) return 1;
     const int interval = std::max(R->bunchInfoPrintInterval, 1);
     samples_.clear();
     // output files start empty, as the reference removes *.sdds at set-up
@@ -1047,10 +1114,14 @@ int MPBeam::Run(Train &, LatticeInterActionPoint &lat, ReadInputSettings &in)
         }
     }
     if (check(cts_sync(gpu), "sync")) return 1;
-    if (tbt.is_open())
-        for (int s = 0; s < nsamp; s++)
+    if (tbt.is_open()) {
+        std::vector<double> cav(c.bl ? (size_t)nb * in.ringParRf->resNum * 8 : 0);
+        for (int s = 0; s < nsamp; s++) {
+            if (c.bl && check(cts_get_cavity_info(gpu, s * interval, cav.data()), "cavity info")) return 1;
             PrintTBTBunchAverData(tbt, s * interval, samples_.data() + (size_t)s * nb, ionq.data() + (size_t)s * nb,
-                                  in, s == 0);
+                                  c.bl ? cav.data() : nullptr, in, s == 0);
+        }
+    }
     if (MPBeamRMSCal(0)) return 1;
     // GetCBMGR fits and writes its file at the turn with turn + interval ==
     // nTurns (src/MPBeam.cpp:1651), i.e. when nTurns is a multiple of the

@@ -39,6 +39,11 @@ public:
         int ringHarm = 0, resNum = 0;
         std::vector<int> resHarm, rfMode;
         std::vector<double> resVol, resPhase;
+        // beam-loaded (rfMode 1) cavities (src/ReadInputSettings.cpp:194-368)
+        std::vector<int> resType, resCold, resExciteIntability, resDirFB;
+        std::vector<double> resShuntImpRs, resQualityQ0, resCouplingBeta, resDetuneFre;
+        int rfBunchBinNum = 0;
+        std::string transResonParWriteTo;
     };
     struct RingFillPatt {
         int totBunchNumber = 1, trainNumber = 1;
@@ -172,6 +177,8 @@ struct RunCalls {
     std::vector<double> lrw_sectors, lrw_res;  // [n][7], [n][9] (cts_lrwake_config's layout)
     double lrw_beta[2] = {0, 0};
     std::string lrw_output;
+    bool bl = false;  // rfMode-1 cavities present
+    cts_beam_loading_config blc{};
 };
 
 // The beam of bunches and the per-turn seams of MPBeam::Run
@@ -213,6 +220,7 @@ public:
     // rates[3][nb] gets the ideal-mode rates
     int GetCBMGR(ReadInputSettings &inputParameter, std::vector<double> &rates);
     void PrintTBTBunchAverData(std::ofstream &fout, int turn, const cts_moments *m, const double *ionCharge,
+                               const double *cavInfo,
                                const ReadInputSettings &inputParameter, bool header);
     // the long-range wake table file of InitialLRWake (src/WakeFunction.cpp:
     // 100-155) from the library's table

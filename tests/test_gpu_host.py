@@ -28,6 +28,8 @@ PIPE = ("radiusX radiusY length betaX betaY num matSigma\n"
         "0.03 0.012 300 5 8 2 1.4e6\n")
 LRW_EDITS = {"runLongRangeWake": "1", "LRWPipeGeoInput": "pipe.dat", "LRWBBRInput": "bbr.dat",
              "LRWNTurnsWakeTrunction": "3", "LRWOutput": "lrw.sdds"}
+BL_EDITS = {"rfMode": "1", "rfResType": "1", "rfResShuntImpRs": "5.0e6", "rfResQualityQ0": "2.0e4",
+            "rfResCouplingBeta": "3.0", "rfResDetuneFre": "-2.0e4", "rfResCold": "0", "rfBunchBinNum": "60"}
 
 
 def run_program(d, edits=None):
@@ -52,6 +54,11 @@ def program_run(tmp_path_factory):
 @pytest.fixture(scope="module")
 def program_run_lrw(tmp_path_factory):
     return run_program(tmp_path_factory.mktemp("run_lrw"), LRW_EDITS)
+
+
+@pytest.fixture(scope="module")
+def program_run_bl(tmp_path_factory):
+    return run_program(tmp_path_factory.mktemp("run_bl"), BL_EDITS)
 
 
 def read_sdds_pages(path):
@@ -84,6 +91,15 @@ def test_program_with_lrwake_matches_capi_run(program_run_lrw):
     check_against_capi(d, D)
 
 
+def test_program_with_beam_loading_matches_capi_run(program_run_bl):
+    """The same with the cavity beam-loaded (rfMode 1, active, detuned):
+    the cavity, generator and beam-induced columns of the turn-by-turn file
+    against the library's cavity tables."""
+    d, D = program_run_bl
+    assert D["settings"]["bl"] == 1 and D["ring"].n_rf == 0
+    check_against_capi(d, D)
+
+
 def check_against_capi(d, D):
     s = D["settings"]
     nb, interval, nturns = s["nb"], s["interval"], s["nTurns"]
@@ -105,6 +121,12 @@ def check_against_capi(d, D):
             row = [float(x) for x in rows[1 + i]]
             assert row[0] == g(-t[i])
             assert row[1:] == [g(v) for v in np.concatenate([rw[:, i], bbr[:, i], w[:, i]])]
+    if s.get("bl"):
+        nres = s["bl_nres"]
+        bi, br = np.array(s["bl_int"]), np.array(s["bl_real"])
+        tr.set_beam_loading(dict(nres=nres, mode=bi[:, 0], type=bi[:, 1], harm=bi[:, 2], cold=bi[:, 3],
+                                 rs=br[:, 0], q0=br[:, 1], coupling=br[:, 2], detune=br[:, 3], vol=br[:, 4],
+                                 phase=br[:, 5], nbins=s["bl_nbins"], nturns=s["bl_nturns"]))
     tr.set_history(nturns)
     turns = list(range(0, nturns, interval))
     ionq = []
@@ -134,7 +156,12 @@ def check_against_capi(d, D):
             exp = [ionq[i][b], mm.mean[0], mm.mean[2], mm.mean[4], mm.mean[1], mm.mean[3], mm.mean[5],
                    der.emit[0], der.emit[1], der.emit[2], der.rms_x, der.rms_y, der.sigma_z, der.sigma_dp]
             np.testing.assert_allclose([float(x) for x in row[3:17]], exp, rtol=2e-15, atol=0)
-            assert len(row) == 17 + 7  # one ideal cavity: 7 phasor columns
+            assert len(row) == 17 + 7  # one cavity: 7 phasor columns
+            if s.get("bl"):
+                f = tr.cavity_info(t)[b, 0]
+                cav, ind, gen = complex(f[0], f[1]), complex(f[2], f[3]), complex(f[4], f[5])
+                exp = [abs(cav), np.angle(cav), cav.real, abs(gen), np.angle(gen), abs(ind), np.angle(ind)]
+                np.testing.assert_allclose([float(x) for x in row[17:24]], exp, rtol=2e-15, atol=0)
     # runCBMGR.sdds: parameters and per-sample columns of every mode, printed
     # with six significant digits
     allm = (capi.Moments * (len(turns) * nb))()

@@ -222,13 +222,39 @@ def test_lrwake_settings(tmp_path):
 
 
 def test_rf_modes(tmp_path):
-    # rfMode 1 (beam loading) is refused; several ideal cavities keep their
-    # input phases (CavityResonator::Initial, src/CavityResonator.cpp:60-139)
-    H.copy_fixture(str(tmp_path), {"rfMode": "1"})
-    d = tmp_path / "d1"
+    # rfMode 1 (beam loading): the cavity keys reach cts_set_beam_loading
+    # (src/ReadInputSettings.cpp:194-368) and the ring keeps no ideal cavity
+    BL = {"rfMode": "1", "rfResType": "1", "rfResShuntImpRs": "5.0e6", "rfResQualityQ0": "2.0e4",
+          "rfResCouplingBeta": "3.0", "rfResDetuneFre": "-2.0e4", "rfResCold": "0", "rfBunchBinNum": "60"}
+    H.copy_fixture(str(tmp_path), BL)
+    d = tmp_path / "d0"
     d.mkdir()
     r = H.run(["--dump", str(d), "input.dat"], str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    dump = H.load_dump(str(d))
+    s = dump["settings"]
+    assert s["bl"] == 1 and s["bl_nres"] == 1 and s["bl_nbins"] == 60 and s["bl_nturns"] == s["nTurns"]
+    assert s["bl_int"] == [[1, 1, 1, 0, 0, 0]]
+    assert s["bl_real"] == [[5.0e6, 2.0e4, 3.0, -2.0e4, 8.0e6, 1.0]]
+    assert dump["ring"].n_rf == 0
+    # an unknown mode is refused; a mixed pair keeps the ideal one in the ring
+    H.copy_fixture(str(tmp_path), {"rfMode": "2"})
+    r = H.run(["--dump", str(d), "input.dat"], str(tmp_path))
     assert r.returncode != 0 and "rfmode" in r.stderr.lower()
+    H.copy_fixture(str(tmp_path), {"rfResNum": "2", "rfMode": "1 0", "rfResHarm": "1 3",
+                                   "rfResVol": "8.0E+6 2.0E+6", "rfResPhase": "1.0 -1.6", "rfResType": "1 0",
+                                   "rfResShuntImpRs": "5e6 1e6", "rfResQualityQ0": "2e4 1e4",
+                                   "rfResCouplingBeta": "3 0", "rfResDetuneFre": "-2e4 1e5", "rfResCold": "0 0"})
+    r = H.run(["--dump", str(d), "input.dat"], str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    dump = H.load_dump(str(d))
+    assert dump["settings"]["bl_int"] == [[1, 1, 1, 0, 0, 0], [0, 0, 3, 0, 0, 0]]
+    ring = dump["ring"]
+    assert ring.n_rf == 1 and ring.rf_harm[0] == 3
+    np.testing.assert_allclose([ring.rf_vr[0], ring.rf_vi[0]], [2e6 * math.cos(-1.6), 2e6 * math.sin(-1.6)],
+                               rtol=1e-15)
+    # several ideal cavities keep their input phases
+    # (CavityResonator::Initial, src/CavityResonator.cpp:60-139)
     H.copy_fixture(str(tmp_path), {"rfResNum": "2", "rfMode": "0 0", "rfResHarm": "1 3",
                                    "rfResVol": "8.0E+6 2.0E+6", "rfResPhase": "1.0 -1.6"})
     d = tmp_path / "d2"
