@@ -321,7 +321,7 @@ def run_train(a):
                                + (f", long-range wake over {a.lrwake} turns (2304 m round resistive-wall pipe "
                                   "and the resonator)" if a.lrwake else "")
                                + (", beam-loaded cavity (rfMode 1, stable model, 60 bins per bunch; the turn "
-                                  "runs as two passes)" if a.beam_loading else ""),
+                                  "adds a read-only pass for the bunch extent)" if a.beam_loading else ""),
                    "macroparticles": n, "bunches": nb, "parallelism": "single GPU",
                    "launch": "eager" if a.eager else "HIP graph replays of two turns"},
         "ms_per_step_kernel_timing_run_plain": dt_ev / a.steps * 1e3,

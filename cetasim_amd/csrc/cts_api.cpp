@@ -1217,33 +1217,25 @@ static int turn_body(cts_handle *h, int32_t turn)
     else ops |= OP_MOMD;  // no deferred kick: dp moments close this turn
     if (h->lr_pending) ops |= OP_LRK;
     if (h->lrw_T > 0) ops |= OP_LRC;
-    FinParams f;
-    if (!h->bl_on) {
-        PassParams p = base_pass(h, ops, 0);
-        if (!h->capturing) p.turn = (uint32_t)turn;
-        if ((rc = run_pass(h, p))) return rc;
-        // finalize: dp part of the previous record (MOMB), this turn's record
-        f = base_fin(h, ops & (OP_MOMB | OP_MOMA | OP_MOMD | OP_LOSS | OP_ZSTAT | OP_LRC), slot_of(h, turn),
-                     slot_of(h, h->last_turn));
-    } else {
+    if (h->bl_on) {
         // beam-loaded cavities bin every bunch over its z range after the
-        // section map (src/MPBeam.cpp:2637, 2716-2752), so the turn runs
-        // as two passes: up to the map (and the skew quad), then from the
-        // RF stage on
-        const uint32_t pre = OP_KICK | OP_MOMB | OP_MAP | OP_SKEW | OP_LRK | OP_LRC;
-        const uint32_t opsA = (ops & pre) | OP_ZSTAT | OP_WRITE;
-        uint32_t opsB = (ops & ~pre) | OP_BL;
-        if (wake && h->wake_mode == 1) opsB &= ~OP_ZSTAT;
+        // section map (src/MPBeam.cpp:2637, 2716-2752): a read-only pass
+        // runs the stages up to the map (and the skew quad) for that range
+        // alone, then the turn's own pass repeats them (same operations on
+        // the same inputs, same values) and applies the kick
+        const uint32_t opsA = (ops & (OP_KICK | OP_LRK | OP_MAP | OP_SKEW)) | OP_ZSTAT;
         PassParams pa = base_pass(h, opsA, 0);
         if (!h->capturing) pa.turn = (uint32_t)turn;
         if ((rc = run_pass(h, pa, "pass_a"))) return rc;
-        FinParams fa = base_fin(h, opsA & (OP_MOMB | OP_ZSTAT | OP_LRC), -1, slot_of(h, h->last_turn));
-        if ((rc = run_finalize(h, fa))) return rc;
-        PassParams pb = base_pass(h, opsB, 0);
-        if (!h->capturing) pb.turn = (uint32_t)turn;
-        if ((rc = run_pass(h, pb))) return rc;
-        f = base_fin(h, opsB & (OP_MOMA | OP_MOMD | OP_LOSS | OP_ZSTAT), slot_of(h, turn), -1);
+        if ((rc = run_finalize(h, base_fin(h, OP_ZSTAT, -1, -1)))) return rc;
+        ops |= OP_BL;
     }
+    PassParams p = base_pass(h, ops, 0);
+    if (!h->capturing) p.turn = (uint32_t)turn;
+    if ((rc = run_pass(h, p))) return rc;
+    // finalize: dp part of the previous record (MOMB), this turn's record
+    FinParams f = base_fin(h, ops & (OP_MOMB | OP_MOMA | OP_MOMD | OP_LOSS | OP_ZSTAT | OP_LRC), slot_of(h, turn),
+                           slot_of(h, h->last_turn));
     if (wake && h->wake_mode == 1) {
         // fixed grid: moments and wake in one launch
         TimedScope ts(h, "post");
@@ -1639,6 +1631,14 @@ int cts_set_beam_loading(cts_handle *h, const cts_beam_loading_config *cfg)
     if (cfg->nres == 0) return CTS_OK;
     if (h->secs.size() != 1) return fail(CTS_ERR_STATE, "cts_set_beam_loading: the fused turn (one section) only");
     if (cfg->nbins < 1 || cfg->nturns < 1) return fail(CTS_ERR_ARG, "cts_set_beam_loading: nbins, nturns >= 1");
+    {
+        // the per-bin kicks of a bunch sit in LDS next to the wake's bins
+        int nbl = 0;
+        for (int i = 0; i < cfg->nres && i < CTS_MAX_RF; i++) nbl += cfg->rf_mode[i] == 1;
+        if ((int64_t)nbl * (cfg->nbins + 1) > kBlMaxLds)
+            return fail(CTS_ERR_ARG, "cts_set_beam_loading: beam-loaded cavities x (nbins + 1) above " +
+                                         std::to_string(kBlMaxLds));
+    }
     for (int i = 0; i < cfg->nres; i++) {
         if (cfg->rf_mode[i] != 0 && cfg->rf_mode[i] != 1) return fail(CTS_ERR_ARG, "cts_set_beam_loading: rf_mode 0 or 1");
         if (!(cfg->q0[i] > 0) || cfg->res_harm[i] < 1 || !(cfg->coupling[i] >= 0))

@@ -297,6 +297,7 @@ hipError_t launch_ion_turn(const IonTurnParams &p, int nspec, int64_t wave_bound
 hipError_t launch_ion_finish(const IonFinishParams &p, hipStream_t st);
 // wave_bound: a host bound of the segment's waves
 hipError_t launch_ion_compact(const IonCompactParams &p, int64_t wave_bound, hipStream_t st);
+constexpr int kBlMaxLds = 4096;  // beam-loaded cavities x (bins + 1): the per-bin kicks in LDS (32 KB)
 constexpr int kIonTurnMaxBlocks = 4096;  // grid bound of an ion segment or compaction (blocks stride over the waves)
 #ifndef CTS_ION_REPLICAS
 #define CTS_ION_REPLICAS 8

@@ -204,6 +204,7 @@ k_pass(PassParams P)
     extern __shared__ double smem_d[];
     double *kick_lds = smem_d;                                            // nbins
     uint32_t *hist_lds = reinterpret_cast<uint32_t *>(smem_d + P.nbins);  // nbins
+    double *bl_lds = smem_d + P.nbins + (P.nbins + 1) / 2;                // bl_nbl * (bl_nbins + 1)
     __shared__ double red[kBlock / 64][kNPart];
 
     // EXACT variants run exactly the stages in M (no runtime stage tests);
@@ -235,14 +236,27 @@ k_pass(PassParams P)
             for (int j = tid; j < P.nbins; j += kBlock) hist_lds[j] = 0u;
         // beam loading: the cavities' voltages this bunch sees this turn and
         // its bins over the z range after the map (GetZMinMax's padding)
-        const double *blv = nullptr;
+        // (MPBunch::BunchMomentumUpdateDueToRFModeStable,
+        // src/MPBunch.cpp:1338-1375); the kick of every bin centre, head
+        // first, goes to LDS: bl_lds[q][k] for cavity q, then the self-loss
+        // term of each cavity
         double bl_zmax = 0, bl_dz = 1;
         if (ops & OP_BL) {
             const int bt = (int)turn < P.bl_nturns ? (int)turn : P.bl_nturns - 1;
-            blv = P.bl_tab + ((int64_t)bt * P.nb + b) * P.bl_nbl * 3;
+            const double *blv = P.bl_tab + ((int64_t)bt * P.nb + b) * P.bl_nbl * 3;
             bl_zmax = P.zrange[2 * b + 1] + 1.e-6;
             const double zmin = P.zrange[2 * b] - 1.e-6;
             bl_dz = (bl_zmax - zmin) / P.bl_nbins;
+            const int nk = P.bl_nbl * P.bl_nbins;
+            for (int j = tid; j < nk; j += kBlock) {
+                const int q = j / P.bl_nbins, kb = j - q * P.bl_nbins;
+                const double zc = bl_zmax - kb * bl_dz - bl_dz / 2.0;
+                double sn, cs;
+                sincos_fast_k(-zc * P.bl_kfac[q], sn, cs, reload(P.cst));
+                bl_lds[j] = (blv[3 * q] * cs - blv[3 * q + 1] * sn) / P.bl_energy / P.bl_beta2;
+            }
+            for (int q = tid; q < P.bl_nbl; q += kBlock)
+                bl_lds[nk + q] = blv[3 * q + 2] / 2.0 / P.bl_energy / P.bl_beta2;
         }
         // the bunch's long-range wake kick of the previous turn (uniform)
         double lrk0 = 0, lrk1 = 0, lrk2 = 0;
@@ -357,16 +371,14 @@ k_pass(PassParams P)
                 }
             }
             if ((ops & OP_BL) && alive) {
-                // MPBunch::BunchMomentumUpdateDueToRFModeStable
-                // (src/MPBunch.cpp:1338-1375): the bin centre, head first
+                // the particle's bin, head first (clamped: the padded range
+                // holds every survivor)
                 int kb = (int)floor((bl_zmax - X[4]) / bl_dz);
                 kb = kb < 0 ? 0 : (kb >= P.bl_nbins ? P.bl_nbins - 1 : kb);
-                const double zc = bl_zmax - kb * bl_dz - bl_dz / 2.0;
+                const int nk = P.bl_nbl * P.bl_nbins;
                 for (int q = 0; q < P.bl_nbl; q++) {
-                    double sn, cs;
-                    sincos_fast_k(-zc * P.bl_kfac[q], sn, cs, reload(P.cst));
-                    X[5] += (blv[3 * q] * cs - blv[3 * q + 1] * sn) / P.bl_energy / P.bl_beta2;
-                    X[5] += blv[3 * q + 2] / 2.0 / P.bl_energy / P.bl_beta2;
+                    X[5] += bl_lds[q * P.bl_nbins + kb];
+                    X[5] += bl_lds[nk + q];
                 }
             }
             if (ops & OP_DRIFT) {
@@ -996,11 +1008,10 @@ constexpr uint32_t V_FIX = OP_KICK | OP_MOMB | OP_MAP | OP_SKEW | OP_RF | OP_DRI
                            OP_BIN | OP_LOSS | OP_MOMA | OP_MOMD | OP_WRITE | OP_LRC | OP_LRK;
 constexpr uint32_t V_REF = OP_KICK | OP_MOMB | OP_MAP | OP_ZSTAT | OP_SKEW | OP_RF | OP_DRIFT | OP_U0 |
                            OP_SYNRAD | OP_LOSS | OP_MOMA | OP_MOMD | OP_WRITE | OP_LRC | OP_LRK;
-// the two passes of a turn with beam-loaded cavities: up to the section
-// map (z range of the RF stage), then from the RF stage on
-constexpr uint32_t V_BLA = OP_KICK | OP_MOMB | OP_MAP | OP_ZSTAT | OP_SKEW | OP_WRITE | OP_LRC | OP_LRK;
-constexpr uint32_t V_BLB = OP_RF | OP_BL | OP_DRIFT | OP_U0 | OP_SYNRAD | OP_BIN | OP_ZSTAT | OP_LOSS | OP_MOMA |
-                           OP_MOMD | OP_WRITE;
+// a turn with beam-loaded cavities: the read-only pass up to the section
+// map (z range of the RF stage), then the turn's pass with the kick
+constexpr uint32_t V_BLA = OP_KICK | OP_MAP | OP_ZSTAT | OP_SKEW | OP_LRK;
+constexpr uint32_t V_BLB = V_FIX | V_REF | OP_BL;
 constexpr uint32_t V_ALL = 0xffffffffu;
 
 // stage sets of the fused turn in its steady state
@@ -1009,24 +1020,29 @@ constexpr uint32_t T_QE = OP_KICK | OP_MOMB | OP_MAP | OP_RF | OP_DRIFT | OP_U0 
 constexpr uint32_t T_NOSR = OP_KICK | OP_MOMB | OP_MAP | OP_RF | OP_DRIFT | OP_U0 | OP_BIN | OP_LOSS |
                             OP_MOMA | OP_WRITE;
 constexpr uint32_t T_QE_LR = T_QE | OP_LRC | OP_LRK;  // with the long-range wake
+constexpr uint32_t T_QE_BL = T_QE | OP_BL;            // a beam-loaded cavity in place of the ideal one
 
 hipError_t launch_pass(const PassParams &p, int nwg, hipStream_t st)
 {
-    const size_t shm = (size_t)p.nbins * (sizeof(double) + sizeof(uint32_t));
     const uint32_t o = p.ops;
+    const size_t shm = (o & OP_BL) ? ((size_t)p.nbins + (p.nbins + 1) / 2 + (size_t)p.bl_nbl * (p.bl_nbins + 1)) *
+                                         sizeof(double)
+                                   : (size_t)p.nbins * (sizeof(double) + sizeof(uint32_t));
     const dim3 g(nwg), b(kBlock);
     const bool one_rf = p.n_rf == 1;
     if (one_rf && o == T_QE && p.noise_mode == 2)
         hipLaunchKernelGGL((k_pass<T_QE, true, 2, 1>), g, b, shm, st, p);
     else if (one_rf && o == T_QE_LR && p.noise_mode == 2)
         hipLaunchKernelGGL((k_pass<T_QE_LR, true, 2, 1>), g, b, shm, st, p);
+    else if (p.n_rf == 0 && o == T_QE_BL && p.noise_mode == 2)
+        hipLaunchKernelGGL((k_pass<T_QE_BL, true, 2, 0>), g, b, shm, st, p);
     else if (one_rf && o == T_QE && p.noise_mode == 1)
         hipLaunchKernelGGL((k_pass<T_QE, true, 1, 1>), g, b, shm, st, p);
     else if (one_rf && o == T_NOSR)
         hipLaunchKernelGGL((k_pass<T_NOSR, true, 0, 1>), g, b, shm, st, p);
     else if ((o & ~V_KICK) == 0) hipLaunchKernelGGL((k_pass<V_KICK, false, -1, -1>), g, b, shm, st, p);
     else if ((o & OP_BL) && (o & ~V_BLB) == 0) hipLaunchKernelGGL((k_pass<V_BLB, false, -1, -1>), g, b, shm, st, p);
-    else if ((o & OP_MAP) && (o & ~V_BLA) == 0 && !(o & OP_RF))
+    else if ((o & OP_MAP) && (o & ~V_BLA) == 0)
         hipLaunchKernelGGL((k_pass<V_BLA, false, -1, -1>), g, b, shm, st, p);
     else if ((o & ~V_MOM) == 0) hipLaunchKernelGGL((k_pass<V_MOM, false, -1, -1>), g, b, shm, st, p);
     else if ((o & ~V_FIX) == 0) hipLaunchKernelGGL((k_pass<V_FIX, false, -1, -1>), g, b, shm, st, p);

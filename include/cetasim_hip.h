@@ -192,7 +192,7 @@ typedef struct {
     int32_t cold[CTS_MAX_RF];         /* rfResCold */
     int32_t excite_instability[CTS_MAX_RF];
     int32_t dir_fb[CTS_MAX_RF];
-    int32_t nbins;                    /* rfBunchBinNum */
+    int32_t nbins;                    /* rfBunchBinNum; (rfMode-1 cavities) x (nbins + 1) <= 4096 */
     int32_t nturns;
 } cts_beam_loading_config;
 int cts_set_beam_loading(cts_handle *h, const cts_beam_loading_config *cfg);

@@ -96,6 +96,9 @@ def test_turn_range_is_checked():
         tr.turn(t)
     with pytest.raises(RuntimeError):
         tr.turn(3)
+    # the per-bin kicks of a bunch must fit in LDS
+    with pytest.raises(RuntimeError):
+        tr.set_beam_loading(cavity(ring, 3, nbins=4096))
     tr.close()
 
 
