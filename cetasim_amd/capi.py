@@ -82,6 +82,11 @@ class LrwakeConfig(ctypes.Structure):
                 ("beta_x", D), ("beta_y", D)]
 
 
+class BbimpConfig(ctypes.Structure):
+    _fields_ = [("nimp", I32), ("f_max", D), ("zz_re", PD), ("zz_im", PD), ("flags", I32 * 5),
+                ("time_domain", I32)]
+
+
 class BeamLoadingConfig(ctypes.Structure):
     _fields_ = [("nres", I32), ("rf_mode", I32 * MAX_RF), ("res_type", I32 * MAX_RF), ("res_harm", I32 * MAX_RF),
                 ("rs", D * MAX_RF), ("q0", D * MAX_RF), ("coupling", D * MAX_RF), ("detune_hz", D * MAX_RF),
@@ -116,6 +121,8 @@ SIGNATURES = {
     "cts_get_lrwake_kick": (I32, [ctypes.c_void_p, PD]),
     "cts_set_beam_loading": (I32, [ctypes.c_void_p, ctypes.POINTER(BeamLoadingConfig)]),
     "cts_get_cavity_info": (I32, [ctypes.c_void_p, I32, PD]),
+    "cts_set_bbimp": (I32, [ctypes.c_void_p, ctypes.POINTER(BbimpConfig)]),
+    "cts_get_bbimp_wake": (I32, [ctypes.c_void_p, ctypes.POINTER(I32), PD, PD]),
     "cts_ion_phase": (I32, [ctypes.c_void_p, I32]),
     "cts_get_ion_cloud": (I32, [ctypes.c_void_p, I32, I64, PI64, PD, PD, PD, PD]),
     "cts_get_ion_passages": (I32, [ctypes.c_void_p, PD]),

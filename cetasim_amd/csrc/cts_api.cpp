@@ -164,6 +164,13 @@ struct cts_handle {
     int32_t *d_lrw_harm = nullptr;      // [nb]
     bool lr_pending = false;
 
+    // broadband impedance (cts_set_bbimp)
+    bool bbi_on = false;
+    BbiTable bbi;
+    uint32_t *d_bbi_hist = nullptr;                 // [nb][n]
+    double *d_bbi_prof = nullptr, *d_bbi_wake = nullptr;  // [nb][n]
+    double *d_bbi_g = nullptr, *d_bbi_scale = nullptr;    // [n], [nb]
+
     // beam-loaded cavities (cts_set_beam_loading)
     bool bl_on = false;
     BlTable bl;
@@ -214,10 +221,37 @@ void free_bl_state(cts_handle *h)
     h->bl = BlTable{};
 }
 
+void free_bbi_state(cts_handle *h)
+{
+    void *ptrs[] = {h->d_bbi_hist, h->d_bbi_prof, h->d_bbi_wake, h->d_bbi_g, h->d_bbi_scale};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    h->d_bbi_hist = nullptr;
+    h->d_bbi_prof = h->d_bbi_wake = h->d_bbi_g = h->d_bbi_scale = nullptr;
+    h->bbi_on = false;
+    h->bbi = BbiTable{};
+}
+
+BbiParams bbi_params(cts_handle *h)
+{
+    BbiParams p{};
+    p.n = h->bbi.n; p.is = h->bbi.is; p.ie = h->bbi.ie; p.nb = h->nb;
+    p.zmin = h->bbi.zmin; p.dz = h->bbi.dz;
+    p.energy = h->ring.energy; p.beta2 = std::pow(h->ring.rbeta, 2);
+    p.seg_begin = h->d_seg_begin; p.seg_end = h->d_seg_end;
+    p.seg_bunch = h->d_seg_bunch; p.wg_seg = h->d_wg_seg;
+    p.z = h->d_z; p.dp = h->d_dp; p.flag = h->d_flag;
+    p.hist = h->d_bbi_hist; p.prof = h->d_bbi_prof; p.wake = h->d_bbi_wake;
+    p.g = h->d_bbi_g; p.scale = h->d_bbi_scale;
+    for (int j = 0; j < kBbiTaps; j++) p.taps[j] = h->bbi.taps[j];
+    return p;
+}
+
 void free_bunch_state(cts_handle *h)
 {
     free_lrw_state(h);
     free_bl_state(h);
+    free_bbi_state(h);
     void *ptrs[] = {h->d_x, h->d_px, h->d_y, h->d_py, h->d_z, h->d_dp, h->d_flag,
                     h->d_seg_begin, h->d_seg_end, h->d_bunch_off, h->d_bunch_count,
                     h->d_seg_bunch, h->d_wg_seg, h->d_bunch_seg, h->d_seg_part, h->d_shift,
@@ -1199,6 +1233,26 @@ static int run_lrwake(cts_handle *h, int32_t turn)
     return CTS_OK;
 }
 
+// MPBeam::BBImpBeamInteraction (src/MPBeam.cpp:1321-1345) after the
+// turn's pass: bin, induced voltage, kick; the delta moments of the turn's
+// record are then completed by the next pass (as with a wake kick)
+int run_bbi(cts_handle *h)
+{
+    const BbiParams p = bbi_params(h);
+    {
+        TimedScope ts(h, "bbi_bin");
+        HIPCHK(launch_bbi_bin(p, h->nwg, h->stream));
+    }
+    {
+        TimedScope ts(h, "bbi_wake");
+        HIPCHK(launch_bbi_wake(p, h->stream));
+    }
+    TimedScope ts(h, "bbi_kick");
+    HIPCHK(launch_bbi_kick(p, h->nwg, h->stream));
+    h->kick_pending = true;
+    return CTS_OK;
+}
+
 // MPBeam::Run body for one section (src/MPBeam.cpp:1059-1102), fused.
 // The kernels of one turn (everything cts_turn enqueues after its status
 // check); also what a graph capture records, twice.
@@ -1214,7 +1268,7 @@ static int turn_body(cts_handle *h, int32_t turn)
     const bool wake = h->nbins > 0;
     if (h->kick_pending) ops |= OP_KICK | OP_MOMB;
     if (wake) ops |= (h->wake_mode == 1) ? OP_BIN : OP_ZSTAT;
-    else ops |= OP_MOMD;  // no deferred kick: dp moments close this turn
+    else if (!h->bbi_on) ops |= OP_MOMD;  // no deferred kick: dp moments close this turn
     if (h->lr_pending) ops |= OP_LRK;
     if (h->lrw_T > 0) ops |= OP_LRC;
     if (h->bl_on) {
@@ -1250,9 +1304,10 @@ static int turn_body(cts_handle *h, int32_t turn)
             h->kick_pending = true;
         }
     }
+    if (h->bbi_on && (rc = run_bbi(h))) return rc;
     if (h->lrw_T > 0 && (rc = run_lrwake(h, turn))) return rc;
     h->last_turn = turn;
-    h->dp_pending = wake;
+    h->dp_pending = wake || h->bbi_on;
     h->cur_valid = true;
     return CTS_OK;
 }
@@ -1344,7 +1399,7 @@ int cts_run_turns(cts_handle *h, int32_t first, int32_t nturns, int32_t use_grap
         // turn was t - 1 and, with a wake, its kick is pending; host noise
         // (pushed per turn) is not replayable
         const bool steady = use_graph && end - t >= 2 && h->cur_valid && h->last_turn == t - 1 &&
-                            h->kick_pending == (h->nbins > 0) && h->lr_pending == (h->lrw_T > 0) &&
+                            h->kick_pending == (h->nbins > 0 || h->bbi_on) && h->lr_pending == (h->lrw_T > 0) &&
                             h->noise_mode != 1;
         if (!steady) {
             if ((rc = cts_turn(h, t))) return rc;
@@ -1394,7 +1449,7 @@ int cts_run_turns(cts_handle *h, int32_t first, int32_t nturns, int32_t use_grap
         h->turn_dev_next = t + 2;
         // the host state after two turns: parities back where they were
         h->last_turn = t + 1;
-        h->dp_pending = h->nbins > 0;
+        h->dp_pending = h->nbins > 0 || h->bbi_on;
         h->lr_pending = h->lrw_T > 0;
         h->cur_valid = true;
         if (h->nspec > 0) h->ion_len_ub = h->ion_cap;
@@ -1617,6 +1672,54 @@ int cts_set_lrwake(cts_handle *h, const cts_lrwake_config *cfg)
     h->lrw = std::move(tab);
     h->lrw_T = cfg->nturns;
     h->lr_pending = false;
+    return CTS_OK;
+}
+
+int cts_set_bbimp(cts_handle *h, const cts_bbimp_config *cfg)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!cfg) return fail(CTS_ERR_ARG, "cts_set_bbimp: null config");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    graph_reset(h);
+    free_bbi_state(h);
+    if (cfg->flags[0] == 0 && cfg->flags[1] == 0 && cfg->flags[2] == 0 && cfg->flags[3] == 0 && cfg->flags[4] == 0)
+        return CTS_OK;  // doTracking = 0 (src/MPBunch.cpp:2152-2157)
+    if (cfg->time_domain != 0)
+        return fail(CTS_ERR_ARG, "cts_set_bbimp: only the frequency-domain method (time_domain 0) is built");
+    if (cfg->flags[1] || cfg->flags[2] || cfg->flags[3] || cfg->flags[4])
+        return fail(CTS_ERR_ARG, "cts_set_bbimp: only the longitudinal plane (flags[0]) is built");
+    if (h->secs.size() != 1) return fail(CTS_ERR_STATE, "cts_set_bbimp: the fused turn (one section) only");
+    std::string err;
+    if ((rc = bbimp_table(*cfg, h->ring, h->bbi, err))) return fail(rc, err);
+    const int n = h->bbi.n, nb = h->nb;
+    if ((rc = dalloc(&h->d_bbi_hist, (size_t)nb * n)) || (rc = dalloc(&h->d_bbi_prof, (size_t)nb * n)) ||
+        (rc = dalloc(&h->d_bbi_wake, (size_t)nb * n)) || (rc = dalloc(&h->d_bbi_g, (size_t)n)) ||
+        (rc = dalloc(&h->d_bbi_scale, (size_t)nb))) {
+        free_bbi_state(h);
+        return rc;
+    }
+    std::vector<double> sc(nb);
+    for (int b = 0; b < nb; b++) sc[b] = h->macro_charge[b] * kEcharge / h->bbi.dz * kClight;
+    HIPCHK(hipMemcpy(h->d_bbi_g, h->bbi.g.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_bbi_scale, sc.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(h->d_bbi_hist, 0, (size_t)nb * n * sizeof(uint32_t)));
+    HIPCHK(hipMemset(h->d_bbi_wake, 0, (size_t)nb * n * sizeof(double)));
+    HIPCHK(hipMemset(h->d_bbi_prof, 0, (size_t)nb * n * sizeof(double)));
+    h->bbi_on = true;
+    return CTS_OK;
+}
+
+int cts_get_bbimp_wake(cts_handle *h, int32_t *n, double *wake, double *prof)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!h->bbi_on) return fail(CTS_ERR_STATE, "cts_get_bbimp_wake: no broadband impedance");
+    if (n) *n = h->bbi.n;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const size_t sz = (size_t)h->nb * h->bbi.n * sizeof(double);
+    if (wake) HIPCHK(hipMemcpy(wake, h->d_bbi_wake, sz, hipMemcpyDeviceToHost));
+    if (prof) HIPCHK(hipMemcpy(prof, h->d_bbi_prof, sz, hipMemcpyDeviceToHost));
     return CTS_OK;
 }
 

@@ -1,0 +1,191 @@
+// cts_bbi.hip -- broadband impedance in the frequency domain, longitudinal
+// plane (MPBunch::BBImpBunchInteraction, src/MPBunch.cpp:2143-2216).
+//
+// The reference bins a bunch on the centred grid of BoardBandImp::ReadInImp
+// (src/BoardBandImp.cpp:71-90: N = 2 nimp - 1 bins of dz = c beta/(2 f_max)
+// from -z_max), smooths the profile over one RF bucket, takes its real FFT,
+// multiplies by Z_k, and returns V = -c2r(...)/N.  With Z extended as a
+// Hermitian sequence (the c2r transform ignores Im Z_0) that is the circular
+// convolution V_j = -sum_i p_i g_{(j - i) mod N} with the real kernel
+//   g_m = (1/N) [Re Z_0 + 2 sum_{k=1}^{nimp-1} Re(Z_k e^{2 pi i k m / N})],
+// computed once on the host.  Per turn: an LDS histogram per workgroup
+// segment, one block per bunch for the smoothing and the convolution, and a
+// kick sweep.  The smoothing is GSL's gsl_filter_gaussian (GSL 2.8, the
+// version the reference builds against) with K = 51, alpha = 10 and
+// GSL_FILTER_END_PADVALUE: y_i = sum_j w_j x_{clamp(i + j - 25)},
+// w_j = exp(-alpha^2 ((j - 25)/25)^2 / 2) normalised to sum 1, followed by
+// the reference's rescaling to the input's sum (src/MPBunch.cpp:2373-2410).
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "cts_internal.hpp"
+
+namespace cts {
+
+int bbimp_table(const cts_bbimp_config &c, const cts_ring &ring, BbiTable &t, std::string &err)
+{
+    if (c.nimp < 2 || 2 * c.nimp - 1 > kBbiMaxBins || !(c.f_max > 0) || !c.zz_re || !c.zz_im) {
+        err = "cts_set_bbimp: nimp in [2, " + std::to_string((kBbiMaxBins + 1) / 2) + "], f_max > 0, Z given";
+        return CTS_ERR_ARG;
+    }
+    const int nimp = c.nimp, n = 2 * nimp - 1;
+    const double dt = 1.0 / (2 * c.f_max);
+    t.n = n;
+    t.dz = dt * kClight * ring.rbeta;
+    const double zmax = t.dz * (nimp - 1);
+    t.zmin = -zmax;
+    // one RF bucket, indexed as GetSmoothedBunchProfileGassionFilter does
+    const double rf_len = ring.t0 / ring.harmonic * kClight;
+    t.is = (int)((-rf_len / 2. - t.zmin + t.dz / 2) / t.dz);
+    t.ie = (int)((rf_len / 2. - t.zmin + t.dz / 2) / t.dz);
+    if (-rf_len / 2. - t.zmin + t.dz / 2 < 0 || t.ie > n || t.ie <= t.is) {
+        err = "cts_set_bbimp: the impedance grid (z_max " + std::to_string(zmax) +
+              " m) must cover one RF bucket (" + std::to_string(rf_len) + " m)";
+        return CTS_ERR_ARG;
+    }
+    t.g.assign(n, 0.0);
+    for (int m = 0; m < n; m++) {
+        double s = 0;
+        for (int k = 1; k < nimp; k++) {
+            // exact reduction of the phase k m / n
+            const double ph = 2 * M_PI * (double)(((int64_t)k * m) % n) / n;
+            s += c.zz_re[k] * std::cos(ph) - c.zz_im[k] * std::sin(ph);
+        }
+        t.g[m] = (c.zz_re[0] + 2 * s) / n;
+    }
+    const int half = (kBbiTaps - 1) / 2;
+    const double alpha = 10.0;
+    double sum = 0;
+    for (int j = 0; j < kBbiTaps; j++) {
+        const double xj = (j - half) / (double)half;
+        t.taps[j] = std::exp(-0.5 * alpha * alpha * xj * xj);
+        sum += t.taps[j];
+    }
+    for (int j = 0; j < kBbiTaps; j++) t.taps[j] /= sum;
+    return CTS_OK;
+}
+
+namespace {
+
+// bin of z as the reference indexes it, (int)((z - zmin + dz/2)/dz), or -1
+// off the grid (the reference does not test the range)
+__device__ __forceinline__ int bbi_bin(double z, const BbiParams &P)
+{
+    const double q = (z - P.zmin + P.dz / 2) / P.dz;
+    return (q > -1.0 && q < (double)P.n) ? (int)q : -1;
+}
+
+// survivors before this turn's loss marking
+__device__ __forceinline__ bool bbi_alive(uint8_t f) { return (f & 3u) == 0u || (f & kNewLost); }
+
+}  // namespace
+
+__global__ void __launch_bounds__(kBlock) k_bbi_bin(BbiParams P)
+{
+    extern __shared__ uint32_t hl[];
+    const int tid = threadIdx.x;
+    for (int s = P.wg_seg[blockIdx.x]; s < P.wg_seg[blockIdx.x + 1]; s++) {
+        const int b = P.seg_bunch[s];
+        if (P.scale[b] == 0.0) continue;  // uniform: the whole block skips
+        for (int j = tid; j < P.n; j += kBlock) hl[j] = 0u;
+        __syncthreads();
+        for (int64_t i = P.seg_begin[s] + tid; i < P.seg_end[s]; i += kBlock) {
+            if (!bbi_alive(P.flag[i])) continue;
+            const int k = bbi_bin(P.z[i], P);
+            if (k >= 0) atomicAdd(&hl[k], 1u);
+        }
+        __syncthreads();
+        for (int j = tid; j < P.n; j += kBlock) {
+            const uint32_t c = hl[j];
+            if (c) atomicAdd(&P.hist[(int64_t)b * P.n + j], c);
+        }
+        __syncthreads();
+    }
+}
+
+// one block per bunch: profile, smoothing over [is, ie), convolution; the
+// histogram row is cleared for the next turn (a fill, no memset node)
+__global__ void __launch_bounds__(kBlock) k_bbi_wake(BbiParams P)
+{
+    __shared__ double red[2][kBlock];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    uint32_t *cnt = P.hist + (int64_t)b * P.n;
+    double *prof = P.prof + (int64_t)b * P.n;
+    double *wake = P.wake + (int64_t)b * P.n;
+    const double sc = P.scale[b];
+    if (sc == 0.0) {
+        for (int j = tid; j < P.n; j += kBlock) { wake[j] = 0.0; prof[j] = 0.0; cnt[j] = 0u; }
+        return;
+    }
+    const int nr = P.ie - P.is, half = (kBbiTaps - 1) / 2;
+    double s0 = 0, s1 = 0;
+    for (int i = tid; i < nr; i += kBlock) {
+        double y = 0;
+        for (int j = 0; j < kBbiTaps; j++) {
+            int k = i + j - half;
+            k = k < 0 ? 0 : (k >= nr ? nr - 1 : k);
+            y += P.taps[j] * ((double)cnt[P.is + k] * sc);
+        }
+        s0 += (double)cnt[P.is + i] * sc;
+        s1 += y;
+        prof[P.is + i] = y;
+    }
+    red[0][tid] = s0;
+    red[1][tid] = s1;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (tid < w) { red[0][tid] += red[0][tid + w]; red[1][tid] += red[1][tid + w]; }
+        __syncthreads();
+    }
+    const double r = red[1][0] != 0.0 ? red[0][0] / red[1][0] : 0.0;
+    for (int i = tid; i < P.n; i += kBlock) {
+        if (i >= P.is && i < P.ie) prof[i] *= r;
+        else prof[i] = (double)cnt[i] * sc;
+    }
+    __syncthreads();
+    for (int j = tid; j < P.n; j += kBlock) {
+        double v = 0;
+        int m = j;  // (j - i) mod n for i = 0, 1, ...
+        for (int i = 0; i < P.n; i++) {
+            v += prof[i] * P.g[m];
+            m = m == 0 ? P.n - 1 : m - 1;
+        }
+        wake[j] = -v;
+    }
+    __syncthreads();
+    for (int j = tid; j < P.n; j += kBlock) cnt[j] = 0u;
+}
+
+__global__ void __launch_bounds__(kBlock) k_bbi_kick(BbiParams P)
+{
+    const int tid = threadIdx.x;
+    for (int s = P.wg_seg[blockIdx.x]; s < P.wg_seg[blockIdx.x + 1]; s++) {
+        const int b = P.seg_bunch[s];
+        if (P.scale[b] == 0.0) continue;
+        const double *wake = P.wake + (int64_t)b * P.n;
+        for (int64_t i = P.seg_begin[s] + tid; i < P.seg_end[s]; i += kBlock) {
+            if (!bbi_alive(P.flag[i])) continue;
+            const int k = bbi_bin(P.z[i], P);
+            if (k >= 0) P.dp[i] += wake[k] / P.energy / P.beta2;
+        }
+    }
+}
+
+hipError_t launch_bbi_bin(const BbiParams &p, int nwg, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_bbi_bin, dim3(nwg), dim3(kBlock), (size_t)p.n * sizeof(uint32_t), st, p);
+    return hipGetLastError();
+}
+hipError_t launch_bbi_wake(const BbiParams &p, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_bbi_wake, dim3(p.nb), dim3(kBlock), 0, st, p);
+    return hipGetLastError();
+}
+hipError_t launch_bbi_kick(const BbiParams &p, int nwg, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_bbi_kick, dim3(nwg), dim3(kBlock), 0, st, p);
+    return hipGetLastError();
+}
+
+}  // namespace cts

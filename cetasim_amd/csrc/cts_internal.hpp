@@ -243,6 +243,35 @@ struct LrwTable {
 };
 int lrwake_table(const cts_lrwake_config &c, double t0, int harmonic, LrwTable &tab, std::string &err);
 
+// Broadband impedance (cts_bbi.hip): the grid, the real circular kernel of
+// Z and the smoothing filter, set up on the host.
+constexpr int kBbiTaps = 51;      // the Gaussian filter's window
+constexpr int kBbiMaxBins = 8191;  // 2 nimp - 1: the per-segment histogram in LDS (32 KB)
+struct BbiTable {
+    int n = 0, is = 0, ie = 0;   // bins; smoothed range [is, ie)
+    double zmin = 0, dz = 0;
+    std::vector<double> g;       // [n] kernel: V = -(profile (*) g)
+    double taps[kBbiTaps];       // normalised Gaussian weights
+};
+int bbimp_table(const cts_bbimp_config &c, const cts_ring &ring, BbiTable &t, std::string &err);
+struct BbiParams {
+    int n, is, ie, nb;
+    double zmin, dz, energy, beta2;
+    const int64_t *seg_begin, *seg_end;
+    const int32_t *seg_bunch, *wg_seg;
+    const double *z;
+    double *dp;
+    const uint8_t *flag;
+    uint32_t *hist;        // [nb][n], zero between turns
+    double *prof, *wake;   // [nb][n]
+    const double *g;       // [n]
+    const double *scale;   // [nb] macro charge e c / dz (0: the bunch is skipped)
+    double taps[kBbiTaps];
+};
+hipError_t launch_bbi_bin(const BbiParams &p, int nwg, hipStream_t st);
+hipError_t launch_bbi_wake(const BbiParams &p, hipStream_t st);
+hipError_t launch_bbi_kick(const BbiParams &p, int nwg, hipStream_t st);
+
 // Host: beam-loaded cavities, tabulated per turn (cts_bl.cpp).
 struct BlTable {
     int nturns = 0, nbl = 0, nbins = 0, nres = 0;

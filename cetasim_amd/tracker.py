@@ -122,6 +122,27 @@ class Tracker:
         capi.check(self.lib.cts_set_beam_loading(self.h, ctypes.byref(cfg)))
         self.bl_nres = cfg.nres
 
+    def set_bbimp(self, zre, zim, f_max, flags=(1, 0, 0, 0, 0), time_domain=0):
+        """Broadband impedance table Z(f_k) (the file's rows, last frequency
+        f_max); zre None switches it off."""
+        cfg = capi.BbimpConfig()
+        if zre is not None:
+            self._bbi_z = (np.ascontiguousarray(zre, np.float64), np.ascontiguousarray(zim, np.float64))
+            cfg.nimp = len(self._bbi_z[0])
+            cfg.f_max = f_max
+            cfg.zz_re, cfg.zz_im = capi.dptr(self._bbi_z[0]), capi.dptr(self._bbi_z[1])
+            cfg.flags[:] = list(flags)
+            cfg.time_domain = time_domain
+        capi.check(self.lib.cts_set_bbimp(self.h, ctypes.byref(cfg)))
+
+    def bbimp_wake(self):
+        """(wake, profile) of every bunch from the last turn: [nb][2 nimp - 1]."""
+        n = capi.I32()
+        capi.check(self.lib.cts_get_bbimp_wake(self.h, ctypes.byref(n), None, None))
+        w, pr = np.empty(self.nb * n.value), np.empty(self.nb * n.value)
+        capi.check(self.lib.cts_get_bbimp_wake(self.h, ctypes.byref(n), capi.dptr(w), capi.dptr(pr)))
+        return w.reshape(self.nb, -1), pr.reshape(self.nb, -1)
+
     def cavity_info(self, turn):
         out = np.empty(self.nb * self.bl_nres * 8)
         capi.check(self.lib.cts_get_cavity_info(self.h, turn, capi.dptr(out)))

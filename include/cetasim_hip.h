@@ -201,6 +201,33 @@ int cts_set_beam_loading(cts_handle *h, const cts_beam_loading_config *cfg);
  * and self-loss voltages (re, im); ideal cavities give zeros here. */
 int cts_get_cavity_info(cts_handle *h, int32_t turn, double *info);
 
+/* Broadband impedance in the frequency domain, longitudinal plane
+ * (MPBunch::BBImpBunchInteraction, src/MPBunch.cpp:2143-2216, with the
+ * impedance table of BoardBandImp::ReadInImp, src/BoardBandImp.cpp:42-90,
+ * and the profile smoothing of GetSmoothedBunchProfileGassionFilter,
+ * src/MPBunch.cpp:2348-2410).  zz[k] = Z_||(f_k) for the file's nimp
+ * frequencies (the reference's zZImp: column 2 - i column 7); f_max is the
+ * last frequency.  Every turn, after the radiation stage, each bunch's
+ * survivors are binned on the centred grid of 2 nimp - 1 bins
+ * (dz = c beta / (2 f_max)), the profile over one RF bucket is smoothed by
+ * a 51-point Gaussian filter (alpha 10, end values padded, sum kept), the
+ * induced voltage -IDFT(DFT(profile) Z) is formed as a circular convolution
+ * with the real kernel of Z, and delta += V(bin) / (E beta^2).  Only
+ * flags[0] (the longitudinal plane) and the frequency-domain method
+ * (time_domain 0) are built; flags[0] = 0 switches it off. */
+typedef struct {
+    int32_t nimp;                     /* rows of the impedance file, >= 2 */
+    double f_max;                     /* its last frequency [Hz] */
+    const double *zz_re, *zz_im;      /* [nimp] Z_|| [Ohm] */
+    int32_t flags[5];                 /* BBIimpedSimLandTFlag: Z, Dx, Dy, Qx, Qy */
+    int32_t time_domain;              /* BBIimpedSimTimeOrFreFlag */
+} cts_bbimp_config;
+int cts_set_bbimp(cts_handle *h, const cts_bbimp_config *cfg);
+/* The induced voltage of every bunch from the last turn: wake[nb][2 nimp - 1]
+ * (bin k at z = -z_max + k dz), and the smoothed profile [C/s] if prof is
+ * not NULL; n receives 2 nimp - 1. */
+int cts_get_bbimp_wake(cts_handle *h, int32_t *n, double *wake, double *prof);
+
 /* Radiation damping switch and quantum-excitation noise source
  * (BunchSynRadDamping, src/Bunch.cpp:940-1044).  noise_mode 0: damping
  * only; 1: host-supplied normals (cts_push_noise) for parity runs;

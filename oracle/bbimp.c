@@ -1,0 +1,112 @@
+/* bbimp.c -- CPU oracle (test infrastructure only) of the broadband
+ * impedance in the frequency domain, longitudinal plane:
+ * MPBunch::BBImpBunchInteraction (src/MPBunch.cpp:2143-2216) with the grid
+ * of BoardBandImp::ReadInImp (src/BoardBandImp.cpp:71-90) and the smoothing
+ * of GetSmoothedBunchProfileGassionFilter (src/MPBunch.cpp:2348-2410).
+ * FFTW's r2c and c2r transforms are written as direct sums (c2r of odd
+ * length n: y_j = Re X_0 + 2 sum_{k=1}^{(n-1)/2} Re(X_k e^{2 pi i j k/n})),
+ * and the GSL 2.8 Gaussian filter (gsl_filter_gaussian, the dependency the
+ * reference links, not part of it) from its published definition: window
+ * K = 51, kernel exp(-alpha^2 x^2/2) on x = (i - 25)/25 scaled to unit sum,
+ * the input padded with its end values.  Pinned in tests/test_bbimp.py by a
+ * constant resistive impedance (V = -R I), a linear reactive one (the
+ * spectral derivative of a cosine), a constant profile (smoothing changes
+ * nothing) and a single spike (it returns the normalised kernel).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+
+#include "cts_oracle.h"
+
+#define PI 3.14159265358979323846
+#define TAPS 51
+
+void orc_bbi_smooth(double *p, int n)
+{
+    const int half = (TAPS - 1) / 2;
+    const double alpha = 10.0;
+    double k[TAPS], sum = 0;
+    for (int i = 0; i < TAPS; i++) {
+        const double xi = ((double)i - 0.5 * (TAPS - 1.0)) / (0.5 * (TAPS - 1.0));
+        k[i] = exp(-0.5 * alpha * alpha * xi * xi);
+        sum += k[i];
+    }
+    for (int i = 0; i < TAPS; i++) k[i] *= 1.0 / sum;
+    double *y = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    for (int i = 0; i < n; i++) {
+        double s = 0;
+        for (int j = 0; j < TAPS; j++) {
+            int q = i + j - half;
+            q = q < 0 ? 0 : (q >= n ? n - 1 : q);
+            s += p[q] * k[j];
+        }
+        y[i] = s;
+    }
+    double s0 = 0, s1 = 0;
+    for (int i = 0; i < n; i++) {
+        s0 += p[i];
+        s1 += y[i];
+        p[i] = y[i];
+    }
+    s0 = s0 / s1;
+    for (int i = 0; i < n; i++) p[i] *= s0;
+    free(y);
+}
+
+/* wake[j] = -c2r(r2c(p) Z)[j] / n, n = 2 nimp - 1 */
+void orc_bbi_potential(const double *p, int nimp, const double *zre, const double *zim, double *wake)
+{
+    const int n = 2 * nimp - 1;
+    double *xr = (double *)malloc(sizeof(double) * nimp), *xi = (double *)malloc(sizeof(double) * nimp);
+    for (int k = 0; k < nimp; k++) {
+        double sr = 0, si = 0;
+        for (int j = 0; j < n; j++) {
+            const double ph = -2 * PI * (double)(((int64_t)j * k) % n) / n;
+            sr += p[j] * cos(ph);
+            si += p[j] * sin(ph);
+        }
+        /* times Z_k */
+        xr[k] = sr * zre[k] - si * zim[k];
+        xi[k] = sr * zim[k] + si * zre[k];
+    }
+    for (int j = 0; j < n; j++) {
+        double s = 0;
+        for (int k = 1; k < nimp; k++) {
+            const double ph = 2 * PI * (double)(((int64_t)j * k) % n) / n;
+            s += xr[k] * cos(ph) - xi[k] * sin(ph);
+        }
+        wake[j] = -(xr[0] + 2 * s) / n;
+    }
+    free(xr);
+    free(xi);
+}
+
+/* One bunch's interaction: survivors binned on the grid
+ * (int)((z + z_max + dz/2)/dz) of 2 nimp - 1 bins, the bucket smoothed, the
+ * induced voltage formed, delta += V/(E beta^2).  wake, prof: [2 nimp - 1]. */
+void orc_bbi_kick(orc_bunch *b, int nimp, double f_max, const double *zre, const double *zim, double macro_charge,
+                  double rbeta, double t0, int harmonic, double energy, double *wake, double *prof)
+{
+    const int n = 2 * nimp - 1;
+    const double dt = 1.0 / (2 * f_max), dz = dt * ORC_CLIGHT * rbeta;
+    const double zmin = -dz * (nimp - 1);
+    int *idx = (int *)malloc(sizeof(int) * (size_t)(b->n > 0 ? b->n : 1));
+    for (int i = 0; i < n; i++) prof[i] = 0;
+    for (int64_t i = 0; i < b->n; i++) {
+        idx[i] = -1;
+        if (b->lost[i] != 0) continue;
+        const double q = (b->z[i] - zmin + dz / 2) / dz;
+        if (!(q > -1.0 && q < (double)n)) continue;
+        idx[i] = (int)q;
+        prof[idx[i]] += macro_charge * ORC_ECHARGE / dz * ORC_CLIGHT;
+    }
+    const double rf_len = t0 / harmonic * ORC_CLIGHT;
+    const int is = (int)((-rf_len / 2. - zmin + dz / 2) / dz);
+    const int ie = (int)((rf_len / 2. - zmin + dz / 2) / dz);
+    orc_bbi_smooth(prof + is, ie - is);
+    orc_bbi_potential(prof, nimp, zre, zim, wake);
+    for (int64_t i = 0; i < b->n; i++)
+        if (idx[i] >= 0) b->dp[i] += wake[idx[i]] / energy / pow(rbeta, 2);
+    free(idx);
+}

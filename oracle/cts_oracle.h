@@ -274,6 +274,12 @@ int orc_bl_tables(int nres, const int *ip, const double *dp, int harmonic, doubl
 void orc_bl_kick(orc_bunch *b, double vr, double vi, double vb0, double zmin, double zmax, int nbins, int res_harm,
                  double f_rf, double rbeta, double energy);
 
+/* Broadband impedance, longitudinal plane (bbimp.c). */
+void orc_bbi_smooth(double *p, int n);
+void orc_bbi_potential(const double *p, int nimp, const double *zre, const double *zim, double *wake);
+void orc_bbi_kick(orc_bunch *b, int nimp, double f_max, const double *zre, const double *zim, double macro_charge,
+                  double rbeta, double t0, int harmonic, double energy, double *wake, double *prof);
+
 int orc_cbm_analysis(const double *mean, int ns, int nb, const int *harm, int harmonics, const double *tunes,
                      const double *twiss, int interval, int fit_start, int fit_end, double t0,
                      double *ideal_amp, double *ideal_arg, double *iq_amp, double *iq_arg, double *ana_re,

@@ -75,6 +75,9 @@ def lib():
         L.orc_bl_tables.restype = I
         L.orc_bl_tables.argtypes = [I, PIc, PD, I, D, D, D, D, I, PD, PIc, I, PD, PD]
         L.orc_bl_kick.argtypes = [ctypes.POINTER(OrcBunch), D, D, D, D, D, I, I, D, D, D]
+        L.orc_bbi_smooth.argtypes = [PD, I]
+        L.orc_bbi_potential.argtypes = [PD, I, PD, PD, PD]
+        L.orc_bbi_kick.argtypes = [ctypes.POINTER(OrcBunch), I, D, PD, PD, D, D, D, I, D, PD, PD]
         L.orc_lrw_table.restype = I
         L.orc_lrw_table.argtypes = [I, I, D, I, PD, I, PD, D, D, I, PD, PD, PD, PD]
         L.orc_lrw_kick.restype = I
@@ -187,7 +190,7 @@ class OracleTurn:
     with the bin edges of the reference (mode 0) or a fixed grid (mode 1)."""
 
     def __init__(self, ring, sec, nbins=0, mode=1, grid=(0, 0), bbr=None, macro_charge=1.0,
-                 synrad=False, noise_seed=None, base=0, bl=None):
+                 synrad=False, noise_seed=None, base=0, bl=None, bbi=None):
         self.L = lib()
         self.ring, self.sec = ring, sec
         self.map = make_map(sec, ring)
@@ -208,6 +211,9 @@ class OracleTurn:
                                       p(self.table))
         # beam-loaded cavities: callable(turn) -> [(vr, vi, vb0, harm)], nbins
         self.bl = bl
+        # broadband impedance: dict(nimp, f_max, zre, zim, q = this bunch's macro charge)
+        self.bbi = bbi
+        self.bbi_wake = self.bbi_prof = None
         self.b1h1 = np.array(sec.b1h1[:]); self.h1b1 = np.array(sec.h1b1[:])
         self.tau = np.array(ring.synrad_tau[:]); self.en = np.array(ring.emit_nat[:])
 
@@ -252,6 +258,15 @@ class OracleTurn:
         """Short-range wake and loss marking."""
         L, R = self.L, self.ring
         sig_stale = self._sig_stale
+        if self.bbi is not None:
+            # BBImpBeamInteraction, after the radiation stage (src/MPBeam.cpp:1084)
+            c = self.bbi
+            n = 2 * c["nimp"] - 1
+            self.bbi_wake, self.bbi_prof = np.zeros(n), np.zeros(n)
+            zre = np.ascontiguousarray(c["zre"], float)
+            zim = np.ascontiguousarray(c["zim"], float)
+            L.orc_bbi_kick(ctypes.byref(b.s), c["nimp"], c["f_max"], p(zre), p(zim), c["q"], R.rbeta, R.t0,
+                           R.harmonic, R.energy, p(self.bbi_wake), p(self.bbi_prof))
         if self.nbins:
             tab = self.table
             L.orc_sr_wake(ctypes.byref(b.s), self.nbins, self.mode, sig_stale, self.grid[0], self.grid[1],
@@ -301,7 +316,7 @@ class OracleTrain:
     interaction point 0 (src/MPBeam.cpp:1059-1102, 2585-2611): moments of
     every bunch, passages in train order, then each bunch's own turn."""
 
-    def __init__(self, ring, sec, counts, n_e, gap, ion_cfg, seed, lrw=None, bl=None, **turn_kw):
+    def __init__(self, ring, sec, counts, n_e, gap, ion_cfg, seed, lrw=None, bl=None, bbi=None, **turn_kw):
         self.L = lib()
         # beam loading: dict(nres, mode, type, harm, cold, rs, q0, coupling, detune, vol, phase, nbins, nturns)
         self.bl_dev = self.bl_info = None
@@ -348,7 +363,8 @@ class OracleTrain:
             if self.bl_dev is None:
                 return None
             return lambda t: ([(*self.bl_dev[t, b, q], bl_harm[q]) for q in range(len(bl_harm))], bl["nbins"])
-        self.turns = [OracleTurn(ring, sec, base=int(self.offsets[b]), bl=bl_of(b), **{**turn_kw, **(
+        self.turns = [OracleTurn(ring, sec, base=int(self.offsets[b]), bl=bl_of(b),
+                                 bbi=None if bbi is None else dict(bbi, q=bbi["q"][b]), **{**turn_kw, **(
             {"macro_charge": turn_kw["macro_charge"][b]} if "macro_charge" in turn_kw else {})})
             for b in range(len(counts))]
         self.last_cloud = None
