@@ -69,6 +69,9 @@ def parse():
     ap.add_argument("--beam-loading", action="store_true",
                     help="c3/c5: the cavity beam-loaded (rfMode 1: active, Rs 5 MOhm, Q0 2e4, coupling 3, "
                          "detuned -20 kHz, 60 bins per bunch) in place of the ideal one")
+    ap.add_argument("--bbimp", action="store_true",
+                    help="c3/c5: add the broadband impedance (longitudinal; a broadband resonator Rs 2 kOhm, Q 1, "
+                         "20 GHz tabulated at 261 frequencies up to 100 GHz: 521 bins of 1.5 mm per bunch)")
     ap.add_argument("--sur-steps", type=int, default=20000, help="c3: Adam steps of the surrogate training")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reads MIB, writes MIB) three times, "
@@ -249,6 +252,13 @@ def train_turns(a, sur=None, mode=0):
         res = [[bbr["rs"][r], bbr["q"][r], bbr["fr"][r], 0.0, 1.0, bbr["fr"][r], 0.0, 1.0, bbr["fr"][r]]
                for r in range(len(bbr["rs"]))]
         tr.set_lrwake(a.lrwake, [[0.01, 0.01, 2304.0, 2.7, 4.25, 1, 3.5e7]], res, secs[0].beta_x, secs[0].beta_y)
+    if a.bbimp:
+        import numpy as np
+        nimp, fmax = 261, 1.0e11
+        f = np.arange(nimp) * fmax / (nimp - 1)
+        z = np.zeros(nimp, complex)
+        z[1:] = 2.0e3 / (1 + 1j * (f[1:] / 2.0e10 - 2.0e10 / f[1:]))
+        tr.set_bbimp(z.real.copy(), z.imag.copy(), fmax)
     if a.beam_loading:
         tr.set_beam_loading(dict(nres=1, mode=[1], type=[1], harm=[1], cold=[0], rs=[5e6], q0=[2e4],
                                  coupling=[3.0], detune=[-2e4], vol=[configs.rf_voltage(p)], phase=[0.0],
@@ -257,7 +267,8 @@ def train_turns(a, sur=None, mode=0):
         tr.set_field_model(sur, mode)
     tr.autotune()
     dt, dt_ev, ktimes = timed_turns(tr, a, ("pass", "pass_a", "post", "finalize", "ion_prep", "ion_turn", "ion_compact",
-                                            "ion_reduce", "ion_finish", "ion_kick", "lrwake"))
+                                            "ion_reduce", "ion_finish", "ion_kick", "lrwake",
+                                            "bbi_bin", "bbi_wake", "bbi_kick"))
     ktimes = {k: v for k, v in ktimes.items() if v[1] > 0}
     cloud = tr.ion_passages()
     m = tr.moments()
@@ -320,6 +331,8 @@ def run_train(a):
                                + (f", BBR wake on {T['nbins']} fixed bins per bunch" if T.get("nbins") else "")
                                + (f", long-range wake over {a.lrwake} turns (2304 m round resistive-wall pipe "
                                   "and the resonator)" if a.lrwake else "")
+                               + (", broadband impedance (longitudinal, 521 bins of 1.5 mm per bunch)"
+                                  if a.bbimp else "")
                                + (", beam-loaded cavity (rfMode 1, stable model, 60 bins per bunch; the turn "
                                   "adds a read-only pass for the bunch extent)" if a.beam_loading else ""),
                    "macroparticles": n, "bunches": nb, "parallelism": "single GPU",

@@ -172,6 +172,11 @@ int dump(const std::string &dir, ReadInputSettings &in, LatticeInterActionPoint 
                      c.lrw_beta[0], c.lrw_beta[1]);
         arr("lrw_sectors", c.lrw_sectors.data(), c.lrw_sectors.size());
         arr("lrw_res", c.lrw_res.data(), c.lrw_res.size());
+        std::fprintf(f, "\"bbi\": %d, \"bbi_fmax\": %.17g, \"bbi_td\": %d, \"bbi_flags\": [%d, %d, %d, %d, %d],\n",
+                     c.bbi, c.bbi_fmax, c.bbi_time_domain, c.bbi_flags[0], c.bbi_flags[1], c.bbi_flags[2],
+                     c.bbi_flags[3], c.bbi_flags[4]);
+        arr("bbi_re", c.bbi_re.data(), c.bbi_re.size());
+        arr("bbi_im", c.bbi_im.data(), c.bbi_im.size());
         const cts_beam_loading_config &b = c.blc;
         std::fprintf(f, "\"bl\": %d, \"bl_nres\": %d, \"bl_nbins\": %d, \"bl_nturns\": %d,\n", c.bl, b.nres,
                      b.nbins, b.nturns);

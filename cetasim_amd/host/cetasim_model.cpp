@@ -183,6 +183,13 @@ int ReadInputSettings::ParamReadFile(const std::string &path)
             else if (k == "runbeamionflag") run_.beamIonFlag = inum(v, 0);
             else if (k == "runshortrangewake") run_.sRWakeFlag = inum(v, 0);
             else if (k == "runlongrangewake") run_.lRWakeFlag = inum(v, 0);
+            else if (k == "runbbiflag") run_.bBImpFlag = inum(v, 0);
+            else if (k == "bbiimpedinput") bbi_.impedInput = v.at(0);
+            else if (k == "bbibunchbinnumberz") bbi_.bunchBinNumberZ = inum(v, 0);
+            else if (k == "bbiimpedsimlandtflag") for (int i = 0; i < 5; i++) bbi_.impedSimFlag[i] = inum(v, i);
+            else if (k == "bbiimpedsimquasigreenbunchlength") bbi_.quasiWakeBunchLen = num(v, 0);
+            else if (k == "bbiimpedsimtimeorfreflag") bbi_.timeDomain = inum(v, 0);
+            else if (k == "bbiwakeinput") bbi_.wakeInput = v.at(0);
             else if (k == "rungrowthratefittingstart") run_.growthRateFittingStart = inum(v, 0);
             else if (k == "rungrowthratefittingend") run_.growthRateFittingEnd = inum(v, 0);
             else if (k == "runtbtbunchaverdata") run_.TBTBunchAverData = v.at(0);
@@ -939,6 +946,43 @@ int MPBeam::RunSettings(const LatticeInterActionPoint &lat, const ReadInputSetti
         c.lrw_beta[1] = lw.betaFunIntPoint[1];
         c.lrw_output = in.ringLRWake->lrwOutput;
     }
+    // broadband impedance: BoardBandImp::ReadInImp (src/BoardBandImp.cpp:42-70):
+    // 5 header lines, then frequency and ten impedance columns per row;
+    // Z_|| = column 2 - i column 7
+    if (R->bBImpFlag) {
+        if (lat.sections.size() != 1) {
+            std::cerr << "broadband impedance: one interaction point (fused turn) only" << std::endl;
+            return 1;
+        }
+        const auto *B = in.ringBBImp;
+        std::ifstream f(B->impedInput);
+        if (!f) {
+            std::cerr << "cannot open the impedance file " << B->impedInput << std::endl;
+            return 1;
+        }
+        std::string line;
+        double fmax = 0;
+        for (int ln = 0; std::getline(f, line); ln++) {
+            if (ln < 5) continue;
+            std::istringstream ss(line);
+            std::vector<double> col;
+            double x;
+            while (ss >> x) col.push_back(x);
+            if (col.empty()) continue;
+            if (col.size() < 11) {
+                std::cerr << B->impedInput << ": row " << ln + 1 << " has " << col.size() << " columns, 11 needed"
+                          << std::endl;
+                return 1;
+            }
+            fmax = col[0];
+            c.bbi_re.push_back(col[1]);
+            c.bbi_im.push_back(-col[6]);
+        }
+        c.bbi = true;
+        c.bbi_fmax = fmax;
+        for (int i = 0; i < 5; i++) c.bbi_flags[i] = B->impedSimFlag[i];
+        c.bbi_time_domain = B->timeDomain;
+    }
     // rfMode-1 cavities: the reference's stable beam-loading model
     // (MPBeam::BeamMomtumUpdateDueToRFTest, src/MPBeam.cpp:2624-2706)
     const auto *RF = in.ringParRf;
@@ -1061,6 +1105,16 @@ int MPBeam::Run(Train &, LatticeInterActionPoint &lat, ReadInputSettings &in)
         if (!c.lrw_output.empty() && PrintLRWakeTable(c.lrw_output)) return 1;
     }
     if (c.bl && check(cts_set_beam_loading(gpu, &c.blc), "beam loading")) return 1;
+    if (c.bbi) {
+        cts_bbimp_config bc{};
+        bc.nimp = (int32_t)c.bbi_re.size();
+        bc.f_max = c.bbi_fmax;
+        bc.zz_re = c.bbi_re.data();
+        bc.zz_im = c.bbi_im.data();
+        for (int i = 0; i < 5; i++) bc.flags[i] = c.bbi_flags[i];
+        bc.time_domain = c.bbi_time_domain;
+        if (check(cts_set_bbimp(gpu, &bc), "broadband impedance")) return 1;
+    }
     const int interval = std::max(R->bunchInfoPrintInterval, 1);
     samples_.clear();
     // output files start empty, as the reference removes *.sdds at set-up

@@ -71,7 +71,13 @@ public:
         std::string pipeGeoInput, bbrInput, lrwOutput;
         int nTurnswakeTrunction = 0;  // lrwNTurnsWakeTrunction + 1, as the reference stores it
     };
+    struct RingBBImp {  // &BoardBandImpedance (src/ReadInputSettings.cpp:726-752)
+        std::string impedInput, wakeInput;
+        int bunchBinNumberZ = 0, impedSimFlag[5] = {0, 0, 0, 0, 0}, timeDomain = 0;
+        double quasiWakeBunchLen = 0;
+    };
     struct RingRun {
+        int bBImpFlag = 0;
         int calSetting = 2, nTurns = 0, bunchInfoPrintInterval = 1, synRadDampingFlag = 0, beamIonFlag = 0;
         int sRWakeFlag = 0, lRWakeFlag = 0, growthRateFittingStart = 0, growthRateFittingEnd = 0;
         std::string TBTBunchAverData = "TBTBunchAverData", runCBMGR = "CBMGR";
@@ -85,6 +91,7 @@ public:
     RingIonEffPara *ringIonEffPara = &ion_;
     RingSRWake *ringSRWake = &srw_;
     RingLRWake *ringLRWake = &lrw_;
+    RingBBImp *ringBBImp = &bbi_;
     RingRun *ringRun = &run_;
 
     // Returns 0; prints the reason and returns a non-zero code where the
@@ -100,6 +107,7 @@ private:
     RingIonEffPara ion_;
     RingSRWake srw_;
     RingLRWake lrw_;
+    RingBBImp bbi_;
     RingRun run_;
     void Derive();
 };
@@ -177,6 +185,10 @@ struct RunCalls {
     std::vector<double> lrw_sectors, lrw_res;  // [n][7], [n][9] (cts_lrwake_config's layout)
     double lrw_beta[2] = {0, 0};
     std::string lrw_output;
+    bool bbi = false;  // broadband impedance (BoardBandImp::ReadInImp's table)
+    int bbi_flags[5] = {0, 0, 0, 0, 0}, bbi_time_domain = 0;
+    double bbi_fmax = 0;
+    std::vector<double> bbi_re, bbi_im;
     bool bl = false;  // rfMode-1 cavities present
     cts_beam_loading_config blc{};
 };

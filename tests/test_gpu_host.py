@@ -36,6 +36,7 @@ def run_program(d, edits=None):
     H.copy_fixture(str(d), edits=edits)
     if edits:
         (d / "pipe.dat").write_text(PIPE)
+        (d / "imp.dat").write_text(H.impedance_file()[0])
     (d / "dump").mkdir()
     r = H.run(["--dump", "dump", "input.dat"], str(d))
     assert r.returncode == 0, r.stderr
@@ -54,6 +55,14 @@ def program_run(tmp_path_factory):
 @pytest.fixture(scope="module")
 def program_run_lrw(tmp_path_factory):
     return run_program(tmp_path_factory.mktemp("run_lrw"), LRW_EDITS)
+
+
+BBI_EDITS = {"runBBIFlag": "1", "BBIimpedInput": "imp.dat", "BBIimpedSimLandTFlag": "1 0 0 0 0"}
+
+
+@pytest.fixture(scope="module")
+def program_run_bbi(tmp_path_factory):
+    return run_program(tmp_path_factory.mktemp("run_bbi"), BBI_EDITS)
 
 
 @pytest.fixture(scope="module")
@@ -100,6 +109,14 @@ def test_program_with_beam_loading_matches_capi_run(program_run_bl):
     check_against_capi(d, D)
 
 
+def test_program_with_bbimp_matches_capi_run(program_run_bbi):
+    """The same with the broadband impedance on (longitudinal plane, a
+    broadband-resonator table in the reference's file layout)."""
+    d, D = program_run_bbi
+    assert D["settings"]["bbi"] == 1
+    check_against_capi(d, D)
+
+
 def check_against_capi(d, D):
     s = D["settings"]
     nb, interval, nturns = s["nb"], s["interval"], s["nTurns"]
@@ -121,6 +138,8 @@ def check_against_capi(d, D):
             row = [float(x) for x in rows[1 + i]]
             assert row[0] == g(-t[i])
             assert row[1:] == [g(v) for v in np.concatenate([rw[:, i], bbr[:, i], w[:, i]])]
+    if s.get("bbi"):
+        tr.set_bbimp(s["bbi_re"], s["bbi_im"], s["bbi_fmax"], flags=s["bbi_flags"], time_domain=s["bbi_td"])
     if s.get("bl"):
         nres = s["bl_nres"]
         bi, br = np.array(s["bl_int"]), np.array(s["bl_real"])

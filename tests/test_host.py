@@ -269,3 +269,26 @@ def test_rf_modes(tmp_path):
 
 def test_exe_exists():
     assert os.access(H.exe(), os.X_OK)
+
+
+def test_bbimp_settings(tmp_path):
+    """runBBIFlag and the &BoardBandImpedance keys (src/ReadInputSettings.cpp:
+    726-752, 808-811); the impedance file's Z_|| = column 2 - i column 7 and
+    its last frequency (src/BoardBandImp.cpp:42-70)."""
+    text, zre, zim, fmax = H.impedance_file(nimp=12, f_max=3.0e10)
+    (tmp_path / "imp.dat").write_text(text)
+    H.copy_fixture(str(tmp_path), {"runBBIFlag": "1", "BBIimpedInput": "imp.dat",
+                                   "BBIimpedSimLandTFlag": "1 0 0 0 0", "BBIimpedSimTimeOrFreFlag": "0"})
+    d = tmp_path / "dump"
+    d.mkdir()
+    r = H.run(["--dump", str(d), "input.dat"], str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    s = H.load_dump(str(d))["settings"]
+    assert s["bbi"] == 1 and s["bbi_flags"] == [1, 0, 0, 0, 0] and s["bbi_td"] == 0
+    assert s["bbi_fmax"] == 3.0e10
+    np.testing.assert_array_equal(s["bbi_re"], zre)
+    np.testing.assert_array_equal(s["bbi_im"], zim)
+    # off by default; a missing file is an error
+    H.copy_fixture(str(tmp_path), {"runBBIFlag": "1", "BBIimpedInput": "none.dat"})
+    r = H.run(["--dump", str(d), "input.dat"], str(tmp_path))
+    assert r.returncode != 0 and "impedance file" in r.stderr
