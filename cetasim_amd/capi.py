@@ -84,7 +84,8 @@ class LrwakeConfig(ctypes.Structure):
 
 class BbimpConfig(ctypes.Structure):
     _fields_ = [("nimp", I32), ("f_max", D), ("zz_re", PD), ("zz_im", PD), ("flags", I32 * 5),
-                ("time_domain", I32)]
+                ("time_domain", I32), ("zqx_re", PD), ("zqx_im", PD), ("zqy_re", PD), ("zqy_im", PD),
+                ("beta_x", D), ("beta_y", D)]
 
 
 class BeamLoadingConfig(ctypes.Structure):

@@ -236,11 +236,13 @@ BbiParams bbi_params(cts_handle *h)
 {
     BbiParams p{};
     p.n = h->bbi.n; p.is = h->bbi.is; p.ie = h->bbi.ie; p.nb = h->nb;
-    p.zmin = h->bbi.zmin; p.dz = h->bbi.dz;
+    p.zmin = h->bbi.zmin; p.dz = h->bbi.dz; p.bx = h->bbi.bx; p.by = h->bbi.by;
+    for (int q = 0; q < 3; q++) p.planes[q] = h->bbi.planes[q];
     p.energy = h->ring.energy; p.beta2 = std::pow(h->ring.rbeta, 2);
     p.seg_begin = h->d_seg_begin; p.seg_end = h->d_seg_end;
     p.seg_bunch = h->d_seg_bunch; p.wg_seg = h->d_wg_seg;
-    p.z = h->d_z; p.dp = h->d_dp; p.flag = h->d_flag;
+    p.x = h->d_x; p.y = h->d_y; p.z = h->d_z;
+    p.px = h->d_px; p.py = h->d_py; p.dp = h->d_dp; p.flag = h->d_flag;
     p.hist = h->d_bbi_hist; p.prof = h->d_bbi_prof; p.wake = h->d_bbi_wake;
     p.g = h->d_bbi_g; p.scale = h->d_bbi_scale;
     for (int j = 0; j < kBbiTaps; j++) p.taps[j] = h->bbi.taps[j];
@@ -1687,24 +1689,24 @@ int cts_set_bbimp(cts_handle *h, const cts_bbimp_config *cfg)
         return CTS_OK;  // doTracking = 0 (src/MPBunch.cpp:2152-2157)
     if (cfg->time_domain != 0)
         return fail(CTS_ERR_ARG, "cts_set_bbimp: only the frequency-domain method (time_domain 0) is built");
-    if (cfg->flags[1] || cfg->flags[2] || cfg->flags[3] || cfg->flags[4])
-        return fail(CTS_ERR_ARG, "cts_set_bbimp: only the longitudinal plane (flags[0]) is built");
+    if (cfg->flags[1] || cfg->flags[2])
+        return fail(CTS_ERR_ARG, "cts_set_bbimp: the dipolar planes (flags[1], flags[2]) are not built");
     if (h->secs.size() != 1) return fail(CTS_ERR_STATE, "cts_set_bbimp: the fused turn (one section) only");
     std::string err;
     if ((rc = bbimp_table(*cfg, h->ring, h->bbi, err))) return fail(rc, err);
     const int n = h->bbi.n, nb = h->nb;
     if ((rc = dalloc(&h->d_bbi_hist, (size_t)nb * n)) || (rc = dalloc(&h->d_bbi_prof, (size_t)nb * n)) ||
-        (rc = dalloc(&h->d_bbi_wake, (size_t)nb * n)) || (rc = dalloc(&h->d_bbi_g, (size_t)n)) ||
+        (rc = dalloc(&h->d_bbi_wake, 3 * (size_t)nb * n)) || (rc = dalloc(&h->d_bbi_g, 3 * (size_t)n)) ||
         (rc = dalloc(&h->d_bbi_scale, (size_t)nb))) {
         free_bbi_state(h);
         return rc;
     }
     std::vector<double> sc(nb);
     for (int b = 0; b < nb; b++) sc[b] = h->macro_charge[b] * kEcharge / h->bbi.dz * kClight;
-    HIPCHK(hipMemcpy(h->d_bbi_g, h->bbi.g.data(), n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_bbi_g, h->bbi.g.data(), 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->d_bbi_scale, sc.data(), nb * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(h->d_bbi_hist, 0, (size_t)nb * n * sizeof(uint32_t)));
-    HIPCHK(hipMemset(h->d_bbi_wake, 0, (size_t)nb * n * sizeof(double)));
+    HIPCHK(hipMemset(h->d_bbi_wake, 0, 3 * (size_t)nb * n * sizeof(double)));
     HIPCHK(hipMemset(h->d_bbi_prof, 0, (size_t)nb * n * sizeof(double)));
     h->bbi_on = true;
     return CTS_OK;
@@ -1718,7 +1720,7 @@ int cts_get_bbimp_wake(cts_handle *h, int32_t *n, double *wake, double *prof)
     if (n) *n = h->bbi.n;
     HIPCHK(hipStreamSynchronize(h->stream));
     const size_t sz = (size_t)h->nb * h->bbi.n * sizeof(double);
-    if (wake) HIPCHK(hipMemcpy(wake, h->d_bbi_wake, sz, hipMemcpyDeviceToHost));
+    if (wake) HIPCHK(hipMemcpy(wake, h->d_bbi_wake, 3 * sz, hipMemcpyDeviceToHost));
     if (prof) HIPCHK(hipMemcpy(prof, h->d_bbi_prof, sz, hipMemcpyDeviceToHost));
     return CTS_OK;
 }

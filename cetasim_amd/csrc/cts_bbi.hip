@@ -1,5 +1,6 @@
 // cts_bbi.hip -- broadband impedance in the frequency domain, longitudinal
-// plane (MPBunch::BBImpBunchInteraction, src/MPBunch.cpp:2143-2216).
+// and quadrupolar planes (MPBunch::BBImpBunchInteraction,
+// src/MPBunch.cpp:2143-2250).
 //
 // The reference bins a bunch on the centred grid of BoardBandImp::ReadInImp
 // (src/BoardBandImp.cpp:71-90: N = 2 nimp - 1 bins of dz = c beta/(2 f_max)
@@ -25,10 +26,23 @@ namespace cts {
 
 int bbimp_table(const cts_bbimp_config &c, const cts_ring &ring, BbiTable &t, std::string &err)
 {
-    if (c.nimp < 2 || 2 * c.nimp - 1 > kBbiMaxBins || !(c.f_max > 0) || !c.zz_re || !c.zz_im) {
-        err = "cts_set_bbimp: nimp in [2, " + std::to_string((kBbiMaxBins + 1) / 2) + "], f_max > 0, Z given";
+    t.planes[0] = c.flags[0] != 0;
+    t.planes[1] = c.flags[3] != 0;
+    t.planes[2] = c.flags[4] != 0;
+    const double *zr[3] = {c.zz_re, c.zqx_re, c.zqy_re}, *zi[3] = {c.zz_im, c.zqx_im, c.zqy_im};
+    bool given = true;
+    for (int q = 0; q < 3; q++) given = given && (!t.planes[q] || (zr[q] && zi[q]));
+    if (c.nimp < 2 || 2 * c.nimp - 1 > kBbiMaxBins || !(c.f_max > 0) || !given) {
+        err = "cts_set_bbimp: nimp in [2, " + std::to_string((kBbiMaxBins + 1) / 2) +
+              "], f_max > 0, the impedance of every plane switched on";
         return CTS_ERR_ARG;
     }
+    if ((t.planes[1] && !(c.beta_x > 0)) || (t.planes[2] && !(c.beta_y > 0))) {
+        err = "cts_set_bbimp: beta_x, beta_y > 0 for the quadrupolar planes";
+        return CTS_ERR_ARG;
+    }
+    t.bx = t.planes[1] ? c.beta_x : 1.0;
+    t.by = t.planes[2] ? c.beta_y : 1.0;
     const int nimp = c.nimp, n = 2 * nimp - 1;
     const double dt = 1.0 / (2 * c.f_max);
     t.n = n;
@@ -44,15 +58,21 @@ int bbimp_table(const cts_bbimp_config &c, const cts_ring &ring, BbiTable &t, st
               " m) must cover one RF bucket (" + std::to_string(rf_len) + " m)";
         return CTS_ERR_ARG;
     }
-    t.g.assign(n, 0.0);
-    for (int m = 0; m < n; m++) {
-        double s = 0;
-        for (int k = 1; k < nimp; k++) {
-            // exact reduction of the phase k m / n
-            const double ph = 2 * M_PI * (double)(((int64_t)k * m) % n) / n;
-            s += c.zz_re[k] * std::cos(ph) - c.zz_im[k] * std::sin(ph);
+    t.g.assign(3 * (size_t)n, 0.0);
+    for (int q = 0; q < 3; q++) {
+        if (!t.planes[q]) continue;
+        // the quadrupolar planes multiply the spectrum by i Z: (re, im) -> (-im, re)
+        auto re = [&](int k) { return q == 0 ? zr[q][k] : -zi[q][k]; };
+        auto im = [&](int k) { return q == 0 ? zi[q][k] : zr[q][k]; };
+        for (int m = 0; m < n; m++) {
+            double s = 0;
+            for (int k = 1; k < nimp; k++) {
+                // exact reduction of the phase k m / n
+                const double ph = 2 * M_PI * (double)(((int64_t)k * m) % n) / n;
+                s += re(k) * std::cos(ph) - im(k) * std::sin(ph);
+            }
+            t.g[(size_t)q * n + m] = (re(0) + 2 * s) / n;
         }
-        t.g[m] = (c.zz_re[0] + 2 * s) / n;
     }
     const int half = (kBbiTaps - 1) / 2;
     const double alpha = 10.0;
@@ -112,10 +132,13 @@ __global__ void __launch_bounds__(kBlock) k_bbi_wake(BbiParams P)
     const int b = blockIdx.x, tid = threadIdx.x;
     uint32_t *cnt = P.hist + (int64_t)b * P.n;
     double *prof = P.prof + (int64_t)b * P.n;
-    double *wake = P.wake + (int64_t)b * P.n;
     const double sc = P.scale[b];
     if (sc == 0.0) {
-        for (int j = tid; j < P.n; j += kBlock) { wake[j] = 0.0; prof[j] = 0.0; cnt[j] = 0u; }
+        for (int j = tid; j < P.n; j += kBlock) {
+            for (int q = 0; q < 3; q++) P.wake[((int64_t)q * P.nb + b) * P.n + j] = 0.0;
+            prof[j] = 0.0;
+            cnt[j] = 0u;
+        }
         return;
     }
     const int nr = P.ie - P.is, half = (kBbiTaps - 1) / 2;
@@ -144,14 +167,20 @@ __global__ void __launch_bounds__(kBlock) k_bbi_wake(BbiParams P)
         else prof[i] = (double)cnt[i] * sc;
     }
     __syncthreads();
-    for (int j = tid; j < P.n; j += kBlock) {
-        double v = 0;
-        int m = j;  // (j - i) mod n for i = 0, 1, ...
-        for (int i = 0; i < P.n; i++) {
-            v += prof[i] * P.g[m];
-            m = m == 0 ? P.n - 1 : m - 1;
+    // longitudinal V = -IDFT(I Z), quadrupolar V_Q = +IDFT(I i Z_Q)
+    for (int q = 0; q < 3; q++) {
+        if (!P.planes[q]) continue;
+        const double *g = P.g + (int64_t)q * P.n;
+        double *wake = P.wake + ((int64_t)q * P.nb + b) * P.n;
+        for (int j = tid; j < P.n; j += kBlock) {
+            double v = 0;
+            int m = j;  // (j - i) mod n for i = 0, 1, ...
+            for (int i = 0; i < P.n; i++) {
+                v += prof[i] * g[m];
+                m = m == 0 ? P.n - 1 : m - 1;
+            }
+            wake[j] = q == 0 ? -v : v;
         }
-        wake[j] = -v;
     }
     __syncthreads();
     for (int j = tid; j < P.n; j += kBlock) cnt[j] = 0u;
@@ -163,11 +192,15 @@ __global__ void __launch_bounds__(kBlock) k_bbi_kick(BbiParams P)
     for (int s = P.wg_seg[blockIdx.x]; s < P.wg_seg[blockIdx.x + 1]; s++) {
         const int b = P.seg_bunch[s];
         if (P.scale[b] == 0.0) continue;
-        const double *wake = P.wake + (int64_t)b * P.n;
+        const double *wz = P.wake + (int64_t)b * P.n;
+        const double *wqx = P.wake + ((int64_t)P.nb + b) * P.n, *wqy = P.wake + (2 * (int64_t)P.nb + b) * P.n;
         for (int64_t i = P.seg_begin[s] + tid; i < P.seg_end[s]; i += kBlock) {
             if (!bbi_alive(P.flag[i])) continue;
             const int k = bbi_bin(P.z[i], P);
-            if (k >= 0) P.dp[i] += wake[k] / P.energy / P.beta2;
+            if (k < 0) continue;
+            if (P.planes[0]) P.dp[i] += wz[k] / P.energy / P.beta2;
+            if (P.planes[1]) P.px[i] += wqx[k] / P.energy / P.beta2 * P.x[i] / P.bx;
+            if (P.planes[2]) P.py[i] += wqy[k] / P.energy / P.beta2 * P.y[i] / P.by;
         }
     }
 }

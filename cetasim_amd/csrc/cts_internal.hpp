@@ -249,22 +249,25 @@ constexpr int kBbiTaps = 51;      // the Gaussian filter's window
 constexpr int kBbiMaxBins = 8191;  // 2 nimp - 1: the per-segment histogram in LDS (32 KB)
 struct BbiTable {
     int n = 0, is = 0, ie = 0;   // bins; smoothed range [is, ie)
-    double zmin = 0, dz = 0;
-    std::vector<double> g;       // [n] kernel: V = -(profile (*) g)
+    int planes[3] = {0, 0, 0};   // longitudinal, quadrupolar x, y
+    double zmin = 0, dz = 0, bx = 1, by = 1;
+    std::vector<double> g;       // [3][n] kernels: V = -(profile (*) g_z), V_Q = +(profile (*) g_Q)
     double taps[kBbiTaps];       // normalised Gaussian weights
 };
 int bbimp_table(const cts_bbimp_config &c, const cts_ring &ring, BbiTable &t, std::string &err);
 struct BbiParams {
     int n, is, ie, nb;
-    double zmin, dz, energy, beta2;
+    int planes[3];
+    double zmin, dz, energy, beta2, bx, by;
     const int64_t *seg_begin, *seg_end;
     const int32_t *seg_bunch, *wg_seg;
-    const double *z;
-    double *dp;
+    const double *x, *y, *z;
+    double *px, *py, *dp;
     const uint8_t *flag;
     uint32_t *hist;        // [nb][n], zero between turns
-    double *prof, *wake;   // [nb][n]
-    const double *g;       // [n]
+    double *prof;          // [nb][n]
+    double *wake;          // [3][nb][n]
+    const double *g;       // [3][n]
     const double *scale;   // [nb] macro charge e c / dz (0: the bunch is skipped)
     double taps[kBbiTaps];
 };

@@ -177,6 +177,11 @@ int dump(const std::string &dir, ReadInputSettings &in, LatticeInterActionPoint 
                      c.bbi_flags[3], c.bbi_flags[4]);
         arr("bbi_re", c.bbi_re.data(), c.bbi_re.size());
         arr("bbi_im", c.bbi_im.data(), c.bbi_im.size());
+        arr("bbi_qx_re", c.bbi_qx_re.data(), c.bbi_qx_re.size());
+        arr("bbi_qx_im", c.bbi_qx_im.data(), c.bbi_qx_im.size());
+        arr("bbi_qy_re", c.bbi_qy_re.data(), c.bbi_qy_re.size());
+        arr("bbi_qy_im", c.bbi_qy_im.data(), c.bbi_qy_im.size());
+        arr("bbi_beta", c.bbi_beta, 2);
         const cts_beam_loading_config &b = c.blc;
         std::fprintf(f, "\"bl\": %d, \"bl_nres\": %d, \"bl_nbins\": %d, \"bl_nturns\": %d,\n", c.bl, b.nres,
                      b.nbins, b.nturns);

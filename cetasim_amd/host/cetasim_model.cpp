@@ -948,7 +948,8 @@ int MPBeam::RunSettings(const LatticeInterActionPoint &lat, const ReadInputSetti
     }
     // broadband impedance: BoardBandImp::ReadInImp (src/BoardBandImp.cpp:42-70):
     // 5 header lines, then frequency and ten impedance columns per row;
-    // Z_|| = column 2 - i column 7
+    // Z_|| = column 2 - i column 7, Z_Qx = -column 10 - i column 5,
+    // Z_Qy = -column 11 - i column 6
     if (R->bBImpFlag) {
         if (lat.sections.size() != 1) {
             std::cerr << "broadband impedance: one interaction point (fused turn) only" << std::endl;
@@ -977,11 +978,17 @@ int MPBeam::RunSettings(const LatticeInterActionPoint &lat, const ReadInputSetti
             fmax = col[0];
             c.bbi_re.push_back(col[1]);
             c.bbi_im.push_back(-col[6]);
+            c.bbi_qx_re.push_back(-col[9]);
+            c.bbi_qx_im.push_back(-col[4]);
+            c.bbi_qy_re.push_back(-col[10]);
+            c.bbi_qy_im.push_back(-col[5]);
         }
         c.bbi = true;
         c.bbi_fmax = fmax;
         for (int i = 0; i < 5; i++) c.bbi_flags[i] = B->impedSimFlag[i];
         c.bbi_time_domain = B->timeDomain;
+        c.bbi_beta[0] = lat.twissBetaX[0];
+        c.bbi_beta[1] = lat.twissBetaY[0];
     }
     // rfMode-1 cavities: the reference's stable beam-loading model
     // (MPBeam::BeamMomtumUpdateDueToRFTest, src/MPBeam.cpp:2624-2706)
@@ -1113,6 +1120,12 @@ int MPBeam::Run(Train &, LatticeInterActionPoint &lat, ReadInputSettings &in)
         bc.zz_im = c.bbi_im.data();
         for (int i = 0; i < 5; i++) bc.flags[i] = c.bbi_flags[i];
         bc.time_domain = c.bbi_time_domain;
+        bc.zqx_re = c.bbi_qx_re.data();
+        bc.zqx_im = c.bbi_qx_im.data();
+        bc.zqy_re = c.bbi_qy_re.data();
+        bc.zqy_im = c.bbi_qy_im.data();
+        bc.beta_x = c.bbi_beta[0];
+        bc.beta_y = c.bbi_beta[1];
         if (check(cts_set_bbimp(gpu, &bc), "broadband impedance")) return 1;
     }
     const int interval = std::max(R->bunchInfoPrintInterval, 1);

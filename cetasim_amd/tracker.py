@@ -122,26 +122,35 @@ class Tracker:
         capi.check(self.lib.cts_set_beam_loading(self.h, ctypes.byref(cfg)))
         self.bl_nres = cfg.nres
 
-    def set_bbimp(self, zre, zim, f_max, flags=(1, 0, 0, 0, 0), time_domain=0):
+    def set_bbimp(self, zre, zim, f_max, flags=(1, 0, 0, 0, 0), time_domain=0, zqx=None, zqy=None,
+                  beta=(1.0, 1.0)):
         """Broadband impedance table Z(f_k) (the file's rows, last frequency
-        f_max); zre None switches it off."""
+        f_max), quadrupolar tables zqx, zqy = (re, im); zre None switches it off."""
         cfg = capi.BbimpConfig()
         if zre is not None:
-            self._bbi_z = (np.ascontiguousarray(zre, np.float64), np.ascontiguousarray(zim, np.float64))
+            a = lambda v: np.ascontiguousarray(v, np.float64)  # noqa: E731
+            self._bbi_z = [a(zre), a(zim)] + ([a(zqx[0]), a(zqx[1])] if zqx else [None, None]) + \
+                          ([a(zqy[0]), a(zqy[1])] if zqy else [None, None])
             cfg.nimp = len(self._bbi_z[0])
             cfg.f_max = f_max
             cfg.zz_re, cfg.zz_im = capi.dptr(self._bbi_z[0]), capi.dptr(self._bbi_z[1])
+            if zqx:
+                cfg.zqx_re, cfg.zqx_im = capi.dptr(self._bbi_z[2]), capi.dptr(self._bbi_z[3])
+            if zqy:
+                cfg.zqy_re, cfg.zqy_im = capi.dptr(self._bbi_z[4]), capi.dptr(self._bbi_z[5])
+            cfg.beta_x, cfg.beta_y = beta
             cfg.flags[:] = list(flags)
             cfg.time_domain = time_domain
         capi.check(self.lib.cts_set_bbimp(self.h, ctypes.byref(cfg)))
 
     def bbimp_wake(self):
-        """(wake, profile) of every bunch from the last turn: [nb][2 nimp - 1]."""
+        """(wake [3][nb][2 nimp - 1]: longitudinal, quadrupolar x, y; profile
+        [nb][2 nimp - 1]) of every bunch from the last turn."""
         n = capi.I32()
         capi.check(self.lib.cts_get_bbimp_wake(self.h, ctypes.byref(n), None, None))
-        w, pr = np.empty(self.nb * n.value), np.empty(self.nb * n.value)
+        w, pr = np.empty(3 * self.nb * n.value), np.empty(self.nb * n.value)
         capi.check(self.lib.cts_get_bbimp_wake(self.h, ctypes.byref(n), capi.dptr(w), capi.dptr(pr)))
-        return w.reshape(self.nb, -1), pr.reshape(self.nb, -1)
+        return w.reshape(3, self.nb, -1), pr.reshape(self.nb, -1)
 
     def cavity_info(self, turn):
         out = np.empty(self.nb * self.bl_nres * 8)

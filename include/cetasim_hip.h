@@ -212,20 +212,26 @@ int cts_get_cavity_info(cts_handle *h, int32_t turn, double *info);
  * (dz = c beta / (2 f_max)), the profile over one RF bucket is smoothed by
  * a 51-point Gaussian filter (alpha 10, end values padded, sum kept), the
  * induced voltage -IDFT(DFT(profile) Z) is formed as a circular convolution
- * with the real kernel of Z, and delta += V(bin) / (E beta^2).  Only
- * flags[0] (the longitudinal plane) and the frequency-domain method
- * (time_domain 0) are built; flags[0] = 0 switches it off. */
+ * with the real kernel of Z, and delta += V(bin) / (E beta^2).  The
+ * quadrupolar planes (flags[3], flags[4], src/MPBunch.cpp:2218-2250) use
+ * the same profile with i Z_Qx, i Z_Qy and kick px += V_Qx(bin) x /
+ * (E beta^2 beta_x), py likewise.  The dipolar planes (flags[1], flags[2])
+ * and the time-domain method are not built; all flags 0 switches it off. */
 typedef struct {
     int32_t nimp;                     /* rows of the impedance file, >= 2 */
     double f_max;                     /* its last frequency [Hz] */
-    const double *zz_re, *zz_im;      /* [nimp] Z_|| [Ohm] */
+    const double *zz_re, *zz_im;      /* [nimp] Z_|| [Ohm] (flags[0]) */
     int32_t flags[5];                 /* BBIimpedSimLandTFlag: Z, Dx, Dy, Qx, Qy */
     int32_t time_domain;              /* BBIimpedSimTimeOrFreFlag */
+    const double *zqx_re, *zqx_im;    /* [nimp] Z_Qx [Ohm/m] (flags[3]) */
+    const double *zqy_re, *zqy_im;    /* [nimp] Z_Qy (flags[4]) */
+    double beta_x, beta_y;            /* beta functions at the interaction point [m] */
 } cts_bbimp_config;
 int cts_set_bbimp(cts_handle *h, const cts_bbimp_config *cfg);
-/* The induced voltage of every bunch from the last turn: wake[nb][2 nimp - 1]
- * (bin k at z = -z_max + k dz), and the smoothed profile [C/s] if prof is
- * not NULL; n receives 2 nimp - 1. */
+/* The induced voltages of every bunch from the last turn:
+ * wake[3][nb][2 nimp - 1] (longitudinal, quadrupolar x, y; bin k at
+ * z = -z_max + k dz; zeros for planes off), and the smoothed profile [C/s]
+ * [nb][2 nimp - 1] if prof is not NULL; n receives 2 nimp - 1. */
 int cts_get_bbimp_wake(cts_handle *h, int32_t *n, double *wake, double *prof);
 
 /* Radiation damping switch and quantum-excitation noise source

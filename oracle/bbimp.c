@@ -1,6 +1,6 @@
 /* bbimp.c -- CPU oracle (test infrastructure only) of the broadband
- * impedance in the frequency domain, longitudinal plane:
- * MPBunch::BBImpBunchInteraction (src/MPBunch.cpp:2143-2216) with the grid
+ * impedance in the frequency domain, longitudinal and quadrupolar planes:
+ * MPBunch::BBImpBunchInteraction (src/MPBunch.cpp:2143-2250) with the grid
  * of BoardBandImp::ReadInImp (src/BoardBandImp.cpp:71-90) and the smoothing
  * of GetSmoothedBunchProfileGassionFilter (src/MPBunch.cpp:2348-2410).
  * FFTW's r2c and c2r transforms are written as direct sums (c2r of odd
@@ -11,7 +11,8 @@
  * the input padded with its end values.  Pinned in tests/test_bbimp.py by a
  * constant resistive impedance (V = -R I), a linear reactive one (the
  * spectral derivative of a cosine), a constant profile (smoothing changes
- * nothing) and a single spike (it returns the normalised kernel).
+ * nothing), a single spike (it returns the normalised kernel) and a
+ * quadrupolar Z_Q = -i R (V_Q = +R I, and the px kick).
  */
 #include <math.h>
 #include <stdint.h>
@@ -54,11 +55,10 @@ void orc_bbi_smooth(double *p, int n)
     free(y);
 }
 
-/* wake[j] = -c2r(r2c(p) Z)[j] / n, n = 2 nimp - 1 */
-void orc_bbi_potential(const double *p, int nimp, const double *zre, const double *zim, double *wake)
+/* X_k = sum_j p_j e^{-2 pi i j k / n}, k < nimp (FFTW r2c) */
+static void spectrum(const double *p, int nimp, double *xr, double *xi)
 {
     const int n = 2 * nimp - 1;
-    double *xr = (double *)malloc(sizeof(double) * nimp), *xi = (double *)malloc(sizeof(double) * nimp);
     for (int k = 0; k < nimp; k++) {
         double sr = 0, si = 0;
         for (int j = 0; j < n; j++) {
@@ -66,27 +66,54 @@ void orc_bbi_potential(const double *p, int nimp, const double *zre, const doubl
             sr += p[j] * cos(ph);
             si += p[j] * sin(ph);
         }
-        /* times Z_k */
-        xr[k] = sr * zre[k] - si * zim[k];
-        xi[k] = sr * zim[k] + si * zre[k];
+        xr[k] = sr;
+        xi[k] = si;
+    }
+}
+
+/* out_j = s c2r(X Z)_j / n, with X Z times i when rot (the quadrupolar planes) */
+static void product_c2r(const double *xr, const double *xi, int nimp, const double *zre, const double *zim, int rot,
+                        double s, double *out)
+{
+    const int n = 2 * nimp - 1;
+    double *yr = (double *)malloc(sizeof(double) * nimp), *yi = (double *)malloc(sizeof(double) * nimp);
+    for (int k = 0; k < nimp; k++) {
+        const double ar = xr[k] * zre[k] - xi[k] * zim[k], ai = xr[k] * zim[k] + xi[k] * zre[k];
+        yr[k] = rot ? -ai : ar;
+        yi[k] = rot ? ar : ai;
     }
     for (int j = 0; j < n; j++) {
-        double s = 0;
+        double t = 0;
         for (int k = 1; k < nimp; k++) {
             const double ph = 2 * PI * (double)(((int64_t)j * k) % n) / n;
-            s += xr[k] * cos(ph) - xi[k] * sin(ph);
+            t += yr[k] * cos(ph) - yi[k] * sin(ph);
         }
-        wake[j] = -(xr[0] + 2 * s) / n;
+        out[j] = s * (yr[0] + 2 * t) / n;
     }
+    free(yr);
+    free(yi);
+}
+
+/* wake[j] = -c2r(r2c(p) Z)[j] / n, n = 2 nimp - 1 */
+void orc_bbi_potential(const double *p, int nimp, const double *zre, const double *zim, double *wake)
+{
+    double *xr = (double *)malloc(sizeof(double) * nimp), *xi = (double *)malloc(sizeof(double) * nimp);
+    spectrum(p, nimp, xr, xi);
+    product_c2r(xr, xi, nimp, zre, zim, 0, -1.0, wake);
     free(xr);
     free(xi);
 }
 
 /* One bunch's interaction: survivors binned on the grid
- * (int)((z + z_max + dz/2)/dz) of 2 nimp - 1 bins, the bucket smoothed, the
- * induced voltage formed, delta += V/(E beta^2).  wake, prof: [2 nimp - 1]. */
-void orc_bbi_kick(orc_bunch *b, int nimp, double f_max, const double *zre, const double *zim, double macro_charge,
-                  double rbeta, double t0, int harmonic, double energy, double *wake, double *prof)
+ * (int)((z + z_max + dz/2)/dz) of n = 2 nimp - 1 bins, the bucket smoothed,
+ * the induced voltages formed from one spectrum: longitudinal
+ * V = -c2r(X Z)/n (planes[0]), quadrupolar V_Q = c2r(X Z_Q i)/n (planes[1],
+ * planes[2]); delta += V/(E beta^2), px += V_Qx/(E beta^2) x/beta_x, py
+ * likewise.  z[3][nimp] = (re, im) pairs of Z, Z_Qx, Z_Qy; wake [3][n],
+ * prof [n]. */
+void orc_bbi_kick(orc_bunch *b, int nimp, double f_max, const int *planes, const double *zre, const double *zim,
+                  double bx, double by, double macro_charge, double rbeta, double t0, int harmonic, double energy,
+                  double *wake, double *prof)
 {
     const int n = 2 * nimp - 1;
     const double dt = 1.0 / (2 * f_max), dz = dt * ORC_CLIGHT * rbeta;
@@ -105,8 +132,21 @@ void orc_bbi_kick(orc_bunch *b, int nimp, double f_max, const double *zre, const
     const int is = (int)((-rf_len / 2. - zmin + dz / 2) / dz);
     const int ie = (int)((rf_len / 2. - zmin + dz / 2) / dz);
     orc_bbi_smooth(prof + is, ie - is);
-    orc_bbi_potential(prof, nimp, zre, zim, wake);
-    for (int64_t i = 0; i < b->n; i++)
-        if (idx[i] >= 0) b->dp[i] += wake[idx[i]] / energy / pow(rbeta, 2);
+    double *xr = (double *)malloc(sizeof(double) * nimp), *xi = (double *)malloc(sizeof(double) * nimp);
+    spectrum(prof, nimp, xr, xi);
+    for (int q = 0; q < 3; q++) {
+        for (int j = 0; j < n; j++) wake[q * n + j] = 0;
+        if (planes[q])
+            product_c2r(xr, xi, nimp, zre + (size_t)q * nimp, zim + (size_t)q * nimp, q > 0, q == 0 ? -1.0 : 1.0,
+                        wake + (size_t)q * n);
+    }
+    for (int64_t i = 0; i < b->n; i++) {
+        if (idx[i] < 0) continue;
+        if (planes[0]) b->dp[i] += wake[idx[i]] / energy / pow(rbeta, 2);
+        if (planes[1]) b->px[i] += wake[n + idx[i]] / energy / pow(rbeta, 2) * b->x[i] / bx;
+        if (planes[2]) b->py[i] += wake[2 * n + idx[i]] / energy / pow(rbeta, 2) * b->y[i] / by;
+    }
+    free(xr);
+    free(xi);
     free(idx);
 }

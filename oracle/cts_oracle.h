@@ -277,8 +277,9 @@ void orc_bl_kick(orc_bunch *b, double vr, double vi, double vb0, double zmin, do
 /* Broadband impedance, longitudinal plane (bbimp.c). */
 void orc_bbi_smooth(double *p, int n);
 void orc_bbi_potential(const double *p, int nimp, const double *zre, const double *zim, double *wake);
-void orc_bbi_kick(orc_bunch *b, int nimp, double f_max, const double *zre, const double *zim, double macro_charge,
-                  double rbeta, double t0, int harmonic, double energy, double *wake, double *prof);
+void orc_bbi_kick(orc_bunch *b, int nimp, double f_max, const int *planes, const double *zre, const double *zim,
+                  double bx, double by, double macro_charge, double rbeta, double t0, int harmonic, double energy,
+                  double *wake, double *prof);
 
 int orc_cbm_analysis(const double *mean, int ns, int nb, const int *harm, int harmonics, const double *tunes,
                      const double *twiss, int interval, int fit_start, int fit_end, double t0,

@@ -77,7 +77,7 @@ def lib():
         L.orc_bl_kick.argtypes = [ctypes.POINTER(OrcBunch), D, D, D, D, D, I, I, D, D, D]
         L.orc_bbi_smooth.argtypes = [PD, I]
         L.orc_bbi_potential.argtypes = [PD, I, PD, PD, PD]
-        L.orc_bbi_kick.argtypes = [ctypes.POINTER(OrcBunch), I, D, PD, PD, D, D, D, I, D, PD, PD]
+        L.orc_bbi_kick.argtypes = [ctypes.POINTER(OrcBunch), I, D, PIc, PD, PD, D, D, D, D, D, I, D, PD, PD]
         L.orc_lrw_table.restype = I
         L.orc_lrw_table.argtypes = [I, I, D, I, PD, I, PD, D, D, I, PD, PD, PD, PD]
         L.orc_lrw_kick.restype = I
@@ -211,7 +211,8 @@ class OracleTurn:
                                       p(self.table))
         # beam-loaded cavities: callable(turn) -> [(vr, vi, vb0, harm)], nbins
         self.bl = bl
-        # broadband impedance: dict(nimp, f_max, zre, zim, q = this bunch's macro charge)
+        # broadband impedance: dict(nimp, f_max, zre, zim, q = this bunch's macro charge;
+        # optional zqx, zqy = (re, im) and beta = (beta_x, beta_y) for the quadrupolar planes)
         self.bbi = bbi
         self.bbi_wake = self.bbi_prof = None
         self.b1h1 = np.array(sec.b1h1[:]); self.h1b1 = np.array(sec.h1b1[:])
@@ -261,12 +262,18 @@ class OracleTurn:
         if self.bbi is not None:
             # BBImpBeamInteraction, after the radiation stage (src/MPBeam.cpp:1084)
             c = self.bbi
-            n = 2 * c["nimp"] - 1
-            self.bbi_wake, self.bbi_prof = np.zeros(n), np.zeros(n)
-            zre = np.ascontiguousarray(c["zre"], float)
-            zim = np.ascontiguousarray(c["zim"], float)
-            L.orc_bbi_kick(ctypes.byref(b.s), c["nimp"], c["f_max"], p(zre), p(zim), c["q"], R.rbeta, R.t0,
-                           R.harmonic, R.energy, p(self.bbi_wake), p(self.bbi_prof))
+            n, ni = 2 * c["nimp"] - 1, c["nimp"]
+            self.bbi_wake, self.bbi_prof = np.zeros(3 * n), np.zeros(n)
+            zero = np.zeros(ni)
+            qx, qy = c.get("zqx"), c.get("zqy")
+            zre = np.ascontiguousarray(np.concatenate([c["zre"], qx[0] if qx else zero, qy[0] if qy else zero]))
+            zim = np.ascontiguousarray(np.concatenate([c["zim"], qx[1] if qx else zero, qy[1] if qy else zero]))
+            planes = np.array([int(c.get("long", 1)), int(qx is not None), int(qy is not None)], np.int32)
+            bx, by = c.get("beta", (1.0, 1.0))
+            L.orc_bbi_kick(ctypes.byref(b.s), ni, c["f_max"], planes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                           p(zre), p(zim), bx, by,
+                           c["q"], R.rbeta, R.t0, R.harmonic, R.energy, p(self.bbi_wake), p(self.bbi_prof))
+            self.bbi_wake = self.bbi_wake.reshape(3, n)
         if self.nbins:
             tab = self.table
             L.orc_sr_wake(ctypes.byref(b.s), self.nbins, self.mode, sig_stale, self.grid[0], self.grid[1],

@@ -29,18 +29,34 @@ def impedance(nimp=261, f_max=1.0e11, rs=2.0e3, q=1.0, fr=2.0e10):
     return z.real.copy(), z.imag.copy(), f_max
 
 
-@pytest.mark.parametrize("sr_wake", [False, True])
-def test_bbimp_turns_match_oracle(sr_wake):
+def quad_impedance(nimp=261, f_max=1.0e11, rs=1.0e3, q=1.0, fr=1.5e10):
+    """Transverse broadband resonator Z_Q(f) = (fr/f) Rs / (1 + i Q (f/fr - fr/f))."""
+    f = np.arange(nimp) * f_max / (nimp - 1)
+    z = np.zeros(nimp, complex)
+    z[1:] = (fr / f[1:]) * rs / (1 + 1j * q * (f[1:] / fr - fr / f[1:]))
+    return z.real.copy(), z.imag.copy()
+
+
+@pytest.mark.parametrize("sr_wake,quad", [(False, False), (True, False), (False, True)])
+def test_bbimp_turns_match_oracle(sr_wake, quad):
     counts = [3000, 7001, 1500, 4096]
     ring, secs, counts, gap, q, ne, _, tr, bunches, off = setup_train(counts, 13)
     zre, zim, fmax = impedance()
-    tr.set_bbimp(zre, zim, fmax)
+    qx, qy = quad_impedance(), quad_impedance(rs=1.5e3, fr=1.2e10)
+    beta = (secs[0].beta_x, secs[0].beta_y)
+    planes = (0, 1, 2) if quad else (0,)
+    if quad:
+        tr.set_bbimp(zre, zim, fmax, flags=(1, 0, 0, 1, 1), zqx=qx, zqy=qy, beta=beta)
+    else:
+        tr.set_bbimp(zre, zim, fmax)
     tr.set_synrad(True, 2, 8)
     kw = {}
     if sr_wake:
         tr.set_wake(100, 1, -5 * SZ, 5 * SZ, BBR["rs"], BBR["q"], BBR["fr"])
         kw = dict(nbins=100, mode=1, grid=(-5 * SZ, 5 * SZ), bbr=BBR, macro_charge=list(q))
     bbi = dict(nimp=len(zre), f_max=fmax, zre=zre, zim=zim, q=list(q))
+    if quad:
+        bbi.update(zqx=qx, zqy=qy, beta=beta)
     ot = O.OracleTrain(ring, secs[0], counts, ne, gap, None, 1, bbi=bbi, synrad=True, noise_seed=8, **kw)
     print(f"broadband impedance train, short-range wake {sr_wake}", flush=True)
     for t in range(5):
@@ -50,7 +66,11 @@ def test_bbimp_turns_match_oracle(sr_wake):
         for b in range(len(counts)):
             ow, op = ot.turns[b].bbi_wake, ot.turns[b].bbi_prof
             np.testing.assert_allclose(pr[b], op, rtol=0, atol=1e-12 * np.max(np.abs(op)))
-            np.testing.assert_allclose(w[b], ow, rtol=0, atol=1e-10 * np.max(np.abs(ow)))
+            for q in range(3):
+                scale = np.max(np.abs(ow[q]))
+                if q in planes:
+                    assert scale > 0
+                np.testing.assert_allclose(w[q][b], ow[q], rtol=0, atol=1e-10 * scale)
     got = tr.download()
     assert not np.any(got["lost"])
     compare_coords(got, bunches, off, 1e-11)
@@ -67,7 +87,7 @@ def test_bbimp_turns_match_oracle(sr_wake):
 def test_bbimp_settings_are_checked():
     ring, secs, counts, gap, q, ne, _, tr, bunches, off = setup_train([2000], 3)
     zre, zim, fmax = impedance()
-    for kw in (dict(flags=(1, 1, 0, 0, 0)), dict(time_domain=1)):
+    for kw in (dict(flags=(1, 1, 0, 0, 0)), dict(time_domain=1), dict(flags=(1, 0, 0, 1, 0))):
         with pytest.raises(RuntimeError):
             tr.set_bbimp(zre, zim, fmax, **kw)
     # a grid shorter than one RF bucket (z_max < c T_rf / 2)

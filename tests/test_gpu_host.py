@@ -57,7 +57,7 @@ def program_run_lrw(tmp_path_factory):
     return run_program(tmp_path_factory.mktemp("run_lrw"), LRW_EDITS)
 
 
-BBI_EDITS = {"runBBIFlag": "1", "BBIimpedInput": "imp.dat", "BBIimpedSimLandTFlag": "1 0 0 0 0"}
+BBI_EDITS = {"runBBIFlag": "1", "BBIimpedInput": "imp.dat", "BBIimpedSimLandTFlag": "1 0 0 1 1"}
 
 
 @pytest.fixture(scope="module")
@@ -110,8 +110,8 @@ def test_program_with_beam_loading_matches_capi_run(program_run_bl):
 
 
 def test_program_with_bbimp_matches_capi_run(program_run_bbi):
-    """The same with the broadband impedance on (longitudinal plane, a
-    broadband-resonator table in the reference's file layout)."""
+    """The same with the broadband impedance on (longitudinal and
+    quadrupolar planes, resonator tables in the reference's file layout)."""
     d, D = program_run_bbi
     assert D["settings"]["bbi"] == 1
     check_against_capi(d, D)
@@ -139,7 +139,9 @@ def check_against_capi(d, D):
             assert row[0] == g(-t[i])
             assert row[1:] == [g(v) for v in np.concatenate([rw[:, i], bbr[:, i], w[:, i]])]
     if s.get("bbi"):
-        tr.set_bbimp(s["bbi_re"], s["bbi_im"], s["bbi_fmax"], flags=s["bbi_flags"], time_domain=s["bbi_td"])
+        tr.set_bbimp(s["bbi_re"], s["bbi_im"], s["bbi_fmax"], flags=s["bbi_flags"], time_domain=s["bbi_td"],
+                     zqx=(s["bbi_qx_re"], s["bbi_qx_im"]), zqy=(s["bbi_qy_re"], s["bbi_qy_im"]),
+                     beta=tuple(s["bbi_beta"]))
     if s.get("bl"):
         nres = s["bl_nres"]
         bi, br = np.array(s["bl_int"]), np.array(s["bl_real"])

@@ -288,6 +288,13 @@ def test_bbimp_settings(tmp_path):
     assert s["bbi_fmax"] == 3.0e10
     np.testing.assert_array_equal(s["bbi_re"], zre)
     np.testing.assert_array_equal(s["bbi_im"], zim)
+    # Z_Qx = -column 10 - i column 5 (src/BoardBandImp.cpp:63-64); beta at the first twiss row
+    rows = [[float(v) for v in ln.split()] for ln in text.splitlines()[5:] if ln.strip()]
+    np.testing.assert_array_equal(s["bbi_qx_re"], [-r[9] for r in rows])
+    np.testing.assert_array_equal(s["bbi_qx_im"], [-r[4] for r in rows])
+    np.testing.assert_array_equal(s["bbi_qy_re"], [-r[10] for r in rows])
+    np.testing.assert_array_equal(s["bbi_qy_im"], [-r[5] for r in rows])
+    assert s["bbi_beta"] == [2.71, 4.25]
     # off by default; a missing file is an error
     H.copy_fixture(str(tmp_path), {"runBBIFlag": "1", "BBIimpedInput": "none.dat"})
     r = H.run(["--dump", str(d), "input.dat"], str(tmp_path))
