@@ -99,25 +99,47 @@ __device__ __forceinline__ int bbi_bin(double z, const BbiParams &P)
 // survivors before this turn's loss marking
 __device__ __forceinline__ bool bbi_alive(uint8_t f) { return (f & 3u) == 0u || (f & kNewLost); }
 
+constexpr int kUnroll = 4;  // particles per thread per step of the sweeps
+
+// histogram copies in LDS: one per wave when they fit in 32 KB (a bunch
+// fills few bins, so the waves' atomics would collide on one copy)
+__host__ __device__ constexpr int bbi_copies(int n) { return n <= 2048 ? kBlock / 64 : 1; }
+
 }  // namespace
 
 __global__ void __launch_bounds__(kBlock) k_bbi_bin(BbiParams P)
 {
     extern __shared__ uint32_t hl[];
     const int tid = threadIdx.x;
+    const int nc = bbi_copies(P.n);
+    uint32_t *mine = hl + (tid / 64 % nc) * P.n;
     for (int s = P.wg_seg[blockIdx.x]; s < P.wg_seg[blockIdx.x + 1]; s++) {
         const int b = P.seg_bunch[s];
         if (P.scale[b] == 0.0) continue;  // uniform: the whole block skips
-        for (int j = tid; j < P.n; j += kBlock) hl[j] = 0u;
+        for (int j = tid; j < nc * P.n; j += kBlock) hl[j] = 0u;
         __syncthreads();
-        for (int64_t i = P.seg_begin[s] + tid; i < P.seg_end[s]; i += kBlock) {
-            if (!bbi_alive(P.flag[i])) continue;
-            const int k = bbi_bin(P.z[i], P);
-            if (k >= 0) atomicAdd(&hl[k], 1u);
+        // four particles per thread in flight: all loads before any use
+        const int64_t e = P.seg_end[s];
+        for (int64_t i0 = P.seg_begin[s] + tid; i0 < e; i0 += kUnroll * kBlock) {
+            double zz[kUnroll];
+            uint8_t ff[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                const int64_t i = i0 + (int64_t)u * kBlock;
+                ff[u] = i < e ? P.flag[i] : (uint8_t)3u;
+                zz[u] = i < e ? P.z[i] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                if (!bbi_alive(ff[u])) continue;
+                const int k = bbi_bin(zz[u], P);
+                if (k >= 0) atomicAdd(&mine[k], 1u);
+            }
         }
         __syncthreads();
         for (int j = tid; j < P.n; j += kBlock) {
-            const uint32_t c = hl[j];
+            uint32_t c = 0;
+            for (int q = 0; q < nc; q++) c += hl[q * P.n + j];
             if (c) atomicAdd(&P.hist[(int64_t)b * P.n + j], c);
         }
         __syncthreads();
@@ -194,20 +216,34 @@ __global__ void __launch_bounds__(kBlock) k_bbi_kick(BbiParams P)
         if (P.scale[b] == 0.0) continue;
         const double *wz = P.wake + (int64_t)b * P.n;
         const double *wqx = P.wake + ((int64_t)P.nb + b) * P.n, *wqy = P.wake + (2 * (int64_t)P.nb + b) * P.n;
-        for (int64_t i = P.seg_begin[s] + tid; i < P.seg_end[s]; i += kBlock) {
-            if (!bbi_alive(P.flag[i])) continue;
-            const int k = bbi_bin(P.z[i], P);
-            if (k < 0) continue;
-            if (P.planes[0]) P.dp[i] += wz[k] / P.energy / P.beta2;
-            if (P.planes[1]) P.px[i] += wqx[k] / P.energy / P.beta2 * P.x[i] / P.bx;
-            if (P.planes[2]) P.py[i] += wqy[k] / P.energy / P.beta2 * P.y[i] / P.by;
+        const int64_t e = P.seg_end[s];
+        for (int64_t i0 = P.seg_begin[s] + tid; i0 < e; i0 += kUnroll * kBlock) {
+            double zz[kUnroll], d[kUnroll];
+            uint8_t ff[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                const int64_t i = i0 + (int64_t)u * kBlock;
+                ff[u] = i < e ? P.flag[i] : (uint8_t)3u;
+                zz[u] = i < e ? P.z[i] : 0.0;
+                d[u] = (i < e && P.planes[0]) ? P.dp[i] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                const int64_t i = i0 + (int64_t)u * kBlock;
+                if (!bbi_alive(ff[u])) continue;
+                const int k = bbi_bin(zz[u], P);
+                if (k < 0) continue;
+                if (P.planes[0]) P.dp[i] = d[u] + wz[k] / P.energy / P.beta2;
+                if (P.planes[1]) P.px[i] += wqx[k] / P.energy / P.beta2 * P.x[i] / P.bx;
+                if (P.planes[2]) P.py[i] += wqy[k] / P.energy / P.beta2 * P.y[i] / P.by;
+            }
         }
     }
 }
 
 hipError_t launch_bbi_bin(const BbiParams &p, int nwg, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_bbi_bin, dim3(nwg), dim3(kBlock), (size_t)p.n * sizeof(uint32_t), st, p);
+    hipLaunchKernelGGL(k_bbi_bin, dim3(nwg), dim3(kBlock), (size_t)bbi_copies(p.n) * p.n * sizeof(uint32_t), st, p);
     return hipGetLastError();
 }
 hipError_t launch_bbi_wake(const BbiParams &p, hipStream_t st)
