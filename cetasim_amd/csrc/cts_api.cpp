@@ -176,6 +176,12 @@ struct cts_handle {
     BlTable bl;
     double *d_bl_tab = nullptr;
 
+    // the settings of the per-bunch effects above, deep copies kept so that
+    // cts_set_bunches can set them up again for a new bunch layout
+    struct KeepLrw { bool on = false; cts_lrwake_config cfg{}; std::vector<double> sectors, res; } keep_lrw;
+    struct KeepBbi { bool on = false; cts_bbimp_config cfg{}; std::vector<double> arr[6]; } keep_bbi;
+    struct KeepBl { bool on = false; cts_beam_loading_config cfg{}; } keep_bl;
+
     // graph replays of two consecutive turns (cts_run_turns): the turn is
     // read from d_turn, which the graph advances by 2; one instance per
     // (wake buffer parity, ion buffer) at capture
@@ -508,6 +514,10 @@ FinParams base_fin(cts_handle *h, uint32_t ops, int slot, int prev_slot)
 
 int run_pass(cts_handle *h, const PassParams &p, const char *name = "pass")
 {
+    // host-supplied normals are read through p.noise: refuse the launch
+    // rather than hand the kernel a null buffer
+    if ((p.ops & OP_SYNRAD) && p.noise_mode == 1 && !p.noise)
+        return fail(CTS_ERR_STATE, "synrad with noise_mode 1 but no noise buffer (call cts_set_synrad again)");
     TimedScope ts(h, name);
     HIPCHK(launch_pass(p, h->nwg, h->stream));
     return CTS_OK;
@@ -945,7 +955,7 @@ int cts_create(const cts_ring *ring, const cts_section *secs, int32_t nsec, cts_
         delete h;
         return fail(CTS_ERR_HIP, std::string("stream: ") + hipGetErrorString(e));
     }
-    // w(z) tables: Algorithm 916 constants, exp(-a^2 n^2), Weideman's coefficients
+    // Weideman's coefficients of the kick field's w(z)
     FadTables ft;
     build_fad_tables(ft);
     e = upload_faddeeva_tables(ft);
@@ -1005,6 +1015,10 @@ int cts_set_bunches(cts_handle *h, int32_t nb, const int64_t *counts, const doub
     HIPCHK(hipStreamSynchronize(h->stream));
     graph_reset(h);
     free_bunch_state(h);
+    // the per-bunch effects are set up again below for the new layout
+    const cts_handle::KeepLrw keep_lrw = h->keep_lrw;
+    const cts_handle::KeepBbi keep_bbi = h->keep_bbi;
+    const cts_handle::KeepBl keep_bl = h->keep_bl;
     const cts_ion_config ions = h->ion;
     const bool ions_on = h->nspec > 0;
     free_ion_state(h);
@@ -1053,6 +1067,23 @@ int cts_set_bunches(cts_handle *h, int32_t nb, const int64_t *counts, const doub
     if (h->sur_mode & 2) {
         if ((rc = dalloc(&h->d_fb_count, 1)) || (rc = dalloc(&h->d_fb_list, (size_t)h->ntot))) return rc;
     }
+    if (h->noise_mode == 1) {
+        // host-supplied normals: one buffer of 5 per particle of the new layout
+        if ((rc = dalloc(&h->d_noise, (size_t)h->ntot * 5))) return rc;
+    }
+    if (keep_lrw.on) {
+        cts_lrwake_config c = keep_lrw.cfg;
+        c.sectors = keep_lrw.sectors.data();
+        c.res = keep_lrw.res.data();
+        if ((rc = cts_set_lrwake(h, &c))) return rc;
+    }
+    if (keep_bbi.on) {
+        cts_bbimp_config c = keep_bbi.cfg;
+        const double **dst[6] = {&c.zz_re, &c.zz_im, &c.zqx_re, &c.zqx_im, &c.zqy_re, &c.zqy_im};
+        for (int i = 0; i < 6; i++) *dst[i] = keep_bbi.arr[i].empty() ? nullptr : keep_bbi.arr[i].data();
+        if ((rc = cts_set_bbimp(h, &c))) return rc;
+    }
+    if (keep_bl.on && (rc = cts_set_beam_loading(h, &keep_bl.cfg))) return rc;
     if (ions_on) return cts_set_ions(h, &ions);
     return CTS_OK;
 }
@@ -1078,6 +1109,17 @@ int cts_upload(cts_handle *h, const double *x, const double *px, const double *y
         }
     }
     HIPCHK(hipMemcpy(h->d_flag, fl.data(), fl.size(), hipMemcpyHostToDevice));
+    if (h->lrw_T > 0) {
+        // a new run starts from an empty centroid history and no pending
+        // long-range kick
+        HIPCHK(hipMemset(h->d_lrw_ring, 0, (size_t)h->lrw_T * h->nb * 3 * sizeof(double)));
+        HIPCHK(hipMemset(h->d_lrw_kick, 0, (size_t)h->nb * 3 * sizeof(double)));
+    }
+    // the moment sums are taken about a per-bunch shift that follows the
+    // centroids; a new run starts from zero, as a new handle does, so that
+    // its rounding is the same
+    HIPCHK(hipMemset(h->d_shift, 0, (size_t)h->nb * 6 * sizeof(double)));
+    h->lr_pending = false;
     h->kick_pending = false;
     h->dp_pending = false;
     h->last_turn = -1;
@@ -1638,6 +1680,7 @@ int cts_set_lrwake(cts_handle *h, const cts_lrwake_config *cfg)
     HIPCHK(hipStreamSynchronize(h->stream));
     graph_reset(h);
     free_lrw_state(h);
+    h->keep_lrw.on = false;
     if (cfg->nturns == 0) return CTS_OK;
     if (h->secs.empty()) return fail(CTS_ERR_STATE, "cts_set_lrwake: no sections");
     if (cfg->n_sectors < 0 || cfg->n_res < 0 || (cfg->n_sectors > 0 && !cfg->sectors) ||
@@ -1674,6 +1717,13 @@ int cts_set_lrwake(cts_handle *h, const cts_lrwake_config *cfg)
     h->lrw = std::move(tab);
     h->lrw_T = cfg->nturns;
     h->lr_pending = false;
+    cts_handle::KeepLrw k;
+    k.on = true;
+    k.cfg = *cfg;
+    k.sectors.assign(cfg->sectors, cfg->sectors + (cfg->n_sectors > 0 ? 7 * (size_t)cfg->n_sectors : 0));
+    k.res.assign(cfg->res, cfg->res + (cfg->n_res > 0 ? 9 * (size_t)cfg->n_res : 0));
+    k.cfg.sectors = k.cfg.res = nullptr;
+    h->keep_lrw = std::move(k);
     return CTS_OK;
 }
 
@@ -1685,6 +1735,7 @@ int cts_set_bbimp(cts_handle *h, const cts_bbimp_config *cfg)
     HIPCHK(hipStreamSynchronize(h->stream));
     graph_reset(h);
     free_bbi_state(h);
+    h->keep_bbi.on = false;
     if (cfg->flags[0] == 0 && cfg->flags[1] == 0 && cfg->flags[2] == 0 && cfg->flags[3] == 0 && cfg->flags[4] == 0)
         return CTS_OK;  // doTracking = 0 (src/MPBunch.cpp:2152-2157)
     if (cfg->time_domain != 0)
@@ -1709,6 +1760,14 @@ int cts_set_bbimp(cts_handle *h, const cts_bbimp_config *cfg)
     HIPCHK(hipMemset(h->d_bbi_wake, 0, 3 * (size_t)nb * n * sizeof(double)));
     HIPCHK(hipMemset(h->d_bbi_prof, 0, (size_t)nb * n * sizeof(double)));
     h->bbi_on = true;
+    cts_handle::KeepBbi k;
+    k.on = true;
+    k.cfg = *cfg;
+    const double *src[6] = {cfg->zz_re, cfg->zz_im, cfg->zqx_re, cfg->zqx_im, cfg->zqy_re, cfg->zqy_im};
+    for (int i = 0; i < 6; i++)
+        if (src[i]) k.arr[i].assign(src[i], src[i] + cfg->nimp);
+    k.cfg.zz_re = k.cfg.zz_im = k.cfg.zqx_re = k.cfg.zqx_im = k.cfg.zqy_re = k.cfg.zqy_im = nullptr;
+    h->keep_bbi = std::move(k);
     return CTS_OK;
 }
 
@@ -1733,6 +1792,7 @@ int cts_set_beam_loading(cts_handle *h, const cts_beam_loading_config *cfg)
     HIPCHK(hipStreamSynchronize(h->stream));
     graph_reset(h);
     free_bl_state(h);
+    h->keep_bl.on = false;
     if (cfg->nres == 0) return CTS_OK;
     if (h->secs.size() != 1) return fail(CTS_ERR_STATE, "cts_set_beam_loading: the fused turn (one section) only");
     if (cfg->nbins < 1 || cfg->nturns < 1) return fail(CTS_ERR_ARG, "cts_set_beam_loading: nbins, nturns >= 1");
@@ -1768,6 +1828,8 @@ int cts_set_beam_loading(cts_handle *h, const cts_beam_loading_config *cfg)
     HIPCHK(hipMemcpy(h->d_bl_tab, t.dev.data(), t.dev.size() * sizeof(double), hipMemcpyHostToDevice));
     h->bl = std::move(t);
     h->bl_on = true;
+    h->keep_bl.on = true;
+    h->keep_bl.cfg = *cfg;
     return CTS_OK;
 }
 

@@ -1,20 +1,36 @@
-// cts_faddeeva.hpp -- FP64 Faddeeva function w(z) and the Bassetti-Erskine
-// field on the device.
+// cts_faddeeva.hpp -- FP64 Faddeeva function w(z) = exp(-z^2) erfc(-iz) and
+// the Bassetti-Erskine field on the device.
 //
-// Region structure of the reference's vendored Faddeeva package (S. G.
-// Johnson, MIT; src/Faddeeva.cpp:682-974): axis special cases, continued
-// fraction (Gautschi 1970, Poppe & Wijers 1990) for large |z|, large-|x|
-// sums near the real axis.  Where the package sums Zaghloul & Ali's
-// series (ACM TOMS Algorithm 916, |x| < 10), the upper half-plane uses
-// Weideman's rational approximation instead (J. A. C. Weideman, SIAM J.
-// Numer. Anal. 31 (1994) 1497-1518): w(z) = 2 p(Z)/(L - iz)^2 +
-// pi^{-1/2}/(L - iz), Z = (L + iz)/(L - iz), p of degree N - 1 = 39,
-// L = (N/sqrt 2)^{1/2}; |w| relative error below 4e-14 there (1e-15
-// typical), one division and 40 complex multiply-adds instead of a
-// division-heavy series; the lower half-plane keeps the series.  erfcx and Im w(x) on the axes are this project's own
-// evaluations (exp(x^2)erfc(x) with an exact x^2 split, Laplace continued
-// fraction for x >= 25, Dawson's Taylor series for |x| < 0.5).  FP64
-// throughout: the asymptotic and lower-half-plane branches cancel.
+// w(z) everywhere (faddeeva_w; host and device).  The first quadrant
+// x = Re z >= 0, y = Im z >= 0 is evaluated directly; the rest follows from
+// w(-x + iy) = conj w(x + iy) and, below the real axis,
+// w(z) = 2 exp(-z^2) - w(-z).  By region of the first quadrant:
+//  * |z| < 1: the Maclaurin series w(z) = sum_n (iz)^n / Gamma(n/2 + 1).
+//  * y >= 8 or x >= 15: the Laplace continued fraction
+//    w(z) = (i/sqrt(pi)) / (z - (1/2) / (z - 1 / (z - (3/2) / (z - ...)))),
+//    its depth set by |z| (its truncation error behaves as the asymptotic
+//    series' next term, (2n+1)!!/(2|z|^2)^(n+1)); next to the real axis it is
+//    completed by exp(-z^2), which the fraction does not carry; beyond
+//    |z| = 1e7 the two leading terms of the asymptotic series.
+//  * elsewhere: the trapezoidal rule for w(z) = (i/pi) int exp(-t^2)/(z - t) dt
+//    on nodes t = c + n h, h = 0.4, |t| <= 7, with the pole correction
+//    2 exp(-z^2) / (1 - exp(-2 pi i (z - c) / h)) (Chiarella & Reichel,
+//    Math. Comp. 22 (1968) 137; Matta & Reichel, Math. Comp. 25 (1971) 339);
+//    what remains is the aliasing error exp(-pi^2/h^2) ~ 2e-27.  For y >= 0.5
+//    the nodes lie symmetric about 0 (c = h/2) and are summed in pairs, which
+//    leaves Im w proportional to x; below, they are centred on x (x - c is an
+//    odd multiple of h/2), so no node comes near the pole and the correction
+//    reduces to exp(-z^2) (1 - tanh(pi y / h)); Re w is then a sum of positive
+//    terms proportional to y plus that correction, exact to the last digits
+//    however small it is.
+// On the axes: w(iy) = erfcx(y), and w(x) = exp(-x^2) + i Im w(x) with Im w(x)
+// from Dawson's Maclaurin series for |x| < 0.5, the centred trapezoidal rule
+// for |x| < 15 and the continued fraction beyond.
+//
+// Near the beam the kicks take Weideman's rational approximation instead
+// (J. A. C. Weideman, SIAM J. Numer. Anal. 31 (1994) 1497-1518; be_field
+// below), one division and 40 complex multiply-adds per point.  FP64
+// throughout: the far-field and lower-half-plane forms cancel.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,30 +39,20 @@
 
 namespace cts {
 
-// exp(-a^2 n^2), n = 1..64, a = pi/sqrt(-log(eps/2)), and {a, c = 2a/pi,
-// a^2}.  The library is built without relocatable device code, so every
-// translation unit that evaluates w(z) holds its own copy of the tables and
-// exports an upload function that fills it; cts_create calls each of them.
-static __constant__ double c_ea2n2[64];
-static __constant__ double c_fad[3];
+// Weideman's coefficients.  The library is built without relocatable device
+// code, so every translation unit that evaluates the kick field holds its
+// own copy of the table and exports an upload function that fills it;
+// cts_create calls each of them.
 constexpr int kWeidN = 40;
 static __constant__ double c_weid[kWeidN + 1];  // p's coefficients, highest degree first, then L
 
-// Host: the tables above.
+// Host: the table above.
 struct FadTables {
-    double ea2n2[64];
-    double fad[3];
     double weid[kWeidN + 1];
 };
 inline void build_fad_tables(FadTables &t)
 {
     const double pi = 3.141592653589793;
-    // Algorithm 916 constants and exp(-a^2 n^2)
-    t.fad[0] = pi / std::sqrt(-std::log(2.220446049250313e-16 * 0.5));
-    t.fad[1] = (2 / pi) * t.fad[0];
-    t.fad[2] = t.fad[0] * t.fad[0];
-    for (int n = 1; n <= 63; n++) t.ea2n2[n - 1] = std::exp(-t.fad[2] * (double)(n * n));
-    t.ea2n2[63] = 0.0;
     // Weideman: f(t) = e^{-t^2}(L^2 + t^2) at t = L tan(theta/2), theta = k pi/M,
     // k = -M+1..M-1 (and 0 at k = -M), M = 2N; the coefficients are the real
     // parts of the 2M-point DFT of f in FFT order, terms 1..N
@@ -69,32 +75,35 @@ inline void build_fad_tables(FadTables &t)
 }
 
 constexpr double kIsPi = 0.56418958354775628694807945156;  // 1/sqrt(pi)
+constexpr double kFadH = 0.4;                              // trapezoid step
+constexpr double kFadTMax = 7.0;                           // nodes |t| <= 7
+constexpr double kFadPiOverH = 7.8539816339744830962;      // pi / h
 
-__device__ __forceinline__ double fad_sinc(double x, double sinx)
+// exp(-t^2) with t^2 split exactly (t^2 = hi + lo, fma), so the large
+// arguments of the node weights keep full relative accuracy
+__host__ __device__ inline double fad_gauss(double t)
 {
-    return fabs(x) < 1e-4 ? 1 - (0.1666666666666666666667) * x * x : sinx / x;
-}
-__device__ __forceinline__ double fad_sinh_taylor(double x)
-{
-    return x * (1 + (x * x) * (0.1666666666666666666667 + 0.00833333333333333333333 * (x * x)));
+    const double hi = t * t;
+    const double lo = fma(t, t, -hi);
+    return exp(-hi) * (1.0 - lo);
 }
 
-__device__ inline double erfcx_pos(double x)
+__host__ __device__ inline double erfcx_pos(double x)
 {
     if (x < 25.0) {
         const double h = x * x;
         const double l = fma(x, x, -h);
         return exp(h) * (1.0 + l) * erfc(x);
     }
-    if (isinf(x)) return 0.0;
+    if (std::isinf(x)) return 0.0;
     double t = x;
     for (int k = 24; k >= 1; k--) t = x + (0.5 * k) / t;
     return kIsPi / t;
 }
 
-__device__ inline double cts_erfcx(double x)
+__host__ __device__ inline double cts_erfcx(double x)
 {
-    if (isnan(x)) return x;
+    if (std::isnan(x)) return x;
     if (x < 0) {
         if (x < -26.7) return HUGE_VAL;
         return 2 * exp(x * x) - erfcx_pos(-x);
@@ -102,230 +111,164 @@ __device__ inline double cts_erfcx(double x)
     return erfcx_pos(x);
 }
 
-// Weideman's approximation for Im z >= 0 (see the header comment).
-__device__ __forceinline__ void w_weideman(double zr, double zi, double &wr, double &wi)
+// Maclaurin series of w for |z| < 1 (48 terms: the last is below 1e-17 |w|).
+// (iz)^n by complex products keeps Im w proportional to x next to the
+// imaginary axis; 1/Gamma(n/2 + 1) by c_{n+2} = c_n / (n/2 + 1).
+__host__ __device__ inline void fad_maclaurin(double x, double y, double &wr, double &wi)
 {
-#pragma clang fp contract(fast)
-    const double L = c_weid[kWeidN];
-    // 1/(L - iz) = u, Z = (L + iz) u
-    const double dr = L + zi, di = -zr;
-    const double inv = 1.0 / (dr * dr + di * di);
-    const double ur = dr * inv, ui = -di * inv;
-    const double nr = L - zi, ni = zr;
-    const double Zr = nr * ur - ni * ui, Zi = nr * ui + ni * ur;
-    double pr = c_weid[0], pi = 0.0;
-#pragma unroll 8
-    for (int k = 1; k < kWeidN; k++) {
-        const double t = pr * Zr - pi * Zi + c_weid[k];
-        pi = pr * Zi + pi * Zr;
+    const double ur = -y, ui = x;  // iz
+    double pr = 1.0, pi = 0.0;      // (iz)^n
+    double ce = 1.0, co = 2.0 * kIsPi;  // 1/Gamma(n/2+1), n even / odd
+    double sr = 1.0, si = 0.0;
+    for (int n = 1; n < 48; n++) {
+        const double t = pr * ur - pi * ui;
+        pi = pr * ui + pi * ur;
         pr = t;
-    }
-    const double u2r = ur * ur - ui * ui, u2i = 2.0 * ur * ui;
-    wr = 2.0 * (pr * u2r - pi * u2i) + kIsPi * ur;
-    wi = 2.0 * (pr * u2i + pi * u2r) + kIsPi * ui;
-}
-
-// Continued fraction / asymptotic forms for large |z| (the package's
-// first region, src/Faddeeva.cpp:740-790), for xs = Re z reflected to
-// Im z >= 0 and ya = |Im z|; x = |xs|.
-__device__ __forceinline__ void w_cf(double xs, double ya, double x, double &retr, double &reti, bool &inf_case)
-{
-    inf_case = false;
-    if (x + ya > 4000) {
-        if (x + ya > 1e7) {
-            if (x > ya) {
-                const double yax = ya / xs;
-                const double denom = kIsPi / (xs + yax * ya);
-                retr = denom * yax; reti = denom;
-            } else if (isinf(ya)) {
-                inf_case = true;
-                retr = 0; reti = 0;
-            } else {
-                const double xya = xs / ya;
-                const double denom = kIsPi / (xya * xs + ya);
-                retr = denom; reti = denom * xya;
-            }
+        double c;
+        if (n & 1) {
+            c = co;
+            co = co / (0.5 * n + 1.0);
         } else {
-            const double dr = xs * xs - ya * ya - 0.5, di = 2 * xs * ya;
-            const double denom = kIsPi / (dr * dr + di * di);
-            retr = denom * (xs * di - ya * dr);
-            reti = denom * (xs * dr + ya * di);
+            ce = ce / (0.5 * n);
+            c = ce;
         }
+        sr += c * pr;
+        si += c * pi;
+    }
+    wr = sr;
+    wi = si;
+}
+
+// i / (sqrt(pi) z) (1 + 1/(2 z^2)) for |z| >= 1e7, with Smith's division
+// (no overflow for |z| up to DBL_MAX).
+__host__ __device__ inline void fad_asymptotic(double x, double y, double &wr, double &wi)
+{
+    double ir, ii;  // 1/z
+    if (fabs(x) >= fabs(y)) {
+        const double r = y / x, d = x + y * r;
+        ir = 1.0 / d;
+        ii = -r / d;
     } else {
-        // depth fit of the reference's package (src/Faddeeva.cpp:758-759)
-        double nu = floor(3.9 + 11.398 / (0.08254 * x + 0.1421 * ya + 0.2023));
-        double wr = xs, wi = ya;
-        for (nu = 0.5 * (nu - 1); nu > 0.4; nu -= 0.5) {
-            const double denom = nu / (wr * wr + wi * wi);
-            wr = xs - wr * denom;
-            wi = ya + wi * denom;
-        }
-        const double denom = kIsPi / (wr * wr + wi * wi);
-        retr = denom * wi;
-        reti = denom * wr;
+        const double r = x / y, d = x * r + y;
+        ir = r / d;
+        ii = -1.0 / d;
+    }
+    // 1 + (1/z)^2 / 2
+    const double fr = 1.0 + 0.5 * (ir * ir - ii * ii), fi = ir * ii;
+    // i (1/z) f / sqrt(pi)
+    const double gr = ir * fr - ii * fi, gi = ir * fi + ii * fr;
+    wr = -kIsPi * gi;
+    wi = kIsPi * gr;
+}
+
+// Laplace continued fraction with n levels, bottom up: r = z, then
+// r = z - (k/2)/r for k = n..1; w = i / (sqrt(pi) r).
+__host__ __device__ inline void fad_cf(double x, double y, int n, double &wr, double &wi)
+{
+    double rr = x, ri = y;
+    for (int k = n; k >= 1; k--) {
+        const double d = 0.5 * k / (rr * rr + ri * ri);
+        rr = x - rr * d;
+        ri = y + ri * d;
+    }
+    const double d = kIsPi / (rr * rr + ri * ri);
+    wr = ri * d;
+    wi = rr * d;
+}
+
+__host__ __device__ inline int fad_cf_depth(double r2)
+{
+    return r2 < 100.0 ? 18 : r2 < 400.0 ? 11 : r2 < 2500.0 ? 7 : r2 < 1e5 ? 4 : 2;
+}
+
+// Trapezoidal rule, nodes +-(n + 1/2) h summed in pairs, with the pole
+// correction; y >= 0.5.  With t^2 the node, A = x^2 - y^2 - t^2, B = 2xy:
+// i (1/(z - t) + 1/(z + t)) = 2 (y (|z|^2 + t^2) + i x (|z|^2 - t^2)) / (A^2 + B^2).
+__host__ __device__ inline void fad_trap_sym(double x, double y, double &wr, double &wi)
+{
+    const double z2 = x * x + y * y, d2 = x * x - y * y, b2 = 4.0 * x * x * y * y;
+    double sr = 0.0, si = 0.0;
+    for (int n = 0; n < 18; n++) {
+        const double t = (n + 0.5) * kFadH, t2 = t * t;
+        const double a = d2 - t2;
+        const double g = fad_gauss(t) / (a * a + b2);
+        sr += g * (z2 + t2);
+        si += g * (z2 - t2);
+    }
+    const double k = 2.0 * kFadH / 3.14159265358979323846;
+    wr = k * y * sr;
+    wi = k * x * si;
+    // 2 exp(-z^2) / (1 + exp(2 pi y/h) exp(-i theta)), theta = 2 pi x/h:
+    // with q = exp(-2 pi y/h),
+    // = 2 q exp(y^2 - x^2) (exp(i(theta - 2xy)) + q exp(-2ixy)) / (1 + 2 q cos theta + q^2)
+    const double ex = (y - x) * (y + x) - 2.0 * kFadPiOverH * y;
+    if (ex > -745.0) {
+        const double q = exp(-2.0 * kFadPiOverH * y);
+        const double th = 2.0 * kFadPiOverH * x, p = 2.0 * x * y;
+        const double s = 2.0 * exp(ex) / (1.0 + 2.0 * q * cos(th) + q * q);
+        wr += s * (cos(th - p) + q * cos(p));
+        wi += s * (sin(th - p) - q * sin(p));
     }
 }
 
-// The package's sums for |x| >= 10 next to the real axis
-// (src/Faddeeva.cpp:880-945); returns w.
-__device__ inline void w_large_x(double zr, double y, double x, double &wr_out, double &wi_out)
+// Trapezoidal rule on nodes x + (m + 1/2) h, |t| <= 7, with the pole
+// correction exp(-z^2) (1 - tanh(pi y/h)); y < 0.5.  Each node adds
+// (h/pi) exp(-t^2) (y - i s) / (s^2 + y^2), s = t - x.
+__host__ __device__ inline void fad_trap_centred(double x, double y, double &wr, double &wi)
 {
-    const double a = c_fad[0], c = c_fad[1], a2 = c_fad[2];
-    const double relerr = 2.220446049250313e-16;
-    double sum3, sum5;
-    const double retr = exp(-x * x);
-    const double n0 = floor(x / a + 0.5);
-    const double dx = a * n0 - x;
-    sum3 = exp(-dx * dx) / (a2 * (n0 * n0) + y * y);
-    sum5 = a * n0 * sum3;
-    const double exp1 = exp(4 * a * dx);
-    double exp1dn = 1;
-    int dn;
-    bool done = false;
-    for (dn = 1; n0 - dn > 0; ++dn) {
-        const double np = n0 + dn, nm = n0 - dn;
-        double tp = exp(-((a * dn + dx) * (a * dn + dx)));
-        double tm = tp * (exp1dn *= exp1);
-        tp /= (a2 * (np * np) + y * y);
-        tm /= (a2 * (nm * nm) + y * y);
-        sum3 += tp + tm;
-        sum5 += a * (np * tp + nm * tm);
-        if (a * (np * tp + nm * tm) < relerr * sum5) { done = true; break; }
+    const int m0 = (int)ceil((-kFadTMax - x) / kFadH - 0.5);
+    const int m1 = (int)floor((kFadTMax - x) / kFadH - 0.5);
+    const double y2 = y * y;
+    double sr = 0.0, si = 0.0;
+    for (int m = m0; m <= m1; m++) {
+        const double s = (m + 0.5) * kFadH;
+        const double g = fad_gauss(x + s) / (s * s + y2);
+        sr += g;
+        si += g * s;
     }
-    for (int guard = 0; !done && guard < 200; guard++) {
-        const double np = n0 + dn++;
-        const double tp = exp(-((a * dn + dx) * (a * dn + dx))) / (a2 * (np * np) + y * y);
-        sum3 += tp;
-        sum5 += a * np * tp;
-        if (a * np * tp < relerr * sum5) done = true;
-    }
-    wr_out = retr + (0.5 * c) * y * sum3;
-    wi_out = (0.5 * c) * copysign(sum5, zr);
-}
-
-__device__ __forceinline__ bool w_cf_region(double x, double ya)
-{
-    return ya > 7 || (x > 6 && (ya > 0.1 || (x > 8 && ya > 1e-10) || x > 28));
-}
-
-// w(z) for Re z >= 0, Im z >= 0, both finite and not on an axis (the
-// arguments of the Bassetti-Erskine field): continued fraction far out,
-// Weideman's approximation near the beam, the large-x sums near the axis.
-__device__ inline void w_quadrant1(double x, double y, double &wr, double &wi)
-{
-    if (w_cf_region(x, y)) {
-        bool inf_case;
-        w_cf(x, y, x, wr, wi, inf_case);
-        if (inf_case) { wr = 0; wi = 0; }
-    } else if (x < 10) {
-        w_weideman(x, y, wr, wi);
-    } else {
-        w_large_x(x, y, x, wr, wi);
+    const double k = kFadH / 3.14159265358979323846;
+    wr = k * y * sr;
+    wi = -k * si;
+    const double ex = (y - x) * (y + x);
+    if (ex > -745.0) {
+        const double e = exp(ex) * (2.0 / (1.0 + exp(2.0 * kFadPiOverH * y)));
+        const double p = 2.0 * x * y;
+        wr += e * cos(p);
+        wi -= e * sin(p);
     }
 }
 
-__device__ inline void w_general(double zr, double zi, double &wr_out, double &wi_out)
+// w(z) for x >= 0, y >= 0, both finite or infinite (not NaN).
+__host__ __device__ inline void fad_q1(double x, double y, double &wr, double &wi)
 {
-    const double a = c_fad[0], c = c_fad[1], a2 = c_fad[2];
-    const double relerr = 2.220446049250313e-16;
-    const double x = fabs(zr);
-    const double y = zi, ya = fabs(y);
-    double retr = 0, reti = 0;
-    double sum1 = 0, sum2 = 0, sum3 = 0, sum4 = 0, sum5 = 0;
-
-    if (w_cf_region(x, ya)) {
-        const double xs = y < 0 ? -zr : zr;
-        bool inf_case;
-        w_cf(xs, ya, x, retr, reti, inf_case);
-        if (inf_case) {
-            if (isnan(x) || y < 0) { wr_out = NAN; wi_out = NAN; }
-            else { wr_out = 0; wi_out = 0; }
-            return;
-        }
-        if (y < 0) {
-            // w(z) = 2 exp(-z^2) - w(-z); exp(-inf + i*anything) = 0
-            const double re = (ya - xs) * (xs + ya);
-            double er_c = 0.0, er_s = 0.0;
-            if (!(isinf(re) && re < 0)) {
-                const double er = exp(re);
-                double s, cph;
-                sincos(2 * xs * y, &s, &cph);
-                er_c = er * cph;
-                er_s = er * s;
+    if (std::isinf(x) || std::isinf(y)) { wr = 0.0; wi = 0.0; return; }
+    if (x >= 1e7 || y >= 1e7) { fad_asymptotic(x, y, wr, wi); return; }
+    const double r2 = x * x + y * y;
+    if (y >= 8.0 || x >= 15.0) {
+        fad_cf(x, y, fad_cf_depth(r2), wr, wi);
+        if (x >= 15.0 && y < 1.0) {
+            // the exponentially small exp(-z^2) next to the real axis
+            const double ex = (y - x) * (y + x);
+            if (ex > -745.0) {
+                const double e = exp(ex), p = 2.0 * x * y;
+                wr += e * cos(p);
+                wi -= e * sin(p);
             }
-            wr_out = 2.0 * er_c - retr;
-            wi_out = 2.0 * er_s - reti;
-        } else {
-            wr_out = retr; wi_out = reti;
         }
         return;
-    } else if (x < 10) {
-        double prod2ax = 1, prodm2ax = 1, expx2;
-        if (isnan(y)) { wr_out = y; wi_out = y; return; }
-        if (y >= 0) {
-            w_weideman(zr, y, wr_out, wi_out);
-            return;
-        }
-        if (x < 5e-4) {
-            const double x2 = x * x;
-            expx2 = 1 - x2 * (1 - 0.5 * x2);
-            const double ax2 = (2 * a) * x;
-            const double exp2ax = 1 + ax2 * (1 + ax2 * (0.5 + 0.166666666666666666667 * ax2));
-            const double expm2ax = 1 - ax2 * (1 - ax2 * (0.5 - 0.166666666666666666667 * ax2));
-            for (int n = 1; n <= 64; ++n) {
-                const double coef = c_ea2n2[n - 1] * expx2 / (a2 * (n * n) + y * y);
-                prod2ax *= exp2ax;
-                prodm2ax *= expm2ax;
-                sum1 += coef;
-                sum2 += coef * prodm2ax;
-                sum3 += coef * prod2ax;
-                sum5 += coef * (2 * a) * n * fad_sinh_taylor((2 * a) * n * x);
-                if (coef * prod2ax < relerr * sum3) break;
-            }
-        } else {
-            expx2 = exp(-x * x);
-            const double exp2ax = exp((2 * a) * x), expm2ax = 1 / exp2ax;
-            for (int n = 1; n <= 64; ++n) {
-                const double coef = c_ea2n2[n - 1] * expx2 / (a2 * (n * n) + y * y);
-                prod2ax *= exp2ax;
-                prodm2ax *= expm2ax;
-                sum1 += coef;
-                sum2 += coef * prodm2ax;
-                sum4 += (coef * prodm2ax) * (a * n);
-                sum3 += coef * prod2ax;
-                sum5 += (coef * prod2ax) * (a * n);
-                if ((coef * prod2ax) * (a * n) < relerr * sum5) break;
-            }
-        }
-        const double expx2erfcxy = y > -6 ? expx2 * cts_erfcx(y) : 2 * exp(y * y - x * x);
-        if (y > 5) {
-            const double sinxy = sin(x * y);
-            retr = (expx2erfcxy - c * y * sum1) * cos(2 * x * y) +
-                   (c * x * expx2) * sinxy * fad_sinc(x * y, sinxy);
-            reti = 0;
-        } else {
-            const double xs = zr;
-            const double sinxy = sin(xs * y);
-            double sin2xy, cos2xy;
-            sincos(2 * xs * y, &sin2xy, &cos2xy);
-            const double coef1 = expx2erfcxy - c * y * sum1;
-            const double coef2 = c * xs * expx2;
-            retr = coef1 * cos2xy + coef2 * sinxy * fad_sinc(xs * y, sinxy);
-            reti = coef2 * fad_sinc(2 * xs * y, sin2xy) - coef1 * sin2xy;
-        }
-    } else {
-        if (isnan(x)) { wr_out = x; wi_out = x; return; }
-        if (isnan(y)) { wr_out = y; wi_out = y; return; }
-        w_large_x(zr, y, x, wr_out, wi_out);
-        return;
     }
-    wr_out = retr + (0.5 * c) * y * (sum2 + sum3);
-    wi_out = reti + (0.5 * c) * copysign(sum5 - sum4, zr);
+    if (r2 < 1.0) { fad_maclaurin(x, y, wr, wi); return; }
+    if (y >= 0.5) fad_trap_sym(x, y, wr, wi);
+    else fad_trap_centred(x, y, wr, wi);
 }
 
-__device__ inline double cts_w_im(double x)
+// Im w(x) = 2 Dawson(x) / sqrt(pi) on the real axis.
+__host__ __device__ inline double cts_w_im(double x)
 {
-    if (fabs(x) < 0.5) {
+    if (std::isnan(x)) return x;
+    const double ax = fabs(x);
+    double v;
+    if (ax < 0.5) {
         const double m2x2 = -2.0 * x * x;
         double term = x, sum = x;
         for (int n = 1; n < 40; n++) {
@@ -335,24 +278,49 @@ __device__ inline double cts_w_im(double x)
         }
         return 2 * kIsPi * sum;
     }
-    double wr, wi;
-    w_general(x, 0.0, wr, wi);
-    return wi;
+    double wr;
+    if (std::isinf(ax)) v = 0.0;
+    else if (ax < 15.0) fad_trap_centred(ax, 0.0, wr, v);
+    else fad_cf(ax, 0.0, fad_cf_depth(ax * ax), wr, v);
+    return x < 0 ? -v : v;
 }
 
-__device__ inline void faddeeva_w(double re, double im, double &wr, double &wi)
+__host__ __device__ inline void faddeeva_w(double re, double im, double &wr, double &wi)
 {
+    if (std::isnan(re) || std::isnan(im)) {
+        if (re == 0.0) { wr = im; wi = 0.0; return; }  // w(i NaN)
+        wr = NAN; wi = NAN; return;
+    }
     if (re == 0.0) { wr = cts_erfcx(im); wi = re; return; }
-    if (im == 0) { wr = exp(-(re * re)); wi = cts_w_im(re); return; }
-    w_general(re, im, wr, wi);
+    if (im == 0.0) { wr = exp(-(re * re)); wi = cts_w_im(re); return; }
+    const double ax = fabs(re);
+    if (im > 0.0) {
+        fad_q1(ax, im, wr, wi);
+        if (re < 0.0) wi = -wi;
+        return;
+    }
+    // below the real axis: w(z) = 2 exp(-z^2) - w(-z), -z = -re + i |im|
+    double vr, vi;
+    fad_q1(ax, -im, vr, vi);
+    if (re > 0.0) vi = -vi;  // w(-re + i|im|) = conj w(re + i|im|) for re > 0
+    const double ex = (-im - ax) * (ax - im);  // im^2 - re^2 without overflow
+    double er = 0.0, ei = 0.0;
+    if (std::isnan(ex)) { wr = NAN; wi = NAN; return; }
+    if (ex > -745.0) {
+        const double e = 2.0 * exp(ex), p = -2.0 * re * im;
+        er = e * cos(p);
+        ei = e * sin(p);
+    }
+    wr = er - vr;
+    wi = ei - vi;
 }
 
 // w(z) for Re z >= 0, Im z >= 0 (any finite values, axes included).
-__device__ inline void faddeeva_w_q1(double re, double im, double &wr, double &wi)
+__host__ __device__ inline void faddeeva_w_q1(double re, double im, double &wr, double &wi)
 {
     if (re == 0.0) { wr = cts_erfcx(im); wi = re; return; }
-    if (im == 0) { wr = exp(-(re * re)); wi = cts_w_im(re); return; }
-    w_quadrant1(re, im, wr, wi);
+    if (im == 0.0) { wr = exp(-(re * re)); wi = cts_w_im(re); return; }
+    fad_q1(re, im, wr, wi);
 }
 
 // Bunch::BassettiErskine1 (src/Bunch.cpp:119-170)
@@ -379,13 +347,14 @@ __device__ inline void bassetti_erskine(double posx, double posy, double sx, dou
 // The field of one Gaussian beam evaluated at many points (the kicks of a
 // bunch passage): the beam's constants once, then per point
 // multiplications by reciprocals, and both w(z) of the Bassetti-Erskine
-// form by Weideman's approximation in one interleaved evaluation.  The
-// kicks take it on all of Re z < 100, 0 <= Im z < 1000, where its |w|
-// relative error stays below 3e-14 (checked against the reference's
-// package on the quadrant's regions: largest next to the real axis for
-// 6 < Re z < 10), so that the lanes of a wave do not split between the
-// package's regions; points beyond take faddeeva_w_q1.  Same formulas as
-// ion_field otherwise, rounding within a few ulps.
+// form by Weideman's approximation in one interleaved evaluation:
+// w(z) = 2 p(Z)/(L - iz)^2 + pi^{-1/2}/(L - iz), Z = (L + iz)/(L - iz), p of
+// degree N - 1 = 39, L = (N/sqrt 2)^{1/2}.  The kicks take it on all of
+// Re z < 100, 0 <= Im z < 1000, where its |w| relative error stays below
+// 3e-14 (largest next to the real axis for 6 < Re z < 10), so that the lanes
+// of a wave do not split between methods; points beyond take
+// faddeeva_w_q1.  Same formulas as ion_field otherwise, rounding within a
+// few ulps.
 __device__ inline void gaussian_field(double posx, double posy, double sx, double &fx, double &fy);
 
 struct BeField {
@@ -419,7 +388,8 @@ __device__ __forceinline__ bool w_kick_region(double x, double y)
     return x < 100.0 && y < 1000.0;
 }
 
-// w_weideman at two points, the two Horner recurrences interleaved.
+// Weideman's approximation at two points, the two Horner recurrences
+// interleaved.
 __device__ __forceinline__ void w_weideman2(const double zr[2], const double zi[2], double wr[2], double wi[2])
 {
 #pragma clang fp contract(fast)

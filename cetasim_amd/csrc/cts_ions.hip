@@ -379,10 +379,7 @@ k_compact_scatter(IonCompactParams P)
 // this translation unit's copy of the Faddeeva tables
 hipError_t upload_faddeeva_tables_ions(const FadTables &t)
 {
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_ea2n2), t.ea2n2, 64 * sizeof(double));
-    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_fad), t.fad, 3 * sizeof(double));
-    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_weid), t.weid, (kWeidN + 1) * sizeof(double));
-    return e;
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_weid), t.weid, (kWeidN + 1) * sizeof(double));
 }
 
 // wave_bound: a host bound of the segment's waves; the kernel reads the

@@ -17,10 +17,7 @@ namespace cts {
 
 hipError_t upload_faddeeva_tables(const FadTables &t)
 {
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_ea2n2), t.ea2n2, 64 * sizeof(double));
-    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_fad), t.fad, 3 * sizeof(double));
-    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_weid), t.weid, (kWeidN + 1) * sizeof(double));
-    return e;
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_weid), t.weid, (kWeidN + 1) * sizeof(double));
 }
 
 // Streaming (nontemporal) global accesses.

@@ -373,10 +373,7 @@ hipError_t launch_mlp_field_error(const float *W, const double *exact, int64_t n
 
 hipError_t upload_faddeeva_tables_mlp(const FadTables &t)
 {
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_ea2n2), t.ea2n2, 64 * sizeof(double));
-    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_fad), t.fad, 3 * sizeof(double));
-    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_weid), t.weid, (kWeidN + 1) * sizeof(double));
-    return e;
+    return hipMemcpyToSymbol(HIP_SYMBOL(c_weid), t.weid, (kWeidN + 1) * sizeof(double));
 }
 
 }  // namespace cts

@@ -51,6 +51,22 @@ def test_kat_57_points():
     assert worst <= kat["tolerance"]
 
 
+def test_against_mpmath_values():
+    """2400 points of tests/golden/wofz_mpmath.json (400-digit values): the
+    oracle's w(z) to 1e-13 in |w| and 5e-13 per component (the reference
+    package's own accuracy next to the imaginary axis is about 2e-13)."""
+    pts = json.load(open(os.path.join(HERE, "golden", "wofz_mpmath.json")))["points"]
+    worst_abs = worst_comp = 0.0
+    for a, b, c, d in pts:
+        z = complex(float(a), float(b))
+        e = complex(float(c), float(d))
+        got = w(z)
+        worst_abs = max(worst_abs, abs(got - e) / abs(e))
+        worst_comp = max(worst_comp, relerr(e.real, got.real), relerr(e.imag, got.imag))
+    assert worst_abs <= 1e-13, worst_abs
+    assert worst_comp <= 5e-13, worst_comp
+
+
 def test_against_reference_build():
     """oracle/_ref is the reference's Faddeeva.cpp compiled as shipped."""
     R = O.ref_lib()

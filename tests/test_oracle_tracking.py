@@ -11,6 +11,7 @@ import oracle as O
 
 L = O.lib()
 C = 299792458.0
+E_CHARGE = 1.602017733e-19  # include/Global.h:28
 
 
 def gaussian(n, seed, sx=20e-6, sy=4e-6, sz=3e-3, sd=1e-3, beta=10.0):
@@ -267,6 +268,69 @@ def test_bbr_wake_against_quadrature():
         left = integ + 0.5 * dz * (f(0.0) - f(zmax))
         exact = math.sqrt(2 / math.pi) * rs * C / sig * left
         assert w[2] == pytest.approx(exact, rel=1e-5)
+
+
+def test_sr_wake_kick_is_the_convolution():
+    """The bin kick is -e q / E times the discrete convolution of the bin
+    counts with the lag table (src/MPBunch.cpp:1983-1998), computed here by
+    numpy, plus the previous call's potential carried over (the reference
+    never zeroes srWakePoten, src/MPBunch.cpp:1894-1897); the table's sign
+    makes the bunch lose energy."""
+    n, nb = 4000, 60
+    parts = gaussian(n, 23)
+    rs, q, om = np.array([1e4]), np.array([1.0]), np.array([2 * math.pi * 1e10])
+    qmp, energy = 6.24e4, 6.0e9
+    grid = (-0.015, 0.015)
+    dz = (grid[1] - grid[0]) / nb
+    tab = np.zeros(2 * nb - 1)
+    L.orc_wake_table(nb, dz / C, 1, O.p(rs), O.p(q), O.p(om), O.p(tab))
+    for carry in (0.0, 1.0):
+        b = O.Bunch(parts)
+        poten = np.zeros(3 * nb)
+        poten[2 * nb:] = carry * np.linspace(-1e-3, 2e-3, nb)
+        prev = poten[2 * nb:].copy()
+        counts = np.zeros(nb, np.int32); edges = np.zeros(2)
+        L.orc_sr_wake(ctypes.byref(b.s), nb, 1, 0.0, grid[0], grid[1], qmp, energy, 1, O.p(rs), O.p(q), O.p(om),
+                      O.p(poten), counts.ctypes.data_as(O.PI32), O.p(edges), O.p(tab))
+        conv = np.convolve(counts.astype(float), tab)[nb - 1:2 * nb - 1]
+        expect = (prev + conv) * (-E_CHARGE * qmp / energy)
+        got = poten[2 * nb:]
+        assert np.allclose(got, expect, rtol=1e-12, atol=1e-12 * np.max(np.abs(expect)))
+        if carry == 0.0:
+            # energy loss: the count-weighted mean kick is negative
+            assert np.dot(counts, got) < 0
+
+
+def test_bbr_wake_trailing_side_quadrature():
+    """u < 0 (a receiver bin behind its source): the Riemann sum against
+    the integral over the same truncated range [0, 3 v / (2 Q)]; the
+    Gaussian's peak moves to z* = -u v / (2 Q), up to the cut-off at u = -3
+    (the range the reference sums, not Chao's untruncated integral)."""
+    from scipy.integrate import quad
+    rs, Q, fr = 1e4, 1.0, 1e10
+    w = np.zeros(3)
+    om = 2 * math.pi * fr
+    sig = 1e-3
+    v = om * sig / C
+    sq = math.sqrt(4 * Q * Q - 1)
+    zmax = 3 * v / (2 * Q)
+    vals = []
+    for u in (-3.0, -1.5, -0.5):
+        L.orc_bbr_wake(u * sig / C, 1, O.p(np.array([rs])), O.p(np.array([Q])), O.p(np.array([om])), None, None,
+                       None, None, None, None, O.p(w))
+        f = lambda z: math.exp(-0.5 * (u + 2 * Q / v * z) ** 2 - z) * (math.cos(z * sq) - math.sin(z * sq) / sq)
+        integ = quad(f, 0, zmax, limit=200, epsabs=0, epsrel=1e-13)[0]
+        dz = zmax / 1000
+        left = integ + 0.5 * dz * (f(0.0) - f(zmax))
+        vals.append((w[2], math.sqrt(2 / math.pi) * rs * C / sig * left))
+    scale = max(abs(e) for _, e in vals)
+    for got, exact in vals:
+        assert abs(got - exact) <= 1e-5 * scale
+    # the truncation: at u = -3 the untruncated integral is far larger
+    u = -3.0
+    f = lambda z: math.exp(-0.5 * (u + 2 * Q / v * z) ** 2 - z) * (math.cos(z * sq) - math.sin(z * sq) / sq)
+    full = quad(f, 0, 40 * zmax, limit=400)[0]
+    assert abs(full) > 1.2 * abs(vals[0][1] / (math.sqrt(2 / math.pi) * rs * C / sig))
 
 
 def test_philox_known_answers():
