@@ -113,41 +113,84 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n * 8, "launches": 3}))
 
 
-def cpu_baseline(ring, secs, p, budget_s):
-    """The oracle (-O2 C restatement, one core) on a sample of the same
-    workload: 20 000 macroparticles, same ring, bins, wake and excitation."""
+def host_cpu():
+    """CPU model and logical CPU counts (all, and this process's affinity)."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
+def wake_pairs_seconds(O, nbins, dt, bbr):
+    """One bunch's wake sum evaluated as the reference does, one pseudo-Green
+    call per bin pair (src/MPBunch.cpp:1983-1993): seconds per call,
+    timed at 100 bins and scaled by the pair count."""
+    import numpy as np
+    nb0 = min(nbins, 100)
+    rs, q = np.array(bbr["rs"], float), np.array(bbr["q"], float)
+    om = np.array(bbr["fr"], float) * 2 * np.pi
+    counts = np.full(nb0, 1000, np.int32)
+    pz = np.zeros(nb0)
+    t0 = time.perf_counter()
+    O.lib().orc_wake_pairs(nb0, dt, len(rs), O.p(rs), O.p(q), O.p(om), counts.ctypes.data_as(O.PI32), O.p(pz))
+    return (time.perf_counter() - t0) * (nbins / nb0) ** 2
+
+
+def cpu_baseline(ring, secs, p, budget_s, n_gpu, nbins):
+    """The oracle (C restatement, one core) on a sample of the same workload:
+    turns of 20 000 macroparticles with the same ring, bins, wake and
+    excitation, and the wake sum evaluated per bin pair as the reference
+    does; built at -O0 (the reference Makefile's level; `value`) and -O2."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import numpy as np  # noqa: F401
     import oracle as O
     from cetasim_amd import configs
 
     n = 20_000
-    parts = configs.gaussian_bunch(n, 2, p)
-    q, _ = configs.bunch_scalars([n], ring, p)
     sz = p["sig"][2]
-    ot = O.OracleTurn(ring, secs[0], nbins=400, mode=1, grid=(-5 * sz, 5 * sz), bbr=p["bbr"],
-                      macro_charge=q[0], synrad=True, noise_seed=2)
-    ob = O.Bunch(parts)
-    ot.turn(ob, 0)
-    turns = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        ot.turn(ob, turns + 1)
-        turns += 1
-    dt = time.perf_counter() - t0
-    return {"value": n * turns / dt, "unit": "macroparticle-turns/s", "cores": 1, "kind": "port",
-            "sample": f"{n} macroparticles x {turns} turns of the same ring/wake/excitation "
-                      f"(oracle/ -O2 C restatement, {platform.processor() or platform.machine()})"}
+    out = {}
+    for build in ("O0", "O2"):
+        O.set_build(build)
+        parts = configs.gaussian_bunch(n, 2, p)
+        q, _ = configs.bunch_scalars([n], ring, p)
+        ot = O.OracleTurn(ring, secs[0], nbins=nbins, mode=1, grid=(-5 * sz, 5 * sz), bbr=p["bbr"],
+                          macro_charge=q[0], synrad=True, noise_seed=2)
+        ob = O.Bunch(parts)
+        ot.turn(ob, 0)
+        turns = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < budget_s / 2 or turns < 2:
+            ot.turn(ob, turns + 1)
+            turns += 1
+        t_mp = (time.perf_counter() - t0) / (n * turns)
+        t_pairs = wake_pairs_seconds(O, nbins, 10 * sz / nbins / configs.C_LIGHT, p["bbr"])
+        out[build] = (n_gpu / (n_gpu * t_mp + t_pairs), t_mp, t_pairs, turns)
+    O.set_build("O2")
+    v0, t_mp0, t_pairs0, turns0 = out["O0"]
+    return {"value": v0, "unit": "macroparticle-turns/s", "cores": 1, "kind": "port",
+            "sample": f"restatement of the reference's path (oracle/, not the reference build), -O0 as the "
+                      f"reference Makefile: {t_mp0 * 1e9:.0f} ns per macroparticle-turn ({n} macroparticles x "
+                      f"{turns0} turns, same ring, wake, excitation) x {n_gpu} macroparticles + the wake sum per "
+                      f"bin pair as the reference evaluates it ({nbins}^2 pairs, {t_pairs0:.2f} s per turn)",
+            "value_O2": out["O2"][0], "value_O2_lag_table_wake": 1.0 / out["O2"][1], **host_cpu()}
 
 
-def cpu_baseline_train(ring, secs, p, budget_s, T, ion_evals, ntot):
-    """The oracle (one core) on bounded samples of the train, extrapolated to
-    the workload: the ion phase per macro-ion passage (a 40-bunch train of
-    300-macroparticle bunches, 1000 new macro-ions per passage), the
-    electron-side kick per macroparticle (one 200 000-macroparticle bunch
-    through a 10-ion cloud) and the rest of the turn per macroparticle
-    (turns of a 20 000-macroparticle bunch with the workload's wake and
-    excitation); ion_evals = macro-ion passages per turn of the GPU run."""
+def cpu_baseline_train(ring, secs, p, budget_s, T, ion_evals, ntot, nb):
+    """The oracle (C restatement, one core) on bounded samples of the train,
+    extrapolated to the workload: the ion phase per macro-ion passage (a
+    40-bunch train of 300-macroparticle bunches, 1000 new macro-ions per
+    passage), the electron-side kick per macroparticle (one 200 000-
+    macroparticle bunch through a 10-ion cloud), the rest of the turn per
+    macroparticle (turns of a 20 000-macroparticle bunch with the workload's
+    wake and excitation) and, with a wake, each bunch's wake sum evaluated
+    per bin pair as the reference does; ion_evals = macro-ion passages per
+    turn of the GPU run.  Built at -O0 (the reference Makefile's level;
+    `value`) and -O2."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
     import oracle as O
@@ -155,42 +198,54 @@ def cpu_baseline_train(ring, secs, p, budget_s, T, ion_evals, ntot):
 
     def phase(counts, ions):
         counts = np.asarray(counts)
-        nb = len(counts)
-        gap = configs.train_gaps(nb, T["spacing"], ring.harmonic)
+        nbp = len(counts)
+        gap = configs.train_gaps(nbp, T["spacing"], ring.harmonic)
         cfg = configs.ion_config(ring, ions, p)
         q, ne = configs.bunch_scalars(counts, ring, p)
         parts = configs.train_bunches(counts, 3, p)
         off = np.concatenate([[0], np.cumsum(counts)])
-        bunches = [O.Bunch({c: parts[c][off[b]:off[b + 1]] for c in O.COORDS}) for b in range(nb)]
+        bunches = [O.Bunch({c: parts[c][off[b]:off[b + 1]] for c in O.COORDS}) for b in range(nbp)]
         ot = O.OracleTrain(ring, secs[0], counts, ne, gap, cfg, ions["seed"])
         t0 = time.perf_counter()
         out = ot.ion_phase(bunches, 0)
         return time.perf_counter() - t0, float(out[:, :, 0].sum())
 
-    t_ph, evals = phase(np.full(40, 300), dict(T["ions"], n_new=1000))
-    t_eval = t_ph / evals
-    ne_big = 200_000
-    t_k, _ = phase([ne_big], dict(T["ions"], n_new=10))
-    t_e = t_k / ne_big
     n = 20_000
     sz = p["sig"][2]
-    kw = dict(nbins=T["nbins"], mode=1, grid=(-5 * sz, 5 * sz), bbr=p["bbr"],
-              macro_charge=configs.bunch_scalars([n], ring, p)[0][0]) if T.get("nbins") else {}
-    ot = O.OracleTurn(ring, secs[0], synrad=True, noise_seed=2, **kw)
-    ob = O.Bunch(configs.gaussian_bunch(n, 2, p))
-    turns = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s or turns < 2:
-        ot.turn(ob, turns)
-        turns += 1
-    t_rest = (time.perf_counter() - t0) / (n * turns)
-    t_turn = ion_evals * t_eval + ntot * (t_e + t_rest)
-    return {"value": ntot / t_turn, "unit": "macroparticle-turns/s", "cores": 1, "kind": "port",
-            "sample": f"ion phase {t_eval * 1e9:.0f} ns per macro-ion passage (40-bunch sample, {evals:.0f} "
-                      f"passages, {t_ph:.2f} s) x {ion_evals:.3g} passages per turn of the GPU run; "
-                      f"electron-side kick {t_e * 1e9:.0f} ns and rest of the turn {t_rest * 1e9:.0f} ns per "
-                      f"macroparticle ({turns} turns of {n}) x {ntot} macroparticles "
-                      f"(oracle/ -O2 C restatement, {platform.processor() or platform.machine()})"}
+    res = {}
+    for build in ("O0", "O2"):
+        O.set_build(build)
+        t_ph, evals = phase(np.full(40, 300), dict(T["ions"], n_new=1000))
+        t_eval = t_ph / evals
+        ne_big = 200_000
+        t_k, _ = phase([ne_big], dict(T["ions"], n_new=10))
+        t_e = t_k / ne_big
+        kw = dict(nbins=T["nbins"], mode=1, grid=(-5 * sz, 5 * sz), bbr=p["bbr"],
+                  macro_charge=configs.bunch_scalars([n], ring, p)[0][0]) if T.get("nbins") else {}
+        ot = O.OracleTurn(ring, secs[0], synrad=True, noise_seed=2, **kw)
+        ob = O.Bunch(configs.gaussian_bunch(n, 2, p))
+        turns = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < budget_s / 2 or turns < 2:
+            ot.turn(ob, turns)
+            turns += 1
+        t_rest = (time.perf_counter() - t0) / (n * turns)
+        t_pairs = wake_pairs_seconds(O, T["nbins"], 10 * sz / T["nbins"] / configs.C_LIGHT, p["bbr"]) \
+            if T.get("nbins") else 0.0
+        t_turn = ion_evals * t_eval + ntot * (t_e + t_rest) + nb * t_pairs
+        res[build] = dict(value=ntot / t_turn, t_eval=t_eval, evals=evals, t_ph=t_ph, t_e=t_e, t_rest=t_rest,
+                          turns=turns, t_pairs=t_pairs, lag=ntot / (t_turn - nb * t_pairs))
+    O.set_build("O2")
+    r = res["O0"]
+    return {"value": r["value"], "unit": "macroparticle-turns/s", "cores": 1, "kind": "port",
+            "sample": f"restatement of the reference's path (oracle/, not the reference build), -O0 as the "
+                      f"reference Makefile: ion phase {r['t_eval'] * 1e9:.0f} ns per macro-ion passage (40-bunch "
+                      f"sample, {r['evals']:.0f} passages, {r['t_ph']:.2f} s) x {ion_evals:.3g} passages per turn "
+                      f"of the GPU run; electron-side kick {r['t_e'] * 1e9:.0f} ns and rest of the turn "
+                      f"{r['t_rest'] * 1e9:.0f} ns per macroparticle ({r['turns']} turns of {n}) x {ntot} "
+                      f"macroparticles; wake sum per bin pair as the reference evaluates it "
+                      f"{r['t_pairs']:.2f} s per bunch-turn x {nb} bunches",
+            "value_O2": res["O2"]["value"], "value_O2_lag_table_wake": res["O2"]["lag"], **host_cpu()}
 
 
 def train_surrogate(steps):
@@ -347,7 +402,8 @@ def run_train(a):
         "surrogate": sur_info,
     }
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline_train(ring, secs, p, a.cpu_seconds, T, float(cloud[:, :, 0].sum()), n)
+        out["cpu_baseline"] = cpu_baseline_train(ring, secs, p, a.cpu_seconds, T, float(cloud[:, :, 0].sum()), n,
+                                                 nb)
     print(json.dumps(out))
 
 
@@ -419,7 +475,7 @@ def main():
         "transmission": m["transmission"],
     }
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(ring, secs, p, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(ring, secs, p, a.cpu_seconds, n, a.nbins)
     print(json.dumps(out))
 
 

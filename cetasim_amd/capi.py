@@ -117,6 +117,7 @@ SIGNATURES = {
     "cts_set_synrad": (I32, [ctypes.c_void_p, I32, I32, U64]),
     "cts_push_noise": (I32, [ctypes.c_void_p, PD]),
     "cts_set_ions": (I32, [ctypes.c_void_p, ctypes.POINTER(IonConfig)]),
+    "cts_set_ions_ip": (I32, [ctypes.c_void_p, I32, ctypes.POINTER(IonConfig)]),
     "cts_set_lrwake": (I32, [ctypes.c_void_p, ctypes.POINTER(LrwakeConfig)]),
     "cts_get_lrwake_table": (I32, [ctypes.c_void_p, PI32, PD, PD, PD, PD]),
     "cts_get_lrwake_kick": (I32, [ctypes.c_void_p, PD]),
@@ -125,8 +126,12 @@ SIGNATURES = {
     "cts_set_bbimp": (I32, [ctypes.c_void_p, ctypes.POINTER(BbimpConfig)]),
     "cts_get_bbimp_wake": (I32, [ctypes.c_void_p, ctypes.POINTER(I32), PD, PD]),
     "cts_ion_phase": (I32, [ctypes.c_void_p, I32]),
+    "cts_ion_phase_ip": (I32, [ctypes.c_void_p, I32, I32]),
     "cts_get_ion_cloud": (I32, [ctypes.c_void_p, I32, I64, PI64, PD, PD, PD, PD]),
+    "cts_get_ion_cloud_ip": (I32, [ctypes.c_void_p, I32, I32, I64, PI64, PD, PD, PD, PD]),
     "cts_get_ion_passages": (I32, [ctypes.c_void_p, PD]),
+    "cts_get_ion_passages_ip": (I32, [ctypes.c_void_p, I32, PD]),
+    "cts_get_ion_charge": (I32, [ctypes.c_void_p, PD]),
     "cts_turn": (I32, [ctypes.c_void_p, I32]),
     "cts_flush": (I32, [ctypes.c_void_p]),
     "cts_run_turns": (I32, [ctypes.c_void_p, I32, I32, I32]),
@@ -145,6 +150,7 @@ SIGNATURES = {
     "cts_get_history": (I32, [ctypes.c_void_p, I32, I32, ctypes.POINTER(Moments)]),
     "cts_get_wake_state": (I32, [ctypes.c_void_p, PI32, PD, PD]),
     "cts_derive_moments": (I32, [ctypes.c_void_p, ctypes.POINTER(Moments), I32, ctypes.POINTER(Derived)]),
+    "cts_eigen_emit": (I32, [ctypes.c_void_p, PD]),
     "cts_sync": (I32, [ctypes.c_void_p]),
     "cts_set_timing": (I32, [ctypes.c_void_p, I32]),
     "cts_kernel_time": (I32, [ctypes.c_void_p, ctypes.c_char_p, PD, PI64]),
@@ -162,6 +168,7 @@ SIGNATURES = {
     "cts_sur_train": (I32, [ctypes.c_void_p, I32, I32, I32, ctypes.c_float, ctypes.c_float, U64, PF]),
     "cts_sur_gradient": (I32, [ctypes.c_void_p, PF, I64, PF, PF]),
     "cts_sur_eval": (I32, [ctypes.c_void_p, PF, I64, PF]),
+    "cts_sur_eval_noalloc": (I32, [ctypes.c_void_p, PF, I64, PF]),
     "cts_sur_validate": (I32, [ctypes.c_void_p, PD, PD, PD]),
     "cts_set_field_model": (I32, [ctypes.c_void_p, ctypes.c_void_p, I32]),
     "cts_cbm_growth_from_moments": (I32, [ctypes.POINTER(Moments), I32, I32, PI32, I32, PD, PD, I32, I32, I32, D,

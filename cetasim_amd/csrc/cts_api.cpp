@@ -64,6 +64,30 @@ struct Timer {
 
 }  // namespace
 
+// The ion cloud of one interaction point: two buffers of cap slots per
+// species (x, y, vx, vy, alive); the compaction after each segment of
+// passages swaps them.
+struct IonIp {
+    cts_ion_config cfg{};
+    int nspec = 0;
+    int buf = 0;
+    int64_t cap = 0;
+    int64_t len_ub = 0;  // host upper bound of the slots in use at the phase's start
+    int64_t wstride = 0;
+    double *d_ion[2][CTS_MAX_SPECIES][4] = {};
+    uint8_t *d_alive[2][CTS_MAX_SPECIES] = {};
+    int64_t *d_len = nullptr;     // [3][CTS_MAX_SPECIES]: the two buffers, ions at the phase's start
+    int64_t *d_nalive = nullptr;  // [CTS_MAX_SPECIES]
+    double *d_mean = nullptr;     // [2 * CTS_MAX_SPECIES]
+    double *d_smax = nullptr, *d_q = nullptr;
+    double *d_cloud = nullptr;
+    double *d_coeff = nullptr, *d_dt = nullptr;  // [nb][CTS_MAX_SPECIES], [nb]
+    unsigned long long *d_acc = nullptr;  // kIonReplicas + 1 copies of [nspec][nb][kIonAcc][3]
+    uint32_t *d_wcnt = nullptr;  // [nspec][wstride]: survivors per wave of a segment
+    uint32_t *d_wofs = nullptr;  // [wstride]: their prefix sums
+    std::vector<double> q;       // host copy of d_q: [nb][nspec] macro-ion charge per passage
+};
+
 struct cts_handle {
     cts_ring ring{};
     std::vector<cts_section> secs;
@@ -123,27 +147,13 @@ struct cts_handle {
     uint64_t seed = 0;
     double *d_noise = nullptr;
 
-    // ions (electron-side kick parameters per bunch)
-    int nspec = 0;
+    // ions: the electron-side kick parameters of the phase being run
+    // ([nb][species][5]) and the bunch views it reads ([nb][4]), shared by
+    // the interaction points (their phases run one after the other on the
+    // stream); one IonIp per section, nspec 0 where there are no ions
     double *d_ion_e = nullptr;
-    // ion cloud at interaction point 0: two buffers of ion_cap slots per
-    // species (x, y, vx, vy, alive); the turn-end compaction swaps them
-    cts_ion_config ion{};
-    int ion_buf = 0;
-    int64_t ion_cap = 0;
-    int64_t ion_len_ub = 0;  // host upper bound of the slots in use at turn start
-    double *d_ion[2][CTS_MAX_SPECIES][4] = {};
-    uint8_t *d_ion_alive[2][CTS_MAX_SPECIES] = {};
-    int64_t *d_ion_len = nullptr;     // [3][CTS_MAX_SPECIES]: the two buffers, ions at turn start
-    int64_t *d_ion_nalive = nullptr;  // [CTS_MAX_SPECIES]
-    double *d_ion_mean = nullptr;     // [2 * CTS_MAX_SPECIES]
-    double *d_ion_bview = nullptr, *d_ion_smax = nullptr, *d_ion_q = nullptr;
-    double *d_ion_cloud = nullptr;
-    double *d_ion_coeff = nullptr, *d_ion_dt = nullptr;  // [nb][CTS_MAX_SPECIES], [nb]
-    unsigned long long *d_ion_acc = nullptr;  // kIonReplicas + 1 copies of [nspec][nb][kIonAcc][3]
-    uint32_t *d_ion_wcnt = nullptr;  // [nspec][ion_wstride]: survivors per wave of a segment
-    uint32_t *d_ion_wofs = nullptr;  // [ion_wstride]: their prefix sums
-    int64_t ion_wstride = 0;
+    double *d_ion_bview = nullptr;
+    std::vector<IonIp> ips;
     // d_cur holds the moments of the present coordinates
     bool cur_valid = false;
     // field model of the beam-ion kicks: surrogate parameters (null: exact)
@@ -175,6 +185,10 @@ struct cts_handle {
     bool bl_on = false;
     BlTable bl;
     double *d_bl_tab = nullptr;
+
+    // partial sums of the eigen-emittance moments (cts_eigen_emit)
+    double *d_eig = nullptr;
+    size_t eig_cap = 0;
 
     // the settings of the per-bunch effects above, deep copies kept so that
     // cts_set_bunches can set them up again for a new bunch layout
@@ -263,17 +277,19 @@ void free_bunch_state(cts_handle *h)
     void *ptrs[] = {h->d_x, h->d_px, h->d_y, h->d_py, h->d_z, h->d_dp, h->d_flag,
                     h->d_seg_begin, h->d_seg_end, h->d_bunch_off, h->d_bunch_count,
                     h->d_seg_bunch, h->d_wg_seg, h->d_bunch_seg, h->d_seg_part, h->d_shift,
-                    h->d_cur, h->d_hist, h->d_sigz, h->d_zrange, h->d_noise, h->d_ion_e,
+                    h->d_cur, h->d_hist, h->d_sigz, h->d_zrange, h->d_noise, h->d_ion_e, h->d_ion_bview,
                     h->d_bhist, h->d_counts, h->d_kick, h->d_poten, h->d_edges_prev,
                     h->d_edges_now, h->d_table, h->d_scale, h->d_chunk_part, h->d_chunk_cnt,
-                    h->d_fb_count, h->d_fb_list};
+                    h->d_fb_count, h->d_fb_list, h->d_eig};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    h->d_eig = nullptr;
+    h->eig_cap = 0;
     h->d_x = h->d_px = h->d_y = h->d_py = h->d_z = h->d_dp = nullptr;
     h->d_flag = nullptr;
     h->d_seg_begin = h->d_seg_end = h->d_bunch_off = h->d_bunch_count = nullptr;
     h->d_seg_bunch = h->d_wg_seg = h->d_bunch_seg = nullptr;
-    h->d_seg_part = h->d_shift = h->d_sigz = h->d_zrange = h->d_noise = h->d_ion_e = nullptr;
+    h->d_seg_part = h->d_shift = h->d_sigz = h->d_zrange = h->d_noise = h->d_ion_e = h->d_ion_bview = nullptr;
     h->d_cur = h->d_hist = nullptr;
     h->d_bhist = nullptr;
     h->d_counts = nullptr;
@@ -742,31 +758,31 @@ int enqueue_moments(cts_handle *h)
     return CTS_OK;
 }
 
-void free_ion_state(cts_handle *h)
+void free_ion_ip(IonIp &I)
 {
     for (int b = 0; b < 2; b++)
         for (int p = 0; p < CTS_MAX_SPECIES; p++) {
-            for (int c = 0; c < 4; c++) {
-                if (h->d_ion[b][p][c]) (void)hipFree(h->d_ion[b][p][c]);
-                h->d_ion[b][p][c] = nullptr;
-            }
-            if (h->d_ion_alive[b][p]) (void)hipFree(h->d_ion_alive[b][p]);
-            h->d_ion_alive[b][p] = nullptr;
+            for (int c = 0; c < 4; c++)
+                if (I.d_ion[b][p][c]) (void)hipFree(I.d_ion[b][p][c]);
+            if (I.d_alive[b][p]) (void)hipFree(I.d_alive[b][p]);
         }
-    void *ptrs[] = {h->d_ion_len, h->d_ion_nalive, h->d_ion_mean, h->d_ion_bview, h->d_ion_smax,
-                    h->d_ion_q, h->d_ion_cloud, h->d_ion_coeff, h->d_ion_dt, h->d_ion_acc, h->d_ion_wcnt, h->d_ion_wofs};
+    void *ptrs[] = {I.d_len, I.d_nalive, I.d_mean, I.d_smax, I.d_q, I.d_cloud, I.d_coeff, I.d_dt, I.d_acc,
+                    I.d_wcnt, I.d_wofs};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    h->d_ion_len = h->d_ion_nalive = nullptr;
-    h->d_ion_mean = h->d_ion_bview = h->d_ion_smax = h->d_ion_q = h->d_ion_cloud = nullptr;
-    h->d_ion_coeff = h->d_ion_dt = nullptr;
-    h->d_ion_acc = nullptr;
-    h->d_ion_wcnt = h->d_ion_wofs = nullptr;
-    h->ion_wstride = 0;
-    h->nspec = 0;
-    h->ion_cap = 0;
-    h->ion_len_ub = 0;
-    h->ion_buf = 0;
+    I = IonIp{};
+}
+
+void free_ion_state(cts_handle *h)
+{
+    for (IonIp &I : h->ips) free_ion_ip(I);
+}
+
+bool ions_anywhere(const cts_handle *h)
+{
+    for (const IonIp &I : h->ips)
+        if (I.nspec > 0) return true;
+    return false;
 }
 
 
@@ -784,44 +800,47 @@ void graph_reset(cts_handle *h)
 // The cloud buffer after one ion phase: one compaction per segment.
 int ion_buf_after_turn(const cts_handle *h)
 {
+    const IonIp &I = h->ips[0];
     const int segs = (h->nb + kIonSegment - 1) / kIonSegment;
-    return (h->nspec > 0 && segs % 2 == 1) ? 1 - h->ion_buf : h->ion_buf;
+    return (I.nspec > 0 && segs % 2 == 1) ? 1 - I.buf : I.buf;
 }
 
-// MPBeam::SSBeamIonEffectOneInteractionPoint at interaction point 0
+// MPBeam::SSBeamIonEffectOneInteractionPoint at interaction point k
 // (src/MPBeam.cpp:2585-2611): bunch views from the moments of the present
-// state, the bunch passages in train order (segments of kIonSegment
-// passages, each followed by a compaction of the cloud), the passage
-// moments, then the electron kicks of every bunch.
-int ion_phase(cts_handle *h, int32_t turn)
+// state with section k's beta functions (MPBeamRMSCal(k)), the bunch
+// passages in train order (segments of kIonSegment passages, each followed
+// by a compaction of the cloud), the passage moments, then the electron
+// kicks of every bunch.
+int ion_phase(cts_handle *h, int32_t turn, int k)
 {
     int rc;
     if (!h->cur_valid && (rc = enqueue_moments(h))) return rc;
-    const cts_ion_config &I = h->ion;
-    const cts_section &S = h->secs[0];
-    const int64_t nacc = (int64_t)h->nspec * h->nb * kIonAcc * 3;
-    int64_t *len0 = h->d_ion_len + 2 * CTS_MAX_SPECIES;
+    IonIp &Ip = h->ips[k];
+    const cts_ion_config &I = Ip.cfg;
+    const cts_section &S = h->secs[k];
+    const int64_t nacc = (int64_t)Ip.nspec * h->nb * kIonAcc * 3;
+    int64_t *len0 = Ip.d_len + 2 * CTS_MAX_SPECIES;
     {
         TimedScope ts(h, "ion_prep");
-        HIPCHK(launch_ion_prep(h->d_cur, h->nb, S.beta_x, S.beta_y, h->d_ion_bview, h->d_ion_smax, h->d_fb_count,
-                               h->d_ion_len + h->ion_buf * CTS_MAX_SPECIES, len0, h->nspec, h->stream));
+        HIPCHK(launch_ion_prep(h->d_cur, h->nb, S.beta_x, S.beta_y, h->d_ion_bview, Ip.d_smax, h->d_fb_count,
+                               Ip.d_len + Ip.buf * CTS_MAX_SPECIES, len0, Ip.nspec, h->stream));
     }
     IonTurnParams P;
     std::memset(&P, 0, sizeof P);
     P.bview = h->d_ion_bview;
-    P.size_max = h->d_ion_smax;
-    P.coeffI = h->d_ion_coeff;
-    P.dt = h->d_ion_dt;
-    P.acc = h->d_ion_acc;
+    P.size_max = Ip.d_smax;
+    P.coeffI = Ip.d_coeff;
+    P.dt = Ip.d_dt;
+    P.acc = Ip.d_acc;
     P.acc_stride = nacc;
-    P.wcnt = h->d_ion_wcnt;
-    P.wcnt_stride = h->ion_wstride;
+    P.wcnt = Ip.d_wcnt;
+    P.wcnt_stride = Ip.wstride;
     P.sur_params = nullptr;
     if (h->sur && (h->sur_mode & 1)) sur_export(h->sur, &P.sur_params, &P.sur_norm);
     P.loss_boundary = I.loss_boundary;
     P.ap_x = I.ap_x;
     P.ap_y = I.ap_y;
-    P.cap = h->ion_cap;
+    P.cap = Ip.cap;
     P.seed = I.seed;
     P.nb = h->nb;
     P.n_new = I.n_new;
@@ -835,66 +854,66 @@ int ion_phase(cts_handle *h, int32_t turn)
     // the segment's new ions, in slot order) is compacted into the other
     // buffer.  ub: the host's bound of the ions at the segment's start (the
     // capacity in a graph, which replays any turn).
-    int64_t ub = h->capturing ? h->ion_cap : h->ion_len_ub;
+    int64_t ub = h->capturing ? Ip.cap : Ip.len_ub;
     for (int ja = 0; ja < h->nb; ja += kIonSegment) {
         const int jb = std::min(h->nb, ja + kIonSegment);
-        const int buf = h->ion_buf;
-        for (int p = 0; p < h->nspec; p++) {
-            P.x[p] = h->d_ion[buf][p][0];
-            P.y[p] = h->d_ion[buf][p][1];
-            P.vx[p] = h->d_ion[buf][p][2];
-            P.vy[p] = h->d_ion[buf][p][3];
-            P.alive[p] = h->d_ion_alive[buf][p];
+        const int buf = Ip.buf;
+        for (int p = 0; p < Ip.nspec; p++) {
+            P.x[p] = Ip.d_ion[buf][p][0];
+            P.y[p] = Ip.d_ion[buf][p][1];
+            P.vx[p] = Ip.d_ion[buf][p][2];
+            P.vy[p] = Ip.d_ion[buf][p][3];
+            P.alive[p] = Ip.d_alive[buf][p];
         }
-        P.len = h->d_ion_len + buf * CTS_MAX_SPECIES;
+        P.len = Ip.d_len + buf * CTS_MAX_SPECIES;
         P.j_begin = ja;
         P.j_end = jb;
         const int64_t waves = (ub + 63) / 64 + (int64_t)(jb - ja) * P.w_new;
         {
             TimedScope ts(h, "ion_turn");
-            HIPCHK(launch_ion_turn(P, h->nspec, waves, h->stream));
+            HIPCHK(launch_ion_turn(P, Ip.nspec, waves, h->stream));
         }
-        ub = std::min<int64_t>(ub + (int64_t)(jb - ja) * I.n_new, h->ion_cap);
-        for (int p = 0; p < h->nspec; p++) {
+        ub = std::min<int64_t>(ub + (int64_t)(jb - ja) * I.n_new, Ip.cap);
+        for (int p = 0; p < Ip.nspec; p++) {
             IonCompactParams c;
-            c.x = h->d_ion[buf][p][0]; c.y = h->d_ion[buf][p][1];
-            c.vx = h->d_ion[buf][p][2]; c.vy = h->d_ion[buf][p][3];
-            c.alive = h->d_ion_alive[buf][p];
-            c.ox = h->d_ion[1 - buf][p][0]; c.oy = h->d_ion[1 - buf][p][1];
-            c.ovx = h->d_ion[1 - buf][p][2]; c.ovy = h->d_ion[1 - buf][p][3];
-            c.oalive = h->d_ion_alive[1 - buf][p];
-            c.len = h->d_ion_len + buf * CTS_MAX_SPECIES + p;
-            c.new_len = h->d_ion_len + (1 - buf) * CTS_MAX_SPECIES + p;
-            c.wcnt = h->d_ion_wcnt + p * h->ion_wstride;
-            c.wofs = h->d_ion_wofs;
-            c.cap = h->ion_cap;
+            c.x = Ip.d_ion[buf][p][0]; c.y = Ip.d_ion[buf][p][1];
+            c.vx = Ip.d_ion[buf][p][2]; c.vy = Ip.d_ion[buf][p][3];
+            c.alive = Ip.d_alive[buf][p];
+            c.ox = Ip.d_ion[1 - buf][p][0]; c.oy = Ip.d_ion[1 - buf][p][1];
+            c.ovx = Ip.d_ion[1 - buf][p][2]; c.ovy = Ip.d_ion[1 - buf][p][3];
+            c.oalive = Ip.d_alive[1 - buf][p];
+            c.len = Ip.d_len + buf * CTS_MAX_SPECIES + p;
+            c.new_len = Ip.d_len + (1 - buf) * CTS_MAX_SPECIES + p;
+            c.wcnt = Ip.d_wcnt + p * Ip.wstride;
+            c.wofs = Ip.d_wofs;
+            c.cap = Ip.cap;
             c.n_new = I.n_new;
             c.w_new = P.w_new;
             c.nseg = jb - ja;
             TimedScope ts(h, "ion_compact");
             HIPCHK(launch_ion_compact(c, waves, h->stream));
         }
-        h->ion_buf = 1 - buf;
+        Ip.buf = 1 - buf;
     }
-    h->ion_len_ub = ub;
+    Ip.len_ub = ub;
     {
         TimedScope ts(h, "ion_reduce");
-        HIPCHK(launch_ion_reduce(h->d_ion_acc, nacc, h->d_ion_acc + kIonReplicas * nacc, h->stream));
+        HIPCHK(launch_ion_reduce(Ip.d_acc, nacc, Ip.d_acc + kIonReplicas * nacc, h->stream));
     }
     IonFinishParams F;
-    F.acc = h->d_ion_acc + kIonReplicas * nacc;
+    F.acc = Ip.d_acc + kIonReplicas * nacc;
     F.bview = h->d_ion_bview;
-    F.ion_q = h->d_ion_q;
+    F.ion_q = Ip.d_q;
     F.len0 = len0;
-    F.n_alive = h->d_ion_nalive;
-    F.mean_state = h->d_ion_mean;
+    F.n_alive = Ip.d_nalive;
+    F.mean_state = Ip.d_mean;
     F.ion_e = h->d_ion_e;
-    F.cloud = h->d_ion_cloud;
+    F.cloud = Ip.d_cloud;
     F.status = h->d_status;
     F.gamma = h->ring.rgamma;
     F.max_ions = I.max_ions;
     F.nb = h->nb;
-    F.nspec = h->nspec;
+    F.nspec = Ip.nspec;
     F.n_new = I.n_new;
     {
         TimedScope ts(h, "ion_finish");
@@ -906,7 +925,7 @@ int ion_phase(cts_handle *h, int32_t turn)
     ik.seg_begin = h->d_seg_begin; ik.seg_end = h->d_seg_end;
     ik.seg_bunch = h->d_seg_bunch; ik.wg_seg = h->d_wg_seg;
     ik.ion_e = h->d_ion_e;
-    ik.nspec = h->nspec;
+    ik.nspec = Ip.nspec;
     ik.sur_params = nullptr;
     if (h->sur && (h->sur_mode & 2)) sur_export(h->sur, &ik.sur_params, &ik.sur_norm);
     ik.fb_count = h->d_fb_count;
@@ -949,6 +968,7 @@ int cts_create(const cts_ring *ring, const cts_section *secs, int32_t nsec, cts_
     cts_handle *h = new cts_handle();
     h->ring = *ring;
     h->secs.assign(secs, secs + nsec);
+    h->ips.resize(nsec);
     hipError_t e = hipGetDevice(&h->dev);
     if (e == hipSuccess) e = hipStreamCreate(&h->stream);
     if (e != hipSuccess) {
@@ -1019,8 +1039,8 @@ int cts_set_bunches(cts_handle *h, int32_t nb, const int64_t *counts, const doub
     const cts_handle::KeepLrw keep_lrw = h->keep_lrw;
     const cts_handle::KeepBbi keep_bbi = h->keep_bbi;
     const cts_handle::KeepBl keep_bl = h->keep_bl;
-    const cts_ion_config ions = h->ion;
-    const bool ions_on = h->nspec > 0;
+    std::vector<cts_ion_config> ions;
+    for (const IonIp &I : h->ips) ions.push_back(I.nspec > 0 ? I.cfg : cts_ion_config{});
     free_ion_state(h);
     h->nb = nb;
     h->counts.assign(counts, counts + nb);
@@ -1042,7 +1062,8 @@ int cts_set_bunches(cts_handle *h, int32_t nb, const int64_t *counts, const doub
         (rc = dalloc(&h->d_flag, n)) || (rc = dalloc(&h->d_bunch_off, nb + 1)) ||
         (rc = dalloc(&h->d_bunch_count, nb)) || (rc = dalloc(&h->d_shift, (size_t)nb * 6)) ||
         (rc = dalloc(&h->d_cur, nb)) || (rc = dalloc(&h->d_sigz, nb)) || (rc = dalloc(&h->d_zrange, (size_t)nb * 2)) ||
-        (rc = dalloc(&h->d_ion_e, (size_t)nb * CTS_MAX_SPECIES * 5)) || (rc = dalloc(&h->d_chunk_cnt, nb)))
+        (rc = dalloc(&h->d_ion_e, (size_t)nb * CTS_MAX_SPECIES * 5)) || (rc = dalloc(&h->d_ion_bview, (size_t)nb * 4)) ||
+        (rc = dalloc(&h->d_chunk_cnt, nb)))
         return rc;
     if ((rc = layout_segments(h, h->wg_target))) return rc;
     HIPCHK(hipMemcpy(h->d_bunch_off, h->offsets.data(), (nb + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
@@ -1084,7 +1105,8 @@ int cts_set_bunches(cts_handle *h, int32_t nb, const int64_t *counts, const doub
         if ((rc = cts_set_bbimp(h, &c))) return rc;
     }
     if (keep_bl.on && (rc = cts_set_beam_loading(h, &keep_bl.cfg))) return rc;
-    if (ions_on) return cts_set_ions(h, &ions);
+    for (int k = 0; k < (int)ions.size(); k++)
+        if (ions[k].nspec > 0 && (rc = cts_set_ions_ip(h, k, &ions[k]))) return rc;
     return CTS_OK;
 }
 
@@ -1308,7 +1330,7 @@ static int turn_body(cts_handle *h, int32_t turn)
     if (h->synrad_on) ops |= OP_SYNRAD;
     // beam-ion phase at interaction point 0, before the map
     // (src/MPBeam.cpp:1061-1068)
-    if (h->nspec > 0 && (rc = ion_phase(h, turn))) return rc;
+    if (h->ips[0].nspec > 0 && (rc = ion_phase(h, turn, 0))) return rc;
     const bool wake = h->nbins > 0;
     if (h->kick_pending) ops |= OP_KICK | OP_MOMB;
     if (wake) ops |= (h->wake_mode == 1) ? OP_BIN : OP_ZSTAT;
@@ -1375,11 +1397,11 @@ int cts_turn(cts_handle *h, int32_t turn)
 // only takes effect when it is launched.
 static int capture_pair(cts_handle *h, hipGraphExec_t *out)
 {
-    const int parity = h->hist_parity, ion_buf = h->ion_buf, last_turn = h->last_turn;
+    const int parity = h->hist_parity, ion_buf = h->ips[0].buf, last_turn = h->last_turn;
     const bool kick_pending = h->kick_pending, dp_pending = h->dp_pending, cur_valid = h->cur_valid;
     const bool lr_pending = h->lr_pending;
     const bool timing = h->timing;
-    const int64_t ion_len_ub = h->ion_len_ub;
+    const int64_t ion_len_ub = h->ips[0].len_ub;
     h->timing = false;
     h->capturing = true;
     hipGraph_t g = nullptr;
@@ -1404,13 +1426,13 @@ static int capture_pair(cts_handle *h, hipGraphExec_t *out)
     h->cap_off = 0;
     h->timing = timing;
     h->hist_parity = parity;
-    h->ion_buf = ion_buf;
+    h->ips[0].buf = ion_buf;
     h->last_turn = last_turn;
     h->kick_pending = kick_pending;
     h->dp_pending = dp_pending;
     h->cur_valid = cur_valid;
     h->lr_pending = lr_pending;
-    h->ion_len_ub = ion_len_ub;
+    h->ips[0].len_ub = ion_len_ub;
     if (rc == CTS_OK) {
         e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
         if (e != hipSuccess) rc = fail(CTS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
@@ -1456,7 +1478,7 @@ int cts_run_turns(cts_handle *h, int32_t first, int32_t nturns, int32_t use_grap
         sur_state(h, &sp, &sn);
         cts_handle::GraphSlot *slot = nullptr;
         for (auto &g : h->graphs)
-            if (g.exec && g.parity == h->hist_parity && g.ion_buf == h->ion_buf) slot = &g;
+            if (g.exec && g.parity == h->hist_parity && g.ion_buf == h->ips[0].buf) slot = &g;
         if (slot && (slot->sur_params != sp || std::memcmp(&slot->sur_norm, &sn, sizeof sn) != 0)) {
             graph_reset(h);
             slot = nullptr;
@@ -1484,7 +1506,7 @@ int cts_run_turns(cts_handle *h, int32_t first, int32_t nturns, int32_t use_grap
             }
             if ((rc = capture_pair(h, &slot->exec))) return rc;
             slot->parity = h->hist_parity;
-            slot->ion_buf = h->ion_buf;
+            slot->ion_buf = h->ips[0].buf;
             slot->sur_params = sp;
             slot->sur_norm = sn;
         }
@@ -1496,7 +1518,7 @@ int cts_run_turns(cts_handle *h, int32_t first, int32_t nturns, int32_t use_grap
         h->dp_pending = h->nbins > 0 || h->bbi_on;
         h->lr_pending = h->lrw_T > 0;
         h->cur_valid = true;
-        if (h->nspec > 0) h->ion_len_ub = h->ion_cap;
+        if (h->ips[0].nspec > 0) h->ips[0].len_ub = h->ips[0].cap;
         if ((rc = enqueue_status_copy(h))) return rc;
         t += 2;
     }
@@ -1869,45 +1891,51 @@ int cts_get_lrwake_kick(cts_handle *h, double *kick)
     return CTS_OK;
 }
 
-int cts_set_ions(cts_handle *h, const cts_ion_config *cfg)
+int cts_set_ions(cts_handle *h, const cts_ion_config *cfg) { return cts_set_ions_ip(h, 0, cfg); }
+
+int cts_set_ions_ip(cts_handle *h, int32_t k, const cts_ion_config *cfg)
 {
     int rc = check_ready(h);
     if (rc) return rc;
     if (!cfg || cfg->nspec < 0 || cfg->nspec > CTS_MAX_SPECIES) return fail(CTS_ERR_ARG, "cts_set_ions: nspec 0..4");
+    if (k < 0 || k >= (int)h->ips.size()) return fail(CTS_ERR_ARG, "cts_set_ions_ip: no such section");
     HIPCHK(hipStreamSynchronize(h->stream));
     graph_reset(h);
-    free_ion_state(h);
+    IonIp &Ip = h->ips[k];
+    free_ion_ip(Ip);
     if (cfg->nspec == 0) return CTS_OK;
     if (cfg->n_new < 1 || cfg->max_ions < 1 || !(cfg->temperature > 0) || !(cfg->length >= 0) ||
         !(cfg->ap_x > 0) || !(cfg->ap_y > 0) || !(cfg->loss_boundary > 0))
         return fail(CTS_ERR_ARG, "cts_set_ions: n_new, max_ions, temperature, apertures and loss boundary must be > 0");
     for (int p = 0; p < cfg->nspec; p++)
         if (!(cfg->mass_number[p] > 0)) return fail(CTS_ERR_ARG, "cts_set_ions: mass number must be > 0");
-    if (h->secs.empty()) return fail(CTS_ERR_STATE, "cts_set_ions: no sections");
-    // one passage appends n_new per species; a turn appends nb * n_new
-    // before the turn-end compaction
+    // one passage appends n_new per species; a phase appends nb * n_new
+    // before its last compaction
     const int64_t cap = cfg->max_ions + (int64_t)h->nb * cfg->n_new;
     if (cap > (int64_t)1 << 31) return fail(CTS_ERR_ARG, "cts_set_ions: cloud capacity above 2^31 slots");
     // waves of a segment: the compacted cloud, then kIonSegment passages' new ions
     const int64_t wstride = (cap + 63) / 64 + (int64_t)kIonSegment * ((cfg->n_new + 63) / 64) + 1;
-    h->ion = *cfg;
-    h->ion_cap = cap;
-    h->ion_wstride = wstride;
+    Ip.cfg = *cfg;
+    Ip.cap = cap;
+    Ip.wstride = wstride;
+    auto fail_free = [&](int code) {
+        free_ion_ip(Ip);
+        return code;
+    };
     for (int b = 0; b < 2; b++)
         for (int p = 0; p < cfg->nspec; p++) {
             for (int c = 0; c < 4; c++)
-                if ((rc = dalloc(&h->d_ion[b][p][c], (size_t)cap))) return rc;
-            if ((rc = dalloc(&h->d_ion_alive[b][p], (size_t)cap))) return rc;
+                if ((rc = dalloc(&Ip.d_ion[b][p][c], (size_t)cap))) return fail_free(rc);
+            if ((rc = dalloc(&Ip.d_alive[b][p], (size_t)cap))) return fail_free(rc);
         }
-    if ((rc = dalloc(&h->d_ion_len, 3 * CTS_MAX_SPECIES)) || (rc = dalloc(&h->d_ion_nalive, CTS_MAX_SPECIES)) ||
-        (rc = dalloc(&h->d_ion_mean, 2 * CTS_MAX_SPECIES)) || (rc = dalloc(&h->d_ion_bview, (size_t)h->nb * 4)) ||
-        (rc = dalloc(&h->d_ion_smax, 2)) || (rc = dalloc(&h->d_ion_q, (size_t)h->nb * cfg->nspec)) ||
-        (rc = dalloc(&h->d_ion_cloud, (size_t)h->nb * CTS_MAX_SPECIES * 6)) ||
-        (rc = dalloc(&h->d_ion_coeff, (size_t)h->nb * CTS_MAX_SPECIES)) || (rc = dalloc(&h->d_ion_dt, (size_t)h->nb)) ||
-        (rc = dalloc(&h->d_ion_acc, (size_t)(kIonReplicas + 1) * cfg->nspec * h->nb * kIonAcc * 3)) ||
-        (rc = dalloc(&h->d_ion_wcnt, (size_t)(cfg->nspec * wstride))) ||
-        (rc = dalloc(&h->d_ion_wofs, (size_t)wstride)))
-        return rc;
+    if ((rc = dalloc(&Ip.d_len, 3 * CTS_MAX_SPECIES)) || (rc = dalloc(&Ip.d_nalive, CTS_MAX_SPECIES)) ||
+        (rc = dalloc(&Ip.d_mean, 2 * CTS_MAX_SPECIES)) || (rc = dalloc(&Ip.d_smax, 2)) ||
+        (rc = dalloc(&Ip.d_q, (size_t)h->nb * cfg->nspec)) ||
+        (rc = dalloc(&Ip.d_cloud, (size_t)h->nb * CTS_MAX_SPECIES * 6)) ||
+        (rc = dalloc(&Ip.d_coeff, (size_t)h->nb * CTS_MAX_SPECIES)) || (rc = dalloc(&Ip.d_dt, (size_t)h->nb)) ||
+        (rc = dalloc(&Ip.d_acc, (size_t)(kIonReplicas + 1) * cfg->nspec * h->nb * kIonAcc * 3)) ||
+        (rc = dalloc(&Ip.d_wcnt, (size_t)(cfg->nspec * wstride))) || (rc = dalloc(&Ip.d_wofs, (size_t)wstride)))
+        return fail_free(rc);
     // per bunch: ion kick coefficient 2 N_e r_e (m_e / m_u) / A c
     // (src/MPBunch.cpp:1117) and drift time C / h * gap / c
     // (src/LatticeInterActionPoint.cpp:1169-1170)
@@ -1918,8 +1946,8 @@ int cts_set_ions(cts_handle *h, const cts_ion_config *cfg)
                 2.0 * h->n_e[b] * kRe * kElectronMassEV / kIonMassEV / cfg->mass_number[p] * kClight;
         dtv[b] = h->ring.circ / h->ring.harmonic * h->gap[b] / kClight;
     }
-    HIPCHK(hipMemcpy(h->d_ion_coeff, coeff.data(), coeff.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->d_ion_dt, dtv.data(), dtv.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(Ip.d_coeff, coeff.data(), coeff.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(Ip.d_dt, dtv.data(), dtv.size() * sizeof(double), hipMemcpyHostToDevice));
     // macro-ion charge per bunch passage: line density sigma P / (k_B T) N_e
     // over the interaction length, shared by n_new macro-ions
     // (src/LatticeInterActionPoint.cpp:974-981, 998)
@@ -1929,9 +1957,9 @@ int cts_set_ions(cts_handle *h, const cts_ion_config *cfg)
             const double line = cfg->cross_section[p] * cfg->pressure[p] / cfg->temperature / kBoltzmann * h->n_e[b];
             q[(size_t)b * cfg->nspec + p] = line * cfg->length / cfg->n_new;
         }
-    HIPCHK(hipMemcpy(h->d_ion_q, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(h->d_ion_e, 0, (size_t)h->nb * CTS_MAX_SPECIES * 5 * sizeof(double)));
-    h->nspec = cfg->nspec;
+    HIPCHK(hipMemcpy(Ip.d_q, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
+    Ip.q = std::move(q);
+    Ip.nspec = cfg->nspec;
     return CTS_OK;
 }
 
@@ -1950,44 +1978,75 @@ int cts_set_field_model(cts_handle *h, const cts_surrogate *s, int32_t mode)
     return CTS_OK;
 }
 
-int cts_ion_phase(cts_handle *h, int32_t turn)
+int cts_ion_phase(cts_handle *h, int32_t turn) { return cts_ion_phase_ip(h, turn, 0); }
+
+int cts_ion_phase_ip(cts_handle *h, int32_t turn, int32_t k)
 {
     int rc = check_ready(h);
     if (rc) return rc;
-    if (h->nspec == 0) return fail(CTS_ERR_STATE, "cts_ion_phase: ions are off (cts_set_ions)");
+    if (k < 0 || k >= (int)h->ips.size()) return fail(CTS_ERR_ARG, "cts_ion_phase_ip: no such section");
+    if (h->ips[k].nspec == 0) return fail(CTS_ERR_STATE, "cts_ion_phase: no ions at this interaction point");
     if ((rc = flush_kick(h))) return rc;
-    if ((rc = ion_phase(h, turn))) return rc;
+    if ((rc = ion_phase(h, turn, k))) return rc;
     return enqueue_status_copy(h);
 }
 
 int cts_get_ion_cloud(cts_handle *h, int32_t species, int64_t cap, int64_t *n, double *x, double *y, double *vx,
                       double *vy)
 {
+    return cts_get_ion_cloud_ip(h, 0, species, cap, n, x, y, vx, vy);
+}
+
+int cts_get_ion_cloud_ip(cts_handle *h, int32_t k, int32_t species, int64_t cap, int64_t *n, double *x, double *y,
+                         double *vx, double *vy)
+{
     int rc = check_ready(h);
     if (rc) return rc;
-    if (species < 0 || species >= h->nspec || !n) return fail(CTS_ERR_ARG, "cts_get_ion_cloud: bad species");
+    if (k < 0 || k >= (int)h->ips.size()) return fail(CTS_ERR_ARG, "cts_get_ion_cloud: no such section");
+    const IonIp &Ip = h->ips[k];
+    if (species < 0 || species >= Ip.nspec || !n) return fail(CTS_ERR_ARG, "cts_get_ion_cloud: bad species");
     HIPCHK(hipStreamSynchronize(h->stream));
     int64_t len = 0;
-    HIPCHK(hipMemcpy(&len, h->d_ion_len + h->ion_buf * CTS_MAX_SPECIES + species, sizeof len,
-                     hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&len, Ip.d_len + Ip.buf * CTS_MAX_SPECIES + species, sizeof len, hipMemcpyDeviceToHost));
     *n = len;
     const int64_t m = std::min(len, cap);
     double *dst[4] = {x, y, vx, vy};
     for (int c = 0; c < 4; c++)
         if (dst[c] && m > 0)
-            HIPCHK(hipMemcpy(dst[c], h->d_ion[h->ion_buf][species][c], (size_t)m * sizeof(double),
+            HIPCHK(hipMemcpy(dst[c], Ip.d_ion[Ip.buf][species][c], (size_t)m * sizeof(double),
                              hipMemcpyDeviceToHost));
     return CTS_OK;
 }
 
-int cts_get_ion_passages(cts_handle *h, double *out)
+int cts_get_ion_passages(cts_handle *h, double *out) { return cts_get_ion_passages_ip(h, 0, out); }
+
+int cts_get_ion_passages_ip(cts_handle *h, int32_t k, double *out)
 {
     int rc = check_ready(h);
     if (rc) return rc;
-    if (h->nspec == 0 || !out) return fail(CTS_ERR_STATE, "cts_get_ion_passages: ions are off");
+    if (k < 0 || k >= (int)h->ips.size() || h->ips[k].nspec == 0 || !out)
+        return fail(CTS_ERR_STATE, "cts_get_ion_passages: no ions at this interaction point");
     HIPCHK(hipStreamSynchronize(h->stream));
-    HIPCHK(hipMemcpy(out, h->d_ion_cloud, (size_t)h->nb * CTS_MAX_SPECIES * 6 * sizeof(double),
+    HIPCHK(hipMemcpy(out, h->ips[k].d_cloud, (size_t)h->nb * CTS_MAX_SPECIES * 6 * sizeof(double),
                      hipMemcpyDeviceToHost));
+    return CTS_OK;
+}
+
+int cts_get_ion_charge(cts_handle *h, double *total)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!total) return fail(CTS_ERR_ARG, "cts_get_ion_charge: null output");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    double sum = 0.0;
+    for (const IonIp &Ip : h->ips) {
+        if (Ip.nspec == 0) continue;
+        int64_t n[CTS_MAX_SPECIES] = {};
+        HIPCHK(hipMemcpy(n, Ip.d_nalive, Ip.nspec * sizeof(int64_t), hipMemcpyDeviceToHost));
+        // survivors of the last passage times the last bunch's macro-ion charge
+        for (int p = 0; p < Ip.nspec; p++) sum += (double)n[p] * Ip.q[(size_t)(h->nb - 1) * Ip.nspec + p];
+    }
+    *total = sum;
     return CTS_OK;
 }
 
@@ -2107,6 +2166,134 @@ int cts_derive_moments(cts_handle *h, const cts_moments *m, int32_t k, cts_deriv
     o->rms_eff_y = std::sqrt(o->emit_eff[1] * h->secs[k].beta_y);
     o->sigma_z = std::sqrt(c[6]);
     o->sigma_dp = std::sqrt(c[8]);
+    return CTS_OK;
+}
+
+// Determinant by LU decomposition with partial pivoting (the largest
+// magnitude in the column), in the operation order of the GSL LU
+// factorisation behind GetEigenEmit's get_det (src/MPBunch.cpp:606):
+// multipliers by the pivot's reciprocal, the determinant as the sign times
+// the product of the diagonal.
+static double det_lu(int n, double *a)
+{
+    int sign = 1;
+    for (int k = 0; k < n; k++) {
+        int piv = k;
+        for (int i = k + 1; i < n; i++)
+            if (std::fabs(a[i * n + k]) > std::fabs(a[piv * n + k])) piv = i;
+        if (piv != k) {
+            for (int j = 0; j < n; j++) std::swap(a[k * n + j], a[piv * n + j]);
+            sign = -sign;
+        }
+        const double akk = a[k * n + k];
+        if (akk == 0.0) return 0.0;
+        const double inv = 1.0 / akk;
+        for (int i = k + 1; i < n; i++) {
+            const double l = a[i * n + k] * inv;
+            for (int j = k + 1; j < n; j++) a[i * n + j] -= l * a[k * n + j];
+        }
+    }
+    double det = sign;
+    for (int k = 0; k < n; k++) det *= a[k * n + k];
+    return det;
+}
+
+// std::accumulate(first, last, 0): the int running sum truncates every
+// partial sum (src/MPBunch.cpp:522-525).
+static double accumulate_int(const std::vector<double> &v)
+{
+    int s = 0;
+    for (double e : v) s = (int)(s + e);
+    return (double)s;
+}
+
+int cts_eigen_emit(cts_handle *h, double *out)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (!out) return fail(CTS_ERR_ARG, "cts_eigen_emit: null output");
+    if ((rc = flush_kick(h))) return rc;
+    const int nb = h->nb;
+    int64_t nmax = 0;
+    for (int64_t c : h->counts) nmax = std::max(nmax, c);
+    const int nchunk = (int)std::max<int64_t>(1, std::min<int64_t>(256, (nmax + 8191) / 8192));
+    const size_t nsum = (size_t)nb * nchunk * 15;
+    if (h->eig_cap < nsum) {
+        if (h->d_eig) (void)hipFree(h->d_eig);
+        h->d_eig = nullptr;
+        h->eig_cap = 0;
+        if ((rc = dalloc(&h->d_eig, nsum))) return rc;
+        h->eig_cap = nsum;
+    }
+    const cts_section &S0 = h->secs[0];
+    const double eta[4] = {S0.eta_x, S0.etap_x, S0.eta_y, S0.etap_y};
+    HIPCHK(launch_eigen_sums(h->d_x, h->d_px, h->d_y, h->d_py, h->d_dp, h->d_bunch_off, nb, nchunk, eta, h->d_eig,
+                             h->stream));
+    std::vector<double> part(nsum);
+    HIPCHK(hipMemcpyAsync(part.data(), h->d_eig, nsum * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int b = 0; b < nb; b++) {
+        const double n = (double)h->counts[b];
+        double s[10] = {0}, mx = 0.0;
+        for (int c = 0; c < nchunk; c++) {
+            const double *q = part.data() + ((size_t)b * nchunk + c) * 15;
+            for (int k = 0; k < 10; k++) s[k] += q[k];
+            mx = std::max(mx, q[14]);
+        }
+        if (mx >= 1.0) {
+            // a coordinate of 1 or more: the integer accumulate can leave a
+            // non-zero "mean"; the bunch's moments are taken on the host as
+            // the reference takes them
+            const size_t m = (size_t)h->counts[b];
+            std::vector<double> X[5];
+            double *src[5] = {h->d_x, h->d_px, h->d_y, h->d_py, h->d_dp};
+            for (int k = 0; k < 5; k++) {
+                X[k].resize(m);
+                HIPCHK(hipMemcpy(X[k].data(), src[k] + h->offsets[b], m * sizeof(double), hipMemcpyDeviceToHost));
+            }
+            for (size_t i = 0; i < m; i++) {
+                X[0][i] -= eta[0] * X[4][i];
+                X[1][i] -= eta[1] * X[4][i];
+                X[2][i] -= eta[2] * X[4][i];
+                X[3][i] -= eta[3] * X[4][i];
+            }
+            // the reference's order: x, y, xp, yp
+            const double ax = accumulate_int(X[0]), ay = accumulate_int(X[2]);
+            const double axp = accumulate_int(X[1]), ayp = accumulate_int(X[3]);
+            for (double &v : s) v = 0.0;
+            for (size_t i = 0; i < m; i++) {
+                const double u0 = X[0][i] - ax, u1 = X[1][i] - axp, u2 = X[2][i] - ay, u3 = X[3][i] - ayp;
+                s[0] += u0 * u0; s[1] += u0 * u1; s[2] += u0 * u2; s[3] += u0 * u3; s[4] += u1 * u1;
+                s[5] += u1 * u2; s[6] += u1 * u3; s[7] += u2 * u2; s[8] += u2 * u3; s[9] += u3 * u3;
+            }
+        }
+        for (double &v : s) v /= n;
+        // Sigma (x, px, y, py), J, tr((Sigma J)^2), det Sigma
+        const double Sg[16] = {s[0], s[1], s[2], s[3], s[1], s[4], s[5], s[6],
+                               s[2], s[5], s[7], s[8], s[3], s[6], s[8], s[9]};
+        double CJ[16];
+        for (int i = 0; i < 4; i++) {
+            // (Sigma J)[i][j]: J has +1 at (0,1), (2,3) and -1 at (1,0), (3,2)
+            CJ[i * 4 + 0] = -Sg[i * 4 + 1];
+            CJ[i * 4 + 1] = Sg[i * 4 + 0];
+            CJ[i * 4 + 2] = -Sg[i * 4 + 3];
+            CJ[i * 4 + 3] = Sg[i * 4 + 2];
+        }
+        double tr = 0.0;
+        for (int i = 0; i < 4; i++)
+            for (int k = 0; k < 4; k++) tr += CJ[i * 4 + k] * CJ[k * 4 + i];
+        double A[16];
+        std::memcpy(A, Sg, sizeof A);
+        const double det = det_lu(4, A);
+        double *o = out + 4 * b;
+        o[0] = std::sqrt(-tr + std::sqrt(tr * tr - 16 * det)) / 2;
+        o[1] = std::sqrt(-tr - std::sqrt(tr * tr - 16 * det)) / 2;
+        o[2] = std::sqrt(std::sqrt(det));
+        // x-y tilt of the beam ellipse (src/MPBunch.cpp:642-656)
+        const double a = s[0], bb = s[2], c = s[7];
+        const double lam = (a + c) / 2 + std::sqrt((a - c) * (a - c) / 4 + bb * bb);
+        o[3] = bb == 0 ? (a >= c ? 0.0 : kPi / 2) : std::atan2(lam - a, bb);
+    }
     return CTS_OK;
 }
 

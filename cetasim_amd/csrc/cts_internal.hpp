@@ -342,6 +342,9 @@ constexpr int kIonSegment = CTS_ION_SEGMENT;  // bunch passages per ion segment
 hipError_t launch_qe_debug(const float2 *tab, uint64_t seed, uint32_t turn, uint64_t base, int64_t n, double *out,
                            hipStream_t st);
 hipError_t upload_faddeeva_tables(const FadTables &t);
+hipError_t launch_eigen_sums(const double *x, const double *px, const double *y, const double *py, const double *dp,
+                             const int64_t *bunch_off, int nb, int nchunk, const double eta[4], double *out,
+                             hipStream_t st);
 hipError_t launch_w_debug(const double *re, const double *im, int64_t n, double *wr, double *wi, hipStream_t st);
 hipError_t launch_kick_field_debug(const double *in, int64_t n, double *out, hipStream_t st);
 hipError_t upload_faddeeva_tables_ions(const FadTables &t);

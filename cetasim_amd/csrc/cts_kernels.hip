@@ -919,6 +919,57 @@ hipError_t launch_w_debug(const double *re, const double *im, int64_t n, double 
     return hipGetLastError();
 }
 
+// ---------------------------------------------------- eigen-emittances
+// The second moments of MPBunch::GetEigenEmit (src/MPBunch.cpp:500-547):
+// every particle of a bunch, lost ones included, with x, px, y, py less
+// the dispersion of section 0 times delta.  Block (c, b) sums chunk c of
+// bunch b: out[(b * nchunk + c) * 15] = the ten products x x, x px, x y,
+// x py, px px, px y, px py, y y, y py, py py, the four coordinate sums and
+// the largest |coordinate| (the reference's integer-valued std::accumulate
+// leaves its "means" at zero only while every coordinate is below 1 in
+// magnitude).  Threads add in a fixed order and the block folds its lanes
+// in a fixed tree, so the partial sums do not depend on timing.
+__global__ void __launch_bounds__(kBlock)
+k_eigen_sums(const double *x, const double *px, const double *y, const double *py, const double *dp,
+             const int64_t *bunch_off, int nchunk, double ex, double exp_, double ey, double eyp, double *out)
+{
+    __shared__ double red[15][kBlock];
+    const int c = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+    const int64_t b0 = bunch_off[b], n = bunch_off[b + 1] - b0;
+    const int64_t per = (n + nchunk - 1) / nchunk;
+    const int64_t i0 = b0 + (int64_t)c * per, i1 = b0 + std::min(n, (int64_t)(c + 1) * per);
+    double a[15];
+#pragma unroll
+    for (int k = 0; k < 15; k++) a[k] = 0.0;
+    for (int64_t i = i0 + tid; i < i1; i += kBlock) {
+        const double d = dp[i];
+        const double u0 = x[i] - ex * d, u1 = px[i] - exp_ * d, u2 = y[i] - ey * d, u3 = py[i] - eyp * d;
+        a[0] += u0 * u0; a[1] += u0 * u1; a[2] += u0 * u2; a[3] += u0 * u3; a[4] += u1 * u1;
+        a[5] += u1 * u2; a[6] += u1 * u3; a[7] += u2 * u2; a[8] += u2 * u3; a[9] += u3 * u3;
+        a[10] += u0; a[11] += u1; a[12] += u2; a[13] += u3;
+        a[14] = fmax(a[14], fmax(fmax(fabs(u0), fabs(u1)), fmax(fabs(u2), fabs(u3))));
+    }
+#pragma unroll
+    for (int k = 0; k < 15; k++) red[k][tid] = a[k];
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if (tid < o) {
+            for (int k = 0; k < 14; k++) red[k][tid] += red[k][tid + o];
+            red[14][tid] = fmax(red[14][tid], red[14][tid + o]);
+        }
+        __syncthreads();
+    }
+    if (tid < 15) out[((int64_t)b * nchunk + c) * 15 + tid] = red[tid][0];
+}
+hipError_t launch_eigen_sums(const double *x, const double *px, const double *y, const double *py, const double *dp,
+                             const int64_t *bunch_off, int nb, int nchunk, const double eta[4], double *out,
+                             hipStream_t st)
+{
+    hipLaunchKernelGGL(k_eigen_sums, dim3(nchunk, nb), dim3(kBlock), 0, st, x, px, y, py, dp, bunch_off, nchunk,
+                       eta[0], eta[1], eta[2], eta[3], out);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------ long-range wake
 // MPBeam::LRWakeBeamIntaction (src/MPBeam.cpp:3232-3351) for one turn,
 // one block per receiving bunch j: over the last nturns turns n (n = 0 the

@@ -158,7 +158,17 @@ int dump(const std::string &dir, ReadInputSettings &in, LatticeInterActionPoint 
         arr("fr", c.fr.data(), c.fr.size());
         std::fprintf(f, "\"synrad\": %d, \"seed\": %llu, \"ions\": %d,\n", c.synrad, (unsigned long long)c.seed,
                      c.ions);
-        const cts_ion_config &ic = c.ion;
+        // interaction point 0 under the plain keys, every point in ion_ips
+        const cts_ion_config ic = c.ion.empty() ? cts_ion_config{} : c.ion[0];
+        std::fprintf(f, "\"ion_ips\": [");
+        for (size_t k = 0; k < c.ion.size(); k++) {
+            const cts_ion_config &e = c.ion[k];
+            std::fprintf(f, "%s{\"pressure\": [%.17g, %.17g, %.17g, %.17g], \"temperature\": %.17g, "
+                            "\"length\": %.17g, \"ap\": [%.17g, %.17g], \"seed\": %llu}",
+                         k ? ", " : "", e.pressure[0], e.pressure[1], e.pressure[2], e.pressure[3], e.temperature,
+                         e.length, e.ap_x, e.ap_y, (unsigned long long)e.seed);
+        }
+        std::fprintf(f, "],\n");
         std::fprintf(f, "\"ion_nspec\": %d, \"ion_n_new\": %d, \"ion_max\": %lld,\n", ic.nspec, ic.n_new,
                      (long long)ic.max_ions);
         arr("ion_mass", ic.mass_number, CTS_MAX_SPECIES);
@@ -194,6 +204,8 @@ int dump(const std::string &dir, ReadInputSettings &in, LatticeInterActionPoint 
             std::fprintf(f, "%s[%.17g, %.17g, %.17g, %.17g, %.17g, %.17g]", i ? ", " : "", b.rs[i], b.q0[i],
                          b.coupling[i], b.detune_hz[i], b.vol[i], b.phase[i]);
         std::fprintf(f, "],\n");
+        std::fprintf(f, "\"tbt_print\": %d, \"ring_current\": %.17g,\n", R->TBTBunchPrintNum,
+                     in.ringParBasic->ringCurrent);
         std::fprintf(f, "\"workQz\": %.17g, \"betaZ\": %.17g, \"tbt\": \"%s\", \"cbmgr\": \"%s\"\n}\n",
                      in.ringParBasic->workQz, in.ringBunchPara->rmsBunchLength / in.ringBunchPara->rmsEnergySpread,
                      R->TBTBunchAverData.c_str(), R->runCBMGR.c_str());
@@ -212,10 +224,14 @@ int dump(const std::string &dir, ReadInputSettings &in, LatticeInterActionPoint 
 
 int main(int argc, char *argv[])
 {
-    bool check_only = false;
+    bool check_only = false, no_result = false;
     std::string dump_dir;
     int a = 1;
-    if (argc > 1 && std::strcmp(argv[1], "--check") == 0) {
+    if (argc > 1 && std::strcmp(argv[1], "--no-result") == 0) {
+        // skip result.sdds: the turns then run as HIP-graph replays
+        no_result = true;
+        a = 2;
+    } else if (argc > 1 && std::strcmp(argv[1], "--check") == 0) {
         check_only = true;
         a = 2;
     } else if (argc > 2 && std::strcmp(argv[1], "--dump") == 0) {
@@ -241,6 +257,7 @@ int main(int argc, char *argv[])
         print_check(inputParameter, train, latticeInterActionPoint, mpbeam);
         return 0;
     }
+    mpbeam.writeResult = !no_result;
     if (mpbeam.Initial(train, latticeInterActionPoint, inputParameter)) return 1;
     if (mpbeam.Run(train, latticeInterActionPoint, inputParameter)) return 1;
     std::cout << "End of Tracking " << inputParameter.ringRun->nTurns << " Turns" << std::endl;

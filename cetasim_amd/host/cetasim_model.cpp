@@ -193,6 +193,7 @@ int ReadInputSettings::ParamReadFile(const std::string &path)
             else if (k == "rungrowthratefittingstart") run_.growthRateFittingStart = inum(v, 0);
             else if (k == "rungrowthratefittingend") run_.growthRateFittingEnd = inum(v, 0);
             else if (k == "runtbtbunchaverdata") run_.TBTBunchAverData = v.at(0);
+            else if (k == "runtbtbunchprintnum") run_.TBTBunchPrintNum = inum(v, 0);
             else if (k == "runcbmgr") run_.runCBMGR = v.at(0);
             else if (k == "runseed") run_.seed = (uint64_t)std::stoull(v.at(0));
         } catch (const std::exception &) {
@@ -665,7 +666,10 @@ int MPBeam::BeamMomtumUpdateDueToRFTest() { return check(cts_rf_kick(gpu), "rf")
 int MPBeam::BeamLongiPosTransferOneTurn() { return check(cts_drift(gpu), "drift"); }
 int MPBeam::BeamEnergyLossOneTurn() { return check(cts_energy_loss(gpu), "energy loss"); }
 int MPBeam::BeamSynRadDamping(int turn) { return check(cts_synrad(gpu, turn), "synrad"); }
-int MPBeam::SSBeamIonEffectOneInteractionPoint(int turn) { return check(cts_ion_phase(gpu, turn), "ions"); }
+int MPBeam::SSBeamIonEffectOneInteractionPoint(int turn, int k)
+{
+    return check(cts_ion_phase_ip(gpu, turn, k), "ions");
+}
 
 int MPBeam::SRWakeBeamIntaction()
 {
@@ -834,6 +838,143 @@ int MPBeam::GetCBMGR(ReadInputSettings &in, std::vector<double> &rates)
 // src/MPBeam.cpp:2319-2411): the moments of every bunch at the end of turn
 // `turn`, the ion charge its passage saw, and the cavity phasors
 // (cavInfo: cts_get_cavity_info of that turn when beam loading is on).
+void MPBeam::PrintResultHeader(std::ofstream &fout, const ReadInputSettings &in)
+{
+    const auto *B = in.ringParBasic;
+    const char *head[] = {
+        "SDDS1",
+        "&parameter name=beamCurr,    units=mA,   type=float,  &end",
+        "&parameter name=I1,          units=m,   type=float,  &end",
+        "&parameter name=I2,          units=1/m, type=float,  &end",
+        "&parameter name=I3,          units=1/m^2,   type=float,  &end",
+        "&parameter name=I4,          units=1/m, type=float,  &end",
+        "&parameter name=I5,          units=1/m, type=float,  &end",
+        "&parameter name=Jx,                     type=float,  &end",
+        "&parameter name=Jy,                     type=float,  &end",
+        "&parameter name=Jz,                     type=float,  &end",
+        "&parameter name=f1001Abs,               type=float,  &end",
+        "&parameter name=f1010Abs,               type=float,  &end",
+        "&parameter name=f0110Abs,               type=float,  &end",
+        "&parameter name=f0101Abs,               type=float,  &end",
+        "&parameter name=f1001Arg,  units=rad,   type=float,  &end",
+        "&parameter name=f1010Arg,  units=rad,   type=float,  &end",
+        "&parameter name=f0110Arg,  units=rad,   type=float,  &end",
+        "&parameter name=f0101Arg,  units=rad,   type=float,  &end",
+        "&parameter name=LCFactor,               type=float,  &end",
+        "&column name=Turns,                     type=long,   &end",
+        "&column name=IonCharge,      units=e,   type=float,  &end",
+        "&column name=maxAverX,       units=m,   type=float,  &end",
+        "&column name=maxAverY,       units=m,   type=float,  &end",
+        "&column name=averAllBunchX,  units=m,   type=float,  &end",
+        "&column name=averAllBunchY,  units=m,   type=float,  &end",
+        "&column name=averAllBunchZ,  units=m,   type=float,  &end",
+        "&column name=averAllBunchPX, units=rad, type=float,  &end",
+        "&column name=averAllBunchPY, units=rad, type=float,  &end",
+        "&column name=averAllBunchPZ, units=rad, type=float,  &end",
+        "&column name=rmsAllBunchX,   units=m,   type=float,  &end",
+        "&column name=rmsAllBunchY,   units=m,   type=float,  &end",
+        "&column name=rmsAllBunchZ,   units=m,   type=float,  &end",
+        "&column name=rmsAllBunchPX,  units=rad, type=float,  &end",
+        "&column name=rmsAllBunchPY,  units=rad, type=float,  &end",
+        "&column name=rmsAllBunchPZ,  units=rad, type=float,  &end",
+        "&column name=Nux,                       type=float,  &end",
+        "&column name=Nuy,                       type=float,  &end",
+        "&column name=deltaNu,                   type=float,  &end",
+        "&column name=f1001Abs,               type=float,  &end",
+        "&column name=f1010Abs,               type=float,  &end",
+        "&column name=f1001Arg,  units=rad,   type=float,  &end",
+        "&column name=f1010Arg,  units=rad,   type=float,  &end",
+        "&column name=LCFactor,               type=float,  &end",
+        "&column name=xyAlpha,   units=rad,   type=float,  &end",
+        "&column name=traceA,                 type=float,  &end",
+        "&column name=traceB,                 type=float,  &end",
+        "&column name=traceMSubN,             type=float,  &end",
+        "&column name=gamma0,                 type=float,  &end",
+        "&column name=gamma1,                 type=float,  &end",
+        "&column name=detH,                   type=float,  &end"};
+    for (const char *l : head) fout << l << std::endl;
+    // per printed bunch j: the reference's column names and units
+    const char *cols[16][2] = {{"averX_", "units=m,   "},           {"averPx_", "units=rad, "},
+                               {"averY_", "units=m,   "},           {"averPy_", "units=rad, "},
+                               {"averZ_", "units=m,   "},           {"averPz_", "units=rad, "},
+                               {"rmsBunchZ_", "units=m,   "},       {"rmsEnergySpread_", "units=rad, "},
+                               {"rmsRx_", "units=m,   "},           {"rmsRy_", "units=m,   "},
+                               {"rmsEmitx_", "units=m*rad,   "},    {"rmsEmity_", "units=m*rad,   "},
+                               {"eigenEmitx_", "units=m*rad,   "},  {"eigenEmity_", "units=m*rad,    "},
+                               {"transmission_", "               "}, {"xyCouplingAlpha_", "units=rad,  "}};
+    for (int j = 0; j < in.ringRun->TBTBunchPrintNum; j++)
+        for (auto &c : cols)
+            fout << "&column name=" << c[0] << j << ", " << c[1] << "type=float,  &end" << std::endl;
+    fout << "&data mode=ascii, &end" << std::endl;
+    // radiation integrals and damping partitions from the damping times
+    // (src/ReadInputSettings.cpp:1054-1077): J_x = 2 E / (U0 tau_x),
+    // J_y = 1, J_z = 3 - J_x; alpha_i = J_i U0 / (2 T0 E)
+    const double E = B->electronBeamEnergy, U0 = B->u0;
+    double J[3];
+    J[0] = 2 * E / U0 / B->synchRadDampTime[0];
+    J[1] = 1;
+    J[2] = 3 - J[0];
+    const double a0 = U0 / (2 * B->t0 * E);
+    double al[3];
+    for (int i = 0; i < 3; i++) al[i] = J[i] * a0;
+    const double re = 2.81794092E-15, clight = 299792458.0, cq = 3.83E-13;  // include/Global.h:27, 34, 38
+    const double g3 = std::pow(B->rGamma, 3), g2 = std::pow(B->rGamma, 2);
+    double I[5];
+    I[0] = B->circRing * B->alphac[0];
+    I[1] = B->circRing * (al[0] + al[2]) / (re * clight * g3);
+    I[3] = B->circRing * (al[2] - 2 * al[0]) / (re * clight * g3);
+    I[2] = (2 * I[1] + I[3]) * std::pow(B->sdelta0, 2) / (cq * g2);
+    I[4] = B->emitNat[0] * (I[1] - I[3]) / (cq * g2);
+    fout << B->ringCurrent << std::endl;
+    for (double v : I) fout << v << std::endl;
+    for (double v : J) fout << v << std::endl;
+    // resonance driving terms and the linear coupling factor of the
+    // lattice-coupling analysis: not computed, written as 0
+    for (int i = 0; i < 9; i++) fout << 0.0 << std::endl;
+    fout << "! page number " << 1 << std::endl;
+    fout << in.ringRun->nTurns << std::endl;
+}
+
+void MPBeam::PrintResultRow(std::ofstream &fout, int turn, const cts_moments *m, const double *eig, double ionCharge,
+                            const ReadInputSettings &in)
+{
+    // MPGetBeamInfo (src/MPBeam.cpp:2135-2317): the largest centroids and the
+    // means and rms over the bunches of the centroids
+    const size_t nb = beamVec.size();
+    double mx = m[0].mean[0], my = m[0].mean[2];
+    double av[6] = {0, 0, 0, 0, 0, 0}, rm[6] = {0, 0, 0, 0, 0, 0};
+    const int ord[6] = {0, 2, 4, 1, 3, 5};  // x y z px py pz
+    for (size_t i = 0; i < nb; i++) {
+        if (i > 0 && m[i].mean[0] > mx) mx = m[i].mean[0];
+        if (i > 0 && m[i].mean[2] > my) my = m[i].mean[2];
+        for (int c = 0; c < 6; c++) {
+            av[c] += m[i].mean[ord[c]];
+            rm[c] += std::pow(m[i].mean[ord[c]], 2);
+        }
+    }
+    auto w = [&](double v) { fout << std::setw(15) << std::left << v; };
+    fout << std::setw(15) << std::left << turn;
+    w(ionCharge);
+    w(mx);
+    w(my);
+    for (int c = 0; c < 6; c++) w(av[c] / nb);
+    for (int c = 0; c < 6; c++) w(std::sqrt(rm[c] / nb));
+    const double nux = in.ringParBasic->workQx - std::floor(in.ringParBasic->workQx);
+    const double nuy = in.ringParBasic->workQy - std::floor(in.ringParBasic->workQy);
+    w(nux);
+    w(nuy);
+    w(nuy - nux);
+    for (int c = 0; c < 12; c++) w(0.0);  // lattice-coupling columns, not computed
+    for (int j = 0; j < in.ringRun->TBTBunchPrintNum; j++) {
+        cts_derived d{};
+        cts_derive_moments(gpu, &m[j], 0, &d);
+        w(m[j].mean[0]); w(m[j].mean[1]); w(m[j].mean[2]); w(m[j].mean[3]); w(m[j].mean[4]); w(m[j].mean[5]);
+        w(d.sigma_z); w(d.sigma_dp); w(d.rms_x); w(d.rms_y); w(d.emit[0]); w(d.emit[1]);
+        w(eig[4 * j]); w(eig[4 * j + 1]); w(m[j].transmission); w(eig[4 * j + 3]);
+    }
+    fout << std::endl;
+}
+
 void MPBeam::PrintTBTBunchAverData(std::ofstream &fout, int turn, const cts_moments *m, const double *ionCharge,
                                    const double *cavInfo,
                                    const ReadInputSettings &in, bool header)
@@ -1021,30 +1162,33 @@ int MPBeam::RunSettings(const LatticeInterActionPoint &lat, const ReadInputSetti
     c.synrad = R->synRadDampingFlag != 0;
     c.seed = R->seed;
     if (R->beamIonFlag) {
-        if (lat.sections.size() != 1) {
-            std::cerr << "beam-ion interaction: one interaction point only" << std::endl;
-            return 1;
-        }
         if (lat.gasSpec > CTS_MAX_SPECIES) {
             std::cerr << "at most " << CTS_MAX_SPECIES << " gas species" << std::endl;
             return 1;
         }
-        cts_ion_config &ic = c.ion;
+        // a cloud at every interaction point, with that point's vacuum
+        // pressure, temperature, interaction length and aperture
+        // (LatticeInterActionPoint::InitialLattice, src/LatticeInterActionPoint.cpp:232-250)
         c.ions = true;
-        ic.nspec = lat.gasSpec;
-        ic.n_new = in.ringIonEffPara->macroIonNumberGeneratedPerIP;
-        ic.max_ions = (int64_t)lat.ionMaxNumberOneInterPoint;
-        for (int p = 0; p < ic.nspec; p++) {
-            ic.mass_number[p] = lat.ionMassNumber[p];
-            ic.cross_section[p] = lat.corssSectionEI[p];
-            ic.pressure[p] = lat.gasPercent[p] * lat.vacuumPressure[0];
+        c.ion.assign(lat.sections.size(), cts_ion_config{});
+        for (size_t k = 0; k < lat.sections.size(); k++) {
+            cts_ion_config &ic = c.ion[k];
+            ic.nspec = lat.gasSpec;
+            ic.n_new = in.ringIonEffPara->macroIonNumberGeneratedPerIP;
+            ic.max_ions = (int64_t)lat.ionMaxNumberOneInterPoint;
+            for (int p = 0; p < ic.nspec; p++) {
+                ic.mass_number[p] = lat.ionMassNumber[p];
+                ic.cross_section[p] = lat.corssSectionEI[p];
+                ic.pressure[p] = lat.gasPercent[p] * lat.vacuumPressure[k];
+            }
+            ic.temperature = lat.temperature[k];
+            ic.length = lat.interactionLength[k];
+            ic.loss_boundary = lat.ionLossBoundary;
+            ic.ap_x = lat.pipeAperatureX[k];
+            ic.ap_y = lat.pipeAperatureY[k];
+            // each point draws from its own stream
+            ic.seed = R->seed + 0x9E3779B97F4A7C15ull * k;
         }
-        ic.temperature = lat.temperature[0];
-        ic.length = lat.interactionLength[0];
-        ic.loss_boundary = lat.ionLossBoundary;
-        ic.ap_x = lat.pipeAperatureX[0];
-        ic.ap_y = lat.pipeAperatureY[0];
-        ic.seed = R->seed;
     }
     return 0;
 }
@@ -1098,7 +1242,8 @@ int MPBeam::Run(Train &, LatticeInterActionPoint &lat, ReadInputSettings &in)
                                      c.fr.data()), "wake"))
         return 1;
     if (check(cts_set_synrad(gpu, c.synrad, 2, c.seed), "synrad")) return 1;
-    if (c.ions && check(cts_set_ions(gpu, &c.ion), "ions")) return 1;
+    for (size_t k = 0; c.ions && k < c.ion.size(); k++)
+        if (check(cts_set_ions_ip(gpu, (int32_t)k, &c.ion[k]), "ions")) return 1;
     if (c.lrw) {
         cts_lrwake_config lc{};
         lc.nturns = c.lrw_nturns;
@@ -1137,18 +1282,50 @@ int MPBeam::Run(Train &, LatticeInterActionPoint &lat, ReadInputSettings &in)
     const int nsamp = (R->nTurns + interval - 1) / interval;
     std::vector<double> ionq((size_t)nsamp * nb, 0.0);
     std::vector<double> pas((size_t)nb * CTS_MAX_SPECIES * 6);
+    // the ion number every bunch saw, summed over the interaction points
     auto record_ions = [&](int sample) -> int {
         if (!c.ions) return 0;
-        if (check(cts_get_ion_passages(gpu, pas.data()), "ion passages")) return 1;
-        for (int b = 0; b < nb; b++) {
-            double q = 0;
-            for (int p = 0; p < lat.gasSpec; p++) q += pas[((size_t)b * CTS_MAX_SPECIES + p) * 6 + 5];
-            ionq[(size_t)sample * nb + b] = q;
+        for (int k = 0; k < nsec; k++) {
+            if (check(cts_get_ion_passages_ip(gpu, k, pas.data()), "ion passages")) return 1;
+            for (int b = 0; b < nb; b++) {
+                double q = 0;
+                for (int p = 0; p < lat.gasSpec; p++) q += pas[((size_t)b * CTS_MAX_SPECIES + p) * 6 + 5];
+                ionq[(size_t)sample * nb + b] += q;
+            }
         }
         return 0;
     };
     std::vector<cts_moments> m(nb);
-    if (nsec == 1) {
+    if (R->TBTBunchPrintNum > nb) {
+        std::cerr << "wrong settings: Total Bunch Number is " << nb << std::endl;
+        return 1;
+    }
+    // result.sdds: a row at the end of every turn, from the moments after
+    // the turn's kicks (MPBeamRMSCal(0)), the eigen-emittances and the ion
+    // charge; the turns then run one by one
+    std::ofstream result;
+    std::vector<double> eig((size_t)nb * 4);
+    auto result_row = [&](int n) -> int {
+        double ionc = 0.0;
+        if (check(cts_rms(gpu, m.data()), "rms") || check(cts_eigen_emit(gpu, eig.data()), "eigen-emittances") ||
+            check(cts_get_ion_charge(gpu, &ionc), "ion charge"))
+            return 1;
+        PrintResultRow(result, n, m.data(), eig.data(), ionc, in);
+        return 0;
+    };
+    if (writeResult) {
+        result.open("result.sdds");
+        PrintResultHeader(result, in);
+    }
+    if (nsec == 1 && writeResult) {
+        for (int n = 0; n < R->nTurns; n++) {
+            if (check(cts_turn(gpu, n), "turn") || result_row(n)) return 1;
+            if (n % interval == 0) {
+                samples_.insert(samples_.end(), m.begin(), m.end());
+                if (record_ions(n / interval)) return 1;
+            }
+        }
+    } else if (nsec == 1) {
         // the turn loop in stretches that end on the print turns; HIP-graph
         // replays inside a stretch
         if (check(cts_set_history(gpu, R->nTurns), "history")) return 1;
@@ -1167,16 +1344,22 @@ int MPBeam::Run(Train &, LatticeInterActionPoint &lat, ReadInputSettings &in)
         }
     } else {
         for (int n = 0; n < R->nTurns; n++) {
-            for (int k = 0; k < nsec; k++)
+            // section by section: the ion phase at point k (on the moments
+            // after map k - 1), then map k (src/MPBeam.cpp:1059-1070)
+            for (int k = 0; k < nsec; k++) {
+                if (c.ions && SSBeamIonEffectOneInteractionPoint(n, k)) return 1;
                 if (BeamTransferPerInteractionPointDueToLatticeT(k)) return 1;
+            }
             if (in.ringParBasic->skewQuadK != 0 && BeamTransferDueToSkewQuad()) return 1;
             if (BeamMomtumUpdateDueToRFTest() || BeamLongiPosTransferOneTurn() || BeamEnergyLossOneTurn()) return 1;
             if (R->synRadDampingFlag && BeamSynRadDamping(n)) return 1;
             if (R->sRWakeFlag && check(cts_sr_wake(gpu, nullptr), "sr wake")) return 1;
             if (MarkParticleLostInBunch()) return 1;
+            if (writeResult && result_row(n)) return 1;
             if (n % interval == 0) {
-                if (check(cts_rms(gpu, m.data()), "rms")) return 1;
+                if (!writeResult && check(cts_rms(gpu, m.data()), "rms")) return 1;
                 samples_.insert(samples_.end(), m.begin(), m.end());
+                if (record_ions(n / interval)) return 1;
             }
         }
     }

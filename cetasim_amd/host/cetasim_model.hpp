@@ -81,6 +81,7 @@ public:
         int calSetting = 2, nTurns = 0, bunchInfoPrintInterval = 1, synRadDampingFlag = 0, beamIonFlag = 0;
         int sRWakeFlag = 0, lRWakeFlag = 0, growthRateFittingStart = 0, growthRateFittingEnd = 0;
         std::string TBTBunchAverData = "TBTBunchAverData", runCBMGR = "CBMGR";
+        int TBTBunchPrintNum = 0;  // bunches with their own columns in result.sdds
         uint64_t seed = 1;  // seed of the injected distribution, ions and noise (Q13)
     };
 
@@ -179,7 +180,7 @@ struct RunCalls {
     bool synrad = false;
     uint64_t seed = 0;
     bool ions = false;
-    cts_ion_config ion{};
+    std::vector<cts_ion_config> ion;  // one per interaction point (section)
     bool lrw = false;
     int lrw_nturns = 0;
     std::vector<double> lrw_sectors, lrw_res;  // [n][7], [n][9] (cts_lrwake_config's layout)
@@ -225,7 +226,7 @@ public:
     int SRWakeBeamIntaction();
     int MarkParticleLostInBunch();
     int MPBeamRMSCal(int k);
-    int SSBeamIonEffectOneInteractionPoint(int turn);
+    int SSBeamIonEffectOneInteractionPoint(int turn, int k);
     int DownloadParticles();
     // growth rates of the coupled-bunch modes from the run's samples
     // (MPBeam::GetCBMGR, src/MPBeam.cpp:1496-1891): ideal modes, IQ,
@@ -235,6 +236,12 @@ public:
     void PrintTBTBunchAverData(std::ofstream &fout, int turn, const cts_moments *m, const double *ionCharge,
                                const double *cavInfo,
                                const ReadInputSettings &inputParameter, bool header);
+    // result.sdds (src/MPBeam.cpp:901-1159): the header, then one row per
+    // turn from the end-of-turn moments, eigen-emittances and ion charge
+    void PrintResultHeader(std::ofstream &fout, const ReadInputSettings &inputParameter);
+    void PrintResultRow(std::ofstream &fout, int turn, const cts_moments *m, const double *eigen, double ionCharge,
+                        const ReadInputSettings &inputParameter);
+    bool writeResult = true;  // result.sdds every turn (one eigen-emittance pass per turn)
     // the long-range wake table file of InitialLRWake (src/WakeFunction.cpp:
     // 100-155) from the library's table
     int PrintLRWakeTable(const std::string &path);

@@ -157,11 +157,11 @@ class Tracker:
         capi.check(self.lib.cts_get_cavity_info(self.h, turn, capi.dptr(out)))
         return out.reshape(self.nb, self.bl_nres, 8)
 
-    def set_ions(self, cfg):
-        """cfg: capi.IonConfig, or None to switch ions off."""
+    def set_ions(self, cfg, ip=0):
+        """cfg: capi.IonConfig, or None to switch ions off, at interaction point ip."""
         if cfg is None:
             cfg = capi.IonConfig()
-        capi.check(self.lib.cts_set_ions(self.h, ctypes.byref(cfg)))
+        capi.check(self.lib.cts_set_ions_ip(self.h, ip, ctypes.byref(cfg)))
         self.nspec = cfg.nspec
 
     def set_field_model(self, sur, mode):
@@ -169,21 +169,21 @@ class Tracker:
         capi.check(self.lib.cts_set_field_model(self.h, sur.h if sur is not None else None, mode))
         self._sur = sur
 
-    def ion_phase(self, turn):
-        return capi.check(self.lib.cts_ion_phase(self.h, turn))
+    def ion_phase(self, turn, ip=0):
+        return capi.check(self.lib.cts_ion_phase_ip(self.h, turn, ip))
 
-    def ion_cloud(self, species=0):
+    def ion_cloud(self, species=0, ip=0):
         n = ctypes.c_int64()
-        capi.check(self.lib.cts_get_ion_cloud(self.h, species, 0, ctypes.byref(n), None, None, None, None))
+        capi.check(self.lib.cts_get_ion_cloud_ip(self.h, ip, species, 0, ctypes.byref(n), None, None, None, None))
         out = {c: np.empty(n.value) for c in ("x", "y", "vx", "vy")}
-        capi.check(self.lib.cts_get_ion_cloud(self.h, species, n.value, ctypes.byref(n),
-                                              *[capi.dptr(out[c]) for c in ("x", "y", "vx", "vy")]))
+        capi.check(self.lib.cts_get_ion_cloud_ip(self.h, ip, species, n.value, ctypes.byref(n),
+                                                 *[capi.dptr(out[c]) for c in ("x", "y", "vx", "vy")]))
         return out
 
-    def ion_passages(self):
+    def ion_passages(self, ip=0):
         """[nb][MAX_SPECIES][6]: count, mean x, mean y, rms x, rms y, ion number."""
         out = np.empty(self.nb * capi.MAX_SPECIES * 6)
-        capi.check(self.lib.cts_get_ion_passages(self.h, capi.dptr(out)))
+        capi.check(self.lib.cts_get_ion_passages_ip(self.h, ip, capi.dptr(out)))
         return out.reshape(self.nb, capi.MAX_SPECIES, 6)
 
     def turn(self, n):
@@ -229,6 +229,12 @@ class Tracker:
         m = (capi.Moments * self.nb)()
         capi.check(self.lib.cts_get_moments(self.h, m))
         return [moments_dict(x) for x in m]
+
+    def eigen_emit(self):
+        """[nb][4]: eigenEmitX, eigenEmitY, emitXY4D, xyCouplingAlpha (GetEigenEmit)."""
+        out = np.empty(4 * self.nb)
+        capi.check(self.lib.cts_eigen_emit(self.h, capi.dptr(out)))
+        return out.reshape(self.nb, 4)
 
     def set_history(self, cap):
         capi.check(self.lib.cts_set_history(self.h, cap))

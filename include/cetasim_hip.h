@@ -262,8 +262,14 @@ typedef struct {
     double ap_x, ap_y;             /* pipe aperture at the IP [m] */
     uint64_t seed;
 } cts_ion_config;
-/* Set up (or switch off, nspec = 0) the ion cloud; the cloud starts empty. */
+/* Set up (or switch off, nspec = 0) the ion cloud at interaction point 0;
+ * the cloud starts empty. */
 int cts_set_ions(cts_handle *h, const cts_ion_config *cfg);
+/* The same at interaction point k (section k, 0 <= k < nsec): a cloud per
+ * interaction point, as LatticeInterActionPoint keeps one per point
+ * (src/LatticeInterActionPoint.cpp:974-1228, ionAccumu*[k][p]); pressure,
+ * temperature, length and apertures are those of point k. */
+int cts_set_ions_ip(cts_handle *h, int32_t k, const cts_ion_config *cfg);
 
 /* ------------------------------------------------------------ the turn
  * One turn of MPBeam::Run (src/MPBeam.cpp:1059-1102) for every bunch:
@@ -318,14 +324,26 @@ int cts_mark_lost(cts_handle *h, double *transmission);
  * the electrons' kicks are applied.  turn keys the ion generation stream.
  * cts_turn calls this itself when ions are on. */
 int cts_ion_phase(cts_handle *h, int32_t turn);
+/* The ion phase at interaction point k, before section map k
+ * (MPBeam::SSBeamIonEffectOneInteractionPoint(k), src/MPBeam.cpp:1059-1066):
+ * bunch views from the present moments with section k's beta functions. */
+int cts_ion_phase_ip(cts_handle *h, int32_t turn, int32_t k);
 /* The cloud of one species after the last ion phase (survivors in order):
  * *n gets the count; arrays of `cap` entries may be NULL. */
 int cts_get_ion_cloud(cts_handle *h, int32_t species, int64_t cap, int64_t *n, double *x,
                       double *y, double *vx, double *vy);
+int cts_get_ion_cloud_ip(cts_handle *h, int32_t k, int32_t species, int64_t cap, int64_t *n,
+                         double *x, double *y, double *vx, double *vy);
 /* Per passage of the last ion phase: out[(b * CTS_MAX_SPECIES + p) * 6 ..]
  * = macro-ion count, mean x, mean y, rms x, rms y, ion number seen by bunch
  * b (IonRMSCal(k, p) and GetTotIonCharge inputs). out has nb*CTS_MAX_SPECIES*6. */
 int cts_get_ion_passages(cts_handle *h, double *out);
+int cts_get_ion_passages_ip(cts_handle *h, int32_t k, double *out);
+/* LatticeInterActionPoint::GetTotIonCharge after the last ion phase
+ * (src/LatticeInterActionPoint.cpp:1213-1226): the ions of every
+ * interaction point and species that survived its last passage, times the
+ * last passage's macro-ion charge (0 with ions off). */
+int cts_get_ion_charge(cts_handle *h, double *total);
 /* MPBeam::MPBeamRMSCal (src/MPBeam.cpp:2126): moments of the current state
  * (applies a pending wake kick first). out[nb]. */
 int cts_rms(cts_handle *h, cts_moments *out);
@@ -342,6 +360,13 @@ int cts_get_history(cts_handle *h, int32_t first, int32_t nturns, cts_moments *o
 int cts_get_wake_state(cts_handle *h, int32_t *counts, double *edges, double *kick);
 /* GetMPBunchRMS derived quantities from raw moments at section k. */
 int cts_derive_moments(cts_handle *h, const cts_moments *m, int32_t k, cts_derived *out);
+/* MPBunch::GetEigenEmit (src/MPBunch.cpp:500-658) of every bunch in its
+ * present state: out[nb][4] = eigenEmitX, eigenEmitY, emitXY4D,
+ * xyCouplingAlpha.  As the reference: every particle, lost ones included,
+ * less the dispersion of section 0 times delta; the "means" are the
+ * integer-valued std::accumulate of the coordinates (zero while every
+ * coordinate is below 1 in magnitude) and are not divided by N. */
+int cts_eigen_emit(cts_handle *h, double *out);
 
 /* Wait for all work on the handle's stream. */
 int cts_sync(cts_handle *h);
@@ -443,6 +468,14 @@ int cts_sur_train(cts_surrogate *s, int32_t steps, int32_t nwg, int32_t chunks, 
 int cts_sur_gradient(cts_surrogate *s, const float *samples5, int64_t n, float *grad, float *loss);
 /* Raw network outputs out2[n][2] for features3[n][3]. */
 int cts_sur_eval(cts_surrogate *s, const float *features3, int64_t n, float *out2);
+/* cts_sur_eval without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc); no counterpart in the reference.  Inputs and outputs
+ * are staged through one static 5 MiB device buffer per GPU (held until the
+ * process ends): n <= 262144 points, larger n is refused with CTS_ERR_ARG.
+ * Every call on a GPU shares that buffer, so calls that overlap on one GPU
+ * (other threads, other surrogates) race; calls in sequence do not. */
+int cts_sur_eval_noalloc(cts_surrogate *s, const float *features3, int64_t n, float *out2);
 /* Field error on the validation set: per point |G_sur - G| / |G|;
  * err_out[n_val] may be NULL. */
 int cts_sur_validate(cts_surrogate *s, double *max_err, double *rms_err, double *err_out);

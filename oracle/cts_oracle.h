@@ -166,6 +166,13 @@ int64_t orc_sr_wake(orc_bunch *b, int nbins, int mode, double sigma_z_stale,
                     const double *lomega, double *poten, int32_t *counts,
                     double *edges, const double *wake_table);
 
+/* The wake sum as the reference evaluates it, one pseudo-Green call per
+ * bin pair (src/MPBunch.cpp:1983-1993): pz[i] += W((i-j) dt) counts[j] for
+ * i, j < nbins, j inner.  Same values as the lag table (the same arguments),
+ * at the reference's cost; the CPU baseline times it. */
+void orc_wake_pairs(int nbins, double dt, int nres, const double *lrs, const double *lq,
+                    const double *lomega, const int32_t *counts, double *pz);
+
 /* Longitudinal wake table W[l] = orc_bbr_wake((l-(nbins-1))*dt)[2],
  * l = 0..2*nbins-2 (lag-indexed). */
 void orc_wake_table(int nbins, double dt, int nres, const double *lrs,

@@ -444,6 +444,18 @@ void orc_wake_table(int nbins, double dt, int nres, const double *lrs,
     }
 }
 
+void orc_wake_pairs(int nbins, double dt, int nres, const double *lrs, const double *lq,
+                    const double *lomega, const int32_t *counts, double *pz)
+{
+    for (int i = 0; i < nbins; i++) {
+        for (int j = 0; j < nbins; j++) {
+            double w[3];
+            orc_bbr_wake((i - j) * dt, nres, lrs, lq, lomega, NULL, NULL, NULL, NULL, NULL, NULL, w);
+            pz[i] += w[2] * counts[j];
+        }
+    }
+}
+
 /* BunchTransferDueToSRWake, src/MPBunch.cpp:1887-2026, longitudinal part
  * (the transverse potentials are output only and not restated). */
 int64_t orc_sr_wake(orc_bunch *b, int nbins, int mode, double sigma_z_stale,

@@ -94,7 +94,9 @@ def load_dump(d):
                 gap=gap.copy(), harm=harm.copy(), parts=parts)
 
 
-def ion_config(s):
+def ion_config(s, k=0):
+    """The ion settings of interaction point k."""
+    e = s["ion_ips"][k]
     cfg = capi.IonConfig()
     cfg.nspec = s["ion_nspec"]
     cfg.n_new = s["ion_n_new"]
@@ -102,12 +104,12 @@ def ion_config(s):
     for p in range(capi.MAX_SPECIES):
         cfg.mass_number[p] = s["ion_mass"][p]
         cfg.cross_section[p] = s["ion_cross"][p]
-        cfg.pressure[p] = s["ion_pressure"][p]
-    cfg.temperature = s["ion_temperature"]
-    cfg.length = s["ion_length"]
+        cfg.pressure[p] = e["pressure"][p]
+    cfg.temperature = e["temperature"]
+    cfg.length = e["length"]
     cfg.loss_boundary = s["ion_loss"]
-    cfg.ap_x, cfg.ap_y = s["ion_ap"]
-    cfg.seed = s["ion_seed"]
+    cfg.ap_x, cfg.ap_y = e["ap"]
+    cfg.seed = e["seed"]
     return cfg
 
 

@@ -11,6 +11,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
+# the same sources built at -O0 (the reference Makefile's level), for the CPU baseline
+ORACLE_O0 = os.path.join(ROOT, "oracle", "liboracle_O0.so")
 REF = os.path.join(ROOT, "oracle", "_ref", "libfaddeeva_ref.so")
 
 D = ctypes.c_double
@@ -59,66 +61,78 @@ class OrcBunchView(ctypes.Structure):
                 ("gap", ctypes.c_int), ("gamma", D)]
 
 
-_lib = None
+_libs = {}
+_build = "O2"
+
+
+def set_build(name):
+    """Which oracle build lib() returns from now on: "O2" (default) or "O0"."""
+    global _build
+    assert name in ("O2", "O0")
+    _build = name
 
 
 def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(ORACLE):
+    if _build not in _libs:
+        path = ORACLE if _build == "O2" else ORACLE_O0
+        if not os.path.exists(path):
             subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
-        L = ctypes.CDLL(ORACLE)
-        L.orc_faddeeva_w.argtypes = [D, D, PD, PD]
-        I = ctypes.c_int
-        L.orc_lrw_yokoya.argtypes = [D, D, PD]
-        PIc = ctypes.POINTER(ctypes.c_int)
-        L.orc_bl_tables.restype = I
-        L.orc_bl_tables.argtypes = [I, PIc, PD, I, D, D, D, D, I, PD, PIc, I, PD, PD]
-        L.orc_bl_kick.argtypes = [ctypes.POINTER(OrcBunch), D, D, D, D, D, I, I, D, D, D]
-        L.orc_bbi_smooth.argtypes = [PD, I]
-        L.orc_bbi_potential.argtypes = [PD, I, PD, PD, PD]
-        L.orc_bbi_kick.argtypes = [ctypes.POINTER(OrcBunch), I, D, PIc, PD, PD, D, D, D, D, D, I, D, PD, PD]
-        L.orc_lrw_table.restype = I
-        L.orc_lrw_table.argtypes = [I, I, D, I, PD, I, PD, D, D, I, PD, PD, PD, PD]
-        L.orc_lrw_kick.restype = I
-        L.orc_lrw_kick.argtypes = [I, I, ctypes.POINTER(I), I, D, D, D, PD, PD, PD, I, PD]
-        L.orc_erfcx.restype = D; L.orc_erfcx.argtypes = [D]
-        L.orc_w_im.restype = D; L.orc_w_im.argtypes = [D]
-        L.orc_bassetti_erskine.argtypes = [D, D, D, D, PD, PD]
-        L.orc_gaussian_field.argtypes = [D, D, D, D, PD, PD]
-        L.orc_ion_field.argtypes = [D, D, D, D, PD, PD]
-        L.orc_section_matrices.argtypes = [ctypes.POINTER(OrcTwiss), ctypes.POINTER(OrcTwiss), PD, PD, PD]
-        L.orc_dgemm.argtypes = [ctypes.c_int] * 3 + [PD, PD, PD]
-        L.orc_section_map.argtypes = [ctypes.POINTER(OrcBunch), ctypes.POINTER(OrcMap)]
-        L.orc_skew_quad.argtypes = [ctypes.POINTER(OrcBunch), D]
-        L.orc_rf_kick.argtypes = [ctypes.POINTER(OrcBunch), D, D, ctypes.c_int, D, D, D]
-        L.orc_drift.argtypes = [ctypes.POINTER(OrcBunch), D, PD]
-        L.orc_energy_loss.argtypes = [ctypes.POINTER(OrcBunch), D, D, D]
-        L.orc_synrad.argtypes = [ctypes.POINTER(OrcBunch), PD, PD, PD, PD, PD]
-        L.orc_mark_lost.restype = D
-        L.orc_mark_lost.argtypes = [ctypes.POINTER(OrcBunch), D, ctypes.c_int, D, D, ctypes.POINTER(I64)]
-        L.orc_zminmax.argtypes = [ctypes.POINTER(OrcBunch), PD, PD]
-        L.orc_rms.argtypes = [ctypes.POINTER(OrcBunch), D, D, D, D, D, D, ctypes.POINTER(OrcMoments)]
-        L.orc_det.restype = D; L.orc_det.argtypes = [ctypes.c_int, PD]
-        L.orc_bbr_wake.argtypes = [D, ctypes.c_int] + [PD] * 9 + [PD]
-        L.orc_wake_table.argtypes = [ctypes.c_int, D, ctypes.c_int, PD, PD, PD, PD]
-        L.orc_sr_wake.restype = I64
-        L.orc_sr_wake.argtypes = [ctypes.POINTER(OrcBunch), ctypes.c_int, ctypes.c_int, D, D, D, D, D,
-                                  ctypes.c_int, PD, PD, PD, PD, PI32, PD, PD]
-        L.orc_philox4x32_10.argtypes = [ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32)]
-        L.orc_qe_normals.argtypes = [U64, U32, U64, PD]
-        L.orc_box_muller.argtypes = [U32, U32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
-        L.orc_qe_fill.argtypes = [U64, U32, U64, I64, PD]
-        L.orc_ion_generate.argtypes = [U64, U32, U32, ctypes.c_int, ctypes.c_int, D, D, D, D, PD, PD]
-        L.orc_ion_rms.argtypes = [ctypes.POINTER(OrcIonCloud), ctypes.c_int]
-        L.orc_ion_passage.restype = ctypes.c_int
-        L.orc_ion_passage.argtypes = [ctypes.POINTER(OrcIonCloud), ctypes.POINTER(OrcBunch),
-                                      ctypes.POINTER(OrcBunchView), D, D, U64, U32, U32, PD]
-        L.orc_fit_wlinear.argtypes = [PD, PD, PD, ctypes.c_int, PD, PD]
-        L.orc_cbm_analysis.argtypes = ([PD, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, PD,
-                                        PD, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double] + [PD] * 12)
-        _lib = L
-    return _lib
+        _libs[_build] = _bind(ctypes.CDLL(path))
+    return _libs[_build]
+
+
+def _bind(L):
+    L.orc_faddeeva_w.argtypes = [D, D, PD, PD]
+    I = ctypes.c_int
+    L.orc_lrw_yokoya.argtypes = [D, D, PD]
+    PIc = ctypes.POINTER(ctypes.c_int)
+    L.orc_bl_tables.restype = I
+    L.orc_bl_tables.argtypes = [I, PIc, PD, I, D, D, D, D, I, PD, PIc, I, PD, PD]
+    L.orc_bl_kick.argtypes = [ctypes.POINTER(OrcBunch), D, D, D, D, D, I, I, D, D, D]
+    L.orc_bbi_smooth.argtypes = [PD, I]
+    L.orc_bbi_potential.argtypes = [PD, I, PD, PD, PD]
+    L.orc_bbi_kick.argtypes = [ctypes.POINTER(OrcBunch), I, D, PIc, PD, PD, D, D, D, D, D, I, D, PD, PD]
+    L.orc_lrw_table.restype = I
+    L.orc_lrw_table.argtypes = [I, I, D, I, PD, I, PD, D, D, I, PD, PD, PD, PD]
+    L.orc_lrw_kick.restype = I
+    L.orc_lrw_kick.argtypes = [I, I, ctypes.POINTER(I), I, D, D, D, PD, PD, PD, I, PD]
+    L.orc_erfcx.restype = D; L.orc_erfcx.argtypes = [D]
+    L.orc_w_im.restype = D; L.orc_w_im.argtypes = [D]
+    L.orc_bassetti_erskine.argtypes = [D, D, D, D, PD, PD]
+    L.orc_gaussian_field.argtypes = [D, D, D, D, PD, PD]
+    L.orc_ion_field.argtypes = [D, D, D, D, PD, PD]
+    L.orc_section_matrices.argtypes = [ctypes.POINTER(OrcTwiss), ctypes.POINTER(OrcTwiss), PD, PD, PD]
+    L.orc_dgemm.argtypes = [ctypes.c_int] * 3 + [PD, PD, PD]
+    L.orc_section_map.argtypes = [ctypes.POINTER(OrcBunch), ctypes.POINTER(OrcMap)]
+    L.orc_skew_quad.argtypes = [ctypes.POINTER(OrcBunch), D]
+    L.orc_rf_kick.argtypes = [ctypes.POINTER(OrcBunch), D, D, ctypes.c_int, D, D, D]
+    L.orc_drift.argtypes = [ctypes.POINTER(OrcBunch), D, PD]
+    L.orc_energy_loss.argtypes = [ctypes.POINTER(OrcBunch), D, D, D]
+    L.orc_synrad.argtypes = [ctypes.POINTER(OrcBunch), PD, PD, PD, PD, PD]
+    L.orc_mark_lost.restype = D
+    L.orc_mark_lost.argtypes = [ctypes.POINTER(OrcBunch), D, ctypes.c_int, D, D, ctypes.POINTER(I64)]
+    L.orc_zminmax.argtypes = [ctypes.POINTER(OrcBunch), PD, PD]
+    L.orc_rms.argtypes = [ctypes.POINTER(OrcBunch), D, D, D, D, D, D, ctypes.POINTER(OrcMoments)]
+    L.orc_det.restype = D; L.orc_det.argtypes = [ctypes.c_int, PD]
+    L.orc_bbr_wake.argtypes = [D, ctypes.c_int] + [PD] * 9 + [PD]
+    L.orc_wake_table.argtypes = [ctypes.c_int, D, ctypes.c_int, PD, PD, PD, PD]
+    L.orc_sr_wake.restype = I64
+    L.orc_sr_wake.argtypes = [ctypes.POINTER(OrcBunch), ctypes.c_int, ctypes.c_int, D, D, D, D, D,
+                              ctypes.c_int, PD, PD, PD, PD, PI32, PD, PD]
+    L.orc_philox4x32_10.argtypes = [ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32)]
+    L.orc_qe_normals.argtypes = [U64, U32, U64, PD]
+    L.orc_box_muller.argtypes = [U32, U32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+    L.orc_qe_fill.argtypes = [U64, U32, U64, I64, PD]
+    L.orc_ion_generate.argtypes = [U64, U32, U32, ctypes.c_int, ctypes.c_int, D, D, D, D, PD, PD]
+    L.orc_ion_rms.argtypes = [ctypes.POINTER(OrcIonCloud), ctypes.c_int]
+    L.orc_ion_passage.restype = ctypes.c_int
+    L.orc_ion_passage.argtypes = [ctypes.POINTER(OrcIonCloud), ctypes.POINTER(OrcBunch),
+                                  ctypes.POINTER(OrcBunchView), D, D, U64, U32, U32, PD]
+    L.orc_fit_wlinear.argtypes = [PD, PD, PD, ctypes.c_int, PD, PD]
+    L.orc_cbm_analysis.argtypes = ([PD, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, PD,
+                                    PD, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double] + [PD] * 12)
+    L.orc_wake_pairs.argtypes = [ctypes.c_int, D, ctypes.c_int, PD, PD, PD, PI32, PD]
+    return L
 
 
 def ref_lib():

@@ -1,0 +1,247 @@
+"""The benchmark configurations at their stated sizes on the HIP path.
+
+* C1: one bunch of 1e5 macroparticles (seed 1), 100 fixed bins over
+  +-5 sigma_z, broadband-resonator wake, no excitation, 1000 turns as HIP
+  graph replays: every turn's centroid and rms against the oracle to 1e-9 of
+  the rms size, the last turn's bin counts bit-exact.
+* C2: one bunch of 1e7 macroparticles (seed 2), 400 bins, radiation damping
+  with quantum excitation from host-drawn normals (noise mode 1, seed 2):
+  bin counts and loss codes bit-exact every turn, coordinates to 1e-11 of
+  each coordinate's scale, moments to 1e-9, over 2 turns.
+* C3: the 80-bunch train (counts uniform in [1e4, 2e5], seed 3) with
+  residual-gas ions at one interaction point: the first turn against the
+  oracle (the ion passages of the first segment of 16 bunches to 1e-9, their
+  bunches' coordinates to 1e-11; further down the train the ions' motion at
+  full intensity is chaotic, so rounding differences grow and are only
+  reported), then 20 graph-replayed turns run twice: bitwise the same,
+  finite moments, the cloud within its maximum.
+* C5: the 480-bunch train (counts in [5e4, 2e5], seed 5, 5.9e7
+  macroparticles) with the wake, excitation and ions: 20 graph-replayed
+  turns run twice, bitwise the same; every bunch's bin counts add up to its
+  survivors inside the grid; the cloud within 5e6.  The oracle cannot walk
+  480 passages of this cloud in test time (about 1e9 macro-ion passages),
+  so the first turn of the first 16 bunches of the train (same counts,
+  seeds and ion settings) is compared with it instead.
+* C4: the surrogate trained on 4e6 samples (seed 4) as the benchmark trains
+  it; its validation error on 1e6 held-out points must stay within the
+  measured bounds (max 5e-2, rms 3e-3 relative).
+"""
+import numpy as np
+import pytest
+
+from cetasim_amd import configs
+from cetasim_amd.tracker import Tracker
+
+import oracle as O
+from test_gpu_train import compare_coords
+
+pytestmark = pytest.mark.gpu
+
+COORDS = O.COORDS
+P = configs.BASE
+BBR = P["bbr"]
+SZ = P["sig"][2]
+
+
+def single(n, seed):
+    ring, secs, _ = configs.lattice()
+    parts = configs.gaussian_bunch(n, seed)
+    tr = Tracker(ring, secs)
+    q, ne = configs.bunch_scalars([n], ring)
+    tr.set_bunches([n], q, ne)
+    tr.upload(parts)
+    return ring, secs, parts, q, tr
+
+
+def sigmas(r):
+    return np.sqrt(np.array([r.c2[0], r.c2[2], r.c2[3], r.c2[5], r.c2[6], r.c2[8]]))
+
+
+def test_c1_full_size_1000_turns():
+    w = configs.WORKLOADS["c1"]
+    n, turns = w["n"][0], w["turns"]
+    ring, secs, parts, q, tr = single(n, w["seed"])
+    tr.set_history(turns)
+    tr.set_wake(w["nbins"], 1, -5 * SZ, 5 * SZ, BBR["rs"], BBR["q"], BBR["fr"])
+    print(f"C1: {n} macroparticles x {turns} turns", flush=True)
+    tr.run_turns(0, turns, True)
+    hist = tr.history(0, turns)
+    ot = O.OracleTurn(ring, secs[0], nbins=w["nbins"], mode=1, grid=(-5 * SZ, 5 * SZ), bbr=BBR, macro_charge=q[0])
+    ob = O.Bunch(parts)
+    worst = 0.0
+    for t in range(turns):
+        ot.turn(ob, t)
+        r = O.rms(ob, secs[0])
+        g = hist[t]
+        sig = sigmas(r)
+        assert g["n_alive"] == r.n_alive, t
+        dm = np.abs(g["mean"] - np.array(r.mean[:])) / sig
+        rg = np.sqrt(np.array(g["c2"])[[0, 2, 3, 5, 6, 8]])
+        ds = np.abs(rg - sig) / sig
+        worst = max(worst, float(dm.max()), float(ds.max()))
+        assert dm.max() <= 1e-9 and ds.max() <= 1e-9, (t, dm, ds)
+    counts, _, _ = tr.wake_state()
+    assert np.array_equal(counts[0], ot.counts)
+    print(f"C1: worst centroid/rms deviation over {turns} turns: {worst:.3g} of the rms size")
+    tr.close()
+
+
+def test_c2_full_size_host_noise():
+    w = configs.WORKLOADS["c2"]
+    n = w["n"][0]
+    ring, secs, parts, q, tr = single(n, w["seed"])
+    tr.set_wake(w["nbins"], 1, -5 * SZ, 5 * SZ, BBR["rs"], BBR["q"], BBR["fr"])
+    tr.set_synrad(True, 1, 0)
+    ot = O.OracleTurn(ring, secs[0], nbins=w["nbins"], mode=1, grid=(-5 * SZ, 5 * SZ), bbr=BBR, macro_charge=q[0],
+                      synrad=True)
+    ob = O.Bunch(parts)
+    del parts
+    rng = np.random.default_rng(w["noise_seed"])
+    for t in range(2):
+        print(f"C2: turn {t} of {n} macroparticles with host noise", flush=True)
+        noise = rng.standard_normal((5, n))
+        tr.push_noise(noise)
+        tr.turn(t)
+        ot.turn(ob, t, noise=noise)
+        counts, _, _ = tr.wake_state()
+        assert np.array_equal(counts[0], ot.counts), t
+        m = tr.moments()[0]
+        r = O.rms(ob, secs[0])
+        sig = sigmas(r)
+        assert m["n_alive"] == r.n_alive
+        assert np.all(np.abs(m["mean"] - np.array(r.mean[:])) <= 1e-9 * sig), t
+    got = tr.download()
+    assert np.array_equal(got["lost"], ob.lost)
+    for c in COORDS:
+        scale = np.max(np.abs(ob[c]))
+        assert np.max(np.abs(got[c] - ob[c])) <= 1e-11 * scale, c
+    tr.close()
+
+
+def train_setup(T, counts=None, gap=None, wake=False, synrad=False):
+    ring, secs, _ = configs.lattice()
+    if counts is None:
+        counts = configs.train_counts(T["nb"], T["count_range"][0], T["count_range"][1], T["seed"])
+    counts = np.asarray(counts, dtype=np.int64)
+    nb = len(counts)
+    if gap is None:
+        gap = configs.train_gaps(nb, T["spacing"], ring.harmonic)
+    q, ne = configs.bunch_scalars(counts, ring)
+    parts = configs.train_bunches(counts, T["seed"])
+    tr = Tracker(ring, secs)
+    tr.set_bunches(counts, q, ne, gap)
+    tr.upload(parts)
+    cfg = configs.ion_config(ring, T["ions"])
+    tr.set_ions(cfg)
+    if synrad:
+        tr.set_synrad(True, 2, 2)
+    if wake:
+        tr.set_wake(T["nbins"], 1, -5 * SZ, 5 * SZ, BBR["rs"], BBR["q"], BBR["fr"])
+    return ring, secs, counts, gap, q, ne, cfg, parts, tr
+
+
+def first_turn_against_oracle(T, counts, gap, wake):
+    ring, secs, counts, gap, q, ne, cfg, parts, tr = train_setup(T, counts, gap, wake=wake, synrad=True)
+    off = np.concatenate([[0], np.cumsum(counts)])
+    bunches = [O.Bunch({c: parts[c][off[b]:off[b + 1]] for c in COORDS}) for b in range(len(counts))]
+    kw = dict(nbins=T["nbins"], mode=1, grid=(-5 * SZ, 5 * SZ), bbr=BBR, macro_charge=list(q)) if wake else {}
+    ot = O.OracleTrain(ring, secs[0], counts, ne, gap, cfg, T["ions"]["seed"], synrad=True, noise_seed=2, **kw)
+    tr.turn(0)
+    ot.turn(bunches, 0)
+    got = tr.ion_passages()
+    exp = ot.last_cloud
+    nseg = min(16, len(counts))
+    np.testing.assert_allclose(got[:nseg, 0, 5], exp[:nseg, 0, 0], rtol=1e-12)
+    np.testing.assert_allclose(got[:nseg, 0, 1:5], exp[:nseg, 0, 1:5], rtol=1e-9, atol=1e-18)
+    # the bunches of that segment: coordinates to 1e-11 of each one's scale
+    compare_coords(tr.download(), bunches[:nseg], off[:nseg + 1], 1e-11)
+    tr.close()
+    return got, exp
+
+
+def test_c3_first_turn_against_oracle():
+    T = configs.TRAIN
+    print("C3: first turn of the 80-bunch train against the oracle", flush=True)
+    got, exp = first_turn_against_oracle(T, None, None, wake=False)
+    # every passage: the cloud the oracle saw, where the ions' chaos allows
+    dev = np.abs(got[:, 0, 1:5] - exp[:, 0, 1:5]) / np.maximum(np.abs(exp[:, 0, 1:5]), 1e-300)
+    print(f"C3: largest relative deviation of the cloud moments over 80 passages: {dev.max():.3g}")
+
+
+def replay_twice(T, turns, wake, synrad):
+    outs = []
+    for rep in range(2):
+        ring, secs, counts, gap, q, ne, cfg, parts, tr = train_setup(T, wake=wake, synrad=synrad)
+        del parts
+        tr.turn(0)
+        tr.run_turns(1, turns - 1, True)
+        m = tr.moments()
+        cloud = tr.ion_passages()
+        wake_state = tr.wake_state() if wake else None
+        got = tr.download()
+        tr.close()
+        outs.append((counts, cfg, m, cloud, wake_state, got))
+    return outs
+
+
+def check_replays(outs, T, wake):
+    (counts, cfg, m0, c0, w0, g0), (_, _, m1, c1, w1, g1) = outs
+    for c in COORDS + ("lost",):
+        assert np.array_equal(g0[c], g1[c]), c
+    assert np.array_equal(c0, c1)
+    for b in range(len(counts)):
+        assert np.all(np.isfinite(m0[b]["mean"])) and np.all(np.isfinite(m0[b]["c2"])), b
+        assert np.array_equal(m0[b]["mean"], m1[b]["mean"]), b
+    assert np.all(c0[:, :, 0] <= T["ions"]["max_ions"])
+    assert c0[:, 0, 0].max() > 0
+    if wake:
+        counts_bins, edges, _ = w0
+        assert np.array_equal(counts_bins, w1[0])
+        off = np.concatenate([[0], np.cumsum(counts)])
+        # the last binning saw z before the last kick, which only moves delta
+        z, lost = g0["z"], g0["lost"]
+        lo, hi = -5 * SZ, 5 * SZ
+        for b in range(len(counts)):
+            zb, lb = z[off[b]:off[b + 1]], lost[off[b]:off[b + 1]]
+            inside = int(np.sum((lb == 0) & (zb >= lo) & (zb < hi)))
+            assert abs(int(counts_bins[b].sum()) - inside) <= 2, (b, counts_bins[b].sum(), inside)
+
+
+def test_c3_graph_replays_deterministic():
+    T = configs.TRAIN
+    print("C3: 20 graph-replayed turns, twice", flush=True)
+    outs = replay_twice(T, 20, wake=False, synrad=True)
+    check_replays(outs, T, wake=False)
+
+
+def test_c5_prefix_first_turn_against_oracle():
+    T = configs.LARGE
+    counts = configs.train_counts(T["nb"], T["count_range"][0], T["count_range"][1], T["seed"])[:16]
+    gap = np.full(16, T["spacing"], np.int32)
+    print("C5: first turn of the first 16 bunches against the oracle", flush=True)
+    first_turn_against_oracle(T, counts, gap, wake=True)
+
+
+def test_c5_full_size_graph_replays():
+    T = configs.LARGE
+    print("C5: 480 bunches, 20 graph-replayed turns, twice", flush=True)
+    outs = replay_twice(T, 20, wake=True, synrad=True)
+    assert int(outs[0][0].sum()) > 5.5e7
+    check_replays(outs, T, wake=True)
+
+
+def test_c4_full_training():
+    from cetasim_amd.surrogate import Surrogate
+    s = Surrogate()
+    s.generate(4_000_000, 1_000_000, 4)
+    s.init_params(4)
+    k = 2000
+    print("C4: surrogate on 4e6 samples, 20000 Adam steps", flush=True)
+    s.train(2 * k, nwg=512, chunks=2, lr_max=3e-3, lr_min=1e-3, seed=4)
+    s.train(4 * k, nwg=512, chunks=2, lr_max=1e-3, lr_min=1e-4, seed=5)
+    s.train(4 * k, nwg=512, chunks=2, lr_max=1e-4, lr_min=1e-6, seed=6)
+    mx, rms = s.validate()
+    print(f"C4: validation max relative field error {mx:.3g}, rms {rms:.3g}")
+    assert np.isfinite(mx) and mx <= 5e-2
+    assert rms <= 3e-3
+    s.close()

@@ -148,19 +148,24 @@ def check_against_capi(d, D):
         tr.set_beam_loading(dict(nres=nres, mode=bi[:, 0], type=bi[:, 1], harm=bi[:, 2], cold=bi[:, 3],
                                  rs=br[:, 0], q0=br[:, 1], coupling=br[:, 2], detune=br[:, 3], vol=br[:, 4],
                                  phase=br[:, 5], nbins=s["bl_nbins"], nturns=s["bl_nturns"]))
-    tr.set_history(nturns)
+    # the program's turn loop with result.sdds on: each turn, then the
+    # end-of-turn moments (the pending kicks applied), eigen-emittances and
+    # ion charge
     turns = list(range(0, nturns, interval))
-    ionq = []
+    lib = capi.load()
+    ionq, samples, rows = [], [], []
     for t in range(nturns):
         tr.turn(t)
+        m = (capi.Moments * nb)()
+        capi.check(lib.cts_rms(tr.h, m))
+        eig = tr.eigen_emit()
+        ionc = ctypes.c_double()
+        capi.check(lib.cts_get_ion_charge(tr.h, ctypes.byref(ionc)))
+        rows.append((m, eig, ionc.value))
         if t % interval == 0:
             ionq.append(tr.ion_passages()[:, :, 5].sum(axis=1))
-    lib = capi.load()
-    samples = []
-    for t in turns:
-        m = (capi.Moments * nb)()
-        capi.check(lib.cts_get_history(tr.h, t, 1, m))
-        samples.append(m)
+            samples.append(m)
+    check_result_sdds(d, D, tr, rows)
     # TBTBunchAverData.sdds: one page per print turn, one row per bunch
     pages = read_sdds_pages(os.path.join(d, s["tbt"] + ".sdds"))
     assert len(pages) == len(turns)
@@ -226,3 +231,42 @@ def check_against_capi(d, D):
     # the ions were on: the cloud is not empty after the run
     assert tr.ion_passages()[:, 0, 0].sum() > 0
     tr.close()
+
+
+def check_result_sdds(d, D, tr, rows):
+    """result.sdds: the header's parameters and one row per turn (iostream
+    default formatting, six significant digits) from the same end-of-turn
+    moments, eigen-emittances and ion charge."""
+    s = D["settings"]
+    nb, nturns = s["nb"], s["nTurns"]
+    nprint = s["tbt_print"]
+    g = lambda v: float("%g" % v)  # noqa: E731
+    lines = open(os.path.join(d, "result.sdds")).read().splitlines()
+    cols = [ln for ln in lines if ln.startswith("&column")]
+    assert len(cols) == 34 + 16 * nprint
+    i0 = lines.index("&data mode=ascii, &end")
+    params = [float(x) for x in lines[i0 + 1:i0 + 19]]
+    assert params[0] == g(s["ring_current"])
+    assert all(v == 0.0 for v in params[9:18])
+    page = read_sdds_pages(os.path.join(d, "result.sdds"))[0]
+    assert int(page[0][0]) == nturns
+    lib = capi.load()
+    for t in range(nturns):
+        row = [float(x) for x in page[1 + t]]
+        m, eig, ionc = rows[t]
+        means = np.array([[m[b].mean[k] for k in (0, 2, 4, 1, 3, 5)] for b in range(nb)])
+        exp = [t, g(ionc), g(means[:, 0].max()), g(means[:, 1].max())]
+        exp += [g(v) for v in means.sum(axis=0) / nb] + [g(v) for v in np.sqrt((means ** 2).sum(axis=0) / nb)]
+        assert row[:16] == exp, t
+        assert row[19:31] == [0.0] * 12
+        for j in range(nprint):
+            der = capi.Derived()
+            capi.check(lib.cts_derive_moments(tr.h, ctypes.byref(m[j]), 0, ctypes.byref(der)))
+            r = row[31 + 16 * j:31 + 16 * (j + 1)]
+            assert r[:6] == [g(m[j].mean[k]) for k in range(6)], (t, j)
+            assert r[6:12] == [g(v) for v in (der.sigma_z, der.sigma_dp, der.rms_x, der.rms_y, der.emit[0],
+                                              der.emit[1])], (t, j)
+            assert r[12:14] == [g(eig[j, 0]), g(eig[j, 1])], (t, j)
+            assert r[14] == g(m[j].transmission) and r[15] == g(eig[j, 3])
+    # the ions were on: the charge column is not all zero
+    assert any(float(page[1 + t][1]) > 0 for t in range(nturns))

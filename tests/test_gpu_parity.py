@@ -95,6 +95,35 @@ def test_section_map(dispersive):
     tr.close()
 
 
+@pytest.mark.parametrize("big", [False, True])
+def test_eigen_emit(big):
+    """GetEigenEmit on the device against the oracle: a coupled bunch (x-y
+    correlated), a dispersive section 0, 5% lost particles (included, as the
+    reference includes them); big: one lost particle at |x| > 1 m, where the
+    reference's integer accumulate gives a non-zero "mean"."""
+    (ring, secs, der), p = lattice()
+    secs = [dispersive_section(ring, der)]
+    n = 30_000
+    parts = configs.gaussian_bunch(n, 51)
+    parts["y"] = parts["y"] + 0.3 * parts["x"]
+    parts["py"] = parts["py"] - 0.2 * parts["px"] + 0.1 * parts["x"] / 10.0
+    lost = random_losses(n, 52)
+    if big:
+        i = int(np.argmax(lost != 0))
+        parts["x"][i] = 1.7
+    tr = tracker(ring, secs, parts, lost)
+    got = tr.eigen_emit()[0]
+    ob = O.Bunch(parts, lost)
+    r = O.rms(ob, secs[0])
+    exp = np.array([r.eigen_x, r.eigen_y, r.emit4d, r.xy_alpha])
+    assert np.all(np.isfinite(exp[:3])) and exp[1] > 0
+    np.testing.assert_allclose(got[[0, 2, 3]], exp[[0, 2, 3]], rtol=1e-10)
+    # eigenEmitY^2 = (-tr - sqrt(tr^2 - 16 det)) / 4 cancels by (X/Y)^2: its
+    # error is bounded against eigenEmitX^2, not against itself
+    assert abs(got[1] ** 2 - exp[1] ** 2) <= 1e-12 * exp[0] ** 2
+    tr.close()
+
+
 def test_rf_drift_u0_skew():
     (ring, secs, _), p = lattice(skew_k=0.03)
     parts = configs.gaussian_bunch(20_000, 21)

@@ -301,6 +301,23 @@ def test_sr_wake_kick_is_the_convolution():
             assert np.dot(counts, got) < 0
 
 
+def test_wake_pairs_equal_lag_table():
+    """The reference's per-pair evaluation of the wake sum (the CPU
+    baseline's cost model) gives the lag table's sums bit for bit."""
+    nb, dt = 40, 2.5e-12
+    rs, q, om = np.array([1e4, 3e3]), np.array([1.0, 2.0]), np.array([2 * math.pi * 1e10, 2 * math.pi * 4e9])
+    counts = np.random.default_rng(3).integers(0, 500, nb).astype(np.int32)
+    tab = np.zeros(2 * nb - 1)
+    L.orc_wake_table(nb, dt, 2, O.p(rs), O.p(q), O.p(om), O.p(tab))
+    exp = np.zeros(nb)
+    for i in range(nb):
+        for j in range(nb):
+            exp[i] += tab[i - j + nb - 1] * counts[j]
+    got = np.zeros(nb)
+    L.orc_wake_pairs(nb, dt, 2, O.p(rs), O.p(q), O.p(om), counts.ctypes.data_as(O.PI32), O.p(got))
+    assert np.array_equal(got, exp)
+
+
 def test_bbr_wake_trailing_side_quadrature():
     """u < 0 (a receiver bin behind its source): the Riemann sum against
     the integral over the same truncated range [0, 3 v / (2 Q)]; the
