@@ -243,7 +243,7 @@ def check_result_sdds(d, D, tr, rows):
     g = lambda v: float("%g" % v)  # noqa: E731
     lines = open(os.path.join(d, "result.sdds")).read().splitlines()
     cols = [ln for ln in lines if ln.startswith("&column")]
-    assert len(cols) == 34 + 16 * nprint
+    assert len(cols) == 31 + 16 * nprint
     i0 = lines.index("&data mode=ascii, &end")
     params = [float(x) for x in lines[i0 + 1:i0 + 19]]
     assert params[0] == g(s["ring_current"])
