@@ -168,6 +168,7 @@ SIGNATURES = {
     "cts_sur_train": (I32, [ctypes.c_void_p, I32, I32, I32, ctypes.c_float, ctypes.c_float, U64, PF]),
     "cts_sur_gradient": (I32, [ctypes.c_void_p, PF, I64, PF, PF]),
     "cts_sur_eval": (I32, [ctypes.c_void_p, PF, I64, PF]),
+    "cts_sur_eval_noalloc": (I32, [ctypes.c_void_p, PF, I64, PF]),
     "cts_sur_validate": (I32, [ctypes.c_void_p, PD, PD, PD]),
     "cts_set_field_model": (I32, [ctypes.c_void_p, ctypes.c_void_p, I32]),
     "cts_cbm_growth_from_moments": (I32, [ctypes.POINTER(Moments), I32, I32, PI32, I32, PD, PD, I32, I32, I32, D,

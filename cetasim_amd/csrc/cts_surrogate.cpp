@@ -7,6 +7,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -28,6 +29,14 @@ hipError_t launch_mlp_field_error(const float *W, const double *exact, int64_t n
                                   hipStream_t st);
 hipError_t upload_faddeeva_tables_mlp(const FadTables &t);
 }  // namespace cts
+
+// Staging of cts_sur_eval_noalloc: features (3 floats) and outputs (2
+// floats) of up to kSurScratchPoints points, 5 MiB per device, held until
+// the process ends.  In host code the array's name is not a device address:
+// the kernels and copies take the per-device pointer that
+// hipGetSymbolAddress returns (scratch_ptr).
+constexpr int64_t kSurScratchPoints = 1 << 18;
+static __device__ __align__(256) float scratch[kSurScratchPoints * 5];
 
 struct cts_surrogate {
     hipStream_t stream = nullptr;
@@ -88,6 +97,27 @@ uint64_t mix(uint64_t &x)
     return z ^ (z >> 31);
 }
 
+// scratch's device address on each device, looked up once per device
+// with that device current
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch[kMaxDevices] = {};
+
+int scratch_ptr(float **out)
+{
+    int dev = 0;
+    SCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return sfail(CTS_ERR_STATE, "scratch: device index above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        SCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = (float *)p;
+    }
+    *out = g_scratch[dev];
+    return CTS_OK;
+}
+
 }  // namespace
 
 namespace cts {
@@ -126,6 +156,11 @@ int cts_sur_create(cts_surrogate **out)
         (void)hipStreamDestroy(s->stream);
         delete s;
         return CTS_ERR_HIP;
+    }
+    float *sp = nullptr;
+    if ((rc = scratch_ptr(&sp))) {
+        cts_sur_destroy(s);
+        return rc;
     }
     *out = s;
     return CTS_OK;
@@ -295,6 +330,25 @@ int cts_sur_eval(cts_surrogate *s, const float *features3, int64_t n, float *out
     (void)hipFree(d);
     (void)hipFree(o);
     if (e != hipSuccess) return sfail(CTS_ERR_HIP, std::string("eval: ") + hipGetErrorString(e));
+    return CTS_OK;
+}
+
+int cts_sur_eval_noalloc(cts_surrogate *s, const float *features3, int64_t n, float *out2)
+{
+    if (!s || !features3 || !out2 || n < 1) return sfail(CTS_ERR_ARG, "cts_sur_eval_noalloc");
+    if (n > kSurScratchPoints)
+        return sfail(CTS_ERR_ARG, "cts_sur_eval_noalloc: n above " + std::to_string(kSurScratchPoints) +
+                                      " points (the static staging buffer); call it in pieces or use cts_sur_eval");
+    float *d = nullptr;
+    int rc = scratch_ptr(&d);
+    if (rc) return rc;
+    // features in [0, 3n), outputs in [3n, 5n): every element used is
+    // written by this call before it is read, nothing needs clearing
+    float *o = d + 3 * n;
+    SCHK(hipMemcpyAsync(d, features3, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice, s->stream));
+    SCHK(launch_mlp_eval(s->d_params, d, n, 3, o, s->stream));
+    SCHK(hipMemcpyAsync(out2, o, (size_t)n * 2 * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    SCHK(hipStreamSynchronize(s->stream));
     return CTS_OK;
 }
 

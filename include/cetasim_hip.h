@@ -468,6 +468,14 @@ int cts_sur_train(cts_surrogate *s, int32_t steps, int32_t nwg, int32_t chunks, 
 int cts_sur_gradient(cts_surrogate *s, const float *samples5, int64_t n, float *grad, float *loss);
 /* Raw network outputs out2[n][2] for features3[n][3]. */
 int cts_sur_eval(cts_surrogate *s, const float *features3, int64_t n, float *out2);
+/* cts_sur_eval without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc); no counterpart in the reference.  Inputs and outputs
+ * are staged through one static 5 MiB device buffer per GPU (held until the
+ * process ends): n <= 262144 points, larger n is refused with CTS_ERR_ARG.
+ * Every call on a GPU shares that buffer, so calls that overlap on one GPU
+ * (other threads, other surrogates) race; calls in sequence do not. */
+int cts_sur_eval_noalloc(cts_surrogate *s, const float *features3, int64_t n, float *out2);
 /* Field error on the validation set: per point |G_sur - G| / |G|;
  * err_out[n_val] may be NULL. */
 int cts_sur_validate(cts_surrogate *s, double *max_err, double *rms_err, double *err_out);
