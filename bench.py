@@ -321,6 +321,7 @@ def train_turns(a, sur=None, mode=0):
     if sur is not None:
         tr.set_field_model(sur, mode)
     tr.autotune()
+    layout = tr.layout()
     dt, dt_ev, ktimes = timed_turns(tr, a, ("pass", "pass_a", "post", "finalize", "ion_prep", "ion_turn", "ion_compact",
                                             "ion_reduce", "ion_finish", "ion_kick", "lrwake",
                                             "bbi_bin", "bbi_wake", "bbi_kick"))
@@ -328,7 +329,7 @@ def train_turns(a, sur=None, mode=0):
     cloud = tr.ion_passages()
     m = tr.moments()
     tr.close()
-    return dt, ktimes, cloud, m, int(counts.sum()), nb, dt_ev
+    return dt, ktimes, cloud, m, int(counts.sum()), nb, dt_ev, layout
 
 
 def run_train(a):
@@ -339,7 +340,7 @@ def run_train(a):
     p = configs.BASE
     T = workload_train(a)
     ring, secs, _ = configs.lattice(p)
-    dt, ktimes, cloud, m_plain, n, nb, dt_ev = train_turns(a)
+    dt, ktimes, cloud, m_plain, n, nb, dt_ev, layout = train_turns(a)
     plain = n * a.steps / dt
     sur_info = None
     # value: the faster kick path whose field is within 1e-4 of the FP64
@@ -349,8 +350,8 @@ def run_train(a):
     if not a.plain_only:
         sur, sur_info = train_surrogate(a.sur_steps)
         # electron-side kicks from the surrogate (mode 2), and both sides (mode 3)
-        dts, kts, cloud_s, m_sur, _, _, _ = train_turns(a, sur, 2)
-        dt3, kt3, _, _, _, _, _ = train_turns(a, sur, 3)
+        dts, kts, cloud_s, m_sur, _, _, _, _ = train_turns(a, sur, 2)
+        dt3, kt3, _, _, _, _, _, _ = train_turns(a, sur, 3)
         sur.close()
         rate_sur = n * a.steps / dts
         sur_info.update({"mode": "electron-side kicks (ions kicked by the exact field)",
@@ -397,6 +398,9 @@ def run_train(a):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": tpm * n if tpm else None,
                      "traffic_source": tsrc, "kernel": "k_pass (fused turn)",
                      "kernel_ms": pass_ms, "launches": ktimes["pass"][1],
+                     "layout": {"workgroups": layout[0], "tile": layout[1]},
+                     # the pass's algorithmic bytes over the whole timed turn (ions included)
+                     "turn_frac": BYTES_PER_MPT * n / (dt / a.steps) / 1e9 / HBM_PEAK_GBS,
                      "other_kernels_ms_per_turn": per_turn},
         "ion_macroparticles_last_passage": float(cloud[-1, 0, 0]),
         "surrogate": sur_info,
@@ -437,6 +441,7 @@ def main():
     tr.set_wake(a.nbins, 1, -5 * sz, 5 * sz, bbr["rs"], bbr["q"], bbr["fr"])
     tr.set_synrad(True, 2, 2)
     nwg = tr.autotune()
+    layout = tr.layout()
 
     dt, dt_ev, kt = timed_turns(tr, a, ("pass", "post"))
     pass_ms, n_pass = kt["pass"]
@@ -471,6 +476,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None if tpm is None else tpm * n,
                      "traffic_source": tsrc,
                      "kernel": "k_pass (fused turn)", "kernel_ms": pass_ms, "launches": n_pass,
+                     "layout": {"workgroups": layout[0], "tile": layout[1]},
+                     "turn_frac": BYTES_PER_MPT * n / (dt / a.steps) / 1e9 / HBM_PEAK_GBS,
                      "other_kernels_ms": {"post (moments + wake)": post_ms}},
         "transmission": m["transmission"],
     }

@@ -85,7 +85,7 @@ class LrwakeConfig(ctypes.Structure):
 class BbimpConfig(ctypes.Structure):
     _fields_ = [("nimp", I32), ("f_max", D), ("zz_re", PD), ("zz_im", PD), ("flags", I32 * 5),
                 ("time_domain", I32), ("zqx_re", PD), ("zqx_im", PD), ("zqy_re", PD), ("zqy_im", PD),
-                ("beta_x", D), ("beta_y", D)]
+                ("beta_x", D), ("beta_y", D), ("zdx_re", PD), ("zdx_im", PD), ("zdy_re", PD), ("zdy_im", PD)]
 
 
 class BeamLoadingConfig(ctypes.Structure):
@@ -136,6 +136,8 @@ SIGNATURES = {
     "cts_flush": (I32, [ctypes.c_void_p]),
     "cts_run_turns": (I32, [ctypes.c_void_p, I32, I32, I32]),
     "cts_autotune": (I32, [ctypes.c_void_p, PI32]),
+    "cts_set_layout": (I32, [ctypes.c_void_p, I32, I32]),
+    "cts_get_layout": (I32, [ctypes.c_void_p, PI32, PI32]),
     "cts_section_map": (I32, [ctypes.c_void_p, I32]),
     "cts_skew_quad": (I32, [ctypes.c_void_p]),
     "cts_rf_kick": (I32, [ctypes.c_void_p]),

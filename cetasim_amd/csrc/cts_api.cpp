@@ -105,8 +105,9 @@ struct cts_handle {
 
     // segments
     int nwg = 0, nseg = 0, wg_target = 1024;
+    int tile = 0;  // 0: contiguous chunk per workgroup; > 0: tiles dealt round-robin
     int64_t *d_seg_begin = nullptr, *d_seg_end = nullptr, *d_bunch_off = nullptr, *d_bunch_count = nullptr;
-    int32_t *d_seg_bunch = nullptr, *d_wg_seg = nullptr, *d_bunch_seg = nullptr;
+    int32_t *d_seg_bunch = nullptr, *d_wg_seg = nullptr, *d_bunch_seg = nullptr, *d_seg_rank = nullptr;
     double *d_seg_part = nullptr;
     double *d_chunk_part = nullptr;
     uint32_t *d_chunk_cnt = nullptr;
@@ -178,7 +179,8 @@ struct cts_handle {
     bool bbi_on = false;
     BbiTable bbi;
     uint32_t *d_bbi_hist = nullptr;                 // [nb][n]
-    double *d_bbi_prof = nullptr, *d_bbi_wake = nullptr;  // [nb][n]
+    unsigned long long *d_bbi_dsum = nullptr;       // [nb][n][4] (dipolar planes)
+    double *d_bbi_prof = nullptr, *d_bbi_wake = nullptr;  // [3][nb][n], [5][nb][n]
     double *d_bbi_g = nullptr, *d_bbi_scale = nullptr;    // [n], [nb]
 
     // beam-loaded cavities (cts_set_beam_loading)
@@ -193,7 +195,7 @@ struct cts_handle {
     // the settings of the per-bunch effects above, deep copies kept so that
     // cts_set_bunches can set them up again for a new bunch layout
     struct KeepLrw { bool on = false; cts_lrwake_config cfg{}; std::vector<double> sectors, res; } keep_lrw;
-    struct KeepBbi { bool on = false; cts_bbimp_config cfg{}; std::vector<double> arr[6]; } keep_bbi;
+    struct KeepBbi { bool on = false; cts_bbimp_config cfg{}; std::vector<double> arr[10]; } keep_bbi;
     struct KeepBl { bool on = false; cts_beam_loading_config cfg{}; } keep_bl;
 
     // graph replays of two consecutive turns (cts_run_turns): the turn is
@@ -243,10 +245,11 @@ void free_bl_state(cts_handle *h)
 
 void free_bbi_state(cts_handle *h)
 {
-    void *ptrs[] = {h->d_bbi_hist, h->d_bbi_prof, h->d_bbi_wake, h->d_bbi_g, h->d_bbi_scale};
+    void *ptrs[] = {h->d_bbi_hist, h->d_bbi_prof, h->d_bbi_wake, h->d_bbi_g, h->d_bbi_scale, h->d_bbi_dsum};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     h->d_bbi_hist = nullptr;
+    h->d_bbi_dsum = nullptr;
     h->d_bbi_prof = h->d_bbi_wake = h->d_bbi_g = h->d_bbi_scale = nullptr;
     h->bbi_on = false;
     h->bbi = BbiTable{};
@@ -257,13 +260,13 @@ BbiParams bbi_params(cts_handle *h)
     BbiParams p{};
     p.n = h->bbi.n; p.is = h->bbi.is; p.ie = h->bbi.ie; p.nb = h->nb;
     p.zmin = h->bbi.zmin; p.dz = h->bbi.dz; p.bx = h->bbi.bx; p.by = h->bbi.by;
-    for (int q = 0; q < 3; q++) p.planes[q] = h->bbi.planes[q];
+    for (int q = 0; q < 5; q++) p.planes[q] = h->bbi.planes[q];
     p.energy = h->ring.energy; p.beta2 = std::pow(h->ring.rbeta, 2);
     p.seg_begin = h->d_seg_begin; p.seg_end = h->d_seg_end;
     p.seg_bunch = h->d_seg_bunch; p.wg_seg = h->d_wg_seg;
     p.x = h->d_x; p.y = h->d_y; p.z = h->d_z;
     p.px = h->d_px; p.py = h->d_py; p.dp = h->d_dp; p.flag = h->d_flag;
-    p.hist = h->d_bbi_hist; p.prof = h->d_bbi_prof; p.wake = h->d_bbi_wake;
+    p.hist = h->d_bbi_hist; p.dsum = h->d_bbi_dsum; p.prof = h->d_bbi_prof; p.wake = h->d_bbi_wake;
     p.g = h->d_bbi_g; p.scale = h->d_bbi_scale;
     for (int j = 0; j < kBbiTaps; j++) p.taps[j] = h->bbi.taps[j];
     return p;
@@ -280,10 +283,11 @@ void free_bunch_state(cts_handle *h)
                     h->d_cur, h->d_hist, h->d_sigz, h->d_zrange, h->d_noise, h->d_ion_e, h->d_ion_bview,
                     h->d_bhist, h->d_counts, h->d_kick, h->d_poten, h->d_edges_prev,
                     h->d_edges_now, h->d_table, h->d_scale, h->d_chunk_part, h->d_chunk_cnt,
-                    h->d_fb_count, h->d_fb_list, h->d_eig};
+                    h->d_fb_count, h->d_fb_list, h->d_eig, h->d_seg_rank};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     h->d_eig = nullptr;
+    h->d_seg_rank = nullptr;
     h->eig_cap = 0;
     h->d_x = h->d_px = h->d_y = h->d_py = h->d_z = h->d_dp = nullptr;
     h->d_flag = nullptr;
@@ -463,6 +467,7 @@ PassParams base_pass(cts_handle *h, uint32_t ops, int k)
     p.seg_bunch = h->d_seg_bunch; p.wg_seg = h->d_wg_seg;
     p.bunch_off = h->d_bunch_off;
     p.seg_part = h->d_seg_part;
+    p.seg_rank = h->d_seg_rank;
     p.bunch_hist = h->d_bhist ? h->d_bhist + (size_t)h->hist_parity * h->nb * h->nbins : nullptr;
     p.shift = h->d_shift;
     p.kick_prev = h->d_kick; p.edges_prev = h->d_edges_prev; p.edges_now = h->d_edges_now;
@@ -675,62 +680,104 @@ int reset_status(cts_handle *h)
     return CTS_OK;
 }
 
-// Segment table: the particle range cut into wg_target contiguous chunks
-// (multiples of 64), each cut again at bunch boundaries; workgroup w owns
-// the segments wg_seg[w] .. wg_seg[w+1].  (Re)allocates everything sized
-// by the number of segments.
-int layout_segments(cts_handle *h, int wg_target)
+// Segment table.  tile == 0: the particle range cut into wg_target
+// contiguous chunks (multiples of 64); tile > 0: cut into tiles of `tile`
+// particles dealt round-robin to min(wg_target, tiles) workgroups, so the
+// workgroups running at any moment stream one compact window of the arrays
+// (as a grid-stride loop does).  Every piece is cut again at bunch
+// boundaries.  Workgroup w owns the segments wg_seg[w] .. wg_seg[w+1]; the
+// segments are numbered in workgroup order, and seg_rank[s] is segment s's
+// place in particle order, where each bunch's segments are consecutive
+// (bunch_seg) and its partial sums are folded (seg_part is indexed by rank).
+// (Re)allocates everything sized by the number of segments.
+int layout_segments(cts_handle *h, int wg_target, int tile)
 {
     const int nb = h->nb;
     const int64_t n = h->ntot;
-    int64_t want = (n + 2047) / 2048;
-    if (want < 1) want = 1;
-    if (want > wg_target) want = wg_target;
-    int64_t chunk = (n + want - 1) / want;
-    chunk = (chunk + 63) / 64 * 64;
-    if (chunk < 64) chunk = 64;
-    const int nwg = (int)((n + chunk - 1) / chunk) > 0 ? (int)((n + chunk - 1) / chunk) : 1;
-    std::vector<int64_t> sb, se;
-    std::vector<int32_t> sbunch, wgseg(nwg + 1, 0), bseg(nb + 1, 0);
-    int b = 0;
-    for (int w = 0; w < nwg; w++) {
-        wgseg[w] = (int32_t)sb.size();
-        const int64_t lo = (int64_t)w * chunk, hi = std::min<int64_t>(n, lo + chunk);
-        int64_t p = lo;
-        while (p < hi) {
-            while (b < nb && h->offsets[b + 1] <= p) b++;
-            const int64_t e2 = std::min<int64_t>(hi, h->offsets[b + 1]);
-            sb.push_back(p);
-            se.push_back(e2);
-            sbunch.push_back(b);
-            p = e2;
+    int64_t piece;
+    int nwg;
+    if (tile > 0) {
+        piece = ((int64_t)tile + kBlock - 1) / kBlock * kBlock;
+        const int64_t ntile = std::max<int64_t>(1, (n + piece - 1) / piece);
+        nwg = (int)std::min<int64_t>(std::max(wg_target, 1), ntile);
+    } else {
+        int64_t want = (n + 2047) / 2048;
+        if (want < 1) want = 1;
+        if (want > wg_target) want = wg_target;
+        piece = (n + want - 1) / want;
+        piece = (piece + 63) / 64 * 64;
+        if (piece < 64) piece = 64;
+        nwg = (int)((n + piece - 1) / piece) > 0 ? (int)((n + piece - 1) / piece) : 1;
+    }
+    const int64_t npiece = std::max<int64_t>(1, (n + piece - 1) / piece);
+    // segments in particle order, with the piece each belongs to
+    std::vector<int64_t> pb, pe;
+    std::vector<int32_t> pbunch, ppiece;
+    {
+        int b = 0;
+        for (int64_t t = 0; t < npiece; t++) {
+            const int64_t lo = t * piece, hi = std::min<int64_t>(n, lo + piece);
+            int64_t p = lo;
+            while (p < hi) {
+                while (b < nb && h->offsets[b + 1] <= p) b++;
+                const int64_t e2 = std::min<int64_t>(hi, h->offsets[b + 1]);
+                pb.push_back(p);
+                pe.push_back(e2);
+                pbunch.push_back(b);
+                ppiece.push_back((int32_t)t);
+                p = e2;
+            }
         }
     }
-    wgseg[nwg] = (int32_t)sb.size();
-    const int nseg = (int)sb.size();
+    const int nseg = (int)pb.size();
+    std::vector<int32_t> bseg(nb + 1, 0);
     {
         int s = 0;
         for (int bb = 0; bb < nb; bb++) {
-            while (s < nseg && sbunch[s] < bb) s++;
+            while (s < nseg && pbunch[s] < bb) s++;
             bseg[bb] = s;
         }
         bseg[nb] = nseg;
     }
+    // workgroup order: piece t goes to workgroup t % nwg (chunked: t == w)
+    std::vector<int32_t> first_seg(npiece + 1, nseg);
+    for (int s = nseg - 1; s >= 0; s--) first_seg[ppiece[s]] = s;
+    for (int64_t t = npiece - 1; t >= 0; t--)
+        if (first_seg[t] == nseg) first_seg[t] = first_seg[t + 1];
+    std::vector<int64_t> sb, se;
+    std::vector<int32_t> sbunch, rank, wgseg(nwg + 1, 0);
+    sb.reserve(nseg); se.reserve(nseg); sbunch.reserve(nseg); rank.reserve(nseg);
+    for (int w = 0; w < nwg; w++) {
+        wgseg[w] = (int32_t)sb.size();
+        for (int64_t t = w; t < npiece; t += nwg)
+            for (int s = first_seg[t]; s < first_seg[t + 1]; s++) {
+                sb.push_back(pb[s]);
+                se.push_back(pe[s]);
+                sbunch.push_back(pbunch[s]);
+                rank.push_back(s);
+            }
+    }
+    wgseg[nwg] = (int32_t)sb.size();
+    bool identity = true;
+    for (int s = 0; s < nseg && identity; s++) identity = rank[s] == s;
     int max_chunks = 1;
     for (int bb = 0; bb < nb; bb++) max_chunks = std::max(max_chunks, (bseg[bb + 1] - bseg[bb] + 63) / 64);
     void *old[] = {h->d_seg_begin, h->d_seg_end, h->d_seg_bunch, h->d_wg_seg, h->d_bunch_seg, h->d_seg_part,
-                   h->d_chunk_part};
+                   h->d_chunk_part, h->d_seg_rank};
     for (void *p : old)
         if (p) (void)hipFree(p);
     h->d_seg_begin = h->d_seg_end = nullptr;
-    h->d_seg_bunch = h->d_wg_seg = h->d_bunch_seg = nullptr;
+    h->d_seg_bunch = h->d_wg_seg = h->d_bunch_seg = h->d_seg_rank = nullptr;
     h->d_seg_part = h->d_chunk_part = nullptr;
     int rc;
     if ((rc = dalloc(&h->d_seg_begin, nseg)) || (rc = dalloc(&h->d_seg_end, nseg)) ||
         (rc = dalloc(&h->d_seg_bunch, nseg)) || (rc = dalloc(&h->d_wg_seg, nwg + 1)) ||
         (rc = dalloc(&h->d_bunch_seg, nb + 1)) || (rc = dalloc(&h->d_seg_part, (size_t)nseg * kNPart)) ||
-        (rc = dalloc(&h->d_chunk_part, (size_t)nb * max_chunks * kNPart)))
+        (rc = dalloc(&h->d_chunk_part, (size_t)nb * max_chunks * kNPart)) ||
+        (!identity && (rc = dalloc(&h->d_seg_rank, nseg))))
         return rc;
+    if (!identity)
+        HIPCHK(hipMemcpy(h->d_seg_rank, rank.data(), rank.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->d_seg_begin, sb.data(), sb.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->d_seg_end, se.data(), se.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->d_seg_bunch, sbunch.data(), sbunch.size() * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -739,6 +786,7 @@ int layout_segments(cts_handle *h, int wg_target)
     h->nwg = nwg;
     h->nseg = nseg;
     h->max_chunks = max_chunks;
+    h->tile = tile;
     return CTS_OK;
 }
 
@@ -1065,7 +1113,7 @@ int cts_set_bunches(cts_handle *h, int32_t nb, const int64_t *counts, const doub
         (rc = dalloc(&h->d_ion_e, (size_t)nb * CTS_MAX_SPECIES * 5)) || (rc = dalloc(&h->d_ion_bview, (size_t)nb * 4)) ||
         (rc = dalloc(&h->d_chunk_cnt, nb)))
         return rc;
-    if ((rc = layout_segments(h, h->wg_target))) return rc;
+    if ((rc = layout_segments(h, h->wg_target, h->tile))) return rc;
     HIPCHK(hipMemcpy(h->d_bunch_off, h->offsets.data(), (nb + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->d_bunch_count, h->counts.data(), nb * sizeof(int64_t), hipMemcpyHostToDevice));
     h->kick_pending = false;
@@ -1100,8 +1148,9 @@ int cts_set_bunches(cts_handle *h, int32_t nb, const int64_t *counts, const doub
     }
     if (keep_bbi.on) {
         cts_bbimp_config c = keep_bbi.cfg;
-        const double **dst[6] = {&c.zz_re, &c.zz_im, &c.zqx_re, &c.zqx_im, &c.zqy_re, &c.zqy_im};
-        for (int i = 0; i < 6; i++) *dst[i] = keep_bbi.arr[i].empty() ? nullptr : keep_bbi.arr[i].data();
+        const double **dst[10] = {&c.zz_re,  &c.zz_im,  &c.zqx_re, &c.zqx_im, &c.zqy_re,
+                                  &c.zqy_im, &c.zdx_re, &c.zdx_im, &c.zdy_re, &c.zdy_im};
+        for (int i = 0; i < 10; i++) *dst[i] = keep_bbi.arr[i].empty() ? nullptr : keep_bbi.arr[i].data();
         if ((rc = cts_set_bbimp(h, &c))) return rc;
     }
     if (keep_bl.on && (rc = cts_set_beam_loading(h, &keep_bl.cfg))) return rc;
@@ -1525,10 +1574,31 @@ int cts_run_turns(cts_handle *h, int32_t first, int32_t nturns, int32_t use_grap
     return CTS_OK;
 }
 
-// Workgroup count of the streaming pass, picked by timing: the candidates
-// are multiples of the number of workgroups the device holds at once (so
-// the last round is not a partial one) and the current setting; each is
-// timed on a copy of the particles with the turn's own kernel variant.
+int cts_get_layout(cts_handle *h, int32_t *nwg, int32_t *tile)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (nwg) *nwg = h->nwg;
+    if (tile) *tile = h->tile;
+    return CTS_OK;
+}
+
+int cts_set_layout(cts_handle *h, int32_t nwg, int32_t tile)
+{
+    int rc = check_ready(h);
+    if (rc) return rc;
+    if (nwg < 1 || tile < 0) return fail(CTS_ERR_ARG, "cts_set_layout: nwg >= 1 and tile >= 0");
+    if ((rc = flush_kick(h))) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    graph_reset(h);
+    h->wg_target = nwg;
+    return layout_segments(h, nwg, tile);
+}
+
+// Layout of the streaming pass (workgroup count and tile), picked by
+// timing: contiguous chunks for multiples of the CU count, round-robin
+// tiles over 2..4 workgroups per CU, and the current setting; each is timed
+// on a copy of the particles with the turn's own kernel variant.
 int cts_autotune(cts_handle *h, int32_t *nwg_out)
 {
     int rc = check_ready(h);
@@ -1571,16 +1641,20 @@ int cts_autotune(cts_handle *h, int32_t *nwg_out)
         release();
         return rc;
     }
-    std::vector<int> cand;
-    for (int m = 1; m <= 8; m++) cand.push_back(cus * m);
-    cand.push_back(h->wg_target);
+    // (workgroups, tile): contiguous chunks for 1..8 workgroups per CU, then
+    // round-robin tiles of 2048..16384 particles over 2..4 per CU
+    std::vector<std::pair<int, int>> cand;
+    for (int m = 1; m <= 8; m++) cand.push_back({cus * m, 0});
+    for (int m = 2; m <= 4; m++)
+        for (int t = 2048; t <= 16384; t *= 2) cand.push_back({cus * m, t});
+    cand.push_back({h->wg_target, h->tile});
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    int best = h->wg_target;
+    std::pair<int, int> best = {h->wg_target, h->tile};
     float best_ms = 1e30f;
-    for (int c : cand) {
-        if ((rc = layout_segments(h, c))) break;
+    for (const auto &c : cand) {
+        if ((rc = layout_segments(h, c.first, c.second))) break;
         PassParams p = base_pass(h, ops, 0);
         p.x = cx[0]; p.px = cx[1]; p.y = cx[2]; p.py = cx[3]; p.z = cx[4]; p.dp = cx[5];
         p.flag = cf;
@@ -1605,8 +1679,8 @@ int cts_autotune(cts_handle *h, int32_t *nwg_out)
     (void)hipEventDestroy(e1);
     release();
     if (rc) return rc;
-    h->wg_target = best;
-    if ((rc = layout_segments(h, best))) return rc;
+    h->wg_target = best.first;
+    if ((rc = layout_segments(h, best.first, best.second))) return rc;
     if (nwg_out) *nwg_out = h->nwg;
     return CTS_OK;
 }
@@ -1762,33 +1836,35 @@ int cts_set_bbimp(cts_handle *h, const cts_bbimp_config *cfg)
         return CTS_OK;  // doTracking = 0 (src/MPBunch.cpp:2152-2157)
     if (cfg->time_domain != 0)
         return fail(CTS_ERR_ARG, "cts_set_bbimp: only the frequency-domain method (time_domain 0) is built");
-    if (cfg->flags[1] || cfg->flags[2])
-        return fail(CTS_ERR_ARG, "cts_set_bbimp: the dipolar planes (flags[1], flags[2]) are not built");
     if (h->secs.size() != 1) return fail(CTS_ERR_STATE, "cts_set_bbimp: the fused turn (one section) only");
     std::string err;
     if ((rc = bbimp_table(*cfg, h->ring, h->bbi, err))) return fail(rc, err);
     const int n = h->bbi.n, nb = h->nb;
-    if ((rc = dalloc(&h->d_bbi_hist, (size_t)nb * n)) || (rc = dalloc(&h->d_bbi_prof, (size_t)nb * n)) ||
-        (rc = dalloc(&h->d_bbi_wake, 3 * (size_t)nb * n)) || (rc = dalloc(&h->d_bbi_g, 3 * (size_t)n)) ||
-        (rc = dalloc(&h->d_bbi_scale, (size_t)nb))) {
+    const bool dip = h->bbi.planes[1] || h->bbi.planes[2];
+    if ((rc = dalloc(&h->d_bbi_hist, (size_t)nb * n)) || (rc = dalloc(&h->d_bbi_prof, 3 * (size_t)nb * n)) ||
+        (rc = dalloc(&h->d_bbi_wake, 5 * (size_t)nb * n)) || (rc = dalloc(&h->d_bbi_g, 5 * (size_t)n)) ||
+        (rc = dalloc(&h->d_bbi_scale, (size_t)nb)) || (dip && (rc = dalloc(&h->d_bbi_dsum, 4 * (size_t)nb * n)))) {
         free_bbi_state(h);
         return rc;
     }
     std::vector<double> sc(nb);
     for (int b = 0; b < nb; b++) sc[b] = h->macro_charge[b] * kEcharge / h->bbi.dz * kClight;
-    HIPCHK(hipMemcpy(h->d_bbi_g, h->bbi.g.data(), 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_bbi_g, h->bbi.g.data(), 5 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->d_bbi_scale, sc.data(), nb * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(h->d_bbi_hist, 0, (size_t)nb * n * sizeof(uint32_t)));
-    HIPCHK(hipMemset(h->d_bbi_wake, 0, 3 * (size_t)nb * n * sizeof(double)));
-    HIPCHK(hipMemset(h->d_bbi_prof, 0, (size_t)nb * n * sizeof(double)));
+    if (dip) HIPCHK(hipMemset(h->d_bbi_dsum, 0, 4 * (size_t)nb * n * sizeof(unsigned long long)));
+    HIPCHK(hipMemset(h->d_bbi_wake, 0, 5 * (size_t)nb * n * sizeof(double)));
+    HIPCHK(hipMemset(h->d_bbi_prof, 0, 3 * (size_t)nb * n * sizeof(double)));
     h->bbi_on = true;
     cts_handle::KeepBbi k;
     k.on = true;
     k.cfg = *cfg;
-    const double *src[6] = {cfg->zz_re, cfg->zz_im, cfg->zqx_re, cfg->zqx_im, cfg->zqy_re, cfg->zqy_im};
-    for (int i = 0; i < 6; i++)
+    const double *src[10] = {cfg->zz_re,  cfg->zz_im,  cfg->zqx_re, cfg->zqx_im, cfg->zqy_re,
+                             cfg->zqy_im, cfg->zdx_re, cfg->zdx_im, cfg->zdy_re, cfg->zdy_im};
+    for (int i = 0; i < 10; i++)
         if (src[i]) k.arr[i].assign(src[i], src[i] + cfg->nimp);
     k.cfg.zz_re = k.cfg.zz_im = k.cfg.zqx_re = k.cfg.zqx_im = k.cfg.zqy_re = k.cfg.zqy_im = nullptr;
+    k.cfg.zdx_re = k.cfg.zdx_im = k.cfg.zdy_re = k.cfg.zdy_im = nullptr;
     h->keep_bbi = std::move(k);
     return CTS_OK;
 }
@@ -1801,8 +1877,8 @@ int cts_get_bbimp_wake(cts_handle *h, int32_t *n, double *wake, double *prof)
     if (n) *n = h->bbi.n;
     HIPCHK(hipStreamSynchronize(h->stream));
     const size_t sz = (size_t)h->nb * h->bbi.n * sizeof(double);
-    if (wake) HIPCHK(hipMemcpy(wake, h->d_bbi_wake, 3 * sz, hipMemcpyDeviceToHost));
-    if (prof) HIPCHK(hipMemcpy(prof, h->d_bbi_prof, sz, hipMemcpyDeviceToHost));
+    if (wake) HIPCHK(hipMemcpy(wake, h->d_bbi_wake, 5 * sz, hipMemcpyDeviceToHost));
+    if (prof) HIPCHK(hipMemcpy(prof, h->d_bbi_prof, 3 * sz, hipMemcpyDeviceToHost));
     return CTS_OK;
 }
 

@@ -252,6 +252,41 @@ __device__ __forceinline__ double wave_sum_dpp(double v)
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
     return __hiloint2double(hi, lo);
 }
+// Four wave sums at once, each step applied to the four values before the
+// next, so the DPP moves of one value fill the wait states of another; the
+// sums are exact in lane 63 only (rows the broadcast steps skip keep
+// unspecified values: the moves carry no old value).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64_any(double v)
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, ROWS, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void dpp_step4(double v[4])
+{
+    double t[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) t[j] = dpp_f64_any<CTRL, ROWS>(v[j]);
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] += t[j];
+}
+__device__ __forceinline__ void wave_sum4_dpp(double v[4])
+{
+    dpp_step4<0xb1, 0xf>(v);   // quad_perm [1, 0, 3, 2]
+    dpp_step4<0x4e, 0xf>(v);   // quad_perm [2, 3, 0, 1]
+    dpp_step4<0x141, 0xf>(v);  // row_half_mirror
+    dpp_step4<0x140, 0xf>(v);  // row_mirror
+    dpp_step4<0x142, 0xa>(v);  // row_bcast15 into rows 1, 3
+    dpp_step4<0x143, 0xc>(v);  // row_bcast31 into rows 2, 3
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int lo = __builtin_amdgcn_readlane(__double2loint(v[j]), 63);
+        const int hi = __builtin_amdgcn_readlane(__double2hiint(v[j]), 63);
+        v[j] = __hiloint2double(hi, lo);
+    }
+}
 __device__ __forceinline__ double wave_min(double v)
 {
 #pragma unroll

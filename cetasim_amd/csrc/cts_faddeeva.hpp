@@ -425,6 +425,19 @@ __device__ __forceinline__ void w_weideman2(const double zr[2], const double zi[
     }
 }
 
+// w(z) outside the kick region (x >= 100 or y >= 1000, both >= 0), as
+// faddeeva_w_q1 evaluates it there: erfcx on the imaginary axis, otherwise
+// the continued fraction (at most 4 levels for |z|^2 >= 1e4) or the
+// asymptotic form; exp(-z^2) has underflowed.  Few registers, so the kick
+// loops keep their occupancy.
+__device__ __forceinline__ void w_far(double x, double y, double &wr, double &wi)
+{
+    if (x == 0.0) { wr = cts_erfcx(y); wi = 0.0; return; }
+    if (std::isinf(x) || std::isinf(y)) { wr = 0.0; wi = 0.0; return; }
+    if (x >= 1e7 || y >= 1e7) { fad_asymptotic(x, y, wr, wi); return; }
+    fad_cf(x, y, fad_cf_depth(x * x + y * y), wr, wi);
+}
+
 // ion_field with the beam's constants from be_plan.
 __device__ __forceinline__ void be_field(double x, double y, const BeField &F, double &fx, double &fy)
 {
@@ -445,12 +458,12 @@ __device__ __forceinline__ void be_field(double x, double y, const BeField &F, d
     zi[0] = ay * F.isig;
     zr[1] = F.r * zr[0];
     zi[1] = zi[0] * F.ir;
-    if (w_kick_region(zr[0], zi[0]) && w_kick_region(zr[1], zi[1])) {
-        w_weideman2(zr, zi, wr, wi);
-    } else {
-        faddeeva_w_q1(zr[0], zi[0], wr[0], wi[0]);
-        faddeeva_w_q1(zr[1], zi[1], wr[1], wi[1]);
-    }
+    // Weideman's approximation for both points; a point outside the kick
+    // region (rare: far from the beam) takes the far-field form instead
+    w_weideman2(zr, zi, wr, wi);
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+        if (!w_kick_region(zr[q], zi[q])) w_far(zr[q], zi[q], wr[q], wi[q]);
     const double w3 = -exp(-(ax * ax) * F.hx - (ay * ay) * F.hy);
     const double w0r = F.coeff * (wr[0] + w3 * wr[1]);
     const double w0i = F.coeff * (wi[0] + w3 * wi[1]);

@@ -76,6 +76,7 @@ struct PassParams {
     const int32_t *seg_bunch, *wg_seg;
     const int64_t *bunch_off;
     double *seg_part;
+    const int32_t *seg_rank;  // segment's place in particle order (null: identity)
     uint32_t *bunch_hist;
     const double *shift;
     const double *kick_prev, *edges_prev, *edges_now;
@@ -247,17 +248,18 @@ int lrwake_table(const cts_lrwake_config &c, double t0, int harmonic, LrwTable &
 // Z and the smoothing filter, set up on the host.
 constexpr int kBbiTaps = 51;      // the Gaussian filter's window
 constexpr int kBbiMaxBins = 8191;  // 2 nimp - 1: the per-segment histogram in LDS (32 KB)
+constexpr int kBbiDipMaxBins = 1535;  // with the dipolar planes: counts and x, y limb sums in LDS (55 KB)
 struct BbiTable {
     int n = 0, is = 0, ie = 0;   // bins; smoothed range [is, ie)
-    int planes[3] = {0, 0, 0};   // longitudinal, quadrupolar x, y
+    int planes[5] = {0, 0, 0, 0, 0};  // the reference's flags: Z, Dx, Dy, Qx, Qy
     double zmin = 0, dz = 0, bx = 1, by = 1;
-    std::vector<double> g;       // [3][n] kernels: V = -(profile (*) g_z), V_Q = +(profile (*) g_Q)
+    std::vector<double> g;       // [5][n] kernels: V = -(profile (*) g_z), transverse V = +(profile (*) g)
     double taps[kBbiTaps];       // normalised Gaussian weights
 };
 int bbimp_table(const cts_bbimp_config &c, const cts_ring &ring, BbiTable &t, std::string &err);
 struct BbiParams {
     int n, is, ie, nb;
-    int planes[3];
+    int planes[5];
     double zmin, dz, energy, beta2, bx, by;
     const int64_t *seg_begin, *seg_end;
     const int32_t *seg_bunch, *wg_seg;
@@ -265,9 +267,10 @@ struct BbiParams {
     double *px, *py, *dp;
     const uint8_t *flag;
     uint32_t *hist;        // [nb][n], zero between turns
-    double *prof;          // [nb][n]
-    double *wake;          // [3][nb][n]
-    const double *g;       // [3][n]
+    unsigned long long *dsum;  // [nb][n][4] x, y limb sums (dipolar planes; else null), zero between turns
+    double *prof;          // [3][nb][n]: charge, x-weighted, y-weighted
+    double *wake;          // [5][nb][n], flag order
+    const double *g;       // [5][n]
     const double *scale;   // [nb] macro charge e c / dz (0: the bunch is skipped)
     double taps[kBbiTaps];
 };
@@ -336,9 +339,18 @@ constexpr int kIonTurnMaxBlocks = 4096;  // grid bound of an ion segment or comp
 #endif
 constexpr int kIonReplicas = CTS_ION_REPLICAS;  // copies of the passage sums
 #ifndef CTS_ION_SEGMENT
-#define CTS_ION_SEGMENT 16
+#define CTS_ION_SEGMENT 32
 #endif
 constexpr int kIonSegment = CTS_ION_SEGMENT;  // bunch passages per ion segment
+#ifndef CTS_ION_COHORT
+#define CTS_ION_COHORT 4
+#endif
+#ifndef CTS_ION_WAVES_PER_EU
+#define CTS_ION_WAVES_PER_EU 4
+#endif
+constexpr int kIonCohort = CTS_ION_COHORT;      // waves of an ion-turn block, walking the passages together
+constexpr int kIonBlock = 64 * kIonCohort;
+constexpr int kIonWavesPerEU = CTS_ION_WAVES_PER_EU;  // occupancy the ion-turn kernel is compiled for
 hipError_t launch_qe_debug(const float2 *tab, uint64_t seed, uint32_t turn, uint64_t base, int64_t n, double *out,
                            hipStream_t st);
 hipError_t upload_faddeeva_tables(const FadTables &t);

@@ -90,42 +90,55 @@ __device__ __forceinline__ bool ion_wave_slot(int w, int w_old, int64_t len, int
     return t < n_new && slot < cap;
 }
 
-// grid = (blocks, nspec); the waves of the segment, four per block at a
-// time, striding over the grid.  Wave w < w_old holds the compacted ions of
-// slots [64 w, 64 w + 64), from the segment's first passage; the other
-// waves hold the new ions of passage j0 (slot len + (j0 - j_begin) n_new + t).
-// The lanes of a wave visit the passages in step; a lane drops out when its
-// ion is lost.  The segment's bunch data sit in LDS and the passage sums
-// are gathered in LDS (64-bit integer atomics), so the passage loop makes
-// no global memory access; each block adds its sums to one copy of the
-// global sums at the end.  SUR: the bunch's field at the ion from the
-// surrogate inside its domain, the exact field elsewhere.
+// grid = (blocks, nspec); a block takes a cohort of kIonCohort consecutive
+// waves of the segment at a time (striding over the grid).  Wave w <
+// w_old holds the compacted ions of slots [64 w, 64 w + 64), from the
+// segment's first passage; the other waves hold the new ions of passage j0
+// (slot len + (j0 - j_begin) n_new + t), which join at their passage.  The
+// cohort walks the passages in step; after each passage, when the ions
+// still alive in the waves already walking fit into fewer waves, they are
+// packed (in wave and lane order) into the first of them through LDS, so
+// the lanes that evaluate the field stay mostly occupied as ions are lost.
+// Every ion sees the same operations in the same order as passage by
+// passage.  The cloud moments of each passage are summed per wave with DPP
+// moves and ballots into LDS, added over the cohort's waves in wave order
+// and turned into fixed-point limbs once per cohort (integer atomics, so
+// the sums do not depend on the order in which blocks finish); each block
+// adds its sums to one copy of the global sums at the end.  SUR: the
+// bunch's field at the ion from the surrogate inside its domain, the exact
+// field elsewhere.
 template <bool SUR>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kIonBlock, kIonWavesPerEU)
 k_ion_turn(IonTurnParams P)
 {
     __shared__ __attribute__((aligned(16))) MlpLds L;
     __shared__ double sbv[kIonSegment][4], scI[kIonSegment], sdt[kIonSegment];
     __shared__ BeField sF[kIonSegment];
     __shared__ unsigned long long sacc[kIonSegment * kIonAcc * 3];
+    __shared__ double wpart[kIonCohort][kIonSegment][kIonAcc];  // per wave, per passage
+    __shared__ double rp[4][kIonBlock];                          // packing: x, y, vx, vy
+    __shared__ int32_t rs[kIonBlock], ro[kIonBlock];             // packing: slot, first wave
+    __shared__ int32_t wn[kIonCohort];                           // alive ions of each wave (-1: not walking)
+    __shared__ uint32_t wkept[kIonCohort];                       // survivors of each first wave
+    __shared__ int32_t s_qmin;
     const int p = blockIdx.y;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t len = P.len[p];
     // waves: the ions alive at the start of the segment (compacted), then
     // the new ions of every passage of the segment
     const int w_old = (int)((len + 63) / 64);
     const int nseg = P.j_end - P.j_begin;
     const int total = w_old + nseg * P.w_new;
-    if (blockIdx.x * 4 >= total) return;  // whole block idle (uniform)
-    if (SUR) load_mlp_lds(L, P.sur_params, threadIdx.x, 256);
-    for (int i = threadIdx.x; i < nseg * 4; i += 256) sbv[i >> 2][i & 3] = P.bview[4 * P.j_begin + i];
-    for (int i = threadIdx.x; i < nseg; i += 256) {
+    const int ncoh = (total + kIonCohort - 1) / kIonCohort;
+    if ((int)blockIdx.x >= ncoh) return;  // whole block idle (uniform)
+    if (SUR) load_mlp_lds(L, P.sur_params, tid, kIonBlock);
+    for (int i = tid; i < nseg * 4; i += kIonBlock) sbv[i >> 2][i & 3] = P.bview[4 * P.j_begin + i];
+    for (int i = tid; i < nseg; i += kIonBlock) {
         scI[i] = P.coeffI[(P.j_begin + i) * CTS_MAX_SPECIES + p];
         sdt[i] = P.dt[P.j_begin + i];
         sF[i] = be_plan(P.bview[4 * (P.j_begin + i) + 2], P.bview[4 * (P.j_begin + i) + 3]);
     }
-    for (int i = threadIdx.x; i < nseg * kIonAcc * 3; i += 256) sacc[i] = 0ull;
-    __syncthreads();
-    const int lane = threadIdx.x & 63;
+    for (int i = tid; i < nseg * kIonAcc * 3; i += kIonBlock) sacc[i] = 0ull;
     const uint32_t turn = P.turn_dev ? *P.turn_dev + P.turn : P.turn;
     double *X = P.x[p], *Y = P.y[p], *VX = P.vx[p], *VY = P.vy[p];
     uint8_t *A = P.alive[p];
@@ -133,12 +146,19 @@ k_ion_turn(IonTurnParams P)
     const double lbx = P.loss_boundary * P.size_max[0], lby = P.loss_boundary * P.size_max[1];
     const double bx = P.ap_x > lbx ? lbx : P.ap_x;
     const double by = P.ap_y > lby ? lby : P.ap_y;
-    for (int w = blockIdx.x * 4 + (threadIdx.x >> 6); w < total; w += gridDim.x * 4) {
-        // slot: this lane's ion in the buffer; mine: the lane holds one;
-        // q0: the wave's first passage, counted from j_begin
-        int q0, t;
-        int64_t slot;
-        const bool mine = ion_wave_slot(w, w_old, len, P.n_new, P.w_new, P.cap, lane, slot, q0, t);
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int c = blockIdx.x; c < ncoh; c += gridDim.x) {
+        const int w = c * kIonCohort + wv;
+        // slot: this lane's ion in the buffer (-1: none); q0: the wave's
+        // first passage, counted from j_begin (nseg: no ions)
+        int q0 = nseg, t = 0;
+        int64_t slot = -1;
+        bool mine = false;
+        if (w < total) {
+            int qq;
+            mine = ion_wave_slot(w, w_old, len, P.n_new, P.w_new, P.cap, lane, slot, qq, t);
+            q0 = qq;
+        }
         double x = 0.0, y = 0.0, vx = 0.0, vy = 0.0;
         if (mine) {
             if (w < w_old) {
@@ -147,59 +167,109 @@ k_ion_turn(IonTurnParams P)
                 ion_generate(P.seed, turn, (uint32_t)(P.j_begin + q0), p, (uint32_t)t, sbv[q0][0], sbv[q0][1],
                              sbv[q0][2], sbv[q0][3], x, y);
             }
+            A[slot] = 0;  // survivors set it again at the end
         }
-        bool alive = mine;
-        for (int q = q0; q < nseg; q++) {
-            if (!__any(alive)) break;
-            const double xm = sbv[q][0], ym = sbv[q][1];
-            const double dx = x - xm, dy = y - ym;
-            double fx = 0.0, fy = 0.0;
-            if (SUR) {
-                const bool ok = sur_field(L, P.sur_norm, alive, dx, dy, sbv[q][2], sbv[q][3], fx, fy);
-                if (alive && !ok) be_field(dx, dy, sF[q], fx, fy);
-            } else if (alive) {
-                be_field(dx, dy, sF[q], fx, fy);
+        int32_t myslot = mine ? (int32_t)slot : -1;
+        int32_t first = wv;  // the wave the ion started in
+        bool alive = false;  // walking and not lost
+        if (tid == 0) s_qmin = nseg;
+        for (int i = tid; i < kIonCohort * kIonSegment * kIonAcc; i += kIonBlock) (&wpart[0][0][0])[i] = 0.0;
+        if (tid < kIonCohort) wkept[tid] = 0u;
+        __syncthreads();
+        const bool any_mine = __any(mine);
+        if (lane == 0 && any_mine) atomicMin(&s_qmin, q0);
+        __syncthreads();
+        for (int q = s_qmin; q < nseg; q++) {
+            // a wave's new ions join at their passage (wave-uniform)
+            if (q == q0) alive = myslot >= 0;
+            const bool walking = q0 <= q;
+            if (__any(alive)) {
+                const double xm = sbv[q][0], ym = sbv[q][1];
+                const double dx = x - xm, dy = y - ym;
+                double fx = 0.0, fy = 0.0;
+                if (SUR) {
+                    const bool ok = sur_field(L, P.sur_norm, alive, dx, dy, sbv[q][2], sbv[q][3], fx, fy);
+                    if (alive && !ok) be_field(dx, dy, sF[q], fx, fy);
+                } else if (alive) {
+                    be_field(dx, dy, sF[q], fx, fy);
+                }
+                bool keep = false;
+                if (alive) {
+                    const double cI = scI[q];
+                    vx += cI * fx;
+                    vy += cI * fy;
+                    x += vx * sdt[q];
+                    y += vy * sdt[q];
+                    keep = !(fabs(x) > bx || fabs(y) > by);
+                }
+                // the cloud as the bunch finds it (before the kick), and the
+                // ions kept: counts by ballot, sums by DPP reductions
+                const double c0 = (double)__popcll(__ballot(alive));
+                double cs[4] = {alive ? dx : 0.0, alive ? dy : 0.0, alive ? dx * dx : 0.0, alive ? dy * dy : 0.0};
+                wave_sum4_dpp(cs);
+                const double c1 = cs[0], c2 = cs[1], c3 = cs[2], c4 = cs[3];
+                const double c5 = (double)__popcll(__ballot(keep));
+                if (lane == 0) {
+                    double *o = wpart[wv][q];
+                    o[0] = c0; o[1] = c1; o[2] = c2; o[3] = c3; o[4] = c4; o[5] = c5;
+                }
+                alive = keep;
             }
-            bool keep = false;
-            if (alive) {
-                const double cI = scI[q];
-                vx += cI * fx;
-                vy += cI * fy;
-                x += vx * sdt[q];
-                y += vy * sdt[q];
-                keep = !(fabs(x) > bx || fabs(y) > by);
+            // pack the walking waves' ions into as few of them as hold them
+            const uint64_t am = __ballot(alive);
+            if (lane == 0) wn[wv] = walking ? __popcll(am) : -1;
+            __syncthreads();
+            int nalive = 0, busy = 0, base = 0, rank = 0;
+            for (int k = 0; k < kIonCohort; k++) {
+                const int n = wn[k];
+                if (n < 0) continue;
+                if (k < wv) { base += n; rank++; }
+                nalive += n;
+                busy += n > 0;
             }
-            // the cloud as the bunch finds it (before the kick), and the ions
-            // kept: counts by ballot, sums by DPP reductions
-            double c[kIonAcc];
-            c[0] = (double)__popcll(__ballot(alive));
-            c[1] = wave_sum_dpp(alive ? dx : 0.0);
-            c[2] = wave_sum_dpp(alive ? dy : 0.0);
-            c[3] = wave_sum_dpp(alive ? dx * dx : 0.0);
-            c[4] = wave_sum_dpp(alive ? dy * dy : 0.0);
-            c[5] = (double)__popcll(__ballot(keep));
-            // lane 3 v + k adds limb k of value v
-            if (lane < 3 * kIonAcc) {
-                const int v = lane / 3, kk = lane - 3 * v;
-                double cv = c[0];
-#pragma unroll
-                for (int u = 1; u < kIonAcc; u++) cv = v == u ? c[u] : cv;
-                const long long l = fx_limb(cv, kk);
-                if (l != 0) atomicAdd(&sacc[(q * kIonAcc + v) * 3 + kk], (unsigned long long)l);
+            if ((nalive + 63) / 64 < busy) {  // block-uniform
+                if (walking && alive) {
+                    const int d = base + __popcll(am & below);
+                    rp[0][d] = x; rp[1][d] = y; rp[2][d] = vx; rp[3][d] = vy;
+                    rs[d] = myslot;
+                    ro[d] = first;
+                }
+                __syncthreads();
+                if (walking) {
+                    const int d = rank * 64 + lane;
+                    alive = d < nalive;
+                    if (alive) {
+                        x = rp[0][d]; y = rp[1][d]; vx = rp[2][d]; vy = rp[3][d];
+                        myslot = rs[d];
+                        first = ro[d];
+                    } else {
+                        myslot = -1;
+                    }
+                }
             }
-            alive = keep;
+            __syncthreads();
         }
-        if (mine) {
-            X[slot] = x; Y[slot] = y; VX[slot] = vx; VY[slot] = vy;
-            A[slot] = alive ? 1 : 0;
+        if (alive) {
+            X[myslot] = x; Y[myslot] = y; VX[myslot] = vx; VY[myslot] = vy;
+            A[myslot] = 1;
+            atomicAdd(&wkept[first], 1u);
         }
-        const uint32_t kept = (uint32_t)__popcll(__ballot(mine && alive));
-        if (lane == 0) P.wcnt[p * P.wcnt_stride + w] = kept;
+        __syncthreads();
+        if (tid < kIonCohort && c * kIonCohort + tid < total) P.wcnt[p * P.wcnt_stride + c * kIonCohort + tid] = wkept[tid];
+        // the cohort's passage sums: over its waves in wave order, then to
+        // fixed point (limb k of value v of passage q per thread)
+        for (int i = tid; i < nseg * kIonAcc * 3; i += kIonBlock) {
+            const int q = i / (kIonAcc * 3), v = (i / 3) % kIonAcc, kk = i % 3;
+            double sv = 0.0;
+            for (int k = 0; k < kIonCohort; k++) sv += wpart[k][q][v];
+            const long long l = fx_limb(sv, kk);
+            if (l != 0) atomicAdd(&sacc[(q * kIonAcc + v) * 3 + kk], (unsigned long long)l);
+        }
+        __syncthreads();
     }
-    __syncthreads();
     unsigned long long *acc = P.acc + (int64_t)(blockIdx.x % kIonReplicas) * P.acc_stride +
                               ((int64_t)p * P.nb + P.j_begin) * kIonAcc * 3;
-    for (int i = threadIdx.x; i < nseg * kIonAcc * 3; i += 256) {
+    for (int i = tid; i < nseg * kIonAcc * 3; i += kIonBlock) {
         const unsigned long long v = sacc[i];
         if (v != 0ull) atomicAdd(acc + i, v);
     }
@@ -386,14 +456,14 @@ hipError_t upload_faddeeva_tables_ions(const FadTables &t)
 // actual count from the device and strides over it.
 hipError_t launch_ion_turn(const IonTurnParams &p, int nspec, int64_t wave_bound, hipStream_t st)
 {
-    int64_t blocks = (wave_bound + 3) / 4;
+    int64_t blocks = (wave_bound + kIonCohort - 1) / kIonCohort;
     if (blocks > kIonTurnMaxBlocks) blocks = kIonTurnMaxBlocks;
     if (blocks < 1) blocks = 1;
     const dim3 grid((unsigned)blocks, (unsigned)nspec);
     if (p.sur_params)
-        hipLaunchKernelGGL(k_ion_turn<true>, grid, dim3(256), 0, st, p);
+        hipLaunchKernelGGL(k_ion_turn<true>, grid, dim3(kIonBlock), 0, st, p);
     else
-        hipLaunchKernelGGL(k_ion_turn<false>, grid, dim3(256), 0, st, p);
+        hipLaunchKernelGGL(k_ion_turn<false>, grid, dim3(kIonBlock), 0, st, p);
     return hipGetLastError();
 }
 

@@ -469,7 +469,7 @@ k_pass(PassParams P)
                     else v = v + red[w][tid];
                 }
             }
-            P.seg_part[(int64_t)tid * P.nseg + s] = v;
+            P.seg_part[(int64_t)tid * P.nseg + (P.seg_rank ? P.seg_rank[s] : s)] = v;
         }
         if (use_hist) {
             for (int j = tid; j < P.nbins; j += kBlock) {

@@ -226,6 +226,9 @@ __device__ __forceinline__ bool sur_field(const MlpLds &L, const SurNorm &nm, bo
     const double au = fabs(u), av = fabs(v);
     const bool dom = active && au <= kSurUMax && av <= kSurUMax && r >= kSurRMin && r <= 1.0 &&
                      au + av >= kSurZero;
+    // a wave with no lane inside the trained domain skips the network (the
+    // callers run this in wave-uniform control flow, as the MFMA needs)
+    if (!__any(dom)) return false;
     float a[3];
     sur_features(dom ? au : 1.0, dom ? av : 1.0, dom ? r : 0.5, a);
     float o0, o1;

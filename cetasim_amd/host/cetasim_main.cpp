@@ -191,6 +191,10 @@ int dump(const std::string &dir, ReadInputSettings &in, LatticeInterActionPoint 
         arr("bbi_qx_im", c.bbi_qx_im.data(), c.bbi_qx_im.size());
         arr("bbi_qy_re", c.bbi_qy_re.data(), c.bbi_qy_re.size());
         arr("bbi_qy_im", c.bbi_qy_im.data(), c.bbi_qy_im.size());
+        arr("bbi_dx_re", c.bbi_dx_re.data(), c.bbi_dx_re.size());
+        arr("bbi_dx_im", c.bbi_dx_im.data(), c.bbi_dx_im.size());
+        arr("bbi_dy_re", c.bbi_dy_re.data(), c.bbi_dy_re.size());
+        arr("bbi_dy_im", c.bbi_dy_im.data(), c.bbi_dy_im.size());
         arr("bbi_beta", c.bbi_beta, 2);
         const cts_beam_loading_config &b = c.blc;
         std::fprintf(f, "\"bl\": %d, \"bl_nres\": %d, \"bl_nbins\": %d, \"bl_nturns\": %d,\n", c.bl, b.nres,

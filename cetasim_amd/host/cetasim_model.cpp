@@ -1123,6 +1123,10 @@ int MPBeam::RunSettings(const LatticeInterActionPoint &lat, const ReadInputSetti
             c.bbi_qx_im.push_back(-col[4]);
             c.bbi_qy_re.push_back(-col[10]);
             c.bbi_qy_im.push_back(-col[5]);
+            c.bbi_dx_re.push_back(-col[7]);
+            c.bbi_dx_im.push_back(-col[2]);
+            c.bbi_dy_re.push_back(-col[8]);
+            c.bbi_dy_im.push_back(-col[3]);
         }
         c.bbi = true;
         c.bbi_fmax = fmax;
@@ -1269,6 +1273,10 @@ int MPBeam::Run(Train &, LatticeInterActionPoint &lat, ReadInputSettings &in)
         bc.zqx_im = c.bbi_qx_im.data();
         bc.zqy_re = c.bbi_qy_re.data();
         bc.zqy_im = c.bbi_qy_im.data();
+        bc.zdx_re = c.bbi_dx_re.data();
+        bc.zdx_im = c.bbi_dx_im.data();
+        bc.zdy_re = c.bbi_dy_re.data();
+        bc.zdy_im = c.bbi_dy_im.data();
         bc.beta_x = c.bbi_beta[0];
         bc.beta_y = c.bbi_beta[1];
         if (check(cts_set_bbimp(gpu, &bc), "broadband impedance")) return 1;

@@ -189,7 +189,8 @@ struct RunCalls {
     bool bbi = false;  // broadband impedance (BoardBandImp::ReadInImp's table)
     int bbi_flags[5] = {0, 0, 0, 0, 0}, bbi_time_domain = 0;
     double bbi_fmax = 0;
-    std::vector<double> bbi_re, bbi_im, bbi_qx_re, bbi_qx_im, bbi_qy_re, bbi_qy_im;
+    std::vector<double> bbi_re, bbi_im, bbi_qx_re, bbi_qx_im, bbi_qy_re, bbi_qy_im, bbi_dx_re, bbi_dx_im, bbi_dy_re,
+        bbi_dy_im;
     double bbi_beta[2] = {1, 1};
     bool bl = false;  // rfMode-1 cavities present
     cts_beam_loading_config blc{};

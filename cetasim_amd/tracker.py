@@ -123,34 +123,39 @@ class Tracker:
         self.bl_nres = cfg.nres
 
     def set_bbimp(self, zre, zim, f_max, flags=(1, 0, 0, 0, 0), time_domain=0, zqx=None, zqy=None,
-                  beta=(1.0, 1.0)):
+                  beta=(1.0, 1.0), zdx=None, zdy=None):
         """Broadband impedance table Z(f_k) (the file's rows, last frequency
-        f_max), quadrupolar tables zqx, zqy = (re, im); zre None switches it off."""
+        f_max); quadrupolar and dipolar tables zqx, zqy, zdx, zdy = (re, im);
+        flags in the reference's order (Z, Dx, Dy, Qx, Qy); zre None
+        switches it off."""
         cfg = capi.BbimpConfig()
         if zre is not None:
             a = lambda v: np.ascontiguousarray(v, np.float64)  # noqa: E731
-            self._bbi_z = [a(zre), a(zim)] + ([a(zqx[0]), a(zqx[1])] if zqx else [None, None]) + \
-                          ([a(zqy[0]), a(zqy[1])] if zqy else [None, None])
+            self._bbi_z = [a(zre), a(zim)]
+            for t in (zqx, zqy, zdx, zdy):
+                self._bbi_z += [a(t[0]), a(t[1])] if t else [None, None]
             cfg.nimp = len(self._bbi_z[0])
             cfg.f_max = f_max
             cfg.zz_re, cfg.zz_im = capi.dptr(self._bbi_z[0]), capi.dptr(self._bbi_z[1])
-            if zqx:
-                cfg.zqx_re, cfg.zqx_im = capi.dptr(self._bbi_z[2]), capi.dptr(self._bbi_z[3])
-            if zqy:
-                cfg.zqy_re, cfg.zqy_im = capi.dptr(self._bbi_z[4]), capi.dptr(self._bbi_z[5])
+            names = (("zqx_re", "zqx_im"), ("zqy_re", "zqy_im"), ("zdx_re", "zdx_im"), ("zdy_re", "zdy_im"))
+            for k, (nre, nim) in enumerate(names):
+                if self._bbi_z[2 + 2 * k] is not None:
+                    setattr(cfg, nre, capi.dptr(self._bbi_z[2 + 2 * k]))
+                    setattr(cfg, nim, capi.dptr(self._bbi_z[3 + 2 * k]))
             cfg.beta_x, cfg.beta_y = beta
             cfg.flags[:] = list(flags)
             cfg.time_domain = time_domain
         capi.check(self.lib.cts_set_bbimp(self.h, ctypes.byref(cfg)))
 
     def bbimp_wake(self):
-        """(wake [3][nb][2 nimp - 1]: longitudinal, quadrupolar x, y; profile
-        [nb][2 nimp - 1]) of every bunch from the last turn."""
+        """(wake [5][nb][2 nimp - 1] in the flags' order: longitudinal,
+        dipolar x, y, quadrupolar x, y; profiles [3][nb][2 nimp - 1]: charge,
+        x-weighted, y-weighted) of every bunch from the last turn."""
         n = capi.I32()
         capi.check(self.lib.cts_get_bbimp_wake(self.h, ctypes.byref(n), None, None))
-        w, pr = np.empty(3 * self.nb * n.value), np.empty(self.nb * n.value)
+        w, pr = np.empty(5 * self.nb * n.value), np.empty(3 * self.nb * n.value)
         capi.check(self.lib.cts_get_bbimp_wake(self.h, ctypes.byref(n), capi.dptr(w), capi.dptr(pr)))
-        return w.reshape(3, self.nb, -1), pr.reshape(self.nb, -1)
+        return w.reshape(5, self.nb, -1), pr.reshape(3, self.nb, -1)
 
     def cavity_info(self, turn):
         out = np.empty(self.nb * self.bl_nres * 8)
@@ -196,6 +201,15 @@ class Tracker:
         n = ctypes.c_int32()
         capi.check(self.lib.cts_autotune(self.h, ctypes.byref(n)))
         return n.value
+
+    def set_layout(self, nwg, tile=0):
+        capi.check(self.lib.cts_set_layout(self.h, nwg, tile))
+
+    def layout(self):
+        """(workgroups, tile) of the streaming pass (tile 0: contiguous chunks)."""
+        n, t = ctypes.c_int32(), ctypes.c_int32()
+        capi.check(self.lib.cts_get_layout(self.h, ctypes.byref(n), ctypes.byref(t)))
+        return n.value, t.value
 
     def flush(self):
         capi.check(self.lib.cts_flush(self.h))

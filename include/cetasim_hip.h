@@ -215,8 +215,14 @@ int cts_get_cavity_info(cts_handle *h, int32_t turn, double *info);
  * with the real kernel of Z, and delta += V(bin) / (E beta^2).  The
  * quadrupolar planes (flags[3], flags[4], src/MPBunch.cpp:2218-2250) use
  * the same profile with i Z_Qx, i Z_Qy and kick px += V_Qx(bin) x /
- * (E beta^2 beta_x), py likewise.  The dipolar planes (flags[1], flags[2])
- * and the time-domain method are not built; all flags 0 switches it off. */
+ * (E beta^2 beta_x), py likewise.  The dipolar planes (flags[1], flags[2],
+ * src/MPBunch.cpp:2252-2312) use the x- and y-weighted profiles (charge
+ * times x summed per bin, smoothed the same way) with i Z_Dx, i Z_Dy and
+ * kick px += V_Dx(bin) / (E beta^2 beta_x), py likewise, after the
+ * quadrupolar kicks; the per-bin sums are two-limb fixed-point integers, so
+ * they do not depend on the order of the atomics, and the grid is then
+ * limited to 1535 bins (nimp <= 768).  The time-domain method is not
+ * built; all flags 0 switches it off. */
 typedef struct {
     int32_t nimp;                     /* rows of the impedance file, >= 2 */
     double f_max;                     /* its last frequency [Hz] */
@@ -226,12 +232,17 @@ typedef struct {
     const double *zqx_re, *zqx_im;    /* [nimp] Z_Qx [Ohm/m] (flags[3]) */
     const double *zqy_re, *zqy_im;    /* [nimp] Z_Qy (flags[4]) */
     double beta_x, beta_y;            /* beta functions at the interaction point [m] */
+    const double *zdx_re, *zdx_im;    /* [nimp] Z_Dx [Ohm/m] (flags[1]; the reference's zDxImp:
+                                         -column 8 - i column 3) */
+    const double *zdy_re, *zdy_im;    /* [nimp] Z_Dy (flags[2]; -column 9 - i column 4) */
 } cts_bbimp_config;
 int cts_set_bbimp(cts_handle *h, const cts_bbimp_config *cfg);
 /* The induced voltages of every bunch from the last turn:
- * wake[3][nb][2 nimp - 1] (longitudinal, quadrupolar x, y; bin k at
- * z = -z_max + k dz; zeros for planes off), and the smoothed profile [C/s]
- * [nb][2 nimp - 1] if prof is not NULL; n receives 2 nimp - 1. */
+ * wake[5][nb][2 nimp - 1] in the flags' order (longitudinal, dipolar x, y,
+ * quadrupolar x, y; bin k at z = -z_max + k dz; zeros for planes off), and
+ * the smoothed profiles prof[3][nb][2 nimp - 1] if prof is not NULL (charge
+ * [C/s], x- and y-weighted [C m/s]; zeros for dipolar planes off); n
+ * receives 2 nimp - 1. */
 int cts_get_bbimp_wake(cts_handle *h, int32_t *n, double *wake, double *prof);
 
 /* Radiation damping switch and quantum-excitation noise source
@@ -291,10 +302,21 @@ int cts_flush(cts_handle *h);
  * transmission/ion status is polled after every replay. */
 int cts_run_turns(cts_handle *h, int32_t first, int32_t nturns, int32_t use_graph);
 /* Autotuner (no counterpart in the reference): times the turn's streaming
- * pass on a copy of the particles for workgroup counts that are multiples
- * of the device's compute units, and keeps the fastest layout; *nwg_out
- * gets the chosen count. */
+ * pass on a copy of the particles for two layouts of the particle range
+ * (one contiguous chunk per workgroup, for workgroup counts that are
+ * multiples of the device's compute units; tiles of 2048-16384 particles
+ * dealt round-robin to 2-4 workgroups per compute unit) and keeps the
+ * fastest; *nwg_out gets the chosen workgroup count.  Moments and bin
+ * counts do not depend on the layout beyond the order of floating-point
+ * partial sums. */
 int cts_autotune(cts_handle *h, int32_t *nwg_out);
+/* Sets the layout the autotuner chooses between (no counterpart in the
+ * reference): tile = 0 cuts the particles into nwg contiguous chunks,
+ * tile > 0 into tiles of that many particles (rounded up to 256) dealt
+ * round-robin to min(nwg, tiles) workgroups. */
+int cts_set_layout(cts_handle *h, int32_t nwg, int32_t tile);
+/* The layout in use: workgroups of the streaming pass and tile (0: chunks). */
+int cts_get_layout(cts_handle *h, int32_t *nwg, int32_t *tile);
 
 /* ------------------------------------------- seams, one kernel each
  * MPBeam::BeamTransferPerInteractionPointDueToLatticeT (src/MPBeam.cpp:2613) */

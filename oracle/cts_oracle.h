@@ -281,7 +281,9 @@ int orc_bl_tables(int nres, const int *ip, const double *dp, int harmonic, doubl
 void orc_bl_kick(orc_bunch *b, double vr, double vi, double vb0, double zmin, double zmax, int nbins, int res_harm,
                  double f_rf, double rbeta, double energy);
 
-/* Broadband impedance, longitudinal plane (bbimp.c). */
+/* Broadband impedance, longitudinal, dipolar and quadrupolar planes (bbimp.c);
+ * orc_bbi_kick: planes[5], zre/zim [5][nimp] and wake [5][n] in the
+ * reference's flag order (Z, Dx, Dy, Qx, Qy), prof [3][n]. */
 void orc_bbi_smooth(double *p, int n);
 void orc_bbi_potential(const double *p, int nimp, const double *zre, const double *zim, double *wake);
 void orc_bbi_kick(orc_bunch *b, int nimp, double f_max, const int *planes, const double *zre, const double *zim,

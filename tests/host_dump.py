@@ -24,9 +24,9 @@ def impedance_file(nimp=261, f_max=1.0e11, rs=2.0e3, q=1.0, fr=2.0e10):
     """A broadband-resonator impedance file in BoardBandImp::ReadInImp's
     layout (src/BoardBandImp.cpp:42-70): 5 header lines, then f, Re Z_||,
     Re Z_Dx, Re Z_Dy, Re Z_Qx, Re Z_Qy, Im Z_||, Im Z_Dx, ... per row (the
-    dipolar columns here are zero, the quadrupolar ones a transverse
-    resonator).  Returns (text, Re Z, Im Z, f_max) with Z = column 2 - i
-    column 7 as the reader forms it."""
+    transverse columns are transverse resonators: quadrupolar x, y at 1/2
+    and 1 of the scale, dipolar x, y at 0.8 and 1.2).  Returns (text, Re Z,
+    Im Z, f_max) with Z = column 2 - i column 7 as the reader forms it."""
     f = [k * f_max / (nimp - 1) for k in range(nimp)]
     rows, zre, zim = [], [], []
     for fk in f:
@@ -34,8 +34,8 @@ def impedance_file(nimp=261, f_max=1.0e11, rs=2.0e3, q=1.0, fr=2.0e10):
         # the file holds Im Z with the opposite sign of the reader's Z
         zq = 0j if fk == 0 else 0.5 * (fr / fk) * rs / (1 + 1j * q * (fk / fr - fr / fk))
         # columns: f, Re Z_||, Re Z_Dx, Re Z_Dy, Re Z_Qx, Re Z_Qy, Im Z_||, Im Z_Dx, Im Z_Dy, Im Z_Qx, Im Z_Qy
-        rows.append(" ".join(repr(v) for v in [fk, z.real, 0, 0, zq.real, 2 * zq.real, -z.imag, 0, 0, zq.imag,
-                                                2 * zq.imag]))
+        rows.append(" ".join(repr(v) for v in [fk, z.real, 1.6 * zq.real, 2.4 * zq.real, zq.real, 2 * zq.real,
+                                                -z.imag, 1.6 * zq.imag, 2.4 * zq.imag, zq.imag, 2 * zq.imag]))
         zre.append(z.real)
         zim.append(z.imag)
     text = "header\nfreq ReZ ...\nunits\n-----\n\n" + "\n".join(rows) + "\n"

@@ -277,17 +277,19 @@ class OracleTurn:
             # BBImpBeamInteraction, after the radiation stage (src/MPBeam.cpp:1084)
             c = self.bbi
             n, ni = 2 * c["nimp"] - 1, c["nimp"]
-            self.bbi_wake, self.bbi_prof = np.zeros(3 * n), np.zeros(n)
+            self.bbi_wake, self.bbi_prof = np.zeros(5 * n), np.zeros(3 * n)
             zero = np.zeros(ni)
-            qx, qy = c.get("zqx"), c.get("zqy")
-            zre = np.ascontiguousarray(np.concatenate([c["zre"], qx[0] if qx else zero, qy[0] if qy else zero]))
-            zim = np.ascontiguousarray(np.concatenate([c["zim"], qx[1] if qx else zero, qy[1] if qy else zero]))
-            planes = np.array([int(c.get("long", 1)), int(qx is not None), int(qy is not None)], np.int32)
+            # the reference's flag order: Z, Dx, Dy, Qx, Qy
+            tabs = [(c["zre"], c["zim"])] + [c.get(k) for k in ("zdx", "zdy", "zqx", "zqy")]
+            zre = np.ascontiguousarray(np.concatenate([t[0] if t else zero for t in tabs]))
+            zim = np.ascontiguousarray(np.concatenate([t[1] if t else zero for t in tabs]))
+            planes = np.array([int(c.get("long", 1))] + [int(t is not None) for t in tabs[1:]], np.int32)
             bx, by = c.get("beta", (1.0, 1.0))
             L.orc_bbi_kick(ctypes.byref(b.s), ni, c["f_max"], planes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                            p(zre), p(zim), bx, by,
                            c["q"], R.rbeta, R.t0, R.harmonic, R.energy, p(self.bbi_wake), p(self.bbi_prof))
-            self.bbi_wake = self.bbi_wake.reshape(3, n)
+            self.bbi_wake = self.bbi_wake.reshape(5, n)
+            self.bbi_prof = self.bbi_prof.reshape(3, n)
         if self.nbins:
             tab = self.table
             L.orc_sr_wake(ctypes.byref(b.s), self.nbins, self.mode, sig_stale, self.grid[0], self.grid[1],

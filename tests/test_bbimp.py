@@ -1,11 +1,15 @@
-"""The oracle's broadband impedance (longitudinal plane) pinned by what the
+"""The oracle's broadband impedance pinned by what the
 mathematics fixes: a constant resistive impedance R gives V = -R I bin by
 bin; a reactive Z_k = i a k (an inductance on the file's frequency grid)
 gives the spectral derivative of a cosine profile, V_j = a m sin(2 pi m j/n)
 for I_j = cos(2 pi m j/n); the smoothing leaves a constant profile as it is,
 keeps the sum, and spreads a single spike into the normalised Gaussian
 window exp(-alpha^2 x^2 / 2), x = (j - 25)/25, alpha = 10; a quadrupolar
-table Z_Q = -i R gives V_Q = +R I and px += V_Q x / (E beta^2 beta_x)."""
+table Z_Q = -i R gives V_Q = +R I and px += V_Q x / (E beta^2 beta_x); a
+dipolar table Z_D = -i R gives V_D = +R I_x, I_x the smoothed x-weighted
+profile formed here from the particles, and px += V_D / (E beta^2 beta_x);
+at a common offset x0 of every particle the dipolar kick equals the
+quadrupolar one for the same table."""
 import numpy as np
 
 import oracle as O
@@ -57,31 +61,102 @@ def test_smoothing():
     assert abs(p[0] / p[1] - 1) < 1e-12
 
 
-def test_quadrupolar_plane_sign_and_kick():
-    """One bunch through orc_bbi_kick with the longitudinal plane off and
-    Z_Qx = -i R: i Z_Qx = R, so V_Qx = +R I (the reference's + sign,
-    src/MPBunch.cpp:2218-2233), and px changes by V_Qx(bin) x/(E beta^2 beta_x)."""
+CLIGHT = 299792458.0
+ECHARGE = 1.602017733e-19  # include/Global.h (the oracle's constant)
+
+
+def kick(b, nimp, fmax, planes, ztab, bx=5.0, by=3.0, q=1e-15, rbeta=0.99999, t0=2e-6, h=1000, E=6e9):
+    """orc_bbi_kick on one bunch; ztab {plane: (re, im)} in flag order
+    (0 Z, 1 Dx, 2 Dy, 3 Qx, 4 Qy).  Returns (wake [5][n], prof [3][n])."""
     import ctypes
-    rng = np.random.default_rng(2)
-    n0 = 3000
+    n = 2 * nimp - 1
+    zre, zim = np.zeros((5, nimp)), np.zeros((5, nimp))
+    for k, (re, im) in ztab.items():
+        zre[k], zim[k] = re, im
+    pl = np.zeros(5, np.int32)
+    pl[list(planes)] = 1
+    wake, prof = np.zeros(5 * n), np.zeros(3 * n)
+    O.lib().orc_bbi_kick(ctypes.byref(b.s), nimp, fmax, pl.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                         O.p(zre.ravel()), O.p(zim.ravel()), bx, by, q, rbeta, t0, h, E, O.p(wake), O.p(prof))
+    return wake.reshape(5, n), prof.reshape(3, n)
+
+
+def bunch_xyz(seed, n0=3000, x0=0.0, y0=0.0, sx=1e-4, sy=5e-5):
+    rng = np.random.default_rng(seed)
     a = {c: np.zeros(n0) for c in O.COORDS}
     a["z"] = rng.normal(0, 4e-3, n0)
-    a["x"] = rng.normal(0, 1e-4, n0)
-    b = O.Bunch(a)
-    nimp, fmax, R = 261, 1.0e11, 500.0
-    zero = np.zeros(nimp)
-    zre = np.concatenate([zero, zero, zero])
-    zim = np.concatenate([zero, np.full(nimp, -R), zero])
-    planes = np.array([0, 1, 0], np.int32)
+    a["x"] = x0 + (rng.normal(0, sx, n0) if sx else np.zeros(n0))
+    a["y"] = y0 + (rng.normal(0, sy, n0) if sy else np.zeros(n0))
+    return a, O.Bunch(a)
+
+
+def grid(nimp, fmax, rbeta=0.99999, t0=2e-6, h=1000):
     n = 2 * nimp - 1
-    wake, prof = np.zeros(3 * n), np.zeros(n)
-    rbeta, t0, h, E, bx = 0.99999, 2e-6, 1000, 6e9, 5.0
-    O.lib().orc_bbi_kick(ctypes.byref(b.s), nimp, fmax, planes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                         O.p(zre), O.p(zim), bx, 1.0, 1e-15, rbeta, t0, h, E, O.p(wake), O.p(prof))
-    wake = wake.reshape(3, n)
-    assert np.all(wake[0] == 0) and np.all(wake[2] == 0)
-    np.testing.assert_allclose(wake[1], R * prof, rtol=0, atol=1e-12 * R * prof.max())
-    dz = 299792458.0 * rbeta / (2 * fmax)
-    k = ((a["z"] + dz * (nimp - 1) + dz / 2) / dz).astype(int)
-    np.testing.assert_allclose(b["px"], wake[1][k] / E / rbeta ** 2 * a["x"] / bx, rtol=1e-13, atol=0)
-    assert np.all(b["dp"] == 0)
+    dz = CLIGHT * rbeta / (2 * fmax)
+    zmin = -dz * (nimp - 1)
+    rf = t0 / h * CLIGHT
+    return n, dz, zmin, int((-rf / 2 - zmin + dz / 2) / dz), int((rf / 2 - zmin + dz / 2) / dz)
+
+
+def test_quadrupolar_plane_sign_and_kick():
+    """One bunch through orc_bbi_kick with only the Qx plane on and
+    Z_Qx = -i R: i Z_Qx = R, so V_Qx = +R I (the reference's + sign,
+    src/MPBunch.cpp:2218-2233), and px changes by V_Qx(bin) x/(E beta^2 beta_x)."""
+    a, b = bunch_xyz(2, sy=0)
+    nimp, fmax, R = 261, 1.0e11, 500.0
+    wake, prof = kick(b, nimp, fmax, (3,), {3: (np.zeros(nimp), np.full(nimp, -R))})
+    assert all(np.all(wake[q] == 0) for q in (0, 1, 2, 4))
+    np.testing.assert_allclose(wake[3], R * prof[0], rtol=0, atol=1e-12 * R * prof[0].max())
+    n, dz, zmin, _, _ = grid(nimp, fmax)
+    k = ((a["z"] - zmin + dz / 2) / dz).astype(int)
+    np.testing.assert_allclose(b["px"], wake[3][k] / 6e9 / 0.99999 ** 2 * a["x"] / 5.0, rtol=1e-13, atol=0)
+    assert np.all(b["dp"] == 0) and np.all(b["py"] == 0)
+
+
+def test_dipolar_plane_sign_profile_and_kick():
+    """Only the Dx plane on, Z_Dx = -i R: V_Dx = +R I_x with I_x the
+    smoothed x-weighted profile (charge times x summed per bin,
+    src/MPBunch.cpp:2252-2281), and px changes by V_Dx(bin)/(E beta^2
+    beta_x), independently of the particle's own x.  The raw weighted
+    profile is formed here from the particles, the smoothing is the one
+    test_smoothing pins."""
+    a, b = bunch_xyz(4, x0=3e-5)
+    nimp, fmax, R = 261, 1.0e11, 700.0
+    q, E, rbeta, bx = 2e-15, 6e9, 0.99999, 5.0
+    wake, prof = kick(b, nimp, fmax, (1,), {1: (np.zeros(nimp), np.full(nimp, -R))}, bx=bx, q=q)
+    n, dz, zmin, i0, i1 = grid(nimp, fmax)
+    k = ((a["z"] - zmin + dz / 2) / dz).astype(int)
+    raw = np.zeros(n)
+    np.add.at(raw, k, q * ECHARGE / dz * CLIGHT * a["x"])
+    bucket = np.ascontiguousarray(raw[i0:i1])
+    O.lib().orc_bbi_smooth(O.p(bucket), i1 - i0)
+    exp = raw.copy()
+    exp[i0:i1] = bucket
+    np.testing.assert_allclose(prof[1], exp, rtol=0, atol=1e-12 * np.abs(exp).max())
+    assert all(np.all(wake[p] == 0) for p in (0, 2, 3, 4))
+    np.testing.assert_allclose(wake[1], R * prof[1], rtol=0, atol=1e-12 * R * np.abs(prof[1]).max())
+    np.testing.assert_allclose(b["px"], wake[1][k] / E / rbeta ** 2 / bx, rtol=1e-13, atol=0)
+    assert np.all(b["py"] == 0) and np.all(b["dp"] == 0)
+
+
+def test_dipolar_equals_quadrupolar_at_a_common_offset():
+    """Every particle at the same (x0, y0): the x-weighted profile is x0
+    times the charge profile, so with the same impedance table the dipolar
+    kick V_Dx/(E beta^2 beta_x) equals the quadrupolar kick
+    V_Qx x0/(E beta^2 beta_x) (and likewise in y)."""
+    nimp, fmax = 261, 1.0e11
+    f = np.arange(nimp) * fmax / (nimp - 1)
+    zt = np.zeros(nimp, complex)
+    zt[1:] = (1.5e10 / f[1:]) * 1e3 / (1 + 1j * (f[1:] / 1.5e10 - 1.5e10 / f[1:]))
+    tab = (zt.real.copy(), zt.imag.copy())
+    x0, y0 = 2e-4, -1.5e-4
+    a, bq = bunch_xyz(6, x0=x0, y0=y0, sx=0, sy=0)
+    _, bd = bunch_xyz(6, x0=x0, y0=y0, sx=0, sy=0)
+    wq, _ = kick(bq, nimp, fmax, (3, 4), {3: tab, 4: tab})
+    wd, _ = kick(bd, nimp, fmax, (1, 2), {1: tab, 2: tab})
+    np.testing.assert_allclose(wd[1], x0 * wq[3], rtol=0, atol=1e-12 * np.abs(x0 * wq[3]).max())
+    np.testing.assert_allclose(wd[2], y0 * wq[4], rtol=0, atol=1e-12 * np.abs(y0 * wq[4]).max())
+    for c in ("px", "py"):
+        scale = np.abs(bq[c]).max()
+        assert scale > 0
+        np.testing.assert_allclose(bd[c], bq[c], rtol=0, atol=1e-12 * scale)

@@ -10,7 +10,7 @@
   each coordinate's scale, moments to 1e-9, over 2 turns.
 * C3: the 80-bunch train (counts uniform in [1e4, 2e5], seed 3) with
   residual-gas ions at one interaction point: the first turn against the
-  oracle (the ion passages of the first segment of 16 bunches to 1e-9, their
+  oracle (the ion passages of the first 16 bunches to 1e-9, their
   bunches' coordinates to 1e-11; further down the train the ions' motion at
   full intensity is chaotic, so rounding differences grow and are only
   reported), then 20 graph-replayed turns run twice: bitwise the same,
@@ -150,12 +150,12 @@ def first_turn_against_oracle(T, counts, gap, wake):
     ot.turn(bunches, 0)
     got = tr.ion_passages()
     exp = ot.last_cloud
-    nseg = min(16, len(counts))
+    nseg = min(16, len(counts))  # the first 16 passages (chaos grows further down the train)
     np.testing.assert_allclose(got[:nseg, 0, 5], exp[:nseg, 0, 0], rtol=1e-12)
     # rms sizes to 1e-9; the means (near zero) to 1e-9 of the rms size
     np.testing.assert_allclose(got[:nseg, 0, 3:5], exp[:nseg, 0, 3:5], rtol=1e-9)
     assert np.all(np.abs(got[:nseg, 0, 1:3] - exp[:nseg, 0, 1:3]) <= 1e-9 * exp[:nseg, 0, 3:5])
-    # the bunches of that segment: coordinates to 1e-11 of each one's scale
+    # those passages' bunches: coordinates to 1e-11 of each one's scale
     compare_coords(tr.download(), bunches[:nseg], off[:nseg + 1], 1e-11)
     tr.close()
     return got, exp

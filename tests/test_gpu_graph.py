@@ -57,7 +57,7 @@ def same(e, g, first, last, what):
     ([3000, 5001, 2500], 0, False),  # reference edges: pass + finalize + bin + table + wake
     ([3000, 5001, 2500], 1, True),   # ion train: ion phase + pass + post
     ([4000, 2000], None, True),      # no wake: the turn closes its own moments
-    ([1500] * 20, 1, True),          # two ion segments: the cloud buffer returns each turn
+    ([1500] * 40, 1, True),          # two ion segments: the cloud buffer returns each turn
 ])
 def test_graph_replays_match_eager(counts, wake_mode, ions):
     e, g = make(counts, wake_mode, ions)

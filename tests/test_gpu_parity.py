@@ -328,6 +328,59 @@ def test_full_size_properties():
     assert np.array_equal(outs[0][2], outs[1][2])
 
 
+@pytest.mark.parametrize("wake_mode", [1, 0])
+def test_layouts_agree(wake_mode):
+    """Every layout the autotuner chooses between runs the same turn:
+    contiguous chunks (one workgroup, several) and round-robin tiles
+    (tiles of 256 particles, tiles holding several bunches, more tiles than
+    workgroups, more workgroups than tiles) on an uneven 6-bunch train with
+    losses, device excitation and the short-range wake.  Loss codes and bin
+    counts bit-identical; on the fixed grid (wake mode 1) the coordinates
+    bit-identical too; with the reference's edges (mode 0) the edges carry
+    the rms bunch length, a sum whose order follows the layout, so the
+    coordinates agree to 1e-12; moments to 1e-12 (partial sums regrouped)."""
+    (ring, secs, _), p = lattice()
+    counts = np.array([700, 5000, 300, 2600, 9000, 40])
+    nb = len(counts)
+    gap = configs.train_gaps(nb, 3, ring.harmonic)
+    q, ne = configs.bunch_scalars(counts, ring, p)
+    parts = configs.train_bunches(counts, 17, p)
+    lost = random_losses(len(parts["x"]), 5)
+    sz = p["sig"][2]
+    layouts = [None, (1, 0), (5, 0), (2, 256), (3, 1024), (40, 2048), (1000, 4096)]
+    outs = []
+    for lay in layouts:
+        print(f"layouts agree: wake mode {wake_mode}, layout {lay}", flush=True)
+        tr = Tracker(ring, secs)
+        tr.set_bunches(counts, q, ne, gap)
+        tr.upload(parts, lost)
+        tr.set_wake(100, wake_mode, -5 * sz, 5 * sz, BBR["rs"], BBR["q"], BBR["fr"])
+        tr.set_synrad(True, 2, 3)
+        if lay:
+            tr.set_layout(*lay)
+        for t in range(3):
+            tr.turn(t)
+        d = tr.download()
+        hist = tr.wake_state()[0]
+        outs.append((d, hist, tr.moments()))
+        tr.close()
+    d0, h0, m0 = outs[0]
+    for lay, (d, h, m) in zip(layouts[1:], outs[1:]):
+        assert np.array_equal(d["lost"], d0["lost"]), lay
+        assert np.array_equal(h, h0), lay
+        for c in COORDS:
+            if wake_mode == 1:
+                assert np.array_equal(d[c], d0[c]), (lay, c)
+            else:
+                scale = np.max(np.abs(d0[c]))
+                assert np.max(np.abs(d[c] - d0[c])) <= 1e-12 * scale, (lay, c)
+        for b in range(nb):
+            assert m[b]["n_alive"] == m0[b]["n_alive"]
+            np.testing.assert_allclose(m[b]["mean"], m0[b]["mean"], rtol=1e-12, atol=1e-12 * np.sqrt(
+                np.abs(m0[b]["c2"]).max()))
+            np.testing.assert_allclose(m[b]["c2"], m0[b]["c2"], rtol=1e-12, atol=1e-13 * np.abs(m0[b]["c2"]).max())
+
+
 def test_autotuned_layout_keeps_parity():
     """The autotuner only changes the workgroup layout: turns after it still
     match the oracle (and its timing runs leave the particles untouched)."""

@@ -140,19 +140,19 @@ def test_ion_train_turns():
     tr.close()
 
 
-@pytest.mark.parametrize("nb", [37, 32])
+@pytest.mark.parametrize("nb", [69, 64])
 def test_ion_segments_match_oracle(nb):
-    """A train longer than one ion segment (16 passages): the cloud is
-    compacted between segments, in slot order, so passages, cloud and kicks
-    still follow the oracle's bunch-by-bunch walk; two turns (the second
-    starts from the first's cloud).  37 bunches: three segments (an odd
-    number of buffer swaps per turn), 32: two (even).
+    """A train longer than one ion segment (32 passages, the library's
+    kIonSegment): the cloud is compacted between segments, in slot order, so
+    passages, cloud and kicks still follow the oracle's bunch-by-bunch walk;
+    two turns (the second starts from the first's cloud).  69 bunches: three
+    segments (an odd number of buffer swaps per turn), 64: two (even).
 
     The bunches carry a tenth of the nominal intensity: at full intensity
     the ions' motion over a long train is chaotic (on the oracle, a 1e-15
     relative change of one bunch size moves the mean y seen by passage 36
     by 1e-3), so rounding differences would swamp the comparison; at a
-    tenth the same change stays below 4e-14 over all 37 passages."""
+    tenth the same change stays below 4e-14 over a 37-passage train."""
     counts = [900 + 37 * (b % 7) for b in range(nb)]
     ring, secs, counts, gap, q, ne, cfg, tr, bunches, off = setup_train(counts, 12, ions=ION_TEST, ne_scale=0.1)
     ot = O.OracleTrain(ring, secs[0], counts, ne, gap, cfg, ION_TEST["seed"])
@@ -165,7 +165,7 @@ def test_ion_segments_match_oracle(nb):
         np.testing.assert_allclose(got[:, 0, 1:5], ot.last_cloud[:, 0, 1:5], rtol=1e-9, atol=1e-18)
     # ions survive from segment to segment (the comparison covers carried
     # ions); the train's closing gap empties most of the cloud
-    assert got[20, 0, 0] > 2 * ION_TEST["n_new"]
+    assert got[40, 0, 0] > 2 * ION_TEST["n_new"]
     cloud = tr.ion_cloud(0)
     ec = ot.cloud.species(0)
     assert len(cloud["x"]) == len(ec["x"])

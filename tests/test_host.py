@@ -294,6 +294,11 @@ def test_bbimp_settings(tmp_path):
     np.testing.assert_array_equal(s["bbi_qx_im"], [-r[4] for r in rows])
     np.testing.assert_array_equal(s["bbi_qy_re"], [-r[10] for r in rows])
     np.testing.assert_array_equal(s["bbi_qy_im"], [-r[5] for r in rows])
+    # zDxImp = (-col 8, -col 3), zDyImp = (-col 9, -col 4) (src/BoardBandImp.cpp:65-66)
+    np.testing.assert_array_equal(s["bbi_dx_re"], [-r[7] for r in rows])
+    np.testing.assert_array_equal(s["bbi_dx_im"], [-r[2] for r in rows])
+    np.testing.assert_array_equal(s["bbi_dy_re"], [-r[8] for r in rows])
+    np.testing.assert_array_equal(s["bbi_dy_im"], [-r[3] for r in rows])
     assert s["bbi_beta"] == [2.71, 4.25]
     # off by default; a missing file is an error
     H.copy_fixture(str(tmp_path), {"runBBIFlag": "1", "BBIimpedInput": "none.dat"})
