@@ -79,6 +79,11 @@ def parse():
     return ap.parse_args()
 
 
+def progress(msg):
+    """A progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def timed_turns(tr, a, names):
     """Warm-up turns (the turn graphs are captured there), then the timed
     turns, then as many turns again launched one by one with HIP events
@@ -322,6 +327,7 @@ def train_turns(a, sur=None, mode=0):
         tr.set_field_model(sur, mode)
     tr.autotune()
     layout = tr.layout()
+    progress(f"train of {nb} bunches set up (field model {mode}), layout {layout}; timing")
     dt, dt_ev, ktimes = timed_turns(tr, a, ("pass", "pass_a", "post", "finalize", "ion_prep", "ion_turn", "ion_compact",
                                             "ion_reduce", "ion_finish", "ion_kick", "lrwake",
                                             "bbi_bin", "bbi_wake", "bbi_kick"))
@@ -348,6 +354,7 @@ def run_train(a):
     value, value_path = plain, "plain: FP64 Faddeeva kicks both ways"
     per_turn = {k: v[0] * v[1] / a.steps for k, v in ktimes.items()}
     if not a.plain_only:
+        progress(f"plain path timed; training the surrogate ({a.sur_steps} Adam steps)")
         sur, sur_info = train_surrogate(a.sur_steps)
         # electron-side kicks from the surrogate (mode 2), and both sides (mode 3)
         dts, kts, cloud_s, m_sur, _, _, _, _ = train_turns(a, sur, 2)
@@ -403,9 +410,13 @@ def run_train(a):
                      "turn_frac": BYTES_PER_MPT * n / (dt / a.steps) / 1e9 / HBM_PEAK_GBS,
                      "other_kernels_ms_per_turn": per_turn},
         "ion_macroparticles_last_passage": float(cloud[-1, 0, 0]),
+        # Bassetti-Erskine evaluations of the ion side per turn (ions met by
+        # each bunch, summed over the train)
+        "ion_macroparticle_passages_per_turn": float(cloud[:, :, 0].sum()),
         "surrogate": sur_info,
     }
     if not a.no_cpu_baseline:
+        progress("GPU runs done; timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline_train(ring, secs, p, a.cpu_seconds, T, float(cloud[:, :, 0].sum()), n,
                                                  nb)
     print(json.dumps(out))

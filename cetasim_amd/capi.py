@@ -92,7 +92,8 @@ class BeamLoadingConfig(ctypes.Structure):
     _fields_ = [("nres", I32), ("rf_mode", I32 * MAX_RF), ("res_type", I32 * MAX_RF), ("res_harm", I32 * MAX_RF),
                 ("rs", D * MAX_RF), ("q0", D * MAX_RF), ("coupling", D * MAX_RF), ("detune_hz", D * MAX_RF),
                 ("vol", D * MAX_RF), ("phase", D * MAX_RF), ("cold", I32 * MAX_RF),
-                ("excite_instability", I32 * MAX_RF), ("dir_fb", I32 * MAX_RF), ("nbins", I32), ("nturns", I32)]
+                ("excite_instability", I32 * MAX_RF), ("dir_fb", I32 * MAX_RF), ("nbins", I32), ("nturns", I32),
+                ("dir_fb_gain", D * MAX_RF), ("dir_fb_phase", D * MAX_RF), ("dir_fb_delay", D * MAX_RF)]
 
 
 class CbmResult(ctypes.Structure):

@@ -187,6 +187,14 @@ struct cts_handle {
     bool bl_on = false;
     BlTable bl;
     double *d_bl_tab = nullptr;
+    // walked on the device: state, bins, kicks, feedback kicks, last turn's
+    // info, and the bunches' populations and gaps
+    double *d_bl_state = nullptr, *d_bl_kick = nullptr, *d_bl_fb = nullptr, *d_bl_info = nullptr;
+    double *d_bl_ne = nullptr, *d_bl_mq = nullptr;
+    uint32_t *d_bl_hist = nullptr;
+    int32_t *d_bl_gap = nullptr;
+    uint32_t bl_fbmask = 0;
+    int bl_info_turn = -1;
 
     // partial sums of the eigen-emittance moments (cts_eigen_emit)
     double *d_eig = nullptr;
@@ -237,8 +245,15 @@ void free_lrw_state(cts_handle *h)
 
 void free_bl_state(cts_handle *h)
 {
-    if (h->d_bl_tab) (void)hipFree(h->d_bl_tab);
-    h->d_bl_tab = nullptr;
+    void *ptrs[] = {h->d_bl_tab, h->d_bl_state, h->d_bl_kick, h->d_bl_fb, h->d_bl_info,
+                    h->d_bl_ne,  h->d_bl_mq,    h->d_bl_hist, h->d_bl_gap};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    h->d_bl_tab = h->d_bl_state = h->d_bl_kick = h->d_bl_fb = h->d_bl_info = h->d_bl_ne = h->d_bl_mq = nullptr;
+    h->d_bl_hist = nullptr;
+    h->d_bl_gap = nullptr;
+    h->bl_fbmask = 0;
+    h->bl_info_turn = -1;
     h->bl_on = false;
     h->bl = BlTable{};
 }
@@ -492,6 +507,12 @@ PassParams base_pass(cts_handle *h, uint32_t ops, int k)
         for (int q = 0; q < h->bl.nbl; q++) p.bl_kfac[q] = h->bl.kfac[q];
         p.bl_energy = h->ring.energy;
         p.bl_beta2 = std::pow(h->ring.rbeta, 2);
+        if (h->bl.walk) {
+            p.bl_walk = h->d_bl_kick;
+            p.bl_fbk = h->bl_fbmask ? h->d_bl_fb : nullptr;
+            p.bl_fbmask = h->bl_fbmask;
+            p.bl_hist = h->d_bl_hist;
+        }
     }
     if (h->capturing) {
         p.turn_dev = h->d_turn;
@@ -1397,6 +1418,38 @@ static int turn_body(cts_handle *h, int32_t turn)
         if (!h->capturing) pa.turn = (uint32_t)turn;
         if ((rc = run_pass(h, pa, "pass_a"))) return rc;
         if ((rc = run_finalize(h, base_fin(h, OP_ZSTAT, -1, -1)))) return rc;
+        if (h->bl.walk) {
+            // the cavity state depends on the particles: a second read-only
+            // pass counts every bunch's survivors per bin over that range,
+            // then one wave per cavity walks the train
+            const uint32_t opsB = (ops & (OP_KICK | OP_LRK | OP_MAP | OP_SKEW)) | OP_BLH;
+            PassParams pb = base_pass(h, opsB, 0);
+            if (!h->capturing) pb.turn = (uint32_t)turn;
+            if ((rc = run_pass(h, pb, "pass_b"))) return rc;
+            BlWalkParams w{};
+            w.state = h->d_bl_state; w.hist = h->d_bl_hist; w.zrange = h->d_zrange; w.gap = h->d_bl_gap;
+            w.ne = h->d_bl_ne; w.mq = h->d_bl_mq; w.kick = h->d_bl_kick; w.fb = h->d_bl_fb; w.info = h->d_bl_info;
+            w.nb = h->nb; w.nbins = h->bl.nbins; w.nbl = h->bl.nbl; w.nres = h->bl.nres;
+            for (int q = 0; q < h->bl.nbl; q++) {
+                w.bl_index[q] = h->bl.bl_index[q];
+                w.kfac[q] = h->bl.kfac[q];
+            }
+            w.nturns = h->bl.nturns;
+            w.turn = (uint32_t)turn;
+            if (h->capturing) {
+                w.turn_dev = h->d_turn;
+                w.turn = (uint32_t)h->cap_off;
+            }
+            w.trf = h->ring.t0 / h->ring.harmonic;
+            w.rbeta = h->ring.rbeta;
+            w.energy = h->ring.energy;
+            w.beta2 = std::pow(h->ring.rbeta, 2);
+            {
+                TimedScope ts(h, "bl_walk");
+                HIPCHK(launch_bl_walk(w, h->stream));
+            }
+            if (!h->capturing) h->bl_info_turn = std::max(h->bl_info_turn, turn);
+        }
         ops |= OP_BL;
     }
     PassParams p = base_pass(h, ops, 0);
@@ -1432,7 +1485,7 @@ int cts_turn(cts_handle *h, int32_t turn)
     int rc = check_ready(h);
     if (rc) return rc;
     if (h->secs.size() != 1) return fail(CTS_ERR_STATE, "cts_turn fuses one section; use the seam calls");
-    if (h->bl_on && (turn < 0 || turn >= h->bl.nturns))
+    if (h->bl_on && !h->bl.walk && (turn < 0 || turn >= h->bl.nturns))
         return fail(CTS_ERR_ARG, "cts_turn: turn outside the beam-loading table (cts_set_beam_loading nturns)");
     // a bunch below 50% transmission ends the run, as the reference exits
     if ((rc = check_status(h, false))) return rc;
@@ -1505,7 +1558,7 @@ int cts_run_turns(cts_handle *h, int32_t first, int32_t nturns, int32_t use_grap
     if (rc) return rc;
     if (h->secs.size() != 1) return fail(CTS_ERR_STATE, "cts_run_turns fuses one section; use the seam calls");
     if (nturns < 0 || first < 0) return fail(CTS_ERR_ARG, "cts_run_turns: first, nturns >= 0");
-    if (h->bl_on && first + nturns > h->bl.nturns)
+    if (h->bl_on && !h->bl.walk && first + nturns > h->bl.nturns)
         return fail(CTS_ERR_ARG, "cts_run_turns: turns beyond the beam-loading table (cts_set_beam_loading nturns)");
     int t = first;
     const int end = first + nturns;
@@ -1568,6 +1621,7 @@ int cts_run_turns(cts_handle *h, int32_t first, int32_t nturns, int32_t use_grap
         h->lr_pending = h->lrw_T > 0;
         h->cur_valid = true;
         if (h->ips[0].nspec > 0) h->ips[0].len_ub = h->ips[0].cap;
+        if (h->bl_on && h->bl.walk) h->bl_info_turn = std::max(h->bl_info_turn, t + 1);
         if ((rc = enqueue_status_copy(h))) return rc;
         t += 2;
     }
@@ -1906,24 +1960,56 @@ int cts_set_beam_loading(cts_handle *h, const cts_beam_loading_config *cfg)
         if (cfg->rf_mode[i] != 0 && cfg->rf_mode[i] != 1) return fail(CTS_ERR_ARG, "cts_set_beam_loading: rf_mode 0 or 1");
         if (!(cfg->q0[i] > 0) || cfg->res_harm[i] < 1 || !(cfg->coupling[i] >= 0))
             return fail(CTS_ERR_ARG, "cts_set_beam_loading: Q0 > 0, harmonic >= 1, coupling >= 0");
-        if (cfg->rf_mode[i] == 1) {
-            if (cfg->excite_instability[i] != 0)
-                return fail(CTS_ERR_ARG, "cts_set_beam_loading: the bin-by-bin variant (rfResExciteIntability 1) "
-                                         "is not built");
-            if (cfg->dir_fb[i] != 0)
-                return fail(CTS_ERR_ARG, "cts_set_beam_loading: the direct cavity feedback is not built");
-            if (!(cfg->rs[i] > 0)) return fail(CTS_ERR_ARG, "cts_set_beam_loading: Rs > 0");
-        }
+        if (cfg->rf_mode[i] == 1 && !(cfg->rs[i] > 0)) return fail(CTS_ERR_ARG, "cts_set_beam_loading: Rs > 0");
     }
+    // the bin-by-bin variant or the direct feedback: the state is walked on
+    // the device every turn
+    bool walk = false;
+    int nbl_all = 0;
+    for (int i = 0; i < cfg->nres; i++)
+        if (cfg->rf_mode[i] == 1) {
+            walk = walk || cfg->excite_instability[i] != 0 || cfg->dir_fb[i] != 0;
+            nbl_all++;
+        }
+    if (walk && (int64_t)nbl_all * (2 * cfg->nbins + 1) > kBlMaxLds)
+        return fail(CTS_ERR_ARG, "cts_set_beam_loading: beam-loaded cavities x (2 nbins + 1) above " +
+                                     std::to_string(kBlMaxLds) + " with the bin-by-bin variant or the feedback");
     if (cfg->nres == 1 && !(std::fabs(h->ring.u0 / cfg->vol[0]) <= 1))
         return fail(CTS_ERR_ARG, "cts_set_beam_loading: one cavity needs |U0 / V| <= 1");
-    const double bytes = (double)cfg->nturns * h->nb * CTS_MAX_RF * 3 * 8;
-    if (bytes > 4e9) return fail(CTS_ERR_ARG, "cts_set_beam_loading: table above 4 GB (fewer turns per call)");
     std::string err;
     BlTable t;
-    if ((rc = beam_loading_tables(*cfg, h->ring, h->n_e, h->gap, t, err))) return fail(rc, err);
-    if ((rc = dalloc(&h->d_bl_tab, t.dev.size()))) return rc;
-    HIPCHK(hipMemcpy(h->d_bl_tab, t.dev.data(), t.dev.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (walk) {
+        const double bytes = (double)cfg->nturns * h->nb * cfg->nres * 8 * 8;
+        if (bytes > 4e9) return fail(CTS_ERR_ARG, "cts_set_beam_loading: cavity record above 4 GB (fewer turns)");
+        if ((rc = beam_loading_walk_state(*cfg, h->ring, h->n_e, h->gap, t, err))) return fail(rc, err);
+        const int nb = h->nb, nbl = t.nbl, nbins = t.nbins;
+        std::vector<double> mq(nb);
+        for (int b = 0; b < nb; b++) mq[b] = h->macro_charge[b];
+        if ((rc = dalloc(&h->d_bl_state, t.state.size())) || (rc = dalloc(&h->d_bl_hist, (size_t)nb * nbins)) ||
+            (rc = dalloc(&h->d_bl_kick, (size_t)nb * nbl * nbins * 2)) || (rc = dalloc(&h->d_bl_fb, (size_t)nb * nbl)) ||
+            (rc = dalloc(&h->d_bl_info, (size_t)t.nturns * nb * t.nres * 8)) || (rc = dalloc(&h->d_bl_ne, (size_t)nb)) ||
+            (rc = dalloc(&h->d_bl_mq, (size_t)nb)) || (rc = dalloc(&h->d_bl_gap, (size_t)nb))) {
+            free_bl_state(h);
+            return rc;
+        }
+        HIPCHK(hipMemcpy(h->d_bl_state, t.state.data(), t.state.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->d_bl_ne, h->n_e.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->d_bl_mq, mq.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->d_bl_gap, h->gap.data(), nb * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(h->d_bl_hist, 0, (size_t)nb * nbins * sizeof(uint32_t)));
+        HIPCHK(hipMemset(h->d_bl_fb, 0, (size_t)nb * nbl * sizeof(double)));
+        HIPCHK(hipMemset(h->d_bl_info, 0, (size_t)t.nturns * nb * t.nres * 8 * sizeof(double)));
+        h->bl_fbmask = 0;
+        for (int q = 0; q < nbl; q++)
+            if (t.state[(size_t)q * kBlS + BS_DFB] != 0.0) h->bl_fbmask |= 1u << q;
+        h->bl_info_turn = -1;
+    } else {
+        const double bytes = (double)cfg->nturns * h->nb * CTS_MAX_RF * 3 * 8;
+        if (bytes > 4e9) return fail(CTS_ERR_ARG, "cts_set_beam_loading: table above 4 GB (fewer turns per call)");
+        if ((rc = beam_loading_tables(*cfg, h->ring, h->n_e, h->gap, t, err))) return fail(rc, err);
+        if ((rc = dalloc(&h->d_bl_tab, t.dev.size()))) return rc;
+        HIPCHK(hipMemcpy(h->d_bl_tab, t.dev.data(), t.dev.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     h->bl = std::move(t);
     h->bl_on = true;
     h->keep_bl.on = true;
@@ -1935,8 +2021,16 @@ int cts_get_cavity_info(cts_handle *h, int32_t turn, double *info)
 {
     if (!h || !info) return fail(CTS_ERR_ARG, "null argument");
     if (!h->bl_on) return fail(CTS_ERR_STATE, "cts_get_cavity_info: no beam loading");
-    if (turn < 0 || turn >= h->bl.nturns) return fail(CTS_ERR_ARG, "cts_get_cavity_info: turn outside the table");
     const size_t n = (size_t)h->nb * h->bl.nres * 8;
+    if (h->bl.walk) {
+        // the walk records turns 0 .. nturns-1 as it runs them
+        if (turn < 0 || turn >= h->bl.nturns || turn > h->bl_info_turn)
+            return fail(CTS_ERR_ARG, "cts_get_cavity_info: turn not walked yet or past the record (nturns)");
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipMemcpy(info, h->d_bl_info + (size_t)turn * n, n * sizeof(double), hipMemcpyDeviceToHost));
+        return CTS_OK;
+    }
+    if (turn < 0 || turn >= h->bl.nturns) return fail(CTS_ERR_ARG, "cts_get_cavity_info: turn outside the table");
     std::memcpy(info, h->bl.info.data() + (size_t)turn * n, n * sizeof(double));
     return CTS_OK;
 }

@@ -11,7 +11,9 @@
 // (Resonator::GetBeamInducedVol, src/Resonator.cpp:205-232).  In the
 // reference's stable variant none of this depends on the particles, so the
 // voltages of every bunch are tabulated here for the run's turns and the
-// device only applies the kick.
+// device only applies the kick.  With the bin-by-bin variant or the direct
+// feedback the state depends on the particles: beam_loading_walk_state
+// sets it up for the device walk (k_bl_walk).
 #include <cmath>
 #include <complex>
 #include <string>
@@ -33,10 +35,11 @@ struct Cavity {
     cplx req, vb, gen, ig;
 };
 
-}  // namespace
-
-int beam_loading_tables(const cts_beam_loading_config &c, const cts_ring &ring, const std::vector<double> &ne,
-                        const std::vector<int32_t> &gap, BlTable &t, std::string &err)
+// CavityResonator::Initial and MPBeam::InitialcavityResonator: the cavities'
+// constants, the induced voltage after 100 filling times, the generator
+// voltage and current
+std::vector<Cavity> setup_cavities(const cts_beam_loading_config &c, const cts_ring &ring, const std::vector<double> &ne,
+                                   const std::vector<int32_t> &gap)
 {
     const int nres = c.nres, nb = (int)ne.size(), H = ring.harmonic;
     const double f0 = ring.f0;
@@ -106,6 +109,17 @@ int beam_loading_tables(const cts_beam_loading_config &c, const cts_ring &ring, 
             r.ig = abs_vgr * std::exp(kI * arg_vgr) * (1.0 + r.beta) / r.rs;
         }
     }
+    return cv;
+}
+
+}  // namespace
+
+int beam_loading_tables(const cts_beam_loading_config &c, const cts_ring &ring, const std::vector<double> &ne,
+                        const std::vector<int32_t> &gap, BlTable &t, std::string &err)
+{
+    const int nres = c.nres, nb = (int)ne.size(), H = ring.harmonic;
+    const double f0 = ring.f0;
+    std::vector<Cavity> cv = setup_cavities(c, ring, ne, gap);
     // the beam-loaded cavities, in input order
     t = BlTable{};
     t.nres = nres;
@@ -162,6 +176,49 @@ int beam_loading_tables(const cts_beam_loading_config &c, const cts_ring &ring, 
                 r.vb *= std::exp(-T / r.tf) * std::exp(kI * (2.0 * kPiB * r.fre * T));
             }
         }
+    }
+    return CTS_OK;
+}
+
+int beam_loading_walk_state(const cts_beam_loading_config &c, const cts_ring &ring, const std::vector<double> &ne,
+                            const std::vector<int32_t> &gap, BlTable &t, std::string &err)
+{
+    const int nres = c.nres, H = ring.harmonic;
+    const double f0 = ring.f0;
+    std::vector<Cavity> cv = setup_cavities(c, ring, ne, gap);
+    t = BlTable{};
+    t.walk = true;
+    t.nres = nres;
+    t.nturns = c.nturns;
+    t.nbins = c.nbins;
+    for (int i = 0; i < nres; i++) {
+        if (cv[i].mode != 1) continue;
+        const Cavity &r = cv[i];
+        t.bl_index.push_back(i);
+        t.kfac.push_back(2. * kPiB * (double)r.harm * (f0 * H) / kClight / ring.rbeta);
+        double s[kBlS] = {};
+        s[BS_EXC] = c.excite_instability[i] != 0;
+        s[BS_DFB] = c.dir_fb[i] != 0;
+        s[BS_HARM] = r.harm;
+        s[BS_RS] = r.rs;
+        s[BS_Q0] = r.q0;
+        s[BS_QL] = r.ql;
+        s[BS_DET] = r.detune;
+        s[BS_FRE] = r.fre;
+        s[BS_TF] = r.tf;
+        s[BS_REQ] = r.req.real(); s[BS_REQ + 1] = r.req.imag();
+        s[BS_VB] = r.vb.real(); s[BS_VB + 1] = r.vb.imag();
+        s[BS_GEN] = r.gen.real(); s[BS_GEN + 1] = r.gen.imag();
+        s[BS_IG] = r.ig.real(); s[BS_IG + 1] = r.ig.imag();
+        s[BS_FBG] = c.dir_fb_gain[i];
+        s[BS_FBP] = c.dir_fb_phase[i];
+        s[BS_FBD] = c.dir_fb_delay[i];
+        t.state.insert(t.state.end(), s, s + kBlS);
+    }
+    t.nbl = (int)t.bl_index.size();
+    if (t.nbl == 0) {
+        err = "cts_set_beam_loading: no rfMode-1 cavity";
+        return CTS_ERR_ARG;
     }
     return CTS_OK;
 }

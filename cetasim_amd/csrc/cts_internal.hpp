@@ -35,6 +35,7 @@ enum : uint32_t {
     OP_LRC = 1u << 15,   // post-map centroids of the long-range wake
     OP_LRK = 1u << 16,   // the previous turn's long-range wake kick (deferred)
     OP_BL = 1u << 17,    // beam-loaded cavities (rfMode 1), after the ideal ones
+    OP_BLH = 1u << 18,   // per-bunch bins over the z range for the cavity walk (read-only pass)
 };
 
 // per-segment partial sums, stored value-major: seg_part[j * nseg + s]
@@ -97,6 +98,12 @@ struct PassParams {
     // f_rf / (c beta) per cavity
     const double *bl_tab;
     const double *zrange;
+    // walked cavities: per bunch the bins' kicks [nb][nbl][bl_nbins][2] and
+    // the feedback kicks [nb][nbl] (bit q of bl_fbmask: cavity q feeds back);
+    // OP_BLH counts survivors per bin into bl_hist [nb][bl_nbins]
+    const double *bl_walk, *bl_fbk;
+    uint32_t *bl_hist;
+    uint32_t bl_fbmask;
     int32_t bl_nbl, bl_nbins, bl_nturns, nb;
     double bl_kfac[CTS_MAX_RF];
     double bl_energy, bl_beta2;
@@ -279,15 +286,45 @@ hipError_t launch_bbi_wake(const BbiParams &p, hipStream_t st);
 hipError_t launch_bbi_kick(const BbiParams &p, int nwg, hipStream_t st);
 
 // Host: beam-loaded cavities, tabulated per turn (cts_bl.cpp).
+// the state of a walked cavity (kBlS doubles): bin-by-bin and feedback
+// switches, harmonic, R_s, Q_0, Q_L, detuning, f_r, t_F, required voltage,
+// beam-induced voltage, generator voltage and current (re, im), feedback
+// gain, phase shift and delay
+enum { BS_EXC, BS_DFB, BS_HARM, BS_RS, BS_Q0, BS_QL, BS_DET, BS_FRE, BS_TF, BS_REQ, BS_VB = BS_REQ + 2,
+       BS_GEN = BS_VB + 2, BS_IG = BS_GEN + 2, BS_FBG = BS_IG + 2, BS_FBP, BS_FBD, kBlS };
 struct BlTable {
     int nturns = 0, nbl = 0, nbins = 0, nres = 0;
+    bool walk = false;           // state walked on the device every turn
     std::vector<int> bl_index;   // cavity index of each beam-loaded cavity
     std::vector<double> kfac;    // 2 pi h f_rf / (c beta) of each
     std::vector<double> dev;     // [nturns][nb][nbl][3]: Re V, Im V, Re V_b0
     std::vector<double> info;    // [nturns][nb][nres][8]
+    std::vector<double> state;   // walk: [nbl][kBlS]
 };
 int beam_loading_tables(const cts_beam_loading_config &c, const cts_ring &ring, const std::vector<double> &ne,
                         const std::vector<int32_t> &gap, BlTable &t, std::string &err);
+int beam_loading_walk_state(const cts_beam_loading_config &c, const cts_ring &ring, const std::vector<double> &ne,
+                            const std::vector<int32_t> &gap, BlTable &t, std::string &err);
+// One turn of the cavities' walk over the train (one wave per cavity).
+struct BlWalkParams {
+    double *state;          // [nbl][kBlS], carried from turn to turn
+    uint32_t *hist;         // [nb][nbins] survivors per bin (cleared by the walk)
+    const double *zrange;   // [nb][2] survivors' z range after the map (unpadded)
+    const int32_t *gap;     // [nb] buckets to the next bunch
+    const double *ne;       // [nb] electrons per bunch
+    const double *mq;       // [nb] electrons per macroparticle
+    double *kick;           // out [nb][nbl][nbins][2]: cavity and self-loss parts of each bin's kick
+    double *fb;             // out [nb][nbl]: the feedback kick
+    double *info;           // out [nturns][nb][nres][8] (turns past nturns are not kept)
+    uint32_t turn;          // the turn, or *turn_dev + turn when turn_dev is set (graph replays)
+    const uint32_t *turn_dev;
+    int32_t nturns;
+    int32_t nb, nbins, nbl, nres;
+    int32_t bl_index[CTS_MAX_RF];
+    double kfac[CTS_MAX_RF];
+    double trf, rbeta, energy, beta2;
+};
+hipError_t launch_bl_walk(const BlWalkParams &p, hipStream_t st);
 void lrwake_yokoya(double rx, double ry, double out[5]);
 
 struct WakeTableParams {

@@ -238,12 +238,25 @@ k_pass(PassParams P)
         // first, goes to LDS: bl_lds[q][k] for cavity q, then the self-loss
         // term of each cavity
         double bl_zmax = 0, bl_dz = 1;
-        if (ops & OP_BL) {
-            const int bt = (int)turn < P.bl_nturns ? (int)turn : P.bl_nturns - 1;
-            const double *blv = P.bl_tab + ((int64_t)bt * P.nb + b) * P.bl_nbl * 3;
+        if (ops & (OP_BL | OP_BLH)) {
             bl_zmax = P.zrange[2 * b + 1] + 1.e-6;
             const double zmin = P.zrange[2 * b] - 1.e-6;
             bl_dz = (bl_zmax - zmin) / P.bl_nbins;
+        }
+        if ((ops & OP_BL) && P.bl_walk) {
+            // walked cavities: each bin's cavity and self-loss kicks, then
+            // each cavity's feedback kick, as the walk left them
+            const int nk = P.bl_nbl * P.bl_nbins;
+            const double *w = P.bl_walk + (int64_t)b * nk * 2;
+            for (int j = tid; j < nk; j += kBlock) {
+                bl_lds[j] = w[2 * j];
+                bl_lds[nk + j] = w[2 * j + 1];
+            }
+            for (int q = tid; q < P.bl_nbl; q += kBlock)
+                bl_lds[2 * nk + q] = P.bl_fbk ? P.bl_fbk[(int64_t)b * P.bl_nbl + q] : 0.0;
+        } else if (ops & OP_BL) {
+            const int bt = (int)turn < P.bl_nturns ? (int)turn : P.bl_nturns - 1;
+            const double *blv = P.bl_tab + ((int64_t)bt * P.nb + b) * P.bl_nbl * 3;
             const int nk = P.bl_nbl * P.bl_nbins;
             for (int j = tid; j < nk; j += kBlock) {
                 const int q = j / P.bl_nbins, kb = j - q * P.bl_nbins;
@@ -255,6 +268,8 @@ k_pass(PassParams P)
             for (int q = tid; q < P.bl_nbl; q += kBlock)
                 bl_lds[nk + q] = blv[3 * q + 2] / 2.0 / P.bl_energy / P.bl_beta2;
         }
+        if (ops & OP_BLH)
+            for (int j = tid; j < P.bl_nbins; j += kBlock) reinterpret_cast<uint32_t *>(bl_lds)[j] = 0u;
         // the bunch's long-range wake kick of the previous turn (uniform)
         double lrk0 = 0, lrk1 = 0, lrk2 = 0;
         if (ops & OP_LRK) {
@@ -375,8 +390,20 @@ k_pass(PassParams P)
                 const int nk = P.bl_nbl * P.bl_nbins;
                 for (int q = 0; q < P.bl_nbl; q++) {
                     X[5] += bl_lds[q * P.bl_nbins + kb];
-                    X[5] += bl_lds[nk + q];
+                    X[5] += P.bl_walk ? bl_lds[nk + q * P.bl_nbins + kb] : bl_lds[nk + q];
                 }
+            }
+            if ((ops & OP_BL) && P.bl_fbmask) {
+                // the cavities' direct feedback, on every particle
+                // (Bunch::GetLongiKickDueToCavFB, src/Bunch.cpp:431-449)
+                const int nk = P.bl_nbl * P.bl_nbins;
+                for (int q = 0; q < P.bl_nbl; q++)
+                    if ((P.bl_fbmask >> q) & 1u) X[5] += bl_lds[2 * nk + q];
+            }
+            if ((ops & OP_BLH) && alive) {
+                int kb = (int)floor((bl_zmax - X[4]) / bl_dz);
+                kb = kb < 0 ? 0 : (kb >= P.bl_nbins ? P.bl_nbins - 1 : kb);
+                atomicAdd(&reinterpret_cast<uint32_t *>(bl_lds)[kb], 1u);
             }
             if (ops & OP_DRIFT) {
                 cdptr K = reload(P.cst);
@@ -475,6 +502,12 @@ k_pass(PassParams P)
             for (int j = tid; j < P.nbins; j += kBlock) {
                 const uint32_t c = hist_lds[j];
                 if (c) atomicAdd(&P.bunch_hist[(int64_t)b * P.nbins + j], c);
+            }
+        }
+        if (ops & OP_BLH) {
+            for (int j = tid; j < P.bl_nbins; j += kBlock) {
+                const uint32_t c = reinterpret_cast<uint32_t *>(bl_lds)[j];
+                if (c) atomicAdd(&P.bl_hist[(int64_t)b * P.bl_nbins + j], c);
             }
         }
         __syncthreads();
@@ -1060,6 +1093,7 @@ constexpr uint32_t V_REF = OP_KICK | OP_MOMB | OP_MAP | OP_ZSTAT | OP_SKEW | OP_
 // map (z range of the RF stage), then the turn's pass with the kick
 constexpr uint32_t V_BLA = OP_KICK | OP_MAP | OP_ZSTAT | OP_SKEW | OP_LRK;
 constexpr uint32_t V_BLB = V_FIX | V_REF | OP_BL;
+constexpr uint32_t V_BLH = OP_KICK | OP_MAP | OP_SKEW | OP_LRK | OP_BLH;  // bins for the cavity walk
 constexpr uint32_t V_ALL = 0xffffffffu;
 
 // stage sets of the fused turn in its steady state
@@ -1070,12 +1104,186 @@ constexpr uint32_t T_NOSR = OP_KICK | OP_MOMB | OP_MAP | OP_RF | OP_DRIFT | OP_U
 constexpr uint32_t T_QE_LR = T_QE | OP_LRC | OP_LRK;  // with the long-range wake
 constexpr uint32_t T_QE_BL = T_QE | OP_BL;            // a beam-loaded cavity in place of the ideal one
 
+// ---------------------------------------------------------- cavity walk
+// One turn of MPBeam::BeamMomtumUpdateDueToRFTest (src/MPBeam.cpp:2624-2706)
+// for cavities whose state depends on the particles: one wave per cavity
+// walks the train bunch by bunch.  At each bunch the lanes take its bins
+// (head first, over the survivors' z range after the map, padded as
+// GetZMinMax pads it): the stable variant's kick Re(V e^{-i k z}) with the
+// rigid self-induced V_b0 of the whole bunch
+// (MPBunch::BunchMomentumUpdateDueToRFModeStable, src/MPBunch.cpp:1323-1417),
+// or the bin-by-bin variant's (MPBunch::BunchMomentumUpdateDueToRFMode,
+// src/MPBunch.cpp:1419-1514): bin k sees the beam-induced voltage
+// V_k = D^k (V_start + sum_{m<k} V0_m D^-m), D = exp(-dt/t_F + 2 pi i f_r dt)
+// (the recurrence V <- (V + V0_m) D in closed form, the sum a prefix sum over
+// the lanes), plus the generator's voltage turned to the bin, and its own
+// charge's V0_k / 2.  Then the cavity sample at the bunch's bucket for the
+// direct feedback (Resonator::GetResonatorInfoAtNTrf, src/Resonator.cpp:
+// 263-275; Bunch::GetLongiKickDueToCavFB, src/Bunch.cpp:431-449), the
+// generator's dynamics bucket by bucket over the gap
+// (Resonator::ResonatorDynamics, src/Resonator.cpp:235-261) and the decay of
+// the beam-induced voltage to the next bunch (Resonator::GetBeamInducedVol,
+// src/Resonator.cpp:205-232): over bunchGap tRF in the stable variant, over
+// the time from this bunch's tail to the next one's head in the other
+// (MPBeam::GetTimeDisToNextBunch, src/MPBeam.cpp:2716-2752).
+namespace {
+
+struct cplx_d { double r, i; };
+__device__ __forceinline__ cplx_d cmul(cplx_d a, cplx_d b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+__device__ __forceinline__ cplx_d cadd(cplx_d a, cplx_d b) { return {a.r + b.r, a.i + b.i}; }
+// exp(-x) exp(i y), formed as the reference's double times complex exponential
+__device__ __forceinline__ cplx_d decay_turn(double x, double y)
+{
+    double sn, cs;
+    sincos(y, &sn, &cs);
+    const double e = exp(-x);
+    return {e * cs, e * sn};
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(64) k_bl_walk(BlWalkParams P)
+{
+    const int q = blockIdx.x, lane = threadIdx.x;
+    double *st = P.state + (int64_t)q * kBlS;
+    const bool exc = st[BS_EXC] != 0.0, dfb = st[BS_DFB] != 0.0;
+    const double harm = st[BS_HARM], rs = st[BS_RS], q0 = st[BS_Q0], ql = st[BS_QL], det = st[BS_DET];
+    const double fre = st[BS_FRE], tf = st[BS_TF];
+    const cplx_d req = {st[BS_REQ], st[BS_REQ + 1]}, ig = {st[BS_IG], st[BS_IG + 1]};
+    cplx_d vb = {st[BS_VB], st[BS_VB + 1]}, gen = {st[BS_GEN], st[BS_GEN + 1]};
+    const double pi = 3.14159265358979323846;
+    const double frf = 1.0 / P.trf;
+    const double c0 = -1 * 2 * pi * fre * rs / q0;  // V0 per electron charge
+    // the generator's step over one bucket (the same each time)
+    const double sigma = 2.0 * pi * fre / (2.0 * ql), dom = 2.0 * pi * det, tB = P.trf;
+    const cplx_d gstep = decay_turn(sigma * tB, dom * tB);
+    const double es = exp(-sigma * tB), sn_d = sin(dom * tB), cs_d = cos(dom * tB);
+    const double alpha = dom * es * sn_d - sigma * es * cs_d + sigma;
+    const double beta = sigma * es * sn_d + dom * es * cs_d - dom;
+    const double coefB = 2 * pi * fre / 2.0 * rs / q0 / (sigma * sigma + dom * dom);
+    const cplx_d gadd = {coefB * (alpha * ig.r + beta * ig.i), coefB * (-beta * ig.r + alpha * ig.i)};
+    const int ci = P.bl_index[q];
+    const int nbins = P.nbins;
+    const uint32_t turn = P.turn_dev ? *P.turn_dev + P.turn : P.turn;
+    double *info_turn = (int64_t)turn < P.nturns ? P.info + (int64_t)turn * P.nb * P.nres * 8 : nullptr;
+    for (int b = 0; b < P.nb; b++) {
+        const double zmin = P.zrange[2 * b] - 1.e-6, zmax = P.zrange[2 * b + 1] + 1.e-6;
+        const double dz = (zmax - zmin) / nbins, dtb = dz / kClight / P.rbeta;
+        uint32_t *hist = P.hist + (int64_t)b * nbins;
+        double *kick = P.kick + ((int64_t)b * P.nbl + q) * nbins * 2;
+        double sn_[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // n, cav re/im, gen re/im, induced re/im (sum n), self re
+        if (!exc) {
+            const double vb0 = c0 * P.ne[b] * kEcharge;
+            const cplx_d cav = cadd(vb, gen);
+            for (int k = lane; k < nbins; k += 64) {
+                const double zc = zmax - k * dz - dz / 2.0;
+                double sn, cs;
+                sincos(-zc * P.kfac[q], &sn, &cs);
+                kick[2 * k] = (cav.r * cs - cav.i * sn) / P.energy / P.beta2;
+                kick[2 * k + 1] = vb0 / 2.0 / P.energy / P.beta2;
+                const double n = (double)hist[k];
+                sn_[0] += n;
+                sn_[1] += cav.r * n; sn_[2] += cav.i * n;
+                sn_[3] += gen.r * n; sn_[4] += gen.i * n;
+                sn_[5] += (vb.r + vb0) * n; sn_[6] += vb.i * n;
+                sn_[7] += vb0 / 2.0 * n;
+            }
+            vb.r += vb0;
+        } else {
+            const double psi = 2.0 * pi * fre * dtb, lam = dtb / tf;
+            cplx_d carry = vb;  // V_start + the sum over the earlier chunks' bins
+            for (int k0 = 0; k0 < nbins; k0 += 64) {
+                const int k = k0 + lane;
+                const bool in = k < nbins;
+                const double n = in ? (double)hist[k] : 0.0;
+                const double v0 = c0 * P.mq[b] * n * kEcharge;
+                // t_k = V0_k D^-k, its exclusive prefix sum over the lanes
+                const cplx_d dinv = decay_turn(-(double)k * lam, -(double)k * psi);
+                cplx_d t = {v0 * dinv.r, v0 * dinv.i};
+                cplx_d inc = t;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const double ur = __shfl_up(inc.r, o, 64), ui = __shfl_up(inc.i, o, 64);
+                    if (lane >= o) { inc.r += ur; inc.i += ui; }
+                }
+                const cplx_d excl = {carry.r + inc.r - t.r, carry.i + inc.i - t.i};
+                const cplx_d dk = decay_turn((double)k * lam, (double)k * psi);
+                const cplx_d vbk = cmul(dk, excl);
+                if (in) {
+                    const double zc = zmax - k * dz - dz / 2.0;
+                    double sn, cs;
+                    sincos(-zc * P.kfac[q], &sn, &cs);
+                    const cplx_d g = {gen.r * cs - gen.i * sn, gen.r * sn + gen.i * cs};
+                    const cplx_d cav = cadd(vbk, g);
+                    kick[2 * k] = cav.r / P.energy / P.beta2;
+                    kick[2 * k + 1] = v0 / 2.0 / P.energy / P.beta2;
+                    sn_[0] += n;
+                    sn_[1] += cav.r * n; sn_[2] += cav.i * n;
+                    sn_[3] += g.r * n; sn_[4] += g.i * n;
+                    sn_[5] += (vbk.r + v0) * n; sn_[6] += vbk.i * n;
+                    sn_[7] += v0 / 2.0 * n;
+                }
+                carry.r += __shfl(inc.r, 63, 64);
+                carry.i += __shfl(inc.i, 63, 64);
+            }
+            // after the last bin: D^nbins (V_start + the whole sum)
+            vb = cmul(decay_turn((double)nbins * lam, (double)nbins * psi), carry);
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) sn_[j] = wave_sum(sn_[j]);
+        if (lane == 0 && info_turn) {
+            const double N = sn_[0];
+            double *f = info_turn + ((int64_t)b * P.nres + ci) * 8;
+            const double selfr = sn_[7] / N;
+            f[0] = sn_[1] / N; f[1] = sn_[2] / N;
+            f[2] = sn_[5] / N + (exc ? selfr : 0.0); f[3] = sn_[6] / N;
+            f[4] = sn_[3] / N; f[5] = sn_[4] / N;
+            f[6] = selfr; f[7] = 0.0;
+        }
+        // the cavity at the bunch's own bucket, read by the direct feedback
+        if (dfb) {
+            const double dt = -zmin / kClight;
+            const cplx_d vB = cmul(vb, decay_turn(dt / tf, 2.0 * pi * fre * dt));
+            const cplx_d dV = {-(vB.r + gen.r - req.r), -(vB.i + gen.i - req.i)};
+            const double a = sqrt(dV.r * dV.r + dV.i * dV.i) * st[BS_FBG];
+            const double ph = atan2(dV.i, dV.r) + st[BS_FBP] + st[BS_FBD] * fre * 2 * pi;
+            if (lane == 0) P.fb[(int64_t)b * P.nbl + q] = a * cos(ph) / P.energy / P.beta2;
+        }
+        // the generator over the bunch's gap, bucket by bucket
+        for (int k = 0; k < P.gap[b]; k++) gen = cadd(cmul(gen, gstep), gadd);
+        // the beam-induced voltage decayed and turned to the next bunch
+        const int nx = b + 1 < P.nb ? b + 1 : 0;
+        const double T = exc ? P.gap[b] * P.trf + (zmin - (P.zrange[2 * nx + 1] + 1.e-6)) / kClight / P.rbeta
+                             : P.gap[b] * P.trf;
+        vb = cmul(vb, decay_turn(T / tf, 2.0 * pi * fre * T));
+        // the bins are counted again next turn
+        for (int k = lane; k < nbins; k += 64) hist[k] = 0u;
+    }
+    if (lane == 0) {
+        st[BS_VB] = vb.r; st[BS_VB + 1] = vb.i;
+        st[BS_GEN] = gen.r; st[BS_GEN + 1] = gen.i;
+    }
+    (void)harm;
+    (void)frf;
+}
+
+hipError_t launch_bl_walk(const BlWalkParams &p, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_bl_walk, dim3(p.nbl), dim3(64), 0, st, p);
+    return hipGetLastError();
+}
+
 hipError_t launch_pass(const PassParams &p, int nwg, hipStream_t st)
 {
     const uint32_t o = p.ops;
-    const size_t shm = (o & OP_BL) ? ((size_t)p.nbins + (p.nbins + 1) / 2 + (size_t)p.bl_nbl * (p.bl_nbins + 1)) *
-                                         sizeof(double)
-                                   : (size_t)p.nbins * (sizeof(double) + sizeof(uint32_t));
+    // LDS: the wake's kicks and bins, then the cavities' per-bin kicks
+    // (walked: cavity and self-loss parts per bin, and the feedback kicks) or
+    // the walk's bin counts
+    size_t bl = 0;
+    if (o & OP_BL) bl = p.bl_walk ? (size_t)p.bl_nbl * (2 * p.bl_nbins + 1) : (size_t)p.bl_nbl * (p.bl_nbins + 1);
+    if (o & OP_BLH) bl = std::max(bl, (size_t)(p.bl_nbins + 1) / 2);
+    const size_t shm = (o & (OP_BL | OP_BLH)) ? ((size_t)p.nbins + (p.nbins + 1) / 2 + bl) * sizeof(double)
+                                              : (size_t)p.nbins * (sizeof(double) + sizeof(uint32_t));
     const dim3 g(nwg), b(kBlock);
     const bool one_rf = p.n_rf == 1;
     if (one_rf && o == T_QE && p.noise_mode == 2)
@@ -1090,6 +1298,7 @@ hipError_t launch_pass(const PassParams &p, int nwg, hipStream_t st)
         hipLaunchKernelGGL((k_pass<T_NOSR, true, 0, 1>), g, b, shm, st, p);
     else if ((o & ~V_KICK) == 0) hipLaunchKernelGGL((k_pass<V_KICK, false, -1, -1>), g, b, shm, st, p);
     else if ((o & OP_BL) && (o & ~V_BLB) == 0) hipLaunchKernelGGL((k_pass<V_BLB, false, -1, -1>), g, b, shm, st, p);
+    else if ((o & OP_BLH) && (o & ~V_BLH) == 0) hipLaunchKernelGGL((k_pass<V_BLH, false, -1, -1>), g, b, shm, st, p);
     else if ((o & OP_MAP) && (o & ~V_BLA) == 0)
         hipLaunchKernelGGL((k_pass<V_BLA, false, -1, -1>), g, b, shm, st, p);
     else if ((o & ~V_MOM) == 0) hipLaunchKernelGGL((k_pass<V_MOM, false, -1, -1>), g, b, shm, st, p);

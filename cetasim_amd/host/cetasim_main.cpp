@@ -205,8 +205,9 @@ int dump(const std::string &dir, ReadInputSettings &in, LatticeInterActionPoint 
                          b.cold[i], b.excite_instability[i], b.dir_fb[i]);
         std::fprintf(f, "],\n\"bl_real\": [");
         for (int i = 0; i < b.nres; i++)
-            std::fprintf(f, "%s[%.17g, %.17g, %.17g, %.17g, %.17g, %.17g]", i ? ", " : "", b.rs[i], b.q0[i],
-                         b.coupling[i], b.detune_hz[i], b.vol[i], b.phase[i]);
+            std::fprintf(f, "%s[%.17g, %.17g, %.17g, %.17g, %.17g, %.17g, %.17g, %.17g, %.17g]", i ? ", " : "",
+                         b.rs[i], b.q0[i], b.coupling[i], b.detune_hz[i], b.vol[i], b.phase[i], b.dir_fb_gain[i],
+                         b.dir_fb_phase[i], b.dir_fb_delay[i]);
         std::fprintf(f, "],\n");
         std::fprintf(f, "\"tbt_print\": %d, \"ring_current\": %.17g,\n", R->TBTBunchPrintNum,
                      in.ringParBasic->ringCurrent);

@@ -110,6 +110,9 @@ int ReadInputSettings::ParamReadFile(const std::string &path)
                 rf_.resCold.assign(rf_.resNum, 0);
                 rf_.resExciteIntability.assign(rf_.resNum, 0);
                 rf_.resDirFB.assign(rf_.resNum, 0);
+                rf_.resDirFBGain.assign(rf_.resNum, 0.0);
+                rf_.resDirFBPhase.assign(rf_.resNum, 0.0);
+                rf_.resDirFBDelay.assign(rf_.resNum, 0.0);
                 rf_.resShuntImpRs.assign(rf_.resNum, 0.0);
                 rf_.resQualityQ0.assign(rf_.resNum, 0.0);
                 rf_.resCouplingBeta.assign(rf_.resNum, 0.0);
@@ -127,6 +130,9 @@ int ReadInputSettings::ParamReadFile(const std::string &path)
             else if (k == "rfresexciteintability")
                 for (int i = 0; i < rf_.resNum; i++) rf_.resExciteIntability[i] = (int)num(v, i);
             else if (k == "rfresdirfb") for (int i = 0; i < rf_.resNum; i++) rf_.resDirFB[i] = (int)num(v, i);
+            else if (k == "rfresdirfbgain") for (int i = 0; i < rf_.resNum; i++) rf_.resDirFBGain[i] = num(v, i);
+            else if (k == "rfresdirfbphaseshift") for (int i = 0; i < rf_.resNum; i++) rf_.resDirFBPhase[i] = num(v, i);
+            else if (k == "rfresdirfbdelay") for (int i = 0; i < rf_.resNum; i++) rf_.resDirFBDelay[i] = num(v, i);
             else if (k == "rfbunchbinnum") rf_.rfBunchBinNum = (int)num(v, 0);
             else if (k == "rftransientcavparwriteto") rf_.transResonParWriteTo = v.at(0);
             // filling pattern (sized by fptrainnumber)
@@ -1159,6 +1165,9 @@ int MPBeam::RunSettings(const LatticeInterActionPoint &lat, const ReadInputSetti
             b.cold[i] = RF->resCold[i];
             b.excite_instability[i] = RF->resExciteIntability[i];
             b.dir_fb[i] = RF->resDirFB[i];
+            b.dir_fb_gain[i] = RF->resDirFBGain[i];
+            b.dir_fb_phase[i] = RF->resDirFBPhase[i];
+            b.dir_fb_delay[i] = RF->resDirFBDelay[i];
         }
         b.nbins = RF->rfBunchBinNum;
         b.nturns = R->nTurns;

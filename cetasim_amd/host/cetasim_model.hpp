@@ -41,6 +41,7 @@ public:
         std::vector<double> resVol, resPhase;
         // beam-loaded (rfMode 1) cavities (src/ReadInputSettings.cpp:194-368)
         std::vector<int> resType, resCold, resExciteIntability, resDirFB;
+        std::vector<double> resDirFBGain, resDirFBPhase, resDirFBDelay;
         std::vector<double> resShuntImpRs, resQualityQ0, resCouplingBeta, resDetuneFre;
         int rfBunchBinNum = 0;
         std::string transResonParWriteTo;

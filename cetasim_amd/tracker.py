@@ -107,7 +107,9 @@ class Tracker:
 
     def set_beam_loading(self, bl):
         """bl: dict(nres, mode, type, harm, cold, rs, q0, coupling, detune, vol,
-        phase, nbins, nturns), per-cavity lists; None switches it off."""
+        phase, nbins, nturns), per-cavity lists, and optionally excite,
+        dirfb, fb_gain, fb_phase, fb_delay (the bin-by-bin variant and the
+        direct feedback); None switches it off."""
         cfg = capi.BeamLoadingConfig()
         if bl is not None:
             cfg.nres = bl["nres"]
@@ -118,6 +120,11 @@ class Tracker:
                 cfg.cold[i] = bl["cold"][i]
                 cfg.rs[i], cfg.q0[i], cfg.coupling[i] = bl["rs"][i], bl["q0"][i], bl["coupling"][i]
                 cfg.detune_hz[i], cfg.vol[i], cfg.phase[i] = bl["detune"][i], bl["vol"][i], bl["phase"][i]
+                cfg.excite_instability[i] = bl.get("excite", [0] * bl["nres"])[i]
+                cfg.dir_fb[i] = bl.get("dirfb", [0] * bl["nres"])[i]
+                cfg.dir_fb_gain[i] = bl.get("fb_gain", [0.0] * bl["nres"])[i]
+                cfg.dir_fb_phase[i] = bl.get("fb_phase", [0.0] * bl["nres"])[i]
+                cfg.dir_fb_delay[i] = bl.get("fb_delay", [0.0] * bl["nres"])[i]
             cfg.nbins, cfg.nturns = bl["nbins"], bl["nturns"]
         capi.check(self.lib.cts_set_beam_loading(self.h, ctypes.byref(cfg)))
         self.bl_nres = cfg.nres

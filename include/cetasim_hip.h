@@ -176,8 +176,22 @@ int cts_get_lrwake_kick(cts_handle *h, double *kick);
  * particles in this model, so they are tabulated on the host for turns
  * 0 .. nturns-1.  The ring's rf_* fields keep the rfMode-0 cavities; this
  * call lists every cavity in input order (rfMode-0 entries only enter the
- * cavity set-up).  excite_instability and dir_fb must be 0 (the bin-by-bin
- * variant and the direct feedback are not built).  nres = 0 switches it off. */
+ * cavity set-up).  nres = 0 switches it off.
+ * With excite_instability = 1 on a cavity (rfResExciteIntability,
+ * MPBunch::BunchMomentumUpdateDueToRFMode, src/MPBunch.cpp:1419-1514) each
+ * bin's own charge induces V_b0 = -2 pi f_r R_s / Q_0 q n_bin e, the later
+ * bins of the bunch see it decayed and turned bin by bin, and the voltage
+ * decays to the next bunch over the time from this bunch's tail to the
+ * next one's head (MPBeam::GetTimeDisToNextBunch, src/MPBeam.cpp:2716-2752);
+ * with dir_fb = 1 (rfResDirFB, Bunch::GetLongiKickDueToCavFB,
+ * src/Bunch.cpp:431-449) every particle of a bunch, lost ones included,
+ * gets Re(g |dV| exp(i(arg dV + phase + 2 pi f_r delay))) / (E beta^2),
+ * dV = V_req - V_cav sampled at the bunch's bucket
+ * (Resonator::GetResonatorInfoAtNTrf, src/Resonator.cpp:263-275).  Either
+ * makes the cavity state depend on the particles: it then lives on the
+ * device and is walked bunch by bunch every turn (no turn limit; a second
+ * read-only pass bins the bunches over their z range first), and the
+ * cavity values of turns 0 .. nturns-1 are recorded as they run. */
 typedef struct {
     int32_t nres;
     int32_t rf_mode[CTS_MAX_RF];      /* 0 ideal, 1 beam loading */
@@ -190,10 +204,14 @@ typedef struct {
     double vol[CTS_MAX_RF];           /* rfResVol [V] */
     double phase[CTS_MAX_RF];         /* rfResPhase [rad] (single cavity: set from U0) */
     int32_t cold[CTS_MAX_RF];         /* rfResCold */
-    int32_t excite_instability[CTS_MAX_RF];
-    int32_t dir_fb[CTS_MAX_RF];
-    int32_t nbins;                    /* rfBunchBinNum; (rfMode-1 cavities) x (nbins + 1) <= 4096 */
-    int32_t nturns;
+    int32_t excite_instability[CTS_MAX_RF];  /* rfResExciteIntability */
+    int32_t dir_fb[CTS_MAX_RF];              /* rfResDirFB */
+    int32_t nbins;                    /* rfBunchBinNum; (rfMode-1 cavities) x (nbins + 1) <= 4096, x (2 nbins + 1)
+                                         when the state is walked on the device */
+    int32_t nturns;                   /* turns tabulated (stable model without feedback) */
+    double dir_fb_gain[CTS_MAX_RF];   /* rfResDirFBGain */
+    double dir_fb_phase[CTS_MAX_RF];  /* rfResDirFBPhase [rad] */
+    double dir_fb_delay[CTS_MAX_RF];  /* rfResDirFBDelay [s] */
 } cts_beam_loading_config;
 int cts_set_beam_loading(cts_handle *h, const cts_beam_loading_config *cfg);
 /* What bunch b saw from every cavity at turn t (the reference's

@@ -280,6 +280,12 @@ int orc_bl_tables(int nres, const int *ip, const double *dp, int harmonic, doubl
                   double rbeta, int nb, const double *ne, const int *gap, int nturns, double *dev, double *info);
 void orc_bl_kick(orc_bunch *b, double vr, double vi, double vb0, double zmin, double zmax, int nbins, int res_harm,
                  double f_rf, double rbeta, double energy);
+int orc_bl_state_size(void);
+void orc_bl_state(int nres, const int *ip, const double *dp, int harmonic, double f0, double u0, int nb,
+                  const double *ne, const int *gap, double *state);
+void orc_bl_walk(double *state, int nres, orc_bunch *bs, int nb, const int *gap, const double *ne,
+                 const double *macro_charge, int nbins, int harmonic, double f0, double t0, double rbeta,
+                 double energy, double *info, double *fb);
 
 /* Broadband impedance, longitudinal, dipolar and quadrupolar planes (bbimp.c);
  * orc_bbi_kick: planes[5], zre/zim [5][nimp] and wake [5][n] in the

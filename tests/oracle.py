@@ -89,6 +89,9 @@ def _bind(L):
     L.orc_bl_tables.restype = I
     L.orc_bl_tables.argtypes = [I, PIc, PD, I, D, D, D, D, I, PD, PIc, I, PD, PD]
     L.orc_bl_kick.argtypes = [ctypes.POINTER(OrcBunch), D, D, D, D, D, I, I, D, D, D]
+    L.orc_bl_state_size.restype = I
+    L.orc_bl_state.argtypes = [I, PIc, PD, I, D, D, I, PD, PIc, PD]
+    L.orc_bl_walk.argtypes = [PD, I, ctypes.POINTER(OrcBunch), I, PIc, PD, PD, I, I, D, D, D, D, PD, PD]
     L.orc_bbi_smooth.argtypes = [PD, I]
     L.orc_bbi_potential.argtypes = [PD, I, PD, PD, PD]
     L.orc_bbi_kick.argtypes = [ctypes.POINTER(OrcBunch), I, D, PIc, PD, PD, D, D, D, D, D, I, D, PD, PD]
@@ -239,6 +242,12 @@ class OracleTurn:
     def first_half(self, b, turn, noise=None):
         """Map, skew, RF, drift, U0, synrad; returns the post-map centroid
         (x, y, z) of the survivors (MPBeamRMSCal, src/MPBeam.cpp:1068)."""
+        cen = self.pre_rf(b)
+        self.post_rf(b, turn, noise)
+        return cen
+
+    def pre_rf(self, b):
+        """Map and skew; returns the post-map centroid."""
         L, R = self.L, self.ring
         L.orc_section_map(ctypes.byref(b.s), ctypes.byref(self.map))
         m = rms(b, self.sec)
@@ -251,6 +260,15 @@ class OracleTurn:
             # GetZMinMax at the RF stage (src/MPBeam.cpp:2637, 2716-2752)
             zmin, zmax = ctypes.c_double(), ctypes.c_double()
             L.orc_zminmax(ctypes.byref(b.s), ctypes.byref(zmin), ctypes.byref(zmax))
+            self._bl_z = (zmin, zmax)
+        return cen
+
+    def post_rf(self, b, turn, noise=None):
+        """The ideal cavities, the tabulated beam-loaded ones, drift, U0,
+        synrad."""
+        L, R = self.L, self.ring
+        if self.bl is not None:
+            zmin, zmax = self._bl_z
         for r in range(R.n_rf):
             L.orc_rf_kick(ctypes.byref(b.s), R.rf_vr[r], R.rf_vi[r], R.rf_harm[r], R.f0 * R.harmonic, R.rbeta,
                           R.energy)
@@ -267,7 +285,6 @@ class OracleTurn:
                 noise = qe_fill(self.noise_seed, turn, self.base, n)
             L.orc_synrad(ctypes.byref(b.s), p(self.b1h1), p(self.h1b1), p(self.tau), p(self.en),
                          None if noise is None else p(np.ascontiguousarray(noise).ravel()))
-        return cen
 
     def second_half(self, b):
         """Short-range wake and loss marking."""
@@ -343,6 +360,27 @@ class OracleTrain:
         self.L = lib()
         # beam loading: dict(nres, mode, type, harm, cold, rs, q0, coupling, detune, vol, phase, nbins, nturns)
         self.bl_dev = self.bl_info = None
+        self.bl_walk = None
+        if bl is not None and any(bl["mode"][i] == 1 and (bl.get("excite", [0] * bl["nres"])[i] or
+                                                        bl.get("dirfb", [0] * bl["nres"])[i])
+                                  for i in range(bl["nres"])):
+            # the cavity state depends on the particles: walked over the
+            # train every turn (orc_bl_walk)
+            nres = bl["nres"]
+            z0 = [0] * nres
+            ip = np.ascontiguousarray(np.concatenate([bl[k] for k in ("mode", "type", "harm", "cold")] +
+                                                     [bl.get("excite", z0), bl.get("dirfb", z0)]), np.int32)
+            dp = np.ascontiguousarray(np.concatenate(
+                [np.asarray(bl[k], float) for k in ("rs", "q0", "coupling", "detune", "vol", "phase")] +
+                [np.asarray(bl.get(k, [0.0] * nres), float) for k in ("fb_gain", "fb_phase", "fb_delay")]))
+            st = np.zeros(nres * self.L.orc_bl_state_size())
+            PIc = ctypes.POINTER(ctypes.c_int)
+            gi = np.ascontiguousarray(gap, np.int32)
+            self.L.orc_bl_state(nres, ip.ctypes.data_as(PIc), p(dp), ring.harmonic, ring.f0, ring.u0, len(counts),
+                                p(np.ascontiguousarray(n_e, float)), gi.ctypes.data_as(PIc), p(st))
+            self.bl_walk = dict(state=st, nres=nres, nbins=bl["nbins"],
+                                mq=np.ascontiguousarray(bl["macro_charge"], float))
+            bl = None
         if bl is not None:
             nb, nres = len(counts), bl["nres"]
             ip = np.ascontiguousarray(np.concatenate([bl[k] for k in ("mode", "type", "harm", "cold")]), np.int32)
@@ -413,6 +451,22 @@ class OracleTrain:
     def turn(self, bunches, turn):
         if self.cloud is not None:
             self.ion_phase(bunches, turn)
+        if self.bl_walk is not None:
+            # every bunch up to the skew quad, the cavities' walk over the
+            # train (MPBeam::BeamMomtumUpdateDueToRFTest), then the rest
+            for t, b in zip(self.turns, bunches):
+                t.pre_rf(b)
+            W, nb, R = self.bl_walk, len(bunches), self.ring
+            arr = (OrcBunch * nb)(*[b.s for b in bunches])
+            info, fb = np.zeros(nb * W["nres"] * 8), np.zeros(nb * W["nres"])
+            self.L.orc_bl_walk(p(W["state"]), W["nres"], arr, nb,
+                               self.gap.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), p(self.n_e), p(W["mq"]),
+                               W["nbins"], R.harmonic, R.f0, R.t0, R.rbeta, R.energy, p(info), p(fb))
+            self.bl_last_info = info.reshape(nb, W["nres"], 8)
+            self.bl_last_fb = fb.reshape(nb, W["nres"])
+            for t, b in zip(self.turns, bunches):
+                t.post_rf(b, turn)
+            return [t.second_half(b) for t, b in zip(self.turns, bunches)]
         if self.lrw is None:
             return [t.turn(b, turn) for t, b in zip(self.turns, bunches)]
         # every bunch up to synrad, then MPBeam::LRWakeBeamIntaction

@@ -70,6 +70,15 @@ def program_run_bl(tmp_path_factory):
     return run_program(tmp_path_factory.mktemp("run_bl"), BL_EDITS)
 
 
+BL_WALK_EDITS = dict(BL_EDITS, rfResExciteIntability="1", rfResDirFB="1", rfResDirFBGain="0.3",
+                     rfResDirFBPhaseShift="0.2", rfResDirFBDelay="1e-10")
+
+
+@pytest.fixture(scope="module")
+def program_run_bl_walk(tmp_path_factory):
+    return run_program(tmp_path_factory.mktemp("run_bl_walk"), BL_WALK_EDITS)
+
+
 def read_sdds_pages(path):
     """Data pages of an ASCII SDDS file as lists of token rows (header up to
     the &data line skipped; '! page number' lines start a page; parameter
@@ -107,6 +116,14 @@ def test_program_with_beam_loading_matches_capi_run(program_run_bl):
     against the library's cavity tables."""
     d, D = program_run_bl
     assert D["settings"]["bl"] == 1 and D["ring"].n_rf == 0
+    check_against_capi(d, D)
+
+
+def test_program_with_walked_cavity_matches_capi_run(program_run_bl_walk):
+    """The same with the cavity's bin-by-bin variant and its direct
+    feedback on (the state walked on the device every turn)."""
+    d, D = program_run_bl_walk
+    assert D["settings"]["bl_int"][0][4:] == [1, 1]
     check_against_capi(d, D)
 
 
@@ -149,8 +166,10 @@ def check_against_capi(d, D):
         nres = s["bl_nres"]
         bi, br = np.array(s["bl_int"]), np.array(s["bl_real"])
         tr.set_beam_loading(dict(nres=nres, mode=bi[:, 0], type=bi[:, 1], harm=bi[:, 2], cold=bi[:, 3],
-                                 rs=br[:, 0], q0=br[:, 1], coupling=br[:, 2], detune=br[:, 3], vol=br[:, 4],
-                                 phase=br[:, 5], nbins=s["bl_nbins"], nturns=s["bl_nturns"]))
+                                 excite=bi[:, 4], dirfb=bi[:, 5], rs=br[:, 0], q0=br[:, 1], coupling=br[:, 2],
+                                 detune=br[:, 3], vol=br[:, 4], phase=br[:, 5], fb_gain=br[:, 6],
+                                 fb_phase=br[:, 7], fb_delay=br[:, 8], nbins=s["bl_nbins"],
+                                 nturns=s["bl_nturns"]))
     # the program's turn loop with result.sdds on: each turn, then the
     # end-of-turn moments (the pending kicks applied), eigen-emittances and
     # ion charge

@@ -235,8 +235,16 @@ def test_rf_modes(tmp_path):
     s = dump["settings"]
     assert s["bl"] == 1 and s["bl_nres"] == 1 and s["bl_nbins"] == 60 and s["bl_nturns"] == s["nTurns"]
     assert s["bl_int"] == [[1, 1, 1, 0, 0, 0]]
-    assert s["bl_real"] == [[5.0e6, 2.0e4, 3.0, -2.0e4, 8.0e6, 1.0]]
+    assert s["bl_real"] == [[5.0e6, 2.0e4, 3.0, -2.0e4, 8.0e6, 1.0, 0.0, 0.0, 0.0]]
     assert dump["ring"].n_rf == 0
+    # the bin-by-bin variant and the direct feedback (src/ReadInputSettings.cpp:215-256)
+    H.copy_fixture(str(tmp_path), dict(BL, rfResExciteIntability="1", rfResDirFB="1", rfResDirFBGain="0.5",
+                                       rfResDirFBPhaseShift="0.1", rfResDirFBDelay="1e-9"))
+    r = H.run(["--dump", str(d), "input.dat"], str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    s = H.load_dump(str(d))["settings"]
+    assert s["bl_int"] == [[1, 1, 1, 0, 1, 1]]
+    assert s["bl_real"] == [[5.0e6, 2.0e4, 3.0, -2.0e4, 8.0e6, 1.0, 0.5, 0.1, 1e-9]]
     # an unknown mode is refused; a mixed pair keeps the ideal one in the ring
     H.copy_fixture(str(tmp_path), {"rfMode": "2"})
     r = H.run(["--dump", str(d), "input.dat"], str(tmp_path))
