@@ -85,7 +85,9 @@ class LrwakeConfig(ctypes.Structure):
 class BbimpConfig(ctypes.Structure):
     _fields_ = [("nimp", I32), ("f_max", D), ("zz_re", PD), ("zz_im", PD), ("flags", I32 * 5),
                 ("time_domain", I32), ("zqx_re", PD), ("zqx_im", PD), ("zqy_re", PD), ("zqy_im", PD),
-                ("beta_x", D), ("beta_y", D), ("zdx_re", PD), ("zdx_im", PD), ("zdy_re", PD), ("zdy_im", PD)]
+                ("beta_x", D), ("beta_y", D), ("zdx_re", PD), ("zdx_im", PD), ("zdy_re", PD), ("zdy_im", PD),
+                ("nwake", I32), ("wake_pos", PD), ("wake_z", PD), ("wake_dx", PD), ("wake_dy", PD),
+                ("wake_qx", PD), ("wake_qy", PD)]
 
 
 class BeamLoadingConfig(ctypes.Structure):

@@ -203,7 +203,7 @@ struct cts_handle {
     // the settings of the per-bunch effects above, deep copies kept so that
     // cts_set_bunches can set them up again for a new bunch layout
     struct KeepLrw { bool on = false; cts_lrwake_config cfg{}; std::vector<double> sectors, res; } keep_lrw;
-    struct KeepBbi { bool on = false; cts_bbimp_config cfg{}; std::vector<double> arr[10]; } keep_bbi;
+    struct KeepBbi { bool on = false; cts_bbimp_config cfg{}; std::vector<double> arr[16]; } keep_bbi;
     struct KeepBl { bool on = false; cts_beam_loading_config cfg{}; } keep_bl;
 
     // graph replays of two consecutive turns (cts_run_turns): the turn is
@@ -273,8 +273,8 @@ void free_bbi_state(cts_handle *h)
 BbiParams bbi_params(cts_handle *h)
 {
     BbiParams p{};
-    p.n = h->bbi.n; p.is = h->bbi.is; p.ie = h->bbi.ie; p.nb = h->nb;
-    p.zmin = h->bbi.zmin; p.dz = h->bbi.dz; p.bx = h->bbi.bx; p.by = h->bbi.by;
+    p.n = h->bbi.n; p.is = h->bbi.is; p.ie = h->bbi.ie; p.nb = h->nb; p.td = h->bbi.td; p.nw = h->bbi.nw;
+    p.zmin = h->bbi.zmin; p.dz = h->bbi.dz; p.half = h->bbi.half; p.bx = h->bbi.bx; p.by = h->bbi.by;
     for (int q = 0; q < 5; q++) p.planes[q] = h->bbi.planes[q];
     p.energy = h->ring.energy; p.beta2 = std::pow(h->ring.rbeta, 2);
     p.seg_begin = h->d_seg_begin; p.seg_end = h->d_seg_end;
@@ -1169,9 +1169,10 @@ int cts_set_bunches(cts_handle *h, int32_t nb, const int64_t *counts, const doub
     }
     if (keep_bbi.on) {
         cts_bbimp_config c = keep_bbi.cfg;
-        const double **dst[10] = {&c.zz_re,  &c.zz_im,  &c.zqx_re, &c.zqx_im, &c.zqy_re,
-                                  &c.zqy_im, &c.zdx_re, &c.zdx_im, &c.zdy_re, &c.zdy_im};
-        for (int i = 0; i < 10; i++) *dst[i] = keep_bbi.arr[i].empty() ? nullptr : keep_bbi.arr[i].data();
+        const double **dst[16] = {&c.zz_re,    &c.zz_im,   &c.zqx_re,  &c.zqx_im,  &c.zqy_re,  &c.zqy_im,
+                                  &c.zdx_re,   &c.zdx_im,  &c.zdy_re,  &c.zdy_im,  &c.wake_pos, &c.wake_z,
+                                  &c.wake_dx,  &c.wake_dy, &c.wake_qx, &c.wake_qy};
+        for (int i = 0; i < 16; i++) *dst[i] = keep_bbi.arr[i].empty() ? nullptr : keep_bbi.arr[i].data();
         if ((rc = cts_set_bbimp(h, &c))) return rc;
     }
     if (keep_bl.on && (rc = cts_set_beam_loading(h, &keep_bl.cfg))) return rc;
@@ -1888,22 +1889,28 @@ int cts_set_bbimp(cts_handle *h, const cts_bbimp_config *cfg)
     h->keep_bbi.on = false;
     if (cfg->flags[0] == 0 && cfg->flags[1] == 0 && cfg->flags[2] == 0 && cfg->flags[3] == 0 && cfg->flags[4] == 0)
         return CTS_OK;  // doTracking = 0 (src/MPBunch.cpp:2152-2157)
-    if (cfg->time_domain != 0)
-        return fail(CTS_ERR_ARG, "cts_set_bbimp: only the frequency-domain method (time_domain 0) is built");
     if (h->secs.size() != 1) return fail(CTS_ERR_STATE, "cts_set_bbimp: the fused turn (one section) only");
     std::string err;
-    if ((rc = bbimp_table(*cfg, h->ring, h->bbi, err))) return fail(rc, err);
+    if ((rc = bbimp_table(*cfg, h->ring, h->bbi, err))) {
+        h->bbi = BbiTable{};
+        return fail(rc, err);
+    }
     const int n = h->bbi.n, nb = h->nb;
     const bool dip = h->bbi.planes[1] || h->bbi.planes[2];
+    const size_t ng = h->bbi.g.size();
     if ((rc = dalloc(&h->d_bbi_hist, (size_t)nb * n)) || (rc = dalloc(&h->d_bbi_prof, 3 * (size_t)nb * n)) ||
-        (rc = dalloc(&h->d_bbi_wake, 5 * (size_t)nb * n)) || (rc = dalloc(&h->d_bbi_g, 5 * (size_t)n)) ||
+        (rc = dalloc(&h->d_bbi_wake, 5 * (size_t)nb * n)) || (rc = dalloc(&h->d_bbi_g, ng)) ||
         (rc = dalloc(&h->d_bbi_scale, (size_t)nb)) || (dip && (rc = dalloc(&h->d_bbi_dsum, 4 * (size_t)nb * n)))) {
         free_bbi_state(h);
         return rc;
     }
+    // charge per macroparticle: a current [C/s] per bin in the frequency
+    // domain (src/MPBunch.cpp:2166-2171), a charge [C] in the time domain
+    // (src/MPBunch.cpp:2053)
     std::vector<double> sc(nb);
-    for (int b = 0; b < nb; b++) sc[b] = h->macro_charge[b] * kEcharge / h->bbi.dz * kClight;
-    HIPCHK(hipMemcpy(h->d_bbi_g, h->bbi.g.data(), 5 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    for (int b = 0; b < nb; b++)
+        sc[b] = h->bbi.td ? h->macro_charge[b] * kEcharge : h->macro_charge[b] * kEcharge / h->bbi.dz * kClight;
+    HIPCHK(hipMemcpy(h->d_bbi_g, h->bbi.g.data(), ng * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->d_bbi_scale, sc.data(), nb * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(h->d_bbi_hist, 0, (size_t)nb * n * sizeof(uint32_t)));
     if (dip) HIPCHK(hipMemset(h->d_bbi_dsum, 0, 4 * (size_t)nb * n * sizeof(unsigned long long)));
@@ -1913,12 +1920,15 @@ int cts_set_bbimp(cts_handle *h, const cts_bbimp_config *cfg)
     cts_handle::KeepBbi k;
     k.on = true;
     k.cfg = *cfg;
-    const double *src[10] = {cfg->zz_re,  cfg->zz_im,  cfg->zqx_re, cfg->zqx_im, cfg->zqy_re,
-                             cfg->zqy_im, cfg->zdx_re, cfg->zdx_im, cfg->zdy_re, cfg->zdy_im};
-    for (int i = 0; i < 10; i++)
-        if (src[i]) k.arr[i].assign(src[i], src[i] + cfg->nimp);
+    const double *src[16] = {cfg->zz_re,   cfg->zz_im,   cfg->zqx_re,  cfg->zqx_im,   cfg->zqy_re, cfg->zqy_im,
+                             cfg->zdx_re,  cfg->zdx_im,  cfg->zdy_re,  cfg->zdy_im,   cfg->wake_pos, cfg->wake_z,
+                             cfg->wake_dx, cfg->wake_dy, cfg->wake_qx, cfg->wake_qy};
+    // only the tables of the chosen method are read (and kept)
+    for (int i = 0; i < 16; i++)
+        if (src[i] && (i < 10) == (h->bbi.td == 0)) k.arr[i].assign(src[i], src[i] + (i < 10 ? cfg->nimp : cfg->nwake));
     k.cfg.zz_re = k.cfg.zz_im = k.cfg.zqx_re = k.cfg.zqx_im = k.cfg.zqy_re = k.cfg.zqy_im = nullptr;
     k.cfg.zdx_re = k.cfg.zdx_im = k.cfg.zdy_re = k.cfg.zdy_im = nullptr;
+    k.cfg.wake_pos = k.cfg.wake_z = k.cfg.wake_dx = k.cfg.wake_dy = k.cfg.wake_qx = k.cfg.wake_qy = nullptr;
     h->keep_bbi = std::move(k);
     return CTS_OK;
 }

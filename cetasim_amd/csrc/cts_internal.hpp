@@ -252,22 +252,28 @@ struct LrwTable {
 int lrwake_table(const cts_lrwake_config &c, double t0, int harmonic, LrwTable &tab, std::string &err);
 
 // Broadband impedance (cts_bbi.hip): the grid, the real circular kernel of
-// Z and the smoothing filter, set up on the host.
-constexpr int kBbiTaps = 51;      // the Gaussian filter's window
-constexpr int kBbiMaxBins = 8191;  // 2 nimp - 1: the per-segment histogram in LDS (32 KB)
-constexpr int kBbiDipMaxBins = 1535;  // with the dipolar planes: counts and x, y limb sums in LDS (55 KB)
+// Z (frequency domain) or the wake table (time domain), and the smoothing
+// filter, set up on the host.
+constexpr int kBbiTaps = 51;          // the Gaussian filter's window
+constexpr int kBbiWin = 8192;         // bins of one LDS histogram window (32 KB of counts)
+constexpr int kBbiWinCopies = 2048;   // narrower ranges: one copy per wave in the same 32 KB
+constexpr int kBbiDipWin = 1535;      // with the dipolar planes: counts and x, y limb sums (55 KB)
+constexpr int kBbiMaxGridFD = 16383;  // 2 nimp - 1 (the host forms the kernel in O(n^2))
+constexpr int kBbiMaxGridTD = 65535;  // (int)(bucket / dz)
 struct BbiTable {
     int n = 0, is = 0, ie = 0;   // bins; smoothed range [is, ie)
+    int td = 0, nw = 0;          // time domain; rows of its wake table
     int planes[5] = {0, 0, 0, 0, 0};  // the reference's flags: Z, Dx, Dy, Qx, Qy
-    double zmin = 0, dz = 0, bx = 1, by = 1;
-    std::vector<double> g;       // [5][n] kernels: V = -(profile (*) g_z), transverse V = +(profile (*) g)
+    double zmin = 0, dz = 0, half = 0, bx = 1, by = 1;  // bin (int)((z - zmin + half)/dz)
+    std::vector<double> g;       // frequency domain [5][n] kernels: V = -(profile (*) g_z), transverse
+                                 // V = +(profile (*) g); time domain [5][nw] the wake table
     double taps[kBbiTaps];       // normalised Gaussian weights
 };
 int bbimp_table(const cts_bbimp_config &c, const cts_ring &ring, BbiTable &t, std::string &err);
 struct BbiParams {
-    int n, is, ie, nb;
+    int n, is, ie, nb, td, nw;
     int planes[5];
-    double zmin, dz, energy, beta2, bx, by;
+    double zmin, dz, half, energy, beta2, bx, by;
     const int64_t *seg_begin, *seg_end;
     const int32_t *seg_bunch, *wg_seg;
     const double *x, *y, *z;
@@ -275,10 +281,10 @@ struct BbiParams {
     const uint8_t *flag;
     uint32_t *hist;        // [nb][n], zero between turns
     unsigned long long *dsum;  // [nb][n][4] x, y limb sums (dipolar planes; else null), zero between turns
-    double *prof;          // [3][nb][n]: charge, x-weighted, y-weighted
+    double *prof;          // [3][nb][n]: charge, x-weighted, y-weighted (time domain: charge, mean x, mean y)
     double *wake;          // [5][nb][n], flag order
-    const double *g;       // [5][n]
-    const double *scale;   // [nb] macro charge e c / dz (0: the bunch is skipped)
+    const double *g;       // [5][n] kernels, or [5][nw] wake table
+    const double *scale;   // [nb] charge per macroparticle (frequency domain: times c / dz; 0: skipped)
     double taps[kBbiTaps];
 };
 hipError_t launch_bbi_bin(const BbiParams &p, int nwg, hipStream_t st);

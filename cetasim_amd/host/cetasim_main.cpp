@@ -196,6 +196,8 @@ int dump(const std::string &dir, ReadInputSettings &in, LatticeInterActionPoint 
         arr("bbi_dy_re", c.bbi_dy_re.data(), c.bbi_dy_re.size());
         arr("bbi_dy_im", c.bbi_dy_im.data(), c.bbi_dy_im.size());
         arr("bbi_beta", c.bbi_beta, 2);
+        const char *wk[6] = {"bbi_wpos", "bbi_wz", "bbi_wdx", "bbi_wdy", "bbi_wqx", "bbi_wqy"};
+        for (int i = 0; i < 6; i++) arr(wk[i], c.bbi_wake[i].data(), c.bbi_wake[i].size());
         const cts_beam_loading_config &b = c.blc;
         std::fprintf(f, "\"bl\": %d, \"bl_nres\": %d, \"bl_nbins\": %d, \"bl_nturns\": %d,\n", c.bl, b.nres,
                      b.nbins, b.nturns);

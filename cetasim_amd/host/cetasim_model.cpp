@@ -442,6 +442,52 @@ int WakeFunction::InitialLRWake(const ReadInputSettings &in, const LatticeInterA
 // ---------------------------------------------------------- distribution
 namespace {
 
+// The time-domain broadband impedance's wake file, as
+// MPBeam::GetQuasiWakePoten reads it (src/MPBeam.cpp:748-812): five header
+// lines, then rows z, wz, wDx, wDy, wQx, wQy; every wake times -1 (the
+// reference's sign convention), positions moved to start at 0, and the
+// longitudinal wake folded about the row whose z is exactly 0.
+int ReadQuasiWake(const std::string &path, std::vector<double> (&out)[6])
+{
+    std::ifstream f(path);
+    if (!f) {
+        std::cerr << "cannot open the wake file " << path << std::endl;
+        return 1;
+    }
+    for (auto &v : out) v.clear();
+    std::string line;
+    int cen = 0;
+    for (int ln = 0; std::getline(f, line); ln++) {
+        if (ln < 5) continue;
+        std::istringstream ss(line);
+        std::vector<double> col;
+        double x;
+        while (ss >> x) col.push_back(x);
+        if (col.empty()) continue;
+        if (col.size() < 6) {
+            std::cerr << path << ": row " << ln + 1 << " has " << col.size() << " columns, 6 needed" << std::endl;
+            return 1;
+        }
+        if (col[0] == 0) cen = (int)out[0].size();
+        out[0].push_back(col[0]);
+        for (int c = 1; c < 6; c++) out[c].push_back(-1 * col[c]);
+    }
+    const int n = (int)out[0].size();
+    if (n < 2 || 2 * cen > n) {
+        std::cerr << path << ": at least two rows, and the z = 0 row in the first half of the table" << std::endl;
+        return 1;
+    }
+    const double z0 = out[0][0];
+    for (double &z : out[0]) z -= z0;
+    const std::vector<double> t = out[1];
+    std::vector<double> &wz = out[1];
+    wz[0] = t[cen];
+    for (int i = 1; i < cen; i++) wz[i] = t[cen + i] + t[cen - i];
+    for (int i = cen; i < n - cen; i++) wz[i] = t[cen + i];
+    for (int i = n - cen; i < n; i++) wz[i] = 0.0;
+    return 0;
+}
+
 // Uniform [0, 1) and standard normal draws from a splitmix64 counter stream.
 struct SeededStream {
     uint64_t key, ctr = 0;
@@ -1096,8 +1142,21 @@ int MPBeam::RunSettings(const LatticeInterActionPoint &lat, const ReadInputSetti
     // broadband impedance: BoardBandImp::ReadInImp (src/BoardBandImp.cpp:42-70):
     // 5 header lines, then frequency and ten impedance columns per row;
     // Z_|| = column 2 - i column 7, Z_Qx = -column 10 - i column 5,
-    // Z_Qy = -column 11 - i column 6
-    if (R->bBImpFlag) {
+    // Z_Qy = -column 11 - i column 6.  In the time domain (the default) the
+    // wake file instead (MPBeam::GetQuasiWakePoten, src/MPBeam.cpp:748-812)
+    if (R->bBImpFlag && in.ringBBImp->timeDomain == 1) {
+        if (lat.sections.size() != 1) {
+            std::cerr << "broadband impedance: one interaction point (fused turn) only" << std::endl;
+            return 1;
+        }
+        const auto *B = in.ringBBImp;
+        if (ReadQuasiWake(B->wakeInput, c.bbi_wake)) return 1;
+        c.bbi = true;
+        for (int i = 0; i < 5; i++) c.bbi_flags[i] = B->impedSimFlag[i];
+        c.bbi_time_domain = 1;
+        c.bbi_beta[0] = lat.twissBetaX[0];
+        c.bbi_beta[1] = lat.twissBetaY[0];
+    } else if (R->bBImpFlag) {
         if (lat.sections.size() != 1) {
             std::cerr << "broadband impedance: one interaction point (fused turn) only" << std::endl;
             return 1;
@@ -1288,6 +1347,9 @@ int MPBeam::Run(Train &, LatticeInterActionPoint &lat, ReadInputSettings &in)
         bc.zdy_im = c.bbi_dy_im.data();
         bc.beta_x = c.bbi_beta[0];
         bc.beta_y = c.bbi_beta[1];
+        bc.nwake = (int32_t)c.bbi_wake[0].size();
+        const double **wk[6] = {&bc.wake_pos, &bc.wake_z, &bc.wake_dx, &bc.wake_dy, &bc.wake_qx, &bc.wake_qy};
+        for (int i = 0; i < 6; i++) *wk[i] = c.bbi_wake[i].empty() ? nullptr : c.bbi_wake[i].data();
         if (check(cts_set_bbimp(gpu, &bc), "broadband impedance")) return 1;
     }
     const int interval = std::max(R->bunchInfoPrintInterval, 1);

@@ -74,7 +74,9 @@ public:
     };
     struct RingBBImp {  // &BoardBandImpedance (src/ReadInputSettings.cpp:726-752)
         std::string impedInput, wakeInput;
-        int bunchBinNumberZ = 0, impedSimFlag[5] = {0, 0, 0, 0, 0}, timeDomain = 0;
+        // timeDomain 1 (tracking in the time domain) unless the input says otherwise
+        // (include/ReadInputSettings.h:213)
+        int bunchBinNumberZ = 0, impedSimFlag[5] = {0, 0, 0, 0, 0}, timeDomain = 1;
         double quasiWakeBunchLen = 0;
     };
     struct RingRun {
@@ -193,6 +195,7 @@ struct RunCalls {
     std::vector<double> bbi_re, bbi_im, bbi_qx_re, bbi_qx_im, bbi_qy_re, bbi_qy_im, bbi_dx_re, bbi_dx_im, bbi_dy_re,
         bbi_dy_im;
     double bbi_beta[2] = {1, 1};
+    std::vector<double> bbi_wake[6];  // time domain: binPosZ, wz, wDx, wDy, wQx, wQy (GetQuasiWakePoten)
     bool bl = false;  // rfMode-1 cavities present
     cts_beam_loading_config blc{};
 };

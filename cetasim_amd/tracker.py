@@ -154,6 +154,27 @@ class Tracker:
             cfg.time_domain = time_domain
         capi.check(self.lib.cts_set_bbimp(self.h, ctypes.byref(cfg)))
 
+    def set_bbimp_td(self, pos, wakes, beta=(1.0, 1.0)):
+        """Broadband impedance in the time domain (the reference's default,
+        BBImpBunchInteractionTD): pos the wake table's positions binPosZ,
+        wakes the five tables in the reference's flag order (wz, wDx, wDy,
+        wQx, wQy; None switches a plane off) as read_quasi_wake returns
+        them."""
+        cfg = capi.BbimpConfig()
+        a = lambda v: np.ascontiguousarray(v, np.float64)  # noqa: E731
+        self._bbi_w = [a(pos)] + [None if w is None else a(w) for w in wakes]
+        cfg.time_domain = 1
+        cfg.nwake = len(self._bbi_w[0])
+        cfg.wake_pos = capi.dptr(self._bbi_w[0])
+        for k, name in enumerate(("wake_z", "wake_dx", "wake_dy", "wake_qx", "wake_qy")):
+            cfg.flags[k] = int(self._bbi_w[1 + k] is not None)
+            if self._bbi_w[1 + k] is not None:
+                if len(self._bbi_w[1 + k]) != cfg.nwake:
+                    raise ValueError("set_bbimp_td: every wake table needs len(pos) rows")
+                setattr(cfg, name, capi.dptr(self._bbi_w[1 + k]))
+        cfg.beta_x, cfg.beta_y = beta
+        capi.check(self.lib.cts_set_bbimp(self.h, ctypes.byref(cfg)))
+
     def bbimp_wake(self):
         """(wake [5][nb][2 nimp - 1] in the flags' order: longitudinal,
         dipolar x, y, quadrupolar x, y; profiles [3][nb][2 nimp - 1]: charge,
@@ -285,6 +306,42 @@ class Tracker:
         out = np.empty(5 * n)
         capi.check(self.lib.cts_debug_qe_normals(self.h, seed, turn, base, n, capi.dptr(out)))
         return out.reshape(5, n)
+
+
+def read_quasi_wake(path, flags=(1, 1, 1, 1, 1)):
+    """The wake file of the time-domain broadband impedance as
+    MPBeam::GetQuasiWakePoten reads it (src/MPBeam.cpp:748-812): five
+    header lines, then rows z, wz, wDx, wDy, wQx, wQy; every wake times -1,
+    positions moved to start at 0, and the longitudinal wake folded about
+    the row whose z is exactly 0.  Returns (pos, [wz, wDx, wDy, wQx, wQy])
+    with None for the planes whose flag is 0."""
+    rows = []
+    with open(path) as f:
+        for ln, line in enumerate(f):
+            if ln < 5 or not line.split():
+                continue
+            v = [float(x) for x in line.split()]
+            if len(v) < 6:
+                raise ValueError(f"{path}: row {ln + 1} has {len(v)} columns, 6 needed")
+            rows.append(v[:6])
+    if len(rows) < 2:
+        raise ValueError(f"{path}: fewer than two wake rows")
+    r = np.array(rows)
+    pos = r[:, 0] - r[0, 0]
+    zero = np.flatnonzero(r[:, 0] == 0)
+    cen = int(zero[-1]) if len(zero) else 0
+    t = -1 * r[:, 1]
+    n = len(t)
+    if 2 * cen > n:
+        # the fold would read past the table's end
+        raise ValueError(f"{path}: the z = 0 row must lie in the first half of the table")
+    wz = np.zeros(n)
+    wz[0] = t[cen]
+    for i in range(1, cen):
+        wz[i] = t[cen + i] + t[cen - i]
+    wz[cen:max(cen, n - cen)] = t[2 * cen:max(2 * cen, n)]
+    wakes = [wz] + [-1 * r[:, c] for c in range(2, 6)]
+    return pos, [w if flags[k] else None for k, w in enumerate(wakes)]
 
 
 def moments_dict(m):

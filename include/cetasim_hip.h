@@ -238,9 +238,23 @@ int cts_get_cavity_info(cts_handle *h, int32_t turn, double *info);
  * times x summed per bin, smoothed the same way) with i Z_Dx, i Z_Dy and
  * kick px += V_Dx(bin) / (E beta^2 beta_x), py likewise, after the
  * quadrupolar kicks; the per-bin sums are two-limb fixed-point integers, so
- * they do not depend on the order of the atomics, and the grid is then
- * limited to 1535 bins (nimp <= 768).  The time-domain method is not
- * built; all flags 0 switches it off. */
+ * they do not depend on the order of the atomics.  The grid holds at most
+ * 16383 bins (nimp <= 8192).
+ * time_domain 1 (the reference's default; MPBunch::BBImpBunchInteractionTD,
+ * src/MPBunch.cpp:2029-2142, dispatched at src/MPBeam.cpp:1337): the
+ * quasi-green wake potential of MPBeam::GetQuasiWakePoten
+ * (src/MPBeam.cpp:748-812: the wake file's columns times -1, positions
+ * from 0, the longitudinal wake folded about z = 0) is passed as nwake rows
+ * wake_pos, wake_z, wake_dx, wake_dy, wake_qx, wake_qy (nimp, f_max and the
+ * impedance tables are not read).  Every turn each bunch's survivors are
+ * binned over one RF bucket, (int)((z + L/2)/dz) for the bucket length
+ * L = c T0/h and dz = |wake_pos[1] - wake_pos[0]|, on (int)(L/dz) <= 65535
+ * bins; the charge profile (charge per bin [C]) is smoothed by the same
+ * Gaussian filter; per bin the mean x and y of its particles; then
+ * V_i = -sum_{j >= i} w[j - i] rho_j (dipolar: rho_j <x>_j) over the
+ * wake's rows, and the kicks delta += V_z/(E beta^2), px += V_Dx/(E beta^2
+ * beta_x), px += V_Qx/(E beta^2 beta_x) x in that order (py likewise).
+ * All flags 0 switches the impedance off. */
 typedef struct {
     int32_t nimp;                     /* rows of the impedance file, >= 2 */
     double f_max;                     /* its last frequency [Hz] */
@@ -253,6 +267,11 @@ typedef struct {
     const double *zdx_re, *zdx_im;    /* [nimp] Z_Dx [Ohm/m] (flags[1]; the reference's zDxImp:
                                          -column 8 - i column 3) */
     const double *zdy_re, *zdy_im;    /* [nimp] Z_Dy (flags[2]; -column 9 - i column 4) */
+    int32_t nwake;                    /* time_domain 1: rows of the wake table, >= 2 */
+    const double *wake_pos;           /* [nwake] binPosZ [m] */
+    const double *wake_z;             /* [nwake] wz [V/C] (flags[0]) */
+    const double *wake_dx, *wake_dy;  /* [nwake] wDx, wDy [V/C/m] (flags[1], flags[2]) */
+    const double *wake_qx, *wake_qy;  /* [nwake] wQx, wQy [V/C/m] (flags[3], flags[4]) */
 } cts_bbimp_config;
 int cts_set_bbimp(cts_handle *h, const cts_bbimp_config *cfg);
 /* The induced voltages of every bunch from the last turn:
@@ -260,7 +279,9 @@ int cts_set_bbimp(cts_handle *h, const cts_bbimp_config *cfg);
  * quadrupolar x, y; bin k at z = -z_max + k dz; zeros for planes off), and
  * the smoothed profiles prof[3][nb][2 nimp - 1] if prof is not NULL (charge
  * [C/s], x- and y-weighted [C m/s]; zeros for dipolar planes off); n
- * receives 2 nimp - 1. */
+ * receives 2 nimp - 1.  Time domain: n = (int)(L/dz) bins from -L/2, the
+ * sums V of every plane in wake, and in prof the smoothed charge profile
+ * [C] and the mean x, y of each bin [m] (zeros for dipolar planes off). */
 int cts_get_bbimp_wake(cts_handle *h, int32_t *n, double *wake, double *prof);
 
 /* Radiation damping switch and quantum-excitation noise source

@@ -17,6 +17,20 @@
  * Z_D = -i R (V_D = +R I_x from the smoothed x-weighted profile, and the
  * px kick), and a bunch at one common offset x0, whose dipolar kick equals
  * its quadrupolar kick for the same impedance table.
+ *
+ * The time-domain method, MPBunch::BBImpBunchInteractionTD
+ * (src/MPBunch.cpp:2029-2142), is orc_bbi_td_kick, and the wake table's
+ * preparation, MPBeam::GetQuasiWakePoten (src/MPBeam.cpp:748-812), is
+ * orc_bbi_td_wake.  Pinned in tests/test_bbimp.py by a wake of one row
+ * (V = -w_0 rho, the smoothed profile itself), a wake of one row at lag d
+ * (V_i = -w_d rho_{i+d}: the sum runs over the bins ahead, j >= i), a
+ * constant wake (V_0 = -w Q, the bunch's total charge, and V decreasing
+ * to the tail), a single particle (rho is the normalised filter kernel
+ * times its charge, centred on bin (int)((z + L/2)/dz)), a bunch at one
+ * common offset x0 (the dipolar kick equals the quadrupolar kick of the
+ * same table, and the mean x of every occupied bin is x0), the grid size
+ * (int)(L/dz) and the fold of a causal wake (a shift) and of a symmetric
+ * one (doubled behind z = 0).
  */
 #include <math.h>
 #include <stdint.h>
@@ -171,5 +185,115 @@ void orc_bbi_kick(orc_bunch *b, int nimp, double f_max, const int *planes, const
     }
     free(xr);
     free(xi);
+    free(idx);
+}
+
+/* MPBeam::GetQuasiWakePoten (src/MPBeam.cpp:748-812) after the file is
+ * read: rows [n][6] = the file's columns z, wz, wDx, wDy, wQx, wQy; out
+ * [6][n] = binPosZ (z moved to start at 0) and the five wakes times -1, the
+ * longitudinal one then folded about the row whose z is exactly 0 (cen; 0
+ * when no row is): w[0] = t[cen], w[i] = t[cen + i] + t[cen - i] for
+ * 0 < i < cen, w[i] = t[cen + i] up to n - cen, zeros after.  Returns cen,
+ * or -1 when 2 cen > n (the reference's fold reads past the table). */
+int orc_bbi_td_wake(int n, const double *rows, double *out)
+{
+    if (n < 1) return -1;
+    int cen = 0;
+    for (int i = 0; i < n; i++) {
+        out[i] = rows[6 * (size_t)i];
+        for (int c = 1; c < 6; c++) out[(size_t)c * n + i] = -1 * rows[6 * (size_t)i + c];
+        if (rows[6 * (size_t)i] == 0) cen = i;
+    }
+    if (2 * cen > n) return -1; /* the fold would read past the table's end */
+    const double z0 = out[0];
+    for (int i = 0; i < n; i++) out[i] -= z0;
+    double *wz = out + n;
+    double *t = (double *)malloc(sizeof(double) * (size_t)n);
+    if (!t) return -1;
+    for (int i = 0; i < n; i++) t[i] = wz[i];
+    wz[0] = t[cen];
+    for (int i = 1; i < cen; i++) wz[i] = t[cen + i] + t[cen - i];
+    for (int i = cen; i < n - cen; i++) wz[i] = t[cen + i];
+    for (int i = n - cen; i < n; i++) wz[i] = 0.0;
+    free(t);
+    return cen;
+}
+
+/* bins of the time-domain grid: (int)(L/dz), L = T0/h c */
+int orc_bbi_td_bins(double t0, int harmonic, double dz)
+{
+    const double rf_len = t0 / harmonic * ORC_CLIGHT;
+    return (int)(1 * rf_len / dz);
+}
+
+/* One bunch's time-domain interaction.  The survivors are binned over one
+ * RF bucket, (int)((z + L/2)/dz) with dz = |pos[1] - pos[0]| (particles
+ * off the grid are skipped), the charge profile rho (a running sum of
+ * macro_charge e per particle) is smoothed over the whole grid, <x>_j and
+ * <y>_j are the bin's sums over its particles divided by their count; then
+ * for each bin i, over j = i, i + 1, ... while j - i < nw:
+ * V_z -= w_z[j - i] rho_j, V_Dx -= w_Dx[j - i] rho_j <x>_j, V_Dy likewise,
+ * V_Qx -= w_Qx[j - i] rho_j, V_Qy likewise, and the particles of bin i get
+ * delta += V_z/E/beta^2, px += V_Dx/E/beta^2/beta_x,
+ * py += V_Dy/E/beta^2/beta_y, px += V_Qx/E/beta^2/beta_x x,
+ * py += V_Qy/E/beta^2/beta_y y, in that order.  w [5][nw] in flag order;
+ * V [5][n] and prof [3][n] (rho, <x>, <y>) for n = orc_bbi_td_bins(). */
+void orc_bbi_td_kick(orc_bunch *b, int nw, const double *pos, const double *w, const int *planes, double bx,
+                     double by, double macro_charge, double t0, int harmonic, double rbeta, double energy, double *V,
+                     double *prof)
+{
+    const double rf_len = t0 / harmonic * ORC_CLIGHT;
+    const double dz = fabs(pos[1] - pos[0]);
+    const double zmin = -1 * rf_len / 2;
+    const int n = (int)(1 * rf_len / dz);
+    int *idx = (int *)malloc(sizeof(int) * (size_t)(b->n > 0 ? b->n : 1));
+    int *cnt = (int *)calloc((size_t)n, sizeof(int));
+    double *rho = prof, *ax = prof + n, *ay = prof + 2 * (size_t)n;
+    for (int i = 0; i < 3 * n; i++) prof[i] = 0;
+    for (int64_t i = 0; i < b->n; i++) {
+        idx[i] = -1;
+        if (b->lost[i] != 0) continue;
+        const double q = (b->z[i] - zmin) / dz;
+        if (!(q > -1.0 && q < (double)n)) continue;
+        idx[i] = (int)q;
+        rho[idx[i]] += macro_charge * ORC_ECHARGE;
+        cnt[idx[i]]++;
+    }
+    orc_bbi_smooth(rho, n);
+    for (int64_t i = 0; i < b->n; i++) {
+        if (idx[i] < 0) continue;
+        ax[idx[i]] += b->x[i];
+        ay[idx[i]] += b->y[i];
+    }
+    for (int j = 0; j < n; j++) {
+        if (cnt[j] == 0) continue;
+        ax[j] /= cnt[j];
+        ay[j] /= cnt[j];
+    }
+    const double *wz = w, *wdx = w + nw, *wdy = w + 2 * (size_t)nw, *wqx = w + 3 * (size_t)nw,
+                 *wqy = w + 4 * (size_t)nw;
+    for (int i = 0; i < 5 * n; i++) V[i] = 0;
+    for (int i = 0; i < n; i++) {
+        double t[5] = {0, 0, 0, 0, 0};
+        for (int j = i; j < n && j - i < nw; j++) {
+            const int tij = j - i;
+            if (planes[0]) t[0] -= wz[tij] * rho[j];
+            if (planes[1]) t[1] -= wdx[tij] * rho[j] * ax[j];
+            if (planes[2]) t[2] -= wdy[tij] * rho[j] * ay[j];
+            if (planes[3]) t[3] -= wqx[tij] * rho[j];
+            if (planes[4]) t[4] -= wqy[tij] * rho[j];
+        }
+        for (int q = 0; q < 5; q++) V[(size_t)q * n + i] = t[q];
+    }
+    for (int64_t i = 0; i < b->n; i++) {
+        const int k = idx[i];
+        if (k < 0) continue;
+        if (planes[0]) b->dp[i] += V[k] / energy / pow(rbeta, 2);
+        if (planes[1]) b->px[i] += V[n + k] / energy / pow(rbeta, 2) / bx;
+        if (planes[2]) b->py[i] += V[2 * (size_t)n + k] / energy / pow(rbeta, 2) / by;
+        if (planes[3]) b->px[i] += V[3 * (size_t)n + k] / energy / pow(rbeta, 2) / bx * b->x[i];
+        if (planes[4]) b->py[i] += V[4 * (size_t)n + k] / energy / pow(rbeta, 2) / by * b->y[i];
+    }
+    free(cnt);
     free(idx);
 }

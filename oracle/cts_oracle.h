@@ -295,6 +295,14 @@ void orc_bbi_potential(const double *p, int nimp, const double *zre, const doubl
 void orc_bbi_kick(orc_bunch *b, int nimp, double f_max, const int *planes, const double *zre, const double *zim,
                   double bx, double by, double macro_charge, double rbeta, double t0, int harmonic, double energy,
                   double *wake, double *prof);
+/* The time-domain method: the wake table from the file's rows [n][6]
+ * (out [6][n]: binPosZ, wz, wDx, wDy, wQx, wQy), the grid size and one
+ * bunch's interaction; w [5][nw] in flag order, V [5][n], prof [3][n]. */
+int orc_bbi_td_wake(int n, const double *rows, double *out);
+int orc_bbi_td_bins(double t0, int harmonic, double dz);
+void orc_bbi_td_kick(orc_bunch *b, int nw, const double *pos, const double *w, const int *planes, double bx,
+                     double by, double macro_charge, double t0, int harmonic, double rbeta, double energy, double *V,
+                     double *prof);
 
 int orc_cbm_analysis(const double *mean, int ns, int nb, const int *harm, int harmonics, const double *tunes,
                      const double *twiss, int interval, int fit_start, int fit_end, double t0,

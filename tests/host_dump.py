@@ -1,6 +1,7 @@
 """Helpers for the tracking program cetasim_amd/cetasim_run: running it on a
 copy of the input fixture and reading what its --dump option writes."""
 import json
+import math
 import os
 import shutil
 import subprocess
@@ -40,6 +41,28 @@ def impedance_file(nimp=261, f_max=1.0e11, rs=2.0e3, q=1.0, fr=2.0e10):
         zim.append(z.imag)
     text = "header\nfreq ReZ ...\nunits\n-----\n\n" + "\n".join(rows) + "\n"
     return text, zre, zim, f_max
+
+
+def wake_file(nrows=400, ahead=40, dz=5.0e-4, a=2.0e12, b=3.0e13, ell=0.02, lam=4.0e-3):
+    """A wake file in MPBeam::GetQuasiWakePoten's layout
+    (src/MPBeam.cpp:748-812): 5 header lines, then z, wz, wDx, wDy, wQx,
+    wQy per row, z = (i - ahead) dz (row `ahead` at z = 0 exactly).  The
+    wakes are smooth stand-ins for a quasi-green function: wz a Gaussian
+    ahead of z = 0 and a damped cosine behind, the transverse ones damped
+    sines behind (dipolar x, y at 1 and 0.6 of b, quadrupolar at 0.3 and
+    0.2), in the file's sign (the reader negates them).  Returns (text,
+    rows [nrows][6])."""
+    rows = []
+    for i in range(nrows):
+        z = (i - ahead) * dz
+        if z < 0:
+            wz, wt = a * math.exp(-0.5 * (z / (8 * dz)) ** 2), 0.0
+        else:
+            wz, wt = a * math.exp(-z / ell) * math.cos(z / lam), b * math.exp(-z / ell) * math.sin(z / lam)
+        rows.append([z, wz, wt, 0.6 * wt, 0.3 * wt, 0.2 * wt])
+    text = "SDDS1\nz wz wdx wdy wqx wqy\nunits\n-----\n\n" + "\n".join(
+        " ".join(repr(v) for v in r) for r in rows) + "\n"
+    return text, rows
 
 
 def copy_fixture(dst, edits=None, name="input.dat"):

@@ -95,6 +95,11 @@ def _bind(L):
     L.orc_bbi_smooth.argtypes = [PD, I]
     L.orc_bbi_potential.argtypes = [PD, I, PD, PD, PD]
     L.orc_bbi_kick.argtypes = [ctypes.POINTER(OrcBunch), I, D, PIc, PD, PD, D, D, D, D, D, I, D, PD, PD]
+    L.orc_bbi_td_wake.argtypes = [I, PD, PD]
+    L.orc_bbi_td_wake.restype = I
+    L.orc_bbi_td_bins.argtypes = [D, I, D]
+    L.orc_bbi_td_bins.restype = I
+    L.orc_bbi_td_kick.argtypes = [ctypes.POINTER(OrcBunch), I, PD, PD, PIc, D, D, D, D, I, D, D, PD, PD]
     L.orc_lrw_table.restype = I
     L.orc_lrw_table.argtypes = [I, I, D, I, PD, I, PD, D, D, I, PD, PD, PD, PD]
     L.orc_lrw_kick.restype = I
@@ -290,7 +295,24 @@ class OracleTurn:
         """Short-range wake and loss marking."""
         L, R = self.L, self.ring
         sig_stale = self._sig_stale
-        if self.bbi is not None:
+        if self.bbi is not None and self.bbi.get("td"):
+            # the time-domain method (src/MPBeam.cpp:1337-1340): bbi["w"] =
+            # the five wakes in flag order (None: plane off), bbi["pos"]
+            c = self.bbi
+            pos = np.ascontiguousarray(c["pos"], np.float64)
+            nw = len(pos)
+            n = L.orc_bbi_td_bins(R.t0, R.harmonic, abs(pos[1] - pos[0]))
+            w = np.ascontiguousarray(np.concatenate([np.zeros(nw) if t is None else np.asarray(t, np.float64)
+                                                     for t in c["w"]]))
+            planes = np.array([int(t is not None) for t in c["w"]], np.int32)
+            bx, by = c.get("beta", (1.0, 1.0))
+            self.bbi_wake, self.bbi_prof = np.zeros(5 * n), np.zeros(3 * n)
+            L.orc_bbi_td_kick(ctypes.byref(b.s), nw, p(pos), p(w), planes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                              bx, by, c["q"], R.t0, R.harmonic, R.rbeta, R.energy, p(self.bbi_wake),
+                              p(self.bbi_prof))
+            self.bbi_wake = self.bbi_wake.reshape(5, n)
+            self.bbi_prof = self.bbi_prof.reshape(3, n)
+        elif self.bbi is not None:
             # BBImpBeamInteraction, after the radiation stage (src/MPBeam.cpp:1084)
             c = self.bbi
             n, ni = 2 * c["nimp"] - 1, c["nimp"]

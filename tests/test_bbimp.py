@@ -160,3 +160,189 @@ def test_dipolar_equals_quadrupolar_at_a_common_offset():
         scale = np.abs(bq[c]).max()
         assert scale > 0
         np.testing.assert_allclose(bd[c], bq[c], rtol=0, atol=1e-12 * scale)
+
+
+# ---- the time-domain method (MPBunch::BBImpBunchInteractionTD) ----
+
+T0, H = 2e-6, 1000  # bucket L = T0/h c = 0.599584916 m
+
+
+def td_kick(b, w, dz=1e-3, bx=5.0, by=3.0, q=1e-15, rbeta=0.99999, E=6e9):
+    """orc_bbi_td_kick on one bunch; w {plane: table} in flag order, all
+    tables of one length.  Returns (V [5][n], prof [3][n])."""
+    import ctypes
+    nw = len(next(iter(w.values())))
+    tabs = np.zeros((5, nw))
+    pl = np.zeros(5, np.int32)
+    for k, t in w.items():
+        tabs[k] = t
+        pl[k] = 1
+    pos = np.arange(nw) * dz
+    L = O.lib()
+    n = L.orc_bbi_td_bins(T0, H, dz)
+    V, prof = np.zeros(5 * n), np.zeros(3 * n)
+    L.orc_bbi_td_kick(ctypes.byref(b.s), nw, O.p(pos), O.p(tabs.ravel()), pl.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                      bx, by, q, T0, H, rbeta, E, O.p(V), O.p(prof))
+    return V.reshape(5, n), prof.reshape(3, n)
+
+
+def filter_kernel():
+    x = (np.arange(51) - 25) / 25
+    k = np.exp(-0.5 * 100 * x ** 2)
+    return k / k.sum()
+
+
+def test_td_grid_and_single_particle():
+    """(int)(L/dz) bins from -L/2; one particle at z = 12.8 mm lands in bin
+    (int)((0.0128 + 0.299792458)/1e-3) = (int)312.59 = 312 (a centred grid
+    would give 313), its charge spread by the
+    filter kernel; a wake of one nonzero row w_0 gives V = -w_0 rho and
+    delta += V(312)/(E beta^2), nothing else moves."""
+    assert O.lib().orc_bbi_td_bins(T0, H, 1e-3) == 599
+    assert O.lib().orc_bbi_td_bins(T0, H, 2.5e-4) == 2398
+    a = {c: np.zeros(1) for c in O.COORDS}
+    a["z"][0], a["x"][0] = 0.0128, 1e-3
+    b = O.Bunch(a)
+    q, w0 = 3e-15, 2.0e12
+    w = np.zeros(10)
+    w[0] = w0
+    V, prof = td_kick(b, {0: w}, q=q)
+    assert V.shape == (5, 599)
+    rho = np.zeros(599)
+    rho[312 - 25:312 + 26] = q * ECHARGE * filter_kernel()
+    np.testing.assert_allclose(prof[0], rho, rtol=1e-13, atol=1e-40)
+    np.testing.assert_allclose(V[0], -w0 * rho, rtol=1e-13, atol=0)
+    assert np.all(V[1:] == 0)
+    np.testing.assert_allclose(b["dp"], V[0][312] / 6e9 / 0.99999 ** 2, rtol=1e-15)
+    assert np.all(b["px"] == 0) and np.all(b["py"] == 0)
+
+
+def test_td_sum_runs_over_the_bins_ahead():
+    """A wake whose only nonzero row is at lag d gives V_i = -w_d rho_{i+d}
+    (the sum over j >= i, src/MPBunch.cpp:2100-2104): a shift towards
+    lower bins; rows past the table's end add nothing."""
+    a, b = bunch_xyz(11)
+    d, wd = 7, 1.5e12
+    w = np.zeros(20)
+    w[d] = wd
+    V, prof = td_kick(b, {0: w})
+    rho = prof[0]
+    exp = np.zeros_like(rho)
+    exp[:-d] = -wd * rho[d:]
+    np.testing.assert_allclose(V[0], exp, rtol=1e-14, atol=0)
+    # a constant wake of 5 rows: V_i = -w (rho_i + ... + rho_{i+4})
+    a, b = bunch_xyz(12)
+    V, prof = td_kick(b, {0: np.full(5, 1.0e12)})
+    rho = prof[0]
+    exp = -1.0e12 * np.array([rho[i:i + 5].sum() for i in range(len(rho))])
+    np.testing.assert_allclose(V[0], exp, rtol=1e-13, atol=1e-13 * np.abs(exp).max())
+
+
+def test_td_constant_wake_sees_the_whole_charge():
+    """A constant wake longer than the grid: V_0 = -w Q, Q the bunch's
+    charge (the smoothing keeps the sum), V rises to 0 behind the tail."""
+    a, b = bunch_xyz(13, n0=5000)
+    q, wc = 2e-15, 3.0e12
+    V, prof = td_kick(b, {0: np.full(700, wc)}, q=q)
+    Q = 5000 * q * ECHARGE
+    np.testing.assert_allclose(V[0][0], -wc * Q, rtol=1e-13)
+    assert np.all(np.diff(V[0]) >= -1e-12 * wc * Q)
+    assert V[0][-1] == 0
+
+
+def lag_sums(w, f):
+    """-sum_{j >= i, j - i < len(w)} w[j - i] f_j for every i, by brute force."""
+    n = len(f)
+    return np.array([-sum(w[j - i] * f[j] for j in range(i, min(n, i + len(w)))) for i in range(n)])
+
+
+def test_td_dipolar_and_quadrupolar_planes():
+    """<x>_j is the mean x of bin j's particles (brute force here), zero in
+    empty bins; V_Dx = -sum w rho_j <x>_j over the bins ahead, so at one
+    common offset x0 it equals x0 times the quadrupolar sum over the
+    occupied bins only (the smoothed rho of an empty bin meets <x> = 0);
+    px += V_Dx(bin)/(E beta^2 beta_x) + V_Qx(bin)/(E beta^2 beta_x) x."""
+    a, b = bunch_xyz(14, x0=2e-4, y0=-1e-4)
+    w = 1e12 * np.exp(-np.arange(60) / 15.0) * np.sin(np.arange(60) / 4.0 + 0.3)
+    V, prof = td_kick(b, {1: w, 2: 0.5 * w})
+    k = ((a["z"] + 0.299792458) / 1e-3).astype(int)
+    cnt = np.bincount(k, minlength=599)
+    mx = np.bincount(k, weights=a["x"], minlength=599)
+    my = np.bincount(k, weights=a["y"], minlength=599)
+    occ = cnt > 0
+    np.testing.assert_allclose(prof[1][occ], mx[occ] / cnt[occ], rtol=1e-12)
+    np.testing.assert_allclose(prof[2][occ], my[occ] / cnt[occ], rtol=1e-12)
+    assert np.all(prof[1][~occ] == 0) and np.all(prof[2][~occ] == 0)
+    exp = lag_sums(w, prof[0] * prof[1])
+    np.testing.assert_allclose(V[1], exp, rtol=0, atol=1e-13 * np.abs(exp).max())
+    assert np.all(b["dp"] == 0)
+    x0, y0 = 3e-4, -2e-4
+    a, bq = bunch_xyz(15, x0=x0, y0=y0, sx=0, sy=0)
+    k = ((a["z"] + 0.299792458) / 1e-3).astype(int)
+    occ = np.bincount(k, minlength=599) > 0
+    V, pd = td_kick(bq, {1: w, 2: w, 3: 2 * w, 4: 3 * w}, bx=5.0, by=3.0)
+    np.testing.assert_allclose(pd[1][occ], x0, rtol=1e-14)
+    np.testing.assert_allclose(pd[2][occ], y0, rtol=1e-14)
+    for plane, c0 in ((1, x0), (2, y0)):
+        exp = c0 * lag_sums(w, pd[0] * occ)
+        np.testing.assert_allclose(V[plane], exp, rtol=0, atol=1e-13 * np.abs(exp).max())
+    E, rb = 6e9, 0.99999
+    np.testing.assert_allclose(bq["px"], V[1][k] / E / rb ** 2 / 5.0 + V[3][k] / E / rb ** 2 / 5.0 * x0, rtol=1e-14)
+    np.testing.assert_allclose(bq["py"], V[2][k] / E / rb ** 2 / 3.0 + V[4][k] / E / rb ** 2 / 3.0 * y0, rtol=1e-14)
+
+
+def test_td_lost_and_off_grid_particles():
+    """Lost particles and particles beyond the bucket's end (bin >= n) are
+    neither binned nor kicked."""
+    a, _ = bunch_xyz(16, n0=400)
+    a["z"][:3] = [0.3005, 0.31, -0.35]
+    lost = np.zeros(400, np.int32)
+    lost[3:10] = 1
+    b = O.Bunch(a, lost=lost)
+    w = np.zeros(8)
+    w[0] = 1e12
+    V, prof = td_kick(b, {0: w}, q=1e-15)
+    np.testing.assert_allclose(prof[0].sum(), 390 * 1e-15 * ECHARGE, rtol=1e-13)
+    assert np.all(b["dp"][:10] == 0) and np.all(b["dp"][10:] != 0)
+
+
+def wake_rows(z, cols):
+    return np.ascontiguousarray(np.column_stack([z] + list(cols)), dtype=float)
+
+
+def td_wake(rows):
+    n = len(rows)
+    out = np.zeros(6 * n)
+    cen = O.lib().orc_bbi_td_wake(n, O.p(rows.ravel()), O.p(out))
+    return cen, out.reshape(6, n)
+
+
+def test_td_wake_file_preparation():
+    """GetQuasiWakePoten's reading (src/MPBeam.cpp:748-812): columns times
+    -1, positions from 0, the longitudinal wake folded about the z = 0 row;
+    the transverse ones only negated.  Hand values: z = -2..3, wz = 1..6
+    gives cen 2 and wz' = (-3, -4 - 2, -5, -6, 0, 0)."""
+    z = np.arange(-2.0, 4.0)
+    cols = [np.arange(1.0, 7.0), 10 + np.arange(6.0), 20 + np.arange(6.0), 30 + np.arange(6.0), 40 + np.arange(6.0)]
+    cen, out = td_wake(wake_rows(z, cols))
+    assert cen == 2
+    np.testing.assert_array_equal(out[0], [0, 1, 2, 3, 4, 5])
+    np.testing.assert_array_equal(out[1], [-3, -6, -5, -6, 0, 0])
+    for c in range(2, 6):
+        np.testing.assert_array_equal(out[c], -cols[c - 1])
+    # a causal wake (zero ahead of z = 0) is only shifted to start at z = 0;
+    # an even one is doubled behind z = 0
+    z = np.arange(-4.0, 9.0) * 1e-4
+    causal = np.where(z >= 0, np.cos(z * 1e4), 0.0)
+    _, out = td_wake(wake_rows(z, [causal] + [np.zeros(13)] * 4))
+    np.testing.assert_array_equal(out[1][:9], -causal[4:])
+    assert np.all(out[1][9:] == 0)
+    even = np.exp(-(z * 1e4) ** 2)
+    _, out = td_wake(wake_rows(z, [even] + [np.zeros(13)] * 4))
+    np.testing.assert_allclose(out[1][1:4], -2 * even[5:8], rtol=1e-15)
+    assert out[1][0] == -even[4]
+    # no row at z = 0: cen 0, the table as read (negated)
+    z = np.arange(6.0) + 0.5
+    cen, out = td_wake(wake_rows(z, cols))
+    assert cen == 0
+    np.testing.assert_array_equal(out[1], -cols[0])

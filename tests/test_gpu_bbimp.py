@@ -5,8 +5,11 @@ five planes on bunches at transverse offsets), over 5 turns.  The device forms t
 induced voltage as a circular convolution with the impedance's real kernel,
 the oracle as the reference does (DFT, product, inverse DFT): voltages agree
 to 1e-10 of their largest value, profiles to 1e-12, coordinates to 1e-11 of
-each coordinate's scale as the other train tests.  Graph replays equal the
-turns run one by one."""
+each coordinate's scale as the other train tests.  The time-domain method
+(the reference's default) likewise, with a wake table of damped
+oscillations, and on a fine grid (10 um bins) where a bunch spans several
+LDS windows of the binning.  Graph replays equal the turns run one by
+one."""
 import numpy as np
 import pytest
 
@@ -104,14 +107,22 @@ def test_bbimp_turns_match_oracle(sr_wake, quad, dip):
 def test_bbimp_settings_are_checked():
     ring, secs, counts, gap, q, ne, _, tr, bunches, off = setup_train([2000], 3)
     zre, zim, fmax = impedance()
-    # a plane on without its table; the time-domain method; the dipolar
-    # planes on a grid above 1535 bins
-    zbig = np.zeros(800)
+    # a plane on without its table; the time domain without its wake table;
+    # a method other than 0 and 1; a grid above 16383 bins
+    zbig = np.zeros(8193)
     for args, kw in (((zre, zim, fmax), dict(flags=(1, 1, 0, 0, 0))), ((zre, zim, fmax), dict(time_domain=1)),
-                     ((zre, zim, fmax), dict(flags=(1, 0, 0, 1, 0))),
+                     ((zre, zim, fmax), dict(time_domain=2)), ((zre, zim, fmax), dict(flags=(1, 0, 0, 1, 0))),
                      ((zbig, zbig, fmax), dict(flags=(0, 1, 0, 0, 0), zdx=(zbig, zbig)))):
         with pytest.raises(RuntimeError):
             tr.set_bbimp(*args, **kw)
+    # time domain: one row; a step that cuts the bucket (0.6 m) into more
+    # than 65535 bins; a zero step
+    pos, wakes = td_table(nw=50)
+    for p_, w_ in ((pos[:1], [w[:1] for w in wakes]), (pos * 0.9e-3, wakes), (np.zeros(50), wakes)):
+        with pytest.raises(RuntimeError):
+            tr.set_bbimp_td(p_, w_, beta=(5.0, 3.0))
+    with pytest.raises(ValueError):
+        tr.set_bbimp_td(pos, [wakes[0][:10]] + [None] * 4)
     # a grid shorter than one RF bucket (z_max < c T_rf / 2)
     with pytest.raises(RuntimeError):
         tr.set_bbimp(zre[:50], zim[:50], fmax)
@@ -120,13 +131,17 @@ def test_bbimp_settings_are_checked():
     tr.close()
 
 
-@pytest.mark.parametrize("wake_mode", [1, None])
-def test_graph_replays_with_bbimp(wake_mode):
+@pytest.mark.parametrize("wake_mode,td", [(1, False), (None, False), (None, True)])
+def test_graph_replays_with_bbimp(wake_mode, td):
     e, g = make([3000, 5001, 2500, 1800], wake_mode, False)
     zre, zim, fmax = impedance()
+    pos, wakes = td_table()
     for tr in (e, g):
-        tr.set_bbimp(zre, zim, fmax)
-    print(f"graph case with broadband impedance, wake={wake_mode}", flush=True)
+        if td:
+            tr.set_bbimp_td(pos, wakes, beta=(5.0, 3.0))
+        else:
+            tr.set_bbimp(zre, zim, fmax)
+    print(f"graph case with broadband impedance, wake={wake_mode}, time domain {td}", flush=True)
     for t in range(9):
         e.turn(t)
     g.run_turns(0, 9)
@@ -137,3 +152,89 @@ def test_graph_replays_with_bbimp(wake_mode):
     same(e, g, 9, 13, "turns 9-13")
     e.close()
     g.close()
+
+
+def td_table(nw=400, dz=5.0e-4, a=-2.0e12, b=-3.0e13, ell=0.02, lam=4.0e-3):
+    """A quasi-green wake table as GetQuasiWakePoten leaves it (positions
+    from 0, step dz): wz a damped cosine, the transverse wakes damped sines
+    (dipolar x, y at 1 and 0.6 of b, quadrupolar at 0.3 and 0.2)."""
+    z = np.arange(nw) * dz
+    wz = a * np.exp(-z / ell) * np.cos(z / lam)
+    wt = b * np.exp(-z / ell) * np.sin(z / lam)
+    return z, [wz, wt, 0.6 * wt, 0.3 * wt, 0.2 * wt]
+
+
+def run_td(counts, seed, flags, pos, wakes, turns, sr_wake=False, x_offsets=False, stretch=None):
+    """Train through the device and the oracle with the time-domain
+    broadband impedance; compares V and the profiles every turn and the
+    coordinates at the end."""
+    ring, secs, counts, gap, q, ne, _, tr, bunches, off = setup_train(counts, seed)
+    if x_offsets or stretch:
+        parts = {c: np.concatenate([b[c] for b in bunches]) for c in O.COORDS}
+        for b in range(len(counts)):
+            if x_offsets:
+                parts["x"][off[b]:off[b + 1]] += 4e-5 * (b + 1)
+                parts["y"][off[b]:off[b + 1]] -= 1.5e-5 * (b + 1)
+            if stretch and b == 0:
+                parts["z"][off[b]:off[b + 1]] = np.random.default_rng(seed).uniform(-stretch, stretch, counts[b])
+        tr.upload(parts)
+        bunches = [O.Bunch({c: parts[c][off[b]:off[b + 1]] for c in O.COORDS}) for b in range(len(counts))]
+    beta = (secs[0].beta_x, secs[0].beta_y)
+    w = [wakes[k] if flags[k] else None for k in range(5)]
+    tr.set_bbimp_td(pos, w, beta=beta)
+    tr.set_synrad(True, 2, 8)
+    kw = {}
+    if sr_wake:
+        tr.set_wake(100, 1, -5 * SZ, 5 * SZ, BBR["rs"], BBR["q"], BBR["fr"])
+        kw = dict(nbins=100, mode=1, grid=(-5 * SZ, 5 * SZ), bbr=BBR, macro_charge=list(q))
+    bbi = dict(td=1, pos=pos, w=w, q=list(q), beta=beta)
+    ot = O.OracleTrain(ring, secs[0], counts, ne, gap, None, 1, bbi=bbi, synrad=True, noise_seed=8, **kw)
+    planes = [k for k in range(5) if flags[k]]
+    dip = flags[1] or flags[2]
+    for t in range(turns):
+        tr.turn(t)
+        ot.turn(bunches, t)
+        V, pr = tr.bbimp_wake()
+        for b in range(len(counts)):
+            oV, op = ot.turns[b].bbi_wake, ot.turns[b].bbi_prof
+            assert V.shape[2] == oV.shape[1]
+            for r in range(3 if dip else 1):
+                if r and not flags[r]:
+                    continue
+                np.testing.assert_allclose(pr[r][b], op[r], rtol=0, atol=1e-12 * np.max(np.abs(op[r])))
+            for k in range(5):
+                scale = np.max(np.abs(oV[k]))
+                if k in planes:
+                    assert scale > 0
+                else:
+                    assert scale == 0 and np.all(V[k][b] == 0)
+                np.testing.assert_allclose(V[k][b], oV[k], rtol=0, atol=1e-10 * scale)
+    got = tr.download()
+    assert not np.any(got["lost"])
+    compare_coords(got, bunches, off, 1e-11)
+    tr.close()
+    return V, ring
+
+
+@pytest.mark.parametrize("sr_wake,flags", [(False, (1, 0, 0, 0, 0)), (True, (1, 0, 0, 0, 0)),
+                                           (False, (1, 0, 0, 1, 1)), (False, (1, 1, 1, 1, 1))])
+def test_bbimp_td_turns_match_oracle(sr_wake, flags):
+    pos, wakes = td_table()
+    print(f"time-domain broadband impedance train, short-range wake {sr_wake}, planes {flags}", flush=True)
+    V, ring = run_td([3000, 7001, 1500, 4096], 13, flags, pos, wakes, 5, sr_wake=sr_wake,
+                     x_offsets=bool(flags[1] or flags[2]))
+    assert V.shape[2] == 1199  # (int)(0.599584916 / 5e-4)
+    # the kick is not negligible against the tolerance
+    assert np.max(np.abs(V)) / ring.energy > 1e-7
+
+
+@pytest.mark.parametrize("flags,stretch", [((1, 0, 0, 0, 0), 0.06), ((1, 1, 1, 0, 0), None)])
+def test_bbimp_td_fine_grid_spans_several_windows(flags, stretch):
+    """10 um bins: 59958 over the bucket.  A bunch stretched over +-60 mm
+    spans 12000 bins (two windows of the 8192-bin count histogram); with
+    the dipolar planes a bunch of sigma_z 3 mm spans ~2400 bins (two
+    windows of 1535)."""
+    pos, wakes = td_table(nw=2000, dz=1e-5, ell=4e-3, lam=1e-3)
+    print(f"time-domain fine grid, planes {flags}, stretch {stretch}", flush=True)
+    V, _ = run_td([6000, 3000, 2000], 17, flags, pos, wakes, 2, x_offsets=bool(flags[1]), stretch=stretch)
+    assert V.shape[2] == 59958

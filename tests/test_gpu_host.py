@@ -37,6 +37,7 @@ def run_program(d, edits=None):
     if edits:
         (d / "pipe.dat").write_text(PIPE)
         (d / "imp.dat").write_text(H.impedance_file()[0])
+        (d / "wake.dat").write_text(H.wake_file()[0])
     (d / "dump").mkdir()
     r = H.run(["--dump", "dump", "input.dat"], str(d))
     assert r.returncode == 0, r.stderr
@@ -57,12 +58,19 @@ def program_run_lrw(tmp_path_factory):
     return run_program(tmp_path_factory.mktemp("run_lrw"), LRW_EDITS)
 
 
-BBI_EDITS = {"runBBIFlag": "1", "BBIimpedInput": "imp.dat", "BBIimpedSimLandTFlag": "1 1 1 1 1"}
+BBI_EDITS = {"runBBIFlag": "1", "BBIimpedInput": "imp.dat", "BBIimpedSimLandTFlag": "1 1 1 1 1",
+             "BBIimpedSimTimeOrFreFlag": "0"}
+BBI_TD_EDITS = {"runBBIFlag": "1", "BBIWakeInput": "wake.dat", "BBIimpedSimLandTFlag": "1 1 1 1 1"}
 
 
 @pytest.fixture(scope="module")
 def program_run_bbi(tmp_path_factory):
     return run_program(tmp_path_factory.mktemp("run_bbi"), BBI_EDITS)
+
+
+@pytest.fixture(scope="module")
+def program_run_bbi_td(tmp_path_factory):
+    return run_program(tmp_path_factory.mktemp("run_bbi_td"), BBI_TD_EDITS)
 
 
 @pytest.fixture(scope="module")
@@ -136,6 +144,15 @@ def test_program_with_bbimp_matches_capi_run(program_run_bbi):
     check_against_capi(d, D)
 
 
+def test_program_with_time_domain_bbimp_matches_capi_run(program_run_bbi_td):
+    """The same with the time-domain broadband impedance (the reference's
+    default method; the wake file read as GetQuasiWakePoten does, all five
+    planes)."""
+    d, D = program_run_bbi_td
+    assert D["settings"]["bbi"] == 1 and D["settings"]["bbi_td"] == 1
+    check_against_capi(d, D)
+
+
 def check_against_capi(d, D):
     s = D["settings"]
     nb, interval, nturns = s["nb"], s["interval"], s["nTurns"]
@@ -157,7 +174,11 @@ def check_against_capi(d, D):
             row = [float(x) for x in rows[1 + i]]
             assert row[0] == g(-t[i])
             assert row[1:] == [g(v) for v in np.concatenate([rw[:, i], bbr[:, i], w[:, i]])]
-    if s.get("bbi"):
+    if s.get("bbi") and s["bbi_td"] == 1:
+        wk = [s[k] if s["bbi_flags"][i] else None
+              for i, k in enumerate(("bbi_wz", "bbi_wdx", "bbi_wdy", "bbi_wqx", "bbi_wqy"))]
+        tr.set_bbimp_td(s["bbi_wpos"], wk, beta=tuple(s["bbi_beta"]))
+    elif s.get("bbi"):
         tr.set_bbimp(s["bbi_re"], s["bbi_im"], s["bbi_fmax"], flags=s["bbi_flags"], time_domain=s["bbi_td"],
                      zqx=(s["bbi_qx_re"], s["bbi_qx_im"]), zqy=(s["bbi_qy_re"], s["bbi_qy_im"]),
                      zdx=(s["bbi_dx_re"], s["bbi_dx_im"]), zdy=(s["bbi_dy_re"], s["bbi_dy_im"]),

@@ -308,7 +308,45 @@ def test_bbimp_settings(tmp_path):
     np.testing.assert_array_equal(s["bbi_dy_re"], [-r[8] for r in rows])
     np.testing.assert_array_equal(s["bbi_dy_im"], [-r[3] for r in rows])
     assert s["bbi_beta"] == [2.71, 4.25]
-    # off by default; a missing file is an error
-    H.copy_fixture(str(tmp_path), {"runBBIFlag": "1", "BBIimpedInput": "none.dat"})
+    # a missing file is an error
+    H.copy_fixture(str(tmp_path), {"runBBIFlag": "1", "BBIimpedInput": "none.dat", "BBIimpedSimTimeOrFreFlag": "0"})
     r = H.run(["--dump", str(d), "input.dat"], str(tmp_path))
     assert r.returncode != 0 and "impedance file" in r.stderr
+
+
+def test_bbimp_time_domain_settings(tmp_path):
+    """BBIimpedSimTimeOrFreFlag defaults to 1 (include/ReadInputSettings.h:
+    213): the time domain, with the wake file BBIWakeInput read as
+    GetQuasiWakePoten does (src/MPBeam.cpp:748-812); the program's table
+    equals read_quasi_wake's and the oracle's restatement, bit for bit."""
+    import oracle as O
+    from cetasim_amd.tracker import read_quasi_wake
+    text, rows = H.wake_file()
+    (tmp_path / "wake.dat").write_text(text)
+    d = tmp_path / "dump"
+    d.mkdir()
+    H.copy_fixture(str(tmp_path), {"runBBIFlag": "1", "BBIWakeInput": "wake.dat",
+                                   "BBIimpedSimLandTFlag": "1 1 0 0 1"})
+    r = H.run(["--dump", str(d), "input.dat"], str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    s = H.load_dump(str(d))["settings"]
+    assert s["bbi"] == 1 and s["bbi_td"] == 1 and s["bbi_flags"] == [1, 1, 0, 0, 1]
+    assert s["bbi_re"] == []  # the impedance file is not read
+    pos, wakes = read_quasi_wake(str(tmp_path / "wake.dat"))
+    rr = np.ascontiguousarray(np.array([[float(repr(v)) for v in r] for r in rows]))
+    out = np.zeros(6 * len(rows))
+    assert O.lib().orc_bbi_td_wake(len(rows), O.p(rr.ravel()), O.p(out)) == 40
+    out = out.reshape(6, -1)
+    np.testing.assert_array_equal(pos, out[0])
+    np.testing.assert_array_equal(s["bbi_wpos"], out[0])
+    for k, name in enumerate(("bbi_wz", "bbi_wdx", "bbi_wdy", "bbi_wqx", "bbi_wqy")):
+        np.testing.assert_array_equal(wakes[k], out[1 + k])
+        np.testing.assert_array_equal(s[name], out[1 + k])
+    # pos starts at 0 with the file's step; wz behind z = 0 folds in the
+    # Gaussian ahead of it
+    assert pos[0] == 0 and abs(pos[1] - 5e-4) < 1e-18
+    assert wakes[0][0] == -rows[40][1] and wakes[0][1] == -(rows[41][1] + rows[39][1])
+    # a missing wake file is an error
+    H.copy_fixture(str(tmp_path), {"runBBIFlag": "1", "BBIWakeInput": "none.dat"})
+    r = H.run(["--dump", str(d), "input.dat"], str(tmp_path))
+    assert r.returncode != 0 and "wake file" in r.stderr
