@@ -238,3 +238,60 @@ def test_bbimp_td_fine_grid_spans_several_windows(flags, stretch):
     print(f"time-domain fine grid, planes {flags}, stretch {stretch}", flush=True)
     V, _ = run_td([6000, 3000, 2000], 17, flags, pos, wakes, 2, x_offsets=bool(flags[1]), stretch=stretch)
     assert V.shape[2] == 59958
+
+
+@pytest.mark.parametrize("td", [False, True])
+def test_bbimp_with_lost_and_off_grid_particles(td):
+    """All five planes with particles lost at the first turn's marking (60
+    beyond the aperture: binned and kicked in that turn, in no later one)
+    and 25 off the grid (time domain: past the bucket's last bin, which ends
+    8.5e-5 m short of L/2; frequency domain: beyond z_max, lost at the same
+    marking): neither binned nor kicked, on the device as in the oracle."""
+    counts = [3000, 4000, 2500]
+    ring, secs, counts, gap, q, ne, _, tr, bunches, off = setup_train(counts, 23)
+    parts = {c: np.concatenate([b[c] for b in bunches]) for c in O.COORDS}
+    for b in range(len(counts)):
+        parts["x"][off[b]:off[b + 1]] += 4e-5 * (b + 1)
+    parts["x"][off[0]:off[0] + 60] = 0.05
+    if td:
+        parts["z"][off[1]:off[1] + 25] = 0.29975
+        parts["dp"][off[1]:off[1] + 25] = 0.0
+    else:
+        parts["z"][off[1]:off[1] + 25] = -0.4
+    tr.upload(parts)
+    bunches = [O.Bunch({c: parts[c][off[b]:off[b + 1]] for c in O.COORDS}) for b in range(len(counts))]
+    beta = (secs[0].beta_x, secs[0].beta_y)
+    if td:
+        pos, wakes = td_table()
+        tr.set_bbimp_td(pos, wakes, beta=beta)
+        bbi = dict(td=1, pos=pos, w=wakes, q=list(q), beta=beta)
+    else:
+        zre, zim, fmax = impedance()
+        qx, qy = quad_impedance(), quad_impedance(rs=1.5e3, fr=1.2e10)
+        dx, dy = quad_impedance(rs=2.5e3, fr=1.8e10), quad_impedance(rs=0.8e3, fr=1.0e10)
+        tr.set_bbimp(zre, zim, fmax, flags=(1, 1, 1, 1, 1), zqx=qx, zqy=qy, zdx=dx, zdy=dy, beta=beta)
+        bbi = dict(nimp=len(zre), f_max=fmax, zre=zre, zim=zim, q=list(q), beta=beta, zqx=qx, zqy=qy, zdx=dx, zdy=dy)
+    tr.set_synrad(True, 2, 8)
+    ot = O.OracleTrain(ring, secs[0], counts, ne, gap, None, 1, bbi=bbi, synrad=True, noise_seed=8)
+    print(f"broadband impedance with lost and off-grid particles, time domain {td}", flush=True)
+    for t in range(3):
+        tr.turn(t)
+        ot.turn(bunches, t)
+        w, pr = tr.bbimp_wake()
+        for b in range(len(counts)):
+            ow, op = ot.turns[b].bbi_wake, ot.turns[b].bbi_prof
+            for r in range(3):
+                np.testing.assert_allclose(pr[r][b], op[r], rtol=0, atol=1e-12 * np.max(np.abs(op[r])))
+            for k in range(5):
+                scale = np.max(np.abs(ow[k]))
+                assert scale > 0
+                np.testing.assert_allclose(w[k][b], ow[k], rtol=0, atol=1e-10 * scale)
+    got = tr.download()
+    lost = got["lost"]
+    assert np.all(lost[off[0]:off[0] + 60] != 0)
+    if td:
+        assert not np.any(lost[off[1]:off[1] + 25])
+    else:
+        assert np.all(lost[off[1]:off[1] + 25] != 0)
+    compare_coords(got, bunches, off, 1e-11)
+    tr.close()
