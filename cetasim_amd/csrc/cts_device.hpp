@@ -210,6 +210,25 @@ __device__ __forceinline__ cdptr reload(uint64_t a)
     return (cdptr)a;
 }
 
+// a * b + c with c a scalar-register operand, as one v_fma_f64 (the same
+// fused rounding as fma()).  Left to itself the compiler forms a Horner
+// step with a constant addend as two moves of c into the accumulator and a
+// v_fmac_f64: three vector instructions for one.
+__device__ __forceinline__ double fma_vvs(double a, double b, double c)
+{
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+    return r;
+}
+
+// -(a * b) + c, likewise (the negation as the instruction's modifier)
+__device__ __forceinline__ double fnma_vvs(double a, double b, double c)
+{
+    double r;
+    asm("v_fma_f64 %0, -%1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
+    return r;
+}
+
 __device__ __forceinline__ void spmv_k(cdptr A, const double x[6], double y[6])
 {
 #pragma clang fp contract(fast)
