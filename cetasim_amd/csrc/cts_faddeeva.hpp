@@ -395,7 +395,11 @@ __device__ __forceinline__ bool w_kick_region(double x, double y)
 __device__ __forceinline__ void w_weideman2(const double zr[2], const double zi[2], double wr[2], double wi[2])
 {
 #pragma clang fp contract(fast)
-    const double L = c_weid[kWeidN];
+    // the coefficients through an opaque pointer: their scalar loads stay in
+    // the step loop instead of being hoisted out of the caller's loops and
+    // kept (and spilled) in scalar registers
+    const cdptr cw = reload((uint64_t)(uintptr_t)c_weid);
+    const double L = cw[kWeidN];
     double ur[2], ui[2], Zr[2], Zi[2], pr[2], pi[2];
 #pragma unroll
     for (int q = 0; q < 2; q++) {
@@ -406,7 +410,7 @@ __device__ __forceinline__ void w_weideman2(const double zr[2], const double zi[
         const double nr = L - zi[q], ni = zr[q];
         Zr[q] = nr * ur[q] - ni * ui[q];
         Zi[q] = nr * ui[q] + ni * ur[q];
-        pr[q] = c_weid[0];
+        pr[q] = cw[0];
         pi[q] = 0.0;
     }
     // p <- p Z + c_k: the coefficient enters as the addend of
@@ -414,7 +418,7 @@ __device__ __forceinline__ void w_weideman2(const double zr[2], const double zi[
     // instructions per point instead of five
 #pragma unroll 8
     for (int k = 1; k < kWeidN; k++) {
-        const double ck = c_weid[k];
+        const double ck = cw[k];
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             const double t = fma(pr[q], Zr[q], fnma_vvs(pi[q], Zi[q], ck));
