@@ -390,9 +390,10 @@ __device__ __forceinline__ bool w_kick_region(double x, double y)
     return x < 100.0 && y < 1000.0;
 }
 
-// Weideman's approximation at two points, the two Horner recurrences
+// Weideman's approximation at NP points, their Horner recurrences
 // interleaved.
-__device__ __forceinline__ void w_weideman2(const double zr[2], const double zi[2], double wr[2], double wi[2])
+template <int NP>
+__device__ __forceinline__ void w_weideman(const double *zr, const double *zi, double *wr, double *wi)
 {
 #pragma clang fp contract(fast)
     // the coefficients through an opaque pointer: their scalar loads stay in
@@ -400,9 +401,9 @@ __device__ __forceinline__ void w_weideman2(const double zr[2], const double zi[
     // kept (and spilled) in scalar registers
     const cdptr cw = reload((uint64_t)(uintptr_t)c_weid);
     const double L = cw[kWeidN];
-    double ur[2], ui[2], Zr[2], Zi[2], pr[2], pi[2];
+    double ur[NP], ui[NP], Zr[NP], Zi[NP], pr[NP], pi[NP];
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
+    for (int q = 0; q < NP; q++) {
         const double dr = L + zi[q], di = -zr[q];
         const double inv = 1.0 / (dr * dr + di * di);
         ur[q] = dr * inv;
@@ -420,14 +421,14 @@ __device__ __forceinline__ void w_weideman2(const double zr[2], const double zi[
     for (int k = 1; k < kWeidN; k++) {
         const double ck = cw[k];
 #pragma unroll
-        for (int q = 0; q < 2; q++) {
+        for (int q = 0; q < NP; q++) {
             const double t = fma(pr[q], Zr[q], fnma_vvs(pi[q], Zi[q], ck));
             pi[q] = pr[q] * Zi[q] + pi[q] * Zr[q];
             pr[q] = t;
         }
     }
 #pragma unroll
-    for (int q = 0; q < 2; q++) {
+    for (int q = 0; q < NP; q++) {
         const double u2r = ur[q] * ur[q] - ui[q] * ui[q], u2i = 2.0 * ur[q] * ui[q];
         wr[q] = 2.0 * (pr[q] * u2r - pi[q] * u2i) + kIsPi * ur[q];
         wi[q] = 2.0 * (pr[q] * u2i + pi[q] * u2r) + kIsPi * ui[q];
@@ -469,7 +470,11 @@ __device__ __forceinline__ void be_field(double x, double y, const BeField &F, d
     zi[1] = zi[0] * F.ir;
     // Weideman's approximation for both points; a point outside the kick
     // region (rare: far from the beam) takes the far-field form instead
-    w_weideman2(zr, zi, wr, wi);
+    // one point after the other: four waves per SIMD interleave enough
+    // work, and the single recurrence needs fewer registers (fewer spills
+    // in the ion turn's passage loop) than the two side by side
+#pragma nounroll
+    for (int q = 0; q < 2; q++) w_weideman<1>(zr + q, zi + q, wr + q, wi + q);
 #pragma unroll
     for (int q = 0; q < 2; q++)
         if (!w_kick_region(zr[q], zi[q])) w_far(zr[q], zi[q], wr[q], wi[q]);
