@@ -92,6 +92,7 @@ struct cts_handle {
     cts_ring ring{};
     std::vector<cts_section> secs;
     int dev = 0;
+    int ncu = 1;  // compute units of the device
     hipStream_t stream = nullptr;
 
     // bunches
@@ -108,6 +109,7 @@ struct cts_handle {
     int tile = 0;  // 0: contiguous chunk per workgroup; > 0: tiles dealt round-robin
     int64_t *d_seg_begin = nullptr, *d_seg_end = nullptr, *d_bunch_off = nullptr, *d_bunch_count = nullptr;
     int32_t *d_seg_bunch = nullptr, *d_wg_seg = nullptr, *d_bunch_seg = nullptr, *d_seg_rank = nullptr;
+    int32_t *d_seg_order = nullptr;  // inverse of d_seg_rank: the segment at each place in particle order
     double *d_seg_part = nullptr;
     double *d_chunk_part = nullptr;
     uint32_t *d_chunk_cnt = nullptr;
@@ -298,11 +300,11 @@ void free_bunch_state(cts_handle *h)
                     h->d_cur, h->d_hist, h->d_sigz, h->d_zrange, h->d_noise, h->d_ion_e, h->d_ion_bview,
                     h->d_bhist, h->d_counts, h->d_kick, h->d_poten, h->d_edges_prev,
                     h->d_edges_now, h->d_table, h->d_scale, h->d_chunk_part, h->d_chunk_cnt,
-                    h->d_fb_count, h->d_fb_list, h->d_eig, h->d_seg_rank};
+                    h->d_fb_count, h->d_fb_list, h->d_eig, h->d_seg_rank, h->d_seg_order};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     h->d_eig = nullptr;
-    h->d_seg_rank = nullptr;
+    h->d_seg_rank = h->d_seg_order = nullptr;
     h->eig_cap = 0;
     h->d_x = h->d_px = h->d_y = h->d_py = h->d_z = h->d_dp = nullptr;
     h->d_flag = nullptr;
@@ -784,21 +786,25 @@ int layout_segments(cts_handle *h, int wg_target, int tile)
     int max_chunks = 1;
     for (int bb = 0; bb < nb; bb++) max_chunks = std::max(max_chunks, (bseg[bb + 1] - bseg[bb] + 63) / 64);
     void *old[] = {h->d_seg_begin, h->d_seg_end, h->d_seg_bunch, h->d_wg_seg, h->d_bunch_seg, h->d_seg_part,
-                   h->d_chunk_part, h->d_seg_rank};
+                   h->d_chunk_part, h->d_seg_rank, h->d_seg_order};
     for (void *p : old)
         if (p) (void)hipFree(p);
     h->d_seg_begin = h->d_seg_end = nullptr;
-    h->d_seg_bunch = h->d_wg_seg = h->d_bunch_seg = h->d_seg_rank = nullptr;
+    h->d_seg_bunch = h->d_wg_seg = h->d_bunch_seg = h->d_seg_rank = h->d_seg_order = nullptr;
     h->d_seg_part = h->d_chunk_part = nullptr;
     int rc;
     if ((rc = dalloc(&h->d_seg_begin, nseg)) || (rc = dalloc(&h->d_seg_end, nseg)) ||
         (rc = dalloc(&h->d_seg_bunch, nseg)) || (rc = dalloc(&h->d_wg_seg, nwg + 1)) ||
         (rc = dalloc(&h->d_bunch_seg, nb + 1)) || (rc = dalloc(&h->d_seg_part, (size_t)nseg * kNPart)) ||
         (rc = dalloc(&h->d_chunk_part, (size_t)nb * max_chunks * kNPart)) ||
-        (!identity && (rc = dalloc(&h->d_seg_rank, nseg))))
+        (!identity && ((rc = dalloc(&h->d_seg_rank, nseg)) || (rc = dalloc(&h->d_seg_order, nseg)))))
         return rc;
-    if (!identity)
+    if (!identity) {
+        std::vector<int32_t> order(nseg);
+        for (int sg = 0; sg < nseg; sg++) order[rank[sg]] = sg;
         HIPCHK(hipMemcpy(h->d_seg_rank, rank.data(), rank.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->d_seg_order, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMemcpy(h->d_seg_begin, sb.data(), sb.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->d_seg_end, se.data(), se.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(h->d_seg_bunch, sbunch.data(), sbunch.size() * sizeof(int32_t), hipMemcpyHostToDevice));
@@ -1001,8 +1007,10 @@ int ion_phase(cts_handle *h, int32_t turn, int k)
     ik.fb_list = h->d_fb_list;
     ik.bunch_off = h->d_bunch_off;
     ik.nb = h->nb;
+    ik.nseg = h->nseg;
+    ik.seg_order = h->d_seg_order;
     TimedScope ts(h, "ion_kick");
-    HIPCHK(launch_ion_kick(ik, h->nwg, h->stream));
+    HIPCHK(launch_ion_kick(ik, 4 * h->ncu, h->stream));
     h->cur_valid = false;
     return CTS_OK;
 }
@@ -1039,6 +1047,7 @@ int cts_create(const cts_ring *ring, const cts_section *secs, int32_t nsec, cts_
     h->secs.assign(secs, secs + nsec);
     h->ips.resize(nsec);
     hipError_t e = hipGetDevice(&h->dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, h->dev);
     if (e == hipSuccess) e = hipStreamCreate(&h->stream);
     if (e != hipSuccess) {
         delete h;

@@ -125,6 +125,8 @@ struct IonKickParams {
     const int64_t *bunch_off;
     int32_t nb;
     int32_t nspec;
+    int32_t nseg;  // segments of the particle layout (seg_begin/seg_end/seg_bunch)
+    const int32_t *seg_order;  // the segment at each place in particle order (null: identity)
 };
 
 // One segment of the ion turn: passages j_begin .. j_end - 1, all species.

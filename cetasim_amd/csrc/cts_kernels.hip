@@ -812,8 +812,11 @@ k_ion_kick(IonKickParams P)
 {
     __shared__ BeField sF[CTS_MAX_SPECIES];
     const int tid = threadIdx.x;
-    const int s_first = P.wg_seg[blockIdx.x], s_last = P.wg_seg[blockIdx.x + 1];
-    for (int s = s_first; s < s_last; s++) {
+    // the segments in particle order dealt round-robin to this launch's own
+    // grid (four workgroups per CU, the occupancy the kernel is compiled
+    // for): the workgroups running at one time stream one compact window
+    for (int r = blockIdx.x; r < P.nseg; r += gridDim.x) {
+        const int s = P.seg_order ? P.seg_order[r] : r;
         const int b = P.seg_bunch[s];
         const double *c = P.ion_e + (int64_t)b * CTS_MAX_SPECIES * 5;
         // the clouds' constants, once per segment
@@ -849,8 +852,8 @@ k_ion_kick_sur(IonKickParams P)
     const int tid = threadIdx.x;
     load_mlp_lds(L, P.sur_params, tid, kBlock);
     __syncthreads();
-    const int s_first = P.wg_seg[blockIdx.x], s_last = P.wg_seg[blockIdx.x + 1];
-    for (int s = s_first; s < s_last; s++) {
+    for (int r = blockIdx.x; r < P.nseg; r += gridDim.x) {
+        const int s = P.seg_order ? P.seg_order[r] : r;
         const int b = P.seg_bunch[s];
         const double *c = P.ion_e + (int64_t)b * CTS_MAX_SPECIES * 5;
         for (int64_t base = P.seg_begin[s]; base < P.seg_end[s]; base += kBlock) {
