@@ -20,6 +20,7 @@ and the CPU oracle's rate extrapolated from bounded samples.
 """
 import argparse
 import json
+import math
 import os
 import platform
 import sys
@@ -72,11 +73,18 @@ def parse():
     ap.add_argument("--bbimp", action="store_true",
                     help="c3/c5: add the broadband impedance (longitudinal; a broadband resonator Rs 2 kOhm, Q 1, "
                          "20 GHz tabulated at 261 frequencies up to 100 GHz: 521 bins of 1.5 mm per bunch)")
+    ap.add_argument("--bbimp-td", action="store_true",
+                    help="c3/c5: add the broadband impedance in the time domain (longitudinal; the broadband "
+                         "resonator's wake function W(z), Rs 2 kOhm, Q 1, 20 GHz, tabulated every 0.5 mm over "
+                         "0.2 m: the 0.6 m bucket on 1199 bins)")
     ap.add_argument("--sur-steps", type=int, default=20000, help="c3: Adam steps of the surrogate training")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reads MIB, writes MIB) three times, "
                          "to scale FETCH_SIZE/WRITE_SIZE counters")
-    return ap.parse_args()
+    a = ap.parse_args()
+    if a.bbimp and a.bbimp_td:
+        ap.error("--bbimp and --bbimp-td are the two methods of one impedance: choose one")
+    return a
 
 
 def progress(msg):
@@ -319,6 +327,16 @@ def train_turns(a, sur=None, mode=0):
         z = np.zeros(nimp, complex)
         z[1:] = 2.0e3 / (1 + 1j * (f[1:] / 2.0e10 - 2.0e10 / f[1:]))
         tr.set_bbimp(z.real.copy(), z.imag.copy(), fmax)
+    if a.bbimp_td:
+        import numpy as np
+        # W(tau) = 2 alpha Rs e^(-alpha tau) (cos wb tau - alpha/wb sin wb tau), W(0) = alpha Rs
+        nw, dzw, rs, wr = 400, 5.0e-4, 2.0e3, 2 * math.pi * 2.0e10
+        al = wr / 2.0
+        wb = math.sqrt(wr * wr - al * al)
+        tau = np.arange(nw) * dzw / configs.C_LIGHT
+        w = 2 * al * rs * np.exp(-al * tau) * (np.cos(wb * tau) - al / wb * np.sin(wb * tau))
+        w[0] = al * rs
+        tr.set_bbimp_td(np.arange(nw) * dzw, [w, None, None, None, None])
     if a.beam_loading:
         tr.set_beam_loading(dict(nres=1, mode=[1], type=[1], harm=[1], cold=[0], rs=[5e6], q0=[2e4],
                                  coupling=[3.0], detune=[-2e4], vol=[configs.rf_voltage(p)], phase=[0.0],
@@ -396,6 +414,8 @@ def run_train(a):
                                   "and the resonator)" if a.lrwake else "")
                                + (", broadband impedance (longitudinal, 521 bins of 1.5 mm per bunch)"
                                   if a.bbimp else "")
+                               + (", broadband impedance in the time domain (longitudinal, a 400-row wake "
+                                  "table, 1199 bins of 0.5 mm over the bucket)" if a.bbimp_td else "")
                                + (", beam-loaded cavity (rfMode 1, stable model, 60 bins per bunch; the turn "
                                   "adds a read-only pass for the bunch extent)" if a.beam_loading else ""),
                    "macroparticles": n, "bunches": nb, "parallelism": "single GPU",

@@ -205,11 +205,13 @@ __global__ void __launch_bounds__(kBlock) k_bbi_bin(BbiParams P)
         const int b = P.seg_bunch[s];
         if (P.scale[b] == 0.0) continue;  // uniform: the whole block skips
         const int64_t e = P.seg_end[s];
-        // the range of bins the segment's counted particles reach
-        if (tid == 0) { krange[0] = INT_MAX; krange[1] = -1; }
+        // the range of bins the segment's counted particles reach; a grid
+        // that fits one window is taken whole (no sweep for the range)
+        const bool whole = P.n <= (dip ? kBbiDipWin : kBbiWinCopies);
+        if (tid == 0) { krange[0] = whole ? 0 : INT_MAX; krange[1] = whole ? P.n - 1 : -1; }
         __syncthreads();
         int rlo = INT_MAX, rhi = -1;
-        for (int64_t i0 = P.seg_begin[s] + tid; i0 < e; i0 += kUnroll * kBlock) {
+        for (int64_t i0 = P.seg_begin[s] + tid; !whole && i0 < e; i0 += kUnroll * kBlock) {
             double zz[kUnroll];
             uint8_t ff[kUnroll];
 #pragma unroll
