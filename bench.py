@@ -32,6 +32,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "macroparticle-turns/s on 1 MI355X: surrogate vs plain kick at \u22641e-4 field err"
 BYTES_PER_MPT = 96.0  # 6 FP64 read + 6 FP64 written per macroparticle-turn
+FP64_PEAK_TFLOPS = 78.6
+# FP64 operations per Bassetti-Erskine evaluation as the counters count them
+# (SQ_INSTS_VALU_FLOPS_FP64 x 64 over the evaluations of a launch): the
+# electron-side kick, and an ion's passage (the field plus drift and moments)
+FP64_SOURCE = "profiles/ion_kernels_train_480_bunches_fp64_counters.md"
+FP64_FLOP_PER_KICK = 688.0
+FP64_FLOP_PER_ION_PASSAGE = 906.0
 
 
 def measured_traffic_per_mpt(name="k_pass_single_bunch_1e7_hbm_traffic.txt"):
@@ -288,6 +295,21 @@ def workload_train(a):
     return configs.LARGE if a.workload == "c5" else configs.TRAIN
 
 
+def fp64_roofline(per_turn, n, ion_passages):
+    """FP64 rate of the electron-side kick (one evaluation per
+    macroparticle and ion species) and of the ion segments (one per ion
+    passage) over their HIP-event times per turn."""
+    out = {"bound": "fp64", "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "flop_source": FP64_SOURCE}
+    for name, key, flop, count in (("k_ion_kick", "ion_kick", FP64_FLOP_PER_KICK, n),
+                                   ("k_ion_turn", "ion_turn", FP64_FLOP_PER_ION_PASSAGE, ion_passages)):
+        ms = per_turn.get(key)
+        if ms:
+            achieved = flop * count / (ms * 1e-3) / 1e12
+            out[name] = {"achieved": achieved, "frac": achieved / FP64_PEAK_TFLOPS, "kernel_ms_per_turn": ms,
+                         "flop_per_evaluation": flop, "evaluations_per_turn": count}
+    return out
+
+
 def train_turns(a, sur=None, mode=0):
     """Time a.steps turns of the train; returns (s, kernel times, cloud, moments, n, nb, event-stretch s)."""
     from cetasim_amd import configs
@@ -429,6 +451,9 @@ def run_train(a):
                      # the pass's algorithmic bytes over the whole timed turn (ions included)
                      "turn_frac": BYTES_PER_MPT * n / (dt / a.steps) / 1e9 / HBM_PEAK_GBS,
                      "other_kernels_ms_per_turn": per_turn},
+        # the FP64 kernels of the turn: the ion segments and the electron-side
+        # kicks, at the FP64 operation counts of the counters' profile
+        "roofline_fp64": fp64_roofline(per_turn, n * len(T["ions"]["mass_number"]), float(cloud[:, :, 0].sum())),
         "ion_macroparticles_last_passage": float(cloud[-1, 0, 0]),
         # Bassetti-Erskine evaluations of the ion side per turn (ions met by
         # each bunch, summed over the train)
