@@ -3,7 +3,7 @@
 * Host-supplied excitation normals (noise mode 1) stay usable after
   cts_set_bunches changes the particle count: the buffer follows the new
   layout and the synrad seam still matches the oracle.
-* The long-range wake, the broadband impedance and the beam-loaded cavities
+* The long-range wake, the broadband impedance (both methods) and the beam-loaded cavities
   stay switched on across cts_set_bunches (set up again for the new bunch
   layout), giving the same turn as a handle configured from scratch.
 * cts_upload starts a new run: the long-range centroid history and any
@@ -105,6 +105,36 @@ def test_bbimp_survives_set_bunches():
     ref = fresh.download()
     wref, pref = fresh.bbimp_wake()
     assert np.max(np.abs(wref)) > 0
+    assert np.array_equal(wake, wref) and np.array_equal(prof, pref)
+    for c in COORDS:
+        assert np.array_equal(got[c], ref[c]), c
+    tr.close()
+    fresh.close()
+
+
+def test_time_domain_bbimp_survives_set_bunches():
+    """The same for the time-domain impedance (its wake table is kept)."""
+    from test_gpu_bbimp import td_table
+    counts = [2000, 3100]
+    pos, wakes = td_table()
+    ring, secs, counts, gap, q, ne, _, tr, bunches, off = setup_train(counts, 19)
+    parts = {c: np.concatenate([b[c] for b in bunches]).copy() for c in COORDS}
+    for b in range(len(counts)):
+        parts["x"][off[b]:off[b + 1]] += 3e-5 * (b + 1)
+    tr.set_bbimp_td(pos, wakes, beta=(5.0, 3.0))
+    tr.set_bunches(counts, q, ne, gap)
+    tr.upload(parts)
+    tr.turn(0)
+    got = tr.download()
+    wake, prof = tr.bbimp_wake()
+
+    _, _, _, _, _, _, _, fresh, _, _ = setup_train(counts, 19)
+    fresh.upload(parts)
+    fresh.set_bbimp_td(pos, wakes, beta=(5.0, 3.0))
+    fresh.turn(0)
+    ref = fresh.download()
+    wref, pref = fresh.bbimp_wake()
+    assert wref.shape[2] == 1199 and all(np.max(np.abs(wref[k])) > 0 for k in range(5))
     assert np.array_equal(wake, wref) and np.array_equal(prof, pref)
     for c in COORDS:
         assert np.array_equal(got[c], ref[c]), c
