@@ -405,7 +405,14 @@ __device__ __forceinline__ void w_weideman(const double *zr, const double *zi, d
 #pragma unroll
     for (int q = 0; q < NP; q++) {
         const double dr = L + zi[q], di = -zr[q];
-        const double inv = 1.0 / (dr * dr + di * di);
+        // 1/|L - i z|^2 from the hardware reciprocal and two Newton steps:
+        // the denominator is at least L^2 (Im z >= 0) and finite in the
+        // kick region, so the IEEE division's scaling and fix-up steps are
+        // not needed (within an ulp of the quotient)
+        const double den = dr * dr + di * di;
+        double inv = __builtin_amdgcn_rcp(den);
+        inv = fma(inv, fma(-den, inv, 1.0), inv);
+        inv = fma(inv, fma(-den, inv, 1.0), inv);
         ur[q] = dr * inv;
         ui[q] = -di * inv;
         const double nr = L - zi[q], ni = zr[q];
