@@ -37,8 +37,8 @@ FP64_PEAK_TFLOPS = 78.6
 # (SQ_INSTS_VALU_FLOPS_FP64 x 64 over the evaluations of a launch): the
 # electron-side kick, and an ion's passage (the field plus drift and moments)
 FP64_SOURCE = "profiles/ion_kernels_train_480_bunches_fp64_counters.md"
-FP64_FLOP_PER_KICK = 688.0
-FP64_FLOP_PER_ION_PASSAGE = 906.0
+FP64_FLOP_PER_KICK = 450.0
+FP64_FLOP_PER_ION_PASSAGE = 610.0
 
 
 def measured_traffic_per_mpt(name="k_pass_single_bunch_1e7_hbm_traffic.txt"):
